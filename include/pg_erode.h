@@ -1,0 +1,151 @@
+/*
+ * liberode_hip -- the particle hydraulic erosion droplet loop on the
+ * equi-angular cube-sphere, as HIP kernels for MI355X (gfx950).
+ *
+ * C ABI.  Every pointer argument named like a state array is a DEVICE
+ * pointer (e.g. a PyTorch-ROCm tensor's data_ptr()); scalars are by value;
+ * parameter structs are host pointers.  Arrays are C-contiguous with the
+ * reference's shapes: extended fields are (F, NE, NE[, C]) with
+ * NE = N + 2H, flattened as ((f * NE + i) * NE + j) * C + c.
+ * The caller owns every buffer (as in the reference, bake/globe/erosion/
+ * maps.py:933-944); no entry point allocates device memory.
+ * Every call returns 0 on success, non-zero on error; pg_last_error()
+ * returns the message of the last failure on the calling thread.  No C++
+ * exception crosses the ABI.  Work is enqueued on `stream` (a hipStream_t;
+ * NULL = the default stream) and is asynchronous unless stated otherwise.
+ *
+ * Modes.  The check mode (pg_trace_particles + pg_apply_changes) reproduces
+ * the reference's algorithm bit for bit: float64 droplet state, the packed
+ * float32 record, chunked trace against frozen maps, then a serial apply in
+ * droplet order.  The fast mode (pg_fast_*) traces every droplet of a chunk
+ * in float32 and accumulates its effects as int64 fixed point with atomics:
+ * deterministic, exactly mass-conserving, with order-independent caps.
+ */
+#ifndef PG_ERODE_H
+#define PG_ERODE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef void* pg_stream; /* hipStream_t */
+
+/* Scalar parameters of trace_particles, same names, order and meaning as
+ * its arguments (bake/globe/erosion/particle.py:292-318). */
+typedef struct pg_trace_params {
+    double dt, density, friction, deposition_rate, evap_rate, k_mom, mom_scale, k_disc, disc_saturation,
+        disc_exponent, slope_gain, slope_sat, erodibility, cover_depth, min_volume;
+    int64_t max_steps;
+    double max_erode, conc_scale;
+    int64_t pit_steps;
+    int64_t deposit_on_exit;
+    double ocean_rate;
+    int64_t ocean_steps;
+    double fan_room, dep_floor, lake_trap, lake_park, lateral_rate;
+} pg_trace_params;
+
+/* Message of the last failed call on this thread ("" if none). */
+const char* pg_last_error(void);
+/* ABI version of this library. */
+int pg_abi_version(void);
+
+/* ---- geometry, host side (bake/globe/cubesphere.py:70-220) ------------- */
+/* The library's own scalar geometry run on the host CPU: the same code the
+ * kernels run, exposed so it can be checked without a GPU.
+ * Replace to_sphere (cubesphere.py:70), from_sphere (:96), jacobian (:130)
+ * and transfer_velocity (:204); pg_geom_tan/atan/sincos are the
+ * double-double functions behind them (math.tan / math.atan / math.cos,
+ * math.sin as numba calls them, particle.py:400-401). */
+int pg_geom_to_sphere(int face, double u, double v, double* xyz3);
+int pg_geom_from_sphere(double x, double y, double z, int* face, double* uv2);
+int pg_geom_jacobian(int face, double u, double v, double* j6);
+int pg_geom_transfer_velocity(int face, double u, double v, double a, double b, int* face_out, double* uvab4);
+double pg_geom_tan(double x);
+double pg_geom_atan(double x);
+void pg_geom_sincos(double x, double* s, double* c);
+
+/* ---- packed samples (particle.py:164-191, pack_samples) ---------------- */
+/* samp (F, NE, NE, 16) float32 from the state arrays.  route may be NULL
+ * when use_route == 0; lake_flag may be NULL (all zero). */
+int pg_pack_samples(float* samp, int64_t F, int64_t NE, const double* height, const double* sediment,
+                    const double* route, int use_route, const double* discharge, const double* momentum,
+                    const float* evap, const float* hardness, const float* metric, const float* metric_inv,
+                    const double* base, const uint8_t* lake_flag, pg_stream stream);
+
+/* ---- check mode: trace (particle.py:275-818, trace_particles) ---------- */
+/* One chunk of P droplets on frozen samples, float64 droplet state.  Fills
+ * the change list exactly like the reference: P * cap entries with
+ * cap = pg_change_list_cap(max_steps, lateral_rate > 0).  sp_dir is read
+ * only where sp_sed > 0.  roots may be NULL when use_roots == 0. */
+int pg_trace_particles(int64_t P, const int64_t* sp_face, const double* sp_x, const double* sp_y,
+                       const double* sp_sed, const double* sp_dir, double volume0, const float* samp, int use_route,
+                       const uint8_t* mask, int64_t N, int64_t H, int spherical, const pg_trace_params* prm,
+                       const float* roots, int use_roots, int32_t* cl_cell, double* cl_delta, float* cl_vol,
+                       float* cl_mom, int64_t* cl_count, int64_t* cl_lo, int64_t* cl_hi, int8_t* sp_death,
+                       pg_stream stream);
+
+/* particle.py:258-272, change_list_cap */
+int64_t pg_change_list_cap(int64_t max_steps, int lateral);
+
+/* ---- check mode: serial apply (particle.py:821-1042, apply_changes) ---- */
+/* Applies the change list in droplet order against the live terrain
+ * (height + sediment + acc), updating acc, pending, samp (S_SURF, S_SED,
+ * S_RFLAG), the discharge/momentum tracks and the lake loads.  Writes
+ * (clamped entries, mass to pending, deficit lost, lost offshore) to the
+ * DEVICE array out4 (float64[4]).  erode_cap may be NULL when
+ * use_erode_cap == 0; lake_load / lake_id / lake_room may be NULL when the
+ * list holds no lake entries (volume -3). */
+int pg_apply_changes(const int32_t* cl_cell, const double* cl_delta, const float* cl_vol, const float* cl_mom,
+                     const int64_t* cl_count, int64_t P, int64_t cap, const double* height, const double* sediment,
+                     double* acc, double* pending, float* samp, double* disch_track, double* mom_track,
+                     const uint8_t* mask, double iter_erode, double iter_deposit, double fan_slope, int use_route,
+                     double dep_floor, double offshore_writeoff, const double* erode_cap, int use_erode_cap,
+                     double* lake_load, const int32_t* lake_id, double* lake_room, double* out4, pg_stream stream);
+
+/* Adds a chunk's entry count and death histogram to the DEVICE array
+ * counts7 (int64[7]: entries, then the six DEATH_* causes), the statistics
+ * run_iteration gathers per chunk (bake/globe/erosion/maps.py:1028-1029). */
+int pg_chunk_stats(const int64_t* cl_count, const int8_t* sp_death, int64_t P, int64_t* counts7, pg_stream stream);
+
+/* ---- per-cell passes ---------------------------------------------------- */
+/* particle.py:1079-1101, fold_changes (n = F * NE * NE) */
+int pg_fold_changes(int64_t n, double* height, double* sediment, double* acc, const uint8_t* mask, pg_stream stream);
+/* particle.py:1230-1242, ema_update */
+int pg_ema_update(int64_t n, double* discharge, double* momentum, double* disch_track, double* mom_track,
+                  const uint8_t* mask, double ema, pg_stream stream);
+/* bake/globe/erosion/maps.py:1132-1134, apply_uplift: height += min(u, cap) - mean
+ * on active cells (cap ignored when use_cap == 0; mean from the caller). */
+int pg_apply_uplift(int64_t n, double* height, const double* uplift, const uint8_t* mask, double cap, int use_cap,
+                    double mean, pg_stream stream);
+
+/* ---- spawning (particle.py:1245-1260, spawn_cells) --------------------- */
+/* out[p] = index of the first cdf entry > u[p] * cdf[M-1] (capped at M-1);
+ * u sorted ascending. */
+int pg_spawn_cells(const double* cdf, int64_t M, const double* u, int64_t P, int64_t* out, pg_stream stream);
+
+/* ---- halo exchange (bake/globe/field.py:104-124 with the tables of
+ * bake/globe/cubesphere.py:540-662) -------------------------------------- */
+/* data (6*NE*NE, ncomp) float64.  Scalar (ncomp 1, rot NULL):
+ * data[dst[k]] = sum_s w[k,s] data[src[k,s]].  Vector (ncomp 2, rot given):
+ * data[dst[k]] = sum_s w[k,s] rot[k,s] . data[src[k,s]].  Sources are
+ * interior cells, destinations halo cells. */
+int pg_exchange_halos(double* data, int64_t ncomp, int64_t K, int64_t S, const int64_t* dst, const int64_t* src,
+                      const double* w, const double* rot, pg_stream stream);
+
+/* ---- datum hold (bake/globe/erosion/maps.py:1446-1508, hold_datum) ----- */
+/* Exact k-th smallest (0-based) of height + sediment over the interior
+ * (F faces, N x N, halo H), written to the DEVICE double *q_out.
+ * work: DEVICE scratch of pg_select_work_bytes() bytes. */
+int64_t pg_select_work_bytes(void);
+int pg_select_kth_surface(const double* height, const double* sediment, int64_t F, int64_t N, int64_t H, int64_t k,
+                          double* q_out, void* work, pg_stream stream);
+/* height -= q; route -= q (when route != NULL); base -= q where base < 0;
+ * q read from the DEVICE double *q (n = F * NE * NE). */
+int pg_datum_shift(int64_t n, double* height, double* route, double* base, const double* q, pg_stream stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* PG_ERODE_H */

@@ -1,0 +1,121 @@
+"""Synthetic upstream fields for the erosion stage's benchmark workloads.
+
+The bedrock is 3-D lattice value noise summed over octaves (fBm) sampled at
+the cell centres on the unit sphere, the recipe of the reference's test
+stubs (bake/globe/stubs.py:14-44: a random lattice per octave, smoothstep
+trilinear interpolation, base frequency 2, frequency x2 and amplitude x0.5
+per octave, normalised by the amplitude sum).  Because the lattice lives in
+3-D space the field is continuous across cube edges, halos included.
+
+Workloads (sizes and parameters of the benchmark configurations):
+
+* ``plain_globe(N=128, ...)``: bedrock fBm x 5000 m, shifted so 30 % of the
+  interior is land, uniform rain 1, evap 1, hardness 0, no uplift;
+* ``earth_globe(N=1024, ...)``: the same bedrock, rain
+  ``2 cos(lat)^2 + 0.3 + 0.2 fBm``, uplift = 64 Gaussian blobs (sigma
+  200 km of arc, peak 1 mm per iteration).
+"""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+
+from .config import ErosionParams, WorldGroup, WorldParams, plain_erosion_params
+from .cubesphere import Grid, get_grid
+
+EARTH_RADIUS_M = 6371.0e3
+
+
+def fbm_at(p: np.ndarray, rng: np.random.Generator, octaves: int = 5, base_freq: float = 2.0) -> np.ndarray:
+    """Value-noise fBm at unit vectors ``p`` (..., 3), roughly in [-1, 1]."""
+    p = np.asarray(p, dtype=np.float64)
+    out = np.zeros(p.shape[:-1])
+    amp, total = 1.0, 0.0
+    for o in range(octaves):
+        freq = base_freq * 2 ** o
+        n = int(np.ceil(freq)) + 2
+        lattice = rng.random((n + 1, n + 1, n + 1))
+        q = (p + 1.0) * 0.5 * freq
+        i0 = np.floor(q).astype(np.int64)
+        t = q - i0
+        t = t * t * (3.0 - 2.0 * t)
+        i0 = np.clip(i0, 0, n - 1)
+        acc = np.zeros_like(out)
+        for dx in (0, 1):
+            wx = t[..., 0] if dx else 1.0 - t[..., 0]
+            for dy in (0, 1):
+                wy = t[..., 1] if dy else 1.0 - t[..., 1]
+                for dz in (0, 1):
+                    wz = t[..., 2] if dz else 1.0 - t[..., 2]
+                    acc += wx * wy * wz * lattice[i0[..., 0] + dx, i0[..., 1] + dy, i0[..., 2] + dz]
+        out += amp * (acc * 2.0 - 1.0)
+        total += amp
+        amp *= 0.5
+    return out / total
+
+
+def earth_cell_size(N: int) -> float:
+    """Cell size (m) at which N cells per face edge give Earth's radius."""
+    return 2.0 * math.pi * EARTH_RADIUS_M / (4.0 * N)
+
+
+def _bedrock(grid: Grid, rng, octaves: int, amp_m: float, land_fraction: float) -> np.ndarray:
+    bed = fbm_at(grid.centers, rng, octaves) * amp_m
+    q = np.quantile(bed[grid.interior], 1.0 - land_fraction)
+    return bed - q
+
+
+def gaussian_blobs(grid: Grid, rng, n: int, sigma_m: float, peak_m: float, radius_m: float) -> np.ndarray:
+    """Sum of ``n`` Gaussian bumps of arc-length width ``sigma_m`` centred at
+    uniform random points of the sphere, on every extended cell."""
+    c = rng.normal(size=(n, 3))
+    c /= np.linalg.norm(c, axis=1, keepdims=True)
+    P = grid.centers.reshape(-1, 3)
+    out = np.zeros(P.shape[0])
+    for k in range(n):
+        arc = np.arccos(np.clip(P @ c[k], -1.0, 1.0)) * radius_m
+        out += peak_m * np.exp(-0.5 * (arc / sigma_m) ** 2)
+    return out.reshape(grid.centers.shape[:-1])
+
+
+def workload_params(N: int, H: int = 2, seed: int = 0, max_steps: int = 64, **erosion) -> WorldParams:
+    """World + plain-profile erosion parameters of the benchmark workloads.
+    The generic droplet-model knobs map onto the reference's fields as:
+    inertia 0.05 -> friction (1 - 0.05) / dt; capacity 4 -> erodibility 4;
+    erode = deposit 0.3 -> deposition_rate 0.3; evaporation 0.01 ->
+    evap_rate 0.01 / dt; gravity 4 -> slope_gain 4; one droplet per cell."""
+    dt = 1.2
+    ep = plain_erosion_params(dt=dt, friction=(1.0 - 0.05) / dt, erodibility=4.0, deposition_rate=0.3,
+                              evap_rate=0.01 / dt, slope_gain=4.0, particles_per_cell=1.0, max_steps=max_steps,
+                              uplift_mode="stack")
+    for k, v in erosion.items():
+        setattr(ep, k, v)
+    return WorldParams(world=WorldGroup(seed=seed, N_c=N, cell_size_m=earth_cell_size(N), land_fraction=0.30, halo=H),
+                       erosion=ep)
+
+
+def plain_globe(N: int = 128, H: int = 2, seed: int = 0, octaves: int = 6):
+    """Plain erosion inputs: (grid, dict of extended float arrays in metres)."""
+    grid = get_grid(N, H, earth_cell_size(N))
+    p = WorldParams(world=WorldGroup(seed=seed, N_c=N, cell_size_m=grid.cell_size_m, halo=H))
+    bed = _bedrock(grid, p.rng("stub", 1), octaves, 5000.0, 0.30)
+    shape = bed.shape
+    return grid, {
+        "bedrock_m": bed,
+        "hardness": np.zeros(shape, np.float32),
+        "precip": np.ones(shape, np.float32),
+        "evap": np.ones(shape, np.float32),
+        "uplift_m": np.zeros(shape),
+    }
+
+
+def earth_globe(N: int = 1024, H: int = 2, seed: int = 0, octaves: int = 6, n_blobs: int = 64):
+    """Earth-scale inputs: latitudinal rain and Gaussian uplift blobs."""
+    grid, f = plain_globe(N, H, seed, octaves)
+    p = WorldParams(world=WorldGroup(seed=seed, N_c=N, cell_size_m=grid.cell_size_m, halo=H))
+    lat = grid.latitude()
+    rain = 2.0 * np.cos(lat) ** 2 + 0.3 + 0.2 * fbm_at(grid.centers, p.rng("stub", 2), 4)
+    f["precip"] = rain.astype(np.float32)
+    f["uplift_m"] = gaussian_blobs(grid, p.rng("stub", 3), n_blobs, 200.0e3, 1.0e-3, EARTH_RADIUS_M)
+    return grid, f
