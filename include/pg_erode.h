@@ -65,6 +65,12 @@ int pg_geom_transfer_velocity(int face, double u, double v, double a, double b, 
 double pg_geom_tan(double x);
 double pg_geom_atan(double x);
 void pg_geom_sincos(double x, double* s, double* c);
+/* The same functions evaluated on the GPU over DEVICE arrays of n inputs:
+ * op 0 tan (in n, out n), 1 atan (in n, out n), 2 to_sphere (in n x (face,
+ * u, v), out n x 3), 3 from_sphere (in n x 3, out n x (u, v), iout face),
+ * 4 transfer_velocity (in n x (face, u, v, a, b), out n x (u, v, a, b),
+ * iout face), 5 sin/cos (in n, out n x (sin, cos)). */
+int pg_geom_eval(int op, int64_t n, const double* in, double* out, int* iout, pg_stream stream);
 
 /* ---- packed samples (particle.py:164-191, pack_samples) ---------------- */
 /* samp (F, NE, NE, 16) float32 from the state arrays.  route may be NULL
@@ -144,6 +150,81 @@ int pg_select_kth_surface(const double* height, const double* sediment, int64_t 
 /* height -= q; route -= q (when route != NULL); base -= q where base < 0;
  * q read from the DEVICE double *q (n = F * NE * NE). */
 int pg_datum_shift(int64_t n, double* height, double* route, double* base, const double* q, pg_stream stream);
+
+/* ---- fast mode (the production path of one iteration's particle pass) -- */
+/* Device pointers to the erosion state (bake/globe/erosion/maps.py:56-115
+ * dtypes): the stage's fields in cell units. */
+typedef struct pg_fast_fields {
+    int64_t F, N, H;
+    int64_t spherical, deposit_on_exit;
+    double *height, *sediment, *discharge, *momentum, *pending;
+    const double* base;
+    const float *precip, *evap, *hardness, *metric, *metric_inv;
+    const uint8_t* mask;
+} pg_fast_fields;
+
+/* The iteration's draws and knobs beyond pg_trace_params. */
+typedef struct pg_fast_params {
+    /* numpy PCG64 state after seeding with params.rng(stage, *key)
+     * (bit_generator.state: state and inc, low / high 64-bit words) */
+    uint64_t rng_state_lo, rng_state_hi, rng_inc_lo, rng_inc_hi;
+    int64_t n_particles;   /* rain droplets: max(1, round(ppc * active cells)) */
+    int64_t chunks;        /* frozen-terrain chunks per iteration (>= 1) */
+    double iter_deposit;   /* net deposit a cell may take per iteration (cell units) */
+    double offshore_writeoff;
+    double ema;
+    double min_volume_frac; /* death by evaporation below this x the spawn volume (prm->min_volume unused) */
+    int64_t height_bits;   /* fixed point: heights in units of 2^-height_bits cells */
+    int64_t track_bits;    /* fixed point: discharge / momentum in units of 2^-track_bits */
+    int64_t grid_blocks;   /* persistent trace grid (0 = sized by occupancy) */
+} pg_fast_params;
+
+/* Statistics of one fast iteration (DEVICE int64[PG_FAST_NSTATS]). */
+enum {
+    PG_FS_PARTICLES = 0, PG_FS_RAIN, PG_FS_ENTRIES, PG_FS_MOVES, PG_FS_DEATHS, /* 6 causes: 4..9 */
+    PG_FS_CLAMPED = 10, PG_FS_TO_PENDING_FX, PG_FS_LOST_OFFSHORE_FX, PG_FS_EXIT_FX, PG_FS_VANISHED_FX,
+    PG_FS_RELEASED_FX, PG_FS_VOLUME0_BITS, PG_FS_ELIGIBLE, PG_FS_CDF_SERIAL, PG_FS_ERODED_FX, PG_FS_DEPOSITED_FX,
+    PG_FAST_NSTATS = 24
+};
+
+/* Bytes of DEVICE workspace the fast mode needs for this grid and rain
+ * droplet count (fixed-point accumulators, packed records, spawn buffers,
+ * sort/scan scratch).  Queries the current device's sort configuration. */
+int64_t pg_fast_workspace_bytes(int64_t F, int64_t NE, int64_t n_particles);
+
+/* Initialise a workspace for a state: static per-cell geometry records and
+ * the fixed-point pending stockpile from fields->pending. */
+int pg_fast_setup(const pg_fast_fields* fields, const pg_fast_params* fp, void* work, int64_t work_bytes,
+                  pg_stream stream);
+
+/* One iteration's particle pass (bake/globe/erosion/maps.py:839-1047) in
+ * fast mode (plain profile: no routing surface, lakes, lateral cuts or
+ * roots; prm->conc_scale and prm->min_volume are derived on the device), entirely on the device: pack, spawn (the params.rng stream:
+ * sorted uniforms, inverse CDF, offsets), pending re-injection, `chunks`
+ * rounds of persistent float32 trace + per-cell resolve against the live
+ * terrain, then fold, EMA and the pending mirror.  Statistics go to the
+ * DEVICE array stats (int64[PG_FAST_NSTATS]), zeroed first. */
+int pg_fast_iterate(const pg_fast_fields* fields, const pg_trace_params* prm, const pg_fast_params* fp, void* work,
+                    int64_t work_bytes, int64_t* stats, pg_stream stream);
+
+/* The two halves of pg_fast_iterate: spawn (reset statistics, pack, rain
+ * and stockpile droplets) and the particle pass (chunks, finish). */
+int pg_fast_spawn(const pg_fast_fields* fields, const pg_trace_params* prm, const pg_fast_params* fp, void* work,
+                  int64_t work_bytes, int64_t* stats, pg_stream stream);
+int pg_fast_particles(const pg_fast_fields* fields, const pg_trace_params* prm, const pg_fast_params* fp, void* work,
+                      int64_t work_bytes, int64_t* stats, pg_stream stream);
+
+/* Byte offsets inside a fast workspace, for reading it: offsets[0..15] =
+ * stencil records (float4), geometry (2 x float4), {sediment, base}
+ * (float2), chunk / iteration / discharge / momentum / stockpile fixed
+ * point (int64), droplet cell (int32), x / y fraction (float32), load
+ * (int64), direction a / b (float32), scalars, total size. */
+int pg_fast_layout(int64_t F, int64_t NE, int64_t n_particles, int64_t* offsets, int n);
+
+/* Fill out[k] = the (start + k)-th double of a PCG64 stream (numpy's
+ * Generator.random() draws), k < n: the fast mode's spawn source. */
+int pg_pcg64_random(uint64_t state_lo, uint64_t state_hi, uint64_t inc_lo, uint64_t inc_hi, int64_t start, int64_t n,
+                    double* out, pg_stream stream);
 
 #ifdef __cplusplus
 }

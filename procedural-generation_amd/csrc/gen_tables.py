@@ -24,7 +24,7 @@ def tables():
         "atan_tab[65][2]": [dd(mp.atan(mp.mpf(k) / 64)) for k in range(65)],
         "tan_tab[41][2]": [dd(mp.tan(mp.mpf(k) / 32)) for k in range(41)],
         "inv_odd[12][2]": [dd(mp.mpf(1) / (2 * n + 1)) for n in range(12)],
-        "inv_fact[18][2]": [dd(mp.mpf(1) / mp.factorial(n)) for n in range(18)],
+        "inv_fact[30][2]": [dd(mp.mpf(1) / mp.factorial(n)) for n in range(30)],
     }
 
 

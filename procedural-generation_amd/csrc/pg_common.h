@@ -216,8 +216,8 @@ struct Tables {
         {0.047619047619047616, 2.64338815386942e-18},
         {0.043478260869565216, 1.206764157201257e-18},
     };
-    // 1 / n!, n = 0..17
-    static constexpr double inv_fact[18][2] = {
+    // 1 / n!, n = 0..29
+    static constexpr double inv_fact[30][2] = {
         {1.0, 0.0},
         {1.0, 0.0},
         {0.5, 0.0},
@@ -236,6 +236,18 @@ struct Tables {
         {7.647163731819816e-13, 7.03872877733453e-30},
         {4.779477332387385e-14, 4.399205485834081e-31},
         {2.8114572543455206e-15, 1.6508842730861433e-31},
+        {1.5619206968586225e-16, 1.1910679660273754e-32},
+        {8.22063524662433e-18, 2.2141894119604265e-34},
+        {4.110317623312165e-19, 1.4412973378659527e-36},
+        {1.9572941063391263e-20, -1.3643503830087908e-36},
+        {8.896791392450574e-22, -7.911402614872376e-38},
+        {3.868170170630684e-23, -8.843177655482344e-40},
+        {1.6117375710961184e-24, -3.6846573564509766e-41},
+        {6.446950284384474e-26, -1.9330404233703465e-42},
+        {2.4795962632247976e-27, -1.2953730964765229e-43},
+        {9.183689863795546e-29, 1.4303150396787322e-45},
+        {3.279889237069838e-30, 1.5117542744029879e-46},
+        {1.1309962886447716e-31, 1.0498015412959506e-47},
     };
     static constexpr double half_pi[2] = {1.5707963267948966, 6.123233995736766e-17};
 };
@@ -304,11 +316,12 @@ PG_HD void pg_sincos(double x, double* os, double* oc) {
     r = dd_sub(r, two_prod(kf, p2));
     r = dd_sub(r, dd{kf * p3, 0.0});
     dd r2 = dd_mul(r, r);
-    dd ss = tab(Tables::inv_fact[17]);
-    for (int n = 15; n >= 1; n -= 2) ss = dd_sub(tab(Tables::inv_fact[n]), dd_mul(r2, ss));
+    // |r| <= pi/4: r^29 / 29! < 2^-110
+    dd ss = tab(Tables::inv_fact[29]);
+    for (int n = 27; n >= 1; n -= 2) ss = dd_sub(tab(Tables::inv_fact[n]), dd_mul(r2, ss));
     ss = dd_mul(r, ss);
-    dd cc = tab(Tables::inv_fact[16]);
-    for (int n = 14; n >= 0; n -= 2) cc = dd_sub(tab(Tables::inv_fact[n]), dd_mul(r2, cc));
+    dd cc = tab(Tables::inv_fact[28]);
+    for (int n = 26; n >= 0; n -= 2) cc = dd_sub(tab(Tables::inv_fact[n]), dd_mul(r2, cc));
     double s = ss.hi + ss.lo, c = cc.hi + cc.lo;
     int q = ((int)(int64_t)kf) & 3;
     if (q == 0) { *os = s; *oc = c; }
@@ -339,9 +352,22 @@ struct Vec3 {
     double x, y, z;
 };
 
+// tan/atan used by the geometry: correctly rounded (check mode, bit parity)
+// or the device libm (fast mode: a face crossing is rare and within a
+// tolerance; the double-double path costs ~500 flops per call).
+struct CRMath {
+    static PG_HD double tan(double x) { return pg_tan(x); }
+    static PG_HD double atan(double x) { return pg_atan(x); }
+};
+struct LibMath {
+    static PG_HD double tan(double x) { return ::tan(x); }
+    static PG_HD double atan(double x) { return ::atan(x); }
+};
+
+template <class M = CRMath>
 PG_HD Vec3 to_sphere(int face, double u, double v) {
-    double s = pg_tan((u - 0.5) * HALF_PI);
-    double t = pg_tan((v - 0.5) * HALF_PI);
+    double s = M::tan((u - 0.5) * HALF_PI);
+    double t = M::tan((v - 0.5) * HALF_PI);
     const double(&B)[3][3] = Bases::b[face];
     double x = B[2][0] + s * B[0][0] + t * B[1][0];
     double y = B[2][1] + s * B[0][1] + t * B[1][1];
@@ -362,19 +388,21 @@ struct FaceUV {
     double u, v;
 };
 
+template <class M = CRMath>
 PG_HD FaceUV from_sphere(double x, double y, double z) {
     int face = face_of(x, y, z);
     const double(&B)[3][3] = Bases::b[face];
     double d = x * B[2][0] + y * B[2][1] + z * B[2][2];
     double s = (x * B[0][0] + y * B[0][1] + z * B[0][2]) / d;
     double t = (x * B[1][0] + y * B[1][1] + z * B[1][2]) / d;
-    return {face, pg_atan(s) * TWO_OVER_PI + 0.5, pg_atan(t) * TWO_OVER_PI + 0.5};
+    return {face, M::atan(s) * TWO_OVER_PI + 0.5, M::atan(t) * TWO_OVER_PI + 0.5};
 }
 
 // dp/du and dp/dv on the unit sphere: j[0..2] = J_u, j[3..5] = J_v
+template <class M = CRMath>
 PG_HD void jacobian(int face, double u, double v, double* j) {
-    double s = pg_tan((u - 0.5) * HALF_PI);
-    double t = pg_tan((v - 0.5) * HALF_PI);
+    double s = M::tan((u - 0.5) * HALF_PI);
+    double t = M::tan((v - 0.5) * HALF_PI);
     double dsdu = HALF_PI * (1.0 + s * s);
     double dtdv = HALF_PI * (1.0 + t * t);
     const double(&B)[3][3] = Bases::b[face];
@@ -397,14 +425,15 @@ PG_HD void jacobian(int face, double u, double v, double* j) {
 
 // Re-express a tangent vector (a, b) given in parameter components on
 // (fs, us, vs) as components on (fd, ud, vd): the 2x2 normal equations.
+template <class M = CRMath>
 PG_HD void transfer_vector(int fs, double us, double vs, double a, double b, int fd, double ud, double vd,
                            double* oa, double* ob) {
     double J[6], K[6];
-    jacobian(fs, us, vs, J);
+    jacobian<M>(fs, us, vs, J);
     double wx = a * J[0] + b * J[3];
     double wy = a * J[1] + b * J[4];
     double wz = a * J[2] + b * J[5];
-    jacobian(fd, ud, vd, K);
+    jacobian<M>(fd, ud, vd, K);
     double guu = K[0] * K[0] + K[1] * K[1] + K[2] * K[2];
     double guv = K[0] * K[3] + K[1] * K[4] + K[2] * K[5];
     double gvv = K[3] * K[3] + K[4] * K[4] + K[5] * K[5];
@@ -423,10 +452,11 @@ struct Transfer {
 // A droplet position (u, v) that may have left [0, 1)^2 on `face`, with its
 // velocity: the face, position and velocity components it has on the face it
 // is actually on (cubesphere.py:204-220).
+template <class M = CRMath>
 PG_HD Transfer transfer_velocity(int face, double u, double v, double a, double b) {
     if (u >= 0.0 && u < 1.0 && v >= 0.0 && v < 1.0) return {face, u, v, a, b};
-    Vec3 p = to_sphere(face, u, v);
-    FaceUV q = from_sphere(p.x, p.y, p.z);
+    Vec3 p = to_sphere<M>(face, u, v);
+    FaceUV q = from_sphere<M>(p.x, p.y, p.z);
     if (q.face == face) {
         double u2 = 0.0 > q.u ? 0.0 : q.u;
         u2 = 0.999999999 < u2 ? 0.999999999 : u2;
@@ -435,7 +465,7 @@ PG_HD Transfer transfer_velocity(int face, double u, double v, double a, double 
         return {face, u2, v2, a, b};
     }
     double a2, b2;
-    transfer_vector(face, u, v, a, b, q.face, q.u, q.v, &a2, &b2);
+    transfer_vector<M>(face, u, v, a, b, q.face, q.u, q.v, &a2, &b2);
     return {q.face, q.u, q.v, a2, b2};
 }
 

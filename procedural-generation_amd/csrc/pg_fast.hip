@@ -1,0 +1,1140 @@
+// Fast mode: one iteration's particle pass entirely on the device.
+//
+//   pack      -- per cell: the 16-byte stencil record {surface, discharge,
+//                momentum a, b} and {sediment, base}; spawn eligibility.
+//   spawn     -- the params.rng stream (numpy PCG64, jump-ahead per thread),
+//                radix-sorted uniforms, inverse CDF over an exact
+//                fixed-point prefix sum of the rain weights, offsets;
+//                pending stockpiles re-injected after the rain in cell order.
+//   K chunks  -- persistent trace kernel: one droplet per lane, float32
+//                droplet state, lanes refilled from a device work queue as
+//                droplets die; every step's effect accumulated with int64
+//                fixed-point atomics (deterministic, order independent);
+//                then a per-cell resolve applies the chunk against the live
+//                terrain (deposit caps, surplus to the cell's stockpile with
+//                the offshore write-off) and refreshes the live surface.
+//   finish    -- fold, discharge/momentum EMA, stockpile mirror.
+//
+// Mass is exact in fixed point: every droplet carries its load as an int64
+// and deposits exactly what it took, so per iteration
+//   sum(applied) + new pending + lost offshore + exits + vanished = released.
+//
+// Semantic differences from the reference (bake/globe/erosion/particle.py),
+// which applies each chunk's change list serially with caps that depend on
+// apply order: here all droplets of a chunk act on the same frozen terrain
+// and only order-independent caps remain -- the trace-time caps (relief /
+// expected visits, max_erode, never below sea level) and, per cell, the
+// iteration deposit budget and the waterline ceiling, whose excess goes to
+// the cell's stockpile.  The per-entry floors/ceilings of the next/previous
+// path cell and the deficit carry have no counterpart.  Plain profile only.
+#include <hipcub/hipcub.hpp>
+
+#include "pg_common.h"
+#include "pg_erode.h"
+#include "pg_internal.h"
+
+#pragma clang fp contract(fast)
+
+namespace pg {
+namespace {
+
+constexpr int TPB = 256;
+constexpr int TRACE_TPB = 256;
+
+inline int grid_for(int64_t n, int cap = 65535) {
+    int b = blocks_for(n, TPB);
+    return b < 1 ? 1 : (b > cap ? cap : b);
+}
+
+#define GRID_STRIDE(i, n) for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < (n); i += (int64_t)gridDim.x * blockDim.x)
+
+typedef unsigned long long ull;
+typedef long long ll;
+
+// ---------------------------------------------------------------------------
+// device scalars of an iteration
+// ---------------------------------------------------------------------------
+struct Scalars {
+    double volume0, q_weight, total;
+    ll eligible, pending, n_total, cdf_exact;
+    int emin;
+    int pad;
+    ll queue[1];  // followed by `chunks` queue counters
+};
+
+// ---------------------------------------------------------------------------
+// PCG64 (numpy's: 128-bit LCG, XSL-RR output, doubles from the top 53 bits)
+// ---------------------------------------------------------------------------
+typedef unsigned __int128 u128;
+__device__ const u128 PCG_MULT = ((u128)0x2360ED051FC65DA4ull << 64) | (u128)0x4385DF649FCCF645ull;
+
+__device__ __forceinline__ u128 pcg_advance(u128 s, u128 inc, ull n) {
+    u128 am = 1, ap = 0, cm = PCG_MULT, cp = inc;
+    while (n) {
+        if (n & 1) {
+            am = am * cm;
+            ap = ap * cm + cp;
+        }
+        cp = (cm + 1) * cp;
+        cm = cm * cm;
+        n >>= 1;
+    }
+    return am * s + ap;
+}
+
+__device__ __forceinline__ double pcg_double(u128 s) {
+    const ull hi = (ull)(s >> 64), lo = (ull)s;
+    const ull x = hi ^ lo;
+    const unsigned r = (unsigned)(s >> 122);
+    const ull v = (x >> r) | (x << ((64 - r) & 63));
+    return (double)(v >> 11) * (1.0 / 9007199254740992.0);
+}
+
+// out[k] = draw (start + k): each thread advances once, then steps L times
+__global__ void pcg_fill_kernel(ull slo, ull shi, ull ilo, ull ihi, ll start, ll n, double* out) {
+    constexpr int L = 16;
+    const u128 s0 = ((u128)shi << 64) | slo, inc = ((u128)ihi << 64) | ilo;
+    for (ll t = (ll)blockIdx.x * blockDim.x + threadIdx.x; t * L < n; t += (ll)gridDim.x * blockDim.x) {
+        const ll k0 = t * L;
+        u128 s = pcg_advance(s0, inc, (ull)(start + k0));
+        for (int j = 0; j < L && k0 + j < n; j++) {
+            s = s * PCG_MULT + inc;
+            out[k0 + j] = pcg_double(s);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// setup: static geometry record, fixed-point stockpile
+// ---------------------------------------------------------------------------
+__global__ void setup_kernel(int64_t n, const float* metric, const float* metric_inv, const float* evap,
+                             const float* hardness, float4* geo, const double* pending, ll* pend_fx, double hscale) {
+    GRID_STRIDE(c, n) {
+        geo[2 * c] = make_float4(metric[3 * c], metric[3 * c + 1], metric[3 * c + 2], metric_inv[3 * c]);
+        geo[2 * c + 1] = make_float4(metric_inv[3 * c + 1], metric_inv[3 * c + 2], evap[c], hardness[c]);
+        pend_fx[c] = llrint(pending[c] * hscale);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// pack + spawn eligibility (maps.py:793-806, particle.py:164-191)
+// ---------------------------------------------------------------------------
+__global__ void pack_kernel(int64_t n, int64_t N, int64_t H, const double* height, const double* sediment,
+                            const double* discharge, const double* momentum, const double* base, const float* precip,
+                            const uint8_t* mask, float4* cell4, float2* dyn, int* eligible, Scalars* sc) {
+    const int64_t NE = N + 2 * H, NE2 = NE * NE;
+    int my_emin = 1 << 30;
+    GRID_STRIDE(c, n) {
+        const double s = sediment[c], surf = height[c] + s;
+        cell4[c] = make_float4((float)surf, (float)discharge[c], (float)momentum[2 * c], (float)momentum[2 * c + 1]);
+        dyn[c] = make_float2((float)s, (float)base[c]);
+        const int64_t r = c % NE2, i = r / NE, j = r - i * NE;
+        const bool inter = i >= H && i < H + N && j >= H && j < H + N;
+        const float w = precip[c];
+        const bool ok = inter && mask[c] == MASK_ACTIVE && surf >= base[c] && w > 0.0f;
+        eligible[c] = ok ? 1 : 0;
+        if (ok) {
+            int e;
+            frexpf(w, &e);
+            my_emin = e < my_emin ? e : my_emin;
+        }
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+        const int v = __shfl_xor(my_emin, o);
+        my_emin = v < my_emin ? v : my_emin;
+    }
+    if ((threadIdx.x & 63) == 0 && my_emin < (1 << 30)) atomicMin(&sc->emin, my_emin);
+}
+
+__global__ void reset_scalars_kernel(Scalars* sc, ll* stats, int nstats, int chunks) {
+    if (threadIdx.x == 0) {
+        sc->volume0 = 0.0;
+        sc->q_weight = 0.0;
+        sc->total = 0.0;
+        sc->eligible = sc->pending = sc->n_total = 0;
+        sc->cdf_exact = 1;
+        sc->emin = 1 << 30;
+    }
+    for (int k = threadIdx.x; k < nstats; k += blockDim.x) stats[k] = 0;
+    for (int k = threadIdx.x; k < chunks; k += blockDim.x) sc->queue[k] = 0;
+}
+
+// rain weight in units of the smallest weight's float32 quantum: exact integers
+__global__ void quantize_kernel(int64_t n, const float* precip, const int* eligible, const Scalars* sc, ll* wq) {
+    const int emin = sc->emin;
+    GRID_STRIDE(c, n) {
+        ll v = 0;
+        if (eligible[c]) v = (ll)ldexp((double)precip[c], 24 - emin);
+        wq[c] = v;
+    }
+}
+
+// compact the eligible cells in flat order with their cumulative weight
+__global__ void compact_kernel(int64_t n, const int* eligible, const int* rank, const ll* cdf_fx, Scalars* sc,
+                               int* comp_idx, double* comp_cdf) {
+    const double q = ldexp(1.0, sc->emin - 24);
+    GRID_STRIDE(c, n) {
+        if (eligible[c]) {
+            const int r = rank[c] - 1;
+            comp_idx[r] = (int)c;
+            comp_cdf[r] = (double)cdf_fx[c] * q;
+        }
+    }
+}
+
+// numpy's pairwise float64 sum (np.add.reduce on a contiguous float64
+// array): blocks of < 8 summed in order, blocks up to 128 with 8 interleaved
+// accumulators, longer ones split at a multiple of 8 below the half.
+__device__ double pairwise_leaf(const float* w, const int* idx, int64_t n) {
+    if (n < 8) {
+        double r = 0.0;  // -0.0 only differs for an all -0.0 input, impossible for weights > 0
+        for (int64_t i = 0; i < n; i++) r += (double)w[idx[i]];
+        return r;
+    }
+    double r[8];
+    for (int k = 0; k < 8; k++) r[k] = (double)w[idx[k]];
+    int64_t i = 8;
+    for (; i < n - (n % 8); i += 8)
+        for (int k = 0; k < 8; k++) r[k] += (double)w[idx[i + k]];
+    double res = ((r[0] + r[1]) + (r[2] + r[3])) + ((r[4] + r[5]) + (r[6] + r[7]));
+    for (; i < n; i++) res += (double)w[idx[i]];
+    return res;
+}
+
+__device__ double pairwise_sum(const float* w, const int* idx, int64_t n) {
+    // iterative post-order walk of the split tree
+    struct Frame {
+        int64_t off, n;
+        int state;
+        double left;
+    };
+    Frame st[64];
+    int sp = 0;
+    st[0] = {0, n, 0, 0.0};
+    double ret = 0.0;
+    while (sp >= 0) {
+        Frame& fr = st[sp];
+        if (fr.n <= 128) {
+            ret = pairwise_leaf(w, idx + fr.off, fr.n);
+            sp--;
+            continue;
+        }
+        int64_t n2 = fr.n / 2;
+        n2 -= n2 % 8;
+        if (fr.state == 0) {
+            fr.state = 1;
+            st[sp + 1] = {fr.off, n2, 0, 0.0};
+            sp++;
+        } else if (fr.state == 1) {
+            fr.left = ret;
+            fr.state = 2;
+            st[sp + 1] = {fr.off + n2, fr.n - n2, 0, 0.0};
+            sp++;
+        } else {
+            ret = fr.left + ret;
+            sp--;
+        }
+    }
+    return ret;
+}
+
+// spawn totals; if the fixed-point sum is not exact in float64, the serial
+// path reproduces numpy's cumsum (sequential) and sum (pairwise) instead
+__global__ void spawn_totals_kernel(int64_t n, const int* rank, const ll* cdf_fx, const int* comp_idx, double* comp_cdf,
+                                    const float* precip, ll n_particles, Scalars* sc, ll* stats) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    const ll M = rank[n - 1];
+    sc->eligible = M;
+    stats[PG_FS_ELIGIBLE] = M;
+    if (M == 0) {
+        sc->volume0 = 0.0;
+        return;
+    }
+    const ll tot = cdf_fx[n - 1];
+    const bool exact = tot < (1ll << 53);
+    sc->cdf_exact = exact;
+    double total;
+    if (exact) {
+        total = (double)tot * ldexp(1.0, sc->emin - 24);
+    } else {
+        double acc = 0.0;
+        for (ll r = 0; r < M; r++) {
+            acc += (double)precip[comp_idx[r]];
+            comp_cdf[r] = acc;
+        }
+        total = pairwise_sum(precip, comp_idx, M);
+        stats[PG_FS_CDF_SERIAL] = 1;
+    }
+    sc->total = total;
+    sc->volume0 = total / (double)n_particles;
+    stats[PG_FS_VOLUME0_BITS] = __double_as_longlong(sc->volume0);
+    stats[PG_FS_RAIN] = n_particles;
+}
+
+// rain droplets: the p-th smallest uniform picks the cell, the p-th offset
+// pair places the droplet inside it (maps.py:821-833, particle.py:1245-1260)
+__global__ void rain_drops_kernel(ll P, const double* u_sorted, const double* draws, const int* comp_idx,
+                                  const double* comp_cdf, const Scalars* sc, int64_t N, int64_t H, int* d_cell,
+                                  float* d_fx, float* d_fy, ll* d_load, float* d_sa, float* d_sb) {
+    const ll M = sc->eligible;
+    if (M == 0) return;
+    const double total = comp_cdf[M - 1];
+    GRID_STRIDE(p, P) {
+        const double target = u_sorted[p] * total;
+        ll lo = 0, hi = M - 1;
+        while (lo < hi) {
+            const ll mid = lo + (hi - lo) / 2;
+            if (comp_cdf[mid] <= target) lo = mid + 1;
+            else hi = mid;
+        }
+        d_cell[p] = comp_idx[lo];
+        d_fx[p] = (float)draws[P + 2 * p];
+        d_fy[p] = (float)draws[P + 2 * p + 1];
+        d_load[p] = 0;
+        d_sa[p] = 0.0f;
+        d_sb[p] = 0.0f;
+    }
+}
+
+__global__ void pending_flags_kernel(int64_t n, const ll* pend_fx, int* flag) {
+    GRID_STRIDE(c, n) flag[c] = pend_fx[c] > 0 ? 1 : 0;
+}
+
+// stockpiles -> loaded droplets appended after the rain, in cell order, each
+// starting at its cell centre in a direction drawn after the offsets
+// (maps.py:886-903)
+__global__ void pending_drops_kernel(int64_t n, const int* flag, const int* rank, ll* pend_fx, ll P, ull slo, ull shi,
+                                     ull ilo, ull ihi, const Scalars* sc, int* d_cell, float* d_fx, float* d_fy,
+                                     ll* d_load, float* d_sa, float* d_sb, ll* stats) {
+    const u128 s0 = ((u128)shi << 64) | slo, inc = ((u128)ihi << 64) | ilo;
+    ll released = 0;
+    const bool any = sc->eligible > 0;
+    GRID_STRIDE(c, n) {
+        if (!flag[c]) continue;
+        const ll r = rank[c] - 1;
+        const ll load = pend_fx[c];
+        if (any) {
+            const ll d = P + r;
+            const double u = pcg_double(pcg_advance(s0, inc, (ull)(3 * P + r + 1)));
+            double sn, cs;
+            sincos(u * (2.0 * PI_D), &sn, &cs);
+            d_cell[d] = (int)c;
+            d_fx[d] = 0.5f;
+            d_fy[d] = 0.5f;
+            d_load[d] = load;
+            d_sa[d] = (float)cs;
+            d_sb[d] = (float)sn;
+            pend_fx[c] = 0;
+            released += load;
+        }
+    }
+    for (int o = 32; o > 0; o >>= 1) released += __shfl_xor(released, o);
+    if ((threadIdx.x & 63) == 0 && released) atomicAdd((ull*)&stats[PG_FS_RELEASED_FX], (ull)released);
+}
+
+__global__ void totals_kernel(int64_t n, const int* rank, ll P, Scalars* sc, ll* stats, int chunks) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    const ll k = sc->eligible > 0 ? (ll)rank[n - 1] : 0;
+    sc->pending = k;
+    sc->n_total = sc->eligible > 0 ? P + k : 0;
+    stats[PG_FS_PARTICLES] = sc->n_total;
+    const ll C = (sc->n_total + chunks - 1) / chunks;
+    for (int c = 0; c < chunks; c++) sc->queue[c] = c * C;
+}
+
+// ---------------------------------------------------------------------------
+// the persistent trace kernel
+// ---------------------------------------------------------------------------
+struct TraceCfg {
+    const float4* cell4;
+    const float4* geo;
+    const float2* dyn;
+    const uint8_t* mask;
+    int N, H, NE, spherical, deposit_on_exit, chunk, chunks;
+    const int* d_cell;
+    const float *d_fx, *d_fy, *d_sa, *d_sb;
+    const ll* d_load;
+    Scalars* sc;
+    // parameters (float32)
+    float dt, density, fr, rate, dt_evap, k_mom, mom_scale, k_disc, inv_sat, disc_exponent, slope_gain, slope_sat,
+        erodibility, cover_depth, max_erode, ocean_rate, fan_room, dep_floor, min_volume_frac;
+    int max_steps, pit_steps, ocean_steps;
+    double hscale, tscale;
+    ll* chunk_fx;
+    ll* dis_fx;
+    ll* mom_fx;
+    ll* stats;
+};
+
+struct Stencil {
+    float4 c00, c10, c01, c11;  // records at (i0, j0), (i0+1, j0), (i0, j0+1), (i0+1, j0+1)
+    float wi, wj;
+};
+
+__device__ __forceinline__ float lerp2(float v00, float v10, float v01, float v11, float wi, float wj) {
+    return v00 * (1.0f - wi) * (1.0f - wj) + v10 * wi * (1.0f - wj) + v01 * (1.0f - wi) * wj + v11 * wi * wj;
+}
+
+// the 2x2 stencil around a position given as (cell, fraction) on face f
+__device__ __forceinline__ Stencil load_stencil(const TraceCfg& k, int f, int ix, int iy, float fx, float fy) {
+    int i0 = ix + k.H - (fx < 0.5f ? 1 : 0), j0 = iy + k.H - (fy < 0.5f ? 1 : 0);
+    float wi = fx < 0.5f ? fx + 0.5f : fx - 0.5f, wj = fy < 0.5f ? fy + 0.5f : fy - 0.5f;
+    if (i0 < 0) { i0 = 0; wi = 0.0f; }
+    if (i0 > k.NE - 2) { i0 = k.NE - 2; wi = 1.0f; }
+    if (j0 < 0) { j0 = 0; wj = 0.0f; }
+    if (j0 > k.NE - 2) { j0 = k.NE - 2; wj = 1.0f; }
+    const int64_t b = ((int64_t)f * k.NE + i0) * k.NE + j0;
+    Stencil s;
+    s.c00 = k.cell4[b];
+    s.c01 = k.cell4[b + 1];
+    s.c10 = k.cell4[b + k.NE];
+    s.c11 = k.cell4[b + k.NE + 1];
+    s.wi = wi;
+    s.wj = wj;
+    return s;
+}
+
+__device__ __forceinline__ ll wave_sum(ll v) {
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    return v;
+}
+
+__device__ __forceinline__ void emit(const TraceCfg& k, int64_t cell, float delta, float vol, float sa, float sb,
+                                     ll& carried) {
+    const ll dq = llrint((double)delta * k.hscale);
+    if (dq) {
+        atomicAdd((ull*)&k.chunk_fx[cell], (ull)dq);
+        carried -= dq;
+    }
+    if (vol > 0.0f) {
+        atomicAdd((ull*)&k.dis_fx[cell], (ull)llrint((double)vol * k.tscale));
+        atomicAdd((ull*)&k.mom_fx[2 * cell], (ull)llrint((double)(vol * sa) * k.tscale));
+        atomicAdd((ull*)&k.mom_fx[2 * cell + 1], (ull)llrint((double)(vol * sb) * k.tscale));
+    }
+}
+
+__global__ void __launch_bounds__(TRACE_TPB) trace_fast_kernel(TraceCfg k) {
+    const int lane = threadIdx.x & 63;
+    const ll n_total = k.sc->n_total;
+    const ll C = (n_total + k.chunks - 1) / k.chunks;
+    const ll d_end = (k.chunk + 1) * C < n_total ? (k.chunk + 1) * C : n_total;
+    const double volume0 = k.sc->volume0;
+    if (volume0 <= 0.0) return;
+    const float inv_v0 = (float)(1.0 / volume0);
+    const float min_volume = (float)(k.min_volume_frac * volume0);
+    const float conc_scale = (float)((double)C / (volume0 * (double)n_total));
+    const float v0f = (float)volume0;
+    ll* queue = &k.sc->queue[k.chunk];
+
+    // lane state
+    bool alive = false, exhausted = false;
+    int f = 0, ix = 0, iy = 0;
+    float fx = 0, fy = 0, vol = 0, sed = 0, sa = 0, sb = 0, h_prev = 0;
+    bool has_prev = false, in_sea = false;
+    int step = 0, uphill = 0, sea_steps = 0, pushes = 0, last_cell = -1;
+    ll carried = 0;
+    Stencil S;
+    // lane totals
+    ll n_entries = 0, n_moves = 0, exit_fx = 0, vanished_fx = 0;
+    ll deaths[N_DEATHS] = {0, 0, 0, 0, 0, 0};
+
+    while (true) {
+        // refill the lanes whose droplet died from the chunk's queue
+        if (!exhausted) {
+            const ull dead = __ballot(!alive);
+            if (dead) {
+                const int leader = __ffsll((ll)dead) - 1;
+                ll base = 0;
+                if (lane == leader) base = (ll)atomicAdd((ull*)queue, (ull)__popcll(dead));
+                base = __shfl(base, leader);
+                if (base >= d_end) exhausted = true;
+                if (!alive) {
+                    const ll d = base + __popcll(dead & ((1ull << lane) - 1));
+                    if (d < d_end) {
+                        const int cell = k.d_cell[d];
+                        const int NE2 = k.NE * k.NE;
+                        f = cell / NE2;
+                        const int r = cell - f * NE2;
+                        ix = r / k.NE - k.H;
+                        iy = r % k.NE - k.H;
+                        fx = k.d_fx[d];
+                        fy = k.d_fy[d];
+                        carried = k.d_load[d];
+                        vol = v0f;
+                        sed = (float)((double)carried / k.hscale);
+                        sa = k.d_sa[d];
+                        sb = k.d_sb[d];
+                        has_prev = false;
+                        in_sea = false;
+                        step = uphill = sea_steps = pushes = 0;
+                        last_cell = -1;
+                        S = load_stencil(k, f, ix, iy, fx, fy);
+                        alive = true;
+                    }
+                }
+            }
+        }
+        if (!__any(alive)) break;
+        if (!alive) continue;
+
+        // ---- one step of this lane's droplet (particle.py:421-769)
+        int cause = -1;
+        bool exited = false;
+        const int ci = ix < 0 ? 0 : (ix > k.N - 1 ? k.N - 1 : ix);
+        const int cj = iy < 0 ? 0 : (iy > k.N - 1 ? k.N - 1 : iy);
+        const int64_t cell = ((int64_t)f * k.NE + ci + k.H) * k.NE + cj + k.H;
+        const int m = step < k.max_steps ? k.mask[cell] : MASK_ACTIVE;
+        float h_here = 0.0f, b_here = 0.0f;
+        if (step >= k.max_steps) {
+            cause = DEATH_AGE;
+        } else if (m == MASK_OUTSIDE) {
+            cause = DEATH_EXIT;
+            exited = true;
+        } else {
+            const float4 rec = k.cell4[cell];
+            const float2 dy = k.dyn[cell];
+            h_here = rec.x;
+            b_here = dy.y;
+            if (step == 0 && h_here < b_here) in_sea = true;
+            if (in_sea) {
+                if (h_here >= b_here || sed * vol * inv_v0 < 1e-4f || sea_steps >= k.ocean_steps) cause = DEATH_OCEAN;
+                else sea_steps += 1;
+            }
+            if (cause < 0) {
+                if (m == MASK_ACTIVE) {
+                    last_cell = (int)cell;
+                    pushes += 1;
+                }
+                const float4 g0 = k.geo[2 * cell], g1 = k.geo[2 * cell + 1];
+                // forces
+                const float h00 = S.c00.x, h10 = S.c10.x, h01 = S.c01.x, h11 = S.c11.x, wi = S.wi, wj = S.wj;
+                const float h0 = lerp2(h00, h10, h01, h11, wi, wj);
+                const float gi = (h10 - h00) * (1.0f - wj) + (h11 - h01) * wj;
+                const float gj = (h01 - h00) * (1.0f - wi) + (h11 - h10) * wi;
+                const float di = g0.w * gi + g1.x * gj;
+                const float dj = g1.x * gi + g1.y * gj;
+                const float s2 = fmaxf(gi * di + gj * dj, 0.0f);
+                const float nrm = k.slope_sat > 0.0f ? k.slope_gain * rsqrtf(s2 + k.slope_sat * k.slope_sat)
+                                                    : k.slope_gain * rsqrtf(1.0f + s2);
+                const float na = -di * nrm, nb = -dj * nrm;
+                const float q = fmaxf(lerp2(S.c00.y, S.c10.y, S.c01.y, S.c11.y, wi, wj), 0.0f);
+                const float vol_rel = vol * inv_v0;
+                sa *= k.fr;
+                sb *= k.fr;
+                if (k.k_mom != 0.0f) {
+                    const float ma = lerp2(S.c00.z, S.c10.z, S.c01.z, S.c11.z, wi, wj);
+                    const float mb = lerp2(S.c00.w, S.c10.w, S.c01.w, S.c11.w, wi, wj);
+                    const float mlen = sqrtf(fmaxf(g0.x * ma * ma + 2.0f * g0.y * ma * mb + g0.z * mb * mb, 0.0f));
+                    const float slen = sqrtf(fmaxf(g0.x * sa * sa + 2.0f * g0.y * sa * sb + g0.z * sb * sb, 0.0f));
+                    if (mlen > 1e-12f && slen > 1e-12f) {
+                        const float cosang = (g0.x * ma * sa + g0.y * (ma * sb + mb * sa) + g0.z * mb * sb) / (mlen * slen);
+                        const float kk = k.k_mom * cosang * k.mom_scale / (vol + k.mom_scale * q);
+                        sa += kk * ma;
+                        sb += kk * mb;
+                    }
+                }
+                const float acc = k.dt / (vol_rel * k.density);
+                sa += acc * na;
+                sb += acc * nb;
+                const float n2 = fmaxf(g0.x * sa * sa + 2.0f * g0.y * sa * sb + g0.z * sb * sb, 0.0f);
+                if (n2 < 1e-24f) {
+                    cause = DEATH_STOP;
+                } else {
+                    const float inv = rsqrtf(n2);
+                    sa *= inv;
+                    sb *= inv;
+                    // one cell step, across a face edge if need be
+                    float nfx = fx + sa, nfy = fy + sb;
+                    float flx = floorf(nfx), fly = floorf(nfy);
+                    int nix = ix + (int)flx, niy = iy + (int)fly;
+                    nfx = fminf(nfx - flx, 0.99999994f);
+                    nfy = fminf(nfy - fly, 0.99999994f);
+                    int nf = f;
+                    if (k.spherical) {
+                        if (nix < 0 || nix >= k.N || niy < 0 || niy >= k.N) {
+                            const double Nd = (double)k.N;
+                            const Transfer t = transfer_velocity<LibMath>(f, ((double)nix + nfx) / Nd, ((double)niy + nfy) / Nd,
+                                                                          (double)sa, (double)sb);
+                            nf = t.face;
+                            double nx = t.u * Nd, ny = t.v * Nd;
+                            if (nx >= Nd) nx = Nd * 0.999999999;
+                            if (ny >= Nd) ny = Nd * 0.999999999;
+                            if (nx < 0.0) nx = 0.0;
+                            if (ny < 0.0) ny = 0.0;
+                            const double fnx = floor(nx), fny = floor(ny);
+                            nix = (int)fnx;
+                            niy = (int)fny;
+                            nfx = fminf((float)(nx - fnx), 0.99999994f);
+                            nfy = fminf((float)(ny - fny), 0.99999994f);
+                            sa = (float)t.a;
+                            sb = (float)t.b;
+                        }
+                    } else if (nix < -k.H || nix >= k.N + k.H || niy < -k.H || niy >= k.N + k.H) {
+                        cause = DEATH_EXIT;
+                        exited = true;
+                    }
+                    if (cause < 0) {
+                        const int64_t ncell = ((int64_t)nf * k.NE + nix + k.H) * k.NE + niy + k.H;
+                        if (k.mask[ncell] == MASK_OUTSIDE) {
+                            cause = DEATH_EXIT;
+                            exited = true;
+                        } else {
+                            const Stencil S1 = load_stencil(k, nf, nix, niy, nfx, nfy);
+                            const float4 nrec = k.cell4[ncell];
+                            const float b1 = k.dyn[ncell].y;
+                            const float h1 = lerp2(S1.c00.x, S1.c10.x, S1.c01.x, S1.c11.x, S1.wi, S1.wj);
+                            const float dh = h0 - fmaxf(h1, b1);
+                            // mass transfer at the cell being left
+                            if (m == MASK_ACTIVE && in_sea) {
+                                float cdiff = -k.ocean_rate * sed;
+                                float room = has_prev ? h_prev - h_here : k.fan_room;
+                                room = fmaxf(room, k.fan_room);
+                                room = fminf(room, b_here - k.dep_floor - h_here);
+                                room = fmaxf(room, 0.0f);
+                                if (-cdiff * vol_rel > room) cdiff = -room / vol_rel;
+                                sed += cdiff;
+                                emit(k, cell, -cdiff * vol_rel, 0.0f, 0.0f, 0.0f, carried);
+                                n_entries += 1;
+                            } else if (m == MASK_ACTIVE) {
+                                float qe;
+                                if (k.inv_sat <= 0.0f) qe = 0.0f;
+                                else if (k.disc_exponent == 0.5f) qe = sqrtf(q * k.inv_sat);
+                                else if (k.disc_exponent > 0.0f) qe = powf(q * k.inv_sat, k.disc_exponent);
+                                else qe = erff(q * k.inv_sat);
+                                const float c_eq = fmaxf(k.erodibility * dh * (1.0f + k.k_disc * qe), 0.0f);
+                                float k_e = k.rate;
+                                if (c_eq > sed) {
+                                    const float sthick = k.dyn[cell].x;
+                                    const float w_cover = k.cover_depth > 0.0f ? fminf(sthick / k.cover_depth, 1.0f)
+                                                                               : (sthick > 0.0f ? 1.0f : 0.0f);
+                                    k_e = k.rate * (w_cover + (1.0f - w_cover) * (1.0f - g1.w));
+                                }
+                                float cdiff = k_e * (c_eq - sed);
+                                const float kexp = 1.0f + q * conc_scale;
+                                if (cdiff > 0.0f) {
+                                    float ecap = fminf(dh / kexp, k.max_erode);
+                                    if (ecap > h_here) ecap = fmaxf(h_here, 0.0f);
+                                    if (cdiff * vol_rel > ecap) cdiff = ecap / vol_rel;
+                                } else if (cdiff < 0.0f) {
+                                    float room = has_prev ? h_prev - h_here : k.dep_floor;
+                                    room = fmaxf(room, k.dep_floor);
+                                    const float dcap = room / kexp;
+                                    if (-cdiff * vol_rel > dcap) cdiff = -dcap / vol_rel;
+                                }
+                                sed += cdiff;
+                                emit(k, cell, -cdiff * vol_rel, vol, sa, sb, carried);
+                                n_entries += 1;
+                            }
+                            // evaporation, advance, stop tests
+                            const float ev = fmaxf(1.0f - k.dt_evap * g1.z, 0.01f);
+                            vol *= ev;
+                            sed /= ev;
+                            f = nf;
+                            ix = nix;
+                            iy = niy;
+                            fx = nfx;
+                            fy = nfy;
+                            S = S1;
+                            h_prev = h_here;
+                            has_prev = true;
+                            n_moves += 1;
+                            step += 1;
+                            if (nrec.x < k.dyn[ncell].y && !in_sea) {
+                                in_sea = true;
+                                if (sed <= 0.0f) cause = DEATH_OCEAN;
+                            }
+                            if (cause < 0 && vol < min_volume) cause = in_sea ? DEATH_OCEAN : DEATH_EVAP;
+                            if (cause < 0) {
+                                if (h0 - h1 < 0.0f) {
+                                    uphill += 1;
+                                    if (uphill > k.pit_steps) cause = in_sea ? DEATH_OCEAN : DEATH_PIT;
+                                } else {
+                                    uphill = 0;
+                                }
+                            }
+                            if (cause < 0 && step >= k.max_steps) cause = DEATH_AGE;
+                        }
+                    }
+                }
+            }
+        }
+        if (cause < 0) continue;
+
+        // ---- the droplet stopped: final deposit (particle.py:770-815)
+        if (in_sea && (cause == DEATH_AGE || cause == DEATH_STOP)) cause = DEATH_OCEAN;
+        deaths[cause] += 1;
+        if (!exited) {
+            const int sci = ix < 0 ? 0 : (ix > k.N - 1 ? k.N - 1 : ix);
+            const int scj = iy < 0 ? 0 : (iy > k.N - 1 ? k.N - 1 : iy);
+            const int64_t scell = ((int64_t)f * k.NE + sci + k.H) * k.NE + scj + k.H;
+            if (k.mask[scell] == MASK_ACTIVE && !(in_sea && k.cell4[scell].x >= k.dyn[scell].y)) {
+                if (pushes == 0 || last_cell != (int)scell) {
+                    last_cell = (int)scell;
+                    pushes += 1;
+                }
+            }
+        }
+        if (pushes > 0 && (!exited || k.deposit_on_exit)) {
+            if (carried) atomicAdd((ull*)&k.chunk_fx[last_cell], (ull)carried);
+            if (sed > 0.0f) n_entries += cause == DEATH_OCEAN ? 1 : (pushes < SPREAD ? pushes : SPREAD);
+        } else if (exited) {
+            exit_fx += carried;
+        } else {
+            vanished_fx += carried;
+        }
+        carried = 0;
+        alive = false;
+    }
+    // lane totals -> device statistics (one atomic per wave and counter)
+    ll v[4 + N_DEATHS] = {n_entries, n_moves, exit_fx, vanished_fx};
+    for (int c = 0; c < N_DEATHS; c++) v[4 + c] = deaths[c];
+    for (int c = 0; c < 4 + N_DEATHS; c++) v[c] = wave_sum(v[c]);
+    if (lane == 0) {
+        const int slot[4] = {PG_FS_ENTRIES, PG_FS_MOVES, PG_FS_EXIT_FX, PG_FS_VANISHED_FX};
+        for (int c = 0; c < 4; c++)
+            if (v[c]) atomicAdd((ull*)&k.stats[slot[c]], (ull)v[c]);
+        for (int c = 0; c < N_DEATHS; c++)
+            if (v[4 + c]) atomicAdd((ull*)&k.stats[PG_FS_DEATHS + c], (ull)v[4 + c]);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// per-cell resolve of a chunk against the live terrain
+// ---------------------------------------------------------------------------
+__global__ void resolve_kernel(int64_t n, const double* height, const double* sediment, const double* base, ll* chunk_fx,
+                               ll* acc_fx, ll* pend_fx, float4* cell4, float2* dyn, double iter_deposit,
+                               double dep_floor, double writeoff, double hscale, ll* stats) {
+    const double inv = 1.0 / hscale;
+    ll clamped = 0, to_pend = 0, lost_off = 0;
+    GRID_STRIDE(c, n) {
+        const ll net = chunk_fx[c];
+        if (net == 0) continue;
+        chunk_fx[c] = 0;
+        const ll acc = acc_fx[c];
+        const double s_c = height[c] + sediment[c] + (double)acc * inv;
+        ll applied = net;
+        if (net > 0) {
+            const double b = base[c];
+            double room = iter_deposit - (double)acc * inv;
+            const bool sub = s_c < b;
+            if (sub) room = fmin(room, b - dep_floor - s_c);
+            const ll room_fx = room > 0.0 ? (ll)floor(room * hscale) : 0;
+            if (net > room_fx) {
+                const ll surplus = net - room_fx;
+                applied = room_fx;
+                clamped += 1;
+                ll keep = surplus;
+                if (sub) {
+                    keep = (ll)floor((double)surplus * (1.0 - writeoff));
+                    if ((double)keep * inv < 1e-4) keep = 0;
+                    lost_off += surplus - keep;
+                }
+                pend_fx[c] += keep;
+                to_pend += keep;
+            }
+        }
+        const ll a2 = acc + applied;
+        acc_fx[c] = a2;
+        const double live = height[c] + sediment[c] + (double)a2 * inv;
+        const double sd = sediment[c] + (double)a2 * inv;
+        cell4[c].x = (float)live;
+        dyn[c].x = (float)(sd > 0.0 ? sd : 0.0);
+    }
+    clamped = wave_sum(clamped);
+    to_pend = wave_sum(to_pend);
+    lost_off = wave_sum(lost_off);
+    if ((threadIdx.x & 63) == 0) {
+        if (clamped) atomicAdd((ull*)&stats[PG_FS_CLAMPED], (ull)clamped);
+        if (to_pend) atomicAdd((ull*)&stats[PG_FS_TO_PENDING_FX], (ull)to_pend);
+        if (lost_off) atomicAdd((ull*)&stats[PG_FS_LOST_OFFSHORE_FX], (ull)lost_off);
+    }
+}
+
+// fold (particle.py:1079-1101) + EMA (particle.py:1230-1242) + pending mirror
+__global__ void finish_kernel(int64_t n, double* height, double* sediment, double* discharge, double* momentum,
+                              double* pending, const uint8_t* mask, ll* acc_fx, ll* dis_fx, ll* mom_fx, const ll* pend_fx,
+                              double ema, double hscale, double tscale, ll* stats) {
+    const double hinv = 1.0 / hscale, tinv = 1.0 / tscale;
+    ll eroded = 0, deposited = 0;
+    GRID_STRIDE(c, n) {
+        const ll af = acc_fx[c];
+        acc_fx[c] = 0;
+        const bool active = mask[c] == MASK_ACTIVE;
+        if (af != 0 && active) {
+            const double a = (double)af * hinv;
+            if (af > 0) {
+                sediment[c] += a;
+                deposited += af;
+            } else {
+                eroded -= af;
+                const double take = -a, s = sediment[c];
+                if (s >= take) {
+                    sediment[c] = s - take;
+                } else {
+                    height[c] -= take - s;
+                    sediment[c] = 0.0;
+                }
+            }
+        }
+        const ll d = dis_fx[c], m0 = mom_fx[2 * c], m1 = mom_fx[2 * c + 1];
+        if (active) {
+            discharge[c] = (1.0 - ema) * discharge[c] + ema * ((double)d * tinv);
+            momentum[2 * c] = (1.0 - ema) * momentum[2 * c] + ema * ((double)m0 * tinv);
+            momentum[2 * c + 1] = (1.0 - ema) * momentum[2 * c + 1] + ema * ((double)m1 * tinv);
+        }
+        dis_fx[c] = 0;
+        mom_fx[2 * c] = 0;
+        mom_fx[2 * c + 1] = 0;
+        pending[c] = (double)pend_fx[c] * hinv;
+    }
+    eroded = wave_sum(eroded);
+    deposited = wave_sum(deposited);
+    if ((threadIdx.x & 63) == 0) {
+        if (eroded) atomicAdd((ull*)&stats[PG_FS_ERODED_FX], (ull)eroded);
+        if (deposited) atomicAdd((ull*)&stats[PG_FS_DEPOSITED_FX], (ull)deposited);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// workspace layout
+// ---------------------------------------------------------------------------
+struct Layout {
+    size_t cell4, geo, dyn, chunk_fx, acc_fx, dis_fx, mom_fx, pend_fx, wq, cdf_fx, flag, rank, comp_idx, comp_cdf,
+        draws, u_sorted, d_cell, d_fx, d_fy, d_load, d_sa, d_sb, scalars, temp, total;
+    size_t temp_bytes;
+};
+
+inline size_t up(size_t x) { return (x + 255) & ~(size_t)255; }
+
+int layout_of(int64_t F, int64_t NE, int64_t P, Layout* L) {
+    const size_t n = (size_t)(F * NE * NE), p = (size_t)P, nd = p + n;
+    size_t o = 0;
+    auto take = [&](size_t bytes) { size_t at = o; o += up(bytes); return at; };
+    L->cell4 = take(16 * n);
+    L->geo = take(32 * n);
+    L->dyn = take(8 * n);
+    L->chunk_fx = take(8 * n);
+    L->acc_fx = take(8 * n);
+    L->dis_fx = take(8 * n);
+    L->mom_fx = take(16 * n);
+    L->pend_fx = take(8 * n);
+    L->wq = take(8 * n);
+    L->cdf_fx = take(8 * n);
+    L->flag = take(4 * n);
+    L->rank = take(4 * n);
+    L->comp_idx = take(4 * n);
+    L->comp_cdf = take(8 * n);
+    L->draws = take(8 * 3 * p);
+    L->u_sorted = take(8 * p);
+    L->d_cell = take(4 * nd);
+    L->d_fx = take(4 * nd);
+    L->d_fy = take(4 * nd);
+    L->d_load = take(8 * nd);
+    L->d_sa = take(4 * nd);
+    L->d_sb = take(4 * nd);
+    L->scalars = take(sizeof(Scalars) + 8 * 4096);
+    size_t t1 = 0, t2 = 0, t3 = 0;
+    if (hipcub::DeviceRadixSort::SortKeys(nullptr, t1, (const double*)nullptr, (double*)nullptr, (int)(p > 0 ? p : 1)) != hipSuccess)
+        return -1;
+    if (hipcub::DeviceScan::InclusiveSum(nullptr, t2, (const ll*)nullptr, (ll*)nullptr, (int)n) != hipSuccess) return -1;
+    if (hipcub::DeviceScan::InclusiveSum(nullptr, t3, (const int*)nullptr, (int*)nullptr, (int)n) != hipSuccess) return -1;
+    size_t t = t1 > t2 ? t1 : t2;
+    t = t > t3 ? t : t3;
+    L->temp_bytes = t;
+    L->temp = take(t + 256);
+    L->total = o;
+    return 0;
+}
+
+template <class T>
+T* at(void* work, size_t off) {
+    return reinterpret_cast<T*>(reinterpret_cast<char*>(work) + off);
+}
+
+int check_fields(const pg_fast_fields* fl) {
+    PG_REQUIRE(fl != nullptr, "fast mode: fields is NULL");
+    PG_REQUIRE(fl->F > 0 && fl->N >= 2 && fl->H >= 1, "fast mode: bad grid F=%lld N=%lld H=%lld", (long long)fl->F,
+               (long long)fl->N, (long long)fl->H);
+    PG_REQUIRE((fl->F * (fl->N + 2 * fl->H) * (fl->N + 2 * fl->H)) < (1ll << 31), "fast mode: grid too large for int32 cell ids");
+    PG_REQUIRE(fl->height && fl->sediment && fl->discharge && fl->momentum && fl->pending && fl->base && fl->precip &&
+                   fl->evap && fl->hardness && fl->metric && fl->metric_inv && fl->mask,
+               "fast mode: NULL field pointer");
+    return 0;
+}
+
+}  // namespace
+}  // namespace pg
+
+using namespace pg;
+
+extern "C" int64_t pg_fast_workspace_bytes(int64_t F, int64_t NE, int64_t n_particles) {
+    clear_error();
+    Layout L;
+    if (F <= 0 || NE <= 1 || n_particles < 0) {
+        set_error("pg_fast_workspace_bytes: bad sizes");
+        return -1;
+    }
+    if (layout_of(F, NE, n_particles, &L) != 0) {
+        set_error("pg_fast_workspace_bytes: hipcub temp-size query failed");
+        return -1;
+    }
+    return (int64_t)L.total;
+}
+
+extern "C" int pg_pcg64_random(uint64_t slo, uint64_t shi, uint64_t ilo, uint64_t ihi, int64_t start, int64_t n,
+                               double* out, pg_stream stream) {
+    clear_error();
+    PG_REQUIRE(n >= 0 && start >= 0 && (out || n == 0), "pg_pcg64_random: bad arguments");
+    if (n == 0) return 0;
+    const int64_t threads = (n + 15) / 16;
+    hipLaunchKernelGGL(pcg_fill_kernel, dim3(grid_for(threads, 8192)), dim3(TPB), 0, (hipStream_t)stream, slo, shi, ilo,
+                       ihi, (ll)start, (ll)n, out);
+    PG_LAUNCHED("pcg_fill_kernel");
+    return 0;
+}
+
+extern "C" int pg_fast_setup(const pg_fast_fields* fl, const pg_fast_params* fp, void* work, int64_t work_bytes,
+                             pg_stream stream) {
+    clear_error();
+    if (check_fields(fl)) return -1;
+    PG_REQUIRE(fp && work, "pg_fast_setup: NULL argument");
+    const int64_t NE = fl->N + 2 * fl->H, n = fl->F * NE * NE;
+    Layout L;
+    PG_REQUIRE(layout_of(fl->F, NE, fp->n_particles, &L) == 0, "pg_fast_setup: hipcub temp-size query failed");
+    PG_REQUIRE((int64_t)L.total <= work_bytes, "pg_fast_setup: workspace of %lld bytes, need %lld", (long long)work_bytes,
+               (long long)L.total);
+    PG_REQUIRE(fp->height_bits > 0 && fp->height_bits < 62, "pg_fast_setup: height_bits out of range");
+    hipStream_t s = (hipStream_t)stream;
+    hipLaunchKernelGGL(setup_kernel, dim3(grid_for(n)), dim3(TPB), 0, s, n, fl->metric, fl->metric_inv, fl->evap,
+                       fl->hardness, at<float4>(work, L.geo), fl->pending, at<ll>(work, L.pend_fx),
+                       ldexp(1.0, (int)fp->height_bits));
+    PG_LAUNCHED("setup_kernel");
+    PG_HIP(hipMemsetAsync(at<char>(work, L.chunk_fx), 0, L.pend_fx - L.chunk_fx, s));
+    return 0;
+}
+
+namespace pg {
+namespace {
+struct Ctx {
+    Layout L;
+    hipStream_t s;
+    double hscale, tscale;
+    Scalars* sc;
+    float4* cell4;
+    float2* dyn;
+    int *flag, *rank;
+    ll *pend_fx, *st;
+    void* temp;
+    int chunks;
+    int64_t N, H, NE, n;
+    ll P;
+};
+
+int prepare(const pg_fast_fields* fl, const pg_trace_params* prm, const pg_fast_params* fp, void* work,
+            int64_t work_bytes, int64_t* stats, pg_stream stream, Ctx* c) {
+    if (check_fields(fl)) return -1;
+    PG_REQUIRE(prm && fp && work && stats, "fast mode: NULL argument");
+    PG_REQUIRE(fp->n_particles >= 1 && fp->n_particles < (1ll << 30), "fast mode: n_particles out of range");
+    PG_REQUIRE(fp->chunks >= 1 && fp->chunks <= 4096, "fast mode: chunks must be in [1, 4096]");
+    PG_REQUIRE(prm->max_steps >= 0 && prm->max_steps < (1ll << 30), "fast mode: bad max_steps");
+    PG_REQUIRE(prm->lateral_rate <= 0.0 && prm->lake_park <= 0.0,
+               "fast mode: lateral cuts and lake parks are outside the fast mode (plain profile)");
+    PG_REQUIRE(fp->height_bits > 0 && fp->height_bits < 62 && fp->track_bits > 0 && fp->track_bits < 62,
+               "fast mode: fixed-point bits out of range");
+    c->N = fl->N;
+    c->H = fl->H;
+    c->NE = fl->N + 2 * fl->H;
+    c->n = fl->F * c->NE * c->NE;
+    c->P = fp->n_particles;
+    PG_REQUIRE(layout_of(fl->F, c->NE, c->P, &c->L) == 0, "fast mode: hipcub temp-size query failed");
+    PG_REQUIRE((int64_t)c->L.total <= work_bytes, "fast mode: workspace of %lld bytes, need %lld",
+               (long long)work_bytes, (long long)c->L.total);
+    c->s = (hipStream_t)stream;
+    c->hscale = ldexp(1.0, (int)fp->height_bits);
+    c->tscale = ldexp(1.0, (int)fp->track_bits);
+    c->sc = at<Scalars>(work, c->L.scalars);
+    c->cell4 = at<float4>(work, c->L.cell4);
+    c->dyn = at<float2>(work, c->L.dyn);
+    c->flag = at<int>(work, c->L.flag);
+    c->rank = at<int>(work, c->L.rank);
+    c->pend_fx = at<ll>(work, c->L.pend_fx);
+    c->st = (ll*)stats;
+    c->temp = at<void>(work, c->L.temp);
+    c->chunks = (int)fp->chunks;
+    return 0;
+}
+
+int fast_spawn(const pg_fast_fields* fl, const pg_fast_params* fp, void* work, Ctx& c) {
+    const int64_t N = c.N, H = c.H, n = c.n;
+    const ll P = c.P;
+    hipStream_t s = c.s;
+    Scalars* sc = c.sc;
+    float4* cell4 = c.cell4;
+    float2* dyn = c.dyn;
+    int *flag = c.flag, *rank = c.rank;
+    ll *pend_fx = c.pend_fx, *st = c.st;
+    void* temp = c.temp;
+    const Layout& L = c.L;
+    size_t tb = L.temp_bytes;
+    const int chunks = c.chunks;
+    hipLaunchKernelGGL(reset_scalars_kernel, dim3(1), dim3(256), 0, s, sc, st, (int)PG_FAST_NSTATS, chunks);
+    hipLaunchKernelGGL(pack_kernel, dim3(grid_for(n)), dim3(TPB), 0, s, n, N, H, fl->height, fl->sediment, fl->discharge,
+                       fl->momentum, fl->base, fl->precip, fl->mask, cell4, dyn, flag, sc);
+    hipLaunchKernelGGL(quantize_kernel, dim3(grid_for(n)), dim3(TPB), 0, s, n, fl->precip, flag, sc, at<ll>(work, L.wq));
+    PG_LAUNCHED("pack/quantize");
+    PG_HIP(hipcub::DeviceScan::InclusiveSum(temp, tb, at<ll>(work, L.wq), at<ll>(work, L.cdf_fx), (int)n, s));
+    tb = L.temp_bytes;
+    PG_HIP(hipcub::DeviceScan::InclusiveSum(temp, tb, flag, rank, (int)n, s));
+    hipLaunchKernelGGL(compact_kernel, dim3(grid_for(n)), dim3(TPB), 0, s, n, flag, rank, at<ll>(work, L.cdf_fx), sc,
+                       at<int>(work, L.comp_idx), at<double>(work, L.comp_cdf));
+    hipLaunchKernelGGL(spawn_totals_kernel, dim3(1), dim3(64), 0, s, n, rank, at<ll>(work, L.cdf_fx),
+                       at<int>(work, L.comp_idx), at<double>(work, L.comp_cdf), fl->precip, P, sc, st);
+    // the rain draws: P uniforms (sorted) then P offset pairs
+    double* draws = at<double>(work, L.draws);
+    hipLaunchKernelGGL(pcg_fill_kernel, dim3(grid_for((3 * P + 15) / 16, 8192)), dim3(TPB), 0, s, fp->rng_state_lo,
+                       fp->rng_state_hi, fp->rng_inc_lo, fp->rng_inc_hi, (ll)0, 3 * P, draws);
+    PG_LAUNCHED("spawn kernels");
+    tb = L.temp_bytes;
+    PG_HIP(hipcub::DeviceRadixSort::SortKeys(temp, tb, (const double*)draws, at<double>(work, L.u_sorted), (int)P, 0, 64, s));
+    int* d_cell = at<int>(work, L.d_cell);
+    float *d_fx = at<float>(work, L.d_fx), *d_fy = at<float>(work, L.d_fy), *d_sa = at<float>(work, L.d_sa),
+          *d_sb = at<float>(work, L.d_sb);
+    ll* d_load = at<ll>(work, L.d_load);
+    hipLaunchKernelGGL(rain_drops_kernel, dim3(grid_for(P)), dim3(TPB), 0, s, P, at<double>(work, L.u_sorted), draws,
+                       at<int>(work, L.comp_idx), at<double>(work, L.comp_cdf), sc, N, H, d_cell, d_fx, d_fy, d_load,
+                       d_sa, d_sb);
+    // pending stockpiles after the rain
+    hipLaunchKernelGGL(pending_flags_kernel, dim3(grid_for(n)), dim3(TPB), 0, s, n, pend_fx, flag);
+    tb = L.temp_bytes;
+    PG_HIP(hipcub::DeviceScan::InclusiveSum(temp, tb, flag, rank, (int)n, s));
+    hipLaunchKernelGGL(pending_drops_kernel, dim3(grid_for(n)), dim3(TPB), 0, s, n, flag, rank, pend_fx, P,
+                       fp->rng_state_lo, fp->rng_state_hi, fp->rng_inc_lo, fp->rng_inc_hi, sc, d_cell, d_fx, d_fy,
+                       d_load, d_sa, d_sb, st);
+    hipLaunchKernelGGL(totals_kernel, dim3(1), dim3(64), 0, s, n, rank, P, sc, st, chunks);
+    PG_LAUNCHED("pending kernels");
+
+    return 0;
+}
+
+int fast_particles(const pg_fast_fields* fl, const pg_trace_params* prm, const pg_fast_params* fp, void* work, Ctx& c) {
+    const int64_t N = c.N, H = c.H, NE = c.NE, n = c.n;
+    hipStream_t s = c.s;
+    Scalars* sc = c.sc;
+    float4* cell4 = c.cell4;
+    float2* dyn = c.dyn;
+    ll *pend_fx = c.pend_fx, *st = c.st;
+    const Layout& L = c.L;
+    const int chunks = c.chunks;
+    const double hscale = c.hscale, tscale = c.tscale;
+    int* d_cell = at<int>(work, L.d_cell);
+    float *d_fx = at<float>(work, L.d_fx), *d_fy = at<float>(work, L.d_fy), *d_sa = at<float>(work, L.d_sa),
+          *d_sb = at<float>(work, L.d_sb);
+    ll* d_load = at<ll>(work, L.d_load);
+    // chunks: persistent trace, then resolve against the live terrain
+    TraceCfg k;
+    k.cell4 = cell4;
+    k.geo = at<float4>(work, L.geo);
+    k.dyn = dyn;
+    k.mask = fl->mask;
+    k.N = (int)N;
+    k.H = (int)H;
+    k.NE = (int)NE;
+    k.spherical = (int)fl->spherical;
+    k.deposit_on_exit = (int)fl->deposit_on_exit;
+    k.chunks = chunks;
+    k.d_cell = d_cell;
+    k.d_fx = d_fx;
+    k.d_fy = d_fy;
+    k.d_sa = d_sa;
+    k.d_sb = d_sb;
+    k.d_load = d_load;
+    k.sc = sc;
+    k.dt = (float)prm->dt;
+    k.density = (float)prm->density;
+    double fr = 1.0 - prm->dt * prm->friction;
+    k.fr = (float)(fr < 0.0 ? 0.0 : fr);
+    k.rate = (float)prm->deposition_rate;
+    k.dt_evap = (float)(prm->dt * prm->evap_rate);
+    k.k_mom = (float)prm->k_mom;
+    k.mom_scale = (float)prm->mom_scale;
+    k.k_disc = (float)prm->k_disc;
+    k.inv_sat = (float)(prm->disc_saturation > 0.0 ? 1.0 / prm->disc_saturation : 0.0);
+    k.disc_exponent = (float)prm->disc_exponent;
+    k.slope_gain = (float)prm->slope_gain;
+    k.slope_sat = (float)prm->slope_sat;
+    k.erodibility = (float)prm->erodibility;
+    k.cover_depth = (float)prm->cover_depth;
+    k.max_erode = (float)prm->max_erode;
+    k.ocean_rate = (float)prm->ocean_rate;
+    k.fan_room = (float)prm->fan_room;
+    k.dep_floor = (float)prm->dep_floor;
+    k.min_volume_frac = (float)fp->min_volume_frac;
+    k.max_steps = (int)prm->max_steps;
+    k.pit_steps = (int)prm->pit_steps;
+    k.ocean_steps = (int)prm->ocean_steps;
+    k.hscale = hscale;
+    k.tscale = tscale;
+    k.chunk_fx = at<ll>(work, L.chunk_fx);
+    k.dis_fx = at<ll>(work, L.dis_fx);
+    k.mom_fx = at<ll>(work, L.mom_fx);
+    k.stats = st;
+    int grid = (int)fp->grid_blocks;
+    if (grid <= 0) {
+        int dev = 0, cus = 0, per = 0;
+        PG_HIP(hipGetDevice(&dev));
+        PG_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+        PG_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, trace_fast_kernel, TRACE_TPB, 0));
+        grid = cus * (per > 0 ? per : 1);
+    }
+    for (int c = 0; c < chunks; c++) {
+        k.chunk = c;
+        hipLaunchKernelGGL(trace_fast_kernel, dim3(grid), dim3(TRACE_TPB), 0, s, k);
+        hipLaunchKernelGGL(resolve_kernel, dim3(grid_for(n)), dim3(TPB), 0, s, n, fl->height, fl->sediment, fl->base,
+                           k.chunk_fx, at<ll>(work, L.acc_fx), pend_fx, cell4, dyn, fp->iter_deposit, prm->dep_floor,
+                           fp->offshore_writeoff, hscale, st);
+    }
+    PG_LAUNCHED("trace/resolve");
+    hipLaunchKernelGGL(finish_kernel, dim3(grid_for(n)), dim3(TPB), 0, s, n, fl->height, fl->sediment, fl->discharge,
+                       fl->momentum, fl->pending, fl->mask, at<ll>(work, L.acc_fx), k.dis_fx, k.mom_fx, pend_fx, fp->ema,
+                       hscale, tscale, st);
+    PG_LAUNCHED("finish_kernel");
+    return 0;
+}
+}  // namespace
+}  // namespace pg
+
+extern "C" int pg_fast_spawn(const pg_fast_fields* fl, const pg_trace_params* prm, const pg_fast_params* fp, void* work,
+                             int64_t work_bytes, int64_t* stats, pg_stream stream) {
+    clear_error();
+    Ctx c;
+    if (prepare(fl, prm, fp, work, work_bytes, stats, stream, &c)) return -1;
+    return fast_spawn(fl, fp, work, c);
+}
+
+extern "C" int pg_fast_particles(const pg_fast_fields* fl, const pg_trace_params* prm, const pg_fast_params* fp,
+                                 void* work, int64_t work_bytes, int64_t* stats, pg_stream stream) {
+    clear_error();
+    Ctx c;
+    if (prepare(fl, prm, fp, work, work_bytes, stats, stream, &c)) return -1;
+    return fast_particles(fl, prm, fp, work, c);
+}
+
+extern "C" int pg_fast_iterate(const pg_fast_fields* fl, const pg_trace_params* prm, const pg_fast_params* fp, void* work,
+                               int64_t work_bytes, int64_t* stats, pg_stream stream) {
+    clear_error();
+    Ctx c;
+    if (prepare(fl, prm, fp, work, work_bytes, stats, stream, &c)) return -1;
+    if (fast_spawn(fl, fp, work, c)) return -1;
+    return fast_particles(fl, prm, fp, work, c);
+}
+
+extern "C" int pg_fast_layout(int64_t F, int64_t NE, int64_t n_particles, int64_t* offsets, int n) {
+    clear_error();
+    PG_REQUIRE(offsets && n >= 16, "pg_fast_layout: need room for 16 offsets");
+    Layout L;
+    PG_REQUIRE(layout_of(F, NE, n_particles, &L) == 0, "pg_fast_layout: hipcub temp-size query failed");
+    const size_t v[16] = {L.cell4, L.geo, L.dyn, L.chunk_fx, L.acc_fx, L.dis_fx, L.mom_fx, L.pend_fx,
+                          L.d_cell, L.d_fx, L.d_fy, L.d_load, L.d_sa, L.d_sb, L.scalars, L.total};
+    for (int k = 0; k < 16; k++) offsets[k] = (int64_t)v[k];
+    return 0;
+}

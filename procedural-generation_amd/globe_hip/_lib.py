@@ -1,0 +1,120 @@
+"""ctypes binding of liberode_hip.so (the C ABI declared in include/pg_erode.h).
+
+The library is built in-tree by ``csrc/build.py`` next to this file.  There
+is no fallback: if it is missing or a call fails, a RuntimeError says why.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liberode_hip.so")
+
+vp, i64, i32, dbl = ctypes.c_void_p, ctypes.c_int64, ctypes.c_int, ctypes.c_double
+pd = ctypes.POINTER(ctypes.c_double)
+
+
+class TraceParams(ctypes.Structure):
+    """``pg_trace_params``: trace_particles' scalar arguments
+    (bake/globe/erosion/particle.py:292-318), same order."""
+
+    _fields_ = [(n, dbl) for n in (
+        "dt", "density", "friction", "deposition_rate", "evap_rate", "k_mom", "mom_scale", "k_disc", "disc_saturation",
+        "disc_exponent", "slope_gain", "slope_sat", "erodibility", "cover_depth", "min_volume")] + [
+        ("max_steps", i64), ("max_erode", dbl), ("conc_scale", dbl), ("pit_steps", i64), ("deposit_on_exit", i64),
+        ("ocean_rate", dbl), ("ocean_steps", i64)] + [(n, dbl) for n in ("fan_room", "dep_floor", "lake_trap", "lake_park", "lateral_rate")]
+
+
+class FastFields(ctypes.Structure):
+    """``pg_fast_fields``: device pointers of the erosion state."""
+
+    _fields_ = [("F", i64), ("N", i64), ("H", i64), ("spherical", i64), ("deposit_on_exit", i64)] + [
+        (n, vp) for n in ("height", "sediment", "discharge", "momentum", "pending", "base", "precip", "evap", "hardness",
+                          "metric", "metric_inv", "mask")]
+
+
+class FastParams(ctypes.Structure):
+    """``pg_fast_params``: the iteration's PCG64 state and the fast mode's knobs."""
+
+    _fields_ = [("rng_state_lo", ctypes.c_uint64), ("rng_state_hi", ctypes.c_uint64), ("rng_inc_lo", ctypes.c_uint64),
+                ("rng_inc_hi", ctypes.c_uint64), ("n_particles", i64), ("chunks", i64), ("iter_deposit", dbl),
+                ("offshore_writeoff", dbl), ("ema", dbl), ("min_volume_frac", dbl), ("height_bits", i64),
+                ("track_bits", i64), ("grid_blocks", i64)]
+
+
+#: statistics slots of a fast iteration (include/pg_erode.h, PG_FS_*)
+FS = dict(PARTICLES=0, RAIN=1, ENTRIES=2, MOVES=3, DEATHS=4, CLAMPED=10, TO_PENDING_FX=11, LOST_OFFSHORE_FX=12,
+          EXIT_FX=13, VANISHED_FX=14, RELEASED_FX=15, VOLUME0_BITS=16, ELIGIBLE=17, CDF_SERIAL=18, ERODED_FX=19,
+          DEPOSITED_FX=20)
+FAST_NSTATS = 24
+
+
+#: (name, restype, argtypes) of every exported entry point
+SIGNATURES = [
+    ("pg_last_error", ctypes.c_char_p, []),
+    ("pg_abi_version", i32, []),
+    ("pg_geom_to_sphere", i32, [i32, dbl, dbl, pd]),
+    ("pg_geom_from_sphere", i32, [dbl, dbl, dbl, ctypes.POINTER(ctypes.c_int), pd]),
+    ("pg_geom_jacobian", i32, [i32, dbl, dbl, pd]),
+    ("pg_geom_transfer_velocity", i32, [i32, dbl, dbl, dbl, dbl, ctypes.POINTER(ctypes.c_int), pd]),
+    ("pg_geom_tan", dbl, [dbl]),
+    ("pg_geom_atan", dbl, [dbl]),
+    ("pg_geom_sincos", None, [dbl, pd, pd]),
+    ("pg_geom_eval", i32, [i32, i64, vp, vp, vp, vp]),
+    ("pg_pack_samples", i32, [vp, i64, i64, vp, vp, vp, i32, vp, vp, vp, vp, vp, vp, vp, vp, vp]),
+    ("pg_trace_particles", i32, [i64, vp, vp, vp, vp, vp, dbl, vp, i32, vp, i64, i64, i32, ctypes.POINTER(TraceParams),
+                                 vp, i32, vp, vp, vp, vp, vp, vp, vp, vp, vp]),
+    ("pg_change_list_cap", i64, [i64, i32]),
+    ("pg_apply_changes", i32, [vp, vp, vp, vp, vp, i64, i64, vp, vp, vp, vp, vp, vp, vp, vp, dbl, dbl, dbl, i32, dbl, dbl,
+                               vp, i32, vp, vp, vp, vp, vp]),
+    ("pg_chunk_stats", i32, [vp, vp, i64, vp, vp]),
+    ("pg_fold_changes", i32, [i64, vp, vp, vp, vp, vp]),
+    ("pg_ema_update", i32, [i64, vp, vp, vp, vp, vp, dbl, vp]),
+    ("pg_apply_uplift", i32, [i64, vp, vp, vp, dbl, i32, dbl, vp]),
+    ("pg_spawn_cells", i32, [vp, i64, vp, i64, vp, vp]),
+    ("pg_exchange_halos", i32, [vp, i64, i64, i64, vp, vp, vp, vp, vp]),
+    ("pg_select_work_bytes", i64, []),
+    ("pg_select_kth_surface", i32, [vp, vp, i64, i64, i64, i64, vp, vp, vp]),
+    ("pg_datum_shift", i32, [i64, vp, vp, vp, vp, vp]),
+    ("pg_fast_workspace_bytes", i64, [i64, i64, i64]),
+    ("pg_fast_setup", i32, [ctypes.POINTER(FastFields), ctypes.POINTER(FastParams), vp, i64, vp]),
+    ("pg_fast_iterate", i32, [ctypes.POINTER(FastFields), ctypes.POINTER(TraceParams), ctypes.POINTER(FastParams), vp,
+                              i64, vp, vp]),
+    ("pg_fast_spawn", i32, [ctypes.POINTER(FastFields), ctypes.POINTER(TraceParams), ctypes.POINTER(FastParams), vp,
+                            i64, vp, vp]),
+    ("pg_fast_particles", i32, [ctypes.POINTER(FastFields), ctypes.POINTER(TraceParams), ctypes.POINTER(FastParams),
+                                vp, i64, vp, vp]),
+    ("pg_fast_layout", i32, [i64, i64, i64, ctypes.POINTER(ctypes.c_int64), i32]),
+    ("pg_pcg64_random", i32, [ctypes.c_uint64, ctypes.c_uint64, ctypes.c_uint64, ctypes.c_uint64, i64, i64, vp, vp]),
+]
+
+_lib = None
+
+
+def lib():
+    """The loaded library; raises RuntimeError when it is not built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is not built: run procedural-generation_amd/csrc/build.py "
+                               "(or __graft_entry__.build())")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, res, args in SIGNATURES:
+            fn = getattr(L, name, None)
+            if fn is None:
+                continue
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def exported_symbols() -> list[str]:
+    return [n for n, _, _ in SIGNATURES if hasattr(lib(), n)]
+
+
+def check(rc: int, what: str) -> None:
+    if rc != 0:
+        msg = lib().pg_last_error().decode(errors="replace")
+        raise RuntimeError(f"{what} failed: {msg}")

@@ -1,0 +1,143 @@
+"""Fast mode of the particle pass: one C-ABI call per iteration, everything
+on the device (see ``pg_fast_iterate`` in include/pg_erode.h).
+
+The spawn stream is still ``params.rng(stage, *iteration_key)``: the host
+seeds numpy's PCG64 exactly as the reference does and hands its 128-bit
+state to the device, which draws the same doubles (jump-ahead per thread),
+sorts them, and spawns through an exact fixed-point prefix sum of the rain
+weights -- so the droplets' cells and offsets are the reference's.  What
+differs from the check mode is the trace (float32, every droplet of a chunk
+against the same frozen terrain) and the apply (int64 fixed-point atomics
+with order-independent caps).
+"""
+from __future__ import annotations
+
+import ctypes
+import time
+
+import numpy as np
+import torch
+
+from .._lib import FAST_NSTATS, FS, FastFields, FastParams, check, lib
+from ..config import cell_units
+from . import particle as pk
+
+#: default frozen-terrain chunks per iteration (the reference splits an
+#: iteration into at least 16, bake/globe/erosion/maps.py:781-787)
+DEFAULT_CHUNKS = 16
+HEIGHT_BITS = 48
+TRACK_BITS = 32
+
+
+class FastContext:
+    """Per-state device workspace of the fast mode."""
+
+    def __init__(self, state, n_particles: int):
+        self.n_particles = int(n_particles)
+        dev = state.device
+        nbytes = int(lib().pg_fast_workspace_bytes(state.F, state.NE, self.n_particles))
+        if nbytes < 0:
+            check(-1, "pg_fast_workspace_bytes")
+        self.work = torch.empty(nbytes, dtype=torch.uint8, device=dev)
+        self.stats = torch.zeros(FAST_NSTATS, dtype=torch.int64, device=dev)
+        self.fields = FastFields(
+            F=state.F, N=state.N, H=state.H, spherical=int(bool(state.spherical)),
+            deposit_on_exit=int(bool(state.deposit_on_exit)), height=state.height.data_ptr(),
+            sediment=state.sediment.data_ptr(), discharge=state.discharge.data_ptr(), momentum=state.momentum.data_ptr(),
+            pending=state.pending.data_ptr(), base=state.base.data_ptr(), precip=state.precip.data_ptr(),
+            evap=state.evap.data_ptr(), hardness=state.hardness.data_ptr(), metric=state.metric.data_ptr(),
+            metric_inv=state.metric_inv.data_ptr(), mask=state.mask.data_ptr())
+        self._ptrs = self._pointers(state)
+        fp = FastParams(n_particles=self.n_particles, chunks=1, height_bits=HEIGHT_BITS, track_bits=TRACK_BITS)
+        check(lib().pg_fast_setup(ctypes.byref(self.fields), ctypes.byref(fp), ctypes.c_void_p(self.work.data_ptr()),
+                                  nbytes, pk._stream()), "pg_fast_setup")
+
+    @staticmethod
+    def _pointers(state):
+        return tuple(t.data_ptr() for t in (state.height, state.sediment, state.discharge, state.momentum, state.pending,
+                                            state.base, state.precip, state.evap, state.hardness, state.metric,
+                                            state.metric_inv, state.mask))
+
+    def valid_for(self, state, n_particles: int) -> bool:
+        return n_particles == self.n_particles and self._pointers(state) == self._ptrs
+
+
+def _fast_context(state, n_particles: int) -> FastContext:
+    ctx = state._fast
+    if ctx is None or not ctx.valid_for(state, n_particles):
+        ctx = FastContext(state, n_particles)
+        state._fast = ctx
+    return ctx
+
+
+def pcg64_state(rng: np.random.Generator):
+    st = rng.bit_generator.state
+    if st.get("bit_generator") != "PCG64":
+        raise ValueError(f"the fast mode draws numpy's PCG64 stream; params.rng gave {st.get('bit_generator')}")
+    s, inc = int(st["state"]["state"]), int(st["state"]["inc"])
+    m = (1 << 64) - 1
+    return s & m, s >> 64, inc & m, inc >> 64
+
+
+def decode_stats(raw: np.ndarray, hscale: float) -> dict:
+    """Device statistics -> the reference's stats keys (cell units)."""
+    raw = np.asarray(raw, np.int64)
+    P = int(raw[FS["PARTICLES"]])
+    deaths = {n: int(raw[FS["DEATHS"] + i]) for i, n in enumerate(pk.DEATH_NAMES)}
+    fx = lambda k: float(raw[FS[k]]) / hscale
+    vol0 = float(np.array([raw[FS["VOLUME0_BITS"]]], np.int64).view(np.float64)[0])
+    return {"particles": P, "entries": int(raw[FS["ENTRIES"]]), "steps_mean": int(raw[FS["ENTRIES"]]) / max(P, 1),
+            "moves": int(raw[FS["MOVES"]]), "volume": vol0, "deaths": deaths, "clamped": int(raw[FS["CLAMPED"]]),
+            "to_pending": fx("TO_PENDING_FX"), "released": fx("RELEASED_FX"), "deficit_out": fx("EXIT_FX"),
+            "lost_offshore": fx("LOST_OFFSHORE_FX"), "vanished": fx("VANISHED_FX"), "eroded": fx("ERODED_FX"),
+            "deposited": fx("DEPOSITED_FX"), "eligible": int(raw[FS["ELIGIBLE"]]),
+            "cdf_serial": bool(raw[FS["CDF_SERIAL"]]), "raw_fx": raw.copy()}
+
+
+def run_iteration(state, params, iteration_key, n_particles=None, particles_per_cell=None, rng_stage="erosion",
+                  log=None, chunks=None, sync_stats: bool = True) -> dict:
+    """Fast-mode particle pass of one iteration; same arguments and stats keys
+    as the reference's run_iteration (bake/globe/erosion/maps.py:839-1047),
+    plus ``moves`` (droplet-steps executed).  With ``sync_stats=False`` the
+    statistics stay on the device (``state._fast.stats``) and nothing waits."""
+    from .maps import _check_scope, _eparams, discharge_scales, lake_park_of, max_steps_of, trace_params
+
+    ep = _eparams(params)
+    _check_scope(ep, state)
+    if float(getattr(ep, "lateral_rate", 0.0)) > 0.0 or state.roots is not None or lake_park_of(state, ep) > 0.0:
+        raise NotImplementedError("lateral cuts, roots and lake parks are outside the fast mode: use mode='check'")
+    if float(ep.disc_saturation_cells) > 0.0 or float(ep.momentum_saturation_cells) > 0.0:
+        raise NotImplementedError("cell-count discharge scales need the spawn volume on the host: use mode='check'")
+    key = iteration_key if isinstance(iteration_key, (tuple, list)) else (int(iteration_key),)
+    rng = params.rng(rng_stage, *key)
+    t0 = time.time()
+    n_cells = int(np.count_nonzero(state._mask_host[state.interior] == pk.MASK_ACTIVE))
+    if n_particles is None:
+        ppc = ep.particles_per_cell if particles_per_cell is None else particles_per_cell
+        n_particles = max(1, int(round(ppc * n_cells)))
+    ctx = _fast_context(state, n_particles)
+    unit = state.height_unit_m
+    max_steps = max_steps_of(ep, state.N)
+    disc_sat, mom_scale = discharge_scales(ep, state, 1.0, n_particles, n_rain=1)
+    prm = trace_params(ep, state, 1.0, 1, 1, disc_sat, mom_scale, max_steps, 0.0)
+    slo, shi, ilo, ihi = pcg64_state(rng)
+    fp = FastParams(rng_state_lo=slo, rng_state_hi=shi, rng_inc_lo=ilo, rng_inc_hi=ihi, n_particles=int(n_particles),
+                    chunks=int(chunks if chunks is not None else getattr(state, "fast_chunks", DEFAULT_CHUNKS)),
+                    iter_deposit=cell_units(ep, "iter_deposit", unit), offshore_writeoff=float(ep.offshore_writeoff),
+                    ema=float(ep.ema), min_volume_frac=float(ep.min_volume_frac), height_bits=HEIGHT_BITS,
+                    track_bits=TRACK_BITS, grid_blocks=int(getattr(state, "fast_grid_blocks", 0)))
+    check(lib().pg_fast_iterate(ctypes.byref(ctx.fields), ctypes.byref(prm), ctypes.byref(fp),
+                                ctypes.c_void_p(ctx.work.data_ptr()), ctx.work.numel(),
+                                ctypes.c_void_p(ctx.stats.data_ptr()), pk._stream()), "pg_fast_iterate")
+    if not sync_stats:
+        return {"device_stats": ctx.stats, "seconds": time.time() - t0}
+    stats = decode_stats(ctx.stats.cpu().numpy(), float(2 ** HEIGHT_BITS))
+    stats["pending_total"] = float(state.pending[state.interior].sum().item())
+    stats["seconds"] = time.time() - t0
+    stats["chunk"] = -(-stats["particles"] // fp.chunks) if stats["particles"] else 0
+    stats["disc_saturation"] = disc_sat
+    stats["mom_scale"] = mom_scale
+    if log is not None:
+        log(f"  particles {stats['particles']} volume {stats['volume']:.3f} mean steps {stats['steps_mean']:.1f} "
+            f"deaths {stats['deaths']} clamped {stats['clamped']} pending {stats['pending_total']:.1f} ({stats['seconds']:.2f}s)")
+    return stats

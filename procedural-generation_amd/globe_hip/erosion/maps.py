@@ -1,0 +1,636 @@
+"""Erosion state and iteration on the GPU: the drop-in for the reference's
+particle pass (bake/globe/erosion/maps.py: ErosionState, run_iteration, step).
+
+Typical use (the reference's own, bake/globe/erosion/maps.py:10-15)::
+
+    state = ErosionState.from_grid(grid, bedrock_m, hardness, precip, evap, uplift_m, params.erosion)
+    for it in range(n):
+        step(state, params, it)
+    fields = state.fields()          # FaceFields in metres, on the host
+
+``params`` is anything with the reference's ``erosion`` group and ``rng``
+(the reference's own ``WorldParams`` works), so the spawn stream is the
+reference's: ``params.rng(stage, *iteration_key)``.
+
+Two modes (``ErosionState.mode``):
+
+* ``"check"`` -- the reference's algorithm bit for bit: the spawn drawn from
+  numpy's generator on the host exactly as the reference draws it, then per
+  chunk of ``erosion.chunk`` droplets a float64 trace on the frozen packed
+  record (``pg_trace_particles``) and a serial apply in droplet order
+  (``pg_apply_changes``), then fold and EMA -- all on the device.
+* ``"fast"`` -- the production path: spawn, trace, fixed-point accumulation
+  and every per-cell pass run on the device with no host round trip
+  (``globe_hip.erosion.fast``).
+
+Everything after the particles (uplift, datum hold, halo exchange) runs on
+the device in both modes.  Side passes outside the plain profile (thermal
+and creep, routing flood, lakes, glacial carving, isostasy, slope limits)
+raise NotImplementedError.
+"""
+from __future__ import annotations
+
+import ctypes
+import math
+import time
+from dataclasses import dataclass, field
+
+import numpy as np
+import torch
+
+from .. import _lib
+from .._lib import TraceParams, check, lib
+from ..config import cell_units
+from ..field import FaceField
+from . import particle as pk
+
+#: an iteration is split into at least this many chunks (maps.py:781-787)
+MIN_CHUNKS = 16
+
+
+def _eparams(params):
+    return params.erosion if hasattr(params, "erosion") else params
+
+
+def height_unit(grid, eparams) -> float:
+    """maps.py:755-763: the kernel works in cell units."""
+    u = float(eparams.height_unit_m) if eparams is not None else 0.0
+    if u in (0.0, 1.0) or u == grid.cell_size_m:
+        return float(grid.cell_size_m)
+    raise ValueError(f"erosion.height_unit_m={u} is not supported: the kernel works in cell units ({grid.cell_size_m} m); set 0")
+
+
+def max_steps_of(ep, N: int) -> int:
+    return int(ep.max_steps) if ep.max_steps > 0 else 2 * int(N)
+
+
+def chunk_size(ep, n_particles: int) -> int:
+    return max(1, min(int(ep.chunk), -(-int(n_particles) // MIN_CHUNKS)))
+
+
+def _host(x, dtype):
+    d = x.data if isinstance(x, FaceField) else x
+    if isinstance(d, torch.Tensor):
+        d = d.detach().cpu().numpy()
+    return np.ascontiguousarray(np.asarray(d), dtype=dtype)
+
+
+def _dev(a: np.ndarray, device) -> torch.Tensor:
+    return torch.from_numpy(np.ascontiguousarray(a)).to(device)
+
+
+class HaloTables:
+    """A grid's halo gather tables resident on the device."""
+
+    def __init__(self, grid, device):
+        hm = grid.halo
+        self.K, self.S = hm.src.shape
+        self.dst = _dev(hm.dst.astype(np.int64), device)
+        self.src = _dev(hm.src.astype(np.int64), device)
+        self.w = _dev(hm.w.astype(np.float64), device)
+        self.w_lin = _dev(hm.w_lin.astype(np.float64), device)
+        self.rot = _dev(hm.rot.astype(np.float64), device)
+
+    def exchange(self, data: torch.Tensor, vector: bool = False, linear: bool = False) -> None:
+        pk._need(data, torch.float64, None, "halo field")
+        w = self.w_lin if linear else self.w
+        check(lib().pg_exchange_halos(pk._ptr(data), 2 if vector else 1, self.K, self.S, pk._ptr(self.dst),
+                                      pk._ptr(self.src), pk._ptr(w), pk._ptr(self.rot if vector else None), pk._stream()),
+              "exchange_halos")
+
+
+@dataclass
+class ErosionState:
+    """Device-resident erosion state; fields and dtypes as the reference's
+    (bake/globe/erosion/maps.py:56-115), as torch tensors on one GPU."""
+
+    height: torch.Tensor  # (F, NE, NE) float64, cell units
+    sediment: torch.Tensor
+    discharge: torch.Tensor
+    momentum: torch.Tensor  # (F, NE, NE, 2) float64
+    hardness: torch.Tensor  # float32
+    precip: torch.Tensor  # float32
+    evap: torch.Tensor  # float32
+    uplift: torch.Tensor  # float64, cell units per iteration
+    mask: torch.Tensor  # uint8
+    metric: torch.Tensor  # (F, NE, NE, 3) float32
+    metric_inv: torch.Tensor
+    N: int
+    H: int
+    spherical: bool
+    height_unit_m: float
+    grid: object = None
+    mode: str = "check"
+    route: torch.Tensor | None = None
+    lake_flag: torch.Tensor | None = None
+    disch_track: torch.Tensor = field(default=None, repr=False)
+    mom_track: torch.Tensor = field(default=None, repr=False)
+    samp: torch.Tensor = field(default=None, repr=False)
+    acc: torch.Tensor = field(default=None, repr=False)
+    pending: torch.Tensor = field(default=None, repr=False)
+    lake_load: torch.Tensor = field(default=None, repr=False)
+    lake_id: torch.Tensor = field(default=None, repr=False)
+    lake_room: torch.Tensor = field(default=None, repr=False)
+    base: torch.Tensor = field(default=None, repr=False)
+    iteration: int = 0
+    land_target: float | None = None
+    deposit_on_exit: bool = False
+    roots: torch.Tensor | None = None
+    inflow_volume: float = 0.0
+    _halo: HaloTables | None = field(default=None, repr=False)
+    _uplift_mean: dict = field(default_factory=dict, repr=False)
+    _mask_host: np.ndarray | None = field(default=None, repr=False)
+    _select_work: torch.Tensor | None = field(default=None, repr=False)
+    _fast: object = field(default=None, repr=False)
+
+    def __post_init__(self):
+        F, NE = self.height.shape[0], self.height.shape[1]
+        dev = self.height.device
+        if dev.type != "cuda":
+            raise ValueError("ErosionState lives on the GPU: pass device tensors (or use from_grid / window)")
+        z = lambda: torch.zeros((F, NE, NE), dtype=torch.float64, device=dev)
+        if self.disch_track is None:
+            self.disch_track = z()
+        if self.mom_track is None:
+            self.mom_track = torch.zeros((F, NE, NE, 2), dtype=torch.float64, device=dev)
+        if self.samp is None:
+            self.samp = torch.zeros((F, NE, NE, pk.NS), dtype=torch.float32, device=dev)
+        if self.acc is None:
+            self.acc = z()
+        if self.pending is None:
+            self.pending = z()
+        if self.lake_load is None:
+            self.lake_load = z()
+        if self.lake_id is None:
+            self.lake_id = torch.full((F, NE, NE), -1, dtype=torch.int32, device=dev)
+        if self.lake_room is None:
+            self.lake_room = torch.zeros(1, dtype=torch.float64, device=dev)
+        if self.base is None:
+            self.base = z()
+        if NE != self.N + 2 * self.H:
+            raise ValueError("array width must be N + 2H")
+        if self.mode not in ("check", "fast"):
+            raise ValueError(f"mode must be 'check' or 'fast', got {self.mode!r}")
+        for a, shape, n in ((self.sediment, (F, NE, NE), "sediment"), (self.discharge, (F, NE, NE), "discharge"),
+                            (self.momentum, (F, NE, NE, 2), "momentum"), (self.hardness, (F, NE, NE), "hardness"),
+                            (self.precip, (F, NE, NE), "precip"), (self.evap, (F, NE, NE), "evap"),
+                            (self.uplift, (F, NE, NE), "uplift"), (self.mask, (F, NE, NE), "mask"),
+                            (self.metric, (F, NE, NE, 3), "metric"), (self.metric_inv, (F, NE, NE, 3), "metric_inv")):
+            if tuple(a.shape) != shape:
+                raise ValueError(f"{n}: shape {tuple(a.shape)}, expected {shape}")
+        self._mask_host = self.mask.cpu().numpy()
+
+    # -- constructors --------------------------------------------------------
+    @classmethod
+    def from_grid(cls, grid, bedrock_m, hardness, precip, evap, uplift_m, eparams=None, mask=None, device="cuda",
+                  mode: str = "check") -> "ErosionState":
+        """Global 6-face state from extended arrays / FaceFields in metres
+        (bake/globe/erosion/maps.py:152-191)."""
+        unit = height_unit(grid, eparams)
+        F, NE = 6, grid.NE
+        m = np.ones((F, NE, NE), np.uint8) if mask is None else _host(mask, np.uint8)
+        z = np.zeros((F, NE, NE))
+        return cls(height=_dev(_host(bedrock_m, np.float64) / unit, device), sediment=_dev(z, device),
+                   discharge=_dev(z, device), momentum=_dev(np.zeros((F, NE, NE, 2)), device),
+                   hardness=_dev(_host(hardness, np.float32), device), precip=_dev(_host(precip, np.float32), device),
+                   evap=_dev(_host(evap, np.float32), device), uplift=_dev(_host(uplift_m, np.float64) / unit, device),
+                   mask=_dev(m, device), metric=_dev(np.asarray(grid.metric, np.float32), device),
+                   metric_inv=_dev(np.asarray(grid.metric_inv, np.float32), device), N=grid.N, H=grid.H,
+                   spherical=True, height_unit_m=unit, grid=grid, mode=mode)
+
+    @classmethod
+    def window(cls, height_m, sediment_m, discharge, momentum, hardness, precip, evap, uplift_m, mask, metric,
+               metric_inv, H, height_unit_m, deposit_on_exit=False, route_m=None, device="cuda", mode: str = "check"):
+        """Single-face window (F = 1, spherical False) from (NE, NE[, C])
+        arrays in metres (bake/globe/erosion/maps.py:193-240)."""
+        NE = np.asarray(height_m).shape[0]
+        a = lambda x, dt, extra=(): _dev(_host(x, dt).reshape((1, NE, NE) + tuple(extra)), device)
+        hu = float(height_unit_m)
+        return cls(height=a(_host(height_m, np.float64) / hu, np.float64),
+                   sediment=a(_host(sediment_m, np.float64) / hu, np.float64), discharge=a(discharge, np.float64),
+                   momentum=a(momentum, np.float64, (2,)), hardness=a(hardness, np.float32), precip=a(precip, np.float32),
+                   evap=a(evap, np.float32), uplift=a(_host(uplift_m, np.float64) / hu, np.float64),
+                   mask=a(mask, np.uint8), metric=a(metric, np.float32, (3,)), metric_inv=a(metric_inv, np.float32, (3,)),
+                   N=NE - 2 * H, H=H, spherical=False, height_unit_m=hu, deposit_on_exit=deposit_on_exit,
+                   route=None if route_m is None else a(_host(route_m, np.float64) / hu, np.float64), mode=mode)
+
+    # -- views ---------------------------------------------------------------
+    @property
+    def F(self) -> int:
+        return self.height.shape[0]
+
+    @property
+    def NE(self) -> int:
+        return self.height.shape[1]
+
+    @property
+    def device(self):
+        return self.height.device
+
+    @property
+    def interior(self):
+        return (slice(None), slice(self.H, self.H + self.N), slice(self.H, self.H + self.N))
+
+    def surface(self) -> torch.Tensor:
+        return self.height + self.sediment
+
+    def height_m(self) -> np.ndarray:
+        return self.height.cpu().numpy() * self.height_unit_m
+
+    def sediment_m(self) -> np.ndarray:
+        return self.sediment.cpu().numpy() * self.height_unit_m
+
+    def total_mass(self) -> float:
+        """Sum of height + sediment + pending + lake_load over active
+        interior cells (maps.py:322-326), summed on the host in float64."""
+        act = self._mask_host[self.interior] == pk.MASK_ACTIVE
+        tot = (self.height + self.sediment + self.pending + self.lake_load)[self.interior].cpu().numpy()
+        return float(np.sum(tot[act]))
+
+    def fields(self, names=("height", "sediment", "discharge", "momentum")) -> dict:
+        """Output FaceFields in metres, on the host (maps.py:738-752)."""
+        if self.grid is None:
+            raise ValueError("fields() needs a Grid")
+        out = {}
+        for n in names:
+            if n == "height":
+                out[n] = FaceField(self.grid, self.height_m().astype(np.float32), name="height")
+            elif n == "sediment":
+                out[n] = FaceField(self.grid, self.sediment_m().astype(np.float32), name="sediment")
+            elif n == "discharge":
+                out[n] = FaceField(self.grid, self.discharge.cpu().numpy().astype(np.float32), name="discharge")
+            elif n == "momentum":
+                out[n] = FaceField(self.grid, self.momentum.cpu().numpy().astype(np.float32), is_vector=True, name="momentum")
+        return out
+
+    def pack(self) -> None:
+        pk.pack_samples(self.samp, self.height, self.sediment, self.route, self.route is not None, self.discharge,
+                        self.momentum, self.evap, self.hardness, self.metric, self.metric_inv, self.base, self.lake_flag)
+
+    # -- halos ---------------------------------------------------------------
+    def halo_tables(self) -> HaloTables:
+        if self._halo is None:
+            self._halo = HaloTables(self.grid, self.device)
+        return self._halo
+
+    def exchange_halos(self) -> None:
+        """Refresh the halos of the evolving fields on the device
+        (maps.py:329-340): height, sediment, discharge by the cubic weights,
+        momentum rotated into each destination face, route bilinear."""
+        if not self.spherical:
+            return
+        if self.grid is None:
+            raise ValueError("spherical ErosionState needs its Grid for halo exchange")
+        ht = self.halo_tables()
+        for arr in (self.height, self.sediment, self.discharge):
+            ht.exchange(arr)
+        ht.exchange(self.momentum, vector=True)
+        if self.route is not None:
+            ht.exchange(self.route, linear=True)
+
+    def uplift_mean(self, cap) -> float:
+        """The active-interior mean of the (capped) uplift field that
+        apply_uplift removes (maps.py:1137-1147).  The field is an input that
+        never changes during a run, so the mean is taken once per cap with
+        numpy's pairwise mean -- the reference's own reduction."""
+        key = None if cap is None else float(cap)
+        if key not in self._uplift_mean:
+            u = self.uplift.cpu().numpy()
+            if cap is not None:
+                u = np.minimum(u, float(cap))
+            mean = 0.0
+            if self.spherical:
+                iact = self._mask_host[self.interior] == pk.MASK_ACTIVE
+                mean = float(u[self.interior][iact].mean()) if iact.any() else 0.0
+            self._uplift_mean[key] = mean
+        return self._uplift_mean[key]
+
+    def select_work(self) -> torch.Tensor:
+        if self._select_work is None:
+            n = int(lib().pg_select_work_bytes())
+            self._select_work = torch.zeros((n + 7) // 8, dtype=torch.int64, device=self.device)
+        return self._select_work
+
+
+# ---------------------------------------------------------------------------
+# spawning (maps.py:793-833)
+# ---------------------------------------------------------------------------
+def spawn_weights(state: ErosionState):
+    """Flat indices and weights of the spawnable cells: interior, active,
+    land (surface >= base), precip > 0 -- on the host, as the reference."""
+    surf = (state.height + state.sediment).cpu().numpy()
+    base = state.base.cpu().numpy()
+    precip = state.precip.cpu().numpy()
+    inter = np.zeros(surf.shape, bool)
+    inter[state.interior] = True
+    ok = inter & (state._mask_host == pk.MASK_ACTIVE) & (surf >= base) & (precip > 0)
+    idx = np.flatnonzero(ok)
+    return idx, precip.reshape(-1)[idx].astype(np.float64)
+
+
+def spawn_particles(state: ErosionState, n_particles: int, rng: np.random.Generator):
+    """Deterministic spawn, the reference's draw order: sorted uniforms for
+    the inverse CDF, then the offsets (maps.py:809-833).  The inverse-CDF
+    search runs on the device.  Returns device tensors (face, x, y) and the
+    spawn volume, or None."""
+    idx, w = spawn_weights(state)
+    if idx.size == 0 or n_particles <= 0:
+        return None
+    total = float(w.sum())
+    if total <= 0.0:
+        return None
+    cdf = np.cumsum(w)
+    u = np.sort(rng.random(n_particles))
+    off = rng.random((n_particles, 2))
+    dev = state.device
+    k = pk.spawn_cells(_dev(cdf, dev), _dev(u, dev)).cpu().numpy()
+    flat = idx[k]
+    NE = state.NE
+    f = flat // (NE * NE)
+    rem = flat - f * NE * NE
+    ei = rem // NE
+    ej = rem - ei * NE
+    x = (ei - state.H).astype(np.float64) + off[:, 0]
+    y = (ej - state.H).astype(np.float64) + off[:, 1]
+    return _dev(f.astype(np.int64), dev), _dev(x, dev), _dev(y, dev), total / n_particles
+
+
+def _check_scope(ep, state):
+    if float(getattr(ep, "slope_limit_erode", 0.0)) > 0.0 or float(getattr(ep, "slope_limit_deposit", 0.0)) > 0.0:
+        raise NotImplementedError("erosion.slope_limit_* are outside the HIP erosion path")
+    if state.route is not None:
+        raise NotImplementedError("the routing surface (erosion.flood_every) is outside the HIP erosion path")
+
+
+def trace_params(ep, state, volume0, P, chunk, disc_sat, mom_scale, max_steps, lake_park) -> TraceParams:
+    """trace_particles' scalar arguments as run_iteration computes them
+    (maps.py:971-997)."""
+    unit = state.height_unit_m
+    return TraceParams(
+        dt=float(ep.dt), density=float(ep.density), friction=float(ep.friction), deposition_rate=float(ep.deposition_rate),
+        evap_rate=float(ep.evap_rate), k_mom=float(ep.k_mom), mom_scale=float(mom_scale), k_disc=float(ep.k_disc),
+        disc_saturation=float(disc_sat), disc_exponent=float(ep.disc_exponent), slope_gain=float(ep.slope_gain),
+        slope_sat=float(ep.slope_saturation), erodibility=float(ep.erodibility),
+        cover_depth=cell_units(ep, "cover_depth", unit), min_volume=float(ep.min_volume_frac) * float(volume0),
+        max_steps=int(max_steps), max_erode=cell_units(ep, "max_erode", unit), conc_scale=float(chunk / (volume0 * P)),
+        pit_steps=int(ep.pit_steps), deposit_on_exit=int(bool(state.deposit_on_exit)),
+        ocean_rate=float(ep.ocean_deposition_rate), ocean_steps=int(ep.ocean_steps),
+        fan_room=cell_units(ep, "fan_room", unit), dep_floor=cell_units(ep, "dep_floor_m", unit),
+        lake_trap=float(ep.lake_trap), lake_park=float(lake_park), lateral_rate=float(getattr(ep, "lateral_rate", 0.0)))
+
+
+def discharge_scales(ep, state, volume0, n_particles, n_rain=None):
+    """disc_saturation and mom_scale in cells of upstream rain (maps.py:911-921)."""
+    disc_sat = float(ep.disc_saturation)
+    mom_scale = 1.0
+    if float(ep.disc_saturation_cells) > 0.0 or float(ep.momentum_saturation_cells) > 0.0:
+        total = float(volume0) * float(n_particles)
+        if n_rain is None:
+            n_rain = int(spawn_weights(state)[0].size)
+        n_rain = max(int(n_rain), 1)
+        rain_per_cell = max(total - float(state.inflow_volume), 0.0) / n_rain
+        rain_per_cell = rain_per_cell if rain_per_cell > 0.0 else total / n_rain
+        if float(ep.disc_saturation_cells) > 0.0:
+            disc_sat = float(ep.disc_saturation_cells) * rain_per_cell
+        if float(ep.momentum_saturation_cells) > 0.0:
+            mom_scale = float(volume0) / (float(ep.momentum_saturation_cells) * rain_per_cell)
+    return disc_sat, mom_scale
+
+
+def lake_park_of(state, ep) -> float:
+    """1 when lake loads are parked (maps.py:996, lake_fill_args)."""
+    return 1.0 if (state.spherical and bool(getattr(ep, "lake_fill", False)) and bool(getattr(ep, "lake_balance", False))) else 0.0
+
+
+# ---------------------------------------------------------------------------
+# one iteration (maps.py:839-1047)
+# ---------------------------------------------------------------------------
+def run_iteration(state: ErosionState, params, iteration_key, n_particles=None, particles_per_cell=None,
+                  rng_stage="erosion", log=None, diag=None, spawn=None) -> dict:
+    """The particle part of an iteration: spawn (rain + re-injected
+    stockpiles), trace and apply, fold, EMA.  Same arguments, stats keys and
+    ``diag`` hook as the reference (maps.py:839-870); ``diag`` is called per
+    chunk with device tensors.  ``spawn`` optionally replaces the rain draw
+    with precomputed device tensors ``(face, x, y, volume0)``."""
+    if state.mode == "fast":
+        from . import fast
+
+        return fast.run_iteration(state, params, iteration_key, n_particles=n_particles,
+                                  particles_per_cell=particles_per_cell, rng_stage=rng_stage, log=log)
+    ep = _eparams(params)
+    _check_scope(ep, state)
+    key = iteration_key if isinstance(iteration_key, (tuple, list)) else (int(iteration_key),)
+    rng = params.rng(rng_stage, *key)
+    t0 = time.time()
+    dev = state.device
+    n_cells = int(np.count_nonzero(state._mask_host[state.interior] == pk.MASK_ACTIVE))
+    if n_particles is None:
+        ppc = ep.particles_per_cell if particles_per_cell is None else particles_per_cell
+        n_particles = max(1, int(round(ppc * n_cells)))
+    sp = spawn_particles(state, n_particles, rng) if spawn is None else spawn
+    stats = {"particles": 0, "entries": 0, "steps_mean": 0.0}
+    if sp is None:
+        return stats
+    sp_face, sp_x, sp_y, volume0 = sp
+    sp_sed = torch.zeros(sp_face.shape[0], dtype=torch.float64, device=dev)
+    sp_dir = torch.zeros(sp_face.shape[0], dtype=torch.float64, device=dev)
+    # re-inject the pending stockpiles after the rain, in cell order (maps.py:886-903)
+    pend = state.pending.reshape(-1).cpu().numpy()
+    pend_cells = np.flatnonzero(pend > 0.0)
+    released = 0.0
+    if pend_cells.size:
+        NE = state.NE
+        pf = pend_cells // (NE * NE)
+        rem = pend_cells - pf * NE * NE
+        pei = rem // NE
+        pej = rem - pei * NE
+        loads = pend[pend_cells]
+        released = float(loads.sum())
+        sp_face = torch.cat([sp_face, _dev(pf.astype(np.int64), dev)])
+        sp_x = torch.cat([sp_x, _dev((pei - state.H) + 0.5, dev)])
+        sp_y = torch.cat([sp_y, _dev((pej - state.H) + 0.5, dev)])
+        sp_sed = torch.cat([sp_sed, _dev(loads, dev)])
+        sp_dir = torch.cat([sp_dir, _dev(rng.random(pend_cells.size) * (2.0 * np.pi), dev)])
+        state.pending.reshape(-1)[_dev(pend_cells, dev)] = 0.0
+    state.pack()
+    state.acc.zero_()
+    disc_sat, mom_scale = discharge_scales(ep, state, volume0, n_particles)
+    unit = state.height_unit_m
+    iter_deposit = cell_units(ep, "iter_deposit", unit)
+    max_steps = max_steps_of(ep, state.N)
+    lateral = float(getattr(ep, "lateral_rate", 0.0)) > 0.0
+    cap = pk.change_list_cap(max_steps, lateral)
+    P = sp_face.shape[0]
+    chunk = chunk_size(ep, P)
+    prm = trace_params(ep, state, volume0, P, chunk, disc_sat, mom_scale, max_steps, lake_park_of(state, ep))
+    cl_cell = torch.empty(chunk * cap, dtype=torch.int32, device=dev)
+    cl_delta = torch.empty(chunk * cap, dtype=torch.float64, device=dev)
+    cl_vol = torch.empty(chunk * cap, dtype=torch.float32, device=dev)
+    cl_mom = torch.empty((chunk * cap, 2), dtype=torch.float32, device=dev)
+    cl_count = torch.zeros(chunk, dtype=torch.int64, device=dev)
+    cl_lo = torch.zeros(chunk, dtype=torch.int64, device=dev)
+    cl_hi = torch.zeros(chunk, dtype=torch.int64, device=dev)
+    sp_death = torch.zeros(chunk, dtype=torch.int8, device=dev)
+    n_chunks = -(-P // chunk)
+    outs = torch.zeros((n_chunks, 4), dtype=torch.float64, device=dev)
+    counts = torch.zeros(7, dtype=torch.int64, device=dev)
+    iter_erode = cell_units(ep, "iter_erode", unit)
+    dep_floor = cell_units(ep, "dep_floor_m", unit)
+    use_roots = state.roots is not None
+    t_trace = t_apply = 0.0
+    for ci, c0 in enumerate(range(0, P, chunk)):
+        c1 = min(P, c0 + chunk)
+        m = c1 - c0
+        t1 = time.time()
+        pk.trace_particles(sp_face[c0:c1], sp_x[c0:c1], sp_y[c0:c1], sp_sed[c0:c1], sp_dir[c0:c1], volume0, state.samp,
+                           state.route is not None, state.mask, state.N, state.H, state.spherical, prm, state.roots,
+                           use_roots, cl_cell, cl_delta, cl_vol, cl_mom, cl_count[:m], cl_lo[:m], cl_hi[:m], sp_death[:m])
+        t2 = time.time()
+        if diag is not None:
+            diag(state, cl_cell, cl_vol, cl_count[:m], cap, sp_death[:m])
+        pk.apply_changes(cl_cell, cl_delta, cl_vol, cl_mom, cl_count[:m], cap, state.height, state.sediment, state.acc,
+                         state.pending, state.samp, state.disch_track, state.mom_track, state.mask, iter_erode,
+                         iter_deposit, float(ep.fan_slope), state.route is not None, dep_floor, float(ep.offshore_writeoff),
+                         None, False, state.lake_load, state.lake_id, state.lake_room, outs[ci])
+        pk.chunk_stats(cl_count[:m], sp_death[:m], m, counts)
+        t_trace += t2 - t1
+        t_apply += time.time() - t2
+    pk.fold_changes(state.height, state.sediment, state.acc, state.mask)
+    pk.ema_update(state.discharge, state.momentum, state.disch_track, state.mom_track, state.mask, float(ep.ema))
+    o = outs.cpu().numpy()
+    cnt = counts.cpu().numpy()
+    # per-chunk partial results summed in chunk order, as the reference's loop does
+    n_clamp = 0
+    to_pending = lost = lost_offshore = 0.0
+    for r in o:
+        n_clamp += int(r[0])
+        to_pending += float(r[1])
+        lost += float(r[2])
+        lost_offshore += float(r[3])
+    entries = int(cnt[0])
+    stats.update({"particles": int(P), "entries": entries, "steps_mean": entries / max(P, 1), "volume": float(volume0),
+                  "seconds": time.time() - t0})
+    stats["deaths"] = {n: int(c) for n, c in zip(pk.DEATH_NAMES, cnt[1:])}
+    stats["clamped"] = n_clamp
+    stats["to_pending"] = to_pending
+    stats["released"] = float(released)
+    stats["pending_total"] = float(state.pending[state.interior].sum().item())
+    stats["deficit_out"] = lost
+    stats["lost_offshore"] = lost_offshore
+    stats["seconds_trace"] = t_trace
+    stats["seconds_apply"] = t_apply
+    stats["chunk"] = int(chunk)
+    stats["disc_saturation"] = disc_sat
+    stats["mom_scale"] = mom_scale
+    if log is not None:
+        log(f"  particles {P} volume {volume0:.3f} mean steps {stats['steps_mean']:.1f} deaths {stats['deaths']} "
+            f"clamped {n_clamp} pending {stats['pending_total']:.1f} ({stats['seconds']:.2f}s)")
+    return stats
+
+
+# ---------------------------------------------------------------------------
+# uplift, replay, datum (maps.py:1088-1213, 1413-1508)
+# ---------------------------------------------------------------------------
+def uplift_cap(state: ErosionState, ep):
+    if not state.spherical or float(getattr(ep, "uplift_max_m", 0.0)) <= 0.0:
+        return None
+    return cell_units(ep, "uplift_max_m", state.height_unit_m)
+
+
+def apply_uplift(state: ErosionState, cap=None) -> None:
+    """height += min(uplift, cap) - mean on active cells, mean-free on the
+    globe (maps.py:1088-1134)."""
+    pk.apply_uplift(state.height, state.uplift, state.mask, cap, state.uplift_mean(cap))
+
+
+def uplift_replay(state: ErosionState, ep):
+    """Total uplift the run will apply, or None (maps.py:1159-1175)."""
+    if not state.spherical or str(getattr(ep, "uplift_mode", "stack")) != "replay":
+        return None
+    cap = uplift_cap(state, ep)
+    u = state.uplift if cap is None else torch.clamp(state.uplift, max=float(cap))
+    act = state.mask == pk.MASK_ACTIVE
+    total = torch.zeros_like(state.height)
+    total[act] = float(int(ep.iterations)) * (u[act] - state.uplift_mean(cap))
+    return total
+
+
+def held_land_fraction(state: ErosionState, params) -> float:
+    """maps.py:1433-1443."""
+    if state.land_target is not None:
+        return float(state.land_target)
+    if float(getattr(getattr(params, "tectonics", None), "shelf_fraction", 0.0)) > 0.0:
+        raise ValueError("shelf mode holds the bedrock's land fraction: set state.land_target")
+    return float(params.world.land_fraction)
+
+
+def hold_datum_device(state: ErosionState, land_fraction: float, level: float = 0.0) -> torch.Tensor:
+    """The datum hold on the device; returns the shift as a device scalar
+    (no host synchronisation).  q is the exact (M - round(lf M))-th smallest
+    interior surface (maps.py:1484-1495)."""
+    M = state.F * state.N * state.N
+    n_land = min(max(int(round(float(land_fraction) * M)), 0), M)
+    q = torch.zeros(1, dtype=torch.float64, device=state.device)
+    k = M - 1 if n_land == 0 else (0 if n_land == M else M - n_land)
+    check(lib().pg_select_kth_surface(pk._ptr(state.height), pk._ptr(state.sediment), state.F, state.N, state.H, k,
+                                      pk._ptr(q), pk._ptr(state.select_work()), pk._stream()), "hold_datum")
+    if n_land == 0:
+        q += 1.0
+    if float(level) != 0.0:
+        q -= float(level)
+    check(lib().pg_datum_shift(state.height.numel(), pk._ptr(state.height), pk._ptr(state.route), pk._ptr(state.base),
+                               pk._ptr(q), pk._stream()), "datum_shift")
+    return q
+
+
+def hold_datum(state: ErosionState, land_fraction: float, level: float = 0.0) -> float:
+    """maps.py:1446-1508; returns the shift applied (cell units)."""
+    return float(hold_datum_device(state, land_fraction, level).item())
+
+
+def start_replay(state: ErosionState, params):
+    """maps.py:1178-1213."""
+    ep = _eparams(params)
+    total = uplift_replay(state, ep)
+    if total is None:
+        return None
+    state.height -= total
+    state.exchange_halos()
+    inter = state.interior
+    surf = (state.height + state.sediment)[inter]
+    info = {"replay_max": float(total[inter].max().item()), "replay_min": float(total[inter].min().item()),
+            "land_fraction": float((surf >= 0).double().mean().item()), "datum_shift": 0.0}
+    if hasattr(params, "world"):
+        info["datum_shift"] = hold_datum(state, held_land_fraction(state, params))
+    return info
+
+
+def step(state: ErosionState, params, iteration_key, log=None, after_refresh=None, **kw) -> dict:
+    """One iteration in the reference's order (maps.py:1516-1617), plain
+    profile: particles, uplift, datum hold (globe), halo exchange."""
+    ep = _eparams(params)
+    if float(ep.thermal_rate) != 0.0 or float(ep.creep_rate) != 0.0:
+        raise NotImplementedError("thermal / creep passes are outside the HIP erosion path (set thermal_rate = creep_rate = 0)")
+    if int(ep.flood_every) > 0 or (state.spherical and int(getattr(ep, "sea_mask_every", 0)) > 0):
+        raise NotImplementedError("routing / sea-mask refreshes are outside the HIP erosion path")
+    if state.spherical and (int(ep.glacial_every) > 0 or float(getattr(ep, "isostasy", 0.0)) > 0.0):
+        raise NotImplementedError("glacial / isostasy passes are outside the HIP erosion path")
+    t0 = time.time()
+    if after_refresh is not None:
+        after_refresh(state)
+    st = run_iteration(state, params, iteration_key, log=log, **kw)
+    t1 = time.time()
+    apply_uplift(state, uplift_cap(state, ep))
+    if state.spherical and hasattr(params, "world"):
+        q = hold_datum_device(state, held_land_fraction(state, params), float(getattr(state, "sea_drop", 0.0)))
+        st["datum_shift"] = q if state.mode == "fast" else float(q.item())
+    state.exchange_halos()
+    state.iteration += 1
+    st["seconds_route"] = 0.0
+    st["seconds_particles"] = t1 - t0
+    st["seconds_total"] = time.time() - t0
+    return st
+
+
+__all__ = ["ErosionState", "run_iteration", "apply_uplift", "uplift_replay", "start_replay", "hold_datum", "step",
+           "spawn_particles", "height_unit", "max_steps_of"]
