@@ -177,6 +177,7 @@ typedef struct pg_fast_params {
     int64_t height_bits;   /* fixed point: heights in units of 2^-height_bits cells */
     int64_t track_bits;    /* fixed point: discharge / momentum in units of 2^-track_bits */
     int64_t grid_blocks;   /* persistent trace grid (0 = sized by occupancy) */
+    int64_t time_trace;    /* record HIP events around every trace launch (see pg_fast_trace_time) */
 } pg_fast_params;
 
 /* Statistics of one fast iteration (DEVICE int64[PG_FAST_NSTATS]). */
@@ -220,6 +221,11 @@ int pg_fast_particles(const pg_fast_fields* fields, const pg_trace_params* prm, 
  * point (int64), droplet cell (int32), x / y fraction (float32), load
  * (int64), direction a / b (float32), scalars, total size. */
 int pg_fast_layout(int64_t F, int64_t NE, int64_t n_particles, int64_t* offsets, int n);
+
+/* Waits for the trace launches recorded since the last call (with
+ * time_trace set) and returns their summed duration (ms, HIP events on the
+ * launch stream) and count.  Library-wide state: not thread-safe. */
+int pg_fast_trace_time(double* ms_total, int64_t* launches);
 
 /* Fill out[k] = the (start + k)-th double of a PCG64 stream (numpy's
  * Generator.random() draws), k < n: the fast mode's spawn source. */

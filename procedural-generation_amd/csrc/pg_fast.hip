@@ -537,7 +537,9 @@ __global__ void __launch_bounds__(TRACE_TPB) trace_fast_kernel(TraceCfg k) {
                 sa += acc * na;
                 sb += acc * nb;
                 const float n2 = fmaxf(g0.x * sa * sa + 2.0f * g0.y * sa * sb + g0.z * sb * sb, 0.0f);
-                if (n2 < 1e-24f) {
+                // (a non-finite direction -- only from non-finite inputs -- stops the droplet
+                // rather than turning into a cell index)
+                if (!(n2 >= 1e-24f) || !(n2 <= 3.0e38f)) {
                     cause = DEATH_STOP;
                 } else {
                     const float inv = rsqrtf(n2);
@@ -851,6 +853,31 @@ T* at(void* work, size_t off) {
     return reinterpret_cast<T*>(reinterpret_cast<char*>(work) + off);
 }
 
+// trace-launch timing (pg_fast_trace_time): event pairs reused across calls
+struct TraceTimer {
+    hipEvent_t start[4096], stop[4096];
+    int created = 0, recorded = 0;
+};
+TraceTimer g_timer;
+
+int timer_record(hipStream_t s, bool is_start) {
+    TraceTimer& t = g_timer;
+    const int i = t.recorded;
+    PG_REQUIRE(i < 4096, "trace timing: more than 4096 launches recorded without pg_fast_trace_time");
+    if (i >= t.created) {
+        PG_HIP(hipEventCreate(&t.start[i]));
+        PG_HIP(hipEventCreate(&t.stop[i]));
+        t.created = i + 1;
+    }
+    if (is_start) {
+        PG_HIP(hipEventRecord(t.start[i], s));
+    } else {
+        PG_HIP(hipEventRecord(t.stop[i], s));
+        t.recorded = i + 1;
+    }
+    return 0;
+}
+
 int check_fields(const pg_fast_fields* fl) {
     PG_REQUIRE(fl != nullptr, "fast mode: fields is NULL");
     PG_REQUIRE(fl->F > 0 && fl->N >= 2 && fl->H >= 1, "fast mode: bad grid F=%lld N=%lld H=%lld", (long long)fl->F,
@@ -1088,7 +1115,9 @@ int fast_particles(const pg_fast_fields* fl, const pg_trace_params* prm, const p
     }
     for (int c = 0; c < chunks; c++) {
         k.chunk = c;
+        if (fp->time_trace && timer_record(s, true)) return -1;
         hipLaunchKernelGGL(trace_fast_kernel, dim3(grid), dim3(TRACE_TPB), 0, s, k);
+        if (fp->time_trace && timer_record(s, false)) return -1;
         hipLaunchKernelGGL(resolve_kernel, dim3(grid_for(n)), dim3(TPB), 0, s, n, fl->height, fl->sediment, fl->base,
                            k.chunk_fx, at<ll>(work, L.acc_fx), pend_fx, cell4, dyn, fp->iter_deposit, prm->dep_floor,
                            fp->offshore_writeoff, hscale, st);
@@ -1126,6 +1155,22 @@ extern "C" int pg_fast_iterate(const pg_fast_fields* fl, const pg_trace_params* 
     if (prepare(fl, prm, fp, work, work_bytes, stats, stream, &c)) return -1;
     if (fast_spawn(fl, fp, work, c)) return -1;
     return fast_particles(fl, prm, fp, work, c);
+}
+
+extern "C" int pg_fast_trace_time(double* ms_total, int64_t* launches) {
+    clear_error();
+    PG_REQUIRE(ms_total && launches, "pg_fast_trace_time: NULL argument");
+    double tot = 0.0;
+    for (int i = 0; i < g_timer.recorded; i++) {
+        PG_HIP(hipEventSynchronize(g_timer.stop[i]));
+        float ms = 0.0f;
+        PG_HIP(hipEventElapsedTime(&ms, g_timer.start[i], g_timer.stop[i]));
+        tot += ms;
+    }
+    *ms_total = tot;
+    *launches = g_timer.recorded;
+    g_timer.recorded = 0;
+    return 0;
 }
 
 extern "C" int pg_fast_layout(int64_t F, int64_t NE, int64_t n_particles, int64_t* offsets, int n) {

@@ -40,7 +40,7 @@ class FastParams(ctypes.Structure):
     _fields_ = [("rng_state_lo", ctypes.c_uint64), ("rng_state_hi", ctypes.c_uint64), ("rng_inc_lo", ctypes.c_uint64),
                 ("rng_inc_hi", ctypes.c_uint64), ("n_particles", i64), ("chunks", i64), ("iter_deposit", dbl),
                 ("offshore_writeoff", dbl), ("ema", dbl), ("min_volume_frac", dbl), ("height_bits", i64),
-                ("track_bits", i64), ("grid_blocks", i64)]
+                ("track_bits", i64), ("grid_blocks", i64), ("time_trace", i64)]
 
 
 #: statistics slots of a fast iteration (include/pg_erode.h, PG_FS_*)
@@ -86,6 +86,7 @@ SIGNATURES = [
     ("pg_fast_particles", i32, [ctypes.POINTER(FastFields), ctypes.POINTER(TraceParams), ctypes.POINTER(FastParams),
                                 vp, i64, vp, vp]),
     ("pg_fast_layout", i32, [i64, i64, i64, ctypes.POINTER(ctypes.c_int64), i32]),
+    ("pg_fast_trace_time", i32, [pd, ctypes.POINTER(ctypes.c_int64)]),
     ("pg_pcg64_random", i32, [ctypes.c_uint64, ctypes.c_uint64, ctypes.c_uint64, ctypes.c_uint64, i64, i64, vp, vp]),
 ]
 

@@ -95,11 +95,12 @@ def decode_stats(raw: np.ndarray, hscale: float) -> dict:
 
 
 def run_iteration(state, params, iteration_key, n_particles=None, particles_per_cell=None, rng_stage="erosion",
-                  log=None, chunks=None, sync_stats: bool = True) -> dict:
+                  log=None, chunks=None, sync_stats: bool | None = None) -> dict:
     """Fast-mode particle pass of one iteration; same arguments and stats keys
     as the reference's run_iteration (bake/globe/erosion/maps.py:839-1047),
     plus ``moves`` (droplet-steps executed).  With ``sync_stats=False`` the
-    statistics stay on the device (``state._fast.stats``) and nothing waits."""
+    statistics stay on the device (``state._fast.stats``) and nothing waits
+    (default: ``state.fast_sync_stats``, True when unset)."""
     from .maps import _check_scope, _eparams, discharge_scales, lake_park_of, max_steps_of, trace_params
 
     ep = _eparams(params)
@@ -125,10 +126,13 @@ def run_iteration(state, params, iteration_key, n_particles=None, particles_per_
                     chunks=int(chunks if chunks is not None else getattr(state, "fast_chunks", DEFAULT_CHUNKS)),
                     iter_deposit=cell_units(ep, "iter_deposit", unit), offshore_writeoff=float(ep.offshore_writeoff),
                     ema=float(ep.ema), min_volume_frac=float(ep.min_volume_frac), height_bits=HEIGHT_BITS,
-                    track_bits=TRACK_BITS, grid_blocks=int(getattr(state, "fast_grid_blocks", 0)))
+                    track_bits=TRACK_BITS, grid_blocks=int(getattr(state, "fast_grid_blocks", 0)),
+                    time_trace=int(bool(getattr(state, "fast_time_trace", False))))
     check(lib().pg_fast_iterate(ctypes.byref(ctx.fields), ctypes.byref(prm), ctypes.byref(fp),
                                 ctypes.c_void_p(ctx.work.data_ptr()), ctx.work.numel(),
                                 ctypes.c_void_p(ctx.stats.data_ptr()), pk._stream()), "pg_fast_iterate")
+    if sync_stats is None:
+        sync_stats = bool(getattr(state, "fast_sync_stats", True))
     if not sync_stats:
         return {"device_stats": ctx.stats, "seconds": time.time() - t0}
     stats = decode_stats(ctx.stats.cpu().numpy(), float(2 ** HEIGHT_BITS))
@@ -141,3 +145,12 @@ def run_iteration(state, params, iteration_key, n_particles=None, particles_per_
         log(f"  particles {stats['particles']} volume {stats['volume']:.3f} mean steps {stats['steps_mean']:.1f} "
             f"deaths {stats['deaths']} clamped {stats['clamped']} pending {stats['pending_total']:.1f} ({stats['seconds']:.2f}s)")
     return stats
+
+
+def trace_time():
+    """(ms, launches) of the trace launches recorded since the last call
+    (states with ``fast_time_trace = True``); waits for them."""
+    ms = ctypes.c_double()
+    n = ctypes.c_int64()
+    check(lib().pg_fast_trace_time(ctypes.byref(ms), ctypes.byref(n)), "pg_fast_trace_time")
+    return ms.value, n.value

@@ -30,29 +30,32 @@ EARTH_RADIUS_M = 6371.0e3
 def fbm_at(p: np.ndarray, rng: np.random.Generator, octaves: int = 5, base_freq: float = 2.0) -> np.ndarray:
     """Value-noise fBm at unit vectors ``p`` (..., 3), roughly in [-1, 1]."""
     p = np.asarray(p, dtype=np.float64)
-    out = np.zeros(p.shape[:-1])
+    shape = p.shape[:-1]
+    q3 = p.reshape(-1, 3)
+    out = np.zeros(q3.shape[0])
     amp, total = 1.0, 0.0
     for o in range(octaves):
         freq = base_freq * 2 ** o
         n = int(np.ceil(freq)) + 2
-        lattice = rng.random((n + 1, n + 1, n + 1))
-        q = (p + 1.0) * 0.5 * freq
+        m = n + 1
+        lattice = rng.random((m, m, m)).reshape(-1)
+        q = (q3 + 1.0) * (0.5 * freq)
         i0 = np.floor(q).astype(np.int64)
         t = q - i0
         t = t * t * (3.0 - 2.0 * t)
-        i0 = np.clip(i0, 0, n - 1)
+        np.clip(i0, 0, n - 1, out=i0)
+        base = (i0[:, 0] * m + i0[:, 1]) * m + i0[:, 2]
         acc = np.zeros_like(out)
         for dx in (0, 1):
-            wx = t[..., 0] if dx else 1.0 - t[..., 0]
+            wx = t[:, 0] if dx else 1.0 - t[:, 0]
             for dy in (0, 1):
-                wy = t[..., 1] if dy else 1.0 - t[..., 1]
-                for dz in (0, 1):
-                    wz = t[..., 2] if dz else 1.0 - t[..., 2]
-                    acc += wx * wy * wz * lattice[i0[..., 0] + dx, i0[..., 1] + dy, i0[..., 2] + dz]
+                wxy = wx * (t[:, 1] if dy else 1.0 - t[:, 1])
+                off = (dx * m + dy) * m
+                acc += wxy * ((1.0 - t[:, 2]) * lattice[base + off] + t[:, 2] * lattice[base + off + 1])
         out += amp * (acc * 2.0 - 1.0)
         total += amp
         amp *= 0.5
-    return out / total
+    return (out / total).reshape(shape)
 
 
 def earth_cell_size(N: int) -> float:
@@ -68,14 +71,19 @@ def _bedrock(grid: Grid, rng, octaves: int, amp_m: float, land_fraction: float) 
 
 def gaussian_blobs(grid: Grid, rng, n: int, sigma_m: float, peak_m: float, radius_m: float) -> np.ndarray:
     """Sum of ``n`` Gaussian bumps of arc-length width ``sigma_m`` centred at
-    uniform random points of the sphere, on every extended cell."""
+    uniform random points of the sphere, on every extended cell (each bump
+    evaluated within 6 sigma of its centre, where it is above 1.5e-8 of its
+    peak)."""
     c = rng.normal(size=(n, 3))
     c /= np.linalg.norm(c, axis=1, keepdims=True)
     P = grid.centers.reshape(-1, 3)
     out = np.zeros(P.shape[0])
+    cos_cut = np.cos(min(6.0 * sigma_m / radius_m, np.pi))
     for k in range(n):
-        arc = np.arccos(np.clip(P @ c[k], -1.0, 1.0)) * radius_m
-        out += peak_m * np.exp(-0.5 * (arc / sigma_m) ** 2)
+        d = P @ c[k]
+        near = np.flatnonzero(d > cos_cut)
+        arc = np.arccos(np.clip(d[near], -1.0, 1.0)) * radius_m
+        out[near] += peak_m * np.exp(-0.5 * (arc / sigma_m) ** 2)
     return out.reshape(grid.centers.shape[:-1])
 
 
