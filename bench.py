@@ -34,7 +34,12 @@ for _p in (ROOT, os.path.join(ROOT, "procedural-generation_amd")):
 import numpy as np  # noqa: E402
 
 METRIC = "droplet-steps/s at N_c=1024 vs numba host; fwd+adjoint/fwd ratio"
-BYTES_PER_STEP = 164.0  # algorithmic HBM bytes per droplet-step (FP32 fast mode, plain profile)
+#: algorithmic HBM bytes per droplet-step and trace pass (FP32 fast mode, plain profile): the applying pass
+#: reads the 2x2 stencil x 4 channels (64), 11 cell scalars (44), next stencil + base + mask (24) and writes
+#: four int64 fixed-point accumulators (32); the census pass reads the same 132 and writes the erosion
+#: census and floor key (16).  Every move is traced by both passes.
+BYTES_PER_STEP = 164.0
+BYTES_PER_STEP_CENSUS = 148.0
 HBM_PEAK_GBS = 8000.0
 
 
@@ -138,7 +143,7 @@ def main():
     r = gpu_run(grid, f, p, a.steps, a.warmup, a.chunks)
     value = r["moves"] / r["seconds"]
     trace_s = r["trace_ms"] / 1000.0
-    achieved = BYTES_PER_STEP * r["moves"] / trace_s / 1e9 if trace_s > 0 else 0.0
+    achieved = (BYTES_PER_STEP + BYTES_PER_STEP_CENSUS) * r["moves"] / trace_s / 1e9 if trace_s > 0 else 0.0
     out = {
         "metric": METRIC,
         "value": value,
@@ -159,7 +164,8 @@ def main():
                    "steps_mean": r["entries"] / max(r["particles"], 1)},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "trace_fast_kernel", "bytes_per_step": BYTES_PER_STEP,
+                     "kernel": "trace_fast_kernel",
+                     "bytes_per_step": {"apply_pass": BYTES_PER_STEP, "census_pass": BYTES_PER_STEP_CENSUS},
                      "launch_ms_mean": r["trace_ms"] / max(r["launches"], 1), "launches": r["launches"]},
     }
     if not a.no_cpu:

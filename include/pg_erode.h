@@ -171,6 +171,9 @@ typedef struct pg_fast_params {
     int64_t n_particles;   /* rain droplets: max(1, round(ppc * active cells)) */
     int64_t chunks;        /* frozen-terrain chunks per iteration (>= 1) */
     double iter_deposit;   /* net deposit a cell may take per iteration (cell units) */
+    double iter_erode;     /* net erosion a cell may take per iteration (cell units) */
+    double conc_chunk;     /* droplets per chunk the trace caps assume (the reference's chunk; 0 = `chunks`) */
+    double fan_slope;      /* a submarine fan descends at least this much per cell (cell units per cell) */
     double offshore_writeoff;
     double ema;
     double min_volume_frac; /* death by evaporation below this x the spawn volume (prm->min_volume unused) */
@@ -178,6 +181,8 @@ typedef struct pg_fast_params {
     int64_t track_bits;    /* fixed point: discharge / momentum in units of 2^-track_bits */
     int64_t grid_blocks;   /* persistent trace grid (0 = sized by occupancy) */
     int64_t time_trace;    /* record HIP events around every trace launch (see pg_fast_trace_time) */
+    const uint64_t* rng_dev; /* optional DEVICE copy of the four rng words, read by the kernels instead of the
+                              * values above: a captured graph then replays with each iteration's own draws */
 } pg_fast_params;
 
 /* Statistics of one fast iteration (DEVICE int64[PG_FAST_NSTATS]). */
@@ -185,6 +190,7 @@ enum {
     PG_FS_PARTICLES = 0, PG_FS_RAIN, PG_FS_ENTRIES, PG_FS_MOVES, PG_FS_DEATHS, /* 6 causes: 4..9 */
     PG_FS_CLAMPED = 10, PG_FS_TO_PENDING_FX, PG_FS_LOST_OFFSHORE_FX, PG_FS_EXIT_FX, PG_FS_VANISHED_FX,
     PG_FS_RELEASED_FX, PG_FS_VOLUME0_BITS, PG_FS_ELIGIBLE, PG_FS_CDF_SERIAL, PG_FS_ERODED_FX, PG_FS_DEPOSITED_FX,
+    PG_FS_DEFICIT_FX,
     PG_FAST_NSTATS = 24
 };
 
@@ -199,7 +205,10 @@ int pg_fast_setup(const pg_fast_fields* fields, const pg_fast_params* fp, void* 
                   pg_stream stream);
 
 /* One iteration's particle pass (bake/globe/erosion/maps.py:839-1047) in
- * fast mode (plain profile: no routing surface, lakes, lateral cuts or
+ * fast mode -- per chunk two trace passes over identical trajectories: an
+ * erosion census, a per-cell erosion scale (iteration budget, base level,
+ * lowest surface stepped to), then the applying pass where each droplet's
+ * shortfall cancels its later deposits (plain profile: no routing surface, lakes, lateral cuts or
  * roots; prm->conc_scale and prm->min_volume are derived on the device), entirely on the device: pack, spawn (the params.rng stream:
  * sorted uniforms, inverse CDF, offsets), pending re-injection, `chunks`
  * rounds of persistent float32 trace + per-cell resolve against the live

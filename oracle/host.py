@@ -74,6 +74,7 @@ def lib():
         L.or_fold_changes.argtypes = [i64, vp, vp, vp, vp]
         L.or_ema_update.argtypes = [i64, vp, vp, vp, vp, vp, dbl]
         L.or_spawn_cells.argtypes = [vp, i64, vp, i64, vp]
+        L.or_set_libm.argtypes = [vp, vp]
         L.or_change_list_cap.argtypes = [i64, i32]
         L.or_change_list_cap.restype = i64
         d = ctypes.POINTER(ctypes.c_double)
@@ -115,6 +116,13 @@ def transfer_velocity(face, u, v, a, b):
     o = [ctypes.c_double() for _ in range(4)]
     lib().or_transfer_velocity(int(face), float(u), float(v), float(a), float(b), ctypes.byref(f), *[ctypes.byref(x) for x in o])
     return (f.value,) + tuple(x.value for x in o)
+
+
+def set_libm(tan_fn=None, atan_fn=None):
+    """Use other tan/atan (C function pointers, e.g. ctypes function
+    objects) in the geometry; None restores glibc's."""
+    as_ptr = lambda fn: ctypes.cast(fn, ctypes.c_void_p) if fn is not None else None
+    lib().or_set_libm(as_ptr(tan_fn), as_ptr(atan_fn))
 
 
 def change_list_cap(max_steps: int, lateral: bool) -> int:

@@ -50,13 +50,25 @@ static const double BASES[6][3][3] = {
     {{-1.0, 0.0, 0.0}, {0.0, -1.0, 0.0}, {0.0, 0.0, -1.0}},
 };
 #define OR_PI 3.141592653589793
+
+/* tan/atan of the geometry: glibc's (the reference's, through numba) by
+ * default.  or_set_libm swaps in other implementations -- a test uses it to
+ * run the oracle with correctly rounded tan/atan, which isolates the one
+ * place where glibc's last-bit rounding differs from the device's. */
+typedef double (*or_fn1)(double);
+static or_fn1 or_tan = tan;
+static or_fn1 or_atan = atan;
+void or_set_libm(or_fn1 t, or_fn1 a) {
+    or_tan = t ? t : tan;
+    or_atan = a ? a : atan;
+}
 static const double HALF_PI = OR_PI / 2.0;     /* cubesphere.py:50 */
 static const double TWO_OVER_PI = 2.0 / OR_PI; /* cubesphere.py:51 */
 
 /* cubesphere.py:70-80 */
 void or_to_sphere(int face, double u, double v, double *ox, double *oy, double *oz) {
-    double s = tan((u - 0.5) * HALF_PI);
-    double t = tan((v - 0.5) * HALF_PI);
+    double s = or_tan((u - 0.5) * HALF_PI);
+    double t = or_tan((v - 0.5) * HALF_PI);
     const double *r = BASES[face][0], *up = BASES[face][1], *n = BASES[face][2];
     double x = n[0] + s * r[0] + t * up[0];
     double y = n[1] + s * r[1] + t * up[1];
@@ -83,16 +95,16 @@ void or_from_sphere(double x, double y, double z, int *oface, double *ou, double
     double s = (x * r[0] + y * r[1] + z * r[2]) / d;
     double t = (x * up[0] + y * up[1] + z * up[2]) / d;
     *oface = face;
-    *ou = atan(s) * TWO_OVER_PI + 0.5;
-    *ov = atan(t) * TWO_OVER_PI + 0.5;
+    *ou = or_atan(s) * TWO_OVER_PI + 0.5;
+    *ov = or_atan(t) * TWO_OVER_PI + 0.5;
 }
 
 /* cubesphere.py:130-167; j = (jux, juy, juz, jvx, jvy, jvz) */
 void or_jacobian(int face, double u, double v, double *j) {
     double a = (u - 0.5) * HALF_PI;
     double b = (v - 0.5) * HALF_PI;
-    double s = tan(a);
-    double t = tan(b);
+    double s = or_tan(a);
+    double t = or_tan(b);
     double dsdu = HALF_PI * (1.0 + s * s);
     double dtdv = HALF_PI * (1.0 + t * t);
     const double *r = BASES[face][0], *up = BASES[face][1], *n = BASES[face][2];

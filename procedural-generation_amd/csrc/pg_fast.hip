@@ -91,8 +91,14 @@ __device__ __forceinline__ double pcg_double(u128 s) {
 }
 
 // out[k] = draw (start + k): each thread advances once, then steps L times
-__global__ void pcg_fill_kernel(ull slo, ull shi, ull ilo, ull ihi, ll start, ll n, double* out) {
+__global__ void pcg_fill_kernel(ull slo, ull shi, ull ilo, ull ihi, const ull* rng_dev, ll start, ll n, double* out) {
     constexpr int L = 16;
+    if (rng_dev) {
+        slo = rng_dev[0];
+        shi = rng_dev[1];
+        ilo = rng_dev[2];
+        ihi = rng_dev[3];
+    }
     const u128 s0 = ((u128)shi << 64) | slo, inc = ((u128)ihi << 64) | ilo;
     for (ll t = (ll)blockIdx.x * blockDim.x + threadIdx.x; t * L < n; t += (ll)gridDim.x * blockDim.x) {
         const ll k0 = t * L;
@@ -156,7 +162,7 @@ __global__ void reset_scalars_kernel(Scalars* sc, ll* stats, int nstats, int chu
         sc->emin = 1 << 30;
     }
     for (int k = threadIdx.x; k < nstats; k += blockDim.x) stats[k] = 0;
-    for (int k = threadIdx.x; k < chunks; k += blockDim.x) sc->queue[k] = 0;
+    for (int k = threadIdx.x; k < 2 * chunks; k += blockDim.x) sc->queue[k] = 0;
 }
 
 // rain weight in units of the smallest weight's float32 quantum: exact integers
@@ -304,8 +310,14 @@ __global__ void pending_flags_kernel(int64_t n, const ll* pend_fx, int* flag) {
 // starting at its cell centre in a direction drawn after the offsets
 // (maps.py:886-903)
 __global__ void pending_drops_kernel(int64_t n, const int* flag, const int* rank, ll* pend_fx, ll P, ull slo, ull shi,
-                                     ull ilo, ull ihi, const Scalars* sc, int* d_cell, float* d_fx, float* d_fy,
+                                     ull ilo, ull ihi, const ull* rng_dev, const Scalars* sc, int* d_cell, float* d_fx, float* d_fy,
                                      ll* d_load, float* d_sa, float* d_sb, ll* stats) {
+    if (rng_dev) {
+        slo = rng_dev[0];
+        shi = rng_dev[1];
+        ilo = rng_dev[2];
+        ihi = rng_dev[3];
+    }
     const u128 s0 = ((u128)shi << 64) | slo, inc = ((u128)ihi << 64) | ilo;
     ll released = 0;
     const bool any = sc->eligible > 0;
@@ -339,7 +351,7 @@ __global__ void totals_kernel(int64_t n, const int* rank, ll P, Scalars* sc, ll*
     sc->n_total = sc->eligible > 0 ? P + k : 0;
     stats[PG_FS_PARTICLES] = sc->n_total;
     const ll C = (sc->n_total + chunks - 1) / chunks;
-    for (int c = 0; c < chunks; c++) sc->queue[c] = c * C;
+    for (int c = 0; c < chunks; c++) sc->queue[2 * c] = sc->queue[2 * c + 1] = c * C;
 }
 
 // ---------------------------------------------------------------------------
@@ -350,17 +362,20 @@ struct TraceCfg {
     const float4* geo;
     const float2* dyn;
     const uint8_t* mask;
-    int N, H, NE, spherical, deposit_on_exit, chunk, chunks;
+    const float* rscale;  // per-cell erosion scale of the chunk (pass 1 only)
+    int N, H, NE, spherical, deposit_on_exit, chunk, chunks, pass;
     const int* d_cell;
     const float *d_fx, *d_fy, *d_sa, *d_sb;
     const ll* d_load;
     Scalars* sc;
     // parameters (float32)
     float dt, density, fr, rate, dt_evap, k_mom, mom_scale, k_disc, inv_sat, disc_exponent, slope_gain, slope_sat,
-        erodibility, cover_depth, max_erode, ocean_rate, fan_room, dep_floor, min_volume_frac;
+        erodibility, cover_depth, max_erode, ocean_rate, fan_room, dep_floor, fan_slope, min_volume_frac;
     int max_steps, pit_steps, ocean_steps;
-    double hscale, tscale;
-    ll* chunk_fx;
+    double hscale, tscale, conc_chunk;
+    ll* chunk_fx;   // pass 1: net applied change per cell
+    ll* ero_fx;     // pass 0: erosion requested per cell
+    ull* floor_key;       // pass 0: lowest (surface key << 32 | cell) stepped to from a cell
     ll* dis_fx;
     ll* mom_fx;
     ll* stats;
@@ -369,6 +384,7 @@ struct TraceCfg {
 struct Stencil {
     float4 c00, c10, c01, c11;  // records at (i0, j0), (i0+1, j0), (i0, j0+1), (i0+1, j0+1)
     float wi, wj;
+    int i0, j0;
 };
 
 __device__ __forceinline__ float lerp2(float v00, float v10, float v01, float v11, float wi, float wj) {
@@ -391,30 +407,45 @@ __device__ __forceinline__ Stencil load_stencil(const TraceCfg& k, int f, int ix
     s.c11 = k.cell4[b + k.NE + 1];
     s.wi = wi;
     s.wj = wj;
+    s.i0 = i0;
+    s.j0 = j0;
     return s;
 }
+
+// the record of extended cell (ei, ej): always one of the stencil's corners
+// when the position lies in that cell
+__device__ __forceinline__ float4 corner_of(const Stencil& s, int ei, int ej) {
+    const bool ri = ei != s.i0, rj = ej != s.j0;
+    return ri ? (rj ? s.c11 : s.c10) : (rj ? s.c01 : s.c00);
+}
+
+// what a droplet reads at the cell it stands in
+struct Here {
+    int64_t cell;
+    int m;
+    float h, base, sthick, rs;
+    float4 g0, g1;
+};
+
+__device__ __forceinline__ int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
 
 __device__ __forceinline__ ll wave_sum(ll v) {
     for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
     return v;
 }
 
-__device__ __forceinline__ void emit(const TraceCfg& k, int64_t cell, float delta, float vol, float sa, float sb,
-                                     ll& carried) {
-    const ll dq = llrint((double)delta * k.hscale);
-    if (dq) {
-        atomicAdd((ull*)&k.chunk_fx[cell], (ull)dq);
-        carried -= dq;
-    }
-    if (vol > 0.0f) {
-        atomicAdd((ull*)&k.dis_fx[cell], (ull)llrint((double)vol * k.tscale));
-        atomicAdd((ull*)&k.mom_fx[2 * cell], (ull)llrint((double)(vol * sa) * k.tscale));
-        atomicAdd((ull*)&k.mom_fx[2 * cell + 1], (ull)llrint((double)(vol * sb) * k.tscale));
-    }
+// order-preserving key of a float (for atomicMin of surfaces)
+__device__ __forceinline__ unsigned fkey(float x) {
+    const unsigned b = __float_as_uint(x);
+    return (b & 0x80000000u) ? ~b : (b | 0x80000000u);
+}
+__device__ __forceinline__ float fkey_value(unsigned k) {
+    return __uint_as_float((k & 0x80000000u) ? (k & 0x7fffffffu) : ~k);
 }
 
 __global__ void __launch_bounds__(TRACE_TPB) trace_fast_kernel(TraceCfg k) {
     const int lane = threadIdx.x & 63;
+    const bool apply = k.pass == 1;
     const ll n_total = k.sc->n_total;
     const ll C = (n_total + k.chunks - 1) / k.chunks;
     const ll d_end = (k.chunk + 1) * C < n_total ? (k.chunk + 1) * C : n_total;
@@ -422,9 +453,12 @@ __global__ void __launch_bounds__(TRACE_TPB) trace_fast_kernel(TraceCfg k) {
     if (volume0 <= 0.0) return;
     const float inv_v0 = (float)(1.0 / volume0);
     const float min_volume = (float)(k.min_volume_frac * volume0);
-    const float conc_scale = (float)((double)C / (volume0 * (double)n_total));
+    // expected concurrent visits of a cell: the reference's chunk (conc_chunk
+    // droplets) unless none is given, then this kernel's own chunk
+    const float conc_scale = (float)((k.conc_chunk > 0.0 ? k.conc_chunk : (double)C) / (volume0 * (double)n_total));
     const float v0f = (float)volume0;
-    ll* queue = &k.sc->queue[k.chunk];
+    ll* queue = &k.sc->queue[2 * k.chunk + k.pass];
+    const int NE2 = k.NE * k.NE;
 
     // lane state
     bool alive = false, exhausted = false;
@@ -432,11 +466,54 @@ __global__ void __launch_bounds__(TRACE_TPB) trace_fast_kernel(TraceCfg k) {
     float fx = 0, fy = 0, vol = 0, sed = 0, sa = 0, sb = 0, h_prev = 0;
     bool has_prev = false, in_sea = false;
     int step = 0, uphill = 0, sea_steps = 0, pushes = 0, last_cell = -1;
-    ll carried = 0;
+    ll real = 0, deficit = 0;  // load actually held (fixed point) and load cancelled by erosion caps
+    ll surplus = 0;            // deposits a ceiling refused, offered to the droplet's next deposits
     Stencil S;
+    Here cur;
     // lane totals
-    ll n_entries = 0, n_moves = 0, exit_fx = 0, vanished_fx = 0;
+    ll n_entries = 0, n_moves = 0, exit_fx = 0, vanished_fx = 0, lost_fx = 0;
     ll deaths[N_DEATHS] = {0, 0, 0, 0, 0, 0};
+
+    auto load_here = [&](int ff, int cx, int cy, Here& h, const float4* rec) {
+        const int ci = clampi(cx, 0, k.N - 1), cj = clampi(cy, 0, k.N - 1);
+        h.cell = ((int64_t)ff * k.NE + ci + k.H) * k.NE + cj + k.H;
+        h.m = k.mask[h.cell];
+        const float2 dy = k.dyn[h.cell];
+        h.sthick = dy.x;
+        h.base = dy.y;
+        h.h = rec ? rec->x : k.cell4[h.cell].x;
+        h.g0 = k.geo[2 * h.cell];
+        h.g1 = k.geo[2 * h.cell + 1];
+        h.rs = apply ? k.rscale[h.cell] : 1.0f;
+    };
+    // deposit request `d` (fixed point) at `cell`: with the surplus of earlier
+    // refused deposits, less the deficit, at most `lim` (particle.py:969-1002)
+    auto deposit = [&](int64_t cell, ll d, ll lim) {
+        ll want = d + surplus;
+        surplus = 0;
+        ll a = want < deficit ? want : deficit;
+        if (a < 0) a = 0;
+        deficit -= a;
+        want -= a;
+        ll dep = want < lim ? want : lim;
+        if (dep < 0) dep = 0;
+        surplus = want - dep;
+        if (dep) atomicAdd((ull*)&k.chunk_fx[cell], (ull)dep);
+        real -= dep;
+    };
+    // ceiling of a step deposit on the frozen terrain: the previous path cell
+    // (+ dep_floor on land, - fan_slope under water) and the waterline
+    auto step_ceiling = [&](float h_here, float b_here) -> ll {
+        float lim = 3.0e38f;
+        if (has_prev) {
+            float ceil_ = h_prev + k.dep_floor;
+            if (h_here < b_here) ceil_ -= k.dep_floor + k.fan_slope;
+            lim = ceil_ - h_here;
+        }
+        if (h_here < b_here) lim = fminf(lim, b_here - k.dep_floor - h_here);
+        if (lim >= 3.0e38f) return (ll)9.0e18;
+        return lim > 0.0f ? llrint((double)lim * k.hscale) : 0;
+    };
 
     while (true) {
         // refill the lanes whose droplet died from the chunk's queue
@@ -452,16 +529,17 @@ __global__ void __launch_bounds__(TRACE_TPB) trace_fast_kernel(TraceCfg k) {
                     const ll d = base + __popcll(dead & ((1ull << lane) - 1));
                     if (d < d_end) {
                         const int cell = k.d_cell[d];
-                        const int NE2 = k.NE * k.NE;
                         f = cell / NE2;
                         const int r = cell - f * NE2;
                         ix = r / k.NE - k.H;
                         iy = r % k.NE - k.H;
                         fx = k.d_fx[d];
                         fy = k.d_fy[d];
-                        carried = k.d_load[d];
+                        real = k.d_load[d];
+                        deficit = 0;
+                        surplus = 0;
                         vol = v0f;
-                        sed = (float)((double)carried / k.hscale);
+                        sed = (float)((double)real / k.hscale);
                         sa = k.d_sa[d];
                         sb = k.d_sb[d];
                         has_prev = false;
@@ -469,6 +547,7 @@ __global__ void __launch_bounds__(TRACE_TPB) trace_fast_kernel(TraceCfg k) {
                         step = uphill = sea_steps = pushes = 0;
                         last_cell = -1;
                         S = load_stencil(k, f, ix, iy, fx, fy);
+                        load_here(f, ix, iy, cur, nullptr);
                         alive = true;
                     }
                 }
@@ -480,32 +559,24 @@ __global__ void __launch_bounds__(TRACE_TPB) trace_fast_kernel(TraceCfg k) {
         // ---- one step of this lane's droplet (particle.py:421-769)
         int cause = -1;
         bool exited = false;
-        const int ci = ix < 0 ? 0 : (ix > k.N - 1 ? k.N - 1 : ix);
-        const int cj = iy < 0 ? 0 : (iy > k.N - 1 ? k.N - 1 : iy);
-        const int64_t cell = ((int64_t)f * k.NE + ci + k.H) * k.NE + cj + k.H;
-        const int m = step < k.max_steps ? k.mask[cell] : MASK_ACTIVE;
-        float h_here = 0.0f, b_here = 0.0f;
         if (step >= k.max_steps) {
             cause = DEATH_AGE;
-        } else if (m == MASK_OUTSIDE) {
+        } else if (cur.m == MASK_OUTSIDE) {
             cause = DEATH_EXIT;
             exited = true;
         } else {
-            const float4 rec = k.cell4[cell];
-            const float2 dy = k.dyn[cell];
-            h_here = rec.x;
-            b_here = dy.y;
+            const float h_here = cur.h, b_here = cur.base;
             if (step == 0 && h_here < b_here) in_sea = true;
             if (in_sea) {
                 if (h_here >= b_here || sed * vol * inv_v0 < 1e-4f || sea_steps >= k.ocean_steps) cause = DEATH_OCEAN;
                 else sea_steps += 1;
             }
             if (cause < 0) {
-                if (m == MASK_ACTIVE) {
-                    last_cell = (int)cell;
+                if (cur.m == MASK_ACTIVE) {
+                    last_cell = (int)cur.cell;
                     pushes += 1;
                 }
-                const float4 g0 = k.geo[2 * cell], g1 = k.geo[2 * cell + 1];
+                const float4 g0 = cur.g0, g1 = cur.g1;
                 // forces
                 const float h00 = S.c00.x, h10 = S.c10.x, h01 = S.c01.x, h11 = S.c11.x, wi = S.wi, wj = S.wj;
                 const float h0 = lerp2(h00, h10, h01, h11, wi, wj);
@@ -547,7 +618,7 @@ __global__ void __launch_bounds__(TRACE_TPB) trace_fast_kernel(TraceCfg k) {
                     sb *= inv;
                     // one cell step, across a face edge if need be
                     float nfx = fx + sa, nfy = fy + sb;
-                    float flx = floorf(nfx), fly = floorf(nfy);
+                    const float flx = floorf(nfx), fly = floorf(nfy);
                     int nix = ix + (int)flx, niy = iy + (int)fly;
                     nfx = fminf(nfx - flx, 0.99999994f);
                     nfy = fminf(nfy - fly, 0.99999994f);
@@ -561,8 +632,8 @@ __global__ void __launch_bounds__(TRACE_TPB) trace_fast_kernel(TraceCfg k) {
                             double nx = t.u * Nd, ny = t.v * Nd;
                             if (nx >= Nd) nx = Nd * 0.999999999;
                             if (ny >= Nd) ny = Nd * 0.999999999;
-                            if (nx < 0.0) nx = 0.0;
-                            if (ny < 0.0) ny = 0.0;
+                            if (!(nx >= 0.0)) nx = 0.0;
+                            if (!(ny >= 0.0)) ny = 0.0;
                             const double fnx = floor(nx), fny = floor(ny);
                             nix = (int)fnx;
                             niy = (int)fny;
@@ -576,18 +647,25 @@ __global__ void __launch_bounds__(TRACE_TPB) trace_fast_kernel(TraceCfg k) {
                         exited = true;
                     }
                     if (cause < 0) {
-                        const int64_t ncell = ((int64_t)nf * k.NE + nix + k.H) * k.NE + niy + k.H;
-                        if (k.mask[ncell] == MASK_OUTSIDE) {
+                        // everything the next step reads, issued together
+                        const int nei = nix + k.H, nej = niy + k.H;
+                        const int64_t ncell = ((int64_t)nf * k.NE + nei) * k.NE + nej;
+                        const Stencil S1 = load_stencil(k, nf, nix, niy, nfx, nfy);
+                        const int nm = k.mask[ncell];
+                        const float2 ndy = k.dyn[ncell];
+                        const float4 nrec = corner_of(S1, nei, nej);
+                        Here nxt;
+                        const bool same = nix >= 0 && nix < k.N && niy >= 0 && niy < k.N;
+                        load_here(nf, nix, niy, nxt, same ? &nrec : nullptr);
+                        if (nm == MASK_OUTSIDE) {
                             cause = DEATH_EXIT;
                             exited = true;
                         } else {
-                            const Stencil S1 = load_stencil(k, nf, nix, niy, nfx, nfy);
-                            const float4 nrec = k.cell4[ncell];
-                            const float b1 = k.dyn[ncell].y;
                             const float h1 = lerp2(S1.c00.x, S1.c10.x, S1.c01.x, S1.c11.x, S1.wi, S1.wj);
+                            const float b1 = ndy.y;
                             const float dh = h0 - fmaxf(h1, b1);
                             // mass transfer at the cell being left
-                            if (m == MASK_ACTIVE && in_sea) {
+                            if (cur.m == MASK_ACTIVE && in_sea) {
                                 float cdiff = -k.ocean_rate * sed;
                                 float room = has_prev ? h_prev - h_here : k.fan_room;
                                 room = fmaxf(room, k.fan_room);
@@ -595,9 +673,11 @@ __global__ void __launch_bounds__(TRACE_TPB) trace_fast_kernel(TraceCfg k) {
                                 room = fmaxf(room, 0.0f);
                                 if (-cdiff * vol_rel > room) cdiff = -room / vol_rel;
                                 sed += cdiff;
-                                emit(k, cell, -cdiff * vol_rel, 0.0f, 0.0f, 0.0f, carried);
-                                n_entries += 1;
-                            } else if (m == MASK_ACTIVE) {
+                                if (apply) {
+                                    deposit(cur.cell, llrint((double)(-cdiff * vol_rel) * k.hscale), step_ceiling(h_here, b_here));
+                                    n_entries += 1;
+                                }
+                            } else if (cur.m == MASK_ACTIVE) {
                                 float qe;
                                 if (k.inv_sat <= 0.0f) qe = 0.0f;
                                 else if (k.disc_exponent == 0.5f) qe = sqrtf(q * k.inv_sat);
@@ -606,9 +686,8 @@ __global__ void __launch_bounds__(TRACE_TPB) trace_fast_kernel(TraceCfg k) {
                                 const float c_eq = fmaxf(k.erodibility * dh * (1.0f + k.k_disc * qe), 0.0f);
                                 float k_e = k.rate;
                                 if (c_eq > sed) {
-                                    const float sthick = k.dyn[cell].x;
-                                    const float w_cover = k.cover_depth > 0.0f ? fminf(sthick / k.cover_depth, 1.0f)
-                                                                               : (sthick > 0.0f ? 1.0f : 0.0f);
+                                    const float w_cover = k.cover_depth > 0.0f ? fminf(cur.sthick / k.cover_depth, 1.0f)
+                                                                               : (cur.sthick > 0.0f ? 1.0f : 0.0f);
                                     k_e = k.rate * (w_cover + (1.0f - w_cover) * (1.0f - g1.w));
                                 }
                                 float cdiff = k_e * (c_eq - sed);
@@ -624,8 +703,40 @@ __global__ void __launch_bounds__(TRACE_TPB) trace_fast_kernel(TraceCfg k) {
                                     if (-cdiff * vol_rel > dcap) cdiff = -dcap / vol_rel;
                                 }
                                 sed += cdiff;
-                                emit(k, cell, -cdiff * vol_rel, vol, sa, sb, carried);
-                                n_entries += 1;
+                                ll dq = llrint((double)(cdiff * vol_rel) * k.hscale);  // > 0: erosion
+                                ll cut = 0;
+                                if (dq > 0) {
+                                    // an entry never takes the cell below its base level nor below the
+                                    // cell it steps to (frozen surfaces; particle.py:951-962)
+                                    float fl = h_here >= b_here ? b_here : h_here;
+                                    fl = fmaxf(fl, nrec.x);
+                                    const float room = h_here - fl;
+                                    const ll lim = room > 0.0f ? llrint((double)room * k.hscale) : 0;
+                                    if (dq > lim) {
+                                        cut = dq - lim;
+                                        dq = lim;
+                                    }
+                                }
+                                if (!apply) {
+                                    if (dq > 0) {
+                                        atomicAdd((ull*)&k.ero_fx[cur.cell], (ull)dq);
+                                        atomicMin(&k.floor_key[cur.cell], ((ull)fkey(nrec.x) << 32) | (ull)(unsigned)ncell);
+                                    }
+                                } else {
+                                    deficit += cut;
+                                    if (dq > 0) {
+                                        const ll taken = (ll)floor((double)dq * (double)cur.rs);
+                                        deficit += dq - taken;
+                                        real += taken;
+                                        if (taken) atomicAdd((ull*)&k.chunk_fx[cur.cell], (ull)(-taken));
+                                    } else if (dq < 0) {
+                                        deposit(cur.cell, -dq, step_ceiling(h_here, b_here));
+                                    }
+                                    atomicAdd((ull*)&k.dis_fx[cur.cell], (ull)llrint((double)vol * k.tscale));
+                                    atomicAdd((ull*)&k.mom_fx[2 * cur.cell], (ull)llrint((double)(vol * sa) * k.tscale));
+                                    atomicAdd((ull*)&k.mom_fx[2 * cur.cell + 1], (ull)llrint((double)(vol * sb) * k.tscale));
+                                    n_entries += 1;
+                                }
                             }
                             // evaporation, advance, stop tests
                             const float ev = fmaxf(1.0f - k.dt_evap * g1.z, 0.01f);
@@ -637,11 +748,12 @@ __global__ void __launch_bounds__(TRACE_TPB) trace_fast_kernel(TraceCfg k) {
                             fx = nfx;
                             fy = nfy;
                             S = S1;
+                            cur = nxt;
                             h_prev = h_here;
                             has_prev = true;
                             n_moves += 1;
                             step += 1;
-                            if (nrec.x < k.dyn[ncell].y && !in_sea) {
+                            if (nrec.x < ndy.y && !in_sea) {
                                 in_sea = true;
                                 if (sed <= 0.0f) cause = DEATH_OCEAN;
                             }
@@ -663,40 +775,100 @@ __global__ void __launch_bounds__(TRACE_TPB) trace_fast_kernel(TraceCfg k) {
         if (cause < 0) continue;
 
         // ---- the droplet stopped: final deposit (particle.py:770-815)
-        if (in_sea && (cause == DEATH_AGE || cause == DEATH_STOP)) cause = DEATH_OCEAN;
-        deaths[cause] += 1;
-        if (!exited) {
-            const int sci = ix < 0 ? 0 : (ix > k.N - 1 ? k.N - 1 : ix);
-            const int scj = iy < 0 ? 0 : (iy > k.N - 1 ? k.N - 1 : iy);
-            const int64_t scell = ((int64_t)f * k.NE + sci + k.H) * k.NE + scj + k.H;
-            if (k.mask[scell] == MASK_ACTIVE && !(in_sea && k.cell4[scell].x >= k.dyn[scell].y)) {
-                if (pushes == 0 || last_cell != (int)scell) {
-                    last_cell = (int)scell;
-                    pushes += 1;
+        if (apply) {
+            if (in_sea && (cause == DEATH_AGE || cause == DEATH_STOP)) cause = DEATH_OCEAN;
+            deaths[cause] += 1;
+            if (!exited) {
+                // the stop cell is `cur` (the cell of the current position)
+                if (cur.m == MASK_ACTIVE && !(in_sea && cur.h >= cur.base)) {
+                    if (pushes == 0 || last_cell != (int)cur.cell) {
+                        last_cell = (int)cur.cell;
+                        pushes += 1;
+                    }
                 }
             }
+            if (pushes > 0 && (!exited || k.deposit_on_exit)) {
+                if (sed > 0.0f) n_entries += cause == DEATH_OCEAN ? 1 : (pushes < SPREAD ? pushes : SPREAD);
+                if (real) atomicAdd((ull*)&k.chunk_fx[last_cell], (ull)real);
+            } else if (exited) {
+                exit_fx += real;
+            } else {
+                vanished_fx += real;
+            }
+            lost_fx += deficit;
         }
-        if (pushes > 0 && (!exited || k.deposit_on_exit)) {
-            if (carried) atomicAdd((ull*)&k.chunk_fx[last_cell], (ull)carried);
-            if (sed > 0.0f) n_entries += cause == DEATH_OCEAN ? 1 : (pushes < SPREAD ? pushes : SPREAD);
-        } else if (exited) {
-            exit_fx += carried;
-        } else {
-            vanished_fx += carried;
-        }
-        carried = 0;
+        real = 0;
+        deficit = 0;
+        surplus = 0;
         alive = false;
     }
+    if (!apply) return;
     // lane totals -> device statistics (one atomic per wave and counter)
-    ll v[4 + N_DEATHS] = {n_entries, n_moves, exit_fx, vanished_fx};
-    for (int c = 0; c < N_DEATHS; c++) v[4 + c] = deaths[c];
-    for (int c = 0; c < 4 + N_DEATHS; c++) v[c] = wave_sum(v[c]);
+    ll v[5 + N_DEATHS] = {n_entries, n_moves, exit_fx, vanished_fx, lost_fx};
+    for (int c = 0; c < N_DEATHS; c++) v[5 + c] = deaths[c];
+    for (int c = 0; c < 5 + N_DEATHS; c++) v[c] = wave_sum(v[c]);
     if (lane == 0) {
-        const int slot[4] = {PG_FS_ENTRIES, PG_FS_MOVES, PG_FS_EXIT_FX, PG_FS_VANISHED_FX};
-        for (int c = 0; c < 4; c++)
+        const int slot[5] = {PG_FS_ENTRIES, PG_FS_MOVES, PG_FS_EXIT_FX, PG_FS_VANISHED_FX, PG_FS_DEFICIT_FX};
+        for (int c = 0; c < 5; c++)
             if (v[c]) atomicAdd((ull*)&k.stats[slot[c]], (ull)v[c]);
         for (int c = 0; c < N_DEATHS; c++)
-            if (v[4 + c]) atomicAdd((ull*)&k.stats[PG_FS_DEATHS + c], (ull)v[4 + c]);
+            if (v[5 + c]) atomicAdd((ull*)&k.stats[PG_FS_DEATHS + c], (ull)v[5 + c]);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// per-cell erosion scale of a chunk (between the two trace passes): a cell
+// may lose at most its iteration budget (iter_erode + this iteration's net
+// change so far) and never sink below its base level (land) nor below the
+// lowest surface a droplet stepped to from it -- the reference's per-entry
+// floor, taken over the whole chunk (particle.py:937-968)
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ float erosion_scale(int64_t c, const double* height, const double* sediment,
+                                               const double* base, const ll* acc_fx, ll e, double fl_next,
+                                               double iter_erode, double inv) {
+    const double acc = (double)acc_fx[c] * inv;
+    const double s_c = height[c] + sediment[c] + acc;
+    const double b = base[c];
+    double fl = s_c >= b ? b : s_c;
+    fl = fmax(fl, fl_next);
+    double allowed = fmin(iter_erode + acc, s_c - fl);
+    if (allowed < 0.0) allowed = 0.0;
+    const double want = (double)e * inv;
+    return want > allowed ? (float)(allowed / want) : 1.0f;
+}
+
+// sweep 1: floor = the lowest surface stepped to (frozen)
+__global__ void erosion_scale_kernel(int64_t n, const double* height, const double* sediment, const double* base,
+                                     const ll* ero_fx, const ull* floor_key, const ll* acc_fx, float* r1,
+                                     double iter_erode, double hscale) {
+    const double inv = 1.0 / hscale;
+    GRID_STRIDE(c, n) {
+        const ll e = ero_fx[c];
+        const ull fk = floor_key[c];
+        r1[c] = e > 0 ? erosion_scale(c, height, sediment, base, acc_fx, e,
+                                      fk != ~0ull ? (double)fkey_value((unsigned)(fk >> 32)) : -1.0e300, iter_erode, inv)
+                      : 1.0f;
+    }
+}
+
+// sweep 2: that floor lowered by what sweep 1 lets the floor cell lose
+__global__ void erosion_scale2_kernel(int64_t n, const double* height, const double* sediment, const double* base,
+                                      const ll* ero_fx, const ull* floor_key, const ll* acc_fx, const float* r1,
+                                      float* rscale, double iter_erode, double hscale) {
+    const double inv = 1.0 / hscale;
+    GRID_STRIDE(c, n) {
+        const ll e = ero_fx[c];
+        const ull fk = floor_key[c];
+        float r = 1.0f;
+        if (e > 0) {
+            double fl_next = -1.0e300;
+            if (fk != ~0ull) {
+                const int64_t nc = (int64_t)(unsigned)(fk & 0xffffffffull);
+                fl_next = (double)fkey_value((unsigned)(fk >> 32)) - (double)ero_fx[nc] * inv * (double)r1[nc];
+            }
+            r = erosion_scale(c, height, sediment, base, acc_fx, e, fl_next, iter_erode, inv);
+        }
+        rscale[c] = r;
     }
 }
 
@@ -705,10 +877,12 @@ __global__ void __launch_bounds__(TRACE_TPB) trace_fast_kernel(TraceCfg k) {
 // ---------------------------------------------------------------------------
 __global__ void resolve_kernel(int64_t n, const double* height, const double* sediment, const double* base, ll* chunk_fx,
                                ll* acc_fx, ll* pend_fx, float4* cell4, float2* dyn, double iter_deposit,
-                               double dep_floor, double writeoff, double hscale, ll* stats) {
+                               double dep_floor, double writeoff, double hscale, ll* stats, ll* ero_fx, ull* floor_key) {
     const double inv = 1.0 / hscale;
     ll clamped = 0, to_pend = 0, lost_off = 0;
     GRID_STRIDE(c, n) {
+        ero_fx[c] = 0;
+        floor_key[c] = ~0ull;
         const ll net = chunk_fx[c];
         if (net == 0) continue;
         chunk_fx[c] = 0;
@@ -801,7 +975,8 @@ __global__ void finish_kernel(int64_t n, double* height, double* sediment, doubl
 // workspace layout
 // ---------------------------------------------------------------------------
 struct Layout {
-    size_t cell4, geo, dyn, chunk_fx, acc_fx, dis_fx, mom_fx, pend_fx, wq, cdf_fx, flag, rank, comp_idx, comp_cdf,
+    size_t cell4, geo, dyn, chunk_fx, acc_fx, dis_fx, mom_fx, pend_fx, ero_fx, floor_key, rscale, r1, wq, cdf_fx, flag,
+        rank, comp_idx, comp_cdf,
         draws, u_sorted, d_cell, d_fx, d_fy, d_load, d_sa, d_sb, scalars, temp, total;
     size_t temp_bytes;
 };
@@ -820,6 +995,10 @@ int layout_of(int64_t F, int64_t NE, int64_t P, Layout* L) {
     L->dis_fx = take(8 * n);
     L->mom_fx = take(16 * n);
     L->pend_fx = take(8 * n);
+    L->ero_fx = take(8 * n);
+    L->floor_key = take(8 * n);
+    L->rscale = take(4 * n);
+    L->r1 = take(4 * n);
     L->wq = take(8 * n);
     L->cdf_fx = take(8 * n);
     L->flag = take(4 * n);
@@ -834,7 +1013,7 @@ int layout_of(int64_t F, int64_t NE, int64_t P, Layout* L) {
     L->d_load = take(8 * nd);
     L->d_sa = take(4 * nd);
     L->d_sb = take(4 * nd);
-    L->scalars = take(sizeof(Scalars) + 8 * 4096);
+    L->scalars = take(sizeof(Scalars) + 8 * 2 * 4096);
     size_t t1 = 0, t2 = 0, t3 = 0;
     if (hipcub::DeviceRadixSort::SortKeys(nullptr, t1, (const double*)nullptr, (double*)nullptr, (int)(p > 0 ? p : 1)) != hipSuccess)
         return -1;
@@ -915,7 +1094,7 @@ extern "C" int pg_pcg64_random(uint64_t slo, uint64_t shi, uint64_t ilo, uint64_
     if (n == 0) return 0;
     const int64_t threads = (n + 15) / 16;
     hipLaunchKernelGGL(pcg_fill_kernel, dim3(grid_for(threads, 8192)), dim3(TPB), 0, (hipStream_t)stream, slo, shi, ilo,
-                       ihi, (ll)start, (ll)n, out);
+                       ihi, (const ull*)nullptr, (ll)start, (ll)n, out);
     PG_LAUNCHED("pcg_fill_kernel");
     return 0;
 }
@@ -937,6 +1116,8 @@ extern "C" int pg_fast_setup(const pg_fast_fields* fl, const pg_fast_params* fp,
                        ldexp(1.0, (int)fp->height_bits));
     PG_LAUNCHED("setup_kernel");
     PG_HIP(hipMemsetAsync(at<char>(work, L.chunk_fx), 0, L.pend_fx - L.chunk_fx, s));
+    PG_HIP(hipMemsetAsync(at<char>(work, L.ero_fx), 0, L.floor_key - L.ero_fx, s));
+    PG_HIP(hipMemsetAsync(at<char>(work, L.floor_key), 0xff, L.rscale - L.floor_key, s));
     return 0;
 }
 
@@ -963,6 +1144,7 @@ int prepare(const pg_fast_fields* fl, const pg_trace_params* prm, const pg_fast_
     PG_REQUIRE(prm && fp && work && stats, "fast mode: NULL argument");
     PG_REQUIRE(fp->n_particles >= 1 && fp->n_particles < (1ll << 30), "fast mode: n_particles out of range");
     PG_REQUIRE(fp->chunks >= 1 && fp->chunks <= 4096, "fast mode: chunks must be in [1, 4096]");
+    PG_REQUIRE(fp->conc_chunk >= 0.0 && fp->iter_erode >= 0.0, "fast mode: conc_chunk and iter_erode must be >= 0");
     PG_REQUIRE(prm->max_steps >= 0 && prm->max_steps < (1ll << 30), "fast mode: bad max_steps");
     PG_REQUIRE(prm->lateral_rate <= 0.0 && prm->lake_park <= 0.0,
                "fast mode: lateral cuts and lake parks are outside the fast mode (plain profile)");
@@ -1019,7 +1201,7 @@ int fast_spawn(const pg_fast_fields* fl, const pg_fast_params* fp, void* work, C
     // the rain draws: P uniforms (sorted) then P offset pairs
     double* draws = at<double>(work, L.draws);
     hipLaunchKernelGGL(pcg_fill_kernel, dim3(grid_for((3 * P + 15) / 16, 8192)), dim3(TPB), 0, s, fp->rng_state_lo,
-                       fp->rng_state_hi, fp->rng_inc_lo, fp->rng_inc_hi, (ll)0, 3 * P, draws);
+                       fp->rng_state_hi, fp->rng_inc_lo, fp->rng_inc_hi, (const ull*)fp->rng_dev, (ll)0, 3 * P, draws);
     PG_LAUNCHED("spawn kernels");
     tb = L.temp_bytes;
     PG_HIP(hipcub::DeviceRadixSort::SortKeys(temp, tb, (const double*)draws, at<double>(work, L.u_sorted), (int)P, 0, 64, s));
@@ -1035,7 +1217,8 @@ int fast_spawn(const pg_fast_fields* fl, const pg_fast_params* fp, void* work, C
     tb = L.temp_bytes;
     PG_HIP(hipcub::DeviceScan::InclusiveSum(temp, tb, flag, rank, (int)n, s));
     hipLaunchKernelGGL(pending_drops_kernel, dim3(grid_for(n)), dim3(TPB), 0, s, n, flag, rank, pend_fx, P,
-                       fp->rng_state_lo, fp->rng_state_hi, fp->rng_inc_lo, fp->rng_inc_hi, sc, d_cell, d_fx, d_fy,
+                       fp->rng_state_lo, fp->rng_state_hi, fp->rng_inc_lo, fp->rng_inc_hi, (const ull*)fp->rng_dev, sc,
+                       d_cell, d_fx, d_fy,
                        d_load, d_sa, d_sb, st);
     hipLaunchKernelGGL(totals_kernel, dim3(1), dim3(64), 0, s, n, rank, P, sc, st, chunks);
     PG_LAUNCHED("pending kernels");
@@ -1102,6 +1285,11 @@ int fast_particles(const pg_fast_fields* fl, const pg_trace_params* prm, const p
     k.hscale = hscale;
     k.tscale = tscale;
     k.chunk_fx = at<ll>(work, L.chunk_fx);
+    k.ero_fx = at<ll>(work, L.ero_fx);
+    k.floor_key = at<ull>(work, L.floor_key);
+    k.fan_slope = (float)fp->fan_slope;
+    k.rscale = at<float>(work, L.rscale);
+    k.conc_chunk = fp->conc_chunk;
     k.dis_fx = at<ll>(work, L.dis_fx);
     k.mom_fx = at<ll>(work, L.mom_fx);
     k.stats = st;
@@ -1115,12 +1303,24 @@ int fast_particles(const pg_fast_fields* fl, const pg_trace_params* prm, const p
     }
     for (int c = 0; c < chunks; c++) {
         k.chunk = c;
+        // pass 0: erosion census on the frozen terrain
+        k.pass = 0;
+        if (fp->time_trace && timer_record(s, true)) return -1;
+        hipLaunchKernelGGL(trace_fast_kernel, dim3(grid), dim3(TRACE_TPB), 0, s, k);
+        if (fp->time_trace && timer_record(s, false)) return -1;
+        hipLaunchKernelGGL(erosion_scale_kernel, dim3(grid_for(n)), dim3(TPB), 0, s, n, fl->height, fl->sediment, fl->base,
+                           k.ero_fx, k.floor_key, at<ll>(work, L.acc_fx), at<float>(work, L.r1), fp->iter_erode, hscale);
+        hipLaunchKernelGGL(erosion_scale2_kernel, dim3(grid_for(n)), dim3(TPB), 0, s, n, fl->height, fl->sediment,
+                           fl->base, k.ero_fx, k.floor_key, at<ll>(work, L.acc_fx), at<float>(work, L.r1),
+                           at<float>(work, L.rscale), fp->iter_erode, hscale);
+        // pass 1: the same trajectories, erosion scaled per cell, deficits cancel deposits
+        k.pass = 1;
         if (fp->time_trace && timer_record(s, true)) return -1;
         hipLaunchKernelGGL(trace_fast_kernel, dim3(grid), dim3(TRACE_TPB), 0, s, k);
         if (fp->time_trace && timer_record(s, false)) return -1;
         hipLaunchKernelGGL(resolve_kernel, dim3(grid_for(n)), dim3(TPB), 0, s, n, fl->height, fl->sediment, fl->base,
                            k.chunk_fx, at<ll>(work, L.acc_fx), pend_fx, cell4, dyn, fp->iter_deposit, prm->dep_floor,
-                           fp->offshore_writeoff, hscale, st);
+                           fp->offshore_writeoff, hscale, st, k.ero_fx, k.floor_key);
     }
     PG_LAUNCHED("trace/resolve");
     hipLaunchKernelGGL(finish_kernel, dim3(grid_for(n)), dim3(TPB), 0, s, n, fl->height, fl->sediment, fl->discharge,

@@ -39,14 +39,15 @@ class FastParams(ctypes.Structure):
 
     _fields_ = [("rng_state_lo", ctypes.c_uint64), ("rng_state_hi", ctypes.c_uint64), ("rng_inc_lo", ctypes.c_uint64),
                 ("rng_inc_hi", ctypes.c_uint64), ("n_particles", i64), ("chunks", i64), ("iter_deposit", dbl),
+                ("iter_erode", dbl), ("conc_chunk", dbl), ("fan_slope", dbl),
                 ("offshore_writeoff", dbl), ("ema", dbl), ("min_volume_frac", dbl), ("height_bits", i64),
-                ("track_bits", i64), ("grid_blocks", i64), ("time_trace", i64)]
+                ("track_bits", i64), ("grid_blocks", i64), ("time_trace", i64), ("rng_dev", vp)]
 
 
 #: statistics slots of a fast iteration (include/pg_erode.h, PG_FS_*)
 FS = dict(PARTICLES=0, RAIN=1, ENTRIES=2, MOVES=3, DEATHS=4, CLAMPED=10, TO_PENDING_FX=11, LOST_OFFSHORE_FX=12,
           EXIT_FX=13, VANISHED_FX=14, RELEASED_FX=15, VOLUME0_BITS=16, ELIGIBLE=17, CDF_SERIAL=18, ERODED_FX=19,
-          DEPOSITED_FX=20)
+          DEPOSITED_FX=20, DEFICIT_FX=21)
 FAST_NSTATS = 24
 
 

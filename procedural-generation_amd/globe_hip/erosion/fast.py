@@ -90,7 +90,7 @@ def decode_stats(raw: np.ndarray, hscale: float) -> dict:
             "moves": int(raw[FS["MOVES"]]), "volume": vol0, "deaths": deaths, "clamped": int(raw[FS["CLAMPED"]]),
             "to_pending": fx("TO_PENDING_FX"), "released": fx("RELEASED_FX"), "deficit_out": fx("EXIT_FX"),
             "lost_offshore": fx("LOST_OFFSHORE_FX"), "vanished": fx("VANISHED_FX"), "eroded": fx("ERODED_FX"),
-            "deposited": fx("DEPOSITED_FX"), "eligible": int(raw[FS["ELIGIBLE"]]),
+            "deposited": fx("DEPOSITED_FX"), "deficit_cancelled": fx("DEFICIT_FX"), "eligible": int(raw[FS["ELIGIBLE"]]),
             "cdf_serial": bool(raw[FS["CDF_SERIAL"]]), "raw_fx": raw.copy()}
 
 
@@ -101,7 +101,7 @@ def run_iteration(state, params, iteration_key, n_particles=None, particles_per_
     plus ``moves`` (droplet-steps executed).  With ``sync_stats=False`` the
     statistics stay on the device (``state._fast.stats``) and nothing waits
     (default: ``state.fast_sync_stats``, True when unset)."""
-    from .maps import _check_scope, _eparams, discharge_scales, lake_park_of, max_steps_of, trace_params
+    from .maps import _check_scope, _eparams, chunk_size, discharge_scales, lake_park_of, max_steps_of, trace_params
 
     ep = _eparams(params)
     _check_scope(ep, state)
@@ -124,10 +124,14 @@ def run_iteration(state, params, iteration_key, n_particles=None, particles_per_
     slo, shi, ilo, ihi = pcg64_state(rng)
     fp = FastParams(rng_state_lo=slo, rng_state_hi=shi, rng_inc_lo=ilo, rng_inc_hi=ihi, n_particles=int(n_particles),
                     chunks=int(chunks if chunks is not None else getattr(state, "fast_chunks", DEFAULT_CHUNKS)),
-                    iter_deposit=cell_units(ep, "iter_deposit", unit), offshore_writeoff=float(ep.offshore_writeoff),
+                    iter_deposit=cell_units(ep, "iter_deposit", unit), iter_erode=cell_units(ep, "iter_erode", unit),
+                    conc_chunk=float(getattr(state, "fast_conc_chunk", -1)) if getattr(state, "fast_conc_chunk", -1) >= 0
+                    else float(chunk_size(ep, n_particles)), fan_slope=float(ep.fan_slope),
+                    offshore_writeoff=float(ep.offshore_writeoff),
                     ema=float(ep.ema), min_volume_frac=float(ep.min_volume_frac), height_bits=HEIGHT_BITS,
                     track_bits=TRACK_BITS, grid_blocks=int(getattr(state, "fast_grid_blocks", 0)),
-                    time_trace=int(bool(getattr(state, "fast_time_trace", False))))
+                    time_trace=int(bool(getattr(state, "fast_time_trace", False))),
+                    rng_dev=(state._fast_rng_dev.data_ptr() if getattr(state, "_fast_rng_dev", None) is not None else None))
     check(lib().pg_fast_iterate(ctypes.byref(ctx.fields), ctypes.byref(prm), ctypes.byref(fp),
                                 ctypes.c_void_p(ctx.work.data_ptr()), ctx.work.numel(),
                                 ctypes.c_void_p(ctx.stats.data_ptr()), pk._stream()), "pg_fast_iterate")
@@ -154,3 +158,63 @@ def trace_time():
     n = ctypes.c_int64()
     check(lib().pg_fast_trace_time(ctypes.byref(ms), ctypes.byref(n)), "pg_fast_trace_time")
     return ms.value, n.value
+
+
+def _signed(w: int) -> int:
+    return w - (1 << 64) if w >= (1 << 63) else w
+
+
+class IterationGraph:
+    """A whole fast-mode erosion iteration -- particle pass, uplift, datum
+    hold, halo exchange (bake/globe/erosion/maps.py:1516-1617 in the plain
+    profile) -- captured once as a HIP graph and replayed per iteration.
+
+    The only per-iteration input from the host is the iteration's PCG64
+    state (``params.rng(stage, *key)``), copied into a device buffer before
+    each replay; the kernels read it from there.  Everything else (droplet
+    counts, the stockpiles to re-inject, the datum shift) stays on the
+    device, so a replay needs no host synchronisation.  Statistics of the
+    last replay: ``self.stats`` (device int64, ``decode_stats``)."""
+
+    def __init__(self, state, params, rng_stage: str = "erosion", n_particles=None):
+        from . import maps
+
+        if state.mode != "fast":
+            raise ValueError("graph capture runs the fast mode")
+        self.state, self.params, self.rng_stage = state, params, rng_stage
+        ep = maps._eparams(params)
+        dev = state.device
+        state.fast_sync_stats = False
+        state.fast_time_trace = False
+        self.rng_dev = torch.zeros(4, dtype=torch.int64, device=dev)
+        state._fast_rng_dev = self.rng_dev
+        # everything the iteration allocates lazily, before capture
+        n_cells = int(np.count_nonzero(state._mask_host[state.interior] == pk.MASK_ACTIVE))
+        if n_particles is None:
+            n_particles = max(1, int(round(ep.particles_per_cell * n_cells)))
+        self.n_particles = n_particles
+        ctx = _fast_context(state, n_particles)
+        state.uplift_mean(maps.uplift_cap(state, ep))
+        state.select_work()
+        if state.spherical:
+            state.halo_tables()
+        torch.cuda.synchronize()
+        self._load_rng((0,))
+        self.graph = torch.cuda.CUDAGraph()
+        it0 = state.iteration
+        with torch.cuda.graph(self.graph):
+            self.result = maps.step(state, params, (0,), n_particles=n_particles)
+        state.iteration = it0
+        self.stats = ctx.stats
+
+    def _load_rng(self, key):
+        rng = self.params.rng(self.rng_stage, *key)
+        words = [_signed(w) for w in pcg64_state(rng)]
+        self.rng_dev.copy_(torch.tensor(words, dtype=torch.int64), non_blocking=False)
+
+    def step(self, iteration_key) -> None:
+        """Run one iteration (asynchronous)."""
+        key = iteration_key if isinstance(iteration_key, (tuple, list)) else (int(iteration_key),)
+        self._load_rng(key)
+        self.graph.replay()
+        self.state.iteration += 1

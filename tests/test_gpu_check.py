@@ -33,15 +33,20 @@ def _capture(store, to_np):
 
 
 def _cmp_chunks(dev, ora):
+    """Number of droplets whose change list (cells, volumes, count, death)
+    differs between the device and the oracle."""
     assert len(dev) == len(ora)
+    bad = 0
     for (c1, e1, v1, d1, cap, n), (c2, e2, v2, d2, cap2, n2) in zip(dev, ora):
         assert cap == cap2 and n == n2
-        np.testing.assert_array_equal(c1, c2)
-        np.testing.assert_array_equal(d1, d2)
         for p in range(n):
+            if c1[p] != c2[p] or d1[p] != d2[p]:
+                bad += 1
+                continue
             sl = slice(p * cap, p * cap + int(c1[p]))
-            np.testing.assert_array_equal(e1[sl], e2[sl])
-            np.testing.assert_array_equal(v1[sl], v2[sl])
+            if not (np.array_equal(e1[sl], e2[sl]) and np.array_equal(v1[sl], v2[sl])):
+                bad += 1
+    return bad
 
 
 def _cmp_float(name, a, b, exact_frac=0.999, rtol=1e-9):
@@ -55,11 +60,15 @@ def _cmp_float(name, a, b, exact_frac=0.999, rtol=1e-9):
     return same
 
 
-def _run_pair(oracle, ostate, dstate, params, key):
+def _run_pair(oracle, ostate, dstate, params, key, allow=0):
     o_chunks, d_chunks = [], []
     so = oracle.run_iteration(ostate, params, key, diag=_capture(o_chunks, lambda x: x))
     sd = maps.run_iteration(dstate, params, key, diag=_capture(d_chunks, lambda t: t.cpu().numpy()))
-    _cmp_chunks(d_chunks, o_chunks)
+    bad = _cmp_chunks(d_chunks, o_chunks)
+    print(f"droplets whose change list differs: {bad} of {so['particles']}", flush=True)
+    assert bad <= allow, bad
+    if bad:
+        return so, sd
     for k in ("particles", "entries", "deaths", "clamped"):
         assert sd[k] == so[k], (k, sd[k], so[k])
     for k in ("to_pending", "deficit_out", "lost_offshore", "volume"):
@@ -67,10 +76,10 @@ def _run_pair(oracle, ostate, dstate, params, key):
     return so, sd
 
 
-def _cmp_state(ostate, dstate):
+def _cmp_state(ostate, dstate, exact_frac=0.999, rtol=1e-9):
     fr = {}
     for n in ("height", "sediment", "pending", "discharge", "momentum"):
-        fr[n] = _cmp_float(n, getattr(dstate, n).cpu().numpy(), getattr(ostate, n))
+        fr[n] = _cmp_float(n, getattr(dstate, n).cpu().numpy(), getattr(ostate, n), exact_frac, rtol)
     return fr
 
 
@@ -91,15 +100,25 @@ def test_window_iterations_match_oracle(oracle, cuda, mode, N):
         assert ost.pending.sum() > 0  # the full shelf parks what it cannot place
 
 
-@pytest.mark.parametrize("N", [64, 128])
-def test_globe_first_iteration_matches_oracle(oracle, cuda, N):
+@pytest.mark.parametrize("N,libm", [(64, "glibc"), (128, "glibc"), (128, "correctly-rounded")])
+def test_globe_first_iteration_matches_oracle(oracle, hiplib, cuda, N, libm):
+    """With the oracle on glibc's tan/atan (the reference's), a handful of
+    face-crossing droplets may differ (glibc misrounds the last bit of
+    ~0.1-0.25 % of tan/atan values); with the oracle switched to the
+    correctly rounded tan/atan the device uses, everything is bit-identical."""
     grid, f = stubs.plain_globe(N, 2)
     p = stubs.workload_params(N, 2)
     ost = oracle.OracleState.from_grid(grid, f["bedrock_m"], f["hardness"], f["precip"], f["evap"], f["uplift_m"], p.erosion)
     dst = maps.ErosionState.from_grid(grid, f["bedrock_m"], f["hardness"], f["precip"], f["evap"], f["uplift_m"], p.erosion, device=cuda)
-    print(f"globe N={N} iteration 0", flush=True)
-    so, sd = _run_pair(oracle, ost, dst, p, 0)
-    fr = _cmp_state(ost, dst)
+    print(f"globe N={N} iteration 0, oracle libm {libm}", flush=True)
+    cr = libm == "correctly-rounded"
+    if cr:
+        oracle.set_libm(hiplib.pg_geom_tan, hiplib.pg_geom_atan)
+    try:
+        so, sd = _run_pair(oracle, ost, dst, p, 0, allow=0 if cr else 10)
+    finally:
+        oracle.set_libm(None, None)
+    fr = _cmp_state(ost, dst, exact_frac=1.0 if cr else 0.999, rtol=0.0 if cr else 1e-6)
     print("bit-identical fractions", fr, flush=True)
 
 
