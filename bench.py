@@ -125,6 +125,20 @@ def cpu_baseline(grid, f, p, fraction=8):
                       f"reference chunking ({s['chunk']}) and serial apply; {moves[0]} moves in {dt:.2f} s"}
 
 
+def copy_probe(mib):
+    """A plain copy of known size: its FETCH_SIZE / WRITE_SIZE per dispatch
+    calibrate the HBM counters (2 x mib MiB of traffic per copy)."""
+    import torch
+
+    n = mib * (1 << 20) // 4
+    src = torch.ones(n, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(3):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    log(f"copy probe: 3 copies of {mib} MiB")
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -134,7 +148,12 @@ def main():
     ap.add_argument("--chunks", type=int, default=16)
     ap.add_argument("--max-steps", type=int, default=128)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-probe-mib", type=int, default=0,
+                    help="only run 3 device-to-device copies of this many MiB (HBM counter calibration) and exit")
     a = ap.parse_args()
+    if a.copy_probe_mib:
+        copy_probe(a.copy_probe_mib)
+        return
     if a.gpus != 1:
         print(f"bench.py measures one GPU: the particle pass shards by droplets with one int64 all-reduce per "
               f"iteration (designed, not built); --gpus {a.gpus} is not supported", file=sys.stderr)
