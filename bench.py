@@ -125,6 +125,56 @@ def cpu_baseline(grid, f, p, fraction=8):
                       f"reference chunking ({s['chunk']}) and serial apply; {moves[0]} moves in {dt:.2f} s"}
 
 
+def adjoint_run(N=256, iterations=50, checkpoint_every=10, max_steps=128, seed=0):
+    """Wall time of the differentiable mode's forward pass alone and of
+    forward + hand-written adjoint (gradient of the hypsometry misfit with
+    respect to the erodibility field and the six scalars), float32, on the
+    N_c=256 earth globe, 50 iterations, checkpoints every 10."""
+    import torch
+
+    from globe_hip import stubs
+    from globe_hip.erosion import adjoint as adj
+    from globe_hip.erosion import maps
+
+    t = time.time()
+    grid, f = stubs.earth_globe(N, 2)
+    p = stubs.workload_params(N, 2, max_steps=max_steps, iterations=iterations)
+    st = maps.ErosionState.from_grid(grid, f["bedrock_m"], f["hardness"], f["precip"], f["evap"], f["uplift_m"],
+                                     p.erosion, device="cuda", mode="fast")
+    run = adj.GradRun(st, p, iterations, real="f32", checkpoint_every=checkpoint_every)
+    g = torch.Generator().manual_seed(seed)
+    erod = (1.0 + 0.2 * (2.0 * torch.rand(tuple(st.height.shape), generator=g, dtype=torch.float64) - 1.0)).float().cuda()
+    theta = adj.theta_nominal(run)
+    log(f"adjoint inputs N={N}: {time.time() - t:.1f}s")
+
+    def fwd():
+        out = run.forward(erod, theta, tapes="none")
+        return float(adj.hypsometry_loss(out["height"] + out["sediment"], run))
+
+    def fwd_adj():
+        e, th = erod.clone().requires_grad_(True), theta.clone().requires_grad_(True)
+        loss = adj.hypsometry_loss(adj.erosion(e, th, run), run)
+        loss.backward()
+        return float(loss), e.grad, th.grad
+
+    fwd_adj()  # warm-up: tapes and checkpoints allocated
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    loss_f = fwd()
+    torch.cuda.synchronize()
+    t_fwd = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    loss_a, ge, gt = fwd_adj()
+    torch.cuda.synchronize()
+    t_fa = time.perf_counter() - t0
+    log(f"adjoint: forward {t_fwd:.3f}s, forward+adjoint {t_fa:.3f}s, ratio {t_fa / t_fwd:.2f}; loss {loss_f:.6g}")
+    return {"workload": f"earth globe N_c={N} H=2, {iterations} iterations, lifetime {max_steps}, float32, "
+                        f"checkpoint every {checkpoint_every}; loss = 64-bin land hypsometric CDF vs Beta(2,5) on "
+                        f"[0, 4000] m + 0.1 (mean land - 800 m)^2; erodibility field 1 +- 0.2",
+            "forward_s": t_fwd, "forward_adjoint_s": t_fa, "ratio": t_fa / t_fwd, "loss": loss_a,
+            "grad_erod_norm": float(ge.double().norm()), "grad_theta": [float(v) for v in gt]}
+
+
 def copy_probe(mib):
     """A plain copy of known size: its FETCH_SIZE / WRITE_SIZE per dispatch
     calibrate the HBM counters (2 x mib MiB of traffic per copy)."""
@@ -148,6 +198,7 @@ def main():
     ap.add_argument("--chunks", type=int, default=16)
     ap.add_argument("--max-steps", type=int, default=128)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="only run 3 device-to-device copies of this many MiB (HBM counter calibration) and exit")
     a = ap.parse_args()
@@ -187,6 +238,12 @@ def main():
                      "bytes_per_step": {"apply_pass": BYTES_PER_STEP, "census_pass": BYTES_PER_STEP_CENSUS},
                      "launch_ms_mean": r["trace_ms"] / max(r["launches"], 1), "launches": r["launches"]},
     }
+    if not a.no_adjoint:
+        import torch
+
+        torch.cuda.empty_cache()
+        out["adjoint"] = adjoint_run()
+        out["fwd_adjoint_ratio"] = out["adjoint"]["ratio"]
     if not a.no_cpu:
         out["cpu_baseline"] = cpu_baseline(grid, f, p)
     print(json.dumps(out), flush=True)

@@ -140,6 +140,13 @@ int pg_spawn_cells(const double* cdf, int64_t M, const double* u, int64_t P, int
 int pg_exchange_halos(double* data, int64_t ncomp, int64_t K, int64_t S, const int64_t* dst, const int64_t* src,
                       const double* w, const double* rot, pg_stream stream);
 
+/* Adjoint of the scalar exchange (no counterpart in the reference, which has
+ * no adjoint): adj[src[k,s]] += w[k,s] adj[dst[k]], then adj[dst[k]] = 0 --
+ * the halo entries of a gradient folded onto the interior cells they are
+ * interpolated from (order of the double additions unspecified). */
+int pg_halo_scalar_adjoint(double* adj, int64_t K, int64_t S, const int64_t* dst, const int64_t* src, const double* w,
+                           pg_stream stream);
+
 /* ---- datum hold (bake/globe/erosion/maps.py:1446-1508, hold_datum) ----- */
 /* Exact k-th smallest (0-based) of height + sediment over the interior
  * (F faces, N x N, halo H), written to the DEVICE double *q_out.
@@ -235,6 +242,44 @@ int pg_fast_layout(int64_t F, int64_t NE, int64_t n_particles, int64_t* offsets,
  * time_trace set) and returns their summed duration (ms, HIP events on the
  * launch stream) and count.  Library-wide state: not thread-safe. */
 int pg_fast_trace_time(double* ms_total, int64_t* launches);
+
+/* ---- differentiable mode (no counterpart in the reference) ---------------
+ * The iteration as a function of the surface, a per-cell erodibility factor
+ * and six scalars with the droplet trajectories frozen, and its adjoint.
+ * Exchange arithmetic of pg_trace_particles' plain profile (particle.py:
+ * 562-660) without apply-time caps, one frozen-terrain chunk per iteration. */
+typedef struct pg_grad_args {
+    int32_t real;               /* 0: float32 exchange arithmetic, 1: float64 */
+    int32_t pad;
+    const void* surf;           /* exchange surface (height + sediment), F*NE*NE float or double */
+    const float* erod;          /* per-cell erodibility factor multiplying c_eq, or NULL (1) */
+    double theta[6];            /* erodibility, deposition_rate, evap_rate, k_disc, ocean_rate, dep_floor (cell units) */
+    void* tape;                 /* NULL or pg_grad_tape_bytes() bytes: every move's exchange inputs */
+    int64_t tape_bytes;
+    const int32_t* fixed_moves; /* NULL, or per droplet (moves | cause << 24) of the run being replayed */
+    int32_t* moves_out;         /* NULL, or per droplet (moves | cause << 24) of this run */
+} pg_grad_args;
+
+/* Bytes of a tape holding `capacity` droplets of up to max_steps moves. */
+int pg_grad_tape_bytes(int64_t capacity, int64_t max_steps, int32_t real, int64_t* bytes);
+
+/* One iteration's particle pass in the differentiable mode: pack and spawn
+ * from `steer` (its packed surface, discharge and base steer the droplets;
+ * fp->chunks must be 1), exchange against args->surf with args->theta and
+ * args->erod, fold + EMA into `target`.  With args->fixed_moves every
+ * droplet stops after the given moves with the given cause (the
+ * trajectory of the run that wrote them). */
+int pg_grad_iterate(const pg_fast_fields* steer, const pg_fast_fields* target, const pg_trace_params* prm,
+                    const pg_fast_params* fp, const pg_grad_args* args, void* work, int64_t work_bytes,
+                    int64_t* stats, pg_stream stream);
+
+/* Reverse sweep of a taped iteration: with g = dL/d(surface after the
+ * particle pass) (F*NE*NE double), adds the pass's share of dL/d(surface
+ * before it) into lam, dL/d(erodibility factor) into grad_erod (F*NE*NE
+ * double) and dL/d(theta) into grad_theta (double[6]); double atomics, so
+ * sums are reproducible to rounding only. */
+int pg_grad_reverse(int64_t F, int64_t N, int64_t H, const pg_trace_params* prm, const pg_grad_args* args,
+                    const double* g, double* lam, double* grad_erod, double* grad_theta, pg_stream stream);
 
 /* Fill out[k] = the (start + k)-th double of a PCG64 stream (numpy's
  * Generator.random() draws), k < n: the fast mode's spawn source. */

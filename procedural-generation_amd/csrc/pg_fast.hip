@@ -35,32 +35,11 @@
 
 #pragma clang fp contract(fast)
 
+#include "pg_fast_shared.h"
+
 namespace pg {
+namespace fastmode {
 namespace {
-
-constexpr int TPB = 256;
-constexpr int TRACE_TPB = 256;
-
-inline int grid_for(int64_t n, int cap = 65535) {
-    int b = blocks_for(n, TPB);
-    return b < 1 ? 1 : (b > cap ? cap : b);
-}
-
-#define GRID_STRIDE(i, n) for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < (n); i += (int64_t)gridDim.x * blockDim.x)
-
-typedef unsigned long long ull;
-typedef long long ll;
-
-// ---------------------------------------------------------------------------
-// device scalars of an iteration
-// ---------------------------------------------------------------------------
-struct Scalars {
-    double volume0, q_weight, total;
-    ll eligible, pending, n_total, cdf_exact;
-    int emin;
-    int pad;
-    ll queue[1];  // followed by `chunks` queue counters
-};
 
 // ---------------------------------------------------------------------------
 // PCG64 (numpy's: 128-bit LCG, XSL-RR output, doubles from the top 53 bits)
@@ -381,44 +360,6 @@ struct TraceCfg {
     ll* stats;
 };
 
-struct Stencil {
-    float4 c00, c10, c01, c11;  // records at (i0, j0), (i0+1, j0), (i0, j0+1), (i0+1, j0+1)
-    float wi, wj;
-    int i0, j0;
-};
-
-__device__ __forceinline__ float lerp2(float v00, float v10, float v01, float v11, float wi, float wj) {
-    return v00 * (1.0f - wi) * (1.0f - wj) + v10 * wi * (1.0f - wj) + v01 * (1.0f - wi) * wj + v11 * wi * wj;
-}
-
-// the 2x2 stencil around a position given as (cell, fraction) on face f
-__device__ __forceinline__ Stencil load_stencil(const TraceCfg& k, int f, int ix, int iy, float fx, float fy) {
-    int i0 = ix + k.H - (fx < 0.5f ? 1 : 0), j0 = iy + k.H - (fy < 0.5f ? 1 : 0);
-    float wi = fx < 0.5f ? fx + 0.5f : fx - 0.5f, wj = fy < 0.5f ? fy + 0.5f : fy - 0.5f;
-    if (i0 < 0) { i0 = 0; wi = 0.0f; }
-    if (i0 > k.NE - 2) { i0 = k.NE - 2; wi = 1.0f; }
-    if (j0 < 0) { j0 = 0; wj = 0.0f; }
-    if (j0 > k.NE - 2) { j0 = k.NE - 2; wj = 1.0f; }
-    const int64_t b = ((int64_t)f * k.NE + i0) * k.NE + j0;
-    Stencil s;
-    s.c00 = k.cell4[b];
-    s.c01 = k.cell4[b + 1];
-    s.c10 = k.cell4[b + k.NE];
-    s.c11 = k.cell4[b + k.NE + 1];
-    s.wi = wi;
-    s.wj = wj;
-    s.i0 = i0;
-    s.j0 = j0;
-    return s;
-}
-
-// the record of extended cell (ei, ej): always one of the stencil's corners
-// when the position lies in that cell
-__device__ __forceinline__ float4 corner_of(const Stencil& s, int ei, int ej) {
-    const bool ri = ei != s.i0, rj = ej != s.j0;
-    return ri ? (rj ? s.c11 : s.c10) : (rj ? s.c01 : s.c00);
-}
-
 // what a droplet reads at the cell it stands in
 struct Here {
     int64_t cell;
@@ -426,13 +367,6 @@ struct Here {
     float h, base, sthick, rs;
     float4 g0, g1;
 };
-
-__device__ __forceinline__ int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
-
-__device__ __forceinline__ ll wave_sum(ll v) {
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
-    return v;
-}
 
 // order-preserving key of a float (for atomicMin of surfaces)
 __device__ __forceinline__ unsigned fkey(float x) {
@@ -546,7 +480,7 @@ __global__ void __launch_bounds__(TRACE_TPB) trace_fast_kernel(TraceCfg k) {
                         in_sea = false;
                         step = uphill = sea_steps = pushes = 0;
                         last_cell = -1;
-                        S = load_stencil(k, f, ix, iy, fx, fy);
+                        S = load_stencil(k.cell4, k.NE, k.H, f, ix, iy, fx, fy);
                         load_here(f, ix, iy, cur, nullptr);
                         alive = true;
                     }
@@ -577,80 +511,18 @@ __global__ void __launch_bounds__(TRACE_TPB) trace_fast_kernel(TraceCfg k) {
                     pushes += 1;
                 }
                 const float4 g0 = cur.g0, g1 = cur.g1;
-                // forces
-                const float h00 = S.c00.x, h10 = S.c10.x, h01 = S.c01.x, h11 = S.c11.x, wi = S.wi, wj = S.wj;
-                const float h0 = lerp2(h00, h10, h01, h11, wi, wj);
-                const float gi = (h10 - h00) * (1.0f - wj) + (h11 - h01) * wj;
-                const float gj = (h01 - h00) * (1.0f - wi) + (h11 - h10) * wi;
-                const float di = g0.w * gi + g1.x * gj;
-                const float dj = g1.x * gi + g1.y * gj;
-                const float s2 = fmaxf(gi * di + gj * dj, 0.0f);
-                const float nrm = k.slope_sat > 0.0f ? k.slope_gain * rsqrtf(s2 + k.slope_sat * k.slope_sat)
-                                                    : k.slope_gain * rsqrtf(1.0f + s2);
-                const float na = -di * nrm, nb = -dj * nrm;
-                const float q = fmaxf(lerp2(S.c00.y, S.c10.y, S.c01.y, S.c11.y, wi, wj), 0.0f);
+                const float q = fmaxf(lerp2(S.c00.y, S.c10.y, S.c01.y, S.c11.y, S.wi, S.wj), 0.0f);
                 const float vol_rel = vol * inv_v0;
-                sa *= k.fr;
-                sb *= k.fr;
-                if (k.k_mom != 0.0f) {
-                    const float ma = lerp2(S.c00.z, S.c10.z, S.c01.z, S.c11.z, wi, wj);
-                    const float mb = lerp2(S.c00.w, S.c10.w, S.c01.w, S.c11.w, wi, wj);
-                    const float mlen = sqrtf(fmaxf(g0.x * ma * ma + 2.0f * g0.y * ma * mb + g0.z * mb * mb, 0.0f));
-                    const float slen = sqrtf(fmaxf(g0.x * sa * sa + 2.0f * g0.y * sa * sb + g0.z * sb * sb, 0.0f));
-                    if (mlen > 1e-12f && slen > 1e-12f) {
-                        const float cosang = (g0.x * ma * sa + g0.y * (ma * sb + mb * sa) + g0.z * mb * sb) / (mlen * slen);
-                        const float kk = k.k_mom * cosang * k.mom_scale / (vol + k.mom_scale * q);
-                        sa += kk * ma;
-                        sb += kk * mb;
-                    }
-                }
-                const float acc = k.dt / (vol_rel * k.density);
-                sa += acc * na;
-                sb += acc * nb;
-                const float n2 = fmaxf(g0.x * sa * sa + 2.0f * g0.y * sa * sb + g0.z * sb * sb, 0.0f);
-                // (a non-finite direction -- only from non-finite inputs -- stops the droplet
-                // rather than turning into a cell index)
-                if (!(n2 >= 1e-24f) || !(n2 <= 3.0e38f)) {
-                    cause = DEATH_STOP;
-                } else {
-                    const float inv = rsqrtf(n2);
-                    sa *= inv;
-                    sb *= inv;
-                    // one cell step, across a face edge if need be
-                    float nfx = fx + sa, nfy = fy + sb;
-                    const float flx = floorf(nfx), fly = floorf(nfy);
-                    int nix = ix + (int)flx, niy = iy + (int)fly;
-                    nfx = fminf(nfx - flx, 0.99999994f);
-                    nfy = fminf(nfy - fly, 0.99999994f);
-                    int nf = f;
-                    if (k.spherical) {
-                        if (nix < 0 || nix >= k.N || niy < 0 || niy >= k.N) {
-                            const double Nd = (double)k.N;
-                            const Transfer t = transfer_velocity<LibMath>(f, ((double)nix + nfx) / Nd, ((double)niy + nfy) / Nd,
-                                                                          (double)sa, (double)sb);
-                            nf = t.face;
-                            double nx = t.u * Nd, ny = t.v * Nd;
-                            if (nx >= Nd) nx = Nd * 0.999999999;
-                            if (ny >= Nd) ny = Nd * 0.999999999;
-                            if (!(nx >= 0.0)) nx = 0.0;
-                            if (!(ny >= 0.0)) ny = 0.0;
-                            const double fnx = floor(nx), fny = floor(ny);
-                            nix = (int)fnx;
-                            niy = (int)fny;
-                            nfx = fminf((float)(nx - fnx), 0.99999994f);
-                            nfy = fminf((float)(ny - fny), 0.99999994f);
-                            sa = (float)t.a;
-                            sb = (float)t.b;
-                        }
-                    } else if (nix < -k.H || nix >= k.N + k.H || niy < -k.H || niy >= k.N + k.H) {
-                        cause = DEATH_EXIT;
-                        exited = true;
-                    }
+                int nf, nix, niy;
+                float nfx, nfy, h0;
+                cause = steer_move(k, S, g0, g1, q, vol, vol_rel, sa, sb, f, ix, iy, fx, fy, nf, nix, niy, nfx, nfy, h0);
+                exited = cause == DEATH_EXIT;
+                {
                     if (cause < 0) {
                         // everything the next step reads, issued together
                         const int nei = nix + k.H, nej = niy + k.H;
                         const int64_t ncell = ((int64_t)nf * k.NE + nei) * k.NE + nej;
-                        const Stencil S1 = load_stencil(k, nf, nix, niy, nfx, nfy);
+                        const Stencil S1 = load_stencil(k.cell4, k.NE, k.H, nf, nix, niy, nfx, nfy);
                         const int nm = k.mask[ncell];
                         const float2 ndy = k.dyn[ncell];
                         const float4 nrec = corner_of(S1, nei, nej);
@@ -971,15 +843,43 @@ __global__ void finish_kernel(int64_t n, double* height, double* sediment, doubl
     }
 }
 
-// ---------------------------------------------------------------------------
-// workspace layout
-// ---------------------------------------------------------------------------
-struct Layout {
-    size_t cell4, geo, dyn, chunk_fx, acc_fx, dis_fx, mom_fx, pend_fx, ero_fx, floor_key, rscale, r1, wq, cdf_fx, flag,
-        rank, comp_idx, comp_cdf,
-        draws, u_sorted, d_cell, d_fx, d_fy, d_load, d_sa, d_sb, scalars, temp, total;
-    size_t temp_bytes;
+// trace-launch timing (pg_fast_trace_time): event pairs reused across calls
+struct TraceTimer {
+    hipEvent_t start[4096], stop[4096];
+    int created = 0, recorded = 0;
 };
+TraceTimer g_timer;
+
+int timer_record(hipStream_t s, bool is_start) {
+    TraceTimer& t = g_timer;
+    const int i = t.recorded;
+    PG_REQUIRE(i < 4096, "trace timing: more than 4096 launches recorded without pg_fast_trace_time");
+    if (i >= t.created) {
+        PG_HIP(hipEventCreate(&t.start[i]));
+        PG_HIP(hipEventCreate(&t.stop[i]));
+        t.created = i + 1;
+    }
+    if (is_start) {
+        PG_HIP(hipEventRecord(t.start[i], s));
+    } else {
+        PG_HIP(hipEventRecord(t.stop[i], s));
+        t.recorded = i + 1;
+    }
+    return 0;
+}
+
+int check_fields(const pg_fast_fields* fl) {
+    PG_REQUIRE(fl != nullptr, "fast mode: fields is NULL");
+    PG_REQUIRE(fl->F > 0 && fl->N >= 2 && fl->H >= 1, "fast mode: bad grid F=%lld N=%lld H=%lld", (long long)fl->F,
+               (long long)fl->N, (long long)fl->H);
+    PG_REQUIRE((fl->F * (fl->N + 2 * fl->H) * (fl->N + 2 * fl->H)) < (1ll << 31), "fast mode: grid too large for int32 cell ids");
+    PG_REQUIRE(fl->height && fl->sediment && fl->discharge && fl->momentum && fl->pending && fl->base && fl->precip &&
+                   fl->evap && fl->hardness && fl->metric && fl->metric_inv && fl->mask,
+               "fast mode: NULL field pointer");
+    return 0;
+}
+
+}  // namespace
 
 inline size_t up(size_t x) { return (x + 255) & ~(size_t)255; }
 
@@ -1027,51 +927,24 @@ int layout_of(int64_t F, int64_t NE, int64_t P, Layout* L) {
     return 0;
 }
 
-template <class T>
-T* at(void* work, size_t off) {
-    return reinterpret_cast<T*>(reinterpret_cast<char*>(work) + off);
-}
-
-// trace-launch timing (pg_fast_trace_time): event pairs reused across calls
-struct TraceTimer {
-    hipEvent_t start[4096], stop[4096];
-    int created = 0, recorded = 0;
-};
-TraceTimer g_timer;
-
-int timer_record(hipStream_t s, bool is_start) {
-    TraceTimer& t = g_timer;
-    const int i = t.recorded;
-    PG_REQUIRE(i < 4096, "trace timing: more than 4096 launches recorded without pg_fast_trace_time");
-    if (i >= t.created) {
-        PG_HIP(hipEventCreate(&t.start[i]));
-        PG_HIP(hipEventCreate(&t.stop[i]));
-        t.created = i + 1;
+int persistent_grid(const void* kernel, int threads, int64_t requested, int* grid) {
+    if (requested > 0) {
+        *grid = (int)requested;
+        return 0;
     }
-    if (is_start) {
-        PG_HIP(hipEventRecord(t.start[i], s));
-    } else {
-        PG_HIP(hipEventRecord(t.stop[i], s));
-        t.recorded = i + 1;
-    }
+    int dev = 0, cus = 0, per = 0;
+    PG_HIP(hipGetDevice(&dev));
+    PG_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    PG_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, kernel, threads, 0));
+    *grid = cus * (per > 0 ? per : 1);
     return 0;
 }
 
-int check_fields(const pg_fast_fields* fl) {
-    PG_REQUIRE(fl != nullptr, "fast mode: fields is NULL");
-    PG_REQUIRE(fl->F > 0 && fl->N >= 2 && fl->H >= 1, "fast mode: bad grid F=%lld N=%lld H=%lld", (long long)fl->F,
-               (long long)fl->N, (long long)fl->H);
-    PG_REQUIRE((fl->F * (fl->N + 2 * fl->H) * (fl->N + 2 * fl->H)) < (1ll << 31), "fast mode: grid too large for int32 cell ids");
-    PG_REQUIRE(fl->height && fl->sediment && fl->discharge && fl->momentum && fl->pending && fl->base && fl->precip &&
-                   fl->evap && fl->hardness && fl->metric && fl->metric_inv && fl->mask,
-               "fast mode: NULL field pointer");
-    return 0;
-}
-
-}  // namespace
+}  // namespace fastmode
 }  // namespace pg
 
 using namespace pg;
+using namespace pg::fastmode;
 
 extern "C" int64_t pg_fast_workspace_bytes(int64_t F, int64_t NE, int64_t n_particles) {
     clear_error();
@@ -1122,21 +995,7 @@ extern "C" int pg_fast_setup(const pg_fast_fields* fl, const pg_fast_params* fp,
 }
 
 namespace pg {
-namespace {
-struct Ctx {
-    Layout L;
-    hipStream_t s;
-    double hscale, tscale;
-    Scalars* sc;
-    float4* cell4;
-    float2* dyn;
-    int *flag, *rank;
-    ll *pend_fx, *st;
-    void* temp;
-    int chunks;
-    int64_t N, H, NE, n;
-    ll P;
-};
+namespace fastmode {
 
 int prepare(const pg_fast_fields* fl, const pg_trace_params* prm, const pg_fast_params* fp, void* work,
             int64_t work_bytes, int64_t* stats, pg_stream stream, Ctx* c) {
@@ -1293,14 +1152,8 @@ int fast_particles(const pg_fast_fields* fl, const pg_trace_params* prm, const p
     k.dis_fx = at<ll>(work, L.dis_fx);
     k.mom_fx = at<ll>(work, L.mom_fx);
     k.stats = st;
-    int grid = (int)fp->grid_blocks;
-    if (grid <= 0) {
-        int dev = 0, cus = 0, per = 0;
-        PG_HIP(hipGetDevice(&dev));
-        PG_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
-        PG_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, trace_fast_kernel, TRACE_TPB, 0));
-        grid = cus * (per > 0 ? per : 1);
-    }
+    int grid = 0;
+    if (persistent_grid((const void*)trace_fast_kernel, TRACE_TPB, fp->grid_blocks, &grid)) return -1;
     for (int c = 0; c < chunks; c++) {
         k.chunk = c;
         // pass 0: erosion census on the frozen terrain
@@ -1323,13 +1176,17 @@ int fast_particles(const pg_fast_fields* fl, const pg_trace_params* prm, const p
                            fp->offshore_writeoff, hscale, st, k.ero_fx, k.floor_key);
     }
     PG_LAUNCHED("trace/resolve");
-    hipLaunchKernelGGL(finish_kernel, dim3(grid_for(n)), dim3(TPB), 0, s, n, fl->height, fl->sediment, fl->discharge,
-                       fl->momentum, fl->pending, fl->mask, at<ll>(work, L.acc_fx), k.dis_fx, k.mom_fx, pend_fx, fp->ema,
-                       hscale, tscale, st);
+    return finish_iteration(fl, fp, work, c);
+}
+
+int finish_iteration(const pg_fast_fields* fl, const pg_fast_params* fp, void* work, Ctx& c) {
+    hipLaunchKernelGGL(finish_kernel, dim3(grid_for(c.n)), dim3(TPB), 0, c.s, c.n, fl->height, fl->sediment,
+                       fl->discharge, fl->momentum, fl->pending, fl->mask, at<ll>(work, c.L.acc_fx),
+                       at<ll>(work, c.L.dis_fx), at<ll>(work, c.L.mom_fx), c.pend_fx, fp->ema, c.hscale, c.tscale, c.st);
     PG_LAUNCHED("finish_kernel");
     return 0;
 }
-}  // namespace
+}  // namespace fastmode
 }  // namespace pg
 
 extern "C" int pg_fast_spawn(const pg_fast_fields* fl, const pg_trace_params* prm, const pg_fast_params* fp, void* work,

@@ -1,0 +1,215 @@
+// What the fast mode's translation unit shares with the differentiable mode:
+// the per-iteration device scalars, the workspace layout, the spawn
+// pipeline, and the droplet steering step (one cell move on the frozen
+// stencil record).  Internal to the library.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "pg_common.h"
+#include "pg_erode.h"
+#include "pg_internal.h"
+
+namespace pg {
+namespace fastmode {
+
+typedef unsigned long long ull;
+typedef long long ll;
+
+constexpr int TPB = 256;
+constexpr int TRACE_TPB = 256;
+
+inline int grid_for(int64_t n, int cap = 65535) {
+    int b = blocks_for(n, TPB);
+    return b < 1 ? 1 : (b > cap ? cap : b);
+}
+
+#define GRID_STRIDE(i, n) for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < (n); i += (int64_t)gridDim.x * blockDim.x)
+
+// device scalars of an iteration
+struct Scalars {
+    double volume0, q_weight, total;
+    ll eligible, pending, n_total, cdf_exact;
+    int emin;
+    int pad;
+    ll queue[1];  // followed by `chunks` queue counters
+};
+
+// ---------------------------------------------------------------------------
+// workspace layout
+// ---------------------------------------------------------------------------
+struct Layout {
+    size_t cell4, geo, dyn, chunk_fx, acc_fx, dis_fx, mom_fx, pend_fx, ero_fx, floor_key, rscale, r1, wq, cdf_fx, flag,
+        rank, comp_idx, comp_cdf,
+        draws, u_sorted, d_cell, d_fx, d_fy, d_load, d_sa, d_sb, scalars, temp, total;
+    size_t temp_bytes;
+};
+
+int layout_of(int64_t F, int64_t NE, int64_t P, Layout* L);
+
+template <class T>
+T* at(void* work, size_t off) {
+    return reinterpret_cast<T*>(reinterpret_cast<char*>(work) + off);
+}
+
+struct Ctx {
+    Layout L;
+    hipStream_t s;
+    double hscale, tscale;
+    Scalars* sc;
+    float4* cell4;
+    float2* dyn;
+    int *flag, *rank;
+    ll *pend_fx, *st;
+    void* temp;
+    int chunks;
+    int64_t N, H, NE, n;
+    ll P;
+};
+
+// argument checks and workspace views of an iteration
+int prepare(const pg_fast_fields* fl, const pg_trace_params* prm, const pg_fast_params* fp, void* work,
+            int64_t work_bytes, int64_t* stats, pg_stream stream, Ctx* c);
+// pack + spawn of `fl` (the droplets land in the workspace's d_* arrays)
+int fast_spawn(const pg_fast_fields* fl, const pg_fast_params* fp, void* work, Ctx& c);
+// fold of acc_fx into `fl`'s height / sediment, EMA of the tracks, stockpile mirror
+int finish_iteration(const pg_fast_fields* fl, const pg_fast_params* fp, void* work, Ctx& c);
+// persistent grid of a trace kernel: one block set per CU at full occupancy
+int persistent_grid(const void* kernel, int threads, int64_t requested, int* grid);
+
+// ---------------------------------------------------------------------------
+// the 2x2 stencil of the packed record {surface, discharge, momentum a, b}
+// ---------------------------------------------------------------------------
+struct Stencil {
+    float4 c00, c10, c01, c11;  // records at (i0, j0), (i0+1, j0), (i0, j0+1), (i0+1, j0+1)
+    float wi, wj;
+    int i0, j0;
+};
+
+__device__ __forceinline__ float lerp2(float v00, float v10, float v01, float v11, float wi, float wj) {
+    return v00 * (1.0f - wi) * (1.0f - wj) + v10 * wi * (1.0f - wj) + v01 * (1.0f - wi) * wj + v11 * wi * wj;
+}
+
+// stencil corner indices and weights of a position given as (cell, fraction)
+// on face f: the extended-array bilinear of particle.py:194-252, clamped
+__device__ __forceinline__ void stencil_at(int NE, int H, int ix, int iy, float fx, float fy, int& i0, int& j0, float& wi,
+                                           float& wj) {
+    i0 = ix + H - (fx < 0.5f ? 1 : 0);
+    j0 = iy + H - (fy < 0.5f ? 1 : 0);
+    wi = fx < 0.5f ? fx + 0.5f : fx - 0.5f;
+    wj = fy < 0.5f ? fy + 0.5f : fy - 0.5f;
+    if (i0 < 0) { i0 = 0; wi = 0.0f; }
+    if (i0 > NE - 2) { i0 = NE - 2; wi = 1.0f; }
+    if (j0 < 0) { j0 = 0; wj = 0.0f; }
+    if (j0 > NE - 2) { j0 = NE - 2; wj = 1.0f; }
+}
+
+__device__ __forceinline__ Stencil load_stencil(const float4* cell4, int NE, int H, int f, int ix, int iy, float fx,
+                                                float fy) {
+    Stencil s;
+    stencil_at(NE, H, ix, iy, fx, fy, s.i0, s.j0, s.wi, s.wj);
+    const int64_t b = ((int64_t)f * NE + s.i0) * NE + s.j0;
+    s.c00 = cell4[b];
+    s.c01 = cell4[b + 1];
+    s.c10 = cell4[b + NE];
+    s.c11 = cell4[b + NE + 1];
+    return s;
+}
+
+// the record of extended cell (ei, ej): always one of the stencil's corners
+// when the position lies in that cell
+__device__ __forceinline__ float4 corner_of(const Stencil& s, int ei, int ej) {
+    const bool ri = ei != s.i0, rj = ej != s.j0;
+    return ri ? (rj ? s.c11 : s.c10) : (rj ? s.c01 : s.c00);
+}
+
+__device__ __forceinline__ int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
+
+__device__ __forceinline__ ll wave_sum(ll v) {
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    return v;
+}
+
+// ---------------------------------------------------------------------------
+// one move of a droplet (particle.py:437-531): gradient of the bilinear
+// surface, inertia, gravity (and the stream push when k_mom != 0), the step
+// normalised to one cell in the metric, face transfer.  `K` supplies fr,
+// k_mom, mom_scale, dt, density, slope_gain, slope_sat, N, H, spherical.
+// Returns -1 (moved), DEATH_STOP (no finite direction) or DEATH_EXIT (left a
+// window).  h0 = the bilinear surface at the current position.
+// ---------------------------------------------------------------------------
+template <class K>
+__device__ __forceinline__ int steer_move(const K& k, const Stencil& S, const float4 g0, const float4 g1, float q,
+                                          float vol, float vol_rel, float& sa, float& sb, int f, int ix, int iy,
+                                          float fx, float fy, int& nf, int& nix, int& niy, float& nfx, float& nfy,
+                                          float& h0) {
+    const float h00 = S.c00.x, h10 = S.c10.x, h01 = S.c01.x, h11 = S.c11.x, wi = S.wi, wj = S.wj;
+    h0 = lerp2(h00, h10, h01, h11, wi, wj);
+    const float gi = (h10 - h00) * (1.0f - wj) + (h11 - h01) * wj;
+    const float gj = (h01 - h00) * (1.0f - wi) + (h11 - h10) * wi;
+    const float di = g0.w * gi + g1.x * gj;
+    const float dj = g1.x * gi + g1.y * gj;
+    const float s2 = fmaxf(gi * di + gj * dj, 0.0f);
+    const float nrm = k.slope_sat > 0.0f ? k.slope_gain * rsqrtf(s2 + k.slope_sat * k.slope_sat)
+                                        : k.slope_gain * rsqrtf(1.0f + s2);
+    const float na = -di * nrm, nb = -dj * nrm;
+    sa *= k.fr;
+    sb *= k.fr;
+    if (k.k_mom != 0.0f) {
+        const float ma = lerp2(S.c00.z, S.c10.z, S.c01.z, S.c11.z, wi, wj);
+        const float mb = lerp2(S.c00.w, S.c10.w, S.c01.w, S.c11.w, wi, wj);
+        const float mlen = sqrtf(fmaxf(g0.x * ma * ma + 2.0f * g0.y * ma * mb + g0.z * mb * mb, 0.0f));
+        const float slen = sqrtf(fmaxf(g0.x * sa * sa + 2.0f * g0.y * sa * sb + g0.z * sb * sb, 0.0f));
+        if (mlen > 1e-12f && slen > 1e-12f) {
+            const float cosang = (g0.x * ma * sa + g0.y * (ma * sb + mb * sa) + g0.z * mb * sb) / (mlen * slen);
+            const float kk = k.k_mom * cosang * k.mom_scale / (vol + k.mom_scale * q);
+            sa += kk * ma;
+            sb += kk * mb;
+        }
+    }
+    const float acc = k.dt / (vol_rel * k.density);
+    sa += acc * na;
+    sb += acc * nb;
+    const float n2 = fmaxf(g0.x * sa * sa + 2.0f * g0.y * sa * sb + g0.z * sb * sb, 0.0f);
+    // (a non-finite direction -- only from non-finite inputs -- stops the droplet
+    // rather than turning into a cell index)
+    if (!(n2 >= 1e-24f) || !(n2 <= 3.0e38f)) return DEATH_STOP;
+    const float inv = rsqrtf(n2);
+    sa *= inv;
+    sb *= inv;
+    // one cell step, across a face edge if need be
+    nfx = fx + sa;
+    nfy = fy + sb;
+    const float flx = floorf(nfx), fly = floorf(nfy);
+    nix = ix + (int)flx;
+    niy = iy + (int)fly;
+    nfx = fminf(nfx - flx, 0.99999994f);
+    nfy = fminf(nfy - fly, 0.99999994f);
+    nf = f;
+    if (k.spherical) {
+        if (nix < 0 || nix >= k.N || niy < 0 || niy >= k.N) {
+            const double Nd = (double)k.N;
+            const Transfer t = transfer_velocity<LibMath>(f, ((double)nix + nfx) / Nd, ((double)niy + nfy) / Nd,
+                                                          (double)sa, (double)sb);
+            nf = t.face;
+            double nx = t.u * Nd, ny = t.v * Nd;
+            if (nx >= Nd) nx = Nd * 0.999999999;
+            if (ny >= Nd) ny = Nd * 0.999999999;
+            if (!(nx >= 0.0)) nx = 0.0;
+            if (!(ny >= 0.0)) ny = 0.0;
+            const double fnx = floor(nx), fny = floor(ny);
+            nix = (int)fnx;
+            niy = (int)fny;
+            nfx = fminf((float)(nx - fnx), 0.99999994f);
+            nfy = fminf((float)(ny - fny), 0.99999994f);
+            sa = (float)t.a;
+            sb = (float)t.b;
+        }
+    } else if (nix < -k.H || nix >= k.N + k.H || niy < -k.H || niy >= k.N + k.H) {
+        return DEATH_EXIT;
+    }
+    return -1;
+}
+
+}  // namespace fastmode
+}  // namespace pg

@@ -121,6 +121,16 @@ __global__ void halo_scalar_kernel(double* data, int64_t K, int64_t S, const int
     }
 }
 
+__global__ void halo_scalar_adjoint_kernel(double* adj, int64_t K, int64_t S, const int64_t* dst, const int64_t* src,
+                                           const double* w) {
+    GRID_STRIDE(k, K) {
+        const double a = adj[dst[k]];
+        adj[dst[k]] = 0.0;
+        if (a != 0.0)
+            for (int64_t s = 0; s < S; s++) unsafeAtomicAdd(&adj[src[k * S + s]], w[k * S + s] * a);
+    }
+}
+
 __global__ void halo_vector_kernel(double* data, int64_t K, int64_t S, const int64_t* dst, const int64_t* src,
                                    const double* w, const double* rot) {
     GRID_STRIDE(k, K) {
@@ -283,6 +293,17 @@ extern "C" int pg_exchange_halos(double* data, int64_t ncomp, int64_t K, int64_t
     else
         hipLaunchKernelGGL(halo_vector_kernel, dim3(grid_for(K)), dim3(TPB), 0, (hipStream_t)stream, data, K, S, dst, src, w, rot);
     PG_LAUNCHED("halo_kernel");
+    return 0;
+}
+
+extern "C" int pg_halo_scalar_adjoint(double* adj, int64_t K, int64_t S, const int64_t* dst, const int64_t* src,
+                                      const double* w, pg_stream stream) {
+    clear_error();
+    PG_REQUIRE(K >= 0 && S > 0 && adj && dst && src && w, "pg_halo_scalar_adjoint: bad arguments");
+    if (K == 0) return 0;
+    hipLaunchKernelGGL(halo_scalar_adjoint_kernel, dim3(grid_for(K)), dim3(TPB), 0, (hipStream_t)stream, adj, K, S, dst,
+                       src, w);
+    PG_LAUNCHED("halo_scalar_adjoint_kernel");
     return 0;
 }
 

@@ -44,6 +44,13 @@ class FastParams(ctypes.Structure):
                 ("track_bits", i64), ("grid_blocks", i64), ("time_trace", i64), ("rng_dev", vp)]
 
 
+class GradArgs(ctypes.Structure):
+    """``pg_grad_args``: the differentiable mode's inputs of one iteration."""
+
+    _fields_ = [("real", ctypes.c_int32), ("pad", ctypes.c_int32), ("surf", vp), ("erod", vp), ("theta", dbl * 6),
+                ("tape", vp), ("tape_bytes", i64), ("fixed_moves", vp), ("moves_out", vp)]
+
+
 #: statistics slots of a fast iteration (include/pg_erode.h, PG_FS_*)
 FS = dict(PARTICLES=0, RAIN=1, ENTRIES=2, MOVES=3, DEATHS=4, CLAMPED=10, TO_PENDING_FX=11, LOST_OFFSHORE_FX=12,
           EXIT_FX=13, VANISHED_FX=14, RELEASED_FX=15, VOLUME0_BITS=16, ELIGIBLE=17, CDF_SERIAL=18, ERODED_FX=19,
@@ -75,6 +82,7 @@ SIGNATURES = [
     ("pg_apply_uplift", i32, [i64, vp, vp, vp, dbl, i32, dbl, vp]),
     ("pg_spawn_cells", i32, [vp, i64, vp, i64, vp, vp]),
     ("pg_exchange_halos", i32, [vp, i64, i64, i64, vp, vp, vp, vp, vp]),
+    ("pg_halo_scalar_adjoint", i32, [vp, i64, i64, vp, vp, vp, vp]),
     ("pg_select_work_bytes", i64, []),
     ("pg_select_kth_surface", i32, [vp, vp, i64, i64, i64, i64, vp, vp, vp]),
     ("pg_datum_shift", i32, [i64, vp, vp, vp, vp, vp]),
@@ -88,6 +96,10 @@ SIGNATURES = [
                                 vp, i64, vp, vp]),
     ("pg_fast_layout", i32, [i64, i64, i64, ctypes.POINTER(ctypes.c_int64), i32]),
     ("pg_fast_trace_time", i32, [pd, ctypes.POINTER(ctypes.c_int64)]),
+    ("pg_grad_tape_bytes", i32, [i64, i64, ctypes.c_int32, ctypes.POINTER(ctypes.c_int64)]),
+    ("pg_grad_iterate", i32, [ctypes.POINTER(FastFields), ctypes.POINTER(FastFields), ctypes.POINTER(TraceParams),
+                              ctypes.POINTER(FastParams), ctypes.POINTER(GradArgs), vp, i64, vp, vp]),
+    ("pg_grad_reverse", i32, [i64, i64, i64, ctypes.POINTER(TraceParams), ctypes.POINTER(GradArgs), vp, vp, vp, vp, vp]),
     ("pg_pcg64_random", i32, [ctypes.c_uint64, ctypes.c_uint64, ctypes.c_uint64, ctypes.c_uint64, i64, i64, vp, vp]),
 ]
 

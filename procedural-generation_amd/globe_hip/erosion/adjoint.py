@@ -1,0 +1,356 @@
+"""Differentiable erosion: gradients of a terrain misfit with respect to a
+per-cell erodibility field and six scalar erosion parameters.
+
+The reference has no adjoint; this module is the differentiable counterpart
+of its erosion loop (bake/globe/erosion/maps.py:1516-1617, plain profile).
+An iteration in the differentiable mode (``pg_grad_iterate``) is the fast
+mode with one frozen-terrain chunk and only the trace-time caps: droplets
+steer on the packed record of the state they start from, exchange mass with
+float32 or float64 arithmetic, and everything they take or lay down is
+summed in int64 fixed point; then uplift, the datum hold and the halo
+exchange follow as in ``maps.step``.
+
+Trajectories are frozen: the cells a droplet visits, where it stops and
+why, the discharge it reads and the datum's order statistic are held at
+the nominal run's; the surface, the erodibility factor and the six scalars
+act through the exchange.  ``GradRun.replay`` evaluates exactly that
+frozen map at other parameters (finite differences use it), and
+``GradRun.backward`` is its adjoint: per iteration, in reverse, the halo
+exchange's transpose, the datum's, and the hand-written reverse kernel
+(``pg_grad_reverse``) over the iteration's tape.  Iterations are
+checkpointed every ``checkpoint_every``; a segment's tapes are recomputed
+from its checkpoint when the backward pass reaches it (the last segment's
+are kept from the forward pass).
+
+The six scalars, in kernel units (``THETA_NAMES``): erodibility,
+deposition_rate, evap_rate, k_disc, ocean_rate, dep_floor (cell units).
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+import torch
+
+from .._lib import FAST_NSTATS, FastFields, FastParams, GradArgs, check, lib
+from . import particle as pk
+
+THETA_NAMES = ("erodibility", "deposition_rate", "evap_rate", "k_disc", "ocean_rate", "dep_floor")
+#: fixed point of the surface change (units of 2^-bits cells): float64 runs
+#: resolve 16x finer than float32 ones (finite differences of the frozen map
+#: see the quantisation as noise); range +-2^(63-bits) cells per iteration
+HEIGHT_BITS = {"f32": 48, "f64": 52}
+TRACK_BITS = 32
+
+_SNAP = ("height", "sediment", "discharge", "momentum", "pending", "base")
+
+
+def _snapshot(src) -> dict:
+    if isinstance(src, dict):
+        return {n: src[n].clone() for n in _SNAP}
+    return {n: getattr(src, n).clone() for n in _SNAP}
+
+
+class GradRun:
+    """One differentiable erosion run of ``iterations`` iterations from the
+    current contents of ``state`` (a fast-mode ErosionState, which it uses
+    for the static fields and as scratch).
+
+    real: "f32" or "f64" exchange arithmetic.  replay_of: another GradRun
+    run with ``record=True`` -- this run then follows its trajectories
+    (steering records, move counts, datum cells) instead of its own."""
+
+    def __init__(self, state, params, iterations: int, real: str = "f32", checkpoint_every: int = 10,
+                 first_key: int = 0, rng_stage: str = "erosion", n_particles=None, replay_of: "GradRun | None" = None):
+        from .maps import _check_scope, _eparams, discharge_scales, max_steps_of, trace_params, uplift_cap
+
+        if state.mode != "fast":
+            raise ValueError("the differentiable mode runs on a fast-mode state")
+        if real not in ("f32", "f64"):
+            raise ValueError("real must be 'f32' or 'f64'")
+        ep = _eparams(params)
+        _check_scope(ep, state)
+        if float(getattr(ep, "lateral_rate", 0.0)) > 0.0 or state.roots is not None:
+            raise NotImplementedError("lateral cuts and roots are outside the differentiable mode")
+        if float(ep.disc_saturation_cells) > 0.0 or float(ep.momentum_saturation_cells) > 0.0:
+            raise NotImplementedError("cell-count discharge scales are outside the differentiable mode")
+        if float(state.pending.abs().max().item()) != 0.0:
+            raise ValueError("the differentiable mode starts without stockpiles (state.pending must be 0)")
+        self.state, self.params, self.ep = state, params, ep
+        self.T = int(iterations)
+        self.ce = max(1, int(checkpoint_every))
+        self.real = real
+        self.dtype = torch.float32 if real == "f32" else torch.float64
+        self.keys = [int(first_key) + t for t in range(self.T)]
+        self.rng_stage = rng_stage
+        self.replay_of = replay_of
+        dev = state.device
+        n_cells = int(np.count_nonzero(state._mask_host[state.interior] == pk.MASK_ACTIVE))
+        if n_particles is None:
+            n_particles = max(1, int(round(ep.particles_per_cell * n_cells)))
+        self.P = int(n_particles)
+        self.max_steps = max_steps_of(ep, state.N)
+        disc_sat, mom_scale = discharge_scales(ep, state, 1.0, self.P, n_rain=1)
+        self.prm = trace_params(ep, state, 1.0, 1, 1, disc_sat, mom_scale, self.max_steps, 0.0)
+        self.cap = uplift_cap(state, ep)
+        self.uplift_mean = state.uplift_mean(self.cap)
+        self.datum = bool(state.spherical and hasattr(params, "world"))
+        if self.datum:
+            from .maps import held_land_fraction
+
+            M = state.F * state.N * state.N
+            n_land = min(max(int(round(float(held_land_fraction(state, params)) * M)), 0), M)
+            if n_land in (0, M):
+                raise ValueError("the differentiable datum hold needs 0 < land fraction < 1")
+            self.kth = M - n_land
+            inter = torch.zeros(state.height.shape, dtype=torch.bool, device=dev)
+            inter[state.interior] = True
+            self.interior_flat = inter.reshape(-1)
+            state.select_work()
+        if state.spherical:
+            self.halo = state.halo_tables()
+        nbytes = int(lib().pg_fast_workspace_bytes(state.F, state.NE, self.P))
+        if nbytes < 0:
+            check(-1, "pg_fast_workspace_bytes")
+        self.work = torch.empty(nbytes, dtype=torch.uint8, device=dev)
+        self.stats = torch.zeros(FAST_NSTATS, dtype=torch.int64, device=dev)
+        self.height_bits = HEIGHT_BITS[real]
+        fp = FastParams(n_particles=self.P, chunks=1, height_bits=self.height_bits, track_bits=TRACK_BITS)
+        check(lib().pg_fast_setup(ctypes.byref(self._fields(state)), ctypes.byref(fp), ctypes.c_void_p(self.work.data_ptr()),
+                                  nbytes, pk._stream()), "pg_fast_setup")
+        tb = ctypes.c_int64()
+        check(lib().pg_grad_tape_bytes(self.P, self.max_steps, 1 if real == "f64" else 0, ctypes.byref(tb)),
+              "pg_grad_tape_bytes")
+        self.tape_bytes = int(tb.value)
+        self._tapes: dict[int, torch.Tensor] = {}
+        self._taped: dict[int, int] = {}  # slot -> iteration whose tape it holds
+        self.s0 = _snapshot(state)
+        self.nominal = None
+        self.moves = None
+        self.idx: list = [None] * self.T
+        self.checkpoints: dict[int, dict] = {}
+
+    # -- plumbing ------------------------------------------------------------
+    def _fields(self, s) -> FastFields:
+        st = self.state
+        get = (lambda n: s[n]) if isinstance(s, dict) else (lambda n: getattr(s, n))
+        return FastFields(F=st.F, N=st.N, H=st.H, spherical=int(bool(st.spherical)),
+                          deposit_on_exit=int(bool(st.deposit_on_exit)), height=get("height").data_ptr(),
+                          sediment=get("sediment").data_ptr(), discharge=get("discharge").data_ptr(),
+                          momentum=get("momentum").data_ptr(), pending=get("pending").data_ptr(),
+                          base=get("base").data_ptr(), precip=st.precip.data_ptr(), evap=st.evap.data_ptr(),
+                          hardness=st.hardness.data_ptr(), metric=st.metric.data_ptr(),
+                          metric_inv=st.metric_inv.data_ptr(), mask=st.mask.data_ptr())
+
+    def _fparams(self, t: int) -> FastParams:
+        from .fast import pcg64_state
+
+        ep = self.ep
+        slo, shi, ilo, ihi = pcg64_state(self.params.rng(self.rng_stage, self.keys[t]))
+        return FastParams(rng_state_lo=slo, rng_state_hi=shi, rng_inc_lo=ilo, rng_inc_hi=ihi, n_particles=self.P,
+                          chunks=1, conc_chunk=0.0, fan_slope=float(ep.fan_slope),
+                          offshore_writeoff=float(ep.offshore_writeoff), ema=float(ep.ema),
+                          min_volume_frac=float(ep.min_volume_frac), height_bits=self.height_bits,
+                          track_bits=TRACK_BITS,
+                          grid_blocks=int(getattr(self.state, "fast_grid_blocks", 0)))
+
+    def _tape(self, slot: int) -> torch.Tensor:
+        if slot not in self._tapes:
+            self._tapes[slot] = torch.empty(self.tape_bytes, dtype=torch.uint8, device=self.state.device)
+        return self._tapes[slot]
+
+    def _args(self, surf, erod, theta, tape=None, fixed=None, moves_out=None) -> GradArgs:
+        th = [float(v) for v in (theta.tolist() if torch.is_tensor(theta) else theta)]
+        if len(th) != 6:
+            raise ValueError("theta holds 6 values: " + ", ".join(THETA_NAMES))
+        if erod is not None:
+            pk._need(erod, torch.float32, tuple(self.state.height.shape), "erodibility field")
+        return GradArgs(real=1 if self.real == "f64" else 0, surf=surf.data_ptr(),
+                        erod=None if erod is None else erod.data_ptr(), theta=(ctypes.c_double * 6)(*th),
+                        tape=None if tape is None else tape.data_ptr(), tape_bytes=0 if tape is None else tape.numel(),
+                        fixed_moves=None if fixed is None else fixed.data_ptr(),
+                        moves_out=None if moves_out is None else moves_out.data_ptr())
+
+    def _steer(self, t: int, target: dict):
+        """(steering state, fixed moves, datum cell) of iteration t."""
+        if self.replay_of is None:
+            return target, None, self.idx[t]
+        r = self.replay_of
+        return r.nominal[t], r.moves[t], r.idx[t]
+
+    def _iterate(self, target: dict, t: int, erod, theta, tape=None, moves_out=None, steer=None, fixed=None,
+                 idx=None):
+        st = self.state
+        steer = target if steer is None else steer
+        surf = (target["height"] + target["sediment"]).to(self.dtype)
+        ga = self._args(surf, erod, theta, tape, fixed, moves_out)
+        check(lib().pg_grad_iterate(ctypes.byref(self._fields(steer)), ctypes.byref(self._fields(target)),
+                                    ctypes.byref(self.prm), ctypes.byref(self._fparams(t)), ctypes.byref(ga),
+                                    ctypes.c_void_p(self.work.data_ptr()), self.work.numel(),
+                                    ctypes.c_void_p(self.stats.data_ptr()), pk._stream()), "pg_grad_iterate")
+        pk.apply_uplift(target["height"], st.uplift, st.mask, self.cap, self.uplift_mean)
+        if self.datum:
+            x = (target["height"] + target["sediment"]).reshape(-1)
+            if idx is None:
+                q = torch.zeros(1, dtype=torch.float64, device=st.device)
+                check(lib().pg_select_kth_surface(pk._ptr(target["height"]), pk._ptr(target["sediment"]), st.F, st.N,
+                                                  st.H, self.kth, pk._ptr(q), pk._ptr(st.select_work()), pk._stream()),
+                      "hold_datum")
+                idx = torch.argmax(((x == q) & self.interior_flat).to(torch.uint8)).reshape(1)
+            q = x.index_select(0, idx).contiguous()
+            check(lib().pg_datum_shift(target["height"].numel(), pk._ptr(target["height"]), None,
+                                       pk._ptr(target["base"]), pk._ptr(q), pk._stream()), "datum_shift")
+        if st.spherical:
+            for n in ("height", "sediment", "discharge"):
+                self.halo.exchange(target[n])
+            self.halo.exchange(target["momentum"], vector=True)
+        return idx
+
+    # -- forward / replay / backward -----------------------------------------
+    def forward(self, erod, theta, record: bool = False, tapes: str = "last") -> dict:
+        """Run all iterations; returns the final state's tensors.  Keeps
+        checkpoints and the datum cells; with record=True also every
+        iteration's starting state and move counts (what a replay follows);
+        tapes: "last" (the last segment's, for the backward pass), "all" or
+        "none"."""
+        T, ce = self.T, self.ce
+        target = _snapshot(self.s0)
+        self.checkpoints = {}
+        self._taped = {}
+        if record:
+            self.nominal, self.moves = [], []
+        last = ((T - 1) // ce) * ce
+        for t in range(T):
+            if t % ce == 0:
+                self.checkpoints[t] = _snapshot(target)
+            mv = None
+            if record:
+                self.nominal.append(_snapshot(target))
+                mv = torch.zeros(self.P, dtype=torch.int32, device=self.state.device)
+                self.moves.append(mv)
+            tape = None
+            if tapes == "all" or (tapes == "last" and t >= last):
+                slot = t if tapes == "all" else t - last
+                tape = self._tape(slot)
+                self._taped[slot] = t
+            steer, fixed, idx = self._steer(t, target)
+            if self.replay_of is None:
+                idx = None  # a nominal run takes its own order statistic
+            self.idx[t] = self._iterate(target, t, erod, theta, tape=tape, moves_out=mv, steer=steer, fixed=fixed,
+                                        idx=idx)
+        self._tapes_all = tapes == "all"
+        return target
+
+    def replay(self, erod, theta) -> dict:
+        """The frozen-trajectory map at (erod, theta): this run's recorded
+        trajectories (or replay_of's) with the exchange re-evaluated."""
+        src = self.replay_of if self.replay_of is not None else self
+        if src.nominal is None:
+            raise ValueError("replay needs a forward pass with record=True")
+        target = _snapshot(self.s0)
+        for t in range(self.T):
+            self._iterate(target, t, erod, theta, steer=src.nominal[t], fixed=src.moves[t], idx=src.idx[t])
+        return target
+
+    def backward(self, g_surface: torch.Tensor, erod, theta):
+        """Adjoint of forward(): g_surface = dL/d(final surface) (F, NE, NE)
+        float64.  Returns (dL/d erod (float64, extended cells), dL/d theta
+        (float64[6]), dL/d initial surface)."""
+        st = self.state
+        T, ce = self.T, self.ce
+        lam = g_surface.to(torch.float64).contiguous().clone()
+        g_erod = torch.zeros(st.height.shape, dtype=torch.float64, device=st.device)
+        g_theta = torch.zeros(6, dtype=torch.float64, device=st.device)
+        segs = [(a, min(a + ce, T)) for a in range(0, T, ce)]
+        for a, b in reversed(segs):
+            slot_of = (lambda t: t) if self._tapes_all else (lambda t: t - a)
+            if not all(self._taped.get(slot_of(t)) == t for t in range(a, b)):
+                target = _snapshot(self.checkpoints[a])
+                for t in range(a, b):
+                    steer, fixed, _ = self._steer(t, target)
+                    self._iterate(target, t, erod, theta, tape=self._tape(slot_of(t)), steer=steer, fixed=fixed,
+                                  idx=self.idx[t])
+                    self._taped[slot_of(t)] = t
+            for t in range(b - 1, a - 1, -1):
+                lam = self._reverse(lam, self._tape(slot_of(t)), self.idx[t], erod, theta, g_erod, g_theta)
+        return g_erod, g_theta, lam
+
+    def _reverse(self, lam, tape, idx, erod, theta, g_erod, g_theta):
+        st = self.state
+        if st.spherical:
+            ht = self.halo
+            check(lib().pg_halo_scalar_adjoint(pk._ptr(lam), ht.K, ht.S, pk._ptr(ht.dst), pk._ptr(ht.src), pk._ptr(ht.w),
+                                               pk._stream()), "halo_scalar_adjoint")
+        if self.datum:
+            total = lam.sum().reshape(1)
+            lam.view(-1).index_add_(0, idx, -total)
+        out = lam.clone()
+        surf = st.height  # unused by the reverse sweep; any device pointer of the right size
+        ga = self._args(surf, erod, theta, tape=tape)
+        check(lib().pg_grad_reverse(st.F, st.N, st.H, ctypes.byref(self.prm), ctypes.byref(ga), pk._ptr(lam),
+                                    pk._ptr(out), pk._ptr(g_erod), pk._ptr(g_theta), pk._stream()), "pg_grad_reverse")
+        return out
+
+
+def theta_nominal(run: GradRun) -> torch.Tensor:
+    """The six scalars of the run's parameters, kernel units (CPU float64)."""
+    p = run.prm
+    return torch.tensor([p.erodibility, p.deposition_rate, p.evap_rate, p.k_disc, p.ocean_rate, p.dep_floor],
+                        dtype=torch.float64)
+
+
+class ErosionFunction(torch.autograd.Function):
+    """surface_T = erosion(erod, theta) with the hand-written adjoint.
+
+    erod: (F, NE, NE) float32 on the run's device; theta: float64[6] on the
+    CPU.  Returns the final surface (height + sediment, cell units, float64,
+    extended cells)."""
+
+    @staticmethod
+    def forward(ctx, erod, theta, run: GradRun, tapes: str = "last"):
+        out = run.forward(erod.detach().contiguous(), theta.detach(), tapes=tapes)
+        ctx.run = run
+        ctx.save_for_backward(erod, theta)
+        return out["height"] + out["sediment"]
+
+    @staticmethod
+    def backward(ctx, g):
+        erod, theta = ctx.saved_tensors
+        g_erod, g_theta, _ = ctx.run.backward(g, erod.detach().contiguous(), theta.detach())
+        return g_erod.to(erod.dtype), g_theta.to(theta.device), None, None
+
+
+def erosion(erod, theta, run: GradRun, tapes: str = "last"):
+    return ErosionFunction.apply(erod, theta, run, tapes)
+
+
+def beta25_cdf(x: torch.Tensor) -> torch.Tensor:
+    """CDF of Beta(2, 5) on [0, 1]: 1 - (1-x)^6 - 6 x (1-x)^5."""
+    x = x.clamp(0.0, 1.0)
+    return 1.0 - (1.0 - x) ** 6 - 6.0 * x * (1.0 - x) ** 5
+
+
+def land_hypsometry_loss(h_m: torch.Tensor, land: torch.Tensor, bins: int = 64, zmax_m: float = 4000.0,
+                         mean_m: float = 800.0, weight: float = 0.1, tau_m: float | None = None):
+    """L2 misfit of the land hypsometric CDF (``bins`` levels z_k = k zmax /
+    bins, k = 1..bins) to Beta(2, 5) on [0, zmax] + weight * (mean land
+    elevation - mean_m)^2; h_m in metres, land a boolean mask.  The CDF is
+    the sigmoid-smoothed fraction of land cells at or below each level
+    (width tau_m, default half a bin)."""
+    hl = h_m[land]
+    z = torch.arange(1, bins + 1, dtype=h_m.dtype, device=h_m.device) * (zmax_m / bins)
+    tau = tau_m if tau_m is not None else 0.5 * zmax_m / bins
+    cdf = torch.sigmoid((z[:, None] - hl[None, :]) / tau).mean(dim=1)
+    target = beta25_cdf(z / zmax_m)
+    return ((cdf - target) ** 2).sum() + weight * (hl.mean() - mean_m) ** 2
+
+
+def hypsometry_loss(surface: torch.Tensor, run: GradRun, land=None, **kw):
+    """land_hypsometry_loss of a surface (cell units, extended cells) over
+    the interior; the land set (surface > 0) is taken from the surface given
+    unless passed, and is not differentiated."""
+    st = run.state
+    h = surface[st.interior].reshape(-1) * st.height_unit_m
+    if land is None:
+        land = (h > 0).detach()
+    return land_hypsometry_loss(h, land, **kw)

@@ -1,0 +1,163 @@
+"""The differentiable mode on the GPU: frozen-trajectory replay, the halo
+exchange's transpose, and the hand-written adjoint against central finite
+differences of the same frozen map and against the float64 adjoint.
+
+Bars: replay at the nominal parameters reproduces the nominal run bit for
+bit; <H^T a, x> = <a, H x> to 1e-12 relative; adjoint vs central
+differences of the frozen map (float64, relative step 1e-5; 1e-3 and 1e-6
+are printed beside it) within 2e-4 relative on every checked component;
+float32 adjoint vs float64 adjoint on the same trajectories within 1e-2.
+"""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+import torch
+
+from globe_hip import stubs
+from globe_hip.erosion import adjoint as adj
+from globe_hip.erosion import maps
+from globe_hip.erosion import particle as pk
+
+pytestmark = pytest.mark.gpu
+
+
+def _state(N, cuda, n_blobs=8, **erosion):
+    grid, f = stubs.earth_globe(N, 2, n_blobs=n_blobs)
+    p = stubs.workload_params(N, 2, **erosion)
+    st = maps.ErosionState.from_grid(grid, f["bedrock_m"], f["hardness"], f["precip"], f["evap"], f["uplift_m"],
+                                     p.erosion, device=cuda, mode="fast")
+    return st, p
+
+
+def _erod(st, seed=0):
+    g = torch.Generator().manual_seed(seed)
+    e = 1.0 + 0.2 * (2.0 * torch.rand(tuple(st.height.shape), generator=g, dtype=torch.float64) - 1.0)
+    return e.to(torch.float32).to(st.device)
+
+
+def test_replay_at_nominal_is_the_nominal_run(cuda):
+    st, p = _state(32, cuda)
+    run = adj.GradRun(st, p, iterations=4, real="f64", checkpoint_every=2)
+    erod, theta = _erod(st), adj.theta_nominal(run)
+    print("grad mode N=32 T=4 f64: nominal then replay", flush=True)
+    a = run.forward(erod, theta, record=True)
+    b = run.replay(erod, theta)
+    for n in ("height", "sediment"):
+        assert torch.equal(a[n], b[n]), n
+    # something happened
+    assert float((a["height"] + a["sediment"] - run.s0["height"] - run.s0["sediment"]).abs().max()) > 0
+
+
+def test_halo_adjoint_is_the_transpose(cuda):
+    st, p = _state(32, cuda)
+    ht = st.halo_tables()
+    g = torch.Generator(device="cpu").manual_seed(3)
+    x = torch.randn(tuple(st.height.shape), generator=g, dtype=torch.float64).to(cuda)
+    a = torch.randn(tuple(st.height.shape), generator=g, dtype=torch.float64).to(cuda)
+    print("halo transpose N=32", flush=True)
+    hx = x.clone()
+    ht.exchange(hx)  # halo cells <- H x_interior
+    hta = a.clone()
+    from globe_hip._lib import check, lib
+    check(lib().pg_halo_scalar_adjoint(pk._ptr(hta), ht.K, ht.S, pk._ptr(ht.dst), pk._ptr(ht.src), pk._ptr(ht.w),
+                                       pk._stream()), "halo adjoint")
+    # the exchange as a linear map y = E x (interior kept, halo replaced);
+    # its transpose applied to a must satisfy <E^T a, x> = <a, E x>
+    lhs = float((hta * x).sum())
+    rhs = float((a * hx).sum())
+    assert lhs == pytest.approx(rhs, rel=1e-12, abs=1e-12)
+    # destinations are never sources
+    assert not np.isin(ht.dst.cpu().numpy(), ht.src.cpu().numpy()).any()
+
+
+def _fd_check(run, erod, theta, loss_of, cells, h_rel=1e-3):
+    """Central differences of the frozen map, relative step h_rel (the
+    erodibility field is float32: its steps are the ones float32 took)."""
+    out = run.replay(erod, theta)
+    base = loss_of(out["height"] + out["sediment"])
+    fd_e, fd_t = [], []
+    for c in cells:
+        e1, e2 = erod.clone(), erod.clone()
+        h = h_rel * float(erod.view(-1)[c])
+        e1.view(-1)[c] += h
+        e2.view(-1)[c] -= h
+        h = float(e1.view(-1)[c].double() - e2.view(-1)[c].double()) / 2.0
+        lp = loss_of(sum(run.replay(e1, theta)[n] for n in ("height", "sediment")))
+        lm = loss_of(sum(run.replay(e2, theta)[n] for n in ("height", "sediment")))
+        fd_e.append((lp - lm) / (2 * h))
+    for i in range(6):
+        t1, t2 = theta.clone(), theta.clone()
+        h = h_rel * (abs(float(theta[i])) if float(theta[i]) != 0.0 else 1.0)
+        t1[i] += h
+        t2[i] -= h
+        lp = loss_of(sum(run.replay(erod, t1)[n] for n in ("height", "sediment")))
+        lm = loss_of(sum(run.replay(erod, t2)[n] for n in ("height", "sediment")))
+        fd_t.append((lp - lm) / (2 * h))
+    return base, np.array(fd_e), np.array(fd_t)
+
+
+@pytest.mark.parametrize("ce", [2, 6])
+def test_adjoint_matches_central_differences(cuda, ce):
+    T = 6
+    st, p = _state(32, cuda)
+    run = adj.GradRun(st, p, iterations=T, real="f64", checkpoint_every=ce)
+    erod = _erod(st).requires_grad_(True)
+    theta = adj.theta_nominal(run).requires_grad_(True)
+    print(f"adjoint vs FD: N=32 T={T} checkpoint every {ce} f64", flush=True)
+    surf = adj.erosion(erod, theta, run)
+    land = (surf[st.interior].reshape(-1) > 0).detach()
+    loss = adj.hypsometry_loss(surf, run, land=land)
+    loss.backward()
+    ge = erod.grad.detach().cpu().numpy().reshape(-1)
+    gt = theta.grad.detach().numpy()
+    # the frozen map: record the nominal trajectories, replay them
+    run.forward(erod.detach(), theta.detach(), record=True, tapes="none")
+    loss_of = lambda s: float(adj.hypsometry_loss(s, run, land=land))
+    inter = np.zeros(st.height.shape, bool)
+    inter[st.interior] = True
+    cand = np.flatnonzero(inter.reshape(-1) & (np.abs(ge) > 0))
+    rng = np.random.default_rng(0)
+    big = cand[np.argsort(-np.abs(ge[cand]))[:8]]
+    cells = np.concatenate([big, rng.choice(cand, 8, replace=False)])
+    ad_e = ge[cells]
+    print("erod  adjoint", ad_e, flush=True)
+    print("theta adjoint", gt, flush=True)
+    # the frozen map is piecewise smooth (caps, the c_eq > sed switch): a
+    # step of 1e-3 crosses the kinks of the droplets nearest a switch, so
+    # the bar is on the converged step; 1e-3 is reported beside it
+    res = {}
+    for h_rel in (1e-3, 1e-5, 1e-6):
+        base, fd_e, fd_t = _fd_check(run, erod.detach(), theta.detach(), loss_of, cells, h_rel=h_rel)
+        assert base == pytest.approx(float(loss), rel=1e-12)
+        res[h_rel] = (fd_e, fd_t)
+        print(f"h_rel {h_rel:g}: erod fd", fd_e, "\n  theta fd", fd_t, flush=True)
+    fd_e, fd_t = res[1e-5]
+    scale_e = np.abs(ad_e).max()
+    assert np.all(np.abs(ad_e - fd_e) <= 2e-4 * np.abs(ad_e) + 1e-5 * scale_e), (ad_e, fd_e)
+    for i in range(6):
+        assert abs(gt[i] - fd_t[i]) <= 2e-4 * max(abs(gt[i]), abs(fd_t[i])) + 1e-9, (adj.THETA_NAMES[i], gt[i], fd_t[i])
+
+
+def test_float32_adjoint_tracks_float64(cuda):
+    T = 6
+    st, p = _state(48, cuda)
+    run32 = adj.GradRun(st, p, iterations=T, real="f32", checkpoint_every=3)
+    erod = _erod(st)
+    theta = adj.theta_nominal(run32)
+    print(f"f32 vs f64 adjoint: N=48 T={T}", flush=True)
+    run32.forward(erod, theta, record=True, tapes="none")
+    e32, t32 = erod.clone().requires_grad_(True), theta.clone().requires_grad_(True)
+    s32 = adj.erosion(e32, t32, run32)
+    land = (s32[st.interior].reshape(-1) > 0).detach()
+    adj.hypsometry_loss(s32, run32, land=land).backward()
+    run64 = adj.GradRun(st, p, iterations=T, real="f64", checkpoint_every=3, replay_of=run32)
+    e64, t64 = erod.clone().requires_grad_(True), theta.clone().requires_grad_(True)
+    s64 = adj.erosion(e64, t64, run64)
+    adj.hypsometry_loss(s64, run64, land=land).backward()
+    a, b = e32.grad.double(), e64.grad.double()
+    rel = float((a - b).norm() / b.norm())
+    print("erod grad rel diff", rel, "theta", t32.grad.numpy(), t64.grad.numpy(), flush=True)
+    assert rel < 1e-2
+    tg32, tg64 = t32.grad.numpy(), t64.grad.numpy()
+    assert np.all(np.abs(tg32 - tg64) <= 1e-2 * np.abs(tg64).max())
