@@ -125,11 +125,12 @@ def cpu_baseline(grid, f, p, fraction=8):
                       f"reference chunking ({s['chunk']}) and serial apply; {moves[0]} moves in {dt:.2f} s"}
 
 
-def adjoint_run(N=256, iterations=50, checkpoint_every=10, max_steps=128, seed=0):
+def adjoint_run(N=256, iterations=50, checkpoint_every=10, max_steps=128, seed=0, feedback=False):
     """Wall time of the differentiable mode's forward pass alone and of
     forward + hand-written adjoint (gradient of the hypsometry misfit with
     respect to the erodibility field and the six scalars), float32, on the
-    N_c=256 earth globe, 50 iterations, checkpoints every 10."""
+    N_c=256 earth globe, 50 iterations, checkpoints every 10; frozen-terrain
+    gradient unless feedback."""
     import torch
 
     from globe_hip import stubs
@@ -141,7 +142,7 @@ def adjoint_run(N=256, iterations=50, checkpoint_every=10, max_steps=128, seed=0
     p = stubs.workload_params(N, 2, max_steps=max_steps, iterations=iterations)
     st = maps.ErosionState.from_grid(grid, f["bedrock_m"], f["hardness"], f["precip"], f["evap"], f["uplift_m"],
                                      p.erosion, device="cuda", mode="fast")
-    run = adj.GradRun(st, p, iterations, real="f32", checkpoint_every=checkpoint_every)
+    run = adj.GradRun(st, p, iterations, real="f32", checkpoint_every=checkpoint_every, feedback=feedback)
     g = torch.Generator().manual_seed(seed)
     erod = (1.0 + 0.2 * (2.0 * torch.rand(tuple(st.height.shape), generator=g, dtype=torch.float64) - 1.0)).float().cuda()
     theta = adj.theta_nominal(run)
@@ -155,8 +156,9 @@ def adjoint_run(N=256, iterations=50, checkpoint_every=10, max_steps=128, seed=0
         e, th = erod.clone().requires_grad_(True), theta.clone().requires_grad_(True)
         loss = adj.hypsometry_loss(adj.erosion(e, th, run), run)
         loss.backward()
-        return float(loss), e.grad, th.grad
+        return float(loss.detach()), e.grad, th.grad
 
+    fwd()
     fwd_adj()  # warm-up: tapes and checkpoints allocated
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -167,12 +169,70 @@ def adjoint_run(N=256, iterations=50, checkpoint_every=10, max_steps=128, seed=0
     loss_a, ge, gt = fwd_adj()
     torch.cuda.synchronize()
     t_fa = time.perf_counter() - t0
-    log(f"adjoint: forward {t_fwd:.3f}s, forward+adjoint {t_fa:.3f}s, ratio {t_fa / t_fwd:.2f}; loss {loss_f:.6g}")
+    log(f"adjoint (feedback {feedback}): forward {t_fwd:.3f}s, forward+adjoint {t_fa:.3f}s, ratio {t_fa / t_fwd:.2f}; "
+        f"loss {loss_f:.6g}")
     return {"workload": f"earth globe N_c={N} H=2, {iterations} iterations, lifetime {max_steps}, float32, "
                         f"checkpoint every {checkpoint_every}; loss = 64-bin land hypsometric CDF vs Beta(2,5) on "
-                        f"[0, 4000] m + 0.1 (mean land - 800 m)^2; erodibility field 1 +- 0.2",
+                        f"[0, 4000] m + 0.1 (mean land - 800 m)^2; erodibility field 1 +- 0.2; "
+                        f"{'surface feedback' if feedback else 'frozen terrain'} gradient",
             "forward_s": t_fwd, "forward_adjoint_s": t_fa, "ratio": t_fa / t_fwd, "loss": loss_a,
             "grad_erod_norm": float(ge.double().norm()), "grad_theta": [float(v) for v in gt]}
+
+
+def batch_run(n=64, iterations=50, seed=0, per_window_iterations=3):
+    """Batched zoom windows: n windows (side log-uniform in [128, 1024],
+    own fBm seed, lifetime log-uniform in [32, 256], one droplet per cell),
+    the batched iteration autotuned then graph-captured and replayed for
+    `iterations` iterations, against the same windows run one fast-mode pass
+    per window."""
+    import torch
+
+    from globe_hip import stubs
+    from globe_hip.erosion import batch as bt
+    from globe_hip.erosion import fast
+
+    t = time.time()
+    wins = stubs.zoom_windows(n, seed=seed)
+    b = bt.WindowBatch([w for w, _ in wins], [p for _, p in wins], device="cuda")
+    cells = int(sum((x - 4) ** 2 for x in b.NE))
+    log(f"batch inputs: {n} windows, {cells} interior cells, {time.time() - t:.1f}s")
+    tuner = bt.Autotuner(b)
+    it = 0
+    while tuner.best is None:
+        tuner.step(b.step, it)
+        it += 1
+    log(f"autotuned persistent grid: {tuner.best} blocks (trace ms per candidate {tuner.times})")
+    g = bt.BatchGraph(b)
+    torch.cuda.synchronize()
+    stats = []
+    t0 = time.perf_counter()
+    for k in range(iterations):
+        g.step(it + k)
+        stats.append(b.stats.clone())
+    torch.cuda.synchronize()
+    t_batch = time.perf_counter() - t0
+    moves = sum(int(s[fast.FS["MOVES"]]) for s in stats)
+    b2 = bt.WindowBatch([w for w, _ in wins], [p for _, p in wins], device="cuda")
+    b2.grid_blocks = tuner.best
+    b2.step_per_window(0)  # warm-up: per-window workspaces
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    pw_moves = 0
+    for k in range(per_window_iterations):
+        b2.step_per_window(1 + k)
+        torch.cuda.synchronize()
+        pw_moves += b2.moves(per_window=True)
+    t_pw = time.perf_counter() - t0
+    per_it_b, per_it_pw = t_batch / iterations, t_pw / per_window_iterations
+    log(f"batched: {iterations} iterations in {t_batch:.2f}s ({1000 * per_it_b:.1f} ms/iteration, "
+        f"{moves / t_batch:.3g} moves/s); per window: {1000 * per_it_pw:.1f} ms/iteration "
+        f"({pw_moves / t_pw:.3g} moves/s); speed-up {per_it_pw / per_it_b:.2f}")
+    return {"workload": f"{n} zoom windows, side log-uniform [128, 1024] ({cells} interior cells), halo 2, own fBm "
+                        f"seed each, lifetime log-uniform [32, 256], 1 droplet/cell, fast mode 16 chunks",
+            "iterations": iterations, "batched_s": t_batch, "batched_ms_per_iteration": 1000 * per_it_b,
+            "batched_moves_per_s": moves / t_batch, "per_window_ms_per_iteration": 1000 * per_it_pw,
+            "per_window_moves_per_s": pw_moves / t_pw, "speedup": per_it_pw / per_it_b,
+            "grid_blocks": tuner.best, "autotune_trace_ms": tuner.times}
 
 
 def copy_probe(mib):
@@ -199,6 +259,7 @@ def main():
     ap.add_argument("--max-steps", type=int, default=128)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="only run 3 device-to-device copies of this many MiB (HBM counter calibration) and exit")
     a = ap.parse_args()
@@ -244,6 +305,13 @@ def main():
         torch.cuda.empty_cache()
         out["adjoint"] = adjoint_run()
         out["fwd_adjoint_ratio"] = out["adjoint"]["ratio"]
+        fb = adjoint_run(feedback=True)
+        out["adjoint"]["ratio_with_surface_feedback"] = fb["ratio"]
+    if not a.no_batch:
+        import torch
+
+        torch.cuda.empty_cache()
+        out["batched_windows"] = batch_run()
     if not a.no_cpu:
         out["cpu_baseline"] = cpu_baseline(grid, f, p)
     print(json.dumps(out), flush=True)

@@ -243,6 +243,25 @@ int pg_fast_layout(int64_t F, int64_t NE, int64_t n_particles, int64_t* offsets,
  * launch stream) and count.  Library-wide state: not thread-safe. */
 int pg_fast_trace_time(double* ms_total, int64_t* launches);
 
+/* ---- batched windows (no counterpart in the reference, which erodes its
+ * zoom tiles one process each: bake/globe/zoom/bake.py:777-923) ----------
+ * n_win single-face windows (F = 1, not spherical, same H and
+ * deposit_on_exit) whose fields lie back to back in window order, one arena
+ * per field.  fps[w]: window w's rng state, n_particles and conc_chunk; the
+ * other knobs are fps[0]'s (chunks must agree).  prm: the shared trace
+ * parameters (prm->max_steps ignored: max_steps[w] per window). */
+int64_t pg_batch_workspace_bytes(int64_t n_win, const int64_t* NE, const int64_t* n_particles);
+int pg_batch_setup(const pg_fast_fields* wins, int64_t n_win, const pg_fast_params* fps, const int64_t* max_steps,
+                   void* work, int64_t work_bytes, pg_stream stream);
+/* One fast-mode iteration of every window: per-window pack and spawn, then
+ * per chunk both trace passes as one persistent launch over all windows'
+ * droplets and the per-cell passes over the arena, then fold / EMA.
+ * Bit-identical to pg_fast_iterate on each window alone.  stats: DEVICE
+ * int64[PG_FAST_NSTATS] of the batch; win_stats: DEVICE
+ * int64[n_win][PG_FAST_NSTATS] of the spawns. */
+int pg_batch_iterate(const pg_fast_fields* wins, int64_t n_win, const pg_trace_params* prm, const pg_fast_params* fps,
+                     void* work, int64_t work_bytes, int64_t* stats, int64_t* win_stats, pg_stream stream);
+
 /* ---- differentiable mode (no counterpart in the reference) ---------------
  * The iteration as a function of the surface, a per-cell erodibility factor
  * and six scalars with the droplet trajectories frozen, and its adjoint.
@@ -250,7 +269,8 @@ int pg_fast_trace_time(double* ms_total, int64_t* launches);
  * 562-660) without apply-time caps, one frozen-terrain chunk per iteration. */
 typedef struct pg_grad_args {
     int32_t real;               /* 0: float32 exchange arithmetic, 1: float64 */
-    int32_t pad;
+    int32_t feedback;           /* reverse: 1 = propagate the surface adjoint through the exchange, 0 = hold the
+                                 * surface the droplets read fixed (its adjoint passes the pass unchanged) */
     const void* surf;           /* exchange surface (height + sediment), F*NE*NE float or double */
     const float* erod;          /* per-cell erodibility factor multiplying c_eq, or NULL (1) */
     double theta[6];            /* erodibility, deposition_rate, evap_rate, k_disc, ocean_rate, dep_floor (cell units) */

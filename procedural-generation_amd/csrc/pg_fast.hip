@@ -29,6 +29,8 @@
 // path cell and the deficit carry have no counterpart.  Plain profile only.
 #include <hipcub/hipcub.hpp>
 
+#include <vector>
+
 #include "pg_common.h"
 #include "pg_erode.h"
 #include "pg_internal.h"
@@ -336,6 +338,15 @@ __global__ void totals_kernel(int64_t n, const int* rank, ll P, Scalars* sc, ll*
 // ---------------------------------------------------------------------------
 // the persistent trace kernel
 // ---------------------------------------------------------------------------
+// one window of a batch: its cells start at arena cell `coff`, its droplets at
+// `dbase`; the per-droplet scalars derived from its spawn as the single-grid
+// kernel derives them
+struct WinDesc {
+    ll coff, dbase, n_total, C;
+    int N, NE, max_steps, pad;
+    float inv_v0, min_volume, conc_scale, v0f;
+};
+
 struct TraceCfg {
     const float4* cell4;
     const float4* geo;
@@ -358,6 +369,12 @@ struct TraceCfg {
     ll* dis_fx;
     ll* mom_fx;
     ll* stats;
+    // batched windows: per-window geometry and spawn totals, per-chunk union
+    // offsets (n_win + 1 per chunk), the batch's queue counters
+    const WinDesc* wins;
+    const ll* win_prefix;
+    int n_win;
+    ll* queue_base;
 };
 
 // what a droplet reads at the cell it stands in
@@ -377,22 +394,34 @@ __device__ __forceinline__ float fkey_value(unsigned k) {
     return __uint_as_float((k & 0x80000000u) ? (k & 0x7fffffffu) : ~k);
 }
 
-__global__ void __launch_bounds__(TRACE_TPB) trace_fast_kernel(TraceCfg k) {
+template <bool BATCH>
+__global__ void __launch_bounds__(TRACE_TPB) trace_kernel(TraceCfg k) {
     const int lane = threadIdx.x & 63;
     const bool apply = k.pass == 1;
-    const ll n_total = k.sc->n_total;
-    const ll C = (n_total + k.chunks - 1) / k.chunks;
-    const ll d_end = (k.chunk + 1) * C < n_total ? (k.chunk + 1) * C : n_total;
-    const double volume0 = k.sc->volume0;
-    if (volume0 <= 0.0) return;
-    const float inv_v0 = (float)(1.0 / volume0);
-    const float min_volume = (float)(k.min_volume_frac * volume0);
-    // expected concurrent visits of a cell: the reference's chunk (conc_chunk
-    // droplets) unless none is given, then this kernel's own chunk
-    const float conc_scale = (float)((k.conc_chunk > 0.0 ? k.conc_chunk : (double)C) / (volume0 * (double)n_total));
-    const float v0f = (float)volume0;
-    ll* queue = &k.sc->queue[2 * k.chunk + k.pass];
-    const int NE2 = k.NE * k.NE;
+    // per-lane geometry and droplet scalars: the grid's (one launch = one
+    // grid), or, batched, those of the window of the lane's droplet
+    int N = k.N, NE = k.NE, max_steps = k.max_steps;
+    int64_t coff = 0;
+    float inv_v0 = 0.0f, min_volume = 0.0f, conc_scale = 0.0f, v0f = 0.0f;
+    ll d_end;
+    ll* queue;
+    if constexpr (BATCH) {
+        d_end = k.win_prefix[(ll)k.chunk * (k.n_win + 1) + k.n_win];
+        queue = &k.queue_base[2 * k.chunk + k.pass];
+    } else {
+        const ll n_total = k.sc->n_total;
+        const ll C = (n_total + k.chunks - 1) / k.chunks;
+        d_end = (k.chunk + 1) * C < n_total ? (k.chunk + 1) * C : n_total;
+        const double volume0 = k.sc->volume0;
+        if (volume0 <= 0.0) return;
+        inv_v0 = (float)(1.0 / volume0);
+        min_volume = (float)(k.min_volume_frac * volume0);
+        // expected concurrent visits of a cell: the reference's chunk (conc_chunk
+        // droplets) unless none is given, then this kernel's own chunk
+        conc_scale = (float)((k.conc_chunk > 0.0 ? k.conc_chunk : (double)C) / (volume0 * (double)n_total));
+        v0f = (float)volume0;
+        queue = &k.sc->queue[2 * k.chunk + k.pass];
+    }
 
     // lane state
     bool alive = false, exhausted = false;
@@ -409,8 +438,8 @@ __global__ void __launch_bounds__(TRACE_TPB) trace_fast_kernel(TraceCfg k) {
     ll deaths[N_DEATHS] = {0, 0, 0, 0, 0, 0};
 
     auto load_here = [&](int ff, int cx, int cy, Here& h, const float4* rec) {
-        const int ci = clampi(cx, 0, k.N - 1), cj = clampi(cy, 0, k.N - 1);
-        h.cell = ((int64_t)ff * k.NE + ci + k.H) * k.NE + cj + k.H;
+        const int ci = clampi(cx, 0, N - 1), cj = clampi(cy, 0, N - 1);
+        h.cell = coff + ((int64_t)ff * NE + ci + k.H) * NE + cj + k.H;
         h.m = k.mask[h.cell];
         const float2 dy = k.dyn[h.cell];
         h.sthick = dy.x;
@@ -460,13 +489,37 @@ __global__ void __launch_bounds__(TRACE_TPB) trace_fast_kernel(TraceCfg k) {
                 base = __shfl(base, leader);
                 if (base >= d_end) exhausted = true;
                 if (!alive) {
-                    const ll d = base + __popcll(dead & ((1ull << lane) - 1));
-                    if (d < d_end) {
+                    const ll u = base + __popcll(dead & ((1ull << lane) - 1));
+                    ll d = u;
+                    if constexpr (BATCH) {
+                        if (u < d_end) {
+                            // the window whose share of this chunk holds u
+                            const ll* pre = k.win_prefix + (ll)k.chunk * (k.n_win + 1);
+                            int lo = 0, hi = k.n_win - 1;
+                            while (lo < hi) {
+                                const int mid = (lo + hi + 1) >> 1;
+                                if (pre[mid] <= u) lo = mid;
+                                else hi = mid - 1;
+                            }
+                            const WinDesc wd = k.wins[lo];
+                            coff = wd.coff;
+                            N = wd.N;
+                            NE = wd.NE;
+                            max_steps = wd.max_steps;
+                            inv_v0 = wd.inv_v0;
+                            min_volume = wd.min_volume;
+                            conc_scale = wd.conc_scale;
+                            v0f = wd.v0f;
+                            d = wd.dbase + (ll)k.chunk * wd.C + (u - pre[lo]);
+                        }
+                    }
+                    if (u < d_end) {
+                        const int NE2 = NE * NE;
                         const int cell = k.d_cell[d];
                         f = cell / NE2;
                         const int r = cell - f * NE2;
-                        ix = r / k.NE - k.H;
-                        iy = r % k.NE - k.H;
+                        ix = r / NE - k.H;
+                        iy = r % NE - k.H;
                         fx = k.d_fx[d];
                         fy = k.d_fy[d];
                         real = k.d_load[d];
@@ -480,7 +533,7 @@ __global__ void __launch_bounds__(TRACE_TPB) trace_fast_kernel(TraceCfg k) {
                         in_sea = false;
                         step = uphill = sea_steps = pushes = 0;
                         last_cell = -1;
-                        S = load_stencil(k.cell4, k.NE, k.H, f, ix, iy, fx, fy);
+                        S = load_stencil(k.cell4 + coff, NE, k.H, f, ix, iy, fx, fy);
                         load_here(f, ix, iy, cur, nullptr);
                         alive = true;
                     }
@@ -493,7 +546,7 @@ __global__ void __launch_bounds__(TRACE_TPB) trace_fast_kernel(TraceCfg k) {
         // ---- one step of this lane's droplet (particle.py:421-769)
         int cause = -1;
         bool exited = false;
-        if (step >= k.max_steps) {
+        if (step >= max_steps) {
             cause = DEATH_AGE;
         } else if (cur.m == MASK_OUTSIDE) {
             cause = DEATH_EXIT;
@@ -515,19 +568,19 @@ __global__ void __launch_bounds__(TRACE_TPB) trace_fast_kernel(TraceCfg k) {
                 const float vol_rel = vol * inv_v0;
                 int nf, nix, niy;
                 float nfx, nfy, h0;
-                cause = steer_move(k, S, g0, g1, q, vol, vol_rel, sa, sb, f, ix, iy, fx, fy, nf, nix, niy, nfx, nfy, h0);
+                cause = steer_move(k, N, S, g0, g1, q, vol, vol_rel, sa, sb, f, ix, iy, fx, fy, nf, nix, niy, nfx, nfy, h0);
                 exited = cause == DEATH_EXIT;
                 {
                     if (cause < 0) {
                         // everything the next step reads, issued together
                         const int nei = nix + k.H, nej = niy + k.H;
-                        const int64_t ncell = ((int64_t)nf * k.NE + nei) * k.NE + nej;
-                        const Stencil S1 = load_stencil(k.cell4, k.NE, k.H, nf, nix, niy, nfx, nfy);
+                        const int64_t ncell = coff + ((int64_t)nf * NE + nei) * NE + nej;
+                        const Stencil S1 = load_stencil(k.cell4 + coff, NE, k.H, nf, nix, niy, nfx, nfy);
                         const int nm = k.mask[ncell];
                         const float2 ndy = k.dyn[ncell];
                         const float4 nrec = corner_of(S1, nei, nej);
                         Here nxt;
-                        const bool same = nix >= 0 && nix < k.N && niy >= 0 && niy < k.N;
+                        const bool same = nix >= 0 && nix < N && niy >= 0 && niy < N;
                         load_here(nf, nix, niy, nxt, same ? &nrec : nullptr);
                         if (nm == MASK_OUTSIDE) {
                             cause = DEATH_EXIT;
@@ -638,7 +691,7 @@ __global__ void __launch_bounds__(TRACE_TPB) trace_fast_kernel(TraceCfg k) {
                                     uphill = 0;
                                 }
                             }
-                            if (cause < 0 && step >= k.max_steps) cause = DEATH_AGE;
+                            if (cause < 0 && step >= max_steps) cause = DEATH_AGE;
                         }
                     }
                 }
@@ -840,6 +893,55 @@ __global__ void finish_kernel(int64_t n, double* height, double* sediment, doubl
     if ((threadIdx.x & 63) == 0) {
         if (eroded) atomicAdd((ull*)&stats[PG_FS_ERODED_FX], (ull)eroded);
         if (deposited) atomicAdd((ull*)&stats[PG_FS_DEPOSITED_FX], (ull)deposited);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// batched windows: the per-window table and the per-chunk union offsets,
+// from each window's spawn totals
+// ---------------------------------------------------------------------------
+struct WinStatic {
+    ll coff, dbase;
+    int N, NE, max_steps, pad;
+    double conc_chunk;
+};
+
+__global__ void batch_plan_kernel(int W, int chunks, const WinStatic* ws, const char* sc_base, size_t sc_slot,
+                                  float min_volume_frac, WinDesc* wd, ll* prefix, ll* queue, ll* stats, int nstats) {
+    for (int w = threadIdx.x; w < W; w += blockDim.x) {
+        const Scalars* sc = reinterpret_cast<const Scalars*>(sc_base + (size_t)w * sc_slot);
+        const double volume0 = sc->volume0;
+        const ll n_total = volume0 > 0.0 ? sc->n_total : 0;
+        WinDesc d;
+        d.coff = ws[w].coff;
+        d.dbase = ws[w].dbase;
+        d.n_total = n_total;
+        d.C = (n_total + chunks - 1) / chunks;
+        d.N = ws[w].N;
+        d.NE = ws[w].NE;
+        d.max_steps = ws[w].max_steps;
+        d.pad = 0;
+        d.inv_v0 = n_total > 0 ? (float)(1.0 / volume0) : 0.0f;
+        d.min_volume = (float)(min_volume_frac * volume0);
+        d.conc_scale = n_total > 0 ? (float)((ws[w].conc_chunk > 0.0 ? ws[w].conc_chunk : (double)d.C) /
+                                             (volume0 * (double)n_total))
+                                   : 0.0f;
+        d.v0f = (float)volume0;
+        wd[w] = d;
+    }
+    for (int k = threadIdx.x; k < 2 * chunks; k += blockDim.x) queue[k] = 0;
+    for (int k = threadIdx.x; k < nstats; k += blockDim.x) stats[k] = 0;
+    __syncthreads();
+    for (int c = threadIdx.x; c < chunks; c += blockDim.x) {
+        ll* pre = prefix + (ll)c * (W + 1);
+        ll acc = 0;
+        pre[0] = 0;
+        for (int w = 0; w < W; w++) {
+            const ll lo = (ll)c * wd[w].C, hi0 = (ll)(c + 1) * wd[w].C;
+            const ll hi = hi0 < wd[w].n_total ? hi0 : wd[w].n_total;
+            acc += hi > lo ? hi - lo : 0;
+            pre[w + 1] = acc;
+        }
     }
 }
 
@@ -1153,13 +1255,13 @@ int fast_particles(const pg_fast_fields* fl, const pg_trace_params* prm, const p
     k.mom_fx = at<ll>(work, L.mom_fx);
     k.stats = st;
     int grid = 0;
-    if (persistent_grid((const void*)trace_fast_kernel, TRACE_TPB, fp->grid_blocks, &grid)) return -1;
+    if (persistent_grid((const void*)trace_kernel<false>, TRACE_TPB, fp->grid_blocks, &grid)) return -1;
     for (int c = 0; c < chunks; c++) {
         k.chunk = c;
         // pass 0: erosion census on the frozen terrain
         k.pass = 0;
         if (fp->time_trace && timer_record(s, true)) return -1;
-        hipLaunchKernelGGL(trace_fast_kernel, dim3(grid), dim3(TRACE_TPB), 0, s, k);
+        hipLaunchKernelGGL(trace_kernel<false>, dim3(grid), dim3(TRACE_TPB), 0, s, k);
         if (fp->time_trace && timer_record(s, false)) return -1;
         hipLaunchKernelGGL(erosion_scale_kernel, dim3(grid_for(n)), dim3(TPB), 0, s, n, fl->height, fl->sediment, fl->base,
                            k.ero_fx, k.floor_key, at<ll>(work, L.acc_fx), at<float>(work, L.r1), fp->iter_erode, hscale);
@@ -1169,7 +1271,7 @@ int fast_particles(const pg_fast_fields* fl, const pg_trace_params* prm, const p
         // pass 1: the same trajectories, erosion scaled per cell, deficits cancel deposits
         k.pass = 1;
         if (fp->time_trace && timer_record(s, true)) return -1;
-        hipLaunchKernelGGL(trace_fast_kernel, dim3(grid), dim3(TRACE_TPB), 0, s, k);
+        hipLaunchKernelGGL(trace_kernel<false>, dim3(grid), dim3(TRACE_TPB), 0, s, k);
         if (fp->time_trace && timer_record(s, false)) return -1;
         hipLaunchKernelGGL(resolve_kernel, dim3(grid_for(n)), dim3(TPB), 0, s, n, fl->height, fl->sediment, fl->base,
                            k.chunk_fx, at<ll>(work, L.acc_fx), pend_fx, cell4, dyn, fp->iter_deposit, prm->dep_floor,
@@ -1238,5 +1340,345 @@ extern "C" int pg_fast_layout(int64_t F, int64_t NE, int64_t n_particles, int64_
     const size_t v[16] = {L.cell4, L.geo, L.dyn, L.chunk_fx, L.acc_fx, L.dis_fx, L.mom_fx, L.pend_fx,
                           L.d_cell, L.d_fx, L.d_fy, L.d_load, L.d_sa, L.d_sb, L.scalars, L.total};
     for (int k = 0; k < 16; k++) offsets[k] = (int64_t)v[k];
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
+// batched windows
+// ---------------------------------------------------------------------------
+namespace pg {
+namespace fastmode {
+namespace {
+
+constexpr int MAX_CHUNKS = 4096;
+constexpr size_t SC_SLOT = (sizeof(Scalars) + 8 * 2 * MAX_CHUNKS + 255) & ~(size_t)255;
+
+// the batch arena: per-cell arrays over all windows' cells (window order),
+// per-droplet arrays over all windows' droplet slots, one scalar slot per
+// window, shared sort/scan scratch, the window tables
+struct BLayout {
+    Layout base;  // arena offsets (per-cell, per-droplet, scalars = first slot, temp)
+    size_t wstatic, wdesc, prefix, queue, total;
+    int64_t ncells;
+};
+
+int batch_layout(int64_t W, const int64_t* NE, const int64_t* P, BLayout* B) {
+    int64_t n = 0, p = 0, nd = 0;
+    size_t tmax = 0;
+    for (int64_t w = 0; w < W; w++) {
+        Layout Lw;
+        if (layout_of(1, NE[w], P[w], &Lw) != 0) return -1;
+        tmax = Lw.temp_bytes > tmax ? Lw.temp_bytes : tmax;
+        n += NE[w] * NE[w];
+        p += P[w];
+        nd += P[w] + NE[w] * NE[w];
+    }
+    Layout& L = B->base;
+    size_t o = 0;
+    auto up256 = [](size_t x) { return (x + 255) & ~(size_t)255; };
+    auto take = [&](size_t bytes) { size_t a = o; o += up256(bytes); return a; };
+    L.cell4 = take(16 * n);
+    L.geo = take(32 * n);
+    L.dyn = take(8 * n);
+    L.chunk_fx = take(8 * n);
+    L.acc_fx = take(8 * n);
+    L.dis_fx = take(8 * n);
+    L.mom_fx = take(16 * n);
+    L.pend_fx = take(8 * n);
+    L.ero_fx = take(8 * n);
+    L.floor_key = take(8 * n);
+    L.rscale = take(4 * n);
+    L.r1 = take(4 * n);
+    L.wq = take(8 * n);
+    L.cdf_fx = take(8 * n);
+    L.flag = take(4 * n);
+    L.rank = take(4 * n);
+    L.comp_idx = take(4 * n);
+    L.comp_cdf = take(8 * n);
+    L.draws = take(8 * 3 * p);
+    L.u_sorted = take(8 * p);
+    L.d_cell = take(4 * nd);
+    L.d_fx = take(4 * nd);
+    L.d_fy = take(4 * nd);
+    L.d_load = take(8 * nd);
+    L.d_sa = take(4 * nd);
+    L.d_sb = take(4 * nd);
+    L.scalars = take(SC_SLOT * (size_t)W);
+    L.temp_bytes = tmax;
+    L.temp = take(tmax + 256);
+    B->wstatic = take(sizeof(WinStatic) * (size_t)W);
+    B->wdesc = take(sizeof(WinDesc) * (size_t)W);
+    B->prefix = take(8 * (size_t)MAX_CHUNKS * (size_t)(W + 1));
+    B->queue = take(8 * 2 * (size_t)MAX_CHUNKS);
+    L.total = B->total = o;
+    B->ncells = n;
+    return 0;
+}
+
+// window w's view of the arena, as the single-grid code addresses it
+void window_layout(const BLayout& B, int64_t coff, int64_t poff, int64_t doff, int64_t w, Layout* Lw) {
+    const Layout& L = B.base;
+    *Lw = L;
+    const size_t c = (size_t)coff;
+    Lw->cell4 = L.cell4 + 16 * c;
+    Lw->geo = L.geo + 32 * c;
+    Lw->dyn = L.dyn + 8 * c;
+    Lw->chunk_fx = L.chunk_fx + 8 * c;
+    Lw->acc_fx = L.acc_fx + 8 * c;
+    Lw->dis_fx = L.dis_fx + 8 * c;
+    Lw->mom_fx = L.mom_fx + 16 * c;
+    Lw->pend_fx = L.pend_fx + 8 * c;
+    Lw->ero_fx = L.ero_fx + 8 * c;
+    Lw->floor_key = L.floor_key + 8 * c;
+    Lw->rscale = L.rscale + 4 * c;
+    Lw->r1 = L.r1 + 4 * c;
+    Lw->wq = L.wq + 8 * c;
+    Lw->cdf_fx = L.cdf_fx + 8 * c;
+    Lw->flag = L.flag + 4 * c;
+    Lw->rank = L.rank + 4 * c;
+    Lw->comp_idx = L.comp_idx + 4 * c;
+    Lw->comp_cdf = L.comp_cdf + 8 * c;
+    Lw->draws = L.draws + 8 * 3 * (size_t)poff;
+    Lw->u_sorted = L.u_sorted + 8 * (size_t)poff;
+    Lw->d_cell = L.d_cell + 4 * (size_t)doff;
+    Lw->d_fx = L.d_fx + 4 * (size_t)doff;
+    Lw->d_fy = L.d_fy + 4 * (size_t)doff;
+    Lw->d_load = L.d_load + 8 * (size_t)doff;
+    Lw->d_sa = L.d_sa + 4 * (size_t)doff;
+    Lw->d_sb = L.d_sb + 4 * (size_t)doff;
+    Lw->scalars = L.scalars + SC_SLOT * (size_t)w;
+}
+
+// the windows' fields must lie back to back in window order (one arena per
+// field); returns the per-window cell offsets
+int check_windows(const pg_fast_fields* wins, int64_t W, std::vector<int64_t>& coff) {
+    PG_REQUIRE(wins && W >= 1 && W <= 65536, "batch: need 1..65536 windows");
+    coff.assign((size_t)W + 1, 0);
+    for (int64_t w = 0; w < W; w++) {
+        const pg_fast_fields& f = wins[w];
+        if (check_fields(&f)) return -1;
+        PG_REQUIRE(f.F == 1 && !f.spherical, "batch: window %lld is not a single-face window", (long long)w);
+        PG_REQUIRE(f.H == wins[0].H && f.deposit_on_exit == wins[0].deposit_on_exit,
+                   "batch: windows differ in halo width or deposit_on_exit");
+        const int64_t NE = f.N + 2 * f.H;
+        coff[w + 1] = coff[w] + NE * NE;
+        const int64_t o = coff[w];
+        PG_REQUIRE(f.height == wins[0].height + o && f.sediment == wins[0].sediment + o &&
+                       f.discharge == wins[0].discharge + o && f.momentum == wins[0].momentum + 2 * o &&
+                       f.pending == wins[0].pending + o && f.base == wins[0].base + o &&
+                       f.precip == wins[0].precip + o && f.evap == wins[0].evap + o &&
+                       f.hardness == wins[0].hardness + o && f.metric == wins[0].metric + 3 * o &&
+                       f.metric_inv == wins[0].metric_inv + 3 * o && f.mask == wins[0].mask + o,
+                   "batch: window %lld's fields do not follow window %lld's in one arena", (long long)w,
+                   (long long)(w - 1));
+    }
+    PG_REQUIRE(coff[W] < (1ll << 31), "batch: arena too large for int32 cell ids");
+    return 0;
+}
+
+}  // namespace
+}  // namespace fastmode
+}  // namespace pg
+
+extern "C" int64_t pg_batch_workspace_bytes(int64_t n_win, const int64_t* NE, const int64_t* n_particles) {
+    clear_error();
+    if (n_win < 1 || !NE || !n_particles) {
+        set_error("pg_batch_workspace_bytes: bad arguments");
+        return -1;
+    }
+    BLayout B;
+    if (batch_layout(n_win, NE, n_particles, &B) != 0) {
+        set_error("pg_batch_workspace_bytes: hipcub temp-size query failed");
+        return -1;
+    }
+    return (int64_t)B.total;
+}
+
+extern "C" int pg_batch_setup(const pg_fast_fields* wins, int64_t n_win, const pg_fast_params* fps,
+                              const int64_t* max_steps, void* work, int64_t work_bytes, pg_stream stream) {
+    clear_error();
+    std::vector<int64_t> coff;
+    if (check_windows(wins, n_win, coff)) return -1;
+    PG_REQUIRE(fps && max_steps && work, "pg_batch_setup: NULL argument");
+    std::vector<int64_t> NE(n_win), P(n_win);
+    for (int64_t w = 0; w < n_win; w++) {
+        NE[w] = wins[w].N + 2 * wins[w].H;
+        P[w] = fps[w].n_particles;
+        PG_REQUIRE(P[w] >= 1 && max_steps[w] >= 0, "pg_batch_setup: window %lld: bad droplet count or lifetime",
+                   (long long)w);
+    }
+    BLayout B;
+    PG_REQUIRE(batch_layout(n_win, NE.data(), P.data(), &B) == 0, "pg_batch_setup: hipcub temp-size query failed");
+    PG_REQUIRE((int64_t)B.total <= work_bytes, "pg_batch_setup: workspace of %lld bytes, need %lld",
+               (long long)work_bytes, (long long)B.total);
+    PG_REQUIRE(fps[0].height_bits > 0 && fps[0].height_bits < 62, "pg_batch_setup: height_bits out of range");
+    hipStream_t s = (hipStream_t)stream;
+    const int64_t n = coff[n_win];
+    const Layout& L = B.base;
+    hipLaunchKernelGGL(setup_kernel, dim3(grid_for(n)), dim3(TPB), 0, s, n, wins[0].metric, wins[0].metric_inv,
+                       wins[0].evap, wins[0].hardness, at<float4>(work, L.geo), wins[0].pending, at<ll>(work, L.pend_fx),
+                       ldexp(1.0, (int)fps[0].height_bits));
+    PG_LAUNCHED("setup_kernel");
+    PG_HIP(hipMemsetAsync(at<char>(work, L.chunk_fx), 0, L.pend_fx - L.chunk_fx, s));
+    PG_HIP(hipMemsetAsync(at<char>(work, L.ero_fx), 0, L.floor_key - L.ero_fx, s));
+    PG_HIP(hipMemsetAsync(at<char>(work, L.floor_key), 0xff, L.rscale - L.floor_key, s));
+    std::vector<WinStatic> ws(n_win);
+    int64_t doff = 0;
+    for (int64_t w = 0; w < n_win; w++) {
+        ws[w].coff = coff[w];
+        ws[w].dbase = doff;
+        ws[w].N = (int)wins[w].N;
+        ws[w].NE = (int)NE[w];
+        ws[w].max_steps = (int)max_steps[w];
+        ws[w].pad = 0;
+        ws[w].conc_chunk = fps[w].conc_chunk;
+        doff += P[w] + NE[w] * NE[w];
+    }
+    PG_HIP(hipMemcpyAsync(at<char>(work, B.wstatic), ws.data(), sizeof(WinStatic) * (size_t)n_win,
+                          hipMemcpyHostToDevice, s));
+    PG_HIP(hipStreamSynchronize(s));
+    return 0;
+}
+
+extern "C" int pg_batch_iterate(const pg_fast_fields* wins, int64_t n_win, const pg_trace_params* prm,
+                                const pg_fast_params* fps, void* work, int64_t work_bytes, int64_t* stats,
+                                int64_t* win_stats, pg_stream stream) {
+    clear_error();
+    std::vector<int64_t> coff;
+    if (check_windows(wins, n_win, coff)) return -1;
+    PG_REQUIRE(prm && fps && work && stats && win_stats, "pg_batch_iterate: NULL argument");
+    const pg_fast_params& f0 = fps[0];
+    PG_REQUIRE(f0.chunks >= 1 && f0.chunks <= MAX_CHUNKS, "pg_batch_iterate: chunks must be in [1, %d]", MAX_CHUNKS);
+    PG_REQUIRE(prm->lateral_rate <= 0.0 && prm->lake_park <= 0.0, "pg_batch_iterate: plain profile only");
+    std::vector<int64_t> NE(n_win), P(n_win);
+    for (int64_t w = 0; w < n_win; w++) {
+        NE[w] = wins[w].N + 2 * wins[w].H;
+        P[w] = fps[w].n_particles;
+        PG_REQUIRE(fps[w].chunks == f0.chunks, "pg_batch_iterate: windows differ in chunks");
+        PG_REQUIRE(P[w] >= 1 && P[w] < (1ll << 30), "pg_batch_iterate: window %lld: n_particles out of range",
+                   (long long)w);
+    }
+    BLayout B;
+    PG_REQUIRE(batch_layout(n_win, NE.data(), P.data(), &B) == 0, "pg_batch_iterate: hipcub temp-size query failed");
+    PG_REQUIRE((int64_t)B.total <= work_bytes, "pg_batch_iterate: workspace of %lld bytes, need %lld",
+               (long long)work_bytes, (long long)B.total);
+    hipStream_t s = (hipStream_t)stream;
+    const double hscale = ldexp(1.0, (int)f0.height_bits), tscale = ldexp(1.0, (int)f0.track_bits);
+    const int chunks = (int)f0.chunks;
+    const int64_t n = coff[n_win];
+    const Layout& L = B.base;
+    // per-window spawn into the window's view of the arena
+    int64_t poff = 0, doff = 0;
+    for (int64_t w = 0; w < n_win; w++) {
+        Ctx c;
+        window_layout(B, coff[w], poff, doff, w, &c.L);
+        c.s = s;
+        c.hscale = hscale;
+        c.tscale = tscale;
+        c.sc = at<Scalars>(work, c.L.scalars);
+        c.cell4 = at<float4>(work, c.L.cell4);
+        c.dyn = at<float2>(work, c.L.dyn);
+        c.flag = at<int>(work, c.L.flag);
+        c.rank = at<int>(work, c.L.rank);
+        c.pend_fx = at<ll>(work, c.L.pend_fx);
+        c.st = (ll*)win_stats + w * PG_FAST_NSTATS;
+        c.temp = at<void>(work, c.L.temp);
+        c.chunks = chunks;
+        c.N = wins[w].N;
+        c.H = wins[w].H;
+        c.NE = NE[w];
+        c.n = NE[w] * NE[w];
+        c.P = P[w];
+        if (fast_spawn(&wins[w], &fps[w], work, c)) return -1;
+        poff += P[w];
+        doff += P[w] + NE[w] * NE[w];
+    }
+    ll* qbase = at<ll>(work, B.queue);
+    ll* prefix = at<ll>(work, B.prefix);
+    WinDesc* wd = at<WinDesc>(work, B.wdesc);
+    hipLaunchKernelGGL(batch_plan_kernel, dim3(1), dim3(256), 0, s, (int)n_win, chunks, at<WinStatic>(work, B.wstatic),
+                       at<char>(work, L.scalars), SC_SLOT, (float)f0.min_volume_frac, wd, prefix, qbase, (ll*)stats,
+                       (int)PG_FAST_NSTATS);
+    PG_LAUNCHED("batch_plan_kernel");
+    // chunks: batched persistent trace over every window's share, per-cell passes over the arena
+    TraceCfg k = {};
+    k.cell4 = at<float4>(work, L.cell4);
+    k.geo = at<float4>(work, L.geo);
+    k.dyn = at<float2>(work, L.dyn);
+    k.mask = wins[0].mask;
+    k.rscale = at<float>(work, L.rscale);
+    k.H = (int)wins[0].H;
+    k.spherical = 0;
+    k.deposit_on_exit = (int)wins[0].deposit_on_exit;
+    k.chunks = chunks;
+    k.d_cell = at<int>(work, L.d_cell);
+    k.d_fx = at<float>(work, L.d_fx);
+    k.d_fy = at<float>(work, L.d_fy);
+    k.d_sa = at<float>(work, L.d_sa);
+    k.d_sb = at<float>(work, L.d_sb);
+    k.d_load = at<ll>(work, L.d_load);
+    k.dt = (float)prm->dt;
+    k.density = (float)prm->density;
+    const double fr = 1.0 - prm->dt * prm->friction;
+    k.fr = (float)(fr < 0.0 ? 0.0 : fr);
+    k.rate = (float)prm->deposition_rate;
+    k.dt_evap = (float)(prm->dt * prm->evap_rate);
+    k.k_mom = (float)prm->k_mom;
+    k.mom_scale = (float)prm->mom_scale;
+    k.k_disc = (float)prm->k_disc;
+    k.inv_sat = (float)(prm->disc_saturation > 0.0 ? 1.0 / prm->disc_saturation : 0.0);
+    k.disc_exponent = (float)prm->disc_exponent;
+    k.slope_gain = (float)prm->slope_gain;
+    k.slope_sat = (float)prm->slope_sat;
+    k.erodibility = (float)prm->erodibility;
+    k.cover_depth = (float)prm->cover_depth;
+    k.max_erode = (float)prm->max_erode;
+    k.ocean_rate = (float)prm->ocean_rate;
+    k.fan_room = (float)prm->fan_room;
+    k.dep_floor = (float)prm->dep_floor;
+    k.min_volume_frac = (float)f0.min_volume_frac;
+    k.pit_steps = (int)prm->pit_steps;
+    k.ocean_steps = (int)prm->ocean_steps;
+    k.hscale = hscale;
+    k.tscale = tscale;
+    k.chunk_fx = at<ll>(work, L.chunk_fx);
+    k.ero_fx = at<ll>(work, L.ero_fx);
+    k.floor_key = at<ull>(work, L.floor_key);
+    k.fan_slope = (float)f0.fan_slope;
+    k.dis_fx = at<ll>(work, L.dis_fx);
+    k.mom_fx = at<ll>(work, L.mom_fx);
+    k.stats = (ll*)stats;
+    k.wins = wd;
+    k.win_prefix = prefix;
+    k.n_win = (int)n_win;
+    k.queue_base = qbase;
+    int grid = 0;
+    if (persistent_grid((const void*)trace_kernel<true>, TRACE_TPB, f0.grid_blocks, &grid)) return -1;
+    const pg_fast_fields& a = wins[0];  // arena bases
+    for (int c = 0; c < chunks; c++) {
+        k.chunk = c;
+        k.pass = 0;
+        if (f0.time_trace && timer_record(s, true)) return -1;
+        hipLaunchKernelGGL(trace_kernel<true>, dim3(grid), dim3(TRACE_TPB), 0, s, k);
+        if (f0.time_trace && timer_record(s, false)) return -1;
+        hipLaunchKernelGGL(erosion_scale_kernel, dim3(grid_for(n)), dim3(TPB), 0, s, n, a.height, a.sediment, a.base,
+                           k.ero_fx, k.floor_key, at<ll>(work, L.acc_fx), at<float>(work, L.r1), f0.iter_erode, hscale);
+        hipLaunchKernelGGL(erosion_scale2_kernel, dim3(grid_for(n)), dim3(TPB), 0, s, n, a.height, a.sediment, a.base,
+                           k.ero_fx, k.floor_key, at<ll>(work, L.acc_fx), at<float>(work, L.r1),
+                           at<float>(work, L.rscale), f0.iter_erode, hscale);
+        k.pass = 1;
+        if (f0.time_trace && timer_record(s, true)) return -1;
+        hipLaunchKernelGGL(trace_kernel<true>, dim3(grid), dim3(TRACE_TPB), 0, s, k);
+        if (f0.time_trace && timer_record(s, false)) return -1;
+        hipLaunchKernelGGL(resolve_kernel, dim3(grid_for(n)), dim3(TPB), 0, s, n, a.height, a.sediment, a.base,
+                           k.chunk_fx, at<ll>(work, L.acc_fx), at<ll>(work, L.pend_fx), at<float4>(work, L.cell4),
+                           at<float2>(work, L.dyn), f0.iter_deposit, prm->dep_floor, f0.offshore_writeoff, hscale,
+                           (ll*)stats, k.ero_fx, k.floor_key);
+    }
+    PG_LAUNCHED("batch trace/resolve");
+    hipLaunchKernelGGL(finish_kernel, dim3(grid_for(n)), dim3(TPB), 0, s, n, a.height, a.sediment, a.discharge,
+                       a.momentum, a.pending, a.mask, at<ll>(work, L.acc_fx), k.dis_fx, k.mom_fx, at<ll>(work, L.pend_fx),
+                       f0.ema, hscale, tscale, (ll*)stats);
+    PG_LAUNCHED("finish_kernel");
     return 0;
 }

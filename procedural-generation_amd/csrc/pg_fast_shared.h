@@ -134,12 +134,13 @@ __device__ __forceinline__ ll wave_sum(ll v) {
 // one move of a droplet (particle.py:437-531): gradient of the bilinear
 // surface, inertia, gravity (and the stream push when k_mom != 0), the step
 // normalised to one cell in the metric, face transfer.  `K` supplies fr,
-// k_mom, mom_scale, dt, density, slope_gain, slope_sat, N, H, spherical.
+// k_mom, mom_scale, dt, density, slope_gain, slope_sat, H, spherical; N is
+// the droplet's grid's interior size.
 // Returns -1 (moved), DEATH_STOP (no finite direction) or DEATH_EXIT (left a
 // window).  h0 = the bilinear surface at the current position.
 // ---------------------------------------------------------------------------
 template <class K>
-__device__ __forceinline__ int steer_move(const K& k, const Stencil& S, const float4 g0, const float4 g1, float q,
+__device__ __forceinline__ int steer_move(const K& k, int N, const Stencil& S, const float4 g0, const float4 g1, float q,
                                           float vol, float vol_rel, float& sa, float& sb, int f, int ix, int iy,
                                           float fx, float fy, int& nf, int& nix, int& niy, float& nfx, float& nfy,
                                           float& h0) {
@@ -187,8 +188,8 @@ __device__ __forceinline__ int steer_move(const K& k, const Stencil& S, const fl
     nfy = fminf(nfy - fly, 0.99999994f);
     nf = f;
     if (k.spherical) {
-        if (nix < 0 || nix >= k.N || niy < 0 || niy >= k.N) {
-            const double Nd = (double)k.N;
+        if (nix < 0 || nix >= N || niy < 0 || niy >= N) {
+            const double Nd = (double)N;
             const Transfer t = transfer_velocity<LibMath>(f, ((double)nix + nfx) / Nd, ((double)niy + nfy) / Nd,
                                                           (double)sa, (double)sb);
             nf = t.face;
@@ -205,7 +206,7 @@ __device__ __forceinline__ int steer_move(const K& k, const Stencil& S, const fl
             sa = (float)t.a;
             sb = (float)t.b;
         }
-    } else if (nix < -k.H || nix >= k.N + k.H || niy < -k.H || niy >= k.N + k.H) {
+    } else if (nix < -k.H || nix >= N + k.H || niy < -k.H || niy >= N + k.H) {
         return DEATH_EXIT;
     }
     return -1;

@@ -303,7 +303,7 @@ __global__ void __launch_bounds__(TRACE_TPB) grad_forward_kernel(GradCfg<Real> k
                 const float q = fmaxf(lerp2(S.c00.y, S.c10.y, S.c01.y, S.c11.y, S.wi, S.wj), 0.0f);
                 int nf, nix, niy;
                 float nfx, nfy, h0s;
-                cause = steer_move(k, S, g0, g1, q, vol_s, vol_s * inv_v0, sa, sb, f, ix, iy, fx, fy, nf, nix, niy, nfx,
+                cause = steer_move(k, k.N, S, g0, g1, q, vol_s, vol_s * inv_v0, sa, sb, f, ix, iy, fx, fy, nf, nix, niy, nfx,
                                    nfy, h0s);
                 exited = cause == DEATH_EXIT;
                 if (cause < 0) {
@@ -479,7 +479,7 @@ struct RevCfg {
     double* lam;       // += the particle pass's share of dL/d(surface before it)
     double* grad_erod; // += dL/d(erodibility factor), extended cells
     double* grad_theta;
-    int N, H, NE;
+    int N, H, NE, feedback;
     float dt, inv_sat, disc_exponent, max_erode, fan_room;
     Real erodibility, rate, evap_rate, k_disc, ocean_rate, dep_floor;
 };
@@ -640,7 +640,7 @@ __global__ void __launch_bounds__(TPB) grad_reverse_kernel(RevCfg<Real> r) {
             const Real a_vol_k = a_vol * ev + a_vr / v0;
             // surface adjoints: h1 of this move sits on the next position's stencil
             const double a_h1 = (rc.flags & F_H1) ? -(double)a_dh : 0.0;
-            scatter(r, rn, st_adj + a_h1, cell_adj);
+            if (r.feedback) scatter(r, rn, st_adj + a_h1, cell_adj);
             st_adj = (double)a_dh;
             cell_adj = (double)a_hh + prev_adj;
             prev_adj = (double)a_hprev;
@@ -649,7 +649,7 @@ __global__ void __launch_bounds__(TPB) grad_reverse_kernel(RevCfg<Real> r) {
             rn = rc;
             rc = rp;
         }
-        scatter(r, rn, st_adj, cell_adj);
+        if (r.feedback) scatter(r, rn, st_adj, cell_adj);
     }
     for (int c = 0; c < N_THETA; c++) {
         double v = (double)gt[c];
@@ -750,6 +750,7 @@ int launch_reverse(int64_t F, int64_t N, int64_t H, const pg_trace_params* prm, 
     r.N = (int)N;
     r.H = (int)H;
     r.NE = (int)(N + 2 * H);
+    r.feedback = ga->feedback != 0;
     r.dt = (float)prm->dt;
     r.inv_sat = (float)(prm->disc_saturation > 0.0 ? 1.0 / prm->disc_saturation : 0.0);
     r.disc_exponent = (float)prm->disc_exponent;

@@ -47,7 +47,7 @@ class FastParams(ctypes.Structure):
 class GradArgs(ctypes.Structure):
     """``pg_grad_args``: the differentiable mode's inputs of one iteration."""
 
-    _fields_ = [("real", ctypes.c_int32), ("pad", ctypes.c_int32), ("surf", vp), ("erod", vp), ("theta", dbl * 6),
+    _fields_ = [("real", ctypes.c_int32), ("feedback", ctypes.c_int32), ("surf", vp), ("erod", vp), ("theta", dbl * 6),
                 ("tape", vp), ("tape_bytes", i64), ("fixed_moves", vp), ("moves_out", vp)]
 
 
@@ -96,6 +96,11 @@ SIGNATURES = [
                                 vp, i64, vp, vp]),
     ("pg_fast_layout", i32, [i64, i64, i64, ctypes.POINTER(ctypes.c_int64), i32]),
     ("pg_fast_trace_time", i32, [pd, ctypes.POINTER(ctypes.c_int64)]),
+    ("pg_batch_workspace_bytes", i64, [i64, ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int64)]),
+    ("pg_batch_setup", i32, [ctypes.POINTER(FastFields), i64, ctypes.POINTER(FastParams), ctypes.POINTER(ctypes.c_int64),
+                             vp, i64, vp]),
+    ("pg_batch_iterate", i32, [ctypes.POINTER(FastFields), i64, ctypes.POINTER(TraceParams), ctypes.POINTER(FastParams),
+                               vp, i64, vp, vp, vp]),
     ("pg_grad_tape_bytes", i32, [i64, i64, ctypes.c_int32, ctypes.POINTER(ctypes.c_int64)]),
     ("pg_grad_iterate", i32, [ctypes.POINTER(FastFields), ctypes.POINTER(FastFields), ctypes.POINTER(TraceParams),
                               ctypes.POINTER(FastParams), ctypes.POINTER(GradArgs), vp, i64, vp, vp]),

@@ -58,10 +58,20 @@ class GradRun:
 
     real: "f32" or "f64" exchange arithmetic.  replay_of: another GradRun
     run with ``record=True`` -- this run then follows its trajectories
-    (steering records, move counts, datum cells) instead of its own."""
+    (steering records, move counts, datum cells) instead of its own.
+
+    feedback: True differentiates the exchange's dependence on the surface
+    the droplets read (h0, h1, their cell's and the previous cell's
+    height) -- the exact derivative of the frozen-trajectory map, whose
+    sensitivity grows with the horizon (every droplet of the frozen chunk
+    answers the same surface change).  False holds that surface at the
+    nominal run's (a replay then reads the nominal surface too), so the
+    surface adjoint passes each particle pass unchanged: the derivative of
+    the frozen-terrain map, stable at any horizon."""
 
     def __init__(self, state, params, iterations: int, real: str = "f32", checkpoint_every: int = 10,
-                 first_key: int = 0, rng_stage: str = "erosion", n_particles=None, replay_of: "GradRun | None" = None):
+                 first_key: int = 0, rng_stage: str = "erosion", n_particles=None, replay_of: "GradRun | None" = None,
+                 feedback: bool = True):
         from .maps import _check_scope, _eparams, discharge_scales, max_steps_of, trace_params, uplift_cap
 
         if state.mode != "fast":
@@ -84,6 +94,7 @@ class GradRun:
         self.keys = [int(first_key) + t for t in range(self.T)]
         self.rng_stage = rng_stage
         self.replay_of = replay_of
+        self.feedback = bool(feedback)
         dev = state.device
         n_cells = int(np.count_nonzero(state._mask_host[state.interior] == pk.MASK_ACTIVE))
         if n_particles is None:
@@ -165,7 +176,7 @@ class GradRun:
             raise ValueError("theta holds 6 values: " + ", ".join(THETA_NAMES))
         if erod is not None:
             pk._need(erod, torch.float32, tuple(self.state.height.shape), "erodibility field")
-        return GradArgs(real=1 if self.real == "f64" else 0, surf=surf.data_ptr(),
+        return GradArgs(real=1 if self.real == "f64" else 0, feedback=1 if self.feedback else 0, surf=surf.data_ptr(),
                         erod=None if erod is None else erod.data_ptr(), theta=(ctypes.c_double * 6)(*th),
                         tape=None if tape is None else tape.data_ptr(), tape_bytes=0 if tape is None else tape.numel(),
                         fixed_moves=None if fixed is None else fixed.data_ptr(),
@@ -182,7 +193,8 @@ class GradRun:
                  idx=None):
         st = self.state
         steer = target if steer is None else steer
-        surf = (target["height"] + target["sediment"]).to(self.dtype)
+        src = target if self.feedback else steer
+        surf = (src["height"] + src["sediment"]).to(self.dtype)
         ga = self._args(surf, erod, theta, tape, fixed, moves_out)
         check(lib().pg_grad_iterate(ctypes.byref(self._fields(steer)), ctypes.byref(self._fields(target)),
                                     ctypes.byref(self.prm), ctypes.byref(self._fparams(t)), ctypes.byref(ga),

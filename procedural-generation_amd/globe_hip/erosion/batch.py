@@ -1,0 +1,257 @@
+"""Batched windows: many single-face windows eroded together on one GPU.
+
+The reference erodes zoom tiles one process per tile (each an
+``ErosionState.window`` driven by ``maps.run_iteration``,
+bake/globe/zoom/bake.py:777-923 and zoom/planet.py:324-437).  Here the
+windows -- each with its own size, seed, droplet count and lifetime, all
+sharing the other erosion parameters and the cell size -- live in one device
+arena (every field's windows back to back), and one iteration is
+
+  - per window: pack and spawn from the window's own ``params.rng`` stream
+    into the window's slice of the arena (``pg_batch_iterate``);
+  - per chunk: the two trace passes as ONE persistent launch over every
+    window's share of the chunk (a device work queue across windows; lanes
+    pick up the geometry and droplet scalars of the window they draw from),
+    the per-cell erosion scale and resolve over the whole arena;
+  - fold / EMA over the arena, then the windows' uplift.
+
+Windows never interact and every accumulation is int64 fixed point, so the
+result is bit-identical to running each window alone in fast mode
+(``step_per_window``, the one-launch-per-window form it is measured against).
+``BatchGraph`` captures the batched iteration in a HIP graph;
+``Autotuner`` picks the persistent grid from timed real iterations (results
+do not depend on it).
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+import torch
+
+from .._lib import FAST_NSTATS, FS, FastFields, FastParams, check, lib
+from ..config import cell_units
+from . import fast
+from . import maps
+from . import particle as pk
+
+
+def _signed(w: int) -> int:
+    return w - (1 << 64) if w >= (1 << 63) else w
+
+
+class WindowBatch:
+    """windows: ``ErosionState.window`` keyword arguments (metres, one dict
+    per window); params: one WorldParams per window (seeds and max_steps may
+    differ, everything else must agree)."""
+
+    def __init__(self, windows: list[dict], params: list, device="cuda", chunks: int = fast.DEFAULT_CHUNKS,
+                 rng_stage: str = "erosion"):
+        if len(windows) != len(params) or not windows:
+            raise ValueError("one WorldParams per window")
+        self.W = len(windows)
+        self.params = list(params)
+        self.rng_stage = rng_stage
+        self.chunks = int(chunks)
+        dev = torch.device(device)
+        eps = [maps._eparams(p) for p in params]
+        ref = {k: v for k, v in vars(eps[0]).items() if k != "max_steps"}
+        for e in eps[1:]:
+            if {k: v for k, v in vars(e).items() if k != "max_steps"} != ref:
+                raise ValueError("batched windows share their erosion parameters (max_steps aside)")
+        hu = {float(w["height_unit_m"]) for w in windows}
+        H = {int(w["H"]) for w in windows}
+        doe = {bool(w.get("deposit_on_exit", False)) for w in windows}
+        if len(hu) != 1 or len(H) != 1 or len(doe) != 1:
+            raise ValueError("batched windows share the cell size, halo width and deposit_on_exit")
+        self.unit, self.H = hu.pop(), H.pop()
+        NE = [int(np.asarray(w["height_m"]).shape[0]) for w in windows]
+        self.NE = NE
+        off = np.concatenate([[0], np.cumsum([n * n for n in NE])]).astype(np.int64)
+        self.off = off
+        n = int(off[-1])
+
+        def arena(key, dtype, comp=(), scale=None):
+            a = np.empty((n,) + comp, dtype)
+            for w, win in enumerate(windows):
+                x = np.asarray(win[key], np.float64 if scale else dtype)
+                a[off[w]:off[w + 1]] = (x / scale if scale else x).reshape((-1,) + comp)
+            return torch.from_numpy(a).to(dev)
+
+        u = self.unit
+        self.height = arena("height_m", np.float64, scale=u)
+        self.sediment = arena("sediment_m", np.float64, scale=u)
+        self.discharge = arena("discharge", np.float64)
+        self.momentum = arena("momentum", np.float64, (2,))
+        self.hardness = arena("hardness", np.float32)
+        self.precip = arena("precip", np.float32)
+        self.evap = arena("evap", np.float32)
+        self.uplift = arena("uplift_m", np.float64, scale=u)
+        self.mask = arena("mask", np.uint8)
+        self.metric = arena("metric", np.float32, (3,))
+        self.metric_inv = arena("metric_inv", np.float32, (3,))
+        self.pending = torch.zeros(n, dtype=torch.float64, device=dev)
+        self.base = torch.zeros(n, dtype=torch.float64, device=dev)
+        empty = torch.empty(0, device=dev)
+        self.states = []
+        for w in range(self.W):
+            s = slice(int(off[w]), int(off[w + 1]))
+            v = lambda t, c=(): t[s].view((1, NE[w], NE[w]) + c)
+            st = maps.ErosionState(height=v(self.height), sediment=v(self.sediment), discharge=v(self.discharge),
+                                   momentum=v(self.momentum, (2,)), hardness=v(self.hardness), precip=v(self.precip),
+                                   evap=v(self.evap), uplift=v(self.uplift), mask=v(self.mask),
+                                   metric=v(self.metric, (3,)), metric_inv=v(self.metric_inv, (3,)),
+                                   N=NE[w] - 2 * self.H, H=self.H, spherical=False, height_unit_m=u,
+                                   deposit_on_exit=doe_val(windows[w]), mode="fast", pending=v(self.pending),
+                                   base=v(self.base), samp=empty, acc=empty, disch_track=empty, mom_track=empty,
+                                   lake_load=empty)
+            st.fast_chunks = self.chunks
+            st.fast_sync_stats = False
+            self.states.append(st)
+        self.P = [max(1, int(round(eps[w].particles_per_cell *
+                                   np.count_nonzero(st._mask_host[st.interior] == pk.MASK_ACTIVE))))
+                  for w, st in enumerate(self.states)]
+        self.max_steps = [maps.max_steps_of(eps[w], st.N) for w, st in enumerate(self.states)]
+        st0 = self.states[0]
+        disc_sat, mom_scale = maps.discharge_scales(eps[0], st0, 1.0, self.P[0], n_rain=1)
+        if float(eps[0].disc_saturation_cells) > 0.0 or float(eps[0].momentum_saturation_cells) > 0.0:
+            raise NotImplementedError("cell-count discharge scales are outside the fast mode")
+        self.prm = maps.trace_params(eps[0], st0, 1.0, 1, 1, disc_sat, mom_scale, self.max_steps[0], 0.0)
+        self.ep = eps[0]
+        self.fields = (FastFields * self.W)(*[self._fields(st) for st in self.states])
+        self.NE_arr = (ctypes.c_int64 * self.W)(*NE)
+        self.P_arr = (ctypes.c_int64 * self.W)(*self.P)
+        self.steps_arr = (ctypes.c_int64 * self.W)(*self.max_steps)
+        nbytes = int(lib().pg_batch_workspace_bytes(self.W, self.NE_arr, self.P_arr))
+        if nbytes < 0:
+            check(-1, "pg_batch_workspace_bytes")
+        self.work = torch.empty(nbytes, dtype=torch.uint8, device=dev)
+        self.stats = torch.zeros(FAST_NSTATS, dtype=torch.int64, device=dev)
+        self.win_stats = torch.zeros((self.W, FAST_NSTATS), dtype=torch.int64, device=dev)
+        self.grid_blocks = 0
+        self.time_trace = False
+        self.rng_dev = None
+        check(lib().pg_batch_setup(self.fields, self.W, self._fparams((0,)), self.steps_arr,
+                                   ctypes.c_void_p(self.work.data_ptr()), nbytes, pk._stream()), "pg_batch_setup")
+
+    @staticmethod
+    def _fields(st) -> FastFields:
+        return FastFields(F=1, N=st.N, H=st.H, spherical=0, deposit_on_exit=int(bool(st.deposit_on_exit)),
+                          height=st.height.data_ptr(), sediment=st.sediment.data_ptr(),
+                          discharge=st.discharge.data_ptr(), momentum=st.momentum.data_ptr(),
+                          pending=st.pending.data_ptr(), base=st.base.data_ptr(), precip=st.precip.data_ptr(),
+                          evap=st.evap.data_ptr(), hardness=st.hardness.data_ptr(), metric=st.metric.data_ptr(),
+                          metric_inv=st.metric_inv.data_ptr(), mask=st.mask.data_ptr())
+
+    def _fparams(self, key):
+        ep = self.ep
+        out = (FastParams * self.W)()
+        for w in range(self.W):
+            slo, shi, ilo, ihi = fast.pcg64_state(self.params[w].rng(self.rng_stage, *key))
+            out[w] = FastParams(rng_state_lo=slo, rng_state_hi=shi, rng_inc_lo=ilo, rng_inc_hi=ihi,
+                                n_particles=self.P[w], chunks=self.chunks,
+                                iter_deposit=cell_units(ep, "iter_deposit", self.unit),
+                                iter_erode=cell_units(ep, "iter_erode", self.unit),
+                                conc_chunk=float(maps.chunk_size(ep, self.P[w])), fan_slope=float(ep.fan_slope),
+                                offshore_writeoff=float(ep.offshore_writeoff), ema=float(ep.ema),
+                                min_volume_frac=float(ep.min_volume_frac), height_bits=fast.HEIGHT_BITS,
+                                track_bits=fast.TRACK_BITS, grid_blocks=int(self.grid_blocks),
+                                time_trace=int(self.time_trace),
+                                rng_dev=None if self.rng_dev is None else self.rng_dev.data_ptr() + 32 * w)
+        return out
+
+    def step(self, iteration_key) -> None:
+        """One batched iteration of every window (asynchronous)."""
+        key = iteration_key if isinstance(iteration_key, (tuple, list)) else (int(iteration_key),)
+        check(lib().pg_batch_iterate(self.fields, self.W, ctypes.byref(self.prm), self._fparams(key),
+                                     ctypes.c_void_p(self.work.data_ptr()), self.work.numel(),
+                                     ctypes.c_void_p(self.stats.data_ptr()), ctypes.c_void_p(self.win_stats.data_ptr()),
+                                     pk._stream()), "pg_batch_iterate")
+        pk.apply_uplift(self.height, self.uplift, self.mask, None, 0.0)
+
+    def step_per_window(self, iteration_key) -> None:
+        """The same iteration as one fast-mode pass per window, one after
+        another (the reference's one-tile-at-a-time form, on the GPU)."""
+        key = iteration_key if isinstance(iteration_key, (tuple, list)) else (int(iteration_key),)
+        for w, st in enumerate(self.states):
+            st.fast_grid_blocks = self.grid_blocks
+            st.fast_time_trace = self.time_trace
+            fast.run_iteration(st, self.params[w], key, chunks=self.chunks, sync_stats=False)
+            maps.apply_uplift(st)
+
+    def moves(self, per_window: bool = False) -> int:
+        """Droplet moves of the last iteration (synchronises)."""
+        if per_window:
+            return int(sum(int(st._fast.stats[FS["MOVES"]]) for st in self.states))
+        return int(self.stats[FS["MOVES"]])
+
+
+def doe_val(win: dict) -> bool:
+    return bool(win.get("deposit_on_exit", False))
+
+
+class BatchGraph:
+    """The batched iteration captured once as a HIP graph; each replay reads
+    its windows' PCG64 states from a device buffer filled from the host."""
+
+    def __init__(self, batch: WindowBatch):
+        self.batch = batch
+        dev = batch.height.device
+        batch.rng_dev = torch.zeros((batch.W, 4), dtype=torch.int64, device=dev)
+        batch.time_trace = False
+        torch.cuda.synchronize()
+        self._load((0,))
+        self.graph = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(self.graph):
+            batch.step((0,))
+
+    def _load(self, key):
+        b = self.batch
+        words = [[_signed(x) for x in fast.pcg64_state(b.params[w].rng(b.rng_stage, *key))] for w in range(b.W)]
+        b.rng_dev.copy_(torch.tensor(words, dtype=torch.int64), non_blocking=False)
+
+    def step(self, iteration_key) -> None:
+        key = iteration_key if isinstance(iteration_key, (tuple, list)) else (int(iteration_key),)
+        self._load(key)
+        self.graph.replay()
+
+
+class Autotuner:
+    """Picks the persistent trace grid of a batch (or of a fast-mode state)
+    from real iterations: the first len(candidates) iterations each run one
+    candidate with the trace launches timed (HIP events on the launch
+    stream), then the fastest is kept.  Results are independent of the grid
+    (fixed-point accumulation), so no iteration is wasted."""
+
+    def __init__(self, target, candidates=None):
+        self.target = target
+        if candidates is None:
+            cus = torch.cuda.get_device_properties(0).multi_processor_count
+            candidates = [cus // 2, cus, 2 * cus, 3 * cus, 4 * cus]
+        self.candidates = [int(c) for c in candidates if int(c) > 0]
+        self.times: dict[int, float] = {}
+        self.best = None
+
+    def _set(self, grid: int, timed: bool):
+        t = self.target
+        if isinstance(t, WindowBatch):
+            t.grid_blocks, t.time_trace = grid, timed
+        else:
+            t.fast_grid_blocks, t.fast_time_trace = grid, timed
+
+    def step(self, run, iteration_key) -> None:
+        """run(iteration_key) performs one iteration of the target."""
+        if self.best is not None:
+            run(iteration_key)
+            return
+        cand = self.candidates[len(self.times)]
+        fast.trace_time()
+        self._set(cand, True)
+        run(iteration_key)
+        ms, _ = fast.trace_time()
+        self.times[cand] = ms
+        if len(self.times) == len(self.candidates):
+            self.best = min(self.times, key=self.times.get)
+            self._set(self.best, False)
+        else:
+            self._set(0, False)

@@ -127,3 +127,58 @@ def earth_globe(N: int = 1024, H: int = 2, seed: int = 0, octaves: int = 6, n_bl
     f["precip"] = rain.astype(np.float32)
     f["uplift_m"] = gaussian_blobs(grid, p.rng("stub", 3), n_blobs, 200.0e3, 1.0e-3, EARTH_RADIUS_M)
     return grid, f
+
+
+def value_noise_2d(NE: int, rng: np.random.Generator, octaves: int = 6, base: float = 4.0) -> np.ndarray:
+    """Normalised 2-D fBm of smoothstep-interpolated lattice value noise on
+    an NE x NE patch (lacunarity 2, gain 0.5), values in about [-1, 1]."""
+    out = np.zeros((NE, NE))
+    x = np.arange(NE) / NE
+    amp, tot = 1.0, 0.0
+    for o in range(octaves):
+        f = base * 2 ** o
+        n = int(np.ceil(f)) + 2
+        lat = rng.random((n + 1, n + 1)) * 2.0 - 1.0
+        q = x * f
+        i0 = np.floor(q).astype(np.int64)
+        t = q - i0
+        t = t * t * (3.0 - 2.0 * t)
+        a = lat[i0][:, i0] * (1 - t)[None, :] + lat[i0][:, i0 + 1] * t[None, :]
+        b = lat[i0 + 1][:, i0] * (1 - t)[None, :] + lat[i0 + 1][:, i0 + 1] * t[None, :]
+        out += amp * (a * (1 - t)[:, None] + b * t[:, None])
+        tot += amp
+        amp *= 0.5
+    return out / tot
+
+
+def zoom_windows(n: int = 64, seed: int = 0, side_lo: int = 128, side_hi: int = 1024, life_lo: int = 32,
+                 life_hi: int = 256, H: int = 2, cell_m: float = 50.0, relief_m: float = 400.0):
+    """Synthetic zoom windows: side log-uniform in [side_lo, side_hi] cells,
+    each with its own fBm bedrock seed (relief_m of relief on a 100 m
+    floor), halo H outside the mask, uniform rain, one droplet per cell and
+    a lifetime log-uniform in [life_lo, life_hi].  Returns (window keyword
+    arguments for ErosionState.window, WorldParams) per window: the plain
+    profile of workload_params with the window's seed and lifetime."""
+    rng = np.random.default_rng(seed)
+    sides = np.exp(rng.uniform(np.log(side_lo), np.log(side_hi), n)).astype(np.int64)
+    lives = np.exp(rng.uniform(np.log(life_lo), np.log(life_hi), n)).astype(np.int64)
+    out = []
+    for w in range(n):
+        N = int(sides[w])
+        NE = N + 2 * H
+        h = 100.0 + relief_m * (0.5 + 0.5 * value_noise_2d(NE, np.random.default_rng(seed * 1000 + w + 1)))
+        mask = np.zeros((NE, NE), np.uint8)
+        mask[H:H + N, H:H + N] = 1
+        metric = np.zeros((NE, NE, 3), np.float32)
+        metric[..., 0] = 1.0
+        metric[..., 2] = 1.0
+        z = np.zeros((NE, NE))
+        win = dict(height_m=h, sediment_m=z, discharge=z, momentum=np.zeros((NE, NE, 2)),
+                   hardness=np.zeros((NE, NE), np.float32), precip=np.ones((NE, NE), np.float32),
+                   evap=np.ones((NE, NE), np.float32), uplift_m=z, mask=mask, metric=metric, metric_inv=metric,
+                   H=H, height_unit_m=cell_m)
+        ep = workload_params(N, H, max_steps=int(lives[w])).erosion
+        p = WorldParams(world=WorldGroup(seed=seed * 1000 + w, N_c=N, cell_size_m=cell_m, land_fraction=0.3, halo=H),
+                        erosion=ep)
+        out.append((win, p))
+    return out

@@ -4,8 +4,10 @@ differences of the same frozen map and against the float64 adjoint.
 
 Bars: replay at the nominal parameters reproduces the nominal run bit for
 bit; <H^T a, x> = <a, H x> to 1e-12 relative; adjoint vs central
-differences of the frozen map (float64, relative step 1e-5; 1e-3 and 1e-6
-are printed beside it) within 2e-4 relative on every checked component;
+differences of the frozen map (float64, relative step 1e-6; 1e-3 and 1e-5
+are printed beside them: larger steps cross the kinks of the capped
+exchange) within 1e-4 relative on every checked component (plus 1e-5 of
+the largest for the field's small components);
 float32 adjoint vs float64 adjoint on the same trajectories within 1e-2.
 """
 from __future__ import annotations
@@ -97,14 +99,14 @@ def _fd_check(run, erod, theta, loss_of, cells, h_rel=1e-3):
     return base, np.array(fd_e), np.array(fd_t)
 
 
-@pytest.mark.parametrize("ce", [2, 6])
-def test_adjoint_matches_central_differences(cuda, ce):
+@pytest.mark.parametrize("ce,feedback", [(2, True), (6, True), (2, False)])
+def test_adjoint_matches_central_differences(cuda, ce, feedback):
     T = 6
     st, p = _state(32, cuda)
-    run = adj.GradRun(st, p, iterations=T, real="f64", checkpoint_every=ce)
+    run = adj.GradRun(st, p, iterations=T, real="f64", checkpoint_every=ce, feedback=feedback)
     erod = _erod(st).requires_grad_(True)
     theta = adj.theta_nominal(run).requires_grad_(True)
-    print(f"adjoint vs FD: N=32 T={T} checkpoint every {ce} f64", flush=True)
+    print(f"adjoint vs FD: N=32 T={T} checkpoint every {ce} feedback {feedback} f64", flush=True)
     surf = adj.erosion(erod, theta, run)
     land = (surf[st.interior].reshape(-1) > 0).detach()
     loss = adj.hypsometry_loss(surf, run, land=land)
@@ -123,20 +125,20 @@ def test_adjoint_matches_central_differences(cuda, ce):
     ad_e = ge[cells]
     print("erod  adjoint", ad_e, flush=True)
     print("theta adjoint", gt, flush=True)
-    # the frozen map is piecewise smooth (caps, the c_eq > sed switch): a
-    # step of 1e-3 crosses the kinks of the droplets nearest a switch, so
-    # the bar is on the converged step; 1e-3 is reported beside it
+    # the frozen map is piecewise smooth (caps, the c_eq > sed switch): steps
+    # of 1e-3 and 1e-5 cross the kinks of the droplets nearest a switch, so
+    # the bar is on the converged step; the others are printed beside it
     res = {}
     for h_rel in (1e-3, 1e-5, 1e-6):
         base, fd_e, fd_t = _fd_check(run, erod.detach(), theta.detach(), loss_of, cells, h_rel=h_rel)
         assert base == pytest.approx(float(loss), rel=1e-12)
         res[h_rel] = (fd_e, fd_t)
         print(f"h_rel {h_rel:g}: erod fd", fd_e, "\n  theta fd", fd_t, flush=True)
-    fd_e, fd_t = res[1e-5]
+    fd_e, fd_t = res[1e-6]
     scale_e = np.abs(ad_e).max()
-    assert np.all(np.abs(ad_e - fd_e) <= 2e-4 * np.abs(ad_e) + 1e-5 * scale_e), (ad_e, fd_e)
+    assert np.all(np.abs(ad_e - fd_e) <= 1e-4 * np.abs(ad_e) + 1e-5 * scale_e), (ad_e, fd_e)
     for i in range(6):
-        assert abs(gt[i] - fd_t[i]) <= 2e-4 * max(abs(gt[i]), abs(fd_t[i])) + 1e-9, (adj.THETA_NAMES[i], gt[i], fd_t[i])
+        assert abs(gt[i] - fd_t[i]) <= 1e-4 * max(abs(gt[i]), abs(fd_t[i])) + 1e-9, (adj.THETA_NAMES[i], gt[i], fd_t[i])
 
 
 def test_float32_adjoint_tracks_float64(cuda):
