@@ -69,9 +69,19 @@ def gpu_run(grid, f, p, steps, warmup, chunks):
     st.fast_chunks = chunks
     st.fast_sync_stats = False
     maps.start_replay(st, p)
+    # warm-up iterations double as the autotuner's trials of the persistent
+    # trace grid (the results do not depend on it)
+    from globe_hip.erosion import batch as bt
+
+    cus = torch.cuda.get_device_properties(0).multi_processor_count
+    tuner = bt.Autotuner(st, candidates=[2 * cus, 3 * cus, 4 * cus, 6 * cus][:max(warmup, 1)])
     log("warmup")
     for it in range(warmup):
-        maps.step(st, p, it)
+        tuner.step(lambda k: maps.step(st, p, k), it)
+    if tuner.best is None and tuner.times:
+        tuner.best = min(tuner.times, key=tuner.times.get)
+    st.fast_grid_blocks = tuner.best or 0
+    log(f"autotuned persistent grid: {st.fast_grid_blocks} blocks (trace ms per candidate {tuner.times})")
     torch.cuda.synchronize()
     fast.trace_time()  # drop warmup launches
     st.fast_time_trace = True
@@ -93,7 +103,7 @@ def gpu_run(grid, f, p, steps, warmup, chunks):
     log(f"{steps} iterations in {dt:.3f}s: {moves} moves, {particles} droplets, steps_mean {entries / max(particles, 1):.1f}; "
         f"trace {trace_ms:.1f} ms over {launches} launches")
     return dict(seconds=dt, moves=moves, particles=particles, entries=entries, trace_ms=trace_ms, launches=launches,
-                last=dec[-1])
+                last=dec[-1], grid_blocks=st.fast_grid_blocks, tune_ms=tuner.times)
 
 
 def cpu_baseline(grid, f, p, fraction=8):
@@ -253,7 +263,7 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
-    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--warmup", type=int, default=4)
     ap.add_argument("--N", type=int, default=1024)
     ap.add_argument("--chunks", type=int, default=16)
     ap.add_argument("--max-steps", type=int, default=128)
@@ -292,10 +302,11 @@ def main():
                                f"64 uplift blobs replayed, 1 droplet/cell, lifetime {a.max_steps}, fast mode "
                                f"{a.chunks} chunks/iteration",
                    "droplets_per_iteration": r["particles"] // a.steps, "moves": r["moves"],
-                   "steps_mean": r["entries"] / max(r["particles"], 1)},
+                   "steps_mean": r["entries"] / max(r["particles"], 1), "grid_blocks": r["grid_blocks"],
+                   "autotune_trace_ms": r["tune_ms"]},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "trace_fast_kernel",
+                     "kernel": "trace_kernel<false>",
                      "bytes_per_step": {"apply_pass": BYTES_PER_STEP, "census_pass": BYTES_PER_STEP_CENSUS},
                      "launch_ms_mean": r["trace_ms"] / max(r["launches"], 1), "launches": r["launches"]},
     }

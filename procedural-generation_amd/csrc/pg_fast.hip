@@ -22,11 +22,15 @@
 // Semantic differences from the reference (bake/globe/erosion/particle.py),
 // which applies each chunk's change list serially with caps that depend on
 // apply order: here all droplets of a chunk act on the same frozen terrain
-// and only order-independent caps remain -- the trace-time caps (relief /
-// expected visits, max_erode, never below sea level) and, per cell, the
-// iteration deposit budget and the waterline ceiling, whose excess goes to
-// the cell's stockpile.  The per-entry floors/ceilings of the next/previous
-// path cell and the deficit carry have no counterpart.  Plain profile only.
+// and the caps are order independent -- a census pass sums each cell's
+// erosion requests and the lowest surface stepped to from it, a per-cell
+// scale limits the cell to its iteration budget and that floor, and the
+// applying pass clamps each entry at its own next cell's frozen surface
+// (deficit carried to later deposits) and each deposit at the previous path
+// cell + dep_floor and the waterline (surplus carried on); the per-cell
+// resolve applies the deposit budget and stockpiles the excess.  Plain
+// profile only.  Batched windows (pg_batch_*) run the same kernels over an
+// arena of single-face windows, one persistent launch per chunk and pass.
 #include <hipcub/hipcub.hpp>
 
 #include <vector>
