@@ -74,14 +74,13 @@ def gpu_run(grid, f, p, steps, warmup, chunks):
     from globe_hip.erosion import batch as bt
 
     cus = torch.cuda.get_device_properties(0).multi_processor_count
-    tuner = bt.Autotuner(st, candidates=[2 * cus, 3 * cus, 4 * cus, 6 * cus][:max(warmup, 1)])
+    tuner = bt.Autotuner(st, candidates=[(2 * cus, 0), (4 * cus, 0), (3 * cus, 1), (4 * cus, 2)][:max(warmup, 1)])
     log("warmup")
     for it in range(warmup):
         tuner.step(lambda k: maps.step(st, p, k), it)
-    if tuner.best is None and tuner.times:
-        tuner.best = min(tuner.times, key=tuner.times.get)
-    st.fast_grid_blocks = tuner.best or 0
-    log(f"autotuned persistent grid: {st.fast_grid_blocks} blocks (trace ms per candidate {tuner.times})")
+    if tuner.best is None:
+        tuner.finish()
+    log(f"autotuned (grid, variant): {tuner.best} (trace ns per move per candidate {tuner.times})")
     torch.cuda.synchronize()
     fast.trace_time()  # drop warmup launches
     st.fast_time_trace = True
@@ -103,7 +102,8 @@ def gpu_run(grid, f, p, steps, warmup, chunks):
     log(f"{steps} iterations in {dt:.3f}s: {moves} moves, {particles} droplets, steps_mean {entries / max(particles, 1):.1f}; "
         f"trace {trace_ms:.1f} ms over {launches} launches")
     return dict(seconds=dt, moves=moves, particles=particles, entries=entries, trace_ms=trace_ms, launches=launches,
-                last=dec[-1], grid_blocks=st.fast_grid_blocks, tune_ms=tuner.times)
+                last=dec[-1], grid_blocks=st.fast_grid_blocks, variant=st.fast_variant,
+                tune_ms={f"{g}/{v}": ms for (g, v), ms in tuner.times.items()})
 
 
 def cpu_baseline(grid, f, p, fraction=8):
@@ -211,7 +211,7 @@ def batch_run(n=64, iterations=50, seed=0, per_window_iterations=3):
     while tuner.best is None:
         tuner.step(b.step, it)
         it += 1
-    log(f"autotuned persistent grid: {tuner.best} blocks (trace ms per candidate {tuner.times})")
+    log(f"autotuned (grid, variant): {tuner.best} (trace ns per move per candidate {tuner.times})")
     g = bt.BatchGraph(b)
     torch.cuda.synchronize()
     stats = []
@@ -223,7 +223,7 @@ def batch_run(n=64, iterations=50, seed=0, per_window_iterations=3):
     t_batch = time.perf_counter() - t0
     moves = sum(int(s[fast.FS["MOVES"]]) for s in stats)
     b2 = bt.WindowBatch([w for w, _ in wins], [p for _, p in wins], device="cuda")
-    b2.grid_blocks = tuner.best
+    # the plain form: each window a default fast-mode pass (occupancy grid, natural register budget)
     b2.step_per_window(0)  # warm-up: per-window workspaces
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -242,7 +242,8 @@ def batch_run(n=64, iterations=50, seed=0, per_window_iterations=3):
             "iterations": iterations, "batched_s": t_batch, "batched_ms_per_iteration": 1000 * per_it_b,
             "batched_moves_per_s": moves / t_batch, "per_window_ms_per_iteration": 1000 * per_it_pw,
             "per_window_moves_per_s": pw_moves / t_pw, "speedup": per_it_pw / per_it_b,
-            "grid_blocks": tuner.best, "autotune_trace_ms": tuner.times}
+            "grid_blocks": tuner.best[0], "trace_variant": tuner.best[1],
+            "autotune_trace_ns_per_move": {f"{g}/{v}": ms for (g, v), ms in tuner.times.items()}}
 
 
 def copy_probe(mib):
@@ -303,7 +304,8 @@ def main():
                                f"{a.chunks} chunks/iteration",
                    "droplets_per_iteration": r["particles"] // a.steps, "moves": r["moves"],
                    "steps_mean": r["entries"] / max(r["particles"], 1), "grid_blocks": r["grid_blocks"],
-                   "autotune_trace_ms": r["tune_ms"]},
+                   "trace_variant": r["variant"],
+                   "autotune_trace_ns_per_move": r["tune_ms"]},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
                      "kernel": "trace_kernel<false>",

@@ -190,6 +190,7 @@ typedef struct pg_fast_params {
     int64_t time_trace;    /* record HIP events around every trace launch (see pg_fast_trace_time) */
     const uint64_t* rng_dev; /* optional DEVICE copy of the four rng words, read by the kernels instead of the
                               * values above: a captured graph then replays with each iteration's own draws */
+    int64_t variant;       /* trace kernel register budget: 0 = 2 waves per SIMD, 1 = 3, 2 = 4 (same results) */
 } pg_fast_params;
 
 /* Statistics of one fast iteration (DEVICE int64[PG_FAST_NSTATS]). */

@@ -398,8 +398,10 @@ __device__ __forceinline__ float fkey_value(unsigned k) {
     return __uint_as_float((k & 0x80000000u) ? (k & 0x7fffffffu) : ~k);
 }
 
-template <bool BATCH>
-__global__ void __launch_bounds__(TRACE_TPB) trace_kernel(TraceCfg k) {
+// WPE: the register budget as waves per SIMD (2 = the kernel's natural
+// allocation; 3 and 4 trade spills for latency hiding; the autotuner picks)
+template <bool BATCH, int WPE>
+__global__ void __launch_bounds__(TRACE_TPB) __attribute__((amdgpu_waves_per_eu(WPE, WPE))) trace_kernel(TraceCfg k) {
     const int lane = threadIdx.x & 63;
     const bool apply = k.pass == 1;
     // per-lane geometry and droplet scalars: the grid's (one launch = one
@@ -743,6 +745,21 @@ __global__ void __launch_bounds__(TRACE_TPB) trace_kernel(TraceCfg k) {
         for (int c = 0; c < N_DEATHS; c++)
             if (v[5 + c]) atomicAdd((ull*)&k.stats[PG_FS_DEATHS + c], (ull)v[5 + c]);
     }
+}
+
+// the trace kernel of a register-budget variant (0: 2 waves per SIMD, 1: 3, 2: 4)
+template <bool BATCH>
+const void* trace_fn(int64_t variant) {
+    if (variant == 1) return (const void*)trace_kernel<BATCH, 3>;
+    if (variant == 2) return (const void*)trace_kernel<BATCH, 4>;
+    return (const void*)trace_kernel<BATCH, 2>;
+}
+
+template <bool BATCH>
+void launch_trace(int64_t variant, int grid, hipStream_t s, const TraceCfg& k) {
+    if (variant == 1) hipLaunchKernelGGL((trace_kernel<BATCH, 3>), dim3(grid), dim3(TRACE_TPB), 0, s, k);
+    else if (variant == 2) hipLaunchKernelGGL((trace_kernel<BATCH, 4>), dim3(grid), dim3(TRACE_TPB), 0, s, k);
+    else hipLaunchKernelGGL((trace_kernel<BATCH, 2>), dim3(grid), dim3(TRACE_TPB), 0, s, k);
 }
 
 // ---------------------------------------------------------------------------
@@ -1259,13 +1276,14 @@ int fast_particles(const pg_fast_fields* fl, const pg_trace_params* prm, const p
     k.mom_fx = at<ll>(work, L.mom_fx);
     k.stats = st;
     int grid = 0;
-    if (persistent_grid((const void*)trace_kernel<false>, TRACE_TPB, fp->grid_blocks, &grid)) return -1;
+    PG_REQUIRE(fp->variant >= 0 && fp->variant <= 2, "fast mode: variant must be 0, 1 or 2");
+    if (persistent_grid(trace_fn<false>(fp->variant), TRACE_TPB, fp->grid_blocks, &grid)) return -1;
     for (int c = 0; c < chunks; c++) {
         k.chunk = c;
         // pass 0: erosion census on the frozen terrain
         k.pass = 0;
         if (fp->time_trace && timer_record(s, true)) return -1;
-        hipLaunchKernelGGL(trace_kernel<false>, dim3(grid), dim3(TRACE_TPB), 0, s, k);
+        launch_trace<false>(fp->variant, grid, s, k);
         if (fp->time_trace && timer_record(s, false)) return -1;
         hipLaunchKernelGGL(erosion_scale_kernel, dim3(grid_for(n)), dim3(TPB), 0, s, n, fl->height, fl->sediment, fl->base,
                            k.ero_fx, k.floor_key, at<ll>(work, L.acc_fx), at<float>(work, L.r1), fp->iter_erode, hscale);
@@ -1275,7 +1293,7 @@ int fast_particles(const pg_fast_fields* fl, const pg_trace_params* prm, const p
         // pass 1: the same trajectories, erosion scaled per cell, deficits cancel deposits
         k.pass = 1;
         if (fp->time_trace && timer_record(s, true)) return -1;
-        hipLaunchKernelGGL(trace_kernel<false>, dim3(grid), dim3(TRACE_TPB), 0, s, k);
+        launch_trace<false>(fp->variant, grid, s, k);
         if (fp->time_trace && timer_record(s, false)) return -1;
         hipLaunchKernelGGL(resolve_kernel, dim3(grid_for(n)), dim3(TPB), 0, s, n, fl->height, fl->sediment, fl->base,
                            k.chunk_fx, at<ll>(work, L.acc_fx), pend_fx, cell4, dyn, fp->iter_deposit, prm->dep_floor,
@@ -1657,13 +1675,14 @@ extern "C" int pg_batch_iterate(const pg_fast_fields* wins, int64_t n_win, const
     k.n_win = (int)n_win;
     k.queue_base = qbase;
     int grid = 0;
-    if (persistent_grid((const void*)trace_kernel<true>, TRACE_TPB, f0.grid_blocks, &grid)) return -1;
+    PG_REQUIRE(f0.variant >= 0 && f0.variant <= 2, "pg_batch_iterate: variant must be 0, 1 or 2");
+    if (persistent_grid(trace_fn<true>(f0.variant), TRACE_TPB, f0.grid_blocks, &grid)) return -1;
     const pg_fast_fields& a = wins[0];  // arena bases
     for (int c = 0; c < chunks; c++) {
         k.chunk = c;
         k.pass = 0;
         if (f0.time_trace && timer_record(s, true)) return -1;
-        hipLaunchKernelGGL(trace_kernel<true>, dim3(grid), dim3(TRACE_TPB), 0, s, k);
+        launch_trace<true>(f0.variant, grid, s, k);
         if (f0.time_trace && timer_record(s, false)) return -1;
         hipLaunchKernelGGL(erosion_scale_kernel, dim3(grid_for(n)), dim3(TPB), 0, s, n, a.height, a.sediment, a.base,
                            k.ero_fx, k.floor_key, at<ll>(work, L.acc_fx), at<float>(work, L.r1), f0.iter_erode, hscale);
@@ -1672,7 +1691,7 @@ extern "C" int pg_batch_iterate(const pg_fast_fields* wins, int64_t n_win, const
                            at<float>(work, L.rscale), f0.iter_erode, hscale);
         k.pass = 1;
         if (f0.time_trace && timer_record(s, true)) return -1;
-        hipLaunchKernelGGL(trace_kernel<true>, dim3(grid), dim3(TRACE_TPB), 0, s, k);
+        launch_trace<true>(f0.variant, grid, s, k);
         if (f0.time_trace && timer_record(s, false)) return -1;
         hipLaunchKernelGGL(resolve_kernel, dim3(grid_for(n)), dim3(TPB), 0, s, n, a.height, a.sediment, a.base,
                            k.chunk_fx, at<ll>(work, L.acc_fx), at<ll>(work, L.pend_fx), at<float4>(work, L.cell4),

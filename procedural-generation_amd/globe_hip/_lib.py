@@ -41,7 +41,7 @@ class FastParams(ctypes.Structure):
                 ("rng_inc_hi", ctypes.c_uint64), ("n_particles", i64), ("chunks", i64), ("iter_deposit", dbl),
                 ("iter_erode", dbl), ("conc_chunk", dbl), ("fan_slope", dbl),
                 ("offshore_writeoff", dbl), ("ema", dbl), ("min_volume_frac", dbl), ("height_bits", i64),
-                ("track_bits", i64), ("grid_blocks", i64), ("time_trace", i64), ("rng_dev", vp)]
+                ("track_bits", i64), ("grid_blocks", i64), ("time_trace", i64), ("rng_dev", vp), ("variant", i64)]
 
 
 class GradArgs(ctypes.Structure):

@@ -129,6 +129,7 @@ class WindowBatch:
         self.stats = torch.zeros(FAST_NSTATS, dtype=torch.int64, device=dev)
         self.win_stats = torch.zeros((self.W, FAST_NSTATS), dtype=torch.int64, device=dev)
         self.grid_blocks = 0
+        self.variant = 0
         self.time_trace = False
         self.rng_dev = None
         check(lib().pg_batch_setup(self.fields, self.W, self._fparams((0,)), self.steps_arr,
@@ -156,7 +157,7 @@ class WindowBatch:
                                 offshore_writeoff=float(ep.offshore_writeoff), ema=float(ep.ema),
                                 min_volume_frac=float(ep.min_volume_frac), height_bits=fast.HEIGHT_BITS,
                                 track_bits=fast.TRACK_BITS, grid_blocks=int(self.grid_blocks),
-                                time_trace=int(self.time_trace),
+                                time_trace=int(self.time_trace), variant=int(self.variant),
                                 rng_dev=None if self.rng_dev is None else self.rng_dev.data_ptr() + 32 * w)
         return out
 
@@ -175,6 +176,7 @@ class WindowBatch:
         key = iteration_key if isinstance(iteration_key, (tuple, list)) else (int(iteration_key),)
         for w, st in enumerate(self.states):
             st.fast_grid_blocks = self.grid_blocks
+            st.fast_variant = self.variant
             st.fast_time_trace = self.time_trace
             fast.run_iteration(st, self.params[w], key, chunks=self.chunks, sync_stats=False)
             maps.apply_uplift(st)
@@ -217,27 +219,33 @@ class BatchGraph:
 
 
 class Autotuner:
-    """Picks the persistent trace grid of a batch (or of a fast-mode state)
+    """Picks the persistent trace grid and the trace kernel's register-budget
+    variant (2, 3 or 4 waves per SIMD) of a batch or of a fast-mode state
     from real iterations: the first len(candidates) iterations each run one
-    candidate with the trace launches timed (HIP events on the launch
-    stream), then the fastest is kept.  Results are independent of the grid
-    (fixed-point accumulation), so no iteration is wasted."""
+    (grid, variant) candidate with the trace launches timed (HIP events on
+    the launch stream) and normalised by the iteration's droplet moves (the
+    iterations differ), then the fastest per move is kept.  Results are
+    independent of both (fixed-point accumulation), so no iteration is
+    wasted."""
 
     def __init__(self, target, candidates=None):
         self.target = target
         if candidates is None:
             cus = torch.cuda.get_device_properties(0).multi_processor_count
-            candidates = [cus // 2, cus, 2 * cus, 3 * cus, 4 * cus]
-        self.candidates = [int(c) for c in candidates if int(c) > 0]
-        self.times: dict[int, float] = {}
+            candidates = [(g * cus, v) for v in (0, 1, 2) for g in (2, 4)]
+        self.candidates = [(int(c[0]), int(c[1])) if isinstance(c, (tuple, list)) else (int(c), 0)
+                           for c in candidates]
+        self.candidates = [c for c in self.candidates if c[0] > 0 and c[1] in (0, 1, 2)]
+        self.times: dict = {}
         self.best = None
 
-    def _set(self, grid: int, timed: bool):
+    def _set(self, cand, timed: bool):
+        grid, variant = cand
         t = self.target
         if isinstance(t, WindowBatch):
-            t.grid_blocks, t.time_trace = grid, timed
+            t.grid_blocks, t.variant, t.time_trace = grid, variant, timed
         else:
-            t.fast_grid_blocks, t.fast_time_trace = grid, timed
+            t.fast_grid_blocks, t.fast_variant, t.fast_time_trace = grid, variant, timed
 
     def step(self, run, iteration_key) -> None:
         """run(iteration_key) performs one iteration of the target."""
@@ -249,9 +257,17 @@ class Autotuner:
         self._set(cand, True)
         run(iteration_key)
         ms, _ = fast.trace_time()
-        self.times[cand] = ms
+        t = self.target
+        moves = t.moves() if isinstance(t, WindowBatch) else int(t._fast.stats[FS["MOVES"]])
+        self.times[cand] = ms / max(moves, 1) * 1e6  # ns per move
         if len(self.times) == len(self.candidates):
+            self.finish()
+        else:
+            self._set((0, 0), False)
+
+    def finish(self):
+        """Keep the fastest candidate measured so far."""
+        if self.times:
             self.best = min(self.times, key=self.times.get)
             self._set(self.best, False)
-        else:
-            self._set(0, False)
+        return self.best

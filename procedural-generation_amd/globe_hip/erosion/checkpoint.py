@@ -1,0 +1,119 @@
+"""Erosion checkpoints in the reference's file layout
+(bake/globe/erosion/run.py:99-225): ``erosion_iterNNNN.npz`` with the state
+arrays (height, sediment, discharge, momentum, pending, lake_load; cell
+units, float64) beside ``erosion_iterNNNN.json`` with ``iteration``,
+``params_hash``, ``height_unit_m`` and ``time``; the two newest checkpoints of
+a parameter family are kept.  A device state is written from and read back
+into its tensors, so a run of this path and a run of the reference resume
+from each other's files.
+
+The parameter hash is the caller's (the reference computes it from its
+WorldParams, stage hashes and kernel version, erosion/run.py:68-97).
+"""
+from __future__ import annotations
+
+import json
+import re
+import time
+from pathlib import Path
+
+import numpy as np
+import torch
+
+ARRAYS = ("height", "sediment", "discharge", "momentum", "pending", "lake_load")
+_CKPT_RE = re.compile(r"erosion_iter(\d+)\.npz$")
+
+
+def _host(t):
+    return t.detach().cpu().numpy() if isinstance(t, torch.Tensor) else np.asarray(t)
+
+
+def save_checkpoint(directory, state, params_hash: str, keep: int = 2) -> Path:
+    """Write state's arrays at state.iteration (atomically: a .tmp file
+    renamed into place, then the .json), then prune this family to `keep`."""
+    d = Path(directory)
+    d.mkdir(parents=True, exist_ok=True)
+    it = int(state.iteration)
+    p = d / f"erosion_iter{it:04d}.npz"
+    tmp = p.with_suffix(".npz.tmp")
+    arrays = {}
+    for n in ARRAYS:
+        t = getattr(state, n, None)
+        if t is None or (isinstance(t, torch.Tensor) and t.numel() == 0):
+            continue
+        arrays[n] = _host(t)
+    if getattr(state, "route", None) is not None:
+        arrays["route"] = _host(state.route)
+    with open(tmp, "wb") as fh:
+        np.savez(fh, **arrays)
+    tmp.replace(p)
+    meta = {"iteration": it, "params_hash": params_hash, "height_unit_m": float(state.height_unit_m),
+            "time": time.strftime("%Y-%m-%dT%H:%M:%S")}
+    with open(p.with_suffix(".json"), "w") as fh:
+        json.dump(meta, fh, indent=1)
+    _prune(d, params_hash, keep)
+    return p
+
+
+def _prune(d: Path, params_hash: str, keep: int) -> None:
+    mine = []
+    for q in d.glob("erosion_iter*.npz"):
+        m = _CKPT_RE.search(q.name)
+        jq = q.with_suffix(".json")
+        if not m or not jq.exists():
+            continue
+        try:
+            if json.loads(jq.read_text()).get("params_hash") != params_hash:
+                continue
+        except json.JSONDecodeError:
+            continue
+        mine.append((int(m.group(1)), q))
+    for _, q in sorted(mine)[: max(0, len(mine) - keep)]:
+        q.unlink(missing_ok=True)
+        q.with_suffix(".json").unlink(missing_ok=True)
+
+
+def find_checkpoint(directory, params_hash: str, max_iteration: int | None = None):
+    """(path, meta) of the newest checkpoint of this parameter family with
+    iteration <= max_iteration, or None."""
+    d = Path(directory)
+    if not d.exists():
+        return None
+    best = None
+    for p in d.glob("erosion_iter*.npz"):
+        if not _CKPT_RE.search(p.name):
+            continue
+        jp = p.with_suffix(".json")
+        if not jp.exists():
+            continue
+        try:
+            meta = json.loads(jp.read_text())
+        except json.JSONDecodeError:
+            continue
+        if meta.get("params_hash") != params_hash:
+            continue
+        it = int(meta.get("iteration", -1))
+        if max_iteration is not None and it > max_iteration:
+            continue
+        if best is None or it > best[1]["iteration"]:
+            best = (p, meta)
+    return best
+
+
+def load_checkpoint(state, path, meta: dict) -> None:
+    """Copy a checkpoint's arrays into state's tensors (absent pending /
+    lake_load read as zero) and set state.iteration."""
+    if abs(float(meta.get("height_unit_m", state.height_unit_m)) - float(state.height_unit_m)) > 1e-9 * float(state.height_unit_m):
+        raise ValueError("checkpoint written with another cell size")
+    with np.load(path) as z:
+        for n in ARRAYS:
+            t = getattr(state, n, None)
+            if t is None or (isinstance(t, torch.Tensor) and t.numel() == 0):
+                continue
+            src = z[n] if n in z.files else np.zeros(tuple(t.shape))
+            if tuple(src.shape) != tuple(t.shape):
+                raise ValueError(f"{n}: checkpoint shape {src.shape}, state {tuple(t.shape)}")
+            t.copy_(torch.from_numpy(np.ascontiguousarray(src)).to(t.dtype))
+        if "route" in z.files and getattr(state, "route", None) is not None:
+            state.route.copy_(torch.from_numpy(np.ascontiguousarray(z["route"])))
+    state.iteration = int(meta["iteration"])

@@ -131,6 +131,7 @@ def run_iteration(state, params, iteration_key, n_particles=None, particles_per_
                     ema=float(ep.ema), min_volume_frac=float(ep.min_volume_frac), height_bits=HEIGHT_BITS,
                     track_bits=TRACK_BITS, grid_blocks=int(getattr(state, "fast_grid_blocks", 0)),
                     time_trace=int(bool(getattr(state, "fast_time_trace", False))),
+                    variant=int(getattr(state, "fast_variant", 0)),
                     rng_dev=(state._fast_rng_dev.data_ptr() if getattr(state, "_fast_rng_dev", None) is not None else None))
     check(lib().pg_fast_iterate(ctypes.byref(ctx.fields), ctypes.byref(prm), ctypes.byref(fp),
                                 ctypes.c_void_p(ctx.work.data_ptr()), ctx.work.numel(),

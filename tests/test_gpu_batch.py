@@ -52,8 +52,8 @@ def test_batch_graph_replays_match_eager(cuda):
 def test_autotuner_keeps_results(cuda):
     a, b = _batch(cuda, seed=7), _batch(cuda, seed=7)
     cus = torch.cuda.get_device_properties(0).multi_processor_count
-    tuner = bt.Autotuner(b, candidates=[cus // 2, cus, 2 * cus])
-    for it in range(4):
+    tuner = bt.Autotuner(b, candidates=[(cus, 0), (2 * cus, 1), (2 * cus, 2)])
+    for it in range(5):
         a.step(it)
         tuner.step(b.step, it)
         torch.cuda.synchronize()

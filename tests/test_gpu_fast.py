@@ -204,3 +204,43 @@ def test_graph_replays_match_eager(cuda):
             assert np.array_equal(a, b), (it, name, float(np.abs(a - b).max()))
         s = fast.decode_stats(runner.stats.cpu().numpy(), 2.0 ** fast.HEIGHT_BITS)
         assert s["moves"] > 0 and s["particles"] > 0
+
+
+def test_checkpoint_resume_is_bitwise(cuda, tmp_path):
+    """A fast-mode run checkpointed after 2 iterations (the reference's
+    .npz / .json layout) and resumed into a fresh state continues bit for
+    bit like the uninterrupted run."""
+    from globe_hip.erosion import checkpoint as ck
+
+    N = 32
+    grid, f = stubs.plain_globe(N, 2)
+    p = stubs.workload_params(N, 2)
+    mk = lambda: maps.ErosionState.from_grid(grid, f["bedrock_m"], f["hardness"], f["precip"], f["evap"], f["uplift_m"],
+                                             p.erosion, device=cuda, mode="fast")
+    a, b = mk(), mk()
+    for it in range(2):
+        maps.step(a, p, it)
+    ck.save_checkpoint(tmp_path, a, "plain")
+    ck.load_checkpoint(b, *ck.find_checkpoint(tmp_path, "plain"))
+    assert b.iteration == 2
+    for it in range(2, 4):
+        maps.step(a, p, it)
+        maps.step(b, p, it)
+    for n in ("height", "sediment", "discharge", "momentum", "pending"):
+        assert torch.equal(getattr(a, n), getattr(b, n)), n
+
+
+def test_facefield_load_fills_halos_like_the_exchange(cuda, tmp_path):
+    from globe_hip.field import FaceField
+
+    N = 16
+    grid, f = stubs.plain_globe(N, 2)
+    st = maps.ErosionState.from_grid(grid, f["bedrock_m"], f["hardness"], f["precip"], f["evap"], f["uplift_m"],
+                                     stubs.workload_params(N, 2).erosion, device=cuda)
+    st.exchange_halos()
+    out = st.fields(("height",))["height"]
+    out.save(tmp_path)
+    back = FaceField.load(tmp_path, "height", grid, device=cuda)
+    dev = FaceField(grid, torch.from_numpy(out.data.astype(np.float64)).to(cuda))
+    dev.exchange_halos()
+    assert np.array_equal(back.data, dev.data.cpu().numpy().astype(np.float32))
