@@ -41,6 +41,10 @@ METRIC = "droplet-steps/s at N_c=1024 vs numba host; fwd+adjoint/fwd ratio"
 BYTES_PER_STEP = 164.0
 BYTES_PER_STEP_CENSUS = 148.0
 HBM_PEAK_GBS = 8000.0
+#: HBM bytes per trace launch of this workload (N_c=1024, 16 chunks) by register-budget variant, from
+#: rocprofv3 FETCH_SIZE (x 2.000, calibrated on a 1-GiB copy) + WRITE_SIZE (x 1.000), mean over both
+#: passes; profiles/trace_kernel_earth_n1024_counters.md holds the runs
+TRACE_TRAFFIC_BYTES = {0: 2.890e9, 1: 7.740e9, 2: 17.990e9}
 
 
 def log(msg):
@@ -307,7 +311,8 @@ def main():
                    "trace_variant": r["variant"],
                    "autotune_trace_ns_per_move": r["tune_ms"]},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": TRACE_TRAFFIC_BYTES.get(r["variant"]) if (a.N, a.chunks) == (1024, 16) else None,
                      "kernel": "trace_kernel<false>",
                      "bytes_per_step": {"apply_pass": BYTES_PER_STEP, "census_pass": BYTES_PER_STEP_CENSUS},
                      "launch_ms_mean": r["trace_ms"] / max(r["launches"], 1), "launches": r["launches"]},

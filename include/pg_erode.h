@@ -234,10 +234,8 @@ int pg_fast_particles(const pg_fast_fields* fields, const pg_trace_params* prm, 
 
 /* Byte offsets inside a fast workspace, for reading it: offsets[0..15] =
  * stencil records (float4), geometry (2 x float4), {sediment, base}
- * (float2), per-cell accumulator records (4 x int64: chunk change,
- * discharge, momentum a, b), iteration change (int64), per-cell census
- * records (int64 erosion, uint64 floor key), erosion scale (float32),
- * stockpile (int64), droplet cell (int32), x / y fraction (float32), load
+ * (float2), chunk / iteration / discharge / momentum / stockpile fixed
+ * point (int64), droplet cell (int32), x / y fraction (float32), load
  * (int64), direction a / b (float32), scalars, total size. */
 int pg_fast_layout(int64_t F, int64_t NE, int64_t n_particles, int64_t* offsets, int n);
 

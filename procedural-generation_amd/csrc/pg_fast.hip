@@ -367,8 +367,12 @@ struct TraceCfg {
         erodibility, cover_depth, max_erode, ocean_rate, fan_room, dep_floor, fan_slope, min_volume_frac;
     int max_steps, pit_steps, ocean_steps;
     double hscale, tscale, conc_chunk;
-    CellAcc* trk;     // pass 1: net applied change of the chunk, discharge and momentum tracks
-    CellCensus* cen;  // pass 0: erosion requested, lowest surface stepped to
+    ll* chunk_fx;   // pass 1: net applied change per cell
+    ll* ero_fx;     // pass 0: erosion requested per cell
+    ull* floor_key;       // pass 0: lowest (surface key << 32 | cell) stepped to from a cell
+    ll* dis_fx;
+    ll* mom_fx;    // momentum track a; track b in its own plane (a move's atomics on separate lines)
+    ll* mom_fx_b;
     ll* stats;
     // batched windows: per-window geometry and spawn totals, per-chunk union
     // offsets (n_win + 1 per chunk), the batch's queue counters
@@ -464,7 +468,7 @@ __global__ void __launch_bounds__(TRACE_TPB) __attribute__((amdgpu_waves_per_eu(
         ll dep = want < lim ? want : lim;
         if (dep < 0) dep = 0;
         surplus = want - dep;
-        if (dep) atomicAdd((ull*)&k.trk[cell].chunk, (ull)dep);
+        if (dep) atomicAdd((ull*)&k.chunk_fx[cell], (ull)dep);
         real -= dep;
     };
     // ceiling of a step deposit on the frozen terrain: the previous path cell
@@ -647,8 +651,8 @@ __global__ void __launch_bounds__(TRACE_TPB) __attribute__((amdgpu_waves_per_eu(
                                 }
                                 if (!apply) {
                                     if (dq > 0) {
-                                        atomicAdd((ull*)&k.cen[cur.cell].ero, (ull)dq);
-                                        atomicMin(&k.cen[cur.cell].floor_key, ((ull)fkey(nrec.x) << 32) | (ull)(unsigned)ncell);
+                                        atomicAdd((ull*)&k.ero_fx[cur.cell], (ull)dq);
+                                        atomicMin(&k.floor_key[cur.cell], ((ull)fkey(nrec.x) << 32) | (ull)(unsigned)ncell);
                                     }
                                 } else {
                                     deficit += cut;
@@ -656,14 +660,13 @@ __global__ void __launch_bounds__(TRACE_TPB) __attribute__((amdgpu_waves_per_eu(
                                         const ll taken = (ll)floor((double)dq * (double)cur.rs);
                                         deficit += dq - taken;
                                         real += taken;
-                                        if (taken) atomicAdd((ull*)&k.trk[cur.cell].chunk, (ull)(-taken));
+                                        if (taken) atomicAdd((ull*)&k.chunk_fx[cur.cell], (ull)(-taken));
                                     } else if (dq < 0) {
                                         deposit(cur.cell, -dq, step_ceiling(h_here, b_here));
                                     }
-                                    CellAcc* t = &k.trk[cur.cell];
-                                    atomicAdd((ull*)&t->dis, (ull)llrint((double)vol * k.tscale));
-                                    atomicAdd((ull*)&t->ma, (ull)llrint((double)(vol * sa) * k.tscale));
-                                    atomicAdd((ull*)&t->mb, (ull)llrint((double)(vol * sb) * k.tscale));
+                                    atomicAdd((ull*)&k.dis_fx[cur.cell], (ull)llrint((double)vol * k.tscale));
+                                    atomicAdd((ull*)&k.mom_fx[cur.cell], (ull)llrint((double)(vol * sa) * k.tscale));
+                                    atomicAdd((ull*)&k.mom_fx_b[cur.cell], (ull)llrint((double)(vol * sb) * k.tscale));
                                     n_entries += 1;
                                 }
                             }
@@ -718,7 +721,7 @@ __global__ void __launch_bounds__(TRACE_TPB) __attribute__((amdgpu_waves_per_eu(
             }
             if (pushes > 0 && (!exited || k.deposit_on_exit)) {
                 if (sed > 0.0f) n_entries += cause == DEATH_OCEAN ? 1 : (pushes < SPREAD ? pushes : SPREAD);
-                if (real) atomicAdd((ull*)&k.trk[last_cell].chunk, (ull)real);
+                if (real) atomicAdd((ull*)&k.chunk_fx[last_cell], (ull)real);
             } else if (exited) {
                 exit_fx += real;
             } else {
@@ -783,12 +786,12 @@ __device__ __forceinline__ float erosion_scale(int64_t c, const double* height, 
 
 // sweep 1: floor = the lowest surface stepped to (frozen)
 __global__ void erosion_scale_kernel(int64_t n, const double* height, const double* sediment, const double* base,
-                                     const CellCensus* cen, const ll* acc_fx, float* r1,
+                                     const ll* ero_fx, const ull* floor_key, const ll* acc_fx, float* r1,
                                      double iter_erode, double hscale) {
     const double inv = 1.0 / hscale;
     GRID_STRIDE(c, n) {
-        const ll e = cen[c].ero;
-        const ull fk = cen[c].floor_key;
+        const ll e = ero_fx[c];
+        const ull fk = floor_key[c];
         r1[c] = e > 0 ? erosion_scale(c, height, sediment, base, acc_fx, e,
                                       fk != ~0ull ? (double)fkey_value((unsigned)(fk >> 32)) : -1.0e300, iter_erode, inv)
                       : 1.0f;
@@ -797,18 +800,18 @@ __global__ void erosion_scale_kernel(int64_t n, const double* height, const doub
 
 // sweep 2: that floor lowered by what sweep 1 lets the floor cell lose
 __global__ void erosion_scale2_kernel(int64_t n, const double* height, const double* sediment, const double* base,
-                                      const CellCensus* cen, const ll* acc_fx, const float* r1,
+                                      const ll* ero_fx, const ull* floor_key, const ll* acc_fx, const float* r1,
                                       float* rscale, double iter_erode, double hscale) {
     const double inv = 1.0 / hscale;
     GRID_STRIDE(c, n) {
-        const ll e = cen[c].ero;
-        const ull fk = cen[c].floor_key;
+        const ll e = ero_fx[c];
+        const ull fk = floor_key[c];
         float r = 1.0f;
         if (e > 0) {
             double fl_next = -1.0e300;
             if (fk != ~0ull) {
                 const int64_t nc = (int64_t)(unsigned)(fk & 0xffffffffull);
-                fl_next = (double)fkey_value((unsigned)(fk >> 32)) - (double)cen[nc].ero * inv * (double)r1[nc];
+                fl_next = (double)fkey_value((unsigned)(fk >> 32)) - (double)ero_fx[nc] * inv * (double)r1[nc];
             }
             r = erosion_scale(c, height, sediment, base, acc_fx, e, fl_next, iter_erode, inv);
         }
@@ -816,27 +819,20 @@ __global__ void erosion_scale2_kernel(int64_t n, const double* height, const dou
     }
 }
 
-__global__ void census_init_kernel(int64_t n, CellCensus* cen) {
-    GRID_STRIDE(c, n) {
-        cen[c].ero = 0;
-        cen[c].floor_key = ~0ull;
-    }
-}
-
 // ---------------------------------------------------------------------------
 // per-cell resolve of a chunk against the live terrain
 // ---------------------------------------------------------------------------
-__global__ void resolve_kernel(int64_t n, const double* height, const double* sediment, const double* base, CellAcc* trk,
+__global__ void resolve_kernel(int64_t n, const double* height, const double* sediment, const double* base, ll* chunk_fx,
                                ll* acc_fx, ll* pend_fx, float4* cell4, float2* dyn, double iter_deposit,
-                               double dep_floor, double writeoff, double hscale, ll* stats, CellCensus* cen) {
+                               double dep_floor, double writeoff, double hscale, ll* stats, ll* ero_fx, ull* floor_key) {
     const double inv = 1.0 / hscale;
     ll clamped = 0, to_pend = 0, lost_off = 0;
     GRID_STRIDE(c, n) {
-        cen[c].ero = 0;
-        cen[c].floor_key = ~0ull;
-        const ll net = trk[c].chunk;
+        ero_fx[c] = 0;
+        floor_key[c] = ~0ull;
+        const ll net = chunk_fx[c];
         if (net == 0) continue;
-        trk[c].chunk = 0;
+        chunk_fx[c] = 0;
         const ll acc = acc_fx[c];
         const double s_c = height[c] + sediment[c] + (double)acc * inv;
         ll applied = net;
@@ -879,7 +875,7 @@ __global__ void resolve_kernel(int64_t n, const double* height, const double* se
 
 // fold (particle.py:1079-1101) + EMA (particle.py:1230-1242) + pending mirror
 __global__ void finish_kernel(int64_t n, double* height, double* sediment, double* discharge, double* momentum,
-                              double* pending, const uint8_t* mask, ll* acc_fx, CellAcc* trk, const ll* pend_fx,
+                              double* pending, const uint8_t* mask, ll* acc_fx, ll* dis_fx, ll* mom_fx, const ll* pend_fx,
                               double ema, double hscale, double tscale, ll* stats) {
     const double hinv = 1.0 / hscale, tinv = 1.0 / tscale;
     ll eroded = 0, deposited = 0;
@@ -903,15 +899,15 @@ __global__ void finish_kernel(int64_t n, double* height, double* sediment, doubl
                 }
             }
         }
-        const ll d = trk[c].dis, m0 = trk[c].ma, m1 = trk[c].mb;
+        const ll d = dis_fx[c], m0 = mom_fx[c], m1 = mom_fx[n + c];
         if (active) {
             discharge[c] = (1.0 - ema) * discharge[c] + ema * ((double)d * tinv);
             momentum[2 * c] = (1.0 - ema) * momentum[2 * c] + ema * ((double)m0 * tinv);
             momentum[2 * c + 1] = (1.0 - ema) * momentum[2 * c + 1] + ema * ((double)m1 * tinv);
         }
-        trk[c].dis = 0;
-        trk[c].ma = 0;
-        trk[c].mb = 0;
+        dis_fx[c] = 0;
+        mom_fx[c] = 0;
+        mom_fx[n + c] = 0;
         pending[c] = (double)pend_fx[c] * hinv;
     }
     eroded = wave_sum(eroded);
@@ -1018,10 +1014,13 @@ int layout_of(int64_t F, int64_t NE, int64_t P, Layout* L) {
     L->cell4 = take(16 * n);
     L->geo = take(32 * n);
     L->dyn = take(8 * n);
-    L->trk = take(sizeof(CellAcc) * n);
+    L->chunk_fx = take(8 * n);
     L->acc_fx = take(8 * n);
+    L->dis_fx = take(8 * n);
+    L->mom_fx = take(16 * n);
     L->pend_fx = take(8 * n);
-    L->cen = take(sizeof(CellCensus) * n);
+    L->ero_fx = take(8 * n);
+    L->floor_key = take(8 * n);
     L->rscale = take(4 * n);
     L->r1 = take(4 * n);
     L->wq = take(8 * n);
@@ -1113,9 +1112,9 @@ extern "C" int pg_fast_setup(const pg_fast_fields* fl, const pg_fast_params* fp,
                        fl->hardness, at<float4>(work, L.geo), fl->pending, at<ll>(work, L.pend_fx),
                        ldexp(1.0, (int)fp->height_bits));
     PG_LAUNCHED("setup_kernel");
-    PG_HIP(hipMemsetAsync(at<char>(work, L.trk), 0, L.pend_fx - L.trk, s));
-    hipLaunchKernelGGL(census_init_kernel, dim3(grid_for(n)), dim3(TPB), 0, s, n, at<CellCensus>(work, L.cen));
-    PG_LAUNCHED("census_init_kernel");
+    PG_HIP(hipMemsetAsync(at<char>(work, L.chunk_fx), 0, L.pend_fx - L.chunk_fx, s));
+    PG_HIP(hipMemsetAsync(at<char>(work, L.ero_fx), 0, L.floor_key - L.ero_fx, s));
+    PG_HIP(hipMemsetAsync(at<char>(work, L.floor_key), 0xff, L.rscale - L.floor_key, s));
     return 0;
 }
 
@@ -1268,11 +1267,15 @@ int fast_particles(const pg_fast_fields* fl, const pg_trace_params* prm, const p
     k.ocean_steps = (int)prm->ocean_steps;
     k.hscale = hscale;
     k.tscale = tscale;
-    k.trk = at<CellAcc>(work, L.trk);
-    k.cen = at<CellCensus>(work, L.cen);
+    k.chunk_fx = at<ll>(work, L.chunk_fx);
+    k.ero_fx = at<ll>(work, L.ero_fx);
+    k.floor_key = at<ull>(work, L.floor_key);
     k.fan_slope = (float)fp->fan_slope;
     k.rscale = at<float>(work, L.rscale);
     k.conc_chunk = fp->conc_chunk;
+    k.dis_fx = at<ll>(work, L.dis_fx);
+    k.mom_fx = at<ll>(work, L.mom_fx);
+    k.mom_fx_b = k.mom_fx + n;
     k.stats = st;
     int grid = 0;
     PG_REQUIRE(fp->variant >= 0 && fp->variant <= 2, "fast mode: variant must be 0, 1 or 2");
@@ -1285,9 +1288,9 @@ int fast_particles(const pg_fast_fields* fl, const pg_trace_params* prm, const p
         launch_trace<false>(fp->variant, grid, s, k);
         if (fp->time_trace && timer_record(s, false)) return -1;
         hipLaunchKernelGGL(erosion_scale_kernel, dim3(grid_for(n)), dim3(TPB), 0, s, n, fl->height, fl->sediment, fl->base,
-                           k.cen, at<ll>(work, L.acc_fx), at<float>(work, L.r1), fp->iter_erode, hscale);
+                           k.ero_fx, k.floor_key, at<ll>(work, L.acc_fx), at<float>(work, L.r1), fp->iter_erode, hscale);
         hipLaunchKernelGGL(erosion_scale2_kernel, dim3(grid_for(n)), dim3(TPB), 0, s, n, fl->height, fl->sediment,
-                           fl->base, k.cen, at<ll>(work, L.acc_fx), at<float>(work, L.r1),
+                           fl->base, k.ero_fx, k.floor_key, at<ll>(work, L.acc_fx), at<float>(work, L.r1),
                            at<float>(work, L.rscale), fp->iter_erode, hscale);
         // pass 1: the same trajectories, erosion scaled per cell, deficits cancel deposits
         k.pass = 1;
@@ -1295,8 +1298,8 @@ int fast_particles(const pg_fast_fields* fl, const pg_trace_params* prm, const p
         launch_trace<false>(fp->variant, grid, s, k);
         if (fp->time_trace && timer_record(s, false)) return -1;
         hipLaunchKernelGGL(resolve_kernel, dim3(grid_for(n)), dim3(TPB), 0, s, n, fl->height, fl->sediment, fl->base,
-                           k.trk, at<ll>(work, L.acc_fx), pend_fx, cell4, dyn, fp->iter_deposit, prm->dep_floor,
-                           fp->offshore_writeoff, hscale, st, k.cen);
+                           k.chunk_fx, at<ll>(work, L.acc_fx), pend_fx, cell4, dyn, fp->iter_deposit, prm->dep_floor,
+                           fp->offshore_writeoff, hscale, st, k.ero_fx, k.floor_key);
     }
     PG_LAUNCHED("trace/resolve");
     return finish_iteration(fl, fp, work, c);
@@ -1305,7 +1308,7 @@ int fast_particles(const pg_fast_fields* fl, const pg_trace_params* prm, const p
 int finish_iteration(const pg_fast_fields* fl, const pg_fast_params* fp, void* work, Ctx& c) {
     hipLaunchKernelGGL(finish_kernel, dim3(grid_for(c.n)), dim3(TPB), 0, c.s, c.n, fl->height, fl->sediment,
                        fl->discharge, fl->momentum, fl->pending, fl->mask, at<ll>(work, c.L.acc_fx),
-                       at<CellAcc>(work, c.L.trk), c.pend_fx, fp->ema, c.hscale, c.tscale, c.st);
+                       at<ll>(work, c.L.dis_fx), at<ll>(work, c.L.mom_fx), c.pend_fx, fp->ema, c.hscale, c.tscale, c.st);
     PG_LAUNCHED("finish_kernel");
     return 0;
 }
@@ -1358,7 +1361,7 @@ extern "C" int pg_fast_layout(int64_t F, int64_t NE, int64_t n_particles, int64_
     PG_REQUIRE(offsets && n >= 16, "pg_fast_layout: need room for 16 offsets");
     Layout L;
     PG_REQUIRE(layout_of(F, NE, n_particles, &L) == 0, "pg_fast_layout: hipcub temp-size query failed");
-    const size_t v[16] = {L.cell4, L.geo, L.dyn, L.trk, L.acc_fx, L.cen, L.rscale, L.pend_fx,
+    const size_t v[16] = {L.cell4, L.geo, L.dyn, L.chunk_fx, L.acc_fx, L.dis_fx, L.mom_fx, L.pend_fx,
                           L.d_cell, L.d_fx, L.d_fy, L.d_load, L.d_sa, L.d_sb, L.scalars, L.total};
     for (int k = 0; k < 16; k++) offsets[k] = (int64_t)v[k];
     return 0;
@@ -1401,10 +1404,13 @@ int batch_layout(int64_t W, const int64_t* NE, const int64_t* P, BLayout* B) {
     L.cell4 = take(16 * n);
     L.geo = take(32 * n);
     L.dyn = take(8 * n);
-    L.trk = take(sizeof(CellAcc) * n);
+    L.chunk_fx = take(8 * n);
     L.acc_fx = take(8 * n);
+    L.dis_fx = take(8 * n);
+    L.mom_fx = take(16 * n);
     L.pend_fx = take(8 * n);
-    L.cen = take(sizeof(CellCensus) * n);
+    L.ero_fx = take(8 * n);
+    L.floor_key = take(8 * n);
     L.rscale = take(4 * n);
     L.r1 = take(4 * n);
     L.wq = take(8 * n);
@@ -1441,10 +1447,13 @@ void window_layout(const BLayout& B, int64_t coff, int64_t poff, int64_t doff, i
     Lw->cell4 = L.cell4 + 16 * c;
     Lw->geo = L.geo + 32 * c;
     Lw->dyn = L.dyn + 8 * c;
-    Lw->trk = L.trk + sizeof(CellAcc) * c;
+    Lw->chunk_fx = L.chunk_fx + 8 * c;
     Lw->acc_fx = L.acc_fx + 8 * c;
+    Lw->dis_fx = L.dis_fx + 8 * c;
+    Lw->mom_fx = L.mom_fx + 16 * c;
     Lw->pend_fx = L.pend_fx + 8 * c;
-    Lw->cen = L.cen + sizeof(CellCensus) * c;
+    Lw->ero_fx = L.ero_fx + 8 * c;
+    Lw->floor_key = L.floor_key + 8 * c;
     Lw->rscale = L.rscale + 4 * c;
     Lw->r1 = L.r1 + 4 * c;
     Lw->wq = L.wq + 8 * c;
@@ -1534,9 +1543,9 @@ extern "C" int pg_batch_setup(const pg_fast_fields* wins, int64_t n_win, const p
                        wins[0].evap, wins[0].hardness, at<float4>(work, L.geo), wins[0].pending, at<ll>(work, L.pend_fx),
                        ldexp(1.0, (int)fps[0].height_bits));
     PG_LAUNCHED("setup_kernel");
-    PG_HIP(hipMemsetAsync(at<char>(work, L.trk), 0, L.pend_fx - L.trk, s));
-    hipLaunchKernelGGL(census_init_kernel, dim3(grid_for(n)), dim3(TPB), 0, s, n, at<CellCensus>(work, L.cen));
-    PG_LAUNCHED("census_init_kernel");
+    PG_HIP(hipMemsetAsync(at<char>(work, L.chunk_fx), 0, L.pend_fx - L.chunk_fx, s));
+    PG_HIP(hipMemsetAsync(at<char>(work, L.ero_fx), 0, L.floor_key - L.ero_fx, s));
+    PG_HIP(hipMemsetAsync(at<char>(work, L.floor_key), 0xff, L.rscale - L.floor_key, s));
     std::vector<WinStatic> ws(n_win);
     int64_t doff = 0;
     for (int64_t w = 0; w < n_win; w++) {
@@ -1656,9 +1665,13 @@ extern "C" int pg_batch_iterate(const pg_fast_fields* wins, int64_t n_win, const
     k.ocean_steps = (int)prm->ocean_steps;
     k.hscale = hscale;
     k.tscale = tscale;
-    k.trk = at<CellAcc>(work, L.trk);
-    k.cen = at<CellCensus>(work, L.cen);
+    k.chunk_fx = at<ll>(work, L.chunk_fx);
+    k.ero_fx = at<ll>(work, L.ero_fx);
+    k.floor_key = at<ull>(work, L.floor_key);
     k.fan_slope = (float)f0.fan_slope;
+    k.dis_fx = at<ll>(work, L.dis_fx);
+    k.mom_fx = at<ll>(work, L.mom_fx);
+    k.mom_fx_b = k.mom_fx + n;
     k.stats = (ll*)stats;
     k.wins = wd;
     k.win_prefix = prefix;
@@ -1675,22 +1688,22 @@ extern "C" int pg_batch_iterate(const pg_fast_fields* wins, int64_t n_win, const
         launch_trace<true>(f0.variant, grid, s, k);
         if (f0.time_trace && timer_record(s, false)) return -1;
         hipLaunchKernelGGL(erosion_scale_kernel, dim3(grid_for(n)), dim3(TPB), 0, s, n, a.height, a.sediment, a.base,
-                           k.cen, at<ll>(work, L.acc_fx), at<float>(work, L.r1), f0.iter_erode, hscale);
+                           k.ero_fx, k.floor_key, at<ll>(work, L.acc_fx), at<float>(work, L.r1), f0.iter_erode, hscale);
         hipLaunchKernelGGL(erosion_scale2_kernel, dim3(grid_for(n)), dim3(TPB), 0, s, n, a.height, a.sediment, a.base,
-                           k.cen, at<ll>(work, L.acc_fx), at<float>(work, L.r1),
+                           k.ero_fx, k.floor_key, at<ll>(work, L.acc_fx), at<float>(work, L.r1),
                            at<float>(work, L.rscale), f0.iter_erode, hscale);
         k.pass = 1;
         if (f0.time_trace && timer_record(s, true)) return -1;
         launch_trace<true>(f0.variant, grid, s, k);
         if (f0.time_trace && timer_record(s, false)) return -1;
         hipLaunchKernelGGL(resolve_kernel, dim3(grid_for(n)), dim3(TPB), 0, s, n, a.height, a.sediment, a.base,
-                           k.trk, at<ll>(work, L.acc_fx), at<ll>(work, L.pend_fx), at<float4>(work, L.cell4),
+                           k.chunk_fx, at<ll>(work, L.acc_fx), at<ll>(work, L.pend_fx), at<float4>(work, L.cell4),
                            at<float2>(work, L.dyn), f0.iter_deposit, prm->dep_floor, f0.offshore_writeoff, hscale,
-                           (ll*)stats, k.cen);
+                           (ll*)stats, k.ero_fx, k.floor_key);
     }
     PG_LAUNCHED("batch trace/resolve");
     hipLaunchKernelGGL(finish_kernel, dim3(grid_for(n)), dim3(TPB), 0, s, n, a.height, a.sediment, a.discharge,
-                       a.momentum, a.pending, a.mask, at<ll>(work, L.acc_fx), k.trk, at<ll>(work, L.pend_fx),
+                       a.momentum, a.pending, a.mask, at<ll>(work, L.acc_fx), k.dis_fx, k.mom_fx, at<ll>(work, L.pend_fx),
                        f0.ema, hscale, tscale, (ll*)stats);
     PG_LAUNCHED("finish_kernel");
     return 0;

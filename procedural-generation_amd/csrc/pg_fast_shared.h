@@ -38,21 +38,9 @@ struct Scalars {
 // ---------------------------------------------------------------------------
 // workspace layout
 // ---------------------------------------------------------------------------
-// a cell's int64 fixed-point atomic targets of the applying pass -- the
-// chunk's net change, the discharge and the two momentum tracks -- in one
-// 32-B record, so a move's four atomics land in one half cache line
-struct CellAcc {
-    ll chunk, dis, ma, mb;
-};
-// a cell's census of the census pass: erosion requested, and the lowest
-// (surface key << 32 | cell) stepped to from it
-struct CellCensus {
-    ll ero;
-    ull floor_key;
-};
-
 struct Layout {
-    size_t cell4, geo, dyn, trk, acc_fx, pend_fx, cen, rscale, r1, wq, cdf_fx, flag, rank, comp_idx, comp_cdf,
+    size_t cell4, geo, dyn, chunk_fx, acc_fx, dis_fx, mom_fx, pend_fx, ero_fx, floor_key, rscale, r1, wq, cdf_fx, flag,
+        rank, comp_idx, comp_cdf,
         draws, u_sorted, d_cell, d_fx, d_fy, d_load, d_sa, d_sb, scalars, temp, total;
     size_t temp_bytes;
 };

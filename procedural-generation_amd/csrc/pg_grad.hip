@@ -112,8 +112,8 @@ struct GradCfg {
     // the differentiable scalars
     Real erodibility, rate, evap_rate, k_disc, ocean_rate, dep_floor;
     double hscale, tscale, conc_chunk;
-    ll *acc_fx, *stats;
-    CellAcc* trk;
+    ll *acc_fx, *dis_fx, *mom_fx, *stats;  // momentum track b one plane (n_cells) after a
+    int64_t n_cells;
     const int* fixed_moves;
     int* moves_out;
     char* tape;
@@ -343,10 +343,9 @@ __global__ void __launch_bounds__(TRACE_TPB) grad_forward_kernel(GradCfg<Real> k
                                                         k.disc_exponent, k.max_erode);
                             flags |= F_LAND;
                             if (!replay) {
-                                CellAcc* t = &k.trk[cell];
-                                atomicAdd((ull*)&t->dis, (ull)llrint((double)vol_s * k.tscale));
-                                atomicAdd((ull*)&t->ma, (ull)llrint((double)(vol_s * sa) * k.tscale));
-                                atomicAdd((ull*)&t->mb, (ull)llrint((double)(vol_s * sb) * k.tscale));
+                                atomicAdd((ull*)&k.dis_fx[cell], (ull)llrint((double)vol_s * k.tscale));
+                                atomicAdd((ull*)&k.mom_fx[cell], (ull)llrint((double)(vol_s * sa) * k.tscale));
+                                atomicAdd((ull*)&k.mom_fx[k.n_cells + cell], (ull)llrint((double)(vol_s * sb) * k.tscale));
                             }
                         }
                         if (taping && d < k.tape_capacity) {
@@ -710,7 +709,9 @@ int launch_forward(const pg_fast_fields* steer, const pg_trace_params* prm, cons
     k.tscale = c.tscale;
     k.conc_chunk = fp->conc_chunk;
     k.acc_fx = at<ll>(work, c.L.acc_fx);
-    k.trk = at<CellAcc>(work, c.L.trk);
+    k.dis_fx = at<ll>(work, c.L.dis_fx);
+    k.mom_fx = at<ll>(work, c.L.mom_fx);
+    k.n_cells = c.n;
     k.stats = c.st;
     k.fixed_moves = ga->fixed_moves;
     k.moves_out = ga->moves_out;
