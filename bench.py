@@ -44,7 +44,7 @@ HBM_PEAK_GBS = 8000.0
 #: HBM bytes per trace launch of this workload (N_c=1024, 16 chunks) by register-budget variant, from
 #: rocprofv3 FETCH_SIZE (x 2.000, calibrated on a 1-GiB copy) + WRITE_SIZE (x 1.000), mean over both
 #: passes; profiles/trace_kernel_earth_n1024_counters.md holds the runs
-TRACE_TRAFFIC_BYTES = {0: 2.890e9, 1: 7.740e9, 2: 17.990e9}
+TRACE_TRAFFIC_BYTES = {0: 2.850e9, 1: 7.733e9, 2: 18.006e9}
 
 
 def log(msg):
