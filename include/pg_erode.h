@@ -158,6 +158,64 @@ int pg_select_kth_surface(const double* height, const double* sediment, int64_t 
  * q read from the DEVICE double *q (n = F * NE * NE). */
 int pg_datum_shift(int64_t n, double* height, double* route, double* base, const double* q, pg_stream stream);
 
+/* ---- mass wasting (bake/globe/erosion/maps.py:1050-1085, thermal_erosion) */
+/* talus[c] = soft + (hard - soft) * hardness[c] in float32, widened
+ * (maps.py:1062). */
+int pg_talus(int64_t n, const float* hardness, double soft, double hard, double* talus, pg_stream stream);
+/* particle.py:1128-1160, thermal_pass_a: per-cell cap scale (and total
+ * outflow into out_total unless it is NULL).  F faces of NE x NE. */
+int pg_thermal_pass_a(int64_t F, int64_t NE, const double* height, const double* sediment, const float* metric,
+                      const double* talus, const uint8_t* mask, double rate, double thermal_max, double* out_total,
+                      double* scale, pg_stream stream);
+/* particle.py:1163-1187, thermal_pass_r: receiver scale of submerged cells. */
+int pg_thermal_pass_r(int64_t F, int64_t NE, const double* height, const double* sediment, const float* metric,
+                      const double* talus, const uint8_t* mask, double rate, const double* scale, double* rscale,
+                      double dep_floor, pg_stream stream);
+/* particle.py:1190-1227, thermal_pass_b: the gather into new_height /
+ * new_sediment (which must not alias height / sediment). */
+int pg_thermal_pass_b(int64_t F, int64_t NE, const double* height, const double* sediment, const float* metric,
+                      const double* talus, const uint8_t* mask, double rate, const double* scale, const double* rscale,
+                      double* new_height, double* new_sediment, pg_stream stream);
+/* maps.py:1073-1085, _mass_wasting_pass: the three passes and the copy back
+ * into height / sediment.  work: DEVICE float64[4 * F * NE * NE]. */
+int pg_mass_wasting_pass(int64_t F, int64_t NE, double* height, double* sediment, const float* metric,
+                         const double* talus, const uint8_t* mask, double rate, double thermal_max, double dep_floor,
+                         double* work, pg_stream stream);
+/* maps.py:1067: out = FROZEN where height + sediment < base, else mask. */
+int pg_creep_mask(int64_t n, const double* height, const double* sediment, const double* base, const uint8_t* mask,
+                  uint8_t* out, pg_stream stream);
+
+/* ---- slope limit (particle.py:1045-1076, slope_erode_cap) -------------- */
+int pg_slope_erode_cap(int64_t F, int64_t NE, const double* height, const double* sediment, const uint8_t* mask,
+                       double coef, double exit_slope, double* out, pg_stream stream);
+
+/* ---- isostasy (maps.py:1325-1410) and its smoother (field.py:179-188,
+ * 372-404) -------------------------------------------------------------- */
+/* The flux-form Laplace-Beltrami operator of data (F x NE x NE, halos
+ * exchanged), lap = float32(raw / cell_size_sq) as FaceField.laplacian:
+ * mode 0: out = raw (the operator before the division; 0 on the outer ring);
+ * mode 1: out = lap * c2;  mode 2: out = data + kappa * lap * c2 (one
+ * smoothing step).  metric_inv float32 (F, NE, NE, 3), sqrt_det float32. */
+int pg_laplacian(int64_t F, int64_t NE, const double* data, const float* metric_inv, const float* sqrt_det,
+                 double cell_size_sq, int32_t mode, double kappa, double c2, double* out, pg_stream stream);
+/* Over the interior cells (mask NULL: all; else mask == ACTIVE only):
+ * out4 = {sum of v (what 0) / v^2 (1) / |v| (2), count, max v, min v}, a
+ * DEVICE double[4].  work: DEVICE scratch of pg_interior_sums_work_bytes(). */
+int64_t pg_interior_sums_work_bytes(void);
+int pg_interior_sums(const double* data, const uint8_t* mask, int64_t F, int64_t N, int64_t H, int32_t what, double* work,
+                     double* out4, pg_stream stream);
+/* maps.py:1405-1406: height += data - mean on active cells, with mean =
+ * sums[0] / sums[1] (0 when sums[1] == 0) read from the DEVICE sums. */
+int pg_rebound(int64_t n, double* height, const double* data, const uint8_t* mask, const double* sums, pg_stream stream);
+/* out = height + sediment (maps.py:1574). */
+int pg_surface(int64_t n, const double* height, const double* sediment, double* out, pg_stream stream);
+/* iso_acc += (height + sediment) - s0 (maps.py:1586). */
+int pg_iso_accumulate(int64_t n, double* iso_acc, const double* height, const double* sediment, const double* s0,
+                      pg_stream stream);
+/* out = a * x (a == 0: zeros, x may be NULL); out = x / v. */
+int pg_scale(int64_t n, const double* x, double a, double* out, pg_stream stream);
+int pg_divide(int64_t n, const double* x, double v, double* out, pg_stream stream);
+
 /* ---- fast mode (the production path of one iteration's particle pass) -- */
 /* Device pointers to the erosion state (bake/globe/erosion/maps.py:56-115
  * dtypes): the stage's fields in cell units. */
@@ -191,6 +249,8 @@ typedef struct pg_fast_params {
     const uint64_t* rng_dev; /* optional DEVICE copy of the four rng words, read by the kernels instead of the
                               * values above: a captured graph then replays with each iteration's own draws */
     int64_t variant;       /* trace kernel register budget: 0 = 2 waves per SIMD, 1 = 3, 2 = 4 (same results) */
+    const double* erode_cap; /* optional DEVICE per-cell erosion cap of the iteration (soillib's slope limit,
+                              * pg_slope_erode_cap; bake/globe/erosion/maps.py:923-926), NULL = none */
 } pg_fast_params;
 
 /* Statistics of one fast iteration (DEVICE int64[PG_FAST_NSTATS]). */

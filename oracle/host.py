@@ -11,10 +11,13 @@ What is restated (each function cites the reference lines it follows):
 ``spawn_weights`` / ``spawn_particles`` (maps.py:793-833), ``run_iteration``
 (maps.py:839-1047: pending re-injection, chunk loop of trace + serial apply,
 fold, EMA, stats), ``apply_uplift`` / ``uplift_replay`` / ``start_replay``
-(maps.py:1088-1213), ``hold_datum`` (maps.py:1446-1508) and ``step`` in the
-plain profile (maps.py:1516-1617: particles -> uplift -> datum -> halos).
-Side passes the plain profile switches off (thermal/creep, routing flood,
-lakes, glacial, isostasy, slope limits) raise NotImplementedError here.
+(maps.py:1088-1213), ``hold_datum`` (maps.py:1446-1508), the slope limits
+(maps.py:923-931 with particle.py:1045-1076), ``thermal_erosion``
+(maps.py:1050-1085 with particle.py:1104-1227), ``apply_isostasy`` and its
+smoothing step (maps.py:1325-1410 with field.py:179-188, 372-404) and
+``step`` (maps.py:1516-1617: particles -> mass wasting -> uplift ->
+isostasy -> datum -> halos).  The remaining side passes (routing flood, sea
+mask, lakes, glacial, ice, strata) raise NotImplementedError here.
 
 Parity pins: the reference's own invariants (mass conservation on a closed
 window, the offshore split, the change-list layout, the stockpile bound,
@@ -82,6 +85,11 @@ def lib():
         L.or_from_sphere.argtypes = [dbl, dbl, dbl, ctypes.POINTER(ctypes.c_int), d, d]
         L.or_jacobian.argtypes = [i32, dbl, dbl, d]
         L.or_transfer_velocity.argtypes = [i32, dbl, dbl, dbl, dbl, ctypes.POINTER(ctypes.c_int), d, d, d, d]
+        L.or_thermal_pass_a.argtypes = [i64, i64, vp, vp, vp, vp, vp, dbl, dbl, vp, vp]
+        L.or_thermal_pass_r.argtypes = [i64, i64, vp, vp, vp, vp, vp, dbl, vp, vp, dbl]
+        L.or_thermal_pass_b.argtypes = [i64, i64, vp, vp, vp, vp, vp, dbl, vp, vp, vp, vp]
+        L.or_slope_erode_cap.argtypes = [i64, i64, vp, vp, vp, dbl, dbl, vp]
+        L.or_laplacian.argtypes = [i64, i64, vp, vp, vp, vp]
         _lib = L
     return _lib
 
@@ -184,6 +192,7 @@ class OracleState:
     deposit_on_exit: bool = False
     roots: np.ndarray | None = None
     inflow_volume: float = 0.0
+    iso_acc: np.ndarray | None = None
 
     def __post_init__(self):
         F, NE = self.height.shape[0], self.height.shape[1]
@@ -329,8 +338,6 @@ def chunk_size(ep, n_particles):
 
 
 def _check_plain(ep, state):
-    if float(getattr(ep, "slope_limit_erode", 0.0)) > 0.0 or float(getattr(ep, "slope_limit_deposit", 0.0)) > 0.0:
-        raise NotImplementedError("slope limits are outside the oracle's scope")
     if state.route is not None:
         raise NotImplementedError("the routing surface is outside the oracle's scope")
 
@@ -388,7 +395,16 @@ def run_iteration(state, params, iteration_key, n_particles=None, particles_per_
         if float(ep.momentum_saturation_cells) > 0.0:
             mom_scale = float(volume0) / (float(ep.momentum_saturation_cells) * rain_per_cell)
     unit = state.height_unit_m
+    use_ecap = float(ep.slope_limit_erode) > 0.0
+    if use_ecap:
+        ecap = np.empty_like(state.height)
+        lib().or_slope_erode_cap(state.F, state.NE, _p(state.height), _p(state.sediment), _p(state.mask),
+                                 float(ep.slope_limit_erode), float(ep.slope_limit_exit), _p(ecap))
+    else:
+        ecap = np.zeros(1, np.float64)
     iter_deposit = cell_units(ep, "iter_deposit", unit)
+    if float(ep.slope_limit_deposit) > 0.0:
+        iter_deposit = min(iter_deposit, float(ep.slope_limit_deposit) * math.sqrt(2.0) * 0.3)
     max_steps = max_steps_of(ep, state.N)
     lateral = float(getattr(ep, "lateral_rate", 0.0)) > 0.0
     cap = change_list_cap(max_steps, lateral)
@@ -417,7 +433,6 @@ def run_iteration(state, params, iteration_key, n_particles=None, particles_per_
     sp_death = np.zeros(chunk, np.int8)
     deaths = np.zeros(len(DEATH_NAMES), np.int64)
     roots = state.roots if state.roots is not None else np.zeros((1, 1, 1), np.float32)
-    ecap = np.zeros(1, np.float64)
     out = np.zeros(4, np.float64)
     entries = n_clamp = 0
     to_pending = lost = lost_offshore = 0.0
@@ -440,7 +455,7 @@ def run_iteration(state, params, iteration_key, n_particles=None, particles_per_
         L.or_apply_changes(_p(cl_cell), _p(cl_delta), _p(cl_vol), _p(cl_mom), _p(cl_count), m, cap, _p(state.height),
                            _p(state.sediment), _p(state.acc), _p(state.pending), _p(state.samp), _p(state.disch_track),
                            _p(state.mom_track), _p(state.mask), iter_erode, iter_deposit, float(ep.fan_slope),
-                           int(state.route is not None), dep_floor, float(ep.offshore_writeoff), _p(ecap), 0,
+                           int(state.route is not None), dep_floor, float(ep.offshore_writeoff), _p(ecap), int(use_ecap),
                            _p(state.lake_load), _p(state.lake_id), _p(state.lake_room), _p(out))
         n_clamp += int(out[0])
         to_pending += out[1]
@@ -469,6 +484,146 @@ def run_iteration(state, params, iteration_key, n_particles=None, particles_per_
     stats["disc_saturation"] = disc_sat
     stats["mom_scale"] = mom_scale
     return stats
+
+
+# ---------------------------------------------------------------------------
+# mass wasting (maps.py:1050-1085, particle.py:1104-1227)
+# ---------------------------------------------------------------------------
+DEP_FLOOR = 0.02  # particle.py:99
+
+
+def talus_of(ep, hardness):
+    """maps.py:1062: the talus slope, numpy's float32 arithmetic on the
+    float32 hardness, widened to float64."""
+    return (ep.talus_slope_soft + (ep.talus_slope_hard - ep.talus_slope_soft) * hardness).astype(np.float64)
+
+
+def mass_wasting_pass(state, talus, rate, cap, mask=None, dep_floor=DEP_FLOOR):
+    """maps.py:1073-1085: the three thermal passes over the whole state."""
+    mask = state.mask if mask is None else np.ascontiguousarray(mask, np.uint8)
+    F, NE = state.F, state.NE
+    talus = np.ascontiguousarray(talus, np.float64)
+    out_total = np.zeros_like(state.height)
+    scale = np.zeros_like(state.height)
+    L = lib()
+    L.or_thermal_pass_a(F, NE, _p(state.height), _p(state.sediment), _p(state.metric), _p(talus), _p(mask), float(rate),
+                        float(cap), _p(out_total), _p(scale))
+    rscale = np.ones_like(state.height)
+    L.or_thermal_pass_r(F, NE, _p(state.height), _p(state.sediment), _p(state.metric), _p(talus), _p(mask), float(rate),
+                        _p(scale), _p(rscale), float(dep_floor))
+    new_h = state.height.copy()
+    new_s = state.sediment.copy()
+    L.or_thermal_pass_b(F, NE, _p(state.height), _p(state.sediment), _p(state.metric), _p(talus), _p(mask), float(rate),
+                        _p(scale), _p(rscale), _p(new_h), _p(new_s))
+    state.height[...] = new_h
+    state.sediment[...] = new_s
+
+
+def thermal_erosion(state, params):
+    """maps.py:1050-1070: the talus pass, then (creep_rate > 0) the creep
+    pass with talus 0 and submerged cells frozen."""
+    ep = _eparams(params)
+    unit = state.height_unit_m
+    talus = talus_of(ep, state.hardness)
+    dep_floor = cell_units(ep, "dep_floor_m", unit)
+    mass_wasting_pass(state, talus, float(ep.thermal_rate), cell_units(ep, "thermal_max", unit), dep_floor=dep_floor)
+    if ep.creep_rate > 0.0:
+        cmask = np.where(state.height + state.sediment < state.base, np.uint8(MASK_FROZEN), state.mask).astype(np.uint8)
+        mass_wasting_pass(state, np.zeros_like(talus), float(ep.creep_rate), cell_units(ep, "thermal_max", unit), cmask,
+                          dep_floor=dep_floor)
+
+
+def slope_erode_cap(state, coef, exit_slope):
+    """particle.py:1045-1076 over the state's surface."""
+    out = np.empty_like(state.height)
+    lib().or_slope_erode_cap(state.F, state.NE, _p(state.height), _p(state.sediment), _p(state.mask), float(coef),
+                             float(exit_slope), _p(out))
+    return out
+
+
+# ---------------------------------------------------------------------------
+# isostasy (maps.py:1325-1410) and the smoother (field.py:104-124, 179-188)
+# ---------------------------------------------------------------------------
+def exchange(grid, data, linear=False):
+    """field.py:104-124 for a float64 scalar field, in place."""
+    hm = grid.halo
+    flat = data.reshape(6 * grid.NE * grid.NE)
+    w = (hm.w_lin if linear else hm.w).astype(flat.dtype)
+    flat[hm.dst] = np.einsum("ks,ks...->k...", w, flat[hm.src])
+    return data
+
+
+def laplacian(grid, data):
+    """field.py:179-188: the flux-form Laplace-Beltrami operator per m^2,
+    returned in float32 as FaceField.laplacian returns it."""
+    d = np.ascontiguousarray(data, np.float64)
+    ginv = np.ascontiguousarray(grid.metric_inv, np.float64)
+    sq = np.ascontiguousarray(grid.sqrt_det_metric, np.float64)
+    out = np.empty_like(d)
+    lib().or_laplacian(6, grid.NE, _p(d), _p(ginv), _p(sq), _p(out))
+    out /= grid.cell_size_m ** 2
+    return out.astype(np.float32)
+
+
+_KAPPA = {}
+
+
+def smoothing_kappa(grid):
+    """maps.py:1325-1363: 0.9 / (largest eigenvalue of -L in cell units), by
+    40 power iterations from seeded white noise; cached per grid."""
+    key = (grid.N, grid.H, float(grid.cell_size_m))
+    if key not in _KAPPA:
+        rng = np.random.default_rng(0)
+        data = np.zeros((6, grid.NE, grid.NE))
+        H, N = grid.H, grid.N
+        data[:, H:H + N, H:H + N] = rng.standard_normal((6, N, N))
+        exchange(grid, data)
+        c2, lam = float(grid.cell_size_m) ** 2, 8.0
+        for _ in range(40):
+            exchange(grid, data, linear=True)
+            Lv = laplacian(grid, data).astype(np.float64) * c2
+            num = float(np.sqrt((Lv[:, H:-H, H:-H] ** 2).sum()))
+            den = float(np.sqrt((data[:, H:H + N, H:H + N] ** 2).sum()))
+            if num <= 0.0 or den <= 0.0:
+                break
+            lam = num / den
+            data = Lv / num
+        _KAPPA[key] = 0.9 / max(lam, 1e-6)
+    return _KAPPA[key]
+
+
+def isostasy_steps(state, ep):
+    """maps.py:1392-1397: (sigma in cells, kappa, diffusion steps)."""
+    cell = float(state.height_unit_m)
+    sig = float(ep.flexure_km) * 1000.0 / cell
+    sig = min(sig, state.N / 8.0)
+    kappa = smoothing_kappa(state.grid)
+    n = int(np.ceil(sig * sig / (2.0 * kappa))) if sig >= 0.5 else 0
+    return sig, kappa, n
+
+
+def apply_isostasy(state, params):
+    """maps.py:1366-1410."""
+    ep = _eparams(params)
+    grid = state.grid
+    d = state.iso_acc
+    k = float(ep.isostasy)
+    data = (-k * d).astype(np.float64)
+    cell = float(state.height_unit_m)
+    sig, kappa, n = isostasy_steps(state, ep)
+    for _ in range(n):
+        exchange(grid, data, linear=True)
+        data += kappa * laplacian(grid, data).astype(np.float64) * (cell * cell)
+    exchange(grid, data, linear=True)
+    act = state.mask == MASK_ACTIVE
+    iact = state.mask[state.interior] == MASK_ACTIVE
+    mean = float(data[state.interior][iact].mean()) if iact.any() else 0.0
+    state.height[act] += (data[act] - mean).astype(state.height.dtype)
+    moved = float(np.abs(d[state.interior]).sum())
+    state.iso_acc[...] = 0.0
+    return {"sigma_cells": sig, "diffusion_steps": n, "kappa": kappa, "load_abs_cells": moved,
+            "rebound_max": float((data[state.interior] - mean).max()),
+            "rebound_min": float((data[state.interior] - mean).min())}
 
 
 # ---------------------------------------------------------------------------
@@ -551,18 +706,27 @@ def start_replay(state, params):
 
 
 def step(state, params, iteration_key, **kw):
-    """maps.py:1516-1617 in the plain profile: particles, uplift, datum, halos."""
+    """maps.py:1516-1617: particles, mass wasting, uplift, isostasy (every
+    isostasy_every iterations), datum, halos."""
     ep = _eparams(params)
-    if float(ep.thermal_rate) != 0.0 or float(ep.creep_rate) != 0.0:
-        raise NotImplementedError("thermal / creep passes are outside the oracle's scope (plain profile)")
     if int(ep.flood_every) > 0 or (state.spherical and int(getattr(ep, "sea_mask_every", 0)) > 0):
-        raise NotImplementedError("routing / sea-mask refreshes are outside the oracle's scope (plain profile)")
-    if state.spherical and (int(ep.glacial_every) > 0 or float(getattr(ep, "isostasy", 0.0)) > 0.0):
-        raise NotImplementedError("glacial / isostasy passes are outside the oracle's scope (plain profile)")
+        raise NotImplementedError("routing / sea-mask refreshes are outside the oracle's scope")
+    if state.spherical and int(ep.glacial_every) > 0:
+        raise NotImplementedError("the glacial pass is outside the oracle's scope")
     t0 = time.time()
+    iso = state.spherical and float(getattr(ep, "isostasy", 0.0)) > 0.0
+    if iso:
+        if state.iso_acc is None:
+            state.iso_acc = np.zeros_like(state.height)
+        s0 = state.height + state.sediment
     st = run_iteration(state, params, iteration_key, **kw)
     t1 = time.time()
+    thermal_erosion(state, params)
+    if iso:
+        state.iso_acc += (state.height + state.sediment) - s0
     apply_uplift(state, uplift_cap(state, ep))
+    if iso and (state.iteration + 1) % max(int(getattr(ep, "isostasy_every", 10)), 1) == 0:
+        st["isostasy"] = apply_isostasy(state, params)
     if state.spherical and hasattr(params, "world"):
         st["datum_shift"] = hold_datum(state, held_land_fraction(state, params))
     state.exchange_halos()

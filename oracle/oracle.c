@@ -843,3 +843,184 @@ void or_spawn_cells(const double *cdf, int64_t M, const double *u, int64_t P, in
         out[p] = k;
     }
 }
+
+/* ---------------------------------------------------------------------- */
+/* mass wasting: thermal erosion and hillslope creep (particle.py:1104-1227) */
+/* ---------------------------------------------------------------------- */
+/* D8 neighbour order (particle.py:101-102); the opposite of k is (k + 4) % 8 */
+static const int64_t OR_D8[8][2] = {{1, 0}, {1, 1}, {0, 1}, {-1, 1}, {-1, 0}, {-1, -1}, {0, -1}, {1, -1}};
+
+/* particle.py:1107-1125: raw outflow from cell c = (f, ei, ej) to its k-th
+ * D8 neighbour.  numba types: metric float32 x int64 offsets -> float64. */
+static double or_thermal_out(const double *h, const double *s, const float *metric, const double *talus, int64_t NE,
+                             int64_t f, int64_t ei, int64_t ej, int k, double rate) {
+    const int64_t di = OR_D8[k][0], dj = OR_D8[k][1];
+    const int64_t c = (f * NE + ei) * NE + ej, n = (f * NE + ei + di) * NE + ej + dj;
+    const double d = (h[c] + s[c]) - (h[n] + s[n]);
+    if (d <= 0.0) return 0.0;
+    const double g0 = metric[3 * c], g1 = metric[3 * c + 1], g2 = metric[3 * c + 2];
+    const double ddi = (double)di, ddj = (double)dj;
+    const double dist = sqrt(g0 * ddi * ddi + 2.0 * g1 * ddi * ddj + g2 * ddj * ddj);
+    const double ex = d - talus[c] * dist;
+    if (ex <= 0.0) return 0.0;
+    return 0.5 * rate * ex;
+}
+
+/* particle.py:1128-1160: per-cell total outflow and cap scale */
+void or_thermal_pass_a(int64_t F, int64_t NE, const double *h, const double *s, const float *metric, const double *talus,
+                       const uint8_t *mask, double rate, double tmax, double *out_total, double *scale) {
+    for (int64_t f = 0; f < F; f++)
+        for (int64_t ei = 1; ei < NE - 1; ei++)
+            for (int64_t ej = 1; ej < NE - 1; ej++) {
+                const int64_t c = (f * NE + ei) * NE + ej;
+                out_total[c] = 0.0;
+                scale[c] = 0.0;
+                if (mask[c] != MASK_ACTIVE) continue;
+                double S = 0.0, mx = 0.0;
+                for (int k = 0; k < 8; k++) {
+                    if (mask[(f * NE + ei + OR_D8[k][0]) * NE + ej + OR_D8[k][1]] != MASK_ACTIVE) continue;
+                    const double o = or_thermal_out(h, s, metric, talus, NE, f, ei, ej, k, rate);
+                    S += o;
+                    if (o > mx) mx = o;
+                }
+                if (S <= 0.0) continue;
+                double sc = 1.0;
+                if (S > 2.0 * mx) sc = 2.0 * mx / S;
+                if (tmax > 0.0 && S * sc > tmax) sc = tmax / S;
+                out_total[c] = S * sc;
+                scale[c] = sc;
+            }
+}
+
+/* particle.py:1163-1187: receiver scale of submerged cells */
+void or_thermal_pass_r(int64_t F, int64_t NE, const double *h, const double *s, const float *metric, const double *talus,
+                       const uint8_t *mask, double rate, const double *scale, double *rscale, double dep_floor) {
+    for (int64_t f = 0; f < F; f++)
+        for (int64_t ei = 1; ei < NE - 1; ei++)
+            for (int64_t ej = 1; ej < NE - 1; ej++) {
+                const int64_t c = (f * NE + ei) * NE + ej;
+                rscale[c] = 1.0;
+                const double surf = h[c] + s[c];
+                if (surf >= 0.0 || mask[c] != MASK_ACTIVE) continue;
+                double inflow = 0.0;
+                for (int k = 0; k < 8; k++) {
+                    const int64_t ni = ei + OR_D8[k][0], nj = ej + OR_D8[k][1];
+                    const int64_t n = (f * NE + ni) * NE + nj;
+                    const double sc = scale[n];
+                    if (sc <= 0.0 || mask[n] == MASK_OUTSIDE) continue;
+                    inflow += sc * or_thermal_out(h, s, metric, talus, NE, f, ni, nj, (k + 4) % 8, rate);
+                }
+                double room = -dep_floor - surf;
+                if (room < 0.0) room = 0.0;
+                if (inflow > room) rscale[c] = room / inflow;
+            }
+}
+
+/* particle.py:1190-1227: gather of outflows (sediment first) and inflows */
+void or_thermal_pass_b(int64_t F, int64_t NE, const double *h, const double *s, const float *metric, const double *talus,
+                       const uint8_t *mask, double rate, const double *scale, const double *rscale, double *new_h,
+                       double *new_s) {
+    for (int64_t f = 0; f < F; f++)
+        for (int64_t ei = 1; ei < NE - 1; ei++)
+            for (int64_t ej = 1; ej < NE - 1; ej++) {
+                const int64_t c = (f * NE + ei) * NE + ej;
+                double hh = h[c], ss = s[c];
+                if (mask[c] != MASK_ACTIVE) {
+                    new_h[c] = hh;
+                    new_s[c] = ss;
+                    continue;
+                }
+                double inflow = 0.0, out = 0.0;
+                const double sc_self = scale[c];
+                for (int k = 0; k < 8; k++) {
+                    const int64_t ni = ei + OR_D8[k][0], nj = ej + OR_D8[k][1];
+                    const int64_t n = (f * NE + ni) * NE + nj;
+                    if (mask[n] == MASK_OUTSIDE) continue;
+                    if (sc_self > 0.0 && mask[n] == MASK_ACTIVE)
+                        out += sc_self * rscale[n] * or_thermal_out(h, s, metric, talus, NE, f, ei, ej, k, rate);
+                    const double sc = scale[n];
+                    if (sc <= 0.0) continue;
+                    inflow += sc * or_thermal_out(h, s, metric, talus, NE, f, ni, nj, (k + 4) % 8, rate);
+                }
+                inflow *= rscale[c];
+                if (out > 0.0) {
+                    if (ss >= out) {
+                        ss -= out;
+                    } else {
+                        hh -= out - ss;
+                        ss = 0.0;
+                    }
+                }
+                new_h[c] = hh;
+                new_s[c] = ss + inflow;
+            }
+}
+
+/* particle.py:1045-1076: soillib's pit-free erosion limit per cell */
+void or_slope_erode_cap(int64_t F, int64_t NE, const double *h, const double *s, const uint8_t *mask, double coef,
+                        double exit_slope, double *out) {
+    const double k = coef * sqrt(2.0);
+#define SURF(i, j) (h[(f * NE + (i)) * NE + (j)] + s[(f * NE + (i)) * NE + (j)])
+    for (int64_t f = 0; f < F; f++)
+        for (int64_t i = 0; i < NE; i++)
+            for (int64_t j = 0; j < NE; j++) {
+                const int64_t c = (f * NE + i) * NE + j;
+                if (mask[c] != MASK_ACTIVE) {
+                    out[c] = 0.0;
+                    continue;
+                }
+                const double hc = h[c] + s[c];
+                const double gxn = i == 0 ? exit_slope : hc - SURF(i - 1, j);
+                const double gxp = i == NE - 1 ? exit_slope : hc - SURF(i + 1, j);
+                const double gyn = j == 0 ? exit_slope : hc - SURF(i, j - 1);
+                const double gyp = j == NE - 1 ? exit_slope : hc - SURF(i, j + 1);
+                double gx = gxn > gxp ? gxn : gxp;
+                double gy = gyn > gyp ? gyn : gyp;
+                if (gx < 0.0) gx = 0.0;
+                if (gy < 0.0) gy = 0.0;
+                out[c] = k * sqrt(gx * gx + gy * gy);
+            }
+#undef SURF
+}
+
+/* ---------------------------------------------------------------------- */
+/* Laplace-Beltrami operator, flux form (field.py:372-404): the raw kernel  */
+/* (before field.py:187's division by the cell size squared); out is zero  */
+/* on the outermost ring                                                    */
+/* ---------------------------------------------------------------------- */
+void or_laplacian(int64_t F, int64_t NE, const double *d, const double *ginv, const double *sq, double *out) {
+#define D(i, j) d[(f * NE + (i)) * NE + (j)]
+#define G(i, j, k) ginv[((f * NE + (i)) * NE + (j)) * 3 + (k)]
+#define Q(i, j) sq[(f * NE + (i)) * NE + (j)]
+    for (int64_t c = 0; c < F * NE * NE; c++) out[c] = 0.0;
+    for (int64_t f = 0; f < F; f++)
+        for (int64_t i = 1; i < NE - 1; i++)
+            for (int64_t j = 1; j < NE - 1; j++) {
+                const double fj_c = 0.5 * (D(i, j + 1) - D(i, j - 1));
+                const double fj_p = 0.5 * (D(i + 1, j + 1) - D(i + 1, j - 1));
+                const double fj_m = 0.5 * (D(i - 1, j + 1) - D(i - 1, j - 1));
+                const double fi_c = 0.5 * (D(i + 1, j) - D(i - 1, j));
+                const double fi_p = 0.5 * (D(i + 1, j + 1) - D(i - 1, j + 1));
+                const double fi_m = 0.5 * (D(i + 1, j - 1) - D(i - 1, j - 1));
+                double gii = 0.5 * (G(i, j, 0) + G(i + 1, j, 0));
+                double gij = 0.5 * (G(i, j, 1) + G(i + 1, j, 1));
+                double s = 0.5 * (Q(i, j) + Q(i + 1, j));
+                const double Fi_p = s * (gii * (D(i + 1, j) - D(i, j)) + gij * 0.5 * (fj_c + fj_p));
+                gii = 0.5 * (G(i, j, 0) + G(i - 1, j, 0));
+                gij = 0.5 * (G(i, j, 1) + G(i - 1, j, 1));
+                s = 0.5 * (Q(i, j) + Q(i - 1, j));
+                const double Fi_m = s * (gii * (D(i, j) - D(i - 1, j)) + gij * 0.5 * (fj_c + fj_m));
+                double gjj = 0.5 * (G(i, j, 2) + G(i, j + 1, 2));
+                gij = 0.5 * (G(i, j, 1) + G(i, j + 1, 1));
+                s = 0.5 * (Q(i, j) + Q(i, j + 1));
+                const double Fj_p = s * (gjj * (D(i, j + 1) - D(i, j)) + gij * 0.5 * (fi_c + fi_p));
+                gjj = 0.5 * (G(i, j, 2) + G(i, j - 1, 2));
+                gij = 0.5 * (G(i, j, 1) + G(i, j - 1, 1));
+                s = 0.5 * (Q(i, j) + Q(i, j - 1));
+                const double Fj_m = s * (gjj * (D(i, j) - D(i, j - 1)) + gij * 0.5 * (fi_c + fi_m));
+                out[(f * NE + i) * NE + j] = (Fi_p - Fi_m + Fj_p - Fj_m) / Q(i, j);
+            }
+#undef D
+#undef G
+#undef Q
+}

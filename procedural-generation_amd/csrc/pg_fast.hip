@@ -772,13 +772,18 @@ void launch_trace(int64_t variant, int grid, hipStream_t s, const TraceCfg& k) {
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ float erosion_scale(int64_t c, const double* height, const double* sediment,
                                                const double* base, const ll* acc_fx, ll e, double fl_next,
-                                               double iter_erode, double inv) {
+                                               double iter_erode, const double* ecap, double inv) {
     const double acc = (double)acc_fx[c] * inv;
     const double s_c = height[c] + sediment[c] + acc;
     const double b = base[c];
     double fl = s_c >= b ? b : s_c;
     fl = fmax(fl, fl_next);
-    double allowed = fmin(iter_erode + acc, s_c - fl);
+    double room = iter_erode + acc;
+    if (ecap) {  // the slope limit: at most erode_cap + this iteration's net change (particle.py:944-947)
+        const double r2 = ecap[c] + acc;
+        if (r2 < room) room = r2;
+    }
+    double allowed = fmin(room, s_c - fl);
     if (allowed < 0.0) allowed = 0.0;
     const double want = (double)e * inv;
     return want > allowed ? (float)(allowed / want) : 1.0f;
@@ -787,13 +792,14 @@ __device__ __forceinline__ float erosion_scale(int64_t c, const double* height, 
 // sweep 1: floor = the lowest surface stepped to (frozen)
 __global__ void erosion_scale_kernel(int64_t n, const double* height, const double* sediment, const double* base,
                                      const ll* ero_fx, const ull* floor_key, const ll* acc_fx, float* r1,
-                                     double iter_erode, double hscale) {
+                                     double iter_erode, const double* ecap, double hscale) {
     const double inv = 1.0 / hscale;
     GRID_STRIDE(c, n) {
         const ll e = ero_fx[c];
         const ull fk = floor_key[c];
         r1[c] = e > 0 ? erosion_scale(c, height, sediment, base, acc_fx, e,
-                                      fk != ~0ull ? (double)fkey_value((unsigned)(fk >> 32)) : -1.0e300, iter_erode, inv)
+                                      fk != ~0ull ? (double)fkey_value((unsigned)(fk >> 32)) : -1.0e300, iter_erode, ecap,
+                                      inv)
                       : 1.0f;
     }
 }
@@ -801,7 +807,7 @@ __global__ void erosion_scale_kernel(int64_t n, const double* height, const doub
 // sweep 2: that floor lowered by what sweep 1 lets the floor cell lose
 __global__ void erosion_scale2_kernel(int64_t n, const double* height, const double* sediment, const double* base,
                                       const ll* ero_fx, const ull* floor_key, const ll* acc_fx, const float* r1,
-                                      float* rscale, double iter_erode, double hscale) {
+                                      float* rscale, double iter_erode, const double* ecap, double hscale) {
     const double inv = 1.0 / hscale;
     GRID_STRIDE(c, n) {
         const ll e = ero_fx[c];
@@ -813,7 +819,7 @@ __global__ void erosion_scale2_kernel(int64_t n, const double* height, const dou
                 const int64_t nc = (int64_t)(unsigned)(fk & 0xffffffffull);
                 fl_next = (double)fkey_value((unsigned)(fk >> 32)) - (double)ero_fx[nc] * inv * (double)r1[nc];
             }
-            r = erosion_scale(c, height, sediment, base, acc_fx, e, fl_next, iter_erode, inv);
+            r = erosion_scale(c, height, sediment, base, acc_fx, e, fl_next, iter_erode, ecap, inv);
         }
         rscale[c] = r;
     }
@@ -1288,10 +1294,11 @@ int fast_particles(const pg_fast_fields* fl, const pg_trace_params* prm, const p
         launch_trace<false>(fp->variant, grid, s, k);
         if (fp->time_trace && timer_record(s, false)) return -1;
         hipLaunchKernelGGL(erosion_scale_kernel, dim3(grid_for(n)), dim3(TPB), 0, s, n, fl->height, fl->sediment, fl->base,
-                           k.ero_fx, k.floor_key, at<ll>(work, L.acc_fx), at<float>(work, L.r1), fp->iter_erode, hscale);
+                           k.ero_fx, k.floor_key, at<ll>(work, L.acc_fx), at<float>(work, L.r1), fp->iter_erode,
+                           fp->erode_cap, hscale);
         hipLaunchKernelGGL(erosion_scale2_kernel, dim3(grid_for(n)), dim3(TPB), 0, s, n, fl->height, fl->sediment,
                            fl->base, k.ero_fx, k.floor_key, at<ll>(work, L.acc_fx), at<float>(work, L.r1),
-                           at<float>(work, L.rscale), fp->iter_erode, hscale);
+                           at<float>(work, L.rscale), fp->iter_erode, fp->erode_cap, hscale);
         // pass 1: the same trajectories, erosion scaled per cell, deficits cancel deposits
         k.pass = 1;
         if (fp->time_trace && timer_record(s, true)) return -1;
@@ -1679,6 +1686,7 @@ extern "C" int pg_batch_iterate(const pg_fast_fields* wins, int64_t n_win, const
     k.queue_base = qbase;
     int grid = 0;
     PG_REQUIRE(f0.variant >= 0 && f0.variant <= 2, "pg_batch_iterate: variant must be 0, 1 or 2");
+    PG_REQUIRE(!f0.erode_cap, "pg_batch_iterate: the slope limit (erode_cap) is not supported on batched windows");
     if (persistent_grid(trace_fn<true>(f0.variant), TRACE_TPB, f0.grid_blocks, &grid)) return -1;
     const pg_fast_fields& a = wins[0];  // arena bases
     for (int c = 0; c < chunks; c++) {
@@ -1688,10 +1696,11 @@ extern "C" int pg_batch_iterate(const pg_fast_fields* wins, int64_t n_win, const
         launch_trace<true>(f0.variant, grid, s, k);
         if (f0.time_trace && timer_record(s, false)) return -1;
         hipLaunchKernelGGL(erosion_scale_kernel, dim3(grid_for(n)), dim3(TPB), 0, s, n, a.height, a.sediment, a.base,
-                           k.ero_fx, k.floor_key, at<ll>(work, L.acc_fx), at<float>(work, L.r1), f0.iter_erode, hscale);
+                           k.ero_fx, k.floor_key, at<ll>(work, L.acc_fx), at<float>(work, L.r1), f0.iter_erode,
+                           (const double*)nullptr, hscale);
         hipLaunchKernelGGL(erosion_scale2_kernel, dim3(grid_for(n)), dim3(TPB), 0, s, n, a.height, a.sediment, a.base,
                            k.ero_fx, k.floor_key, at<ll>(work, L.acc_fx), at<float>(work, L.r1),
-                           at<float>(work, L.rscale), f0.iter_erode, hscale);
+                           at<float>(work, L.rscale), f0.iter_erode, (const double*)nullptr, hscale);
         k.pass = 1;
         if (f0.time_trace && timer_record(s, true)) return -1;
         launch_trace<true>(f0.variant, grid, s, k);

@@ -41,7 +41,8 @@ class FastParams(ctypes.Structure):
                 ("rng_inc_hi", ctypes.c_uint64), ("n_particles", i64), ("chunks", i64), ("iter_deposit", dbl),
                 ("iter_erode", dbl), ("conc_chunk", dbl), ("fan_slope", dbl),
                 ("offshore_writeoff", dbl), ("ema", dbl), ("min_volume_frac", dbl), ("height_bits", i64),
-                ("track_bits", i64), ("grid_blocks", i64), ("time_trace", i64), ("rng_dev", vp), ("variant", i64)]
+                ("track_bits", i64), ("grid_blocks", i64), ("time_trace", i64), ("rng_dev", vp), ("variant", i64),
+                ("erode_cap", vp)]
 
 
 class GradArgs(ctypes.Structure):
@@ -106,6 +107,21 @@ SIGNATURES = [
                               ctypes.POINTER(FastParams), ctypes.POINTER(GradArgs), vp, i64, vp, vp]),
     ("pg_grad_reverse", i32, [i64, i64, i64, ctypes.POINTER(TraceParams), ctypes.POINTER(GradArgs), vp, vp, vp, vp, vp]),
     ("pg_pcg64_random", i32, [ctypes.c_uint64, ctypes.c_uint64, ctypes.c_uint64, ctypes.c_uint64, i64, i64, vp, vp]),
+    ("pg_talus", i32, [i64, vp, dbl, dbl, vp, vp]),
+    ("pg_thermal_pass_a", i32, [i64, i64, vp, vp, vp, vp, vp, dbl, dbl, vp, vp, vp]),
+    ("pg_thermal_pass_r", i32, [i64, i64, vp, vp, vp, vp, vp, dbl, vp, vp, dbl, vp]),
+    ("pg_thermal_pass_b", i32, [i64, i64, vp, vp, vp, vp, vp, dbl, vp, vp, vp, vp, vp]),
+    ("pg_mass_wasting_pass", i32, [i64, i64, vp, vp, vp, vp, vp, dbl, dbl, dbl, vp, vp]),
+    ("pg_creep_mask", i32, [i64, vp, vp, vp, vp, vp, vp]),
+    ("pg_slope_erode_cap", i32, [i64, i64, vp, vp, vp, dbl, dbl, vp, vp]),
+    ("pg_laplacian", i32, [i64, i64, vp, vp, vp, dbl, ctypes.c_int32, dbl, dbl, vp, vp]),
+    ("pg_interior_sums_work_bytes", i64, []),
+    ("pg_interior_sums", i32, [vp, vp, i64, i64, i64, ctypes.c_int32, vp, vp, vp]),
+    ("pg_rebound", i32, [i64, vp, vp, vp, vp, vp]),
+    ("pg_surface", i32, [i64, vp, vp, vp, vp]),
+    ("pg_iso_accumulate", i32, [i64, vp, vp, vp, vp, vp]),
+    ("pg_scale", i32, [i64, vp, dbl, vp, vp]),
+    ("pg_divide", i32, [i64, vp, dbl, vp, vp]),
 ]
 
 _lib = None

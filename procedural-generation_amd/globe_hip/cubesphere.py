@@ -223,6 +223,13 @@ class Grid:
         det = g[..., 0] * g[..., 2] - g[..., 1] ** 2
         return np.stack([g[..., 2] / det, -g[..., 1] / det, g[..., 0] / det], axis=-1).astype(np.float32)
 
+    @cached_property
+    def sqrt_det_metric(self) -> np.ndarray:
+        """sqrt(det g) of the cell metric, float32 (cubesphere.py:504-507):
+        the area of a cell in units of cell_size_m^2."""
+        g = self.metric.astype(np.float64)
+        return np.sqrt(g[..., 0] * g[..., 2] - g[..., 1] ** 2).astype(np.float32)
+
     def latitude(self) -> np.ndarray:
         """Latitude (radians) of every extended cell centre, +Z the pole."""
         return np.arcsin(np.clip(self.centers[..., 2], -1.0, 1.0))

@@ -44,6 +44,8 @@ def save_checkpoint(directory, state, params_hash: str, keep: int = 2) -> Path:
         arrays[n] = _host(t)
     if getattr(state, "route", None) is not None:
         arrays["route"] = _host(state.route)
+    if getattr(state, "iso_acc", None) is not None:  # run.py:112-113: a rebound not yet applied is state
+        arrays["iso_acc"] = _host(state.iso_acc)
     with open(tmp, "wb") as fh:
         np.savez(fh, **arrays)
     tmp.replace(p)
@@ -116,4 +118,13 @@ def load_checkpoint(state, path, meta: dict) -> None:
             t.copy_(torch.from_numpy(np.ascontiguousarray(src)).to(t.dtype))
         if "route" in z.files and getattr(state, "route", None) is not None:
             state.route.copy_(torch.from_numpy(np.ascontiguousarray(z["route"])))
+        if "iso_acc" in z.files:  # run.py:209
+            acc = np.ascontiguousarray(z["iso_acc"], np.float64)
+            if tuple(acc.shape) != tuple(state.height.shape):
+                raise ValueError(f"iso_acc: checkpoint shape {acc.shape}, state {tuple(state.height.shape)}")
+            if getattr(state, "iso_acc", None) is None:
+                state.iso_acc = torch.zeros_like(state.height)
+            state.iso_acc.copy_(torch.from_numpy(acc))
+        elif getattr(state, "iso_acc", None) is not None:
+            state.iso_acc.zero_()
     state.iteration = int(meta["iteration"])

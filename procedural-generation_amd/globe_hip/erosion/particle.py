@@ -187,3 +187,158 @@ def apply_uplift(height, uplift, mask, cap, mean):
     _need(mask, torch.uint8, e, "mask")
     check(lib().pg_apply_uplift(height.numel(), _ptr(height), _ptr(uplift), _ptr(mask), 0.0 if cap is None else float(cap),
                                 int(cap is not None), float(mean), _stream()), "apply_uplift")
+
+
+# ---------------------------------------------------------------------------
+# side passes: mass wasting, slope limit, the isostasy smoother
+# ---------------------------------------------------------------------------
+#: particle.py:99
+DEP_FLOOR = 0.02
+
+
+def _stencil_args(height, sediment, metric, talus, mask):
+    F, NE, _ = height.shape
+    e = (F, NE, NE)
+    _need(height, torch.float64, e, "height")
+    _need(sediment, torch.float64, e, "sediment")
+    _need(metric, torch.float32, e + (3,), "metric")
+    _need(talus, torch.float64, e, "talus")
+    _need(mask, torch.uint8, e, "mask")
+    if NE < 3:
+        raise ValueError("thermal passes need NE >= 3")
+    return F, NE, e
+
+
+def talus(hardness, soft, hard, out):
+    """maps.py:1062: soft + (hard - soft) * hardness, in float32, widened."""
+    _need(hardness, torch.float32, None, "hardness")
+    _need(out, torch.float64, tuple(hardness.shape), "talus")
+    check(lib().pg_talus(hardness.numel(), _ptr(hardness), float(soft), float(hard), _ptr(out), _stream()), "talus")
+
+
+def thermal_pass_a(height, sediment, metric, talus, mask, thermal_rate, thermal_max, out_total, scale):
+    """particle.py:1128-1160 (out_total may be None)."""
+    F, NE, e = _stencil_args(height, sediment, metric, talus, mask)
+    _need(scale, torch.float64, e, "scale")
+    if out_total is not None:
+        _need(out_total, torch.float64, e, "out_total")
+    check(lib().pg_thermal_pass_a(F, NE, _ptr(height), _ptr(sediment), _ptr(metric), _ptr(talus), _ptr(mask),
+                                  float(thermal_rate), float(thermal_max), _ptr(out_total), _ptr(scale), _stream()),
+          "thermal_pass_a")
+
+
+def thermal_pass_r(height, sediment, metric, talus, mask, thermal_rate, scale, rscale, dep_floor):
+    """particle.py:1163-1187."""
+    F, NE, e = _stencil_args(height, sediment, metric, talus, mask)
+    _need(scale, torch.float64, e, "scale")
+    _need(rscale, torch.float64, e, "rscale")
+    check(lib().pg_thermal_pass_r(F, NE, _ptr(height), _ptr(sediment), _ptr(metric), _ptr(talus), _ptr(mask),
+                                  float(thermal_rate), _ptr(scale), _ptr(rscale), float(dep_floor), _stream()),
+          "thermal_pass_r")
+
+
+def thermal_pass_b(height, sediment, metric, talus, mask, thermal_rate, out_total, scale, rscale, new_height, new_sediment):
+    """particle.py:1190-1227 (out_total is not read, as in the reference)."""
+    F, NE, e = _stencil_args(height, sediment, metric, talus, mask)
+    for t, n in ((scale, "scale"), (rscale, "rscale"), (new_height, "new_height"), (new_sediment, "new_sediment")):
+        _need(t, torch.float64, e, n)
+    check(lib().pg_thermal_pass_b(F, NE, _ptr(height), _ptr(sediment), _ptr(metric), _ptr(talus), _ptr(mask),
+                                  float(thermal_rate), _ptr(scale), _ptr(rscale), _ptr(new_height), _ptr(new_sediment),
+                                  _stream()), "thermal_pass_b")
+
+
+def mass_wasting_pass(height, sediment, metric, talus, mask, rate, cap, dep_floor, work):
+    """maps.py:1073-1085: the three passes, written back into height /
+    sediment; work: float64 scratch of 4 x height.numel()."""
+    F, NE, e = _stencil_args(height, sediment, metric, talus, mask)
+    _need(work, torch.float64, None, "work")
+    if work.numel() < 4 * height.numel():
+        raise ValueError("mass_wasting_pass: work needs 4 x the cells")
+    check(lib().pg_mass_wasting_pass(F, NE, _ptr(height), _ptr(sediment), _ptr(metric), _ptr(talus), _ptr(mask),
+                                     float(rate), float(cap), float(dep_floor), _ptr(work), _stream()),
+          "mass_wasting_pass")
+
+
+def creep_mask(height, sediment, base, mask, out):
+    """maps.py:1067: FROZEN where the surface is below base, else mask."""
+    e = tuple(height.shape)
+    for t, n in ((height, "height"), (sediment, "sediment"), (base, "base")):
+        _need(t, torch.float64, e, n)
+    _need(mask, torch.uint8, e, "mask")
+    _need(out, torch.uint8, e, "out")
+    check(lib().pg_creep_mask(height.numel(), _ptr(height), _ptr(sediment), _ptr(base), _ptr(mask), _ptr(out), _stream()),
+          "creep_mask")
+
+
+def slope_erode_cap(height, sediment, mask, coef, exit_slope, out):
+    """particle.py:1045-1076."""
+    F, NE, _ = height.shape
+    e = (F, NE, NE)
+    _need(height, torch.float64, e, "height")
+    _need(sediment, torch.float64, e, "sediment")
+    _need(mask, torch.uint8, e, "mask")
+    _need(out, torch.float64, e, "out")
+    check(lib().pg_slope_erode_cap(F, NE, _ptr(height), _ptr(sediment), _ptr(mask), float(coef), float(exit_slope),
+                                   _ptr(out), _stream()), "slope_erode_cap")
+
+
+def laplacian(data, metric_inv, sqrt_det, cell_size_sq, mode, kappa, c2, out):
+    """field.py:179-188 / 372-404 (mode 0: the raw operator; 1: float32
+    Laplacian x c2; 2: data + kappa x Laplacian x c2)."""
+    F, NE, _ = data.shape
+    e = (F, NE, NE)
+    _need(data, torch.float64, e, "data")
+    _need(metric_inv, torch.float32, e + (3,), "metric_inv")
+    _need(sqrt_det, torch.float32, e, "sqrt_det")
+    _need(out, torch.float64, e, "out")
+    check(lib().pg_laplacian(F, NE, _ptr(data), _ptr(metric_inv), _ptr(sqrt_det), float(cell_size_sq), int(mode),
+                             float(kappa), float(c2), _ptr(out), _stream()), "laplacian")
+
+
+def interior_sums(data, mask, N, H, what, work, out4):
+    """{sum of v / v^2 / |v|, count, max, min} over interior cells (mask
+    None: all; else mask == ACTIVE), into the device double[4] out4."""
+    F = data.shape[0]
+    _need(data, torch.float64, None, "data")
+    if mask is not None:
+        _need(mask, torch.uint8, tuple(data.shape), "mask")
+    _need(out4, torch.float64, (4,), "out4")
+    check(lib().pg_interior_sums(_ptr(data), _ptr(mask), F, int(N), int(H), int(what), _ptr(work), _ptr(out4), _stream()),
+          "interior_sums")
+
+
+def rebound(height, data, mask, sums):
+    e = tuple(height.shape)
+    _need(height, torch.float64, e, "height")
+    _need(data, torch.float64, e, "data")
+    _need(mask, torch.uint8, e, "mask")
+    check(lib().pg_rebound(height.numel(), _ptr(height), _ptr(data), _ptr(mask), _ptr(sums), _stream()), "rebound")
+
+
+def surface(height, sediment, out):
+    e = tuple(height.shape)
+    for t, n in ((height, "height"), (sediment, "sediment"), (out, "out")):
+        _need(t, torch.float64, e, n)
+    check(lib().pg_surface(height.numel(), _ptr(height), _ptr(sediment), _ptr(out), _stream()), "surface")
+
+
+def iso_accumulate(iso_acc, height, sediment, s0):
+    e = tuple(height.shape)
+    for t, n in ((iso_acc, "iso_acc"), (height, "height"), (sediment, "sediment"), (s0, "s0")):
+        _need(t, torch.float64, e, n)
+    check(lib().pg_iso_accumulate(height.numel(), _ptr(iso_acc), _ptr(height), _ptr(sediment), _ptr(s0), _stream()),
+          "iso_accumulate")
+
+
+def scale(x, a, out):
+    """out = a * x (a == 0: zeros, x may be None)."""
+    _need(out, torch.float64, None, "out")
+    if x is not None:
+        _need(x, torch.float64, tuple(out.shape), "x")
+    check(lib().pg_scale(out.numel(), _ptr(x), float(a), _ptr(out), _stream()), "scale")
+
+
+def divide(x, v, out):
+    _need(x, torch.float64, None, "x")
+    _need(out, torch.float64, tuple(x.shape), "out")
+    check(lib().pg_divide(x.numel(), _ptr(x), float(v), _ptr(out), _stream()), "divide")
