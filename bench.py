@@ -250,6 +250,77 @@ def batch_run(n=64, iterations=50, seed=0, per_window_iterations=3):
             "autotune_trace_ns_per_move": {f"{g}/{v}": ms for (g, v), ms in tuner.times.items()}}
 
 
+#: algorithmic bytes per extended cell of the side passes (float64 state, float32 metric / sqrt g)
+#: thermal a: h, s, metric, talus, mask in + scale out = 45; r: + scale in, rscale out = 53;
+#: b: + scale, rscale in, new h, s out = 69; write-back copy 32  -> one mass-wasting pass 199 B;
+#: thermal_erosion = talus (12) + pass + creep mask (26) + zero talus (8) + pass = 444 B
+WASTING_BYTES_PER_CELL = 199.0
+THERMAL_CREEP_BYTES_PER_CELL = 2 * WASTING_BYTES_PER_CELL + 12.0 + 26.0 + 8.0
+#: one smoothing step: data, metric_inv, sqrt g in, data out
+SMOOTH_BYTES_PER_CELL = 32.0
+
+
+def side_run(grid, f, p, reps=5, cpu=True):
+    """The side passes at the earth preset's settings on the same globe:
+    thermal + creep (maps.thermal_erosion) and one isostatic rebound
+    (maps.apply_isostasy), timed with HIP events on the launch stream."""
+    import torch
+
+    from globe_hip.erosion import maps
+
+    dev = torch.device("cuda:0")
+    ep = p.erosion
+    for k, v in dict(thermal_rate=0.5, creep_rate=0.02, thermal_max=50.0, isostasy=0.8, flexure_km=60.0).items():
+        setattr(ep, k, v)
+    args = (grid, f["bedrock_m"], f["hardness"], f["precip"], f["evap"], f["uplift_m"], ep)
+    st = maps.ErosionState.from_grid(*args, device=dev, mode="fast")
+    maps.prepare_side_passes(st, p)
+    n = st.height.numel()
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+
+    def timed(fn, setup=None):
+        ms = []
+        for r in range(reps + 1):
+            if setup is not None:
+                setup()
+            torch.cuda.synchronize()
+            ev[0].record()
+            fn()
+            ev[1].record()
+            torch.cuda.synchronize()
+            if r:
+                ms.append(ev[0].elapsed_time(ev[1]))
+        return sorted(ms)[len(ms) // 2]
+
+    t_w = timed(lambda: maps.thermal_erosion(st, p))
+    rng = np.random.default_rng(0)
+    acc = rng.standard_normal(tuple(st.height.shape)) * 1e-3
+    sig, kappa, steps = maps.isostasy_steps(st, ep)
+
+    acc_dev = torch.from_numpy(acc).to(dev)
+    t_i = timed(lambda: maps.apply_isostasy(st, p), setup=lambda: st.iso_acc.copy_(acc_dev))
+    wasting_gbs = THERMAL_CREEP_BYTES_PER_CELL * n / (t_w * 1e-3) / 1e9
+    smooth_gbs = steps * SMOOTH_BYTES_PER_CELL * n / (t_i * 1e-3) / 1e9
+    out = {"workload": f"the same globe, thermal_rate 0.5 + creep_rate 0.02 (two mass-wasting passes), isostasy 0.8 "
+                       f"with flexure 60 km = {sig:.2f} cells: {steps} smoothing steps of kappa {kappa:.4f}",
+           "thermal_creep_ms": t_w, "thermal_creep_GBps": wasting_gbs, "thermal_frac_of_hbm": wasting_gbs / HBM_PEAK_GBS,
+           "isostasy_ms": t_i, "smoothing_steps": steps, "smoothing_GBps": smooth_gbs,
+           "smoothing_frac_of_hbm": smooth_gbs / HBM_PEAK_GBS,
+           "bytes_per_cell": {"thermal_erosion": THERMAL_CREEP_BYTES_PER_CELL, "smoothing_step": SMOOTH_BYTES_PER_CELL}}
+    log(f"side passes: thermal+creep {t_w:.2f} ms ({wasting_gbs:.0f} GB/s), isostasy {t_i:.1f} ms for {steps} steps "
+        f"({smooth_gbs:.0f} GB/s)")
+    if cpu:
+        from oracle import host
+
+        ost = host.OracleState.from_grid(*args)
+        t = time.perf_counter()
+        host.mass_wasting_pass(ost, host.talus_of(ep, ost.hardness), 0.5, 50.0 / grid.cell_size_m,
+                               dep_floor=float(ep.dep_floor_m) / grid.cell_size_m)
+        out["cpu_thermal_pass_ms"] = 1000.0 * (time.perf_counter() - t)
+        log(f"side passes: oracle thermal pass (1 thread) {out['cpu_thermal_pass_ms']:.0f} ms")
+    return out
+
+
 def copy_probe(mib):
     """A plain copy of known size: its FETCH_SIZE / WRITE_SIZE per dispatch
     calibrate the HBM counters (2 x mib MiB of traffic per copy)."""
@@ -275,6 +346,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--no-side", action="store_true")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="only run 3 device-to-device copies of this many MiB (HBM counter calibration) and exit")
     a = ap.parse_args()
@@ -330,6 +402,11 @@ def main():
 
         torch.cuda.empty_cache()
         out["batched_windows"] = batch_run()
+    if not a.no_side:
+        import torch
+
+        torch.cuda.empty_cache()
+        out["side_passes"] = side_run(grid, f, p, cpu=not a.no_cpu)
     if not a.no_cpu:
         out["cpu_baseline"] = cpu_baseline(grid, f, p)
     print(json.dumps(out), flush=True)

@@ -48,7 +48,8 @@ typedef struct pg_trace_params {
 
 /* Message of the last failed call on this thread ("" if none). */
 const char* pg_last_error(void);
-/* ABI version of this library. */
+/* ABI version of this library: 2 (bumped whenever a struct layout or a
+ * signature changes; 2 added pg_fast_params.erode_cap). */
 int pg_abi_version(void);
 
 /* ---- geometry, host side (bake/globe/cubesphere.py:70-220) ------------- */

@@ -29,6 +29,17 @@
  *                               Equality with numba's own codegen is NOT pinned:
  *                               numba is not installable here.
  *   fold / ema / pack / spawn_cells -- closed forms (elementwise / brute force).
+ *   thermal passes a / r / b  -- closed forms (a spike on a plane: D8 offsets,
+ *                               diagonal distance, the 2 x max and thermal_max
+ *                               caps, sediment before bedrock), exact
+ *                               conservation between active cells, the
+ *                               submerged receiver's room, mirror symmetry.
+ *   slope_erode_cap          -- the reference's own known-answer test
+ *                               (test_erosion.py:1599-1619) and a tilted plane.
+ *   laplacian                -- quadratics under a constant metric (each
+ *                               inverse-metric component's role) and the
+ *                               reference's spherical-harmonic eigenfunction
+ *                               test (test_cubesphere.py:314-336).
  */
 #include <math.h>
 #include <stdint.h>

@@ -101,7 +101,8 @@ def run_iteration(state, params, iteration_key, n_particles=None, particles_per_
     plus ``moves`` (droplet-steps executed).  With ``sync_stats=False`` the
     statistics stay on the device (``state._fast.stats``) and nothing waits
     (default: ``state.fast_sync_stats``, True when unset)."""
-    from .maps import _check_scope, _eparams, chunk_size, discharge_scales, lake_park_of, max_steps_of, trace_params
+    from .maps import (_check_scope, _eparams, chunk_size, discharge_scales, lake_park_of, max_steps_of,
+                       slope_limit_cap, slope_limited_deposit, trace_params)
 
     ep = _eparams(params)
     _check_scope(ep, state)
@@ -121,10 +122,11 @@ def run_iteration(state, params, iteration_key, n_particles=None, particles_per_
     max_steps = max_steps_of(ep, state.N)
     disc_sat, mom_scale = discharge_scales(ep, state, 1.0, n_particles, n_rain=1)
     prm = trace_params(ep, state, 1.0, 1, 1, disc_sat, mom_scale, max_steps, 0.0)
+    ecap = slope_limit_cap(state, ep)
     slo, shi, ilo, ihi = pcg64_state(rng)
     fp = FastParams(rng_state_lo=slo, rng_state_hi=shi, rng_inc_lo=ilo, rng_inc_hi=ihi, n_particles=int(n_particles),
                     chunks=int(chunks if chunks is not None else getattr(state, "fast_chunks", DEFAULT_CHUNKS)),
-                    iter_deposit=cell_units(ep, "iter_deposit", unit), iter_erode=cell_units(ep, "iter_erode", unit),
+                    iter_deposit=slope_limited_deposit(ep, unit), iter_erode=cell_units(ep, "iter_erode", unit),
                     conc_chunk=float(getattr(state, "fast_conc_chunk", -1)) if getattr(state, "fast_conc_chunk", -1) >= 0
                     else float(chunk_size(ep, n_particles)), fan_slope=float(ep.fan_slope),
                     offshore_writeoff=float(ep.offshore_writeoff),
@@ -132,7 +134,8 @@ def run_iteration(state, params, iteration_key, n_particles=None, particles_per_
                     track_bits=TRACK_BITS, grid_blocks=int(getattr(state, "fast_grid_blocks", 0)),
                     time_trace=int(bool(getattr(state, "fast_time_trace", False))),
                     variant=int(getattr(state, "fast_variant", 0)),
-                    rng_dev=(state._fast_rng_dev.data_ptr() if getattr(state, "_fast_rng_dev", None) is not None else None))
+                    rng_dev=(state._fast_rng_dev.data_ptr() if getattr(state, "_fast_rng_dev", None) is not None else None),
+                    erode_cap=None if ecap is None else ecap.data_ptr())
     check(lib().pg_fast_iterate(ctypes.byref(ctx.fields), ctypes.byref(prm), ctypes.byref(fp),
                                 ctypes.c_void_p(ctx.work.data_ptr()), ctx.work.numel(),
                                 ctypes.c_void_p(ctx.stats.data_ptr()), pk._stream()), "pg_fast_iterate")
@@ -166,9 +169,10 @@ def _signed(w: int) -> int:
 
 
 class IterationGraph:
-    """A whole fast-mode erosion iteration -- particle pass, uplift, datum
-    hold, halo exchange (bake/globe/erosion/maps.py:1516-1617 in the plain
-    profile) -- captured once as a HIP graph and replayed per iteration.
+    """A whole fast-mode erosion iteration -- particle pass, mass wasting,
+    uplift, isostasy, datum hold, halo exchange (bake/globe/erosion/maps.py:
+    1516-1617) -- captured once as a HIP graph and replayed per iteration
+    (two graphs when isostasy is on: with and without its application).
 
     The only per-iteration input from the host is the iteration's PCG64
     state (``params.rng(stage, *key)``), copied into a device buffer before
@@ -199,12 +203,22 @@ class IterationGraph:
         state.select_work()
         if state.spherical:
             state.halo_tables()
+        maps.prepare_side_passes(state, params)
         torch.cuda.synchronize()
         self._load_rng((0,))
-        self.graph = torch.cuda.CUDAGraph()
+        # with isostasy on, a second graph holds the iterations that apply it
+        self.iso_every = max(int(getattr(ep, "isostasy_every", 10)), 1) \
+            if state.spherical and float(getattr(ep, "isostasy", 0.0)) > 0.0 else 0
         it0 = state.iteration
+        self.graph = torch.cuda.CUDAGraph()
         with torch.cuda.graph(self.graph):
-            self.result = maps.step(state, params, (0,), n_particles=n_particles)
+            self.result = maps.step(state, params, (0,), n_particles=n_particles, isostasy_due=False)
+        self.graph_iso = None
+        if self.iso_every:
+            state.iteration = it0
+            self.graph_iso = torch.cuda.CUDAGraph()
+            with torch.cuda.graph(self.graph_iso):
+                self.result_iso = maps.step(state, params, (0,), n_particles=n_particles, isostasy_due=True)
         state.iteration = it0
         self.stats = ctx.stats
 
@@ -217,5 +231,8 @@ class IterationGraph:
         """Run one iteration (asynchronous)."""
         key = iteration_key if isinstance(iteration_key, (tuple, list)) else (int(iteration_key),)
         self._load_rng(key)
-        self.graph.replay()
+        if self.graph_iso is not None and (self.state.iteration + 1) % self.iso_every == 0:
+            self.graph_iso.replay()
+        else:
+            self.graph.replay()
         self.state.iteration += 1

@@ -23,9 +23,9 @@ Two modes (``ErosionState.mode``):
   and every per-cell pass run on the device with no host round trip
   (``globe_hip.erosion.fast``).
 
-Everything after the particles (uplift, datum hold, halo exchange) runs on
-the device in both modes.  Side passes outside the plain profile (thermal
-and creep, routing flood, lakes, glacial carving, isostasy, slope limits)
+Everything after the particles (mass wasting, uplift, isostasy, datum
+hold, halo exchange) and the slope limits run on the device in both modes.
+The remaining side passes (routing flood, sea mask, lakes, glacial carving)
 raise NotImplementedError.
 """
 from __future__ import annotations
@@ -137,7 +137,9 @@ class ErosionState:
     deposit_on_exit: bool = False
     roots: torch.Tensor | None = None
     inflow_volume: float = 0.0
+    iso_acc: torch.Tensor | None = None  # maps.py:1369-1371: surface change not yet compensated
     _halo: HaloTables | None = field(default=None, repr=False)
+    _side: dict = field(default_factory=dict, repr=False)
     _uplift_mean: dict = field(default_factory=dict, repr=False)
     _mask_host: np.ndarray | None = field(default=None, repr=False)
     _select_work: torch.Tensor | None = field(default=None, repr=False)
@@ -311,6 +313,31 @@ class ErosionState:
             self._select_work = torch.zeros((n + 7) // 8, dtype=torch.int64, device=self.device)
         return self._select_work
 
+    def side_buffer(self, name: str) -> torch.Tensor:
+        """Device scratch of the side passes, allocated on first use and kept
+        (so a captured graph sees the same buffers on every replay)."""
+        t = self._side.get(name)
+        if t is None:
+            shape, dev = tuple(self.height.shape), self.device
+            if name == "wasting":
+                t = torch.empty((4,) + shape, dtype=torch.float64, device=dev)
+            elif name == "cmask":
+                t = torch.empty(shape, dtype=torch.uint8, device=dev)
+            elif name == "sums":
+                t = torch.zeros((3, 4), dtype=torch.float64, device=dev)
+            elif name == "reduce":
+                t = torch.empty(int(lib().pg_interior_sums_work_bytes()) // 8, dtype=torch.float64, device=dev)
+            elif name == "sqrt_det":
+                if self.grid is None:
+                    raise ValueError("the isostasy smoother needs the state's Grid")
+                t = _dev(np.ascontiguousarray(self.grid.sqrt_det_metric, np.float32), dev)
+            elif name in ("talus", "s0", "load", "load2", "ecap"):
+                t = torch.empty(shape, dtype=torch.float64, device=dev)
+            else:
+                raise KeyError(name)
+            self._side[name] = t
+        return t
+
 
 # ---------------------------------------------------------------------------
 # spawning (maps.py:793-833)
@@ -356,8 +383,6 @@ def spawn_particles(state: ErosionState, n_particles: int, rng: np.random.Genera
 
 
 def _check_scope(ep, state):
-    if float(getattr(ep, "slope_limit_erode", 0.0)) > 0.0 or float(getattr(ep, "slope_limit_deposit", 0.0)) > 0.0:
-        raise NotImplementedError("erosion.slope_limit_* are outside the HIP erosion path")
     if state.route is not None:
         raise NotImplementedError("the routing surface (erosion.flood_every) is outside the HIP erosion path")
 
@@ -395,6 +420,27 @@ def discharge_scales(ep, state, volume0, n_particles, n_rain=None):
         if float(ep.momentum_saturation_cells) > 0.0:
             mom_scale = float(volume0) / (float(ep.momentum_saturation_cells) * rain_per_cell)
     return disc_sat, mom_scale
+
+
+def slope_limit_cap(state: ErosionState, ep):
+    """maps.py:923-927: soillib's per-cell erosion cap of this iteration on
+    the surface it starts from (particle.py:1045-1076), or None when
+    ``slope_limit_erode`` is off."""
+    if not float(getattr(ep, "slope_limit_erode", 0.0)) > 0.0:
+        return None
+    ecap = state.side_buffer("ecap")
+    pk.slope_erode_cap(state.height, state.sediment, state.mask, float(ep.slope_limit_erode),
+                       float(ep.slope_limit_exit), ecap)
+    return ecap
+
+
+def slope_limited_deposit(ep, unit: float) -> float:
+    """maps.py:928-931: iter_deposit, capped by soillib's deposition limit
+    when ``slope_limit_deposit`` is on."""
+    iter_deposit = cell_units(ep, "iter_deposit", unit)
+    if float(getattr(ep, "slope_limit_deposit", 0.0)) > 0.0:
+        iter_deposit = min(iter_deposit, float(ep.slope_limit_deposit) * math.sqrt(2.0) * 0.3)
+    return iter_deposit
 
 
 def lake_park_of(state, ep) -> float:
@@ -456,7 +502,8 @@ def run_iteration(state: ErosionState, params, iteration_key, n_particles=None, 
     state.acc.zero_()
     disc_sat, mom_scale = discharge_scales(ep, state, volume0, n_particles)
     unit = state.height_unit_m
-    iter_deposit = cell_units(ep, "iter_deposit", unit)
+    ecap = slope_limit_cap(state, ep)
+    iter_deposit = slope_limited_deposit(ep, unit)
     max_steps = max_steps_of(ep, state.N)
     lateral = float(getattr(ep, "lateral_rate", 0.0)) > 0.0
     cap = pk.change_list_cap(max_steps, lateral)
@@ -491,7 +538,7 @@ def run_iteration(state: ErosionState, params, iteration_key, n_particles=None, 
         pk.apply_changes(cl_cell, cl_delta, cl_vol, cl_mom, cl_count[:m], cap, state.height, state.sediment, state.acc,
                          state.pending, state.samp, state.disch_track, state.mom_track, state.mask, iter_erode,
                          iter_deposit, float(ep.fan_slope), state.route is not None, dep_floor, float(ep.offshore_writeoff),
-                         None, False, state.lake_load, state.lake_id, state.lake_room, outs[ci])
+                         ecap, ecap is not None, state.lake_load, state.lake_id, state.lake_room, outs[ci])
         pk.chunk_stats(cl_count[:m], sp_death[:m], m, counts)
         t_trace += t2 - t1
         t_apply += time.time() - t2
@@ -526,6 +573,153 @@ def run_iteration(state: ErosionState, params, iteration_key, n_particles=None, 
         log(f"  particles {P} volume {volume0:.3f} mean steps {stats['steps_mean']:.1f} deaths {stats['deaths']} "
             f"clamped {n_clamp} pending {stats['pending_total']:.1f} ({stats['seconds']:.2f}s)")
     return stats
+
+
+# ---------------------------------------------------------------------------
+# mass wasting (maps.py:1050-1085) and isostasy (maps.py:1325-1410)
+# ---------------------------------------------------------------------------
+def thermal_erosion(state: ErosionState, params) -> None:
+    """One pass of talus-limited mass wasting, then (``creep_rate > 0``) the
+    creep pass with talus 0 and submerged cells frozen (maps.py:1050-1070),
+    on the device.  A pass at rate 0 moves nothing (every outflow is
+    0.5 x 0 x excess and the state is rewritten unchanged), so it is skipped."""
+    ep = _eparams(params)
+    unit = state.height_unit_m
+    rate, creep = float(ep.thermal_rate), float(ep.creep_rate)
+    if rate == 0.0 and not creep > 0.0:
+        return
+    talus = state.side_buffer("talus")
+    cap = cell_units(ep, "thermal_max", unit)
+    dep_floor = cell_units(ep, "dep_floor_m", unit)
+    if rate != 0.0:
+        pk.talus(state.hardness, float(ep.talus_slope_soft), float(ep.talus_slope_hard), talus)
+        _mass_wasting_pass(state, talus, rate, cap, None, dep_floor)
+    if creep > 0.0:
+        cmask = state.side_buffer("cmask")
+        pk.creep_mask(state.height, state.sediment, state.base, state.mask, cmask)
+        pk.scale(None, 0.0, talus)
+        _mass_wasting_pass(state, talus, creep, cap, cmask, dep_floor)
+
+
+def _mass_wasting_pass(state: ErosionState, talus, rate: float, cap: float, mask=None, dep_floor: float = pk.DEP_FLOOR):
+    """maps.py:1073-1085: thermal_pass_a, _r, _b and the write-back."""
+    mask = state.mask if mask is None else mask
+    pk.mass_wasting_pass(state.height, state.sediment, state.metric, talus, mask, rate, cap, dep_floor,
+                         state.side_buffer("wasting"))
+
+
+_KAPPA_CACHE: dict = {}
+
+
+def smoothing_kappa(state: ErosionState) -> float:
+    """The largest monotone explicit step of the Laplace-Beltrami smoother,
+    0.9 / lam_max with lam_max by 40 power iterations from seeded white
+    noise (maps.py:1325-1363), on the device; cached per grid.  The norms
+    are device reductions (fixed order), so kappa agrees with numpy's to
+    rounding."""
+    grid = state.grid
+    if grid is None:
+        raise ValueError("the isostasy smoother needs the state's Grid")
+    key = (grid.N, grid.H, float(grid.cell_size_m))
+    if key not in _KAPPA_CACHE:
+        rng = np.random.default_rng(0)
+        N, H, NE = grid.N, grid.H, grid.NE
+        host = np.zeros((6, NE, NE))
+        host[:, H:H + N, H:H + N] = rng.standard_normal((6, N, N))
+        dev = state.device
+        data = _dev(host, dev)
+        ht = state.halo_tables()
+        ht.exchange(data)
+        Lv = torch.empty_like(data)
+        sq = state.side_buffer("sqrt_det")
+        sums = state.side_buffer("sums")
+        red = state.side_buffer("reduce")
+        cs2 = float(grid.cell_size_m) ** 2
+        c2, lam = float(grid.cell_size_m) ** 2, 8.0
+        for _ in range(40):
+            ht.exchange(data, linear=True)
+            pk.laplacian(data, state.metric_inv, sq, cs2, 1, 0.0, c2, Lv)
+            pk.interior_sums(Lv, None, N, H, 1, red, sums[0])
+            pk.interior_sums(data, None, N, H, 1, red, sums[1])
+            s = sums[:2, 0].cpu().numpy()
+            num, den = math.sqrt(float(s[0])), math.sqrt(float(s[1]))
+            if num <= 0.0 or den <= 0.0:
+                break
+            lam = num / den
+            pk.divide(Lv, num, data)
+        _KAPPA_CACHE[key] = 0.9 / max(lam, 1e-6)
+    return _KAPPA_CACHE[key]
+
+
+def isostasy_steps(state: ErosionState, ep):
+    """maps.py:1392-1397: (sigma in cells, kappa, diffusion steps)."""
+    cell = float(state.height_unit_m)
+    sig = float(ep.flexure_km) * 1000.0 / cell
+    sig = min(sig, state.N / 8.0)
+    kappa = smoothing_kappa(state)
+    n = int(np.ceil(sig * sig / (2.0 * kappa))) if sig >= 0.5 else 0
+    return sig, kappa, n
+
+
+def apply_isostasy(state: ErosionState, params) -> dict:
+    """Flexural isostatic response to the surface change accumulated in
+    ``state.iso_acc`` (maps.py:1366-1410), on the device: the load -k x
+    iso_acc, n explicit steps of the seam-aware Laplace-Beltrami smoother
+    (linear halo exchange before each), the mean over the active interior
+    removed, added to ``height`` of the active cells; iso_acc is zeroed.
+    The stats' rebound extremes and load are device scalars in fast mode
+    (nothing waits) and floats in check mode."""
+    ep = _eparams(params)
+    if not state.spherical:
+        raise ValueError("isostasy is a global pass (spherical states only)")
+    if state.iso_acc is None:
+        state.iso_acc = torch.zeros_like(state.height)
+    cell = float(state.height_unit_m)
+    sig, kappa, n = isostasy_steps(state, ep)
+    a, b = state.side_buffer("load"), state.side_buffer("load2")
+    pk.scale(state.iso_acc, -float(ep.isostasy), a)
+    ht = state.halo_tables()
+    sq = state.side_buffer("sqrt_det")
+    cs2 = float(state.grid.cell_size_m) ** 2
+    c2 = cell * cell
+    for _ in range(n):
+        ht.exchange(a, linear=True)
+        pk.laplacian(a, state.metric_inv, sq, cs2, 2, kappa, c2, b)
+        a, b = b, a
+    ht.exchange(a, linear=True)
+    sums, red = state.side_buffer("sums"), state.side_buffer("reduce")
+    pk.interior_sums(a, state.mask, state.N, state.H, 0, red, sums[0])       # mean over the active interior
+    pk.interior_sums(a, None, state.N, state.H, 0, red, sums[1])             # extremes over the interior
+    pk.interior_sums(state.iso_acc, None, state.N, state.H, 2, red, sums[2])  # load moved, |d|
+    pk.rebound(state.height, a, state.mask, sums[0])
+    pk.scale(None, 0.0, state.iso_acc)
+    st = {"sigma_cells": sig, "diffusion_steps": n, "kappa": kappa}
+    if state.mode == "fast":
+        st["device_sums"] = sums
+        return st
+    v = sums.cpu().numpy()
+    mean = v[0, 0] / v[0, 1] if v[0, 1] > 0 else 0.0
+    st.update({"load_abs_cells": float(v[2, 0]), "rebound_max": float(v[1, 2] - mean),
+               "rebound_min": float(v[1, 3] - mean)})
+    return st
+
+
+def prepare_side_passes(state: ErosionState, params) -> None:
+    """Allocate the side passes' buffers and compute the smoother's step
+    before a graph capture (both wait on the device)."""
+    ep = _eparams(params)
+    if float(ep.thermal_rate) != 0.0 or float(ep.creep_rate) > 0.0:
+        for n in ("talus", "wasting", "cmask"):
+            state.side_buffer(n)
+    if float(getattr(ep, "slope_limit_erode", 0.0)) > 0.0:
+        state.side_buffer("ecap")
+    if state.spherical and float(getattr(ep, "isostasy", 0.0)) > 0.0:
+        if state.iso_acc is None:
+            state.iso_acc = torch.zeros_like(state.height)
+        for n in ("s0", "load", "load2", "sums", "reduce", "sqrt_det"):
+            state.side_buffer(n)
+        isostasy_steps(state, ep)
+    torch.cuda.synchronize()
 
 
 # ---------------------------------------------------------------------------
@@ -605,22 +799,36 @@ def start_replay(state: ErosionState, params):
     return info
 
 
-def step(state: ErosionState, params, iteration_key, log=None, after_refresh=None, **kw) -> dict:
-    """One iteration in the reference's order (maps.py:1516-1617), plain
-    profile: particles, uplift, datum hold (globe), halo exchange."""
+def step(state: ErosionState, params, iteration_key, log=None, after_refresh=None, isostasy_due=None, **kw) -> dict:
+    """One iteration in the reference's order (maps.py:1516-1617):
+    particles, mass wasting, uplift, isostasy (every ``isostasy_every``
+    iterations; ``isostasy_due`` overrides that choice, for graph capture),
+    datum hold (globe), halo exchange."""
     ep = _eparams(params)
-    if float(ep.thermal_rate) != 0.0 or float(ep.creep_rate) != 0.0:
-        raise NotImplementedError("thermal / creep passes are outside the HIP erosion path (set thermal_rate = creep_rate = 0)")
     if int(ep.flood_every) > 0 or (state.spherical and int(getattr(ep, "sea_mask_every", 0)) > 0):
         raise NotImplementedError("routing / sea-mask refreshes are outside the HIP erosion path")
-    if state.spherical and (int(ep.glacial_every) > 0 or float(getattr(ep, "isostasy", 0.0)) > 0.0):
-        raise NotImplementedError("glacial / isostasy passes are outside the HIP erosion path")
+    if state.spherical and int(ep.glacial_every) > 0:
+        raise NotImplementedError("the glacial pass is outside the HIP erosion path (set glacial_every = 0)")
     t0 = time.time()
     if after_refresh is not None:
         after_refresh(state)
+    iso = state.spherical and float(getattr(ep, "isostasy", 0.0)) > 0.0
+    if iso:
+        if state.iso_acc is None:
+            state.iso_acc = torch.zeros_like(state.height)
+        s0 = state.side_buffer("s0")
+        pk.surface(state.height, state.sediment, s0)
     st = run_iteration(state, params, iteration_key, log=log, **kw)
     t1 = time.time()
+    thermal_erosion(state, params)
+    if iso:
+        pk.iso_accumulate(state.iso_acc, state.height, state.sediment, s0)
     apply_uplift(state, uplift_cap(state, ep))
+    if iso:
+        due = isostasy_due if isostasy_due is not None else \
+            (state.iteration + 1) % max(int(getattr(ep, "isostasy_every", 10)), 1) == 0
+        if due:
+            st["isostasy"] = apply_isostasy(state, params)
     if state.spherical and hasattr(params, "world"):
         q = hold_datum_device(state, held_land_fraction(state, params), float(getattr(state, "sea_drop", 0.0)))
         st["datum_shift"] = q if state.mode == "fast" else float(q.item())
@@ -632,5 +840,5 @@ def step(state: ErosionState, params, iteration_key, log=None, after_refresh=Non
     return st
 
 
-__all__ = ["ErosionState", "run_iteration", "apply_uplift", "uplift_replay", "start_replay", "hold_datum", "step",
-           "spawn_particles", "height_unit", "max_steps_of"]
+__all__ = ["ErosionState", "run_iteration", "thermal_erosion", "apply_uplift", "uplift_replay", "start_replay",
+           "apply_isostasy", "hold_datum", "step", "spawn_particles", "height_unit", "max_steps_of"]
