@@ -45,6 +45,11 @@ HBM_PEAK_GBS = 8000.0
 #: rocprofv3 FETCH_SIZE (x 2.000, calibrated on a 1-GiB copy) + WRITE_SIZE (x 1.000), mean over both
 #: passes; profiles/trace_kernel_earth_n1024_counters.md holds the runs
 TRACE_TRAFFIC_BYTES = {0: 2.850e9, 1: 7.733e9, 2: 18.006e9}
+#: relaxed 64-bit atomicAdd / atomicMin per droplet move of this workload (both passes: the census's erosion
+#: request and floor key on eroding steps, the applying pass's height change and three tracks), from
+#: rocprofv3 TCC_ATOMIC_sum (which counts one per lane-atomic: 268.43 M for the probe's 268.44 M adds);
+#: every one is performed past L2 (TCC_EA0_ATOMIC_sum equal), profiles/trace_kernel_earth_n1024_atomics.md
+ATOMICS_PER_MOVE = 3.25
 
 
 def log(msg):
@@ -309,6 +314,12 @@ def side_run(grid, f, p, reps=5, cpu=True):
            "bytes_per_cell": {"thermal_erosion": THERMAL_CREEP_BYTES_PER_CELL, "smoothing_step": SMOOTH_BYTES_PER_CELL}}
     log(f"side passes: thermal+creep {t_w:.2f} ms ({wasting_gbs:.0f} GB/s), isostasy {t_i:.1f} ms for {steps} steps "
         f"({smooth_gbs:.0f} GB/s)")
+    eps = float(ep.route_eps) / float(grid.cell_size_m)
+    sweeps = st.refresh_route(eps)
+    t_r = timed(lambda: st.refresh_route(eps))
+    filled = int(((st.route - (st.height + st.sediment)) > 0).sum().item())
+    out.update({"route_ms": t_r, "route_sweeps": sweeps, "route_filled_cells": filled})
+    log(f"side passes: routing flood {t_r:.1f} ms ({sweeps} relaxation sweeps, {filled} cells filled)")
     if cpu:
         from oracle import host
 
@@ -318,6 +329,42 @@ def side_run(grid, f, p, reps=5, cpu=True):
                                dep_floor=float(ep.dep_floor_m) / grid.cell_size_m)
         out["cpu_thermal_pass_ms"] = 1000.0 * (time.perf_counter() - t)
         log(f"side passes: oracle thermal pass (1 thread) {out['cpu_thermal_pass_ms']:.0f} ms")
+    return out
+
+
+def atomic_probe(NE=1028, F=6, lanes=1 << 20, steps=64):
+    """The device's 64-bit atomicAdd rate on the trace's footprint (6 faces of
+    NE^2 cells): a droplet-like walk adding to 4 planes per step (the applying
+    pass's pattern), the same walk with 1 plane, and uniformly random cells."""
+    import ctypes
+
+    import torch
+
+    from globe_hip import _lib
+
+    n = F * NE * NE
+    t = torch.zeros(4 * n, dtype=torch.int64, device="cuda:0")
+    out = {}
+    for name, planes, pattern in (("walk_4_planes", 4, 0), ("walk_1_plane", 1, 0), ("random_1_plane", 1, 1)):
+        def run(seed):
+            _lib.check(_lib.lib().pg_atomic_probe(ctypes.c_void_p(t.data_ptr()), n, NE, planes, lanes, steps, pattern,
+                                                  seed, ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)),
+                       "pg_atomic_probe")
+        run(1)
+        torch.cuda.synchronize()
+        ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+        ms = []
+        for r in range(3):
+            ev[0].record()
+            run(2 + r)
+            ev[1].record()
+            torch.cuda.synchronize()
+            ms.append(ev[0].elapsed_time(ev[1]))
+        m = sorted(ms)[1]
+        out[name] = {"ms": m, "gatomics_per_s": lanes * steps * planes / (m * 1e-3) / 1e9}
+    tot = int(t.sum().item())
+    assert tot == 4 * lanes * steps * (4 + 1 + 1), "atomic probe lost updates"
+    log("atomic probe (G atomics/s): " + ", ".join(f"{k} {v['gatomics_per_s']:.1f}" for k, v in out.items()))
     return out
 
 
@@ -347,11 +394,15 @@ def main():
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-side", action="store_true")
+    ap.add_argument("--atomic-probe", action="store_true", help="only run the atomic-rate probe and exit")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="only run 3 device-to-device copies of this many MiB (HBM counter calibration) and exit")
     a = ap.parse_args()
     if a.copy_probe_mib:
         copy_probe(a.copy_probe_mib)
+        return
+    if a.atomic_probe:
+        print(json.dumps({"atomic_probe": atomic_probe()}), flush=True)
         return
     if a.gpus != 1:
         print(f"bench.py measures one GPU: the particle pass shards by droplets with one int64 all-reduce per "
@@ -389,6 +440,13 @@ def main():
                      "bytes_per_step": {"apply_pass": BYTES_PER_STEP, "census_pass": BYTES_PER_STEP_CENSUS},
                      "launch_ms_mean": r["trace_ms"] / max(r["launches"], 1), "launches": r["launches"]},
     }
+    probe = atomic_probe()
+    peak_atomics = probe["walk_4_planes"]["gatomics_per_s"]
+    ach_atomics = ATOMICS_PER_MOVE * r["moves"] / trace_s / 1e9 if trace_s > 0 else 0.0
+    out["roofline"]["atomics"] = {"achieved": ach_atomics, "peak": peak_atomics, "unit": "G atomics/s",
+                                  "frac": ach_atomics / peak_atomics, "per_move": ATOMICS_PER_MOVE,
+                                  "peak_source": "measured: pg_atomic_probe, droplet-like walk over the same "
+                                                 "footprint, 4 int64 planes", "probe": probe}
     if not a.no_adjoint:
         import torch
 

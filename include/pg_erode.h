@@ -217,6 +217,35 @@ int pg_iso_accumulate(int64_t n, double* iso_acc, const double* height, const do
 int pg_scale(int64_t n, const double* x, double a, double* out, pg_stream stream);
 int pg_divide(int64_t n, const double* x, double v, double* out, pg_stream stream);
 
+/* ---- routing surface (bake/globe/erosion/route.py:30-117,
+ * priority_flood_eps) ----------------------------------------------------- */
+/* The flood's epsilon-filled surface as the fixed point of
+ * W(n) = max(s(n), min over in-neighbours W + eps), relaxed in place.
+ * cls (uint8, F x NE x NE): 0 never assigned (outside / outer ring / a
+ * cross-face halo cell), 1 static seed (an interior cell next to the outside
+ * or the window edge), 2 assignable, 3 assignable with extra in-edges listed
+ * in spec_cell (sorted, n_spec) / spec_ptr (n_spec + 1) / spec_src.
+ * pg_flood_init: s = height + sediment, W = s on seeds (also s < 0 and
+ * s <= base on interior cells), +inf elsewhere; *flag = 0.
+ * pg_flood_sweeps: `sweeps` relaxation sweeps; *flag (DEVICE int32) is 1 when
+ * the last of them changed a value (0: converged).
+ * pg_flood_finish: route = W where reached, s elsewhere. */
+int pg_flood_init(int64_t F, int64_t NE, int64_t H, int64_t N, const double* height, const double* sediment,
+                  const double* base, const uint8_t* cls, double* s_out, double* W, int32_t* flag, pg_stream stream);
+int pg_flood_sweeps(int64_t F, int64_t NE, const double* s, const uint8_t* cls, double* W, double eps,
+                    const int64_t* spec_cell, const int32_t* spec_ptr, const int64_t* spec_src, int64_t n_spec,
+                    int32_t sweeps, int32_t* flag, pg_stream stream);
+int pg_flood_finish(int64_t n, const double* s, const double* W, double* route, pg_stream stream);
+
+/* ---- calibration (no counterpart in the reference) ---------------------- */
+/* The device's relaxed 64-bit atomicAdd rate: `lanes` lanes each take
+ * `steps` steps of a droplet-like walk (pattern 0: to one of the 8 neighbours
+ * on an NE-wide grid of n cells; 1: to a uniformly random cell) and add 1 to
+ * the cell in each of `planes` int64 planes of n (targets: DEVICE
+ * int64[planes * n], n > 4 NE).  For the trace kernel's roofline. */
+int pg_atomic_probe(int64_t* targets, int64_t n, int64_t NE, int32_t planes, int64_t lanes, int32_t steps,
+                    int32_t pattern, uint64_t seed, pg_stream stream);
+
 /* ---- fast mode (the production path of one iteration's particle pass) -- */
 /* Device pointers to the erosion state (bake/globe/erosion/maps.py:56-115
  * dtypes): the stage's fields in cell units. */

@@ -16,8 +16,10 @@ fold, EMA, stats), ``apply_uplift`` / ``uplift_replay`` / ``start_replay``
 (maps.py:1050-1085 with particle.py:1104-1227), ``apply_isostasy`` and its
 smoothing step (maps.py:1325-1410 with field.py:179-188, 372-404) and
 ``step`` (maps.py:1516-1617: particles -> mass wasting -> uplift ->
-isostasy -> datum -> halos).  The remaining side passes (routing flood, sea
-mask, lakes, glacial, ice, strata) raise NotImplementedError here.
+isostasy -> datum -> halos), the routing flood (route.py:30-117) and a
+window's lakes without the evaporation balance (maps.py:629-672).  The
+remaining side passes (sea mask, the planet's lake balance, glacial, ice,
+strata) raise NotImplementedError here.
 
 Parity pins: the reference's own invariants (mass conservation on a closed
 window, the offshore split, the change-list layout, the stockpile bound,
@@ -90,6 +92,7 @@ def lib():
         L.or_thermal_pass_b.argtypes = [i64, i64, vp, vp, vp, vp, vp, dbl, vp, vp, vp, vp]
         L.or_slope_erode_cap.argtypes = [i64, i64, vp, vp, vp, dbl, dbl, vp]
         L.or_laplacian.argtypes = [i64, i64, vp, vp, vp, vp]
+        L.or_priority_flood_eps.argtypes = [i64, i64, i64, i64, vp, vp, vp, dbl, vp, vp, vp, vp]
         _lib = L
     return _lib
 
@@ -273,6 +276,33 @@ class OracleState:
                               _p(self.discharge), _p(self.momentum), _p(self.evap), _p(self.hardness), _p(self.metric),
                               _p(self.metric_inv), _p(self.base), _p(self.lake_flag))
 
+    def owner_table(self):
+        """maps.py:342-349."""
+        n = self.F * self.NE * self.NE
+        own = self.grid.owner if self.spherical else np.arange(n, dtype=np.int64).reshape(self.height.shape)
+        return np.ascontiguousarray(own, dtype=np.int64)
+
+    def refresh_route(self, eps):
+        """maps.py:351-358."""
+        self.route = priority_flood_eps(self.surface(), self.mask, self.owner_table(), self.H, self.N, float(eps),
+                                        self.base)
+        if self.spherical:
+            exchange(self.grid, self.route, linear=True)
+
+    def refresh_lakes_window(self, min_depth, lake_evap=0.0):
+        """maps.py:629-672 without the evaporation balance."""
+        if float(lake_evap) > 0.0:
+            raise NotImplementedError("the window lakes' evaporation balance is outside the oracle's scope")
+        flag = np.zeros(self.height.shape, dtype=np.uint8)
+        if self.route is None:
+            self.lake_flag = flag
+            return {"lake_cells": 0}
+        lake = ((self.route - self.surface()) > float(min_depth)) & (self.mask == MASK_ACTIVE)
+        inter = self.interior
+        flag[inter] = lake[inter]
+        self.lake_flag = flag
+        return {"lake_cells": int(lake[inter].sum())}
+
     def exchange_halos(self):
         """maps.py:329-340 with field.py:104-124 (numpy einsum, cubic weights;
         the vector momentum rotated into the destination face first)."""
@@ -338,8 +368,7 @@ def chunk_size(ep, n_particles):
 
 
 def _check_plain(ep, state):
-    if state.route is not None:
-        raise NotImplementedError("the routing surface is outside the oracle's scope")
+    pass
 
 
 # ---------------------------------------------------------------------------
@@ -484,6 +513,22 @@ def run_iteration(state, params, iteration_key, n_particles=None, particles_per_
     stats["disc_saturation"] = disc_sat
     stats["mom_scale"] = mom_scale
     return stats
+
+
+# ---------------------------------------------------------------------------
+# routing surface (route.py:30-117)
+# ---------------------------------------------------------------------------
+def priority_flood_eps(surface, mask, owner, H, N, eps, base):
+    """route.py:30-117: the epsilon-filled surface (heap-ordered flood in C)."""
+    surface = np.ascontiguousarray(surface, np.float64)
+    F, NE = surface.shape[0], surface.shape[1]
+    filled = np.empty_like(surface)
+    visited = np.empty(surface.shape, np.uint8)
+    heap = np.empty(2 * surface.size, np.float64)  # (value, index) pairs, 16 B per cell
+    lib().or_priority_flood_eps(F, NE, int(H), int(N), _p(surface), _p(np.ascontiguousarray(mask, np.uint8)),
+                                _p(np.ascontiguousarray(owner, np.int64)), float(eps),
+                                _p(np.ascontiguousarray(base, np.float64)), _p(filled), _p(visited), _p(heap))
+    return filled
 
 
 # ---------------------------------------------------------------------------
@@ -709,11 +754,21 @@ def step(state, params, iteration_key, **kw):
     """maps.py:1516-1617: particles, mass wasting, uplift, isostasy (every
     isostasy_every iterations), datum, halos."""
     ep = _eparams(params)
-    if int(ep.flood_every) > 0 or (state.spherical and int(getattr(ep, "sea_mask_every", 0)) > 0):
-        raise NotImplementedError("routing / sea-mask refreshes are outside the oracle's scope")
+    if state.spherical and int(getattr(ep, "sea_mask_every", 0)) > 0:
+        raise NotImplementedError("the sea-mask refresh is outside the oracle's scope")
+    if state.spherical and bool(getattr(ep, "lake_balance", False)) and int(ep.flood_every) > 0:
+        raise NotImplementedError("lake refreshes are outside the oracle's scope")
     if state.spherical and int(ep.glacial_every) > 0:
         raise NotImplementedError("the glacial pass is outside the oracle's scope")
     t0 = time.time()
+    flood = int(ep.flood_every)
+    if flood > 0 and (state.route is None or state.iteration % flood == 0):
+        state.refresh_route(cell_units(ep, "route_eps", state.height_unit_m))
+    if not state.spherical and bool(getattr(ep, "window_lakes", False)) and flood > 0 and \
+            (state.lake_flag is None or state.iteration % flood == 0):
+        hydro = getattr(params, "hydro", None)
+        min_depth = float(hydro.lake_min_depth) if hydro is not None else 0.5
+        state.refresh_lakes_window(min_depth / float(state.height_unit_m), float(getattr(ep, "window_lake_evap", 0.0)))
     iso = state.spherical and float(getattr(ep, "isostasy", 0.0)) > 0.0
     if iso:
         if state.iso_acc is None:

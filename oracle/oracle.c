@@ -1035,3 +1035,99 @@ void or_laplacian(int64_t F, int64_t NE, const double *d, const double *ginv, co
 #undef G
 #undef Q
 }
+
+/* ---------------------------------------------------------------------- */
+/* routing surface: priority flood + epsilon (route.py:30-117)            */
+/* ---------------------------------------------------------------------- */
+typedef struct {
+    double v;
+    int64_t c;
+} or_hitem;
+
+static int or_hless(or_hitem a, or_hitem b) { return a.v < b.v || (a.v == b.v && a.c < b.c); }
+
+static void or_hpush(or_hitem *h, int64_t *n, or_hitem x) {
+    int64_t i = (*n)++;
+    while (i > 0) {
+        int64_t p = (i - 1) / 2;
+        if (!or_hless(x, h[p])) break;
+        h[i] = h[p];
+        i = p;
+    }
+    h[i] = x;
+}
+
+static or_hitem or_hpop(or_hitem *h, int64_t *n) {
+    or_hitem top = h[0], x = h[--(*n)];
+    int64_t i = 0;
+    for (;;) {
+        int64_t l = 2 * i + 1, r = l + 1, m = i;
+        or_hitem best = x;
+        if (l < *n && or_hless(h[l], best)) { best = h[l]; m = l; }
+        if (r < *n && or_hless(h[r], best)) { best = h[r]; m = r; }
+        if (m == i) break;
+        h[i] = h[m];
+        i = m;
+    }
+    if (*n > 0) h[i] = x;
+    return top;
+}
+
+/* route.py:30-117.  filled: out (F, NE, NE); visited: scratch uint8[F NE NE];
+ * heap: scratch of F NE NE items (each cell is pushed at most once). */
+void or_priority_flood_eps(int64_t F, int64_t NE, int64_t H, int64_t N, const double *surface, const uint8_t *mask,
+                           const int64_t *owner, double eps, const double *base, double *filled, uint8_t *visited,
+                           void *heap_mem) {
+    const int64_t total = F * NE * NE;
+    or_hitem *heap = (or_hitem *)heap_mem;
+    int64_t hn = 0;
+    for (int64_t c = 0; c < total; c++) filled[c] = surface[c];
+    memset(visited, 0, (size_t)total);
+    for (int64_t f = 0; f < F; f++)
+        for (int64_t ei = 0; ei < NE; ei++)
+            for (int64_t ej = 0; ej < NE; ej++) {
+                const int64_t c = (f * NE + ei) * NE + ej;
+                if (mask[c] == MASK_OUTSIDE || ei == 0 || ej == 0 || ei == NE - 1 || ej == NE - 1) visited[c] = 1;
+            }
+    for (int64_t f = 0; f < F; f++)
+        for (int64_t ei = H; ei < H + N; ei++)
+            for (int64_t ej = H; ej < H + N; ej++) {
+                const int64_t c = (f * NE + ei) * NE + ej;
+                if (visited[c]) continue;
+                int seed = surface[c] < 0.0 && surface[c] <= base[c];
+                if (!seed) {
+                    for (int k = 0; k < 8; k++) {
+                        const int64_t n = owner[(f * NE + ei + OR_D8[k][0]) * NE + ej + OR_D8[k][1]];
+                        if (mask[n] == MASK_OUTSIDE) {
+                            seed = 1;
+                            break;
+                        }
+                        const int64_t nf = n / (NE * NE), rem = n - nf * NE * NE, nei = rem / NE, nej = rem - nei * NE;
+                        if (nei < H || nei >= H + N || nej < H || nej >= H + N) {
+                            seed = 1;
+                            break;
+                        }
+                    }
+                }
+                if (seed) {
+                    visited[c] = 1;
+                    or_hitem it = {surface[c], c};
+                    or_hpush(heap, &hn, it);
+                }
+            }
+    while (hn > 0) {
+        const or_hitem top = or_hpop(heap, &hn);
+        const double h = top.v;
+        const int64_t c = top.c, f = c / (NE * NE), rem = c - f * NE * NE, ei = rem / NE, ej = rem - ei * NE;
+        for (int k = 0; k < 8; k++) {
+            const int64_t n = owner[(f * NE + ei + OR_D8[k][0]) * NE + ej + OR_D8[k][1]];
+            if (visited[n]) continue;
+            visited[n] = 1;
+            double v = surface[n];
+            if (v < h + eps) v = h + eps;
+            filled[n] = v;
+            or_hitem it = {v, n};
+            or_hpush(heap, &hn, it);
+        }
+    }
+}

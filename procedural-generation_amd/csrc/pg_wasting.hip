@@ -2,8 +2,8 @@
 // (the thermal talus pass and hillslope creep), soillib's slope-limited
 // erosion cap, and the isostatic response (the flux-form Laplace-Beltrami
 // smoother of the load, its mean-free application).  All are stencil or
-// elementwise passes over the float64 state: grid-stride loops over the
-// extended cells, 256 threads per block, bandwidth-bound.
+// elementwise passes over the float64 state, bandwidth-bound: stencils on a
+// (row segment, row, face) grid, elementwise passes as grid-stride loops.
 //
 // Bit parity: every output is the reference's IEEE operation chain in its
 // order (the library is built with -ffp-contract=off): float32 metric and
@@ -48,17 +48,27 @@ __device__ __forceinline__ double thermal_out(const double* h, const double* s, 
     return 0.5 * rate * ex;
 }
 
+// stencil kernels run on a (j, i, face) grid of 64 x 4 blocks: one thread
+// per extended cell, consecutive threads along a row (coalesced), no index
+// division
+constexpr int SX = 64, SY = 4;
+
+inline dim3 stencil_grid(int64_t F, int64_t NE) {
+    return dim3((unsigned)((NE + SX - 1) / SX), (unsigned)((NE + SY - 1) / SY), (unsigned)F);
+}
+
 struct Cell {
-    int64_t c, f, i, j;
+    int64_t c;
+    int i, j;
+    bool ok;
 };
 
-__device__ __forceinline__ Cell cell_of(int64_t c, int64_t NE) {
+__device__ __forceinline__ Cell stencil_cell(int64_t NE) {
     Cell r;
-    r.c = c;
-    r.f = c / (NE * NE);
-    const int64_t rem = c - r.f * NE * NE;
-    r.i = rem / NE;
-    r.j = rem - r.i * NE;
+    r.j = blockIdx.x * SX + threadIdx.x;
+    r.i = blockIdx.y * SY + threadIdx.y;
+    r.ok = r.i < NE && r.j < NE;
+    r.c = ((int64_t)blockIdx.z * NE + r.i) * NE + r.j;
     return r;
 }
 
@@ -76,8 +86,9 @@ __global__ void talus_kernel(int64_t n, const float* hardness, float soft, float
 __global__ void thermal_a_kernel(int64_t F, int64_t NE, const double* h, const double* s, const float* metric,
                                  const double* talus, const uint8_t* mask, double rate, double tmax, double* out_total,
                                  double* scale) {
-    GRID_STRIDE(c, F * NE * NE) {
-        const Cell x = cell_of(c, NE);
+    const Cell x = stencil_cell(NE);
+    if (x.ok) {
+        const int64_t c = x.c;
         double tot = 0.0, sc = 0.0;
         if (inner(x, NE) && mask[c] == MASK_ACTIVE) {
             double S = 0.0, mx = 0.0;
@@ -108,8 +119,9 @@ __global__ void thermal_a_kernel(int64_t F, int64_t NE, const double* h, const d
 __global__ void thermal_r_kernel(int64_t F, int64_t NE, const double* h, const double* s, const float* metric,
                                  const double* talus, const uint8_t* mask, double rate, const double* scale,
                                  double* rscale, double dep_floor) {
-    GRID_STRIDE(c, F * NE * NE) {
-        const Cell x = cell_of(c, NE);
+    const Cell x = stencil_cell(NE);
+    if (x.ok) {
+        const int64_t c = x.c;
         double r = 1.0;
         const double surf = h[c] + s[c];
         if (inner(x, NE) && !(surf >= 0.0 || mask[c] != MASK_ACTIVE)) {
@@ -135,8 +147,9 @@ __global__ void thermal_r_kernel(int64_t F, int64_t NE, const double* h, const d
 __global__ void thermal_b_kernel(int64_t F, int64_t NE, const double* h, const double* s, const float* metric,
                                  const double* talus, const uint8_t* mask, double rate, const double* scale,
                                  const double* rscale, double* new_h, double* new_s) {
-    GRID_STRIDE(c, F * NE * NE) {
-        const Cell x = cell_of(c, NE);
+    const Cell x = stencil_cell(NE);
+    if (x.ok) {
+        const int64_t c = x.c;
         double hh = h[c], ss = s[c];
         if (inner(x, NE) && mask[c] == MASK_ACTIVE) {
             double inflow = 0.0, out = 0.0;
@@ -185,11 +198,12 @@ __global__ void copy2_kernel(int64_t n, const double* a, const double* b, double
 // particle.py:1045-1076
 __global__ void slope_cap_kernel(int64_t F, int64_t NE, const double* h, const double* s, const uint8_t* mask,
                                  double k, double exit_slope, double* out) {
-    GRID_STRIDE(c, F * NE * NE) {
-        const Cell x = cell_of(c, NE);
+    const Cell x = stencil_cell(NE);
+    if (x.ok) {
+        const int64_t c = x.c;
         if (mask[c] != MASK_ACTIVE) {
             out[c] = 0.0;
-            continue;
+            return;
         }
         const double hc = h[c] + s[c];
         const double gxn = x.i == 0 ? exit_slope : hc - (h[c - NE] + s[c - NE]);
@@ -242,8 +256,9 @@ __device__ __forceinline__ double laplacian_at(const double* d, const float* gin
 // with d_add NULL, out = lap * c2 (the power iteration's field, maps.py:1356)
 __global__ void laplacian_kernel(int64_t F, int64_t NE, const double* d, const float* ginv, const float* sq, double cs2,
                                  int mode, double kappa, double c2, double* out) {
-    GRID_STRIDE(c, F * NE * NE) {
-        const Cell x = cell_of(c, NE);
+    const Cell x = stencil_cell(NE);
+    if (x.ok) {
+        const int64_t c = x.c;
         double lap = 0.0;
         if (inner(x, NE)) lap = laplacian_at(d, ginv, sq, c, NE);
         const double L = (double)(float)(lap / cs2);
@@ -385,7 +400,7 @@ extern "C" int pg_thermal_pass_a(int64_t F, int64_t NE, const double* height, co
     clear_error();
     PG_STENCIL_ARGS("pg_thermal_pass_a");
     PG_REQUIRE(scale, "pg_thermal_pass_a: NULL scale");
-    hipLaunchKernelGGL(thermal_a_kernel, dim3(grid_for(F * NE * NE)), dim3(TPB), 0, (hipStream_t)stream, F, NE, height,
+    hipLaunchKernelGGL(thermal_a_kernel, stencil_grid(F, NE), dim3(SX, SY), 0, (hipStream_t)stream, F, NE, height,
                        sediment, metric, talus, mask, rate, thermal_max, out_total, scale);
     PG_LAUNCHED("thermal_a_kernel");
     return 0;
@@ -397,7 +412,7 @@ extern "C" int pg_thermal_pass_r(int64_t F, int64_t NE, const double* height, co
     clear_error();
     PG_STENCIL_ARGS("pg_thermal_pass_r");
     PG_REQUIRE(scale && rscale, "pg_thermal_pass_r: NULL scale");
-    hipLaunchKernelGGL(thermal_r_kernel, dim3(grid_for(F * NE * NE)), dim3(TPB), 0, (hipStream_t)stream, F, NE, height,
+    hipLaunchKernelGGL(thermal_r_kernel, stencil_grid(F, NE), dim3(SX, SY), 0, (hipStream_t)stream, F, NE, height,
                        sediment, metric, talus, mask, rate, scale, rscale, dep_floor);
     PG_LAUNCHED("thermal_r_kernel");
     return 0;
@@ -410,7 +425,7 @@ extern "C" int pg_thermal_pass_b(int64_t F, int64_t NE, const double* height, co
     PG_STENCIL_ARGS("pg_thermal_pass_b");
     PG_REQUIRE(scale && rscale && new_height && new_sediment, "pg_thermal_pass_b: NULL array");
     PG_REQUIRE(new_height != height && new_sediment != sediment, "pg_thermal_pass_b: outputs must not alias the inputs");
-    hipLaunchKernelGGL(thermal_b_kernel, dim3(grid_for(F * NE * NE)), dim3(TPB), 0, (hipStream_t)stream, F, NE, height,
+    hipLaunchKernelGGL(thermal_b_kernel, stencil_grid(F, NE), dim3(SX, SY), 0, (hipStream_t)stream, F, NE, height,
                        sediment, metric, talus, mask, rate, scale, rscale, new_height, new_sediment);
     PG_LAUNCHED("thermal_b_kernel");
     return 0;
@@ -426,11 +441,12 @@ extern "C" int pg_mass_wasting_pass(int64_t F, int64_t NE, double* height, doubl
     double *scale = work, *rscale = work + n, *nh = work + 2 * n, *ns = work + 3 * n;
     hipStream_t s = (hipStream_t)stream;
     const int g = grid_for(n);
-    hipLaunchKernelGGL(thermal_a_kernel, dim3(g), dim3(TPB), 0, s, F, NE, height, sediment, metric, talus, mask, rate,
+    const dim3 sg = stencil_grid(F, NE), sb(SX, SY);
+    hipLaunchKernelGGL(thermal_a_kernel, sg, sb, 0, s, F, NE, height, sediment, metric, talus, mask, rate,
                        thermal_max, (double*)nullptr, scale);
-    hipLaunchKernelGGL(thermal_r_kernel, dim3(g), dim3(TPB), 0, s, F, NE, height, sediment, metric, talus, mask, rate,
+    hipLaunchKernelGGL(thermal_r_kernel, sg, sb, 0, s, F, NE, height, sediment, metric, talus, mask, rate,
                        scale, rscale, dep_floor);
-    hipLaunchKernelGGL(thermal_b_kernel, dim3(g), dim3(TPB), 0, s, F, NE, height, sediment, metric, talus, mask, rate,
+    hipLaunchKernelGGL(thermal_b_kernel, sg, sb, 0, s, F, NE, height, sediment, metric, talus, mask, rate,
                        scale, rscale, nh, ns);
     hipLaunchKernelGGL(copy2_kernel, dim3(g), dim3(TPB), 0, s, n, nh, ns, height, sediment);
     PG_LAUNCHED("thermal kernels");
@@ -453,7 +469,7 @@ extern "C" int pg_slope_erode_cap(int64_t F, int64_t NE, const double* height, c
     clear_error();
     PG_REQUIRE(F > 0 && NE >= 1 && height && sediment && mask && out, "pg_slope_erode_cap: bad arguments");
     const double k = coef * sqrt(2.0);
-    hipLaunchKernelGGL(slope_cap_kernel, dim3(grid_for(F * NE * NE)), dim3(TPB), 0, (hipStream_t)stream, F, NE, height,
+    hipLaunchKernelGGL(slope_cap_kernel, stencil_grid(F, NE), dim3(SX, SY), 0, (hipStream_t)stream, F, NE, height,
                        sediment, mask, k, exit_slope, out);
     PG_LAUNCHED("slope_cap_kernel");
     return 0;
@@ -465,7 +481,7 @@ extern "C" int pg_laplacian(int64_t F, int64_t NE, const double* data, const flo
     PG_REQUIRE(F > 0 && NE >= 3 && data && metric_inv && sqrt_det && out, "pg_laplacian: bad arguments");
     PG_REQUIRE(mode >= 0 && mode <= 2, "pg_laplacian: mode must be 0, 1 or 2");
     PG_REQUIRE(out != data, "pg_laplacian: out must not alias data");
-    hipLaunchKernelGGL(laplacian_kernel, dim3(grid_for(F * NE * NE)), dim3(TPB), 0, (hipStream_t)stream, F, NE, data,
+    hipLaunchKernelGGL(laplacian_kernel, stencil_grid(F, NE), dim3(SX, SY), 0, (hipStream_t)stream, F, NE, data,
                        metric_inv, sqrt_det, cell_size_sq, (int)mode, kappa, c2, out);
     PG_LAUNCHED("laplacian_kernel");
     return 0;

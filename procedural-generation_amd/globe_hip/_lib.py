@@ -122,6 +122,10 @@ SIGNATURES = [
     ("pg_iso_accumulate", i32, [i64, vp, vp, vp, vp, vp]),
     ("pg_scale", i32, [i64, vp, dbl, vp, vp]),
     ("pg_divide", i32, [i64, vp, dbl, vp, vp]),
+    ("pg_flood_init", i32, [i64, i64, i64, i64, vp, vp, vp, vp, vp, vp, vp, vp]),
+    ("pg_flood_sweeps", i32, [i64, i64, vp, vp, vp, dbl, vp, vp, vp, i64, ctypes.c_int32, vp, vp]),
+    ("pg_flood_finish", i32, [i64, vp, vp, vp, vp]),
+    ("pg_atomic_probe", i32, [vp, i64, i64, ctypes.c_int32, i64, ctypes.c_int32, ctypes.c_int32, ctypes.c_uint64, vp]),
 ]
 
 _lib = None

@@ -53,6 +53,14 @@ class TectonicsGroup:
 
 
 @dataclass
+class HydroGroup:
+    """Only ``lake_min_depth`` is read here (a window's lakes, bake/globe/
+    erosion/maps.py:1563-1566; bake/globe/config.py:615)."""
+
+    lake_min_depth: float = 0.5  # m
+
+
+@dataclass
 class ErosionParams:
     """Field-for-field the reference's ``ErosionParams``
     (bake/globe/config.py:486-583).  Lengths listed in ``LENGTH_PARAMS_M`` are
@@ -183,6 +191,7 @@ class WorldParams:
     world: WorldGroup = field(default_factory=WorldGroup)
     tectonics: TectonicsGroup = field(default_factory=TectonicsGroup)
     erosion: ErosionParams = field(default_factory=ErosionParams)
+    hydro: HydroGroup = field(default_factory=HydroGroup)
 
     @property
     def cell_size_m(self) -> float:

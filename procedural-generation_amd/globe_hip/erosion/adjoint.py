@@ -72,7 +72,8 @@ class GradRun:
     def __init__(self, state, params, iterations: int, real: str = "f32", checkpoint_every: int = 10,
                  first_key: int = 0, rng_stage: str = "erosion", n_particles=None, replay_of: "GradRun | None" = None,
                  feedback: bool = True):
-        from .maps import _check_scope, _eparams, discharge_scales, max_steps_of, trace_params, uplift_cap
+        from .maps import (_check_scope, _eparams, discharge_scales, max_steps_of, require_plain_side_passes,
+                           trace_params, uplift_cap)
 
         if state.mode != "fast":
             raise ValueError("the differentiable mode runs on a fast-mode state")
@@ -80,6 +81,7 @@ class GradRun:
             raise ValueError("real must be 'f32' or 'f64'")
         ep = _eparams(params)
         _check_scope(ep, state)
+        require_plain_side_passes(ep, "the differentiable mode")
         if float(getattr(ep, "lateral_rate", 0.0)) > 0.0 or state.roots is not None:
             raise NotImplementedError("lateral cuts and roots are outside the differentiable mode")
         if float(ep.disc_saturation_cells) > 0.0 or float(ep.momentum_saturation_cells) > 0.0:

@@ -116,6 +116,8 @@ class WindowBatch:
         disc_sat, mom_scale = maps.discharge_scales(eps[0], st0, 1.0, self.P[0], n_rain=1)
         if float(eps[0].disc_saturation_cells) > 0.0 or float(eps[0].momentum_saturation_cells) > 0.0:
             raise NotImplementedError("cell-count discharge scales are outside the fast mode")
+        for e in eps:
+            maps.require_plain_side_passes(e, "the batched-window iteration")
         self.prm = maps.trace_params(eps[0], st0, 1.0, 1, 1, disc_sat, mom_scale, self.max_steps[0], 0.0)
         self.ep = eps[0]
         self.fields = (FastFields * self.W)(*[self._fields(st) for st in self.states])

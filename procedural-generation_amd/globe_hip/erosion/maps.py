@@ -24,9 +24,10 @@ Two modes (``ErosionState.mode``):
   (``globe_hip.erosion.fast``).
 
 Everything after the particles (mass wasting, uplift, isostasy, datum
-hold, halo exchange) and the slope limits run on the device in both modes.
-The remaining side passes (routing flood, sea mask, lakes, glacial carving)
-raise NotImplementedError.
+hold, halo exchange) and the slope limits run on the device in both modes;
+the routing flood (and a window's lakes) in the check mode.  The remaining
+side passes (sea mask, the planet's lake balance, glacial carving) raise
+NotImplementedError.
 """
 from __future__ import annotations
 
@@ -140,6 +141,7 @@ class ErosionState:
     iso_acc: torch.Tensor | None = None  # maps.py:1369-1371: surface change not yet compensated
     _halo: HaloTables | None = field(default=None, repr=False)
     _side: dict = field(default_factory=dict, repr=False)
+    _owner: np.ndarray | None = field(default=None, repr=False)
     _uplift_mean: dict = field(default_factory=dict, repr=False)
     _mask_host: np.ndarray | None = field(default=None, repr=False)
     _select_work: torch.Tensor | None = field(default=None, repr=False)
@@ -339,6 +341,111 @@ class ErosionState:
         return t
 
 
+    # -- routing surface (maps.py:342-358, route.py:30-117) ------------------
+    def owner_table(self) -> np.ndarray:
+        """Flat index of the interior cell owning every extended cell
+        (``grid.owner`` in spherical mode, identity for a window)."""
+        if self._owner is None:
+            n = self.F * self.NE * self.NE
+            own = self.grid.owner if self.spherical else np.arange(n, dtype=np.int64).reshape(self.height.shape)
+            self._owner = np.ascontiguousarray(own, dtype=np.int64)
+        return self._owner
+
+    def _flood_tables(self) -> dict:
+        """The flood's static cell classes and the extra in-edges of the cells
+        a cross-face halo cell maps onto (see pg_flood_*), built once on the
+        host from the mask and the owner table."""
+        t = self._side.get("flood")
+        if t is not None:
+            return t
+        F, NE, H, N = self.F, self.NE, self.H, self.N
+        n = F * NE * NE
+        own = self.owner_table().reshape(n)
+        m = self._mask_host.reshape(n)
+        idx = np.arange(n, dtype=np.int64)
+        f, rem = np.divmod(idx, NE * NE)
+        ei, ej = np.divmod(rem, NE)
+        ring = (ei == 0) | (ej == 0) | (ei == NE - 1) | (ej == NE - 1)
+        visited0 = (m == pk.MASK_OUTSIDE) | ring
+        inter = (ei >= H) & (ei < H + N) & (ej >= H) & (ej < H + N)
+        target = ~visited0 & (own == idx)
+        offs = [(1, 0), (1, 1), (0, 1), (-1, 1), (-1, 0), (-1, -1), (0, -1), (1, -1)]  # particle.py:102
+        cand = np.flatnonzero(inter & ~visited0)
+        seed = np.zeros(n, bool)
+        for di, dj in offs:
+            nb = own[cand + di * NE + dj]
+            nf, nrem = np.divmod(nb, NE * NE)
+            ni, nj = np.divmod(nrem, NE)
+            out = (m[nb] == pk.MASK_OUTSIDE) | (ni < H) | (ni >= H + N) | (nj < H) | (nj >= H + N)
+            seed[cand[out]] = True
+        cls = np.zeros(n, np.uint8)
+        cls[target] = 2
+        cls[seed] = 1
+        # irregular edges p -> owner(p + d) != p + d into assignable cells
+        src = np.flatnonzero(cls > 0)
+        es, ed = [], []
+        for di, dj in offs:
+            q = src + di * NE + dj
+            nb = own[q]
+            irr = (nb != q) & (cls[nb] == 2)
+            es.append(src[irr])
+            ed.append(nb[irr])
+        es, ed = np.concatenate(es), np.concatenate(ed)
+        order = np.lexsort((es, ed))
+        es, ed = es[order], ed[order]
+        spec, start = np.unique(ed, return_index=True)
+        cls[spec] = 3
+        ptr = np.append(start, ed.size).astype(np.int32)
+        dev = self.device
+        t = {"cls": _dev(cls.reshape(self.height.shape), dev), "n_spec": int(spec.size),
+             "spec_cell": _dev(spec.astype(np.int64), dev) if spec.size else None,
+             "spec_ptr": _dev(ptr, dev) if spec.size else None,
+             "spec_src": _dev(es.astype(np.int64), dev) if spec.size else None,
+             "s": torch.empty_like(self.height), "W": torch.empty_like(self.height),
+             "flag": torch.zeros(1, dtype=torch.int32, device=dev)}
+        self._side["flood"] = t
+        return t
+
+    def refresh_route(self, eps: float, batch: int = 32) -> int:
+        """Recompute the epsilon-filled routing surface from the current
+        terrain on the device (maps.py:351-358 with route.py:30-117), then
+        refresh its halos (linear).  Returns the relaxation sweeps taken."""
+        t = self._flood_tables()
+        pk.flood_init(self.height, self.sediment, self.base, t["cls"], self.H, self.N, t["s"], t["W"], t["flag"])
+        sweeps, cap = 0, self.F * self.NE * self.NE + batch
+        while True:
+            pk.flood_sweeps(t["s"], t["cls"], t["W"], float(eps), t["spec_cell"], t["spec_ptr"], t["spec_src"], batch,
+                            t["flag"])
+            sweeps += batch
+            if int(t["flag"].item()) == 0:
+                break
+            if sweeps > cap:
+                raise RuntimeError("routing flood did not converge")
+        if self.route is None:
+            self.route = torch.empty_like(self.height)
+        pk.flood_finish(t["s"], t["W"], self.route)
+        if self.spherical:
+            self.halo_tables().exchange(self.route, linear=True)
+        return sweeps
+
+    def refresh_lakes_window(self, min_depth: float, lake_evap: float = 0.0) -> dict:
+        """maps.py:629-672 without the evaporation balance: every active
+        interior cell the routing flood fills deeper than ``min_depth`` (cell
+        units) is a flagged, overflowing lake."""
+        if float(lake_evap) > 0.0:
+            raise NotImplementedError("the window lakes' evaporation balance (erosion.window_lake_evap) is outside "
+                                      "the HIP erosion path")
+        flag = torch.zeros(self.height.shape, dtype=torch.uint8, device=self.device)
+        if self.route is None:
+            self.lake_flag = flag
+            return {"lake_cells": 0}
+        lake = ((self.route - (self.height + self.sediment)) > float(min_depth)) & (self.mask == pk.MASK_ACTIVE)
+        inter = self.interior
+        flag[inter] = lake[inter].to(torch.uint8)
+        self.lake_flag = flag
+        return {"lake_cells": int(lake[inter].sum().item())}
+
+
 # ---------------------------------------------------------------------------
 # spawning (maps.py:793-833)
 # ---------------------------------------------------------------------------
@@ -383,8 +490,21 @@ def spawn_particles(state: ErosionState, n_particles: int, rng: np.random.Genera
 
 
 def _check_scope(ep, state):
+    """The fast mode's limits: it steers on the terrain (no routing surface)."""
     if state.route is not None:
-        raise NotImplementedError("the routing surface (erosion.flood_every) is outside the HIP erosion path")
+        raise NotImplementedError("the routing surface (erosion.flood_every) runs in the check mode only")
+
+
+def require_plain_side_passes(ep, what: str) -> None:
+    """Refuse the side passes a driver with its own iteration loop (batched
+    windows, the differentiable mode) does not run."""
+    on = [n for n, v in (("thermal_rate", float(ep.thermal_rate) != 0.0), ("creep_rate", float(ep.creep_rate) > 0.0),
+                         ("slope_limit_erode", float(getattr(ep, "slope_limit_erode", 0.0)) > 0.0),
+                         ("slope_limit_deposit", float(getattr(ep, "slope_limit_deposit", 0.0)) > 0.0),
+                         ("isostasy", float(getattr(ep, "isostasy", 0.0)) > 0.0),
+                         ("flood_every", int(ep.flood_every) > 0)) if v]
+    if on:
+        raise NotImplementedError(f"{what} runs the plain profile: switch off {', '.join(on)}")
 
 
 def trace_params(ep, state, volume0, P, chunk, disc_sat, mom_scale, max_steps, lake_park) -> TraceParams:
@@ -464,7 +584,6 @@ def run_iteration(state: ErosionState, params, iteration_key, n_particles=None, 
         return fast.run_iteration(state, params, iteration_key, n_particles=n_particles,
                                   particles_per_cell=particles_per_cell, rng_stage=rng_stage, log=log)
     ep = _eparams(params)
-    _check_scope(ep, state)
     key = iteration_key if isinstance(iteration_key, (tuple, list)) else (int(iteration_key),)
     rng = params.rng(rng_stage, *key)
     t0 = time.time()
@@ -800,18 +919,30 @@ def start_replay(state: ErosionState, params):
 
 
 def step(state: ErosionState, params, iteration_key, log=None, after_refresh=None, isostasy_due=None, **kw) -> dict:
-    """One iteration in the reference's order (maps.py:1516-1617):
-    particles, mass wasting, uplift, isostasy (every ``isostasy_every``
+    """One iteration in the reference's order (maps.py:1516-1617): the
+    routing flood (every ``flood_every``) and a window's lakes, particles,
+    mass wasting, uplift, isostasy (every ``isostasy_every``
     iterations; ``isostasy_due`` overrides that choice, for graph capture),
     datum hold (globe), halo exchange."""
     ep = _eparams(params)
-    if int(ep.flood_every) > 0 or (state.spherical and int(getattr(ep, "sea_mask_every", 0)) > 0):
-        raise NotImplementedError("routing / sea-mask refreshes are outside the HIP erosion path")
+    if state.spherical and int(getattr(ep, "sea_mask_every", 0)) > 0:
+        raise NotImplementedError("the sea-mask refresh (erosion.sea_mask_every) is outside the HIP erosion path")
+    if state.spherical and bool(getattr(ep, "lake_balance", False)) and int(ep.flood_every) > 0:
+        raise NotImplementedError("lake refreshes (erosion.lake_balance) are outside the HIP erosion path")
     if state.spherical and int(ep.glacial_every) > 0:
         raise NotImplementedError("the glacial pass is outside the HIP erosion path (set glacial_every = 0)")
     t0 = time.time()
+    flood = int(ep.flood_every)
+    if flood > 0 and (state.route is None or state.iteration % flood == 0):
+        state.refresh_route(cell_units(ep, "route_eps", state.height_unit_m))
+    if not state.spherical and bool(getattr(ep, "window_lakes", False)) and flood > 0 and \
+            (state.lake_flag is None or state.iteration % flood == 0):
+        hydro = getattr(params, "hydro", None)
+        min_depth = float(hydro.lake_min_depth) if hydro is not None else 0.5
+        state.refresh_lakes_window(min_depth / float(state.height_unit_m), float(getattr(ep, "window_lake_evap", 0.0)))
     if after_refresh is not None:
         after_refresh(state)
+    tr = time.time() - t0
     iso = state.spherical and float(getattr(ep, "isostasy", 0.0)) > 0.0
     if iso:
         if state.iso_acc is None:
@@ -834,8 +965,8 @@ def step(state: ErosionState, params, iteration_key, log=None, after_refresh=Non
         st["datum_shift"] = q if state.mode == "fast" else float(q.item())
     state.exchange_halos()
     state.iteration += 1
-    st["seconds_route"] = 0.0
-    st["seconds_particles"] = t1 - t0
+    st["seconds_route"] = tr
+    st["seconds_particles"] = t1 - t0 - tr
     st["seconds_total"] = time.time() - t0
     return st
 

@@ -342,3 +342,36 @@ def divide(x, v, out):
     _need(x, torch.float64, None, "x")
     _need(out, torch.float64, tuple(x.shape), "out")
     check(lib().pg_divide(x.numel(), _ptr(x), float(v), _ptr(out), _stream()), "divide")
+
+
+# ---------------------------------------------------------------------------
+# the routing surface (route.py:30-117)
+# ---------------------------------------------------------------------------
+def flood_init(height, sediment, base, cls, H, N, s_out, W, flag):
+    F, NE, _ = height.shape
+    e = (F, NE, NE)
+    for t, n in ((height, "height"), (sediment, "sediment"), (base, "base"), (s_out, "s"), (W, "W")):
+        _need(t, torch.float64, e, n)
+    _need(cls, torch.uint8, e, "cls")
+    _need(flag, torch.int32, (1,), "flag")
+    check(lib().pg_flood_init(F, NE, int(H), int(N), _ptr(height), _ptr(sediment), _ptr(base), _ptr(cls), _ptr(s_out),
+                              _ptr(W), _ptr(flag), _stream()), "flood_init")
+
+
+def flood_sweeps(s, cls, W, eps, spec_cell, spec_ptr, spec_src, sweeps, flag):
+    F, NE, _ = s.shape
+    n_spec = 0 if spec_cell is None else int(spec_cell.numel())
+    if n_spec:
+        _need(spec_cell, torch.int64, (n_spec,), "spec_cell")
+        _need(spec_ptr, torch.int32, (n_spec + 1,), "spec_ptr")
+        _need(spec_src, torch.int64, None, "spec_src")
+    check(lib().pg_flood_sweeps(F, NE, _ptr(s), _ptr(cls), _ptr(W), float(eps), _ptr(spec_cell if n_spec else None),
+                                _ptr(spec_ptr if n_spec else None), _ptr(spec_src if n_spec else None), n_spec,
+                                int(sweeps), _ptr(flag), _stream()), "flood_sweeps")
+
+
+def flood_finish(s, W, route):
+    e = tuple(s.shape)
+    for t, n in ((s, "s"), (W, "W"), (route, "route")):
+        _need(t, torch.float64, e, n)
+    check(lib().pg_flood_finish(s.numel(), _ptr(s), _ptr(W), _ptr(route), _stream()), "flood_finish")

@@ -30,8 +30,10 @@ def window_params(seed: int = 0, **erosion) -> WorldParams:
     """small_world with the side passes the HIP path leaves out switched off
     (thermal, creep, routing flood, sea mask, lakes, glacial, isostasy)."""
     p = small_world(seed)
-    for k, v in dict(thermal_rate=0.0, creep_rate=0.0, flood_every=0, sea_mask_every=0, lake_balance=False,
-                     lake_fill=False, glacial_every=0, isostasy=0.0, **erosion).items():
+    d = dict(thermal_rate=0.0, creep_rate=0.0, flood_every=0, sea_mask_every=0, lake_balance=False, lake_fill=False,
+             glacial_every=0, isostasy=0.0)
+    d.update(erosion)
+    for k, v in d.items():
         setattr(p.erosion, k, v)
     return p
 
@@ -99,3 +101,20 @@ def window_inputs(N: int, mode: str, params: WorldParams, seed: int = 0, relief:
                 hardness=np.full((NE, NE), 0.5, np.float32), precip=precip, evap=np.ones((NE, NE), np.float32),
                 uplift_m=upl * cs, mask=mask, metric=metric, metric_inv=metric, H=H, height_unit_m=cs,
                 deposit_on_exit=deposit_on_exit)
+
+
+def relief_globe(N: int = 48, H: int = 2, seed: int = 0):
+    """A globe of 50 m cells with ~60 cells of relief, noise and a sea (so
+    slopes exceed the talus and depressions abound): (grid, dict of extended
+    inputs in metres) for ``ErosionState.from_grid``."""
+    from globe_hip.cubesphere import get_grid
+
+    g = get_grid(N, H, 50.0)
+    rng = np.random.default_rng(seed)
+    p = g.centers
+    bed = 1500.0 * (np.sin(4 * p[..., 0] + 1.0) * np.cos(3 * p[..., 1]) + 0.6 * np.sin(7 * p[..., 2]))
+    bed = bed + 250.0 * rng.standard_normal(bed.shape)
+    F, NE = 6, g.NE
+    hard = rng.uniform(0, 1, (F, NE, NE)).astype(np.float32)
+    ones = np.ones((F, NE, NE), np.float32)
+    return g, dict(bedrock_m=bed, hardness=hard, precip=ones, evap=ones, uplift_m=np.zeros((F, NE, NE)))
