@@ -34,12 +34,13 @@ for _p in (ROOT, os.path.join(ROOT, "procedural-generation_amd")):
 import numpy as np  # noqa: E402
 
 METRIC = "droplet-steps/s at N_c=1024 vs numba host; fwd+adjoint/fwd ratio"
-#: algorithmic HBM bytes per droplet-step and trace pass (FP32 fast mode, plain profile): the applying pass
-#: reads the 2x2 stencil x 4 channels (64), 11 cell scalars (44), next stencil + base + mask (24) and writes
-#: four int64 fixed-point accumulators (32); the census pass reads the same 132 and writes the erosion
-#: census and floor key (16).  Every move is traced by both passes.
-BYTES_PER_STEP = 164.0
-BYTES_PER_STEP_CENSUS = 148.0
+#: algorithmic HBM bytes per droplet-step and pass (FP32 fast mode, plain profile): the census pass
+#: (trace_kernel) reads the 2x2 stencil x 4 channels (64), 11 cell scalars (44), next stencil + base + mask
+#: (24), writes the erosion census and floor key (16) and the move's 32-byte tape record; the applying pass
+#: (replay_kernel) reads the record (32) and the cell's erosion scale (4) and updates four int64
+#: fixed-point accumulators (32).
+BYTES_PER_STEP_CENSUS = 180.0
+BYTES_PER_STEP = 68.0
 HBM_PEAK_GBS = 8000.0
 #: HBM bytes per trace launch of this workload (N_c=1024, 16 chunks) by register-budget variant, from
 #: rocprofv3 FETCH_SIZE (x 2.000, calibrated on a 1-GiB copy) + WRITE_SIZE (x 1.000), mean over both
@@ -332,7 +333,7 @@ def side_run(grid, f, p, reps=5, cpu=True):
     return out
 
 
-def atomic_probe(NE=1028, F=6, lanes=1 << 20, steps=64):
+def atomic_probe(NE=1028, F=6, lanes=1 << 20, steps=64, hot=False):
     """The device's 64-bit atomicAdd rate on the trace's footprint (6 faces of
     NE^2 cells): a droplet-like walk adding to 4 planes per step (the applying
     pass's pattern), the same walk with 1 plane, and uniformly random cells."""
@@ -345,7 +346,10 @@ def atomic_probe(NE=1028, F=6, lanes=1 << 20, steps=64):
     n = F * NE * NE
     t = torch.zeros(4 * n, dtype=torch.int64, device="cuda:0")
     out = {}
-    for name, planes, pattern in (("walk_4_planes", 4, 0), ("walk_1_plane", 1, 0), ("random_1_plane", 1, 1)):
+    cases = (("walk_4_planes", 4, 0), ("walk_1_plane", 1, 0), ("random_1_plane", 1, 1))
+    if hot:
+        cases = cases + tuple((f"hot_{h}_cells", 1, h) for h in (1 << 20, 1 << 16, 1 << 12))
+    for name, planes, pattern in cases:
         def run(seed):
             _lib.check(_lib.lib().pg_atomic_probe(ctypes.c_void_p(t.data_ptr()), n, NE, planes, lanes, steps, pattern,
                                                   seed, ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)),
@@ -363,7 +367,7 @@ def atomic_probe(NE=1028, F=6, lanes=1 << 20, steps=64):
         m = sorted(ms)[1]
         out[name] = {"ms": m, "gatomics_per_s": lanes * steps * planes / (m * 1e-3) / 1e9}
     tot = int(t.sum().item())
-    assert tot == 4 * lanes * steps * (4 + 1 + 1), "atomic probe lost updates"
+    assert tot == 4 * lanes * steps * sum(c[1] for c in cases), "atomic probe lost updates"
     log("atomic probe (G atomics/s): " + ", ".join(f"{k} {v['gatomics_per_s']:.1f}" for k, v in out.items()))
     return out
 
@@ -402,7 +406,7 @@ def main():
         copy_probe(a.copy_probe_mib)
         return
     if a.atomic_probe:
-        print(json.dumps({"atomic_probe": atomic_probe()}), flush=True)
+        print(json.dumps({"atomic_probe": atomic_probe(hot=True)}), flush=True)
         return
     if a.gpus != 1:
         print(f"bench.py measures one GPU: the particle pass shards by droplets with one int64 all-reduce per "
@@ -436,8 +440,8 @@ def main():
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
                      "traffic": TRACE_TRAFFIC_BYTES.get(r["variant"]) if (a.N, a.chunks) == (1024, 16) else None,
-                     "kernel": "trace_kernel<false>",
-                     "bytes_per_step": {"apply_pass": BYTES_PER_STEP, "census_pass": BYTES_PER_STEP_CENSUS},
+                     "kernel": "trace_kernel<false> (census) + replay_kernel (applying pass)",
+                     "bytes_per_step": {"applying_pass": BYTES_PER_STEP, "census_pass": BYTES_PER_STEP_CENSUS},
                      "launch_ms_mean": r["trace_ms"] / max(r["launches"], 1), "launches": r["launches"]},
     }
     probe = atomic_probe()

@@ -48,8 +48,8 @@ typedef struct pg_trace_params {
 
 /* Message of the last failed call on this thread ("" if none). */
 const char* pg_last_error(void);
-/* ABI version of this library: 2 (bumped whenever a struct layout or a
- * signature changes; 2 added pg_fast_params.erode_cap). */
+/* ABI version of this library: 3 (bumped whenever a struct layout or a
+ * signature changes; 2 added pg_fast_params.erode_cap, 3 its tape). */
 int pg_abi_version(void);
 
 /* ---- geometry, host side (bake/globe/cubesphere.py:70-220) ------------- */
@@ -240,7 +240,8 @@ int pg_flood_finish(int64_t n, const double* s, const double* W, double* route, 
 /* ---- calibration (no counterpart in the reference) ---------------------- */
 /* The device's relaxed 64-bit atomicAdd rate: `lanes` lanes each take
  * `steps` steps of a droplet-like walk (pattern 0: to one of the 8 neighbours
- * on an NE-wide grid of n cells; 1: to a uniformly random cell) and add 1 to
+ * on an NE-wide grid of n cells; 1: to a uniformly random cell; h >= 2: to one
+ * of the first h cells, same-address contention) and add 1 to
  * the cell in each of `planes` int64 planes of n (targets: DEVICE
  * int64[planes * n], n > 4 NE).  For the trace kernel's roofline. */
 int pg_atomic_probe(int64_t* targets, int64_t n, int64_t NE, int32_t planes, int64_t lanes, int32_t steps,
@@ -281,7 +282,21 @@ typedef struct pg_fast_params {
     int64_t variant;       /* trace kernel register budget: 0 = 2 waves per SIMD, 1 = 3, 2 = 4 (same results) */
     const double* erode_cap; /* optional DEVICE per-cell erosion cap of the iteration (soillib's slope limit,
                               * pg_slope_erode_cap; bake/globe/erosion/maps.py:923-926), NULL = none */
+    void* tape;            /* optional DEVICE move tape of pg_fast_tape_bytes() bytes: the census pass records
+                            * every move and the applying pass replays it instead of tracing again (same
+                            * results); NULL = trace twice.  pg_fast_iterate / pg_fast_particles only */
+    int64_t tape_bytes;
 } pg_fast_params;
+
+/* Bytes of the move tape for n_particles rain droplets on F faces of N x N
+ * (plus at most one stockpile droplet per cell) in `chunks` chunks of at most
+ * max_steps moves, for the persistent trace grid of (grid_blocks, variant) as
+ * in pg_fast_params: per wave of that grid, rows of 64 lane records of 32
+ * bytes -- twice the wave's share of the largest chunk's droplet-iterations
+ * plus one droplet's worth -- and a row count; and a stop cell per droplet
+ * of the largest chunk. */
+int64_t pg_fast_tape_bytes(int64_t n_particles, int64_t F, int64_t N, int64_t chunks, int64_t max_steps,
+                           int64_t grid_blocks, int64_t variant);
 
 /* Statistics of one fast iteration (DEVICE int64[PG_FAST_NSTATS]). */
 enum {
@@ -289,6 +304,7 @@ enum {
     PG_FS_CLAMPED = 10, PG_FS_TO_PENDING_FX, PG_FS_LOST_OFFSHORE_FX, PG_FS_EXIT_FX, PG_FS_VANISHED_FX,
     PG_FS_RELEASED_FX, PG_FS_VOLUME0_BITS, PG_FS_ELIGIBLE, PG_FS_CDF_SERIAL, PG_FS_ERODED_FX, PG_FS_DEPOSITED_FX,
     PG_FS_DEFICIT_FX,
+    PG_FS_TAPE_SHORT = 23, /* droplets a chunk's census left untraced, its tape rows full (always 0) */
     PG_FAST_NSTATS = 24
 };
 

@@ -374,6 +374,15 @@ struct TraceCfg {
     ll* mom_fx;    // momentum track a; track b in its own plane (a move's atomics on separate lines)
     ll* mom_fx_b;
     ll* stats;
+    // the census's move tape (single grid only; NULL = the applying pass
+    // traces again): per wave of the persistent grid, `tape_rows` rows of 64
+    // lane records (two float4 each) -- row t holds what every lane of the
+    // wave did in its t-th loop iteration -- and the rows used (tape_len)
+    float4* tape;
+    ll* tape_len;
+    ll tape_rows;
+    int* tape_last;  // per droplet of the chunk: the stop cell of its final deposit (a row can hold a move and an end)
+    ll tape_waves;   // waves of the census grid (the replay's grid strides over them)
     // batched windows: per-window geometry and spawn totals, per-chunk union
     // offsets (n_win + 1 per chunk), the batch's queue counters
     const WinDesc* wins;
@@ -399,18 +408,36 @@ __device__ __forceinline__ float fkey_value(unsigned k) {
     return __uint_as_float((k & 0x80000000u) ? (k & 0x7fffffffu) : ~k);
 }
 
+// ceiling of a step deposit on the frozen terrain as a float (3e38: none)
+// and in fixed point: the previous path cell (+ dep_floor on land,
+// - fan_slope under water) and the waterline
+__device__ __forceinline__ ll ceiling_fx(float lim, double hscale) {
+    if (lim >= 3.0e38f) return (ll)9.0e18;
+    return lim > 0.0f ? llrint((double)lim * hscale) : 0;
+}
+
+// tape record: flags word = transfer kind (bits 0-1: none / sea deposit /
+// land step), MOVE (bit 2), START of a droplet (bit 3), END (bit 4), the end's
+// cause (bits 8-15), exited (16), final deposit (17), final entries (20-23)
+enum : int { REC_NONE = 0, REC_SEA = 1, REC_LAND = 2 };
+enum : int { RF_MOVE = 1 << 2, RF_START = 1 << 3, RF_END = 1 << 4, RF_EXITED = 1 << 16, RF_DEPOSIT = 1 << 17 };
+
 // WPE: the register budget as waves per SIMD (2 = the kernel's natural
-// allocation; 3 and 4 trade spills for latency hiding; the autotuner picks)
-template <bool BATCH, int WPE>
+// allocation; 3 and 4 trade spills for latency hiding; the autotuner picks).
+// APPLY: false = the census pass (erosion requests and floors; with a tape,
+// every move recorded for replay_kernel), true = the applying pass traced
+// again (batched windows, or no tape).
+template <bool BATCH, int WPE, bool APPLY>
 __global__ void __launch_bounds__(TRACE_TPB) __attribute__((amdgpu_waves_per_eu(WPE, WPE))) trace_kernel(TraceCfg k) {
     const int lane = threadIdx.x & 63;
-    const bool apply = k.pass == 1;
+    constexpr bool apply = APPLY;
+    const bool taping = !APPLY && !BATCH && k.tape != nullptr;
     // per-lane geometry and droplet scalars: the grid's (one launch = one
     // grid), or, batched, those of the window of the lane's droplet
     int N = k.N, NE = k.NE, max_steps = k.max_steps;
     int64_t coff = 0;
     float inv_v0 = 0.0f, min_volume = 0.0f, conc_scale = 0.0f, v0f = 0.0f;
-    ll d_end;
+    ll d_end, d_begin = 0;
     ll* queue;
     if constexpr (BATCH) {
         d_end = k.win_prefix[(ll)k.chunk * (k.n_win + 1) + k.n_win];
@@ -418,6 +445,7 @@ __global__ void __launch_bounds__(TRACE_TPB) __attribute__((amdgpu_waves_per_eu(
     } else {
         const ll n_total = k.sc->n_total;
         const ll C = (n_total + k.chunks - 1) / k.chunks;
+        d_begin = k.chunk * C;
         d_end = (k.chunk + 1) * C < n_total ? (k.chunk + 1) * C : n_total;
         const double volume0 = k.sc->volume0;
         if (volume0 <= 0.0) return;
@@ -438,6 +466,11 @@ __global__ void __launch_bounds__(TRACE_TPB) __attribute__((amdgpu_waves_per_eu(
     int step = 0, uphill = 0, sea_steps = 0, pushes = 0, last_cell = -1;
     ll real = 0, deficit = 0;  // load actually held (fixed point) and load cancelled by erosion caps
     ll surplus = 0;            // deposits a ceiling refused, offered to the droplet's next deposits
+    ll dl = 0;                 // the droplet's slot in the chunk
+    // the wave's tape rows (census with a tape): wave-uniform row counter
+    const ll gw = (ll)blockIdx.x * (TRACE_TPB / 64) + (threadIdx.x >> 6);
+    float4* wtape = taping ? k.tape + 2 * gw * k.tape_rows * 64 : nullptr;
+    ll t_row = 0;
     Stencil S;
     Here cur;
     // lane totals
@@ -473,7 +506,7 @@ __global__ void __launch_bounds__(TRACE_TPB) __attribute__((amdgpu_waves_per_eu(
     };
     // ceiling of a step deposit on the frozen terrain: the previous path cell
     // (+ dep_floor on land, - fan_slope under water) and the waterline
-    auto step_ceiling = [&](float h_here, float b_here) -> ll {
+    auto ceiling_lim = [&](float h_here, float b_here) -> float {
         float lim = 3.0e38f;
         if (has_prev) {
             float ceil_ = h_prev + k.dep_floor;
@@ -481,11 +514,16 @@ __global__ void __launch_bounds__(TRACE_TPB) __attribute__((amdgpu_waves_per_eu(
             lim = ceil_ - h_here;
         }
         if (h_here < b_here) lim = fminf(lim, b_here - k.dep_floor - h_here);
-        if (lim >= 3.0e38f) return (ll)9.0e18;
-        return lim > 0.0f ? llrint((double)lim * k.hscale) : 0;
+        return lim;
     };
+    auto step_ceiling = [&](float h_here, float b_here) -> ll { return ceiling_fx(ceiling_lim(h_here, b_here), k.hscale); };
 
     while (true) {
+        // this iteration's tape record
+        int r_flags = 0, r_cell = 0, r_dl = 0;
+        float r_x = 0.0f, r_aux = 0.0f, r_vol = 0.0f, r_sa = 0.0f, r_sb = 0.0f;
+        // a wave whose tape rows could not hold one more droplet takes none
+        if (taping && !exhausted && t_row + k.max_steps + 2 > k.tape_rows) exhausted = true;
         // refill the lanes whose droplet died from the chunk's queue
         if (!exhausted) {
             const ull dead = __ballot(!alive);
@@ -521,6 +559,11 @@ __global__ void __launch_bounds__(TRACE_TPB) __attribute__((amdgpu_waves_per_eu(
                         }
                     }
                     if (u < d_end) {
+                        dl = u - d_begin;
+                        if (taping) {
+                            r_flags |= RF_START;
+                            r_dl = (int)dl;
+                        }
                         const int NE2 = NE * NE;
                         const int cell = k.d_cell[d];
                         f = cell / NE2;
@@ -548,7 +591,8 @@ __global__ void __launch_bounds__(TRACE_TPB) __attribute__((amdgpu_waves_per_eu(
             }
         }
         if (!__any(alive)) break;
-        if (!alive) continue;
+        do {
+        if (!alive) break;
 
         // ---- one step of this lane's droplet (particle.py:421-769)
         int cause = -1;
@@ -596,6 +640,8 @@ __global__ void __launch_bounds__(TRACE_TPB) __attribute__((amdgpu_waves_per_eu(
                             const float h1 = lerp2(S1.c00.x, S1.c10.x, S1.c01.x, S1.c11.x, S1.wi, S1.wj);
                             const float b1 = ndy.y;
                             const float dh = h0 - fmaxf(h1, b1);
+                            int rk = REC_NONE;
+                            float rx = 0.0f, rroom = 0.0f;
                             // mass transfer at the cell being left
                             if (cur.m == MASK_ACTIVE && in_sea) {
                                 float cdiff = -k.ocean_rate * sed;
@@ -608,6 +654,9 @@ __global__ void __launch_bounds__(TRACE_TPB) __attribute__((amdgpu_waves_per_eu(
                                 if (apply) {
                                     deposit(cur.cell, llrint((double)(-cdiff * vol_rel) * k.hscale), step_ceiling(h_here, b_here));
                                     n_entries += 1;
+                                } else if (taping) {
+                                    rk = REC_SEA;
+                                    rx = -cdiff * vol_rel;
                                 }
                             } else if (cur.m == MASK_ACTIVE) {
                                 float qe;
@@ -635,14 +684,20 @@ __global__ void __launch_bounds__(TRACE_TPB) __attribute__((amdgpu_waves_per_eu(
                                     if (-cdiff * vol_rel > dcap) cdiff = -dcap / vol_rel;
                                 }
                                 sed += cdiff;
-                                ll dq = llrint((double)(cdiff * vol_rel) * k.hscale);  // > 0: erosion
+                                const float amount = cdiff * vol_rel;
+                                ll dq = llrint((double)amount * k.hscale);  // > 0: erosion
                                 ll cut = 0;
+                                if (taping) {
+                                    rk = REC_LAND;
+                                    rx = amount;
+                                }
                                 if (dq > 0) {
                                     // an entry never takes the cell below its base level nor below the
                                     // cell it steps to (frozen surfaces; particle.py:951-962)
                                     float fl = h_here >= b_here ? b_here : h_here;
                                     fl = fmaxf(fl, nrec.x);
                                     const float room = h_here - fl;
+                                    rroom = room;
                                     const ll lim = room > 0.0f ? llrint((double)room * k.hscale) : 0;
                                     if (dq > lim) {
                                         cut = dq - lim;
@@ -669,6 +724,18 @@ __global__ void __launch_bounds__(TRACE_TPB) __attribute__((amdgpu_waves_per_eu(
                                     atomicAdd((ull*)&k.mom_fx_b[cur.cell], (ull)llrint((double)(vol * sb) * k.tscale));
                                     n_entries += 1;
                                 }
+                            }
+                            if (taping) {
+                                // the move as the applying pass needs it: cell, kind, amount, and the
+                                // floor room of an erosion or the ceiling of a deposit; the track inputs
+                                r_flags |= RF_MOVE | rk;
+                                r_cell = (int)cur.cell;
+                                r_x = rx;
+                                r_aux = (rk == REC_LAND && rx > 0.0f) ? rroom
+                                        : (rk != REC_NONE ? ceiling_lim(h_here, b_here) : 0.0f);
+                                r_vol = vol;
+                                r_sa = sa;
+                                r_sb = sb;
                             }
                             // evaporation, advance, stop tests
                             const float ev = fmaxf(1.0f - k.dt_evap * g1.z, 0.01f);
@@ -704,9 +771,22 @@ __global__ void __launch_bounds__(TRACE_TPB) __attribute__((amdgpu_waves_per_eu(
                 }
             }
         }
-        if (cause < 0) continue;
+        if (cause < 0) break;
 
         // ---- the droplet stopped: final deposit (particle.py:770-815)
+        if (taping) {
+            int cz = cause;
+            if (in_sea && (cz == DEATH_AGE || cz == DEATH_STOP)) cz = DEATH_OCEAN;
+            int lc = last_cell, pu = pushes;
+            if (!exited && cur.m == MASK_ACTIVE && !(in_sea && cur.h >= cur.base) && (pu == 0 || lc != (int)cur.cell)) {
+                lc = (int)cur.cell;
+                pu += 1;
+            }
+            const bool dodep = pu > 0 && (!exited || k.deposit_on_exit);
+            const int nfin = dodep && sed > 0.0f ? (cz == DEATH_OCEAN ? 1 : (pu < SPREAD ? pu : SPREAD)) : 0;
+            r_flags |= RF_END | (cz << 8) | (exited ? RF_EXITED : 0) | (dodep ? RF_DEPOSIT : 0) | (nfin << 20);
+            k.tape_last[dl] = lc;
+        }
         if (apply) {
             if (in_sea && (cause == DEATH_AGE || cause == DEATH_STOP)) cause = DEATH_OCEAN;
             deaths[cause] += 1;
@@ -733,7 +813,16 @@ __global__ void __launch_bounds__(TRACE_TPB) __attribute__((amdgpu_waves_per_eu(
         deficit = 0;
         surplus = 0;
         alive = false;
+        } while (false);
+        if (taping) {
+            // one row per loop iteration: the wave's 64 records side by side
+            float4* row = wtape + 2 * (t_row * 64 + lane);
+            row[0] = make_float4(__int_as_float(r_flags), __int_as_float(r_cell), r_x, r_aux);
+            row[1] = make_float4(r_vol, r_sa, r_sb, __int_as_float(r_dl));
+        }
+        t_row += 1;
     }
+    if (taping && lane == 0) k.tape_len[gw] = t_row;
     if (!apply) return;
     // lane totals -> device statistics (one atomic per wave and counter)
     ll v[5 + N_DEATHS] = {n_entries, n_moves, exit_fx, vanished_fx, lost_fx};
@@ -748,19 +837,139 @@ __global__ void __launch_bounds__(TRACE_TPB) __attribute__((amdgpu_waves_per_eu(
     }
 }
 
-// the trace kernel of a register-budget variant (0: 2 waves per SIMD, 1: 3, 2: 4)
-template <bool BATCH>
+// the trace kernel of a register-budget variant (0: 2 waves per SIMD, 1: 3, 2: 4) and pass
+template <bool BATCH, bool APPLY>
 const void* trace_fn(int64_t variant) {
-    if (variant == 1) return (const void*)trace_kernel<BATCH, 3>;
-    if (variant == 2) return (const void*)trace_kernel<BATCH, 4>;
-    return (const void*)trace_kernel<BATCH, 2>;
+    if (variant == 1) return (const void*)trace_kernel<BATCH, 3, APPLY>;
+    if (variant == 2) return (const void*)trace_kernel<BATCH, 4, APPLY>;
+    return (const void*)trace_kernel<BATCH, 2, APPLY>;
 }
 
-template <bool BATCH>
+template <bool BATCH, bool APPLY>
 void launch_trace(int64_t variant, int grid, hipStream_t s, const TraceCfg& k) {
-    if (variant == 1) hipLaunchKernelGGL((trace_kernel<BATCH, 3>), dim3(grid), dim3(TRACE_TPB), 0, s, k);
-    else if (variant == 2) hipLaunchKernelGGL((trace_kernel<BATCH, 4>), dim3(grid), dim3(TRACE_TPB), 0, s, k);
-    else hipLaunchKernelGGL((trace_kernel<BATCH, 2>), dim3(grid), dim3(TRACE_TPB), 0, s, k);
+    if (variant == 1) hipLaunchKernelGGL((trace_kernel<BATCH, 3, APPLY>), dim3(grid), dim3(TRACE_TPB), 0, s, k);
+    else if (variant == 2) hipLaunchKernelGGL((trace_kernel<BATCH, 4, APPLY>), dim3(grid), dim3(TRACE_TPB), 0, s, k);
+    else hipLaunchKernelGGL((trace_kernel<BATCH, 2, APPLY>), dim3(grid), dim3(TRACE_TPB), 0, s, k);
+}
+
+// ---------------------------------------------------------------------------
+// the applying pass from the census's tape: every wave of the census's grid
+// replays its own rows -- the same lanes, droplets and moves in the same
+// order, each row one coalesced read -- with the chunk's per-cell erosion
+// scales: the fixed-point arithmetic of the traced applying pass
+// (bit-identical results) without tracing again.
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(TRACE_TPB) replay_kernel(TraceCfg k) {
+    const int lane = threadIdx.x & 63;
+    const ll rw = (ll)blockIdx.x * (TRACE_TPB / 64) + (threadIdx.x >> 6);
+    const ll rwaves = (ll)gridDim.x * (TRACE_TPB / 64);
+    const ll n_total = k.sc->n_total;
+    const ll C = (n_total + k.chunks - 1) / k.chunks;
+    const ll d_begin = k.chunk * C;
+    const ll d_end = (k.chunk + 1) * C < n_total ? (k.chunk + 1) * C : n_total;
+    if (k.sc->volume0 <= 0.0) return;
+    ll real = 0, deficit = 0, surplus = 0;
+    int dl = 0;
+    ll n_entries = 0, n_moves = 0, exit_fx = 0, vanished_fx = 0, lost_fx = 0;
+    ll deaths[N_DEATHS] = {0, 0, 0, 0, 0, 0};
+    // a deposit request: the amount laid down (the caller adds it to the cell)
+    auto deposit = [&](ll d, ll lim) -> ll {
+        ll want = d + surplus;
+        surplus = 0;
+        ll a = want < deficit ? want : deficit;
+        if (a < 0) a = 0;
+        deficit -= a;
+        want -= a;
+        ll dep = want < lim ? want : lim;
+        if (dep < 0) dep = 0;
+        surplus = want - dep;
+        real -= dep;
+        return dep;
+    };
+    // each replay wave takes census waves rw, rw + rwaves, ...: every census
+    // wave's rows are replayed by one wave, lane for lane, in order
+    for (ll gw = rw; gw < k.tape_waves; gw += rwaves) {
+        const float4* wtape = k.tape + 2 * gw * k.tape_rows * 64;
+        const ll rows = k.tape_len[gw];
+        for (ll t = 0; t < rows; t++) {
+            const float4* row = wtape + 2 * (t * 64 + lane);
+            const float4 A = row[0];
+            const int fl = __float_as_int(A.x);
+            if (!fl) continue;
+            const float4 B = row[1];
+            if (fl & RF_START) {
+                dl = __float_as_int(B.w);
+                real = k.d_load[d_begin + dl];
+                deficit = surplus = 0;
+            }
+            if (fl & RF_MOVE) {
+                // one height-change atomic per row for erosion and deposit alike (full lanes per instruction)
+                const int64_t cell = (int64_t)__float_as_int(A.y);
+                const int kind = fl & 3;
+                ll change = 0;
+                if (kind == REC_SEA) {
+                    change = deposit(llrint((double)A.z * k.hscale), ceiling_fx(A.w, k.hscale));
+                    n_entries += 1;
+                } else if (kind == REC_LAND) {
+                    ll dq = llrint((double)A.z * k.hscale);
+                    ll cut = 0;
+                    if (dq > 0) {
+                        const ll lim = A.w > 0.0f ? llrint((double)A.w * k.hscale) : 0;
+                        if (dq > lim) {
+                            cut = dq - lim;
+                            dq = lim;
+                        }
+                    }
+                    deficit += cut;
+                    if (dq > 0) {
+                        const ll taken = (ll)floor((double)dq * (double)k.rscale[cell]);
+                        deficit += dq - taken;
+                        real += taken;
+                        change = -taken;
+                    } else if (dq < 0) {
+                        change = deposit(-dq, ceiling_fx(A.w, k.hscale));
+                    }
+                    n_entries += 1;
+                }
+                if (change) atomicAdd((ull*)&k.chunk_fx[cell], (ull)change);
+                if (kind == REC_LAND) {
+                    atomicAdd((ull*)&k.dis_fx[cell], (ull)llrint((double)B.x * k.tscale));
+                    atomicAdd((ull*)&k.mom_fx[cell], (ull)llrint((double)(B.x * B.y) * k.tscale));
+                    atomicAdd((ull*)&k.mom_fx_b[cell], (ull)llrint((double)(B.x * B.z) * k.tscale));
+                }
+                n_moves += 1;
+            }
+            if (fl & RF_END) {
+                // the droplet's end (particle.py:770-815): cause, stop cell and final entries as the census saw them
+                deaths[(fl >> 8) & 0xff] += 1;
+                n_entries += (fl >> 20) & 0xf;
+                if (fl & RF_DEPOSIT) {
+                    if (real) atomicAdd((ull*)&k.chunk_fx[k.tape_last[dl]], (ull)real);
+                } else if (fl & RF_EXITED) {
+                    exit_fx += real;
+                } else {
+                    vanished_fx += real;
+                }
+                lost_fx += deficit;
+                real = deficit = surplus = 0;
+            }
+        }
+    }
+    ll v[5 + N_DEATHS] = {n_entries, n_moves, exit_fx, vanished_fx, lost_fx};
+    for (int c = 0; c < N_DEATHS; c++) v[5 + c] = deaths[c];
+    for (int c = 0; c < 5 + N_DEATHS; c++) v[c] = wave_sum(v[c]);
+    if (lane == 0) {
+        const int slot[5] = {PG_FS_ENTRIES, PG_FS_MOVES, PG_FS_EXIT_FX, PG_FS_VANISHED_FX, PG_FS_DEFICIT_FX};
+        for (int c = 0; c < 5; c++)
+            if (v[c]) atomicAdd((ull*)&k.stats[slot[c]], (ull)v[c]);
+        for (int c = 0; c < N_DEATHS; c++)
+            if (v[5 + c]) atomicAdd((ull*)&k.stats[PG_FS_DEATHS + c], (ull)v[5 + c]);
+    }
+    // droplets the census left untraced because every wave's rows were full: must be none
+    if (rw == 0 && lane == 0) {
+        const ll taken = k.sc->queue[2 * k.chunk];
+        if (taken < d_end) atomicAdd((ull*)&k.stats[PG_FS_TAPE_SHORT], (ull)(d_end - taken));
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -1076,6 +1285,33 @@ int persistent_grid(const void* kernel, int threads, int64_t requested, int* gri
 using namespace pg;
 using namespace pg::fastmode;
 
+namespace pg {
+namespace fastmode {
+// rows per wave of the census tape for a grid of `grid` blocks
+int64_t tape_rows(int64_t n_particles, int64_t F, int64_t N, int64_t chunks, int64_t max_steps, int grid) {
+    const int64_t cmax = (n_particles + F * N * N + chunks - 1) / chunks;
+    const int64_t waves = (int64_t)grid * (TRACE_TPB / 64);
+    const int64_t share = (cmax * (max_steps + 1) + 64 * waves - 1) / (64 * waves);
+    return 2 * share + max_steps + 3;
+}
+}  // namespace fastmode
+}  // namespace pg
+
+extern "C" int64_t pg_fast_tape_bytes(int64_t n_particles, int64_t F, int64_t N, int64_t chunks, int64_t max_steps,
+                                      int64_t grid_blocks, int64_t variant) {
+    clear_error();
+    if (n_particles < 0 || F <= 0 || N <= 0 || chunks <= 0 || max_steps <= 0 || grid_blocks < 0 || variant < 0 ||
+        variant > 2) {
+        set_error("pg_fast_tape_bytes: bad arguments");
+        return -1;
+    }
+    int grid = 0;
+    if (persistent_grid(trace_fn<false, false>(variant), TRACE_TPB, grid_blocks, &grid)) return -1;
+    const int64_t waves = (int64_t)grid * (TRACE_TPB / 64);
+    const int64_t cmax = (n_particles + F * N * N + chunks - 1) / chunks;
+    return waves * tape_rows(n_particles, F, N, chunks, max_steps, grid) * 64 * 32 + waves * 8 + cmax * 4;
+}
+
 extern "C" int64_t pg_fast_workspace_bytes(int64_t F, int64_t NE, int64_t n_particles) {
     clear_error();
     Layout L;
@@ -1283,15 +1519,34 @@ int fast_particles(const pg_fast_fields* fl, const pg_trace_params* prm, const p
     k.mom_fx = at<ll>(work, L.mom_fx);
     k.mom_fx_b = k.mom_fx + n;
     k.stats = st;
-    int grid = 0;
+    int grid = 0, rgrid = 0;
     PG_REQUIRE(fp->variant >= 0 && fp->variant <= 2, "fast mode: variant must be 0, 1 or 2");
-    if (persistent_grid(trace_fn<false>(fp->variant), TRACE_TPB, fp->grid_blocks, &grid)) return -1;
+    if (persistent_grid(trace_fn<false, false>(fp->variant), TRACE_TPB, fp->grid_blocks, &grid)) return -1;
+    if (fp->tape && persistent_grid((const void*)replay_kernel, TRACE_TPB, 0, &rgrid)) return -1;
+    k.tape = nullptr;
+    k.tape_len = nullptr;
+    k.tape_rows = 0;
+    k.tape_last = nullptr;
+    k.tape_waves = 0;
+    if (fp->tape) {
+        const int64_t waves = (int64_t)grid * (TRACE_TPB / 64);
+        const int64_t rows = tape_rows(fp->n_particles, fl->F, fl->N, chunks, prm->max_steps, grid);
+        const int64_t cmax = (fp->n_particles + fl->F * fl->N * fl->N + chunks - 1) / chunks;
+        const int64_t need = waves * rows * 64 * 32 + waves * 8 + cmax * 4;
+        PG_REQUIRE(fp->tape_bytes >= need, "fast mode: tape of %lld bytes, need %lld", (long long)fp->tape_bytes,
+                   (long long)need);
+        k.tape = reinterpret_cast<float4*>(fp->tape);
+        k.tape_rows = rows;
+        k.tape_len = reinterpret_cast<ll*>(k.tape + 2 * waves * rows * 64);
+        k.tape_last = reinterpret_cast<int*>(k.tape_len + waves);
+        k.tape_waves = waves;
+    }
     for (int c = 0; c < chunks; c++) {
         k.chunk = c;
-        // pass 0: erosion census on the frozen terrain
+        // pass 0: erosion census on the frozen terrain (and, with a tape, every move recorded)
         k.pass = 0;
         if (fp->time_trace && timer_record(s, true)) return -1;
-        launch_trace<false>(fp->variant, grid, s, k);
+        launch_trace<false, false>(fp->variant, grid, s, k);
         if (fp->time_trace && timer_record(s, false)) return -1;
         hipLaunchKernelGGL(erosion_scale_kernel, dim3(grid_for(n)), dim3(TPB), 0, s, n, fl->height, fl->sediment, fl->base,
                            k.ero_fx, k.floor_key, at<ll>(work, L.acc_fx), at<float>(work, L.r1), fp->iter_erode,
@@ -1302,7 +1557,8 @@ int fast_particles(const pg_fast_fields* fl, const pg_trace_params* prm, const p
         // pass 1: the same trajectories, erosion scaled per cell, deficits cancel deposits
         k.pass = 1;
         if (fp->time_trace && timer_record(s, true)) return -1;
-        launch_trace<false>(fp->variant, grid, s, k);
+        if (k.tape) hipLaunchKernelGGL(replay_kernel, dim3(rgrid), dim3(TRACE_TPB), 0, s, k);
+        else launch_trace<false, true>(fp->variant, grid, s, k);
         if (fp->time_trace && timer_record(s, false)) return -1;
         hipLaunchKernelGGL(resolve_kernel, dim3(grid_for(n)), dim3(TPB), 0, s, n, fl->height, fl->sediment, fl->base,
                            k.chunk_fx, at<ll>(work, L.acc_fx), pend_fx, cell4, dyn, fp->iter_deposit, prm->dep_floor,
@@ -1687,13 +1943,13 @@ extern "C" int pg_batch_iterate(const pg_fast_fields* wins, int64_t n_win, const
     int grid = 0;
     PG_REQUIRE(f0.variant >= 0 && f0.variant <= 2, "pg_batch_iterate: variant must be 0, 1 or 2");
     PG_REQUIRE(!f0.erode_cap, "pg_batch_iterate: the slope limit (erode_cap) is not supported on batched windows");
-    if (persistent_grid(trace_fn<true>(f0.variant), TRACE_TPB, f0.grid_blocks, &grid)) return -1;
+    if (persistent_grid(trace_fn<true, false>(f0.variant), TRACE_TPB, f0.grid_blocks, &grid)) return -1;
     const pg_fast_fields& a = wins[0];  // arena bases
     for (int c = 0; c < chunks; c++) {
         k.chunk = c;
         k.pass = 0;
         if (f0.time_trace && timer_record(s, true)) return -1;
-        launch_trace<true>(f0.variant, grid, s, k);
+        launch_trace<true, false>(f0.variant, grid, s, k);
         if (f0.time_trace && timer_record(s, false)) return -1;
         hipLaunchKernelGGL(erosion_scale_kernel, dim3(grid_for(n)), dim3(TPB), 0, s, n, a.height, a.sediment, a.base,
                            k.ero_fx, k.floor_key, at<ll>(work, L.acc_fx), at<float>(work, L.r1), f0.iter_erode,
@@ -1703,7 +1959,7 @@ extern "C" int pg_batch_iterate(const pg_fast_fields* wins, int64_t n_win, const
                            at<float>(work, L.rscale), f0.iter_erode, (const double*)nullptr, hscale);
         k.pass = 1;
         if (f0.time_trace && timer_record(s, true)) return -1;
-        launch_trace<true>(f0.variant, grid, s, k);
+        launch_trace<true, true>(f0.variant, grid, s, k);
         if (f0.time_trace && timer_record(s, false)) return -1;
         hipLaunchKernelGGL(resolve_kernel, dim3(grid_for(n)), dim3(TPB), 0, s, n, a.height, a.sediment, a.base,
                            k.chunk_fx, at<ll>(work, L.acc_fx), at<ll>(work, L.pend_fx), at<float4>(work, L.cell4),

@@ -4,7 +4,8 @@
 // an NE-wide grid -- and at every step adds to the cell in `planes` separate
 // int64 planes (the applying pass adds to four: height change, discharge,
 // two momentum components).  Pattern 1 replaces the walk by uniformly random
-// cells (no locality at all).  Not part of the erosion path.
+// cells (no locality at all); pattern h >= 2 by a set of h hot cells (many
+// lanes on one address, as on a river).  Not part of the erosion path.
 #include "pg_erode.h"
 #include "pg_internal.h"
 
@@ -30,6 +31,8 @@ __global__ void atomic_probe_kernel(ull* t, int64_t n, int64_t NE, int planes, i
         st = mix64(st + 0x9e3779b97f4a7c15ull);
         if (pattern == 1) {
             pos = (int64_t)(st % (ull)n);
+        } else if (pattern >= 2) {
+            pos = (int64_t)(st % (ull)pattern);  // `pattern` hot cells: same-address traffic
         } else {
             const int k = (int)(st & 7);
             const int64_t di = (k <= 1 || k == 7) ? 1 : ((k >= 3 && k <= 5) ? -1 : 0);
@@ -51,7 +54,7 @@ extern "C" int pg_atomic_probe(int64_t* targets, int64_t n, int64_t NE, int32_t 
     clear_error();
     PG_REQUIRE(targets && NE >= 3 && n > 4 * NE && planes >= 1 && planes <= 8 && lanes > 0 && steps > 0,
                "pg_atomic_probe: bad arguments");
-    PG_REQUIRE(pattern == 0 || pattern == 1, "pg_atomic_probe: pattern must be 0 (walk) or 1 (random)");
+    PG_REQUIRE(pattern >= 0 && pattern < n, "pg_atomic_probe: pattern must be 0 (walk), 1 (random) or a hot-set size < n");
     const int64_t blocks = (lanes + 255) / 256;
     PG_REQUIRE(blocks <= 0x7fffffff, "pg_atomic_probe: too many lanes");
     hipLaunchKernelGGL(atomic_probe_kernel, dim3((unsigned)blocks), dim3(256), 0, (hipStream_t)stream,

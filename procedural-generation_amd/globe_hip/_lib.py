@@ -42,7 +42,7 @@ class FastParams(ctypes.Structure):
                 ("iter_erode", dbl), ("conc_chunk", dbl), ("fan_slope", dbl),
                 ("offshore_writeoff", dbl), ("ema", dbl), ("min_volume_frac", dbl), ("height_bits", i64),
                 ("track_bits", i64), ("grid_blocks", i64), ("time_trace", i64), ("rng_dev", vp), ("variant", i64),
-                ("erode_cap", vp)]
+                ("erode_cap", vp), ("tape", vp), ("tape_bytes", i64)]
 
 
 class GradArgs(ctypes.Structure):
@@ -55,7 +55,7 @@ class GradArgs(ctypes.Structure):
 #: statistics slots of a fast iteration (include/pg_erode.h, PG_FS_*)
 FS = dict(PARTICLES=0, RAIN=1, ENTRIES=2, MOVES=3, DEATHS=4, CLAMPED=10, TO_PENDING_FX=11, LOST_OFFSHORE_FX=12,
           EXIT_FX=13, VANISHED_FX=14, RELEASED_FX=15, VOLUME0_BITS=16, ELIGIBLE=17, CDF_SERIAL=18, ERODED_FX=19,
-          DEPOSITED_FX=20, DEFICIT_FX=21)
+          DEPOSITED_FX=20, DEFICIT_FX=21, TAPE_SHORT=23)
 FAST_NSTATS = 24
 
 
@@ -122,6 +122,7 @@ SIGNATURES = [
     ("pg_iso_accumulate", i32, [i64, vp, vp, vp, vp, vp]),
     ("pg_scale", i32, [i64, vp, dbl, vp, vp]),
     ("pg_divide", i32, [i64, vp, dbl, vp, vp]),
+    ("pg_fast_tape_bytes", i64, [i64, i64, i64, i64, i64, i64, i64]),
     ("pg_flood_init", i32, [i64, i64, i64, i64, vp, vp, vp, vp, vp, vp, vp, vp]),
     ("pg_flood_sweeps", i32, [i64, i64, vp, vp, vp, dbl, vp, vp, vp, i64, ctypes.c_int32, vp, vp]),
     ("pg_flood_finish", i32, [i64, vp, vp, vp, vp]),

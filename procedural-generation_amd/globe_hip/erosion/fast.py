@@ -61,6 +61,21 @@ class FastContext:
     def valid_for(self, state, n_particles: int) -> bool:
         return n_particles == self.n_particles and self._pointers(state) == self._ptrs
 
+    def tape_for(self, state, chunks: int, max_steps: int, grid_blocks: int, variant: int):
+        """The move tape for this chunking and trace grid (allocated on first
+        use, grown when a larger one is needed), or None when
+        ``state.fast_tape`` is False."""
+        if not bool(getattr(state, "fast_tape", True)):
+            return None
+        need = int(lib().pg_fast_tape_bytes(self.n_particles, state.F, state.N, int(chunks), int(max_steps),
+                                            int(grid_blocks), int(variant)))
+        if need < 0:
+            check(-1, "pg_fast_tape_bytes")
+        if getattr(self, "tape", None) is None or self.tape.numel() < need:
+            self.tape = None
+            self.tape = torch.empty(need, dtype=torch.uint8, device=state.device)
+        return self.tape
+
 
 def _fast_context(state, n_particles: int) -> FastContext:
     ctx = state._fast
@@ -91,7 +106,7 @@ def decode_stats(raw: np.ndarray, hscale: float) -> dict:
             "to_pending": fx("TO_PENDING_FX"), "released": fx("RELEASED_FX"), "deficit_out": fx("EXIT_FX"),
             "lost_offshore": fx("LOST_OFFSHORE_FX"), "vanished": fx("VANISHED_FX"), "eroded": fx("ERODED_FX"),
             "deposited": fx("DEPOSITED_FX"), "deficit_cancelled": fx("DEFICIT_FX"), "eligible": int(raw[FS["ELIGIBLE"]]),
-            "cdf_serial": bool(raw[FS["CDF_SERIAL"]]), "raw_fx": raw.copy()}
+            "cdf_serial": bool(raw[FS["CDF_SERIAL"]]), "tape_short": int(raw[FS["TAPE_SHORT"]]), "raw_fx": raw.copy()}
 
 
 def run_iteration(state, params, iteration_key, n_particles=None, particles_per_cell=None, rng_stage="erosion",
@@ -123,9 +138,12 @@ def run_iteration(state, params, iteration_key, n_particles=None, particles_per_
     disc_sat, mom_scale = discharge_scales(ep, state, 1.0, n_particles, n_rain=1)
     prm = trace_params(ep, state, 1.0, 1, 1, disc_sat, mom_scale, max_steps, 0.0)
     ecap = slope_limit_cap(state, ep)
+    n_chunks = int(chunks if chunks is not None else getattr(state, "fast_chunks", DEFAULT_CHUNKS))
+    tape = ctx.tape_for(state, n_chunks, max_steps, int(getattr(state, "fast_grid_blocks", 0)),
+                        int(getattr(state, "fast_variant", 0)))
     slo, shi, ilo, ihi = pcg64_state(rng)
     fp = FastParams(rng_state_lo=slo, rng_state_hi=shi, rng_inc_lo=ilo, rng_inc_hi=ihi, n_particles=int(n_particles),
-                    chunks=int(chunks if chunks is not None else getattr(state, "fast_chunks", DEFAULT_CHUNKS)),
+                    chunks=n_chunks,
                     iter_deposit=slope_limited_deposit(ep, unit), iter_erode=cell_units(ep, "iter_erode", unit),
                     conc_chunk=float(getattr(state, "fast_conc_chunk", -1)) if getattr(state, "fast_conc_chunk", -1) >= 0
                     else float(chunk_size(ep, n_particles)), fan_slope=float(ep.fan_slope),
@@ -135,7 +153,8 @@ def run_iteration(state, params, iteration_key, n_particles=None, particles_per_
                     time_trace=int(bool(getattr(state, "fast_time_trace", False))),
                     variant=int(getattr(state, "fast_variant", 0)),
                     rng_dev=(state._fast_rng_dev.data_ptr() if getattr(state, "_fast_rng_dev", None) is not None else None),
-                    erode_cap=None if ecap is None else ecap.data_ptr())
+                    erode_cap=None if ecap is None else ecap.data_ptr(),
+                    tape=None if tape is None else tape.data_ptr(), tape_bytes=0 if tape is None else tape.numel())
     check(lib().pg_fast_iterate(ctypes.byref(ctx.fields), ctypes.byref(prm), ctypes.byref(fp),
                                 ctypes.c_void_p(ctx.work.data_ptr()), ctx.work.numel(),
                                 ctypes.c_void_p(ctx.stats.data_ptr()), pk._stream()), "pg_fast_iterate")
