@@ -521,7 +521,7 @@ __global__ void __launch_bounds__(TRACE_TPB) __attribute__((amdgpu_waves_per_eu(
     while (true) {
         // this iteration's tape record
         int r_flags = 0, r_cell = 0, r_dl = 0;
-        float r_x = 0.0f, r_aux = 0.0f, r_vol = 0.0f, r_sa = 0.0f, r_sb = 0.0f;
+        float r_x = 0.0f, r_aux = 0.0f;
         // a wave whose tape rows could not hold one more droplet takes none
         if (taping && !exhausted && t_row + k.max_steps + 2 > k.tape_rows) exhausted = true;
         // refill the lanes whose droplet died from the chunk's queue
@@ -709,6 +709,13 @@ __global__ void __launch_bounds__(TRACE_TPB) __attribute__((amdgpu_waves_per_eu(
                                         atomicAdd((ull*)&k.ero_fx[cur.cell], (ull)dq);
                                         atomicMin(&k.floor_key[cur.cell], ((ull)fkey(nrec.x) << 32) | (ull)(unsigned)ncell);
                                     }
+                                    if (taping) {
+                                        // the tracks do not depend on the erosion scale: added here, where the
+                                        // trace leaves the atomic units idle, not in the replay
+                                        atomicAdd((ull*)&k.dis_fx[cur.cell], (ull)llrint((double)vol * k.tscale));
+                                        atomicAdd((ull*)&k.mom_fx[cur.cell], (ull)llrint((double)(vol * sa) * k.tscale));
+                                        atomicAdd((ull*)&k.mom_fx_b[cur.cell], (ull)llrint((double)(vol * sb) * k.tscale));
+                                    }
                                 } else {
                                     deficit += cut;
                                     if (dq > 0) {
@@ -733,9 +740,6 @@ __global__ void __launch_bounds__(TRACE_TPB) __attribute__((amdgpu_waves_per_eu(
                                 r_x = rx;
                                 r_aux = (rk == REC_LAND && rx > 0.0f) ? rroom
                                         : (rk != REC_NONE ? ceiling_lim(h_here, b_here) : 0.0f);
-                                r_vol = vol;
-                                r_sa = sa;
-                                r_sb = sb;
                             }
                             // evaporation, advance, stop tests
                             const float ev = fmaxf(1.0f - k.dt_evap * g1.z, 0.01f);
@@ -818,7 +822,7 @@ __global__ void __launch_bounds__(TRACE_TPB) __attribute__((amdgpu_waves_per_eu(
             // one row per loop iteration: the wave's 64 records side by side
             float4* row = wtape + 2 * (t_row * 64 + lane);
             row[0] = make_float4(__int_as_float(r_flags), __int_as_float(r_cell), r_x, r_aux);
-            row[1] = make_float4(r_vol, r_sa, r_sb, __int_as_float(r_dl));
+            if (r_flags & RF_START) row[1] = make_float4(0.0f, 0.0f, 0.0f, __int_as_float(r_dl));
         }
         t_row += 1;
     }
@@ -857,7 +861,8 @@ void launch_trace(int64_t variant, int grid, hipStream_t s, const TraceCfg& k) {
 // replays its own rows -- the same lanes, droplets and moves in the same
 // order, each row one coalesced read -- with the chunk's per-cell erosion
 // scales: the fixed-point arithmetic of the traced applying pass
-// (bit-identical results) without tracing again.
+// (bit-identical results) without tracing again.  The discharge and
+// momentum tracks, which the scales do not touch, were added by the census.
 // ---------------------------------------------------------------------------
 __global__ void __launch_bounds__(TRACE_TPB) replay_kernel(TraceCfg k) {
     const int lane = threadIdx.x & 63;
@@ -896,9 +901,8 @@ __global__ void __launch_bounds__(TRACE_TPB) replay_kernel(TraceCfg k) {
             const float4 A = row[0];
             const int fl = __float_as_int(A.x);
             if (!fl) continue;
-            const float4 B = row[1];
             if (fl & RF_START) {
-                dl = __float_as_int(B.w);
+                dl = __float_as_int(row[1].w);
                 real = k.d_load[d_begin + dl];
                 deficit = surplus = 0;
             }
@@ -932,11 +936,6 @@ __global__ void __launch_bounds__(TRACE_TPB) replay_kernel(TraceCfg k) {
                     n_entries += 1;
                 }
                 if (change) atomicAdd((ull*)&k.chunk_fx[cell], (ull)change);
-                if (kind == REC_LAND) {
-                    atomicAdd((ull*)&k.dis_fx[cell], (ull)llrint((double)B.x * k.tscale));
-                    atomicAdd((ull*)&k.mom_fx[cell], (ull)llrint((double)(B.x * B.y) * k.tscale));
-                    atomicAdd((ull*)&k.mom_fx_b[cell], (ull)llrint((double)(B.x * B.z) * k.tscale));
-                }
                 n_moves += 1;
             }
             if (fl & RF_END) {
