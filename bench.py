@@ -34,23 +34,26 @@ for _p in (ROOT, os.path.join(ROOT, "procedural-generation_amd")):
 import numpy as np  # noqa: E402
 
 METRIC = "droplet-steps/s at N_c=1024 vs numba host; fwd+adjoint/fwd ratio"
-#: algorithmic HBM bytes per droplet-step and pass (FP32 fast mode, plain profile): the census pass
-#: (trace_kernel) reads the 2x2 stencil x 4 channels (64), 11 cell scalars (44), next stencil + base + mask
-#: (24), writes the erosion census and floor key (16) and the move's 32-byte tape record; the applying pass
-#: (replay_kernel) reads the record (32) and the cell's erosion scale (4) and updates four int64
-#: fixed-point accumulators (32).
-BYTES_PER_STEP_CENSUS = 180.0
-BYTES_PER_STEP = 68.0
+#: algorithmic HBM bytes per droplet-step (FP32 fast mode, plain profile, layout-independent, each value
+#: counted once per step): the current 2x2 stencil x 4 channels (64), 11 cell scalars (44), the next stencil
+#: + base + mask (24) and four int64 fixed-point writes -- height change, discharge, two momentum tracks (32)
+B_STEP = 164.0
+#: what this implementation's two launches per chunk move per step on top of that: the census pass
+#: (trace_kernel) reads the 132 B, adds the erosion request and floor key (16) and the three tracks (24) and
+#: writes the move's 16-B tape row; the applying pass (replay_kernel) reads the row (16) and the cell's
+#: erosion scale (4) and adds the height change (8)
+BYTES_PER_STEP_CENSUS = 188.0
+BYTES_PER_STEP_APPLY = 28.0
 HBM_PEAK_GBS = 8000.0
-#: HBM bytes per trace launch of this workload (N_c=1024, 16 chunks) by register-budget variant, from
-#: rocprofv3 FETCH_SIZE (x 2.000, calibrated on a 1-GiB copy) + WRITE_SIZE (x 1.000), mean over both
-#: passes; profiles/trace_kernel_earth_n1024_counters.md holds the runs
-TRACE_TRAFFIC_BYTES = {0: 2.850e9, 1: 7.733e9, 2: 18.006e9}
-#: relaxed 64-bit atomicAdd / atomicMin per droplet move of this workload (both passes: the census's erosion
-#: request and floor key on eroding steps, the applying pass's height change and three tracks), from
-#: rocprofv3 TCC_ATOMIC_sum (which counts one per lane-atomic: 268.43 M for the probe's 268.44 M adds);
-#: every one is performed past L2 (TCC_EA0_ATOMIC_sum equal), profiles/trace_kernel_earth_n1024_atomics.md
-ATOMICS_PER_MOVE = 3.25
+#: HBM bytes per chunk (census launch, applying launch) of this workload (N_c=1024, 16 chunks) by
+#: register-budget variant, from rocprofv3 FETCH_SIZE (x 2.000, calibrated on a 1-GiB copy) + WRITE_SIZE
+#: (x 1.000); profiles/trace_kernel_earth_n1024_counters.md holds the runs
+TRAFFIC_PER_CHUNK = {0: (7.113e9, 1.861e9), 1: (9.628e9, 2.073e9), 2: (15.216e9, 2.218e9)}
+#: relaxed 64-bit atomicAdd / atomicMin per droplet move of this workload by pass (census: the erosion
+#: request and floor key on eroding steps and the three tracks on land; applying pass: the height change),
+#: from rocprofv3 TCC_ATOMIC_sum (one per lane-atomic: 268.43 M for the probe's 268.44 M adds); every one is
+#: performed past L2 (TCC_EA0_ATOMIC_sum equal); profiles/trace_kernel_earth_n1024_atomics.md
+ATOMICS_PER_MOVE = {"census": 3.00, "applying": 0.32}
 
 
 def log(msg):
@@ -103,15 +106,18 @@ def gpu_run(grid, f, p, steps, warmup, chunks):
         stats.append(st._fast.stats.clone())
     torch.cuda.synchronize()
     dt = time.perf_counter() - t0
-    trace_ms, launches = fast.trace_time()
+    (cen_ms, cen_n), (app_ms, app_n) = fast.trace_time_passes()
+    trace_ms, launches = cen_ms + app_ms, cen_n + app_n
     raw = [s.cpu().numpy() for s in stats]
     dec = [fast.decode_stats(r, 2.0 ** fast.HEIGHT_BITS) for r in raw]
     moves = sum(d["moves"] for d in dec)
     particles = sum(d["particles"] for d in dec)
     entries = sum(d["entries"] for d in dec)
     log(f"{steps} iterations in {dt:.3f}s: {moves} moves, {particles} droplets, steps_mean {entries / max(particles, 1):.1f}; "
-        f"trace {trace_ms:.1f} ms over {launches} launches")
+        f"trace {trace_ms:.1f} ms over {launches} launches (census {cen_ms:.1f} ms / {cen_n}, applying {app_ms:.1f} ms "
+        f"/ {app_n})")
     return dict(seconds=dt, moves=moves, particles=particles, entries=entries, trace_ms=trace_ms, launches=launches,
+                census_ms=cen_ms, census_launches=cen_n, apply_ms=app_ms, apply_launches=app_n,
                 last=dec[-1], grid_blocks=st.fast_grid_blocks, variant=st.fast_variant,
                 tune_ms={f"{g}/{v}": ms for (g, v), ms in tuner.times.items()})
 
@@ -416,7 +422,9 @@ def main():
     r = gpu_run(grid, f, p, a.steps, a.warmup, a.chunks)
     value = r["moves"] / r["seconds"]
     trace_s = r["trace_ms"] / 1000.0
-    achieved = (BYTES_PER_STEP + BYTES_PER_STEP_CENSUS) * r["moves"] / trace_s / 1e9 if trace_s > 0 else 0.0
+    achieved = B_STEP * r["moves"] / trace_s / 1e9 if trace_s > 0 else 0.0
+    known = (a.N, a.chunks) == (1024, 16)
+    traffic = TRAFFIC_PER_CHUNK.get(r["variant"]) if known else None
     out = {
         "metric": METRIC,
         "value": value,
@@ -439,18 +447,33 @@ def main():
                    "autotune_trace_ns_per_move": r["tune_ms"]},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": TRACE_TRAFFIC_BYTES.get(r["variant"]) if (a.N, a.chunks) == (1024, 16) else None,
-                     "kernel": "trace_kernel<false> (census) + replay_kernel (applying pass)",
-                     "bytes_per_step": {"applying_pass": BYTES_PER_STEP, "census_pass": BYTES_PER_STEP_CENSUS},
-                     "launch_ms_mean": r["trace_ms"] / max(r["launches"], 1), "launches": r["launches"]},
+                     "traffic": sum(traffic) if traffic else None,
+                     "kernel": "one chunk's particle pass: trace_kernel<false, W, false> (census) + replay_kernel "
+                               "(applying pass)",
+                     "bytes_per_step": B_STEP,
+                     "launch_ms_mean": r["trace_ms"] / max(r["census_launches"], 1),
+                     "launches": r["census_launches"],
+                     "passes": {
+                         "census": {"kernel": f"trace_kernel<false, {2 + r['variant']}, false>",
+                                    "ms_mean": r["census_ms"] / max(r["census_launches"], 1),
+                                    "launches": r["census_launches"], "bytes_per_step": BYTES_PER_STEP_CENSUS,
+                                    "traffic": traffic[0] if traffic else None},
+                         "applying": {"kernel": "replay_kernel",
+                                      "ms_mean": r["apply_ms"] / max(r["apply_launches"], 1),
+                                      "launches": r["apply_launches"], "bytes_per_step": BYTES_PER_STEP_APPLY,
+                                      "traffic": traffic[1] if traffic else None}}},
     }
     probe = atomic_probe()
     peak_atomics = probe["walk_4_planes"]["gatomics_per_s"]
-    ach_atomics = ATOMICS_PER_MOVE * r["moves"] / trace_s / 1e9 if trace_s > 0 else 0.0
+    per_move = sum(ATOMICS_PER_MOVE.values())
+    ach_atomics = per_move * r["moves"] / trace_s / 1e9 if trace_s > 0 else 0.0
     out["roofline"]["atomics"] = {"achieved": ach_atomics, "peak": peak_atomics, "unit": "G atomics/s",
                                   "frac": ach_atomics / peak_atomics, "per_move": ATOMICS_PER_MOVE,
                                   "peak_source": "measured: pg_atomic_probe, droplet-like walk over the same "
                                                  "footprint, 4 int64 planes", "probe": probe}
+    for name, ms in (("census", r["census_ms"]), ("applying", r["apply_ms"])):
+        g = ATOMICS_PER_MOVE[name] * r["moves"] / (ms / 1000.0) / 1e9 if ms > 0 else 0.0
+        out["roofline"]["passes"][name]["atomics"] = {"achieved": g, "frac": g / peak_atomics}
     if not a.no_adjoint:
         import torch
 

@@ -350,6 +350,11 @@ int pg_fast_layout(int64_t F, int64_t NE, int64_t n_particles, int64_t* offsets,
  * launch stream) and count.  Library-wide state: not thread-safe. */
 int pg_fast_trace_time(double* ms_total, int64_t* launches);
 
+/* As pg_fast_trace_time, split by pass: ms[0] / launches[0] the census
+ * launches (trace pass 0), ms[1] / launches[1] the applying pass (the
+ * replay, or the traced applying pass without a tape).  Same reset. */
+int pg_fast_trace_time_passes(double* ms, int64_t* launches);
+
 /* ---- batched windows (no counterpart in the reference, which erodes its
  * zoom tiles one process each: bake/globe/zoom/bake.py:777-923) ----------
  * n_win single-face windows (F = 1, not spherical, same H and

@@ -1184,11 +1184,12 @@ __global__ void batch_plan_kernel(int W, int chunks, const WinStatic* ws, const 
 // trace-launch timing (pg_fast_trace_time): event pairs reused across calls
 struct TraceTimer {
     hipEvent_t start[4096], stop[4096];
+    int pass[4096];
     int created = 0, recorded = 0;
 };
 TraceTimer g_timer;
 
-int timer_record(hipStream_t s, bool is_start) {
+int timer_record(hipStream_t s, bool is_start, int pass) {
     TraceTimer& t = g_timer;
     const int i = t.recorded;
     PG_REQUIRE(i < 4096, "trace timing: more than 4096 launches recorded without pg_fast_trace_time");
@@ -1199,6 +1200,7 @@ int timer_record(hipStream_t s, bool is_start) {
     }
     if (is_start) {
         PG_HIP(hipEventRecord(t.start[i], s));
+        t.pass[i] = pass;
     } else {
         PG_HIP(hipEventRecord(t.stop[i], s));
         t.recorded = i + 1;
@@ -1544,9 +1546,9 @@ int fast_particles(const pg_fast_fields* fl, const pg_trace_params* prm, const p
         k.chunk = c;
         // pass 0: erosion census on the frozen terrain (and, with a tape, every move recorded)
         k.pass = 0;
-        if (fp->time_trace && timer_record(s, true)) return -1;
+        if (fp->time_trace && timer_record(s, true, k.pass)) return -1;
         launch_trace<false, false>(fp->variant, grid, s, k);
-        if (fp->time_trace && timer_record(s, false)) return -1;
+        if (fp->time_trace && timer_record(s, false, k.pass)) return -1;
         hipLaunchKernelGGL(erosion_scale_kernel, dim3(grid_for(n)), dim3(TPB), 0, s, n, fl->height, fl->sediment, fl->base,
                            k.ero_fx, k.floor_key, at<ll>(work, L.acc_fx), at<float>(work, L.r1), fp->iter_erode,
                            fp->erode_cap, hscale);
@@ -1555,10 +1557,10 @@ int fast_particles(const pg_fast_fields* fl, const pg_trace_params* prm, const p
                            at<float>(work, L.rscale), fp->iter_erode, fp->erode_cap, hscale);
         // pass 1: the same trajectories, erosion scaled per cell, deficits cancel deposits
         k.pass = 1;
-        if (fp->time_trace && timer_record(s, true)) return -1;
+        if (fp->time_trace && timer_record(s, true, k.pass)) return -1;
         if (k.tape) hipLaunchKernelGGL(replay_kernel, dim3(rgrid), dim3(TRACE_TPB), 0, s, k);
         else launch_trace<false, true>(fp->variant, grid, s, k);
-        if (fp->time_trace && timer_record(s, false)) return -1;
+        if (fp->time_trace && timer_record(s, false, k.pass)) return -1;
         hipLaunchKernelGGL(resolve_kernel, dim3(grid_for(n)), dim3(TPB), 0, s, n, fl->height, fl->sediment, fl->base,
                            k.chunk_fx, at<ll>(work, L.acc_fx), pend_fx, cell4, dyn, fp->iter_deposit, prm->dep_floor,
                            fp->offshore_writeoff, hscale, st, k.ero_fx, k.floor_key);
@@ -1602,19 +1604,30 @@ extern "C" int pg_fast_iterate(const pg_fast_fields* fl, const pg_trace_params* 
     return fast_particles(fl, prm, fp, work, c);
 }
 
-extern "C" int pg_fast_trace_time(double* ms_total, int64_t* launches) {
+extern "C" int pg_fast_trace_time_passes(double* ms, int64_t* launches) {
     clear_error();
-    PG_REQUIRE(ms_total && launches, "pg_fast_trace_time: NULL argument");
-    double tot = 0.0;
+    PG_REQUIRE(ms && launches, "pg_fast_trace_time_passes: NULL argument");
+    ms[0] = ms[1] = 0.0;
+    launches[0] = launches[1] = 0;
     for (int i = 0; i < g_timer.recorded; i++) {
         PG_HIP(hipEventSynchronize(g_timer.stop[i]));
-        float ms = 0.0f;
-        PG_HIP(hipEventElapsedTime(&ms, g_timer.start[i], g_timer.stop[i]));
-        tot += ms;
+        float t = 0.0f;
+        PG_HIP(hipEventElapsedTime(&t, g_timer.start[i], g_timer.stop[i]));
+        const int p = g_timer.pass[i] ? 1 : 0;
+        ms[p] += t;
+        launches[p] += 1;
     }
-    *ms_total = tot;
-    *launches = g_timer.recorded;
     g_timer.recorded = 0;
+    return 0;
+}
+
+extern "C" int pg_fast_trace_time(double* ms_total, int64_t* launches) {
+    PG_REQUIRE(ms_total && launches, "pg_fast_trace_time: NULL argument");
+    double ms[2];
+    int64_t n[2];
+    if (pg_fast_trace_time_passes(ms, n)) return -1;
+    *ms_total = ms[0] + ms[1];
+    *launches = n[0] + n[1];
     return 0;
 }
 
@@ -1947,9 +1960,9 @@ extern "C" int pg_batch_iterate(const pg_fast_fields* wins, int64_t n_win, const
     for (int c = 0; c < chunks; c++) {
         k.chunk = c;
         k.pass = 0;
-        if (f0.time_trace && timer_record(s, true)) return -1;
+        if (f0.time_trace && timer_record(s, true, k.pass)) return -1;
         launch_trace<true, false>(f0.variant, grid, s, k);
-        if (f0.time_trace && timer_record(s, false)) return -1;
+        if (f0.time_trace && timer_record(s, false, k.pass)) return -1;
         hipLaunchKernelGGL(erosion_scale_kernel, dim3(grid_for(n)), dim3(TPB), 0, s, n, a.height, a.sediment, a.base,
                            k.ero_fx, k.floor_key, at<ll>(work, L.acc_fx), at<float>(work, L.r1), f0.iter_erode,
                            (const double*)nullptr, hscale);
@@ -1957,9 +1970,9 @@ extern "C" int pg_batch_iterate(const pg_fast_fields* wins, int64_t n_win, const
                            k.ero_fx, k.floor_key, at<ll>(work, L.acc_fx), at<float>(work, L.r1),
                            at<float>(work, L.rscale), f0.iter_erode, (const double*)nullptr, hscale);
         k.pass = 1;
-        if (f0.time_trace && timer_record(s, true)) return -1;
+        if (f0.time_trace && timer_record(s, true, k.pass)) return -1;
         launch_trace<true, true>(f0.variant, grid, s, k);
-        if (f0.time_trace && timer_record(s, false)) return -1;
+        if (f0.time_trace && timer_record(s, false, k.pass)) return -1;
         hipLaunchKernelGGL(resolve_kernel, dim3(grid_for(n)), dim3(TPB), 0, s, n, a.height, a.sediment, a.base,
                            k.chunk_fx, at<ll>(work, L.acc_fx), at<ll>(work, L.pend_fx), at<float4>(work, L.cell4),
                            at<float2>(work, L.dyn), f0.iter_deposit, prm->dep_floor, f0.offshore_writeoff, hscale,

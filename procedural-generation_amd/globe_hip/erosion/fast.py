@@ -27,6 +27,9 @@ from . import particle as pk
 DEFAULT_CHUNKS = 16
 HEIGHT_BITS = 48
 TRACK_BITS = 32
+#: largest move tape allocated by default (a planet at N_c=8192 would need more: it traces twice instead,
+#: or uses more chunks)
+TAPE_MAX_BYTES = 24 << 30
 
 
 class FastContext:
@@ -71,6 +74,8 @@ class FastContext:
                                             int(grid_blocks), int(variant)))
         if need < 0:
             check(-1, "pg_fast_tape_bytes")
+        if need > int(getattr(state, "fast_tape_max_bytes", TAPE_MAX_BYTES)):
+            return None  # too large for this chunking: the applying pass traces again (same results)
         if getattr(self, "tape", None) is None or self.tape.numel() < need:
             self.tape = None
             self.tape = torch.empty(need, dtype=torch.uint8, device=state.device)
@@ -181,6 +186,15 @@ def trace_time():
     n = ctypes.c_int64()
     check(lib().pg_fast_trace_time(ctypes.byref(ms), ctypes.byref(n)), "pg_fast_trace_time")
     return ms.value, n.value
+
+
+def trace_time_passes():
+    """((census ms, launches), (applying-pass ms, launches)) of the trace
+    launches recorded since the last call, split by pass; waits for them."""
+    ms = (ctypes.c_double * 2)()
+    n = (ctypes.c_int64 * 2)()
+    check(lib().pg_fast_trace_time_passes(ms, n), "pg_fast_trace_time_passes")
+    return (ms[0], n[0]), (ms[1], n[1])
 
 
 def _signed(w: int) -> int:
