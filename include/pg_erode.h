@@ -63,6 +63,10 @@ int pg_geom_to_sphere(int face, double u, double v, double* xyz3);
 int pg_geom_from_sphere(double x, double y, double z, int* face, double* uv2);
 int pg_geom_jacobian(int face, double u, double v, double* j6);
 int pg_geom_transfer_velocity(int face, double u, double v, double a, double b, int* face_out, double* uvab4);
+/* The fast mode's face transfer: transfer_velocity in single precision
+ * (float tan / atan; no counterpart in the reference, whose droplets are
+ * float64), on the host. */
+int pg_geom_transfer_velocity_f32(int face, float u, float v, float a, float b, int* face_out, float* uvab4);
 double pg_geom_tan(double x);
 double pg_geom_atan(double x);
 void pg_geom_sincos(double x, double* s, double* c);
@@ -70,7 +74,8 @@ void pg_geom_sincos(double x, double* s, double* c);
  * op 0 tan (in n, out n), 1 atan (in n, out n), 2 to_sphere (in n x (face,
  * u, v), out n x 3), 3 from_sphere (in n x 3, out n x (u, v), iout face),
  * 4 transfer_velocity (in n x (face, u, v, a, b), out n x (u, v, a, b),
- * iout face), 5 sin/cos (in n, out n x (sin, cos)). */
+ * iout face), 5 sin/cos (in n, out n x (sin, cos)), 6 the fast mode's
+ * single-precision transfer (as 4, computed in float32). */
 int pg_geom_eval(int op, int64_t n, const double* in, double* out, int* iout, pg_stream stream);
 
 /* ---- packed samples (particle.py:164-191, pack_samples) ---------------- */

@@ -375,9 +375,11 @@ def _check_plain(ep, state):
 # one iteration (maps.py:839-1047)
 # ---------------------------------------------------------------------------
 def run_iteration(state, params, iteration_key, n_particles=None, particles_per_cell=None, rng_stage="erosion",
-                  diag=None, nthreads=0, spawn=None):
+                  diag=None, nthreads=0, spawn=None, diag_lists=None):
     """``spawn``: optionally a precomputed ``(face, x, y, volume0)`` to use in
-    place of the rain spawn (the same value ``spawn_particles`` returns)."""
+    place of the rain spawn (the same value ``spawn_particles`` returns).
+    ``diag_lists(cl_cell, cl_delta, cl_vol, cl_mom, cl_count, cap)``: a test
+    hook that sees each chunk's whole change lists before they are applied."""
     ep = _eparams(params)
     _check_plain(ep, state)
     key = iteration_key if isinstance(iteration_key, (tuple, list)) else (int(iteration_key),)
@@ -481,6 +483,8 @@ def run_iteration(state, params, iteration_key, n_particles=None, particles_per_
         t2 = time.time()
         if diag is not None:
             diag(state, cl_cell, cl_vol, cl_count[:m], cap, sp_death[:m])
+        if diag_lists is not None:
+            diag_lists(cl_cell, cl_delta, cl_vol, cl_mom, cl_count[:m], cap)
         L.or_apply_changes(_p(cl_cell), _p(cl_delta), _p(cl_vol), _p(cl_mom), _p(cl_count), m, cap, _p(state.height),
                            _p(state.sediment), _p(state.acc), _p(state.pending), _p(state.samp), _p(state.disch_track),
                            _p(state.mom_track), _p(state.mask), iter_erode, iter_deposit, float(ep.fan_slope),

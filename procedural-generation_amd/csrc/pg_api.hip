@@ -68,6 +68,18 @@ extern "C" int pg_geom_transfer_velocity(int face, double u, double v, double a,
     return 0;
 }
 
+extern "C" int pg_geom_transfer_velocity_f32(int face, float u, float v, float a, float b, int* face_out, float* uvab4) {
+    clear_error();
+    PG_REQUIRE(face >= 0 && face < 6 && face_out && uvab4, "pg_geom_transfer_velocity_f32: bad arguments");
+    const TransferF t = transfer_velocity_f(face, u, v, a, b);
+    *face_out = t.face;
+    uvab4[0] = t.u;
+    uvab4[1] = t.v;
+    uvab4[2] = t.a;
+    uvab4[3] = t.b;
+    return 0;
+}
+
 extern "C" double pg_geom_tan(double x) { return pg_tan(x); }
 extern "C" double pg_geom_atan(double x) { return pg_atan(x); }
 extern "C" void pg_geom_sincos(double x, double* s, double* c) { pg_sincos(x, s, c); }
@@ -101,6 +113,14 @@ __global__ void geom_eval_kernel(int op, int64_t n, const double* in, double* ou
             out[4 * k + 3] = t.b;
         } else if (op == 5) {
             pg_sincos(in[k], &out[2 * k], &out[2 * k + 1]);
+        } else if (op == 6) {
+            const TransferF t = transfer_velocity_f((int)in[5 * k], (float)in[5 * k + 1], (float)in[5 * k + 2],
+                                                   (float)in[5 * k + 3], (float)in[5 * k + 4]);
+            iout[k] = t.face;
+            out[4 * k] = t.u;
+            out[4 * k + 1] = t.v;
+            out[4 * k + 2] = t.a;
+            out[4 * k + 3] = t.b;
         }
     }
 }
@@ -109,9 +129,9 @@ __global__ void geom_eval_kernel(int op, int64_t n, const double* in, double* ou
 
 extern "C" int pg_geom_eval(int op, int64_t n, const double* in, double* out, int* iout, pg_stream stream) {
     clear_error();
-    PG_REQUIRE(op >= 0 && op <= 5 && n >= 0, "pg_geom_eval: bad op %d or n", op);
+    PG_REQUIRE(op >= 0 && op <= 6 && n >= 0, "pg_geom_eval: bad op %d or n", op);
     if (n == 0) return 0;
-    PG_REQUIRE(in && out && (iout || (op != 3 && op != 4)), "pg_geom_eval: NULL array");
+    PG_REQUIRE(in && out && (iout || (op != 3 && op != 4 && op != 6)), "pg_geom_eval: NULL array");
     int blocks = blocks_for(n, 256);
     if (blocks > 4096) blocks = 4096;
     hipLaunchKernelGGL(geom_eval_kernel, dim3(blocks), dim3(256), 0, (hipStream_t)stream, op, n, in, out, iout);

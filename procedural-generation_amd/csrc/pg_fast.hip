@@ -393,7 +393,7 @@ struct TraceCfg {
 
 // what a droplet reads at the cell it stands in
 struct Here {
-    int64_t cell;
+    int cell;  // extended cell id (grids and batch arenas hold < 2^31 cells)
     int m;
     float h, base, sthick, rs;
     float4 g0, g1;
@@ -466,12 +466,12 @@ __global__ void __launch_bounds__(TRACE_TPB) __attribute__((amdgpu_waves_per_eu(
     int step = 0, uphill = 0, sea_steps = 0, pushes = 0, last_cell = -1;
     ll real = 0, deficit = 0;  // load actually held (fixed point) and load cancelled by erosion caps
     ll surplus = 0;            // deposits a ceiling refused, offered to the droplet's next deposits
-    ll dl = 0;                 // the droplet's slot in the chunk
+    int dl = 0;                // the droplet's slot in the chunk (a chunk holds < 2^31 droplets)
     // the wave's tape rows (census with a tape): wave-uniform row counter
     const ll gw = (ll)blockIdx.x * (TRACE_TPB / 64) + (threadIdx.x >> 6);
     float4* wtape = taping ? k.tape + 2 * gw * k.tape_rows * 64 : nullptr;
     ll t_row = 0;
-    Stencil S;
+    Samp S;  // the sample of the stencil at the droplet's position
     Here cur;
     // lane totals
     ll n_entries = 0, n_moves = 0, exit_fx = 0, vanished_fx = 0, lost_fx = 0;
@@ -479,7 +479,7 @@ __global__ void __launch_bounds__(TRACE_TPB) __attribute__((amdgpu_waves_per_eu(
 
     auto load_here = [&](int ff, int cx, int cy, Here& h, const float4* rec) {
         const int ci = clampi(cx, 0, N - 1), cj = clampi(cy, 0, N - 1);
-        h.cell = coff + ((int64_t)ff * NE + ci + k.H) * NE + cj + k.H;
+        h.cell = (int)(coff + ((int64_t)ff * NE + ci + k.H) * NE + cj + k.H);
         h.m = k.mask[h.cell];
         const float2 dy = k.dyn[h.cell];
         h.sthick = dy.x;
@@ -559,7 +559,7 @@ __global__ void __launch_bounds__(TRACE_TPB) __attribute__((amdgpu_waves_per_eu(
                         }
                     }
                     if (u < d_end) {
-                        dl = u - d_begin;
+                        dl = (int)(u - d_begin);
                         if (taping) {
                             r_flags |= RF_START;
                             r_dl = (int)dl;
@@ -583,7 +583,7 @@ __global__ void __launch_bounds__(TRACE_TPB) __attribute__((amdgpu_waves_per_eu(
                         in_sea = false;
                         step = uphill = sea_steps = pushes = 0;
                         last_cell = -1;
-                        S = load_stencil(k.cell4 + coff, NE, k.H, f, ix, iy, fx, fy);
+                        S = sample_of(load_stencil(k.cell4 + coff, NE, k.H, f, ix, iy, fx, fy));
                         load_here(f, ix, iy, cur, nullptr);
                         alive = true;
                     }
@@ -615,11 +615,12 @@ __global__ void __launch_bounds__(TRACE_TPB) __attribute__((amdgpu_waves_per_eu(
                     pushes += 1;
                 }
                 const float4 g0 = cur.g0, g1 = cur.g1;
-                const float q = fmaxf(lerp2(S.c00.y, S.c10.y, S.c01.y, S.c11.y, S.wi, S.wj), 0.0f);
+                const float q = S.q;
                 const float vol_rel = vol * inv_v0;
                 int nf, nix, niy;
-                float nfx, nfy, h0;
-                cause = steer_move(k, N, S, g0, g1, q, vol, vol_rel, sa, sb, f, ix, iy, fx, fy, nf, nix, niy, nfx, nfy, h0);
+                float nfx, nfy;
+                const float h0 = S.h;
+                cause = steer_move(k, N, S, g0, g1, vol, vol_rel, sa, sb, f, ix, iy, fx, fy, nf, nix, niy, nfx, nfy);
                 exited = cause == DEATH_EXIT;
                 {
                     if (cause < 0) {
@@ -630,6 +631,7 @@ __global__ void __launch_bounds__(TRACE_TPB) __attribute__((amdgpu_waves_per_eu(
                         const int nm = k.mask[ncell];
                         const float2 ndy = k.dyn[ncell];
                         const float4 nrec = corner_of(S1, nei, nej);
+                        const Samp P1 = sample_of(S1);
                         Here nxt;
                         const bool same = nix >= 0 && nix < N && niy >= 0 && niy < N;
                         load_here(nf, nix, niy, nxt, same ? &nrec : nullptr);
@@ -637,7 +639,7 @@ __global__ void __launch_bounds__(TRACE_TPB) __attribute__((amdgpu_waves_per_eu(
                             cause = DEATH_EXIT;
                             exited = true;
                         } else {
-                            const float h1 = lerp2(S1.c00.x, S1.c10.x, S1.c01.x, S1.c11.x, S1.wi, S1.wj);
+                            const float h1 = P1.h;
                             const float b1 = ndy.y;
                             const float dh = h0 - fmaxf(h1, b1);
                             int rk = REC_NONE;
@@ -750,7 +752,7 @@ __global__ void __launch_bounds__(TRACE_TPB) __attribute__((amdgpu_waves_per_eu(
                             iy = niy;
                             fx = nfx;
                             fy = nfy;
-                            S = S1;
+                            S = P1;
                             cur = nxt;
                             h_prev = h_here;
                             has_prev = true;

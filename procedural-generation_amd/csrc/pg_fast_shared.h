@@ -123,6 +123,25 @@ __device__ __forceinline__ float4 corner_of(const Stencil& s, int ei, int ej) {
     return ri ? (rj ? s.c11 : s.c10) : (rj ? s.c01 : s.c00);
 }
 
+// what a droplet needs of a stencil: the bilinear surface and its gradient,
+// the clamped discharge and the momentum -- six values, evaluated once when
+// the stencil is read, so a droplet carries 6 registers instead of 16
+struct Samp {
+    float h, gi, gj, q, ma, mb;
+};
+
+__device__ __forceinline__ Samp sample_of(const Stencil& S) {
+    Samp p;
+    const float h00 = S.c00.x, h10 = S.c10.x, h01 = S.c01.x, h11 = S.c11.x, wi = S.wi, wj = S.wj;
+    p.h = lerp2(h00, h10, h01, h11, wi, wj);
+    p.gi = (h10 - h00) * (1.0f - wj) + (h11 - h01) * wj;
+    p.gj = (h01 - h00) * (1.0f - wi) + (h11 - h10) * wi;
+    p.q = fmaxf(lerp2(S.c00.y, S.c10.y, S.c01.y, S.c11.y, wi, wj), 0.0f);
+    p.ma = lerp2(S.c00.z, S.c10.z, S.c01.z, S.c11.z, wi, wj);
+    p.mb = lerp2(S.c00.w, S.c10.w, S.c01.w, S.c11.w, wi, wj);
+    return p;
+}
+
 __device__ __forceinline__ int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
 
 __device__ __forceinline__ ll wave_sum(ll v) {
@@ -137,17 +156,13 @@ __device__ __forceinline__ ll wave_sum(ll v) {
 // k_mom, mom_scale, dt, density, slope_gain, slope_sat, H, spherical; N is
 // the droplet's grid's interior size.
 // Returns -1 (moved), DEATH_STOP (no finite direction) or DEATH_EXIT (left a
-// window).  h0 = the bilinear surface at the current position.
+// window).  P = the stencil's sample at the current position.
 // ---------------------------------------------------------------------------
 template <class K>
-__device__ __forceinline__ int steer_move(const K& k, int N, const Stencil& S, const float4 g0, const float4 g1, float q,
+__device__ __forceinline__ int steer_move(const K& k, int N, const Samp& P, const float4 g0, const float4 g1,
                                           float vol, float vol_rel, float& sa, float& sb, int f, int ix, int iy,
-                                          float fx, float fy, int& nf, int& nix, int& niy, float& nfx, float& nfy,
-                                          float& h0) {
-    const float h00 = S.c00.x, h10 = S.c10.x, h01 = S.c01.x, h11 = S.c11.x, wi = S.wi, wj = S.wj;
-    h0 = lerp2(h00, h10, h01, h11, wi, wj);
-    const float gi = (h10 - h00) * (1.0f - wj) + (h11 - h01) * wj;
-    const float gj = (h01 - h00) * (1.0f - wi) + (h11 - h10) * wi;
+                                          float fx, float fy, int& nf, int& nix, int& niy, float& nfx, float& nfy) {
+    const float gi = P.gi, gj = P.gj, q = P.q;
     const float di = g0.w * gi + g1.x * gj;
     const float dj = g1.x * gi + g1.y * gj;
     const float s2 = fmaxf(gi * di + gj * dj, 0.0f);
@@ -157,8 +172,7 @@ __device__ __forceinline__ int steer_move(const K& k, int N, const Stencil& S, c
     sa *= k.fr;
     sb *= k.fr;
     if (k.k_mom != 0.0f) {
-        const float ma = lerp2(S.c00.z, S.c10.z, S.c01.z, S.c11.z, wi, wj);
-        const float mb = lerp2(S.c00.w, S.c10.w, S.c01.w, S.c11.w, wi, wj);
+        const float ma = P.ma, mb = P.mb;
         const float mlen = sqrtf(fmaxf(g0.x * ma * ma + 2.0f * g0.y * ma * mb + g0.z * mb * mb, 0.0f));
         const float slen = sqrtf(fmaxf(g0.x * sa * sa + 2.0f * g0.y * sa * sb + g0.z * sb * sb, 0.0f));
         if (mlen > 1e-12f && slen > 1e-12f) {
@@ -189,27 +203,38 @@ __device__ __forceinline__ int steer_move(const K& k, int N, const Stencil& S, c
     nf = f;
     if (k.spherical) {
         if (nix < 0 || nix >= N || niy < 0 || niy >= N) {
-            const double Nd = (double)N;
-            const Transfer t = transfer_velocity<LibMath>(f, ((double)nix + nfx) / Nd, ((double)niy + nfy) / Nd,
-                                                          (double)sa, (double)sb);
+            // across a face edge, in single precision (transfer_velocity_f)
+            const float Nf = (float)N;
+            const TransferF t = transfer_velocity_f(f, ((float)nix + nfx) / Nf, ((float)niy + nfy) / Nf, sa, sb);
             nf = t.face;
-            double nx = t.u * Nd, ny = t.v * Nd;
-            if (nx >= Nd) nx = Nd * 0.999999999;
-            if (ny >= Nd) ny = Nd * 0.999999999;
-            if (!(nx >= 0.0)) nx = 0.0;
-            if (!(ny >= 0.0)) ny = 0.0;
-            const double fnx = floor(nx), fny = floor(ny);
+            const float top = __int_as_float(__float_as_int(Nf) - 1);  // the float below N
+            float nx = fminf(t.u * Nf, top), ny = fminf(t.v * Nf, top);
+            if (!(nx >= 0.0f)) nx = 0.0f;
+            if (!(ny >= 0.0f)) ny = 0.0f;
+            const float fnx = floorf(nx), fny = floorf(ny);
             nix = (int)fnx;
             niy = (int)fny;
-            nfx = fminf((float)(nx - fnx), 0.99999994f);
-            nfy = fminf((float)(ny - fny), 0.99999994f);
-            sa = (float)t.a;
-            sb = (float)t.b;
+            nfx = fminf(nx - fnx, 0.99999994f);
+            nfy = fminf(ny - fny, 0.99999994f);
+            sa = t.a;
+            sb = t.b;
         }
     } else if (nix < -k.H || nix >= N + k.H || niy < -k.H || niy >= N + k.H) {
         return DEATH_EXIT;
     }
     return -1;
+}
+
+// the same move from a stencil (h0 = the bilinear surface at the position)
+template <class K>
+__device__ __forceinline__ int steer_move(const K& k, int N, const Stencil& S, const float4 g0, const float4 g1, float q,
+                                          float vol, float vol_rel, float& sa, float& sb, int f, int ix, int iy,
+                                          float fx, float fy, int& nf, int& nix, int& niy, float& nfx, float& nfy,
+                                          float& h0) {
+    Samp P = sample_of(S);
+    P.q = q;
+    h0 = P.h;
+    return steer_move(k, N, P, g0, g1, vol, vol_rel, sa, sb, f, ix, iy, fx, fy, nf, nix, niy, nfx, nfy);
 }
 
 }  // namespace fastmode

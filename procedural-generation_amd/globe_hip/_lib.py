@@ -67,6 +67,8 @@ SIGNATURES = [
     ("pg_geom_from_sphere", i32, [dbl, dbl, dbl, ctypes.POINTER(ctypes.c_int), pd]),
     ("pg_geom_jacobian", i32, [i32, dbl, dbl, pd]),
     ("pg_geom_transfer_velocity", i32, [i32, dbl, dbl, dbl, dbl, ctypes.POINTER(ctypes.c_int), pd]),
+    ("pg_geom_transfer_velocity_f32", i32, [i32, ctypes.c_float, ctypes.c_float, ctypes.c_float, ctypes.c_float,
+                                            ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_float)]),
     ("pg_geom_tan", dbl, [dbl]),
     ("pg_geom_atan", dbl, [dbl]),
     ("pg_geom_sincos", None, [dbl, pd, pd]),

@@ -155,3 +155,27 @@ def test_globe_steps_stay_close(oracle, cuda):
     inter = ost.interior
     scale = np.abs(h_o[inter]).max()
     assert np.abs(h_d[inter] - h_o[inter]).max() <= 1e-6 * scale
+
+
+@pytest.mark.parametrize("inputs,rate", [("plain", 50.0), ("tilt", 5.0)])
+def test_window_lateral_cuts_match_oracle(oracle, cuda, inputs, rate):
+    """Lateral erosion (particle.py:699-740; volume -2 entries applied under
+    the erosion caps, particle.py:934-1018): change lists and state bit for
+    bit over two iterations."""
+    from test_oracle_lateral import _plain_window_inputs
+
+    p = window_params(0, lateral_rate=rate, max_steps=60)
+    kw = _plain_window_inputs(p) if inputs == "plain" else window_inputs(48, "tilt", p, seed=4, deposit_on_exit=True)
+    ost = oracle.OracleState.window(**kw)
+    dst = maps.ErosionState.window(**kw, device=cuda)
+    cuts = []
+    for it in range(2):
+        print(f"lateral {inputs} rate {rate} iteration {it}", flush=True)
+        d_chunks = []
+        oracle.run_iteration(ost, p, it, diag=_capture(cuts, lambda x: x))
+        maps.run_iteration(dst, p, it, diag=_capture(d_chunks, lambda t: t.cpu().numpy()))
+        assert _cmp_chunks(d_chunks, cuts[-len(d_chunks):]) == 0
+        _cmp_state(ost, dst, exact_frac=1.0, rtol=0.0)
+    n_cut = sum(int((v[q * cap: q * cap + int(c[q])] == -2.0).sum())
+                for c, _, v, _, cap, n in cuts for q in range(n))
+    assert n_cut > 100, n_cut
