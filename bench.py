@@ -48,7 +48,7 @@ HBM_PEAK_GBS = 8000.0
 #: HBM bytes per chunk (census launch, applying launch) of this workload (N_c=1024, 16 chunks) by
 #: register-budget variant, from rocprofv3 FETCH_SIZE (x 2.000, calibrated on a 1-GiB copy) + WRITE_SIZE
 #: (x 1.000); profiles/trace_kernel_earth_n1024_counters.md holds the runs
-TRAFFIC_PER_CHUNK = {0: (7.113e9, 1.861e9), 1: (9.628e9, 2.073e9), 2: (15.216e9, 2.218e9)}
+TRAFFIC_PER_CHUNK = {}
 #: relaxed 64-bit atomicAdd / atomicMin per droplet move of this workload by pass (census: the erosion
 #: request and floor key on eroding steps and the three tracks on land; applying pass: the height change),
 #: from rocprofv3 TCC_ATOMIC_sum (one per lane-atomic: 268.43 M for the probe's 268.44 M adds); every one is
@@ -87,7 +87,7 @@ def gpu_run(grid, f, p, steps, warmup, chunks):
     from globe_hip.erosion import batch as bt
 
     cus = torch.cuda.get_device_properties(0).multi_processor_count
-    tuner = bt.Autotuner(st, candidates=[(2 * cus, 0), (4 * cus, 0), (3 * cus, 1), (4 * cus, 2)][:max(warmup, 1)])
+    tuner = bt.Autotuner(st, candidates=[(4 * cus, 0), (5 * cus, 1), (6 * cus, 2), (10 * cus, 1)][:max(warmup, 1)])
     log("warmup")
     for it in range(warmup):
         tuner.step(lambda k: maps.step(st, p, k), it)
@@ -454,7 +454,7 @@ def main():
                      "launch_ms_mean": r["trace_ms"] / max(r["census_launches"], 1),
                      "launches": r["census_launches"],
                      "passes": {
-                         "census": {"kernel": f"trace_kernel<false, {2 + r['variant']}, false>",
+                         "census": {"kernel": f"trace_kernel<false, {4 + r['variant']}, false>",
                                     "ms_mean": r["census_ms"] / max(r["census_launches"], 1),
                                     "launches": r["census_launches"], "bytes_per_step": BYTES_PER_STEP_CENSUS,
                                     "traffic": traffic[0] if traffic else None},

@@ -242,6 +242,25 @@ int pg_flood_sweeps(int64_t F, int64_t NE, const double* s, const uint8_t* cls, 
                     int32_t sweeps, int32_t* flag, pg_stream stream);
 int pg_flood_finish(int64_t n, const double* s, const double* W, double* route, pg_stream stream);
 
+/* ---- the sea mask (bake/globe/erosion/maps.py:360-421 refresh_base, on
+ * hydro/run.py:64-83 below_sea_components) -------------------------------
+ * Replaces ErosionState.refresh_base on the globe (F = 6): base (DEVICE
+ * float64, (6, NE, NE)) becomes 0 on the open ocean and on land and, on the
+ * cells of a closed basin (a component of the cells whose float32 surface
+ * height + sediment is < 0, 8-connected through owner, covering less than
+ * the threshold and not the largest), the basin's lowest float64 surface;
+ * halo cells copy their owner.  area_fx (DEVICE int64, (6, N, N)): each
+ * interior cell's area as an exact fixed-point integer; threshold_fx: the
+ * ocean area threshold (min_fraction x the globe's area) in the same units.
+ * work: DEVICE scratch of pg_sea_work_bytes(N).  Host-synchronous (the
+ * union-find's rounds are checked on the host).  info5 (HOST): closed cells,
+ * closed basins, sea cells, deepest basin floor (cell units; 0 if none),
+ * union-find rounds. */
+int64_t pg_sea_work_bytes(int64_t N);
+int pg_refresh_base(const double* height, const double* sediment, const int64_t* owner, const int64_t* area_fx,
+                    double threshold_fx, int64_t N, int64_t H, double* base, void* work, int64_t work_bytes,
+                    double* info5, pg_stream stream);
+
 /* ---- calibration (no counterpart in the reference) ---------------------- */
 /* The device's relaxed 64-bit atomicAdd rate: `lanes` lanes each take
  * `steps` steps of a droplet-like walk (pattern 0: to one of the 8 neighbours
@@ -284,7 +303,7 @@ typedef struct pg_fast_params {
     int64_t time_trace;    /* record HIP events around every trace launch (see pg_fast_trace_time) */
     const uint64_t* rng_dev; /* optional DEVICE copy of the four rng words, read by the kernels instead of the
                               * values above: a captured graph then replays with each iteration's own draws */
-    int64_t variant;       /* trace kernel register budget: 0 = 2 waves per SIMD, 1 = 3, 2 = 4 (same results) */
+    int64_t variant;       /* trace kernels' register budget: 0 = natural (census 4 waves per SIMD, traced applying pass 3), 1 / 2 = one / two waves more (same results) */
     const double* erode_cap; /* optional DEVICE per-cell erosion cap of the iteration (soillib's slope limit,
                               * pg_slope_erode_cap; bake/globe/erosion/maps.py:923-926), NULL = none */
     void* tape;            /* optional DEVICE move tape of pg_fast_tape_bytes() bytes: the census pass records

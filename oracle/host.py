@@ -93,6 +93,8 @@ def lib():
         L.or_slope_erode_cap.argtypes = [i64, i64, vp, vp, vp, dbl, dbl, vp]
         L.or_laplacian.argtypes = [i64, i64, vp, vp, vp, vp]
         L.or_priority_flood_eps.argtypes = [i64, i64, i64, i64, vp, vp, vp, dbl, vp, vp, vp, vp]
+        L.or_label_components.argtypes = [vp, vp, i64, i64, vp, vp]
+        L.or_label_components.restype = i64
         _lib = L
     return _lib
 
@@ -179,6 +181,7 @@ class OracleState:
     spherical: bool
     height_unit_m: float
     grid: object = None
+    base_at: object = None
     route: np.ndarray | None = None
     lake_flag: np.ndarray | None = None
     disch_track: np.ndarray = None
@@ -536,6 +539,80 @@ def priority_flood_eps(surface, mask, owner, H, N, eps, base):
 
 
 # ---------------------------------------------------------------------------
+# the sea mask (maps.py:360-421, hydro/run.py:64-83)
+# ---------------------------------------------------------------------------
+def label_components(mask, owner, N, H):
+    """hydro/lakes.py:19-46 (level all equal): (labels int32 (M,), n)."""
+    mask = np.ascontiguousarray(np.asarray(mask, bool).reshape(-1), np.uint8)
+    labels = np.empty(mask.size, np.int32)
+    stack = np.empty(mask.size, np.int64)
+    n = lib().or_label_components(_p(mask), _p(np.ascontiguousarray(owner, np.int64)), int(N), int(H), _p(labels),
+                                  _p(stack))
+    return labels, int(n)
+
+
+def interior_cell_area(grid):
+    """cubesphere.py:509-517: cell areas in m^2, float32, over the interior."""
+    H, N = grid.H, grid.N
+    area = (np.asarray(grid.sqrt_det_metric).astype(np.float64) * float(grid.cell_size_m) ** 2).astype(np.float32)
+    return area[:, H:H + N, H:H + N]
+
+
+def below_sea_components(surface, grid, min_fraction=0.02):
+    """hydro/run.py:64-83: (below, labels, keep) flat over the interior."""
+    N, H = grid.N, grid.H
+    below = np.ascontiguousarray(np.asarray(surface) < 0.0).reshape(-1)
+    if not below.any():
+        return below, None, None
+    labels, n = label_components(below, grid.owner, N, H)
+    area = interior_cell_area(grid).reshape(-1).astype(np.float64)
+    per = np.bincount(labels[below], weights=area[below], minlength=max(n, 1))
+    keep = per >= float(min_fraction) * float(area.sum())
+    keep[int(np.argmax(per))] = True
+    return below, labels, keep
+
+
+def ocean_min_fraction(params) -> float:
+    """maps.py:770-774."""
+    hydro = getattr(params, "hydro", None)
+    return float(hydro.ocean_min_fraction) if hydro is not None and hasattr(params, "world") else 0.02
+
+
+def refresh_base(state, min_fraction):
+    """maps.py:360-421: base level 0 on the open ocean and on land, a closed
+    basin's deepest surface inside it; halos copied from their owners."""
+    base = np.zeros_like(state.height)
+    info = {"closed_cells": 0, "closed_basins": 0, "sea_cells": 0, "deepest_basin_floor": 0.0}
+    if not state.spherical or state.grid is None:
+        state.base = base
+        state.base_at = state.iteration
+        return info
+    inter = state.interior
+    surf = (state.height[inter] + state.sediment[inter]).astype(np.float64)
+    below, labels, keep = below_sea_components(surf.astype(np.float32), state.grid, float(min_fraction))
+    if labels is not None:
+        sea = below & keep[np.maximum(labels, 0)]
+        closed = below & ~sea
+        info["sea_cells"] = int(sea.sum())
+        info["closed_cells"] = int(closed.sum())
+        if closed.any():
+            lab = labels[closed]
+            floor = np.zeros(keep.size, dtype=np.float64)
+            np.minimum.at(floor, lab, surf.reshape(-1)[closed])
+            inner = np.zeros(surf.size, dtype=np.float64)
+            inner[closed] = floor[lab]
+            base[inter] = inner.reshape(surf.shape)
+            info["closed_basins"] = int(np.unique(lab).size)
+            info["deepest_basin_floor"] = float(floor.min())
+    own = np.asarray(state.grid.owner).reshape(-1)
+    flat = base.reshape(-1)
+    flat[:] = flat[own]
+    state.base = base
+    state.base_at = state.iteration
+    return info
+
+
+# ---------------------------------------------------------------------------
 # mass wasting (maps.py:1050-1085, particle.py:1104-1227)
 # ---------------------------------------------------------------------------
 DEP_FLOOR = 0.02  # particle.py:99
@@ -758,13 +835,17 @@ def step(state, params, iteration_key, **kw):
     """maps.py:1516-1617: particles, mass wasting, uplift, isostasy (every
     isostasy_every iterations), datum, halos."""
     ep = _eparams(params)
-    if state.spherical and int(getattr(ep, "sea_mask_every", 0)) > 0:
-        raise NotImplementedError("the sea-mask refresh is outside the oracle's scope")
     if state.spherical and bool(getattr(ep, "lake_balance", False)) and int(ep.flood_every) > 0:
         raise NotImplementedError("lake refreshes are outside the oracle's scope")
     if state.spherical and int(ep.glacial_every) > 0:
         raise NotImplementedError("the glacial pass is outside the oracle's scope")
     t0 = time.time()
+    sea_every = int(getattr(ep, "sea_mask_every", 0))
+    if state.spherical and sea_every > 0 and (getattr(state, "base_at", None) is None or state.iteration % sea_every == 0):
+        # before the route: the flood seeds on the sea
+        st_base = refresh_base(state, ocean_min_fraction(params))
+    else:
+        st_base = None
     flood = int(ep.flood_every)
     if flood > 0 and (state.route is None or state.iteration % flood == 0):
         state.refresh_route(cell_units(ep, "route_eps", state.height_unit_m))
@@ -790,6 +871,8 @@ def step(state, params, iteration_key, **kw):
         st["datum_shift"] = hold_datum(state, held_land_fraction(state, params))
     state.exchange_halos()
     state.iteration += 1
+    if st_base is not None:
+        st["sea"] = st_base
     st["seconds_particles"] = t1 - t0
     st["seconds_total"] = time.time() - t0
     return st

@@ -40,6 +40,11 @@
  *                               inverse-metric component's role) and the
  *                               reference's spherical-harmonic eigenfunction
  *                               test (test_cubesphere.py:314-336).
+ *   label_components         -- the reference's closed-basin tests
+ *                               (test_hydro.py:292-309, test_erosion.py:
+ *                               1109-1136) on the sea classification built on
+ *                               it, and a union-find over the same neighbour
+ *                               graph (another algorithm) on random masks.
  */
 #include <math.h>
 #include <stdint.h>
@@ -1130,4 +1135,38 @@ void or_priority_flood_eps(int64_t F, int64_t NE, int64_t H, int64_t N, const do
             or_hpush(heap, &hn, it);
         }
     }
+}
+
+/* ---------------------------------------------------------------------- */
+/* connected components of a mask of interior cells (bake/globe/hydro/     */
+/* lakes.py:19-46 with the level all equal, as below_sea_components calls  */
+/* it, hydro/run.py:64-83): 8-connected across faces through the owner     */
+/* table (hydro/d8.py:71-77), numbered in order of their lowest cell id.   */
+/* ---------------------------------------------------------------------- */
+int64_t or_label_components(const uint8_t *mask, const int64_t *owner, int64_t N, int64_t H, int32_t *labels,
+                            int64_t *stack) {
+    const int64_t NE = N + 2 * H, M = 6 * N * N;
+    for (int64_t c = 0; c < M; c++) labels[c] = -1;
+    int64_t n = 0;
+    for (int64_t s = 0; s < M; s++) {
+        if (!mask[s] || labels[s] >= 0) continue;
+        labels[s] = (int32_t)n;
+        int64_t sp = 0;
+        stack[sp++] = s;
+        while (sp > 0) {
+            const int64_t c = stack[--sp];
+            const int64_t f = c / (N * N), r = c - f * N * N, i = r / N, j = r - i * N;
+            for (int k = 0; k < 8; k++) {
+                const int64_t o = owner[(f * NE + i + H + OR_D8[k][0]) * NE + j + H + OR_D8[k][1]];
+                const int64_t f2 = o / (NE * NE), r2 = o - f2 * NE * NE;
+                const int64_t q = (f2 * N + r2 / NE - H) * N + (r2 % NE) - H;
+                if (mask[q] && labels[q] < 0) {
+                    labels[q] = (int32_t)n;
+                    stack[sp++] = q;
+                }
+            }
+        }
+        n += 1;
+    }
+    return n;
 }

@@ -422,8 +422,9 @@ __device__ __forceinline__ ll ceiling_fx(float lim, double hscale) {
 enum : int { REC_NONE = 0, REC_SEA = 1, REC_LAND = 2 };
 enum : int { RF_MOVE = 1 << 2, RF_START = 1 << 3, RF_END = 1 << 4, RF_EXITED = 1 << 16, RF_DEPOSIT = 1 << 17 };
 
-// WPE: the register budget as waves per SIMD (2 = the kernel's natural
-// allocation; 3 and 4 trade spills for latency hiding; the autotuner picks).
+// WPE: the register budget as waves per SIMD (trace_waves: the census's
+// natural allocation fits 4, the traced applying pass's 3; one or two more
+// waves trade spills for latency hiding; the autotuner picks).
 // APPLY: false = the census pass (erosion requests and floors; with a tape,
 // every move recorded for replay_kernel), true = the applying pass traced
 // again (batched windows, or no tape).
@@ -843,19 +844,30 @@ __global__ void __launch_bounds__(TRACE_TPB) __attribute__((amdgpu_waves_per_eu(
     }
 }
 
-// the trace kernel of a register-budget variant (0: 2 waves per SIMD, 1: 3, 2: 4) and pass
+// waves per SIMD of a register-budget variant (0: the kernel's natural
+// budget -- 121 VGPRs for the census, 141 for the traced applying pass --
+// 1 and 2: one and two waves more)
+template <bool APPLY>
+constexpr int trace_waves(int variant) {
+    return (APPLY ? 3 : 4) + variant;
+}
+
+// the trace kernel of a register-budget variant and pass
 template <bool BATCH, bool APPLY>
 const void* trace_fn(int64_t variant) {
-    if (variant == 1) return (const void*)trace_kernel<BATCH, 3, APPLY>;
-    if (variant == 2) return (const void*)trace_kernel<BATCH, 4, APPLY>;
-    return (const void*)trace_kernel<BATCH, 2, APPLY>;
+    if (variant == 1) return (const void*)trace_kernel<BATCH, trace_waves<APPLY>(1), APPLY>;
+    if (variant == 2) return (const void*)trace_kernel<BATCH, trace_waves<APPLY>(2), APPLY>;
+    return (const void*)trace_kernel<BATCH, trace_waves<APPLY>(0), APPLY>;
 }
 
 template <bool BATCH, bool APPLY>
 void launch_trace(int64_t variant, int grid, hipStream_t s, const TraceCfg& k) {
-    if (variant == 1) hipLaunchKernelGGL((trace_kernel<BATCH, 3, APPLY>), dim3(grid), dim3(TRACE_TPB), 0, s, k);
-    else if (variant == 2) hipLaunchKernelGGL((trace_kernel<BATCH, 4, APPLY>), dim3(grid), dim3(TRACE_TPB), 0, s, k);
-    else hipLaunchKernelGGL((trace_kernel<BATCH, 2, APPLY>), dim3(grid), dim3(TRACE_TPB), 0, s, k);
+    if (variant == 1)
+        hipLaunchKernelGGL((trace_kernel<BATCH, trace_waves<APPLY>(1), APPLY>), dim3(grid), dim3(TRACE_TPB), 0, s, k);
+    else if (variant == 2)
+        hipLaunchKernelGGL((trace_kernel<BATCH, trace_waves<APPLY>(2), APPLY>), dim3(grid), dim3(TRACE_TPB), 0, s, k);
+    else
+        hipLaunchKernelGGL((trace_kernel<BATCH, trace_waves<APPLY>(0), APPLY>), dim3(grid), dim3(TRACE_TPB), 0, s, k);
 }
 
 // ---------------------------------------------------------------------------

@@ -100,6 +100,8 @@ SIGNATURES = [
     ("pg_fast_layout", i32, [i64, i64, i64, ctypes.POINTER(ctypes.c_int64), i32]),
     ("pg_fast_trace_time", i32, [pd, ctypes.POINTER(ctypes.c_int64)]),
     ("pg_fast_trace_time_passes", i32, [pd, ctypes.POINTER(ctypes.c_int64)]),
+    ("pg_sea_work_bytes", i64, [i64]),
+    ("pg_refresh_base", i32, [vp, vp, vp, vp, dbl, i64, i64, vp, vp, i64, pd, vp]),
     ("pg_batch_workspace_bytes", i64, [i64, ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int64)]),
     ("pg_batch_setup", i32, [ctypes.POINTER(FastFields), i64, ctypes.POINTER(FastParams), ctypes.POINTER(ctypes.c_int64),
                              vp, i64, vp]),

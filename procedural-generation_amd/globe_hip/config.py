@@ -54,10 +54,13 @@ class TectonicsGroup:
 
 @dataclass
 class HydroGroup:
-    """Only ``lake_min_depth`` is read here (a window's lakes, bake/globe/
-    erosion/maps.py:1563-1566; bake/globe/config.py:615)."""
+    """The fields the erosion stage reads: ``lake_min_depth`` (a window's
+    lakes, bake/globe/erosion/maps.py:1563-1566; bake/globe/config.py:615)
+    and ``ocean_min_fraction`` (the sea mask, maps.py:770-774;
+    config.py:622)."""
 
     lake_min_depth: float = 0.5  # m
+    ocean_min_fraction: float = 0.02
 
 
 @dataclass

@@ -24,6 +24,8 @@ do not depend on it).
 """
 from __future__ import annotations
 
+import math
+
 import ctypes
 
 import numpy as np
@@ -174,11 +176,16 @@ class WindowBatch:
 
     def step_per_window(self, iteration_key) -> None:
         """The same iteration as one fast-mode pass per window, one after
-        another (the reference's one-tile-at-a-time form, on the GPU)."""
+        another (the reference's one-tile-at-a-time form, on the GPU).  Each
+        window's persistent grid is sized to its chunk (1.5 lanes per droplet
+        of a chunk, at most the census's natural occupancy): a small window
+        does not launch a full-chip grid that would idle."""
         key = iteration_key if isinstance(iteration_key, (tuple, list)) else (int(iteration_key),)
+        cus = torch.cuda.get_device_properties(0).multi_processor_count
         for w, st in enumerate(self.states):
-            st.fast_grid_blocks = self.grid_blocks
-            st.fast_variant = self.variant
+            per_chunk = self.P[w] / max(self.chunks, 1)
+            st.fast_grid_blocks = int(min(4 * cus, max(cus // 4, math.ceil(1.5 * per_chunk / 256))))
+            st.fast_variant = 0
             st.fast_time_trace = self.time_trace
             fast.run_iteration(st, self.params[w], key, chunks=self.chunks, sync_stats=False)
             maps.apply_uplift(st)
@@ -222,7 +229,8 @@ class BatchGraph:
 
 class Autotuner:
     """Picks the persistent trace grid and the trace kernel's register-budget
-    variant (2, 3 or 4 waves per SIMD) of a batch or of a fast-mode state
+    variant (0: the kernels' natural budget, 1 / 2: one / two more waves per
+    SIMD) of a batch or of a fast-mode state
     from real iterations: the first len(candidates) iterations each run one
     (grid, variant) candidate with the trace launches timed (HIP events on
     the launch stream) and normalised by the iteration's droplet moves (the
@@ -234,7 +242,7 @@ class Autotuner:
         self.target = target
         if candidates is None:
             cus = torch.cuda.get_device_properties(0).multi_processor_count
-            candidates = [(g * cus, v) for v in (0, 1, 2) for g in (2, 4)]
+            candidates = [(g * cus, v) for v in (0, 1, 2) for g in (4 + v, 2 * (4 + v))]
         self.candidates = [(int(c[0]), int(c[1])) if isinstance(c, (tuple, list)) else (int(c), 0)
                            for c in candidates]
         self.candidates = [c for c in self.candidates if c[0] > 0 and c[1] in (0, 1, 2)]

@@ -44,6 +44,8 @@ def save_checkpoint(directory, state, params_hash: str, keep: int = 2) -> Path:
         arrays[n] = _host(t)
     if getattr(state, "route", None) is not None:
         arrays["route"] = _host(state.route)
+    if getattr(state, "base_at", None) is not None:  # run.py:110-111: the sea mask's base level is state
+        arrays["base"] = _host(state.base)
     if getattr(state, "iso_acc", None) is not None:  # run.py:112-113: a rebound not yet applied is state
         arrays["iso_acc"] = _host(state.iso_acc)
     with open(tmp, "wb") as fh:
@@ -127,4 +129,12 @@ def load_checkpoint(state, path, meta: dict) -> None:
             state.iso_acc.copy_(torch.from_numpy(acc))
         elif getattr(state, "iso_acc", None) is not None:
             state.iso_acc.zero_()
+        if "base" in z.files:  # run.py:204-208: a resume sees the mask the continuous run had
+            b = np.ascontiguousarray(z["base"], np.float64)
+            if tuple(b.shape) != tuple(state.base.shape):
+                raise ValueError(f"base: checkpoint shape {b.shape}, state {tuple(state.base.shape)}")
+            state.base.copy_(torch.from_numpy(b))
+            state.base_at = int(meta["iteration"])
+        else:
+            state.base_at = None
     state.iteration = int(meta["iteration"])

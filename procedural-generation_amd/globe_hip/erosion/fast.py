@@ -205,7 +205,8 @@ class IterationGraph:
     """A whole fast-mode erosion iteration -- particle pass, mass wasting,
     uplift, isostasy, datum hold, halo exchange (bake/globe/erosion/maps.py:
     1516-1617) -- captured once as a HIP graph and replayed per iteration
-    (two graphs when isostasy is on: with and without its application).
+    (two graphs when isostasy is on: with and without its application).  The
+    sea mask, when due, runs eagerly before the replay.
 
     The only per-iteration input from the host is the iteration's PCG64
     state (``params.rng(stage, *key)``), copied into a device buffer before
@@ -245,13 +246,14 @@ class IterationGraph:
         it0 = state.iteration
         self.graph = torch.cuda.CUDAGraph()
         with torch.cuda.graph(self.graph):
-            self.result = maps.step(state, params, (0,), n_particles=n_particles, isostasy_due=False)
+            self.result = maps.step(state, params, (0,), n_particles=n_particles, isostasy_due=False, sea_due=False)
         self.graph_iso = None
         if self.iso_every:
             state.iteration = it0
             self.graph_iso = torch.cuda.CUDAGraph()
             with torch.cuda.graph(self.graph_iso):
-                self.result_iso = maps.step(state, params, (0,), n_particles=n_particles, isostasy_due=True)
+                self.result_iso = maps.step(state, params, (0,), n_particles=n_particles, isostasy_due=True,
+                                            sea_due=False)
         state.iteration = it0
         self.stats = ctx.stats
 
@@ -262,7 +264,12 @@ class IterationGraph:
 
     def step(self, iteration_key) -> None:
         """Run one iteration (asynchronous)."""
+        from . import maps
+
         key = iteration_key if isinstance(iteration_key, (tuple, list)) else (int(iteration_key),)
+        if maps.sea_refresh_due(self.state, maps._eparams(self.params)):
+            # host-synchronous, so outside the graph: the replay reads the new base
+            self.state.refresh_base(maps.ocean_min_fraction(self.params))
         self._load_rng(key)
         if self.graph_iso is not None and (self.state.iteration + 1) % self.iso_every == 0:
             self.graph_iso.replay()

@@ -139,6 +139,7 @@ class ErosionState:
     roots: torch.Tensor | None = None
     inflow_volume: float = 0.0
     iso_acc: torch.Tensor | None = None  # maps.py:1369-1371: surface change not yet compensated
+    base_at: int | None = None  # the iteration of the last sea-mask refresh (maps.py:420)
     _halo: HaloTables | None = field(default=None, repr=False)
     _side: dict = field(default_factory=dict, repr=False)
     _owner: np.ndarray | None = field(default=None, repr=False)
@@ -341,6 +342,49 @@ class ErosionState:
         return t
 
 
+    # -- the sea mask (maps.py:360-421) --------------------------------------
+    def _sea_tables(self) -> dict:
+        """Each interior cell's area (cubesphere.py:509-517: float32 m^2) as
+        an exact integer multiple of 2^-shift m^2, so the device sums a
+        component's area exactly; the owner table and the union-find's
+        scratch; built once."""
+        t = self._side.get("sea")
+        if t is None:
+            H, N = self.H, self.N
+            area = (np.asarray(self.grid.sqrt_det_metric).astype(np.float64) * float(self.grid.cell_size_m) ** 2)
+            area = area.astype(np.float32)[:, H:H + N, H:H + N].astype(np.float64)
+            _, e = np.frexp(area)
+            shift = 24 - int(e.min())
+            fx = np.ldexp(area, shift)
+            assert np.array_equal(fx, np.floor(fx)) and fx.sum() < 2.0 ** 62
+            dev = self.device
+            n = int(lib().pg_sea_work_bytes(N))
+            t = {"area_fx": _dev(fx.astype(np.int64), dev), "shift": shift, "area_sum": float(area.sum()),
+                 "owner": _dev(self.owner_table(), dev),
+                 "work": torch.empty((n + 7) // 8, dtype=torch.int64, device=dev)}
+            self._side["sea"] = t
+        return t
+
+    def refresh_base(self, min_fraction: float) -> dict:
+        """Recompute the per-cell base level (maps.py:360-421): 0 on the open
+        ocean and on land, a closed basin's deepest surface inside it; the
+        halos copy their owners.  The components, areas and floors are
+        found on the device (pg_refresh_base); a component's area is summed
+        exactly, where the reference sums float64 in cell order."""
+        info = {"closed_cells": 0, "closed_basins": 0, "sea_cells": 0, "deepest_basin_floor": 0.0}
+        if not self.spherical or self.grid is None:
+            self.base.zero_()
+            self.base_at = self.iteration
+            return info
+        t = self._sea_tables()
+        thr = math.ldexp(float(min_fraction) * t["area_sum"], t["shift"])
+        got = pk.refresh_base(self.height, self.sediment, t["owner"], t["area_fx"], thr, self.N, self.H, self.base,
+                              t["work"])
+        got.pop("rounds")
+        info.update(got)
+        self.base_at = self.iteration
+        return info
+
     # -- routing surface (maps.py:342-358, route.py:30-117) ------------------
     def owner_table(self) -> np.ndarray:
         """Flat index of the interior cell owning every extended cell
@@ -487,6 +531,13 @@ def spawn_particles(state: ErosionState, n_particles: int, rng: np.random.Genera
     x = (ei - state.H).astype(np.float64) + off[:, 0]
     y = (ej - state.H).astype(np.float64) + off[:, 1]
     return _dev(f.astype(np.int64), dev), _dev(x, dev), _dev(y, dev), total / n_particles
+
+
+def ocean_min_fraction(params) -> float:
+    """``hydro.ocean_min_fraction`` (maps.py:770-774); 0.02 without a hydro
+    group."""
+    hydro = getattr(params, "hydro", None)
+    return float(hydro.ocean_min_fraction) if hydro is not None and hasattr(params, "world") else 0.02
 
 
 def _check_scope(ep, state):
@@ -918,20 +969,31 @@ def start_replay(state: ErosionState, params):
     return info
 
 
-def step(state: ErosionState, params, iteration_key, log=None, after_refresh=None, isostasy_due=None, **kw) -> dict:
-    """One iteration in the reference's order (maps.py:1516-1617): the
-    routing flood (every ``flood_every``) and a window's lakes, particles,
-    mass wasting, uplift, isostasy (every ``isostasy_every``
-    iterations; ``isostasy_due`` overrides that choice, for graph capture),
-    datum hold (globe), halo exchange."""
+def sea_refresh_due(state: ErosionState, ep) -> bool:
+    """Whether this iteration refreshes the sea mask (maps.py:1541-1543)."""
+    sea_every = int(getattr(ep, "sea_mask_every", 0))
+    return bool(state.spherical and sea_every > 0 and (state.base_at is None or state.iteration % sea_every == 0))
+
+
+def step(state: ErosionState, params, iteration_key, log=None, after_refresh=None, isostasy_due=None, sea_due=None,
+         **kw) -> dict:
+    """One iteration in the reference's order (maps.py:1516-1617): the sea
+    mask (every ``sea_mask_every``), the routing flood (every
+    ``flood_every``) and a window's lakes, particles, mass wasting, uplift,
+    isostasy (every ``isostasy_every`` iterations), datum hold (globe), halo
+    exchange.  ``isostasy_due`` and ``sea_due`` override the stride's choice
+    (graph capture: the sea mask is host-synchronous and runs outside the
+    graph)."""
     ep = _eparams(params)
-    if state.spherical and int(getattr(ep, "sea_mask_every", 0)) > 0:
-        raise NotImplementedError("the sea-mask refresh (erosion.sea_mask_every) is outside the HIP erosion path")
     if state.spherical and bool(getattr(ep, "lake_balance", False)) and int(ep.flood_every) > 0:
         raise NotImplementedError("lake refreshes (erosion.lake_balance) are outside the HIP erosion path")
     if state.spherical and int(ep.glacial_every) > 0:
         raise NotImplementedError("the glacial pass is outside the HIP erosion path (set glacial_every = 0)")
     t0 = time.time()
+    st_base = None
+    if sea_refresh_due(state, ep) if sea_due is None else sea_due:
+        # before the route: the flood seeds on the sea (maps.py:1541-1547)
+        st_base = state.refresh_base(ocean_min_fraction(params))
     flood = int(ep.flood_every)
     if flood > 0 and (state.route is None or state.iteration % flood == 0):
         state.refresh_route(cell_units(ep, "route_eps", state.height_unit_m))
@@ -965,6 +1027,8 @@ def step(state: ErosionState, params, iteration_key, log=None, after_refresh=Non
         st["datum_shift"] = q if state.mode == "fast" else float(q.item())
     state.exchange_halos()
     state.iteration += 1
+    if st_base is not None:
+        st["sea"] = st_base
     st["seconds_route"] = tr
     st["seconds_particles"] = t1 - t0 - tr
     st["seconds_total"] = time.time() - t0

@@ -347,6 +347,16 @@ def divide(x, v, out):
 # ---------------------------------------------------------------------------
 # the routing surface (route.py:30-117)
 # ---------------------------------------------------------------------------
+def refresh_base(height, sediment, owner, area_fx, threshold_fx, N, H, base, work) -> dict:
+    """The sea mask's base levels (pg_refresh_base; host-synchronous)."""
+    info = (ctypes.c_double * 5)()
+    check(lib().pg_refresh_base(_ptr(height), _ptr(sediment), _ptr(owner), _ptr(area_fx), float(threshold_fx), int(N),
+                                int(H), _ptr(base), _ptr(work), int(work.numel() * work.element_size()), info,
+                                _stream()), "pg_refresh_base")
+    return {"closed_cells": int(info[0]), "closed_basins": int(info[1]), "sea_cells": int(info[2]),
+            "deepest_basin_floor": float(info[3]), "rounds": int(info[4])}
+
+
 def flood_init(height, sediment, base, cls, H, N, s_out, W, flag):
     F, NE, _ = height.shape
     e = (F, NE, NE)
