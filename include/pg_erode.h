@@ -308,19 +308,26 @@ typedef struct pg_fast_params {
                               * pg_slope_erode_cap; bake/globe/erosion/maps.py:923-926), NULL = none */
     void* tape;            /* optional DEVICE move tape of pg_fast_tape_bytes() bytes: the census pass records
                             * every move and the applying pass replays it instead of tracing again (same
-                            * results); NULL = trace twice.  pg_fast_iterate / pg_fast_particles only */
+                            * results); NULL = trace twice.  pg_batch_iterate takes it from the first
+                            * window's params, sized by pg_batch_tape_bytes */
     int64_t tape_bytes;
 } pg_fast_params;
 
-/* Bytes of the move tape for n_particles rain droplets on F faces of N x N
- * (plus at most one stockpile droplet per cell) in `chunks` chunks of at most
- * max_steps moves, for the persistent trace grid of (grid_blocks, variant) as
- * in pg_fast_params: per wave of that grid, rows of 64 lane records of 32
- * bytes -- twice the wave's share of the largest chunk's droplet-iterations
- * plus one droplet's worth -- and a row count; and a stop cell per droplet
- * of the largest chunk. */
-int64_t pg_fast_tape_bytes(int64_t n_particles, int64_t F, int64_t N, int64_t chunks, int64_t max_steps,
+/* Bytes of the move tape for n_particles rain droplets on F faces of NE x NE
+ * extended cells (plus at most one stockpile droplet per cell) in `chunks`
+ * chunks of at most max_steps moves, for the persistent trace grid of
+ * (grid_blocks, variant) as in pg_fast_params: per wave of that grid, rows of
+ * 64 lane records of 32 bytes -- twice the wave's share of the largest
+ * chunk's droplet-iterations plus one droplet's worth -- and a row count; and
+ * a stop cell per droplet of the largest chunk. */
+int64_t pg_fast_tape_bytes(int64_t n_particles, int64_t F, int64_t NE, int64_t chunks, int64_t max_steps,
                            int64_t grid_blocks, int64_t variant);
+/* The same for a batch (pg_batch_iterate): every window's share of a chunk
+ * (fps[w].n_particles and its extended cells, fps[0].chunks), max_steps the
+ * windows' largest lifetime (which pg_batch_iterate reads from
+ * prm->max_steps when a tape is given). */
+int64_t pg_batch_tape_bytes(const pg_fast_fields* wins, const pg_fast_params* fps, int64_t n_win, int64_t max_steps,
+                            int64_t grid_blocks, int64_t variant);
 
 /* Statistics of one fast iteration (DEVICE int64[PG_FAST_NSTATS]). */
 enum {

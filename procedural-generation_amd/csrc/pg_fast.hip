@@ -432,7 +432,7 @@ template <bool BATCH, int WPE, bool APPLY>
 __global__ void __launch_bounds__(TRACE_TPB) __attribute__((amdgpu_waves_per_eu(WPE, WPE))) trace_kernel(TraceCfg k) {
     const int lane = threadIdx.x & 63;
     constexpr bool apply = APPLY;
-    const bool taping = !APPLY && !BATCH && k.tape != nullptr;
+    const bool taping = !APPLY && k.tape != nullptr;
     // per-lane geometry and droplet scalars: the grid's (one launch = one
     // grid), or, batched, those of the window of the lane's droplet
     int N = k.N, NE = k.NE, max_steps = k.max_steps;
@@ -521,7 +521,7 @@ __global__ void __launch_bounds__(TRACE_TPB) __attribute__((amdgpu_waves_per_eu(
 
     while (true) {
         // this iteration's tape record
-        int r_flags = 0, r_cell = 0, r_dl = 0;
+        int r_flags = 0, r_cell = 0, r_dl = 0, r_d = 0;
         float r_x = 0.0f, r_aux = 0.0f;
         // a wave whose tape rows could not hold one more droplet takes none
         if (taping && !exhausted && t_row + k.max_steps + 2 > k.tape_rows) exhausted = true;
@@ -564,6 +564,7 @@ __global__ void __launch_bounds__(TRACE_TPB) __attribute__((amdgpu_waves_per_eu(
                         if (taping) {
                             r_flags |= RF_START;
                             r_dl = (int)dl;
+                            r_d = (int)d;
                         }
                         const int NE2 = NE * NE;
                         const int cell = k.d_cell[d];
@@ -825,7 +826,7 @@ __global__ void __launch_bounds__(TRACE_TPB) __attribute__((amdgpu_waves_per_eu(
             // one row per loop iteration: the wave's 64 records side by side
             float4* row = wtape + 2 * (t_row * 64 + lane);
             row[0] = make_float4(__int_as_float(r_flags), __int_as_float(r_cell), r_x, r_aux);
-            if (r_flags & RF_START) row[1] = make_float4(0.0f, 0.0f, 0.0f, __int_as_float(r_dl));
+            if (r_flags & RF_START) row[1] = make_float4(0.0f, 0.0f, __int_as_float(r_d), __int_as_float(r_dl));
         }
         t_row += 1;
     }
@@ -878,15 +879,24 @@ void launch_trace(int64_t variant, int grid, hipStream_t s, const TraceCfg& k) {
 // (bit-identical results) without tracing again.  The discharge and
 // momentum tracks, which the scales do not touch, were added by the census.
 // ---------------------------------------------------------------------------
+template <bool BATCH>
 __global__ void __launch_bounds__(TRACE_TPB) replay_kernel(TraceCfg k) {
     const int lane = threadIdx.x & 63;
     const ll rw = (ll)blockIdx.x * (TRACE_TPB / 64) + (threadIdx.x >> 6);
     const ll rwaves = (ll)gridDim.x * (TRACE_TPB / 64);
-    const ll n_total = k.sc->n_total;
-    const ll C = (n_total + k.chunks - 1) / k.chunks;
-    const ll d_begin = k.chunk * C;
-    const ll d_end = (k.chunk + 1) * C < n_total ? (k.chunk + 1) * C : n_total;
-    if (k.sc->volume0 <= 0.0) return;
+    // the chunk's droplets (the census's queue counted them out) and the end of its range
+    ll d_end;
+    const ll* queue;
+    if constexpr (BATCH) {
+        d_end = k.win_prefix[(ll)k.chunk * (k.n_win + 1) + k.n_win];
+        queue = &k.queue_base[2 * k.chunk];
+    } else {
+        const ll n_total = k.sc->n_total;
+        const ll C = (n_total + k.chunks - 1) / k.chunks;
+        d_end = (k.chunk + 1) * C < n_total ? (k.chunk + 1) * C : n_total;
+        queue = &k.sc->queue[2 * k.chunk];
+        if (k.sc->volume0 <= 0.0) return;  // the census traced nothing (and wrote no rows)
+    }
     ll real = 0, deficit = 0, surplus = 0;
     int dl = 0;
     ll n_entries = 0, n_moves = 0, exit_fx = 0, vanished_fx = 0, lost_fx = 0;
@@ -916,8 +926,9 @@ __global__ void __launch_bounds__(TRACE_TPB) replay_kernel(TraceCfg k) {
             const int fl = __float_as_int(A.x);
             if (!fl) continue;
             if (fl & RF_START) {
-                dl = __float_as_int(row[1].w);
-                real = k.d_load[d_begin + dl];
+                const float4 B = row[1];
+                dl = __float_as_int(B.w);
+                real = k.d_load[__float_as_int(B.z)];
                 deficit = surplus = 0;
             }
             if (fl & RF_MOVE) {
@@ -980,7 +991,7 @@ __global__ void __launch_bounds__(TRACE_TPB) replay_kernel(TraceCfg k) {
     }
     // droplets the census left untraced because every wave's rows were full: must be none
     if (rw == 0 && lane == 0) {
-        const ll taken = k.sc->queue[2 * k.chunk];
+        const ll taken = *queue;
         if (taken < d_end) atomicAdd((ull*)&k.stats[PG_FS_TAPE_SHORT], (ull)(d_end - taken));
     }
 }
@@ -1303,28 +1314,56 @@ using namespace pg::fastmode;
 namespace pg {
 namespace fastmode {
 // rows per wave of the census tape for a grid of `grid` blocks
-int64_t tape_rows(int64_t n_particles, int64_t F, int64_t N, int64_t chunks, int64_t max_steps, int grid) {
-    const int64_t cmax = (n_particles + F * N * N + chunks - 1) / chunks;
+// (cmax: the most droplets a chunk can hold -- the spawn plus a stockpile
+// re-injected from every extended cell -- over the chunks)
+int64_t tape_rows_c(int64_t cmax, int64_t max_steps, int grid) {
     const int64_t waves = (int64_t)grid * (TRACE_TPB / 64);
     const int64_t share = (cmax * (max_steps + 1) + 64 * waves - 1) / (64 * waves);
     return 2 * share + max_steps + 3;
 }
+int64_t tape_bytes_c(int64_t cmax, int64_t max_steps, int grid) {
+    const int64_t waves = (int64_t)grid * (TRACE_TPB / 64);
+    return waves * tape_rows_c(cmax, max_steps, grid) * 64 * 32 + waves * 8 + cmax * 4;
+}
+int64_t tape_rows(int64_t n_particles, int64_t F, int64_t NE, int64_t chunks, int64_t max_steps, int grid) {
+    return tape_rows_c((n_particles + F * NE * NE + chunks - 1) / chunks, max_steps, grid);
+}
+// the union of every window's share of a chunk
+int64_t batch_chunk_max(const pg_fast_fields* wins, const pg_fast_params* fps, int64_t n_win, int64_t chunks) {
+    int64_t cmax = 0;
+    for (int64_t w = 0; w < n_win; w++) {
+        const int64_t NE = wins[w].N + 2 * wins[w].H;
+        cmax += (fps[w].n_particles + NE * NE + chunks - 1) / chunks;
+    }
+    return cmax;
+}
 }  // namespace fastmode
 }  // namespace pg
 
-extern "C" int64_t pg_fast_tape_bytes(int64_t n_particles, int64_t F, int64_t N, int64_t chunks, int64_t max_steps,
+extern "C" int64_t pg_fast_tape_bytes(int64_t n_particles, int64_t F, int64_t NE, int64_t chunks, int64_t max_steps,
                                       int64_t grid_blocks, int64_t variant) {
     clear_error();
-    if (n_particles < 0 || F <= 0 || N <= 0 || chunks <= 0 || max_steps <= 0 || grid_blocks < 0 || variant < 0 ||
+    if (n_particles < 0 || F <= 0 || NE <= 0 || chunks <= 0 || max_steps <= 0 || grid_blocks < 0 || variant < 0 ||
         variant > 2) {
         set_error("pg_fast_tape_bytes: bad arguments");
         return -1;
     }
     int grid = 0;
     if (persistent_grid(trace_fn<false, false>(variant), TRACE_TPB, grid_blocks, &grid)) return -1;
-    const int64_t waves = (int64_t)grid * (TRACE_TPB / 64);
-    const int64_t cmax = (n_particles + F * N * N + chunks - 1) / chunks;
-    return waves * tape_rows(n_particles, F, N, chunks, max_steps, grid) * 64 * 32 + waves * 8 + cmax * 4;
+    return tape_bytes_c((n_particles + F * NE * NE + chunks - 1) / chunks, max_steps, grid);
+}
+
+extern "C" int64_t pg_batch_tape_bytes(const pg_fast_fields* wins, const pg_fast_params* fps, int64_t n_win,
+                                       int64_t max_steps, int64_t grid_blocks, int64_t variant) {
+    clear_error();
+    if (!wins || !fps || n_win <= 0 || max_steps <= 0 || grid_blocks < 0 || variant < 0 || variant > 2 ||
+        fps[0].chunks <= 0) {
+        set_error("pg_batch_tape_bytes: bad arguments");
+        return -1;
+    }
+    int grid = 0;
+    if (persistent_grid(trace_fn<true, false>(variant), TRACE_TPB, grid_blocks, &grid)) return -1;
+    return tape_bytes_c(batch_chunk_max(wins, fps, n_win, fps[0].chunks), max_steps, grid);
 }
 
 extern "C" int64_t pg_fast_workspace_bytes(int64_t F, int64_t NE, int64_t n_particles) {
@@ -1537,7 +1576,7 @@ int fast_particles(const pg_fast_fields* fl, const pg_trace_params* prm, const p
     int grid = 0, rgrid = 0;
     PG_REQUIRE(fp->variant >= 0 && fp->variant <= 2, "fast mode: variant must be 0, 1 or 2");
     if (persistent_grid(trace_fn<false, false>(fp->variant), TRACE_TPB, fp->grid_blocks, &grid)) return -1;
-    if (fp->tape && persistent_grid((const void*)replay_kernel, TRACE_TPB, 0, &rgrid)) return -1;
+    if (fp->tape && persistent_grid((const void*)replay_kernel<false>, TRACE_TPB, 0, &rgrid)) return -1;
     k.tape = nullptr;
     k.tape_len = nullptr;
     k.tape_rows = 0;
@@ -1545,9 +1584,9 @@ int fast_particles(const pg_fast_fields* fl, const pg_trace_params* prm, const p
     k.tape_waves = 0;
     if (fp->tape) {
         const int64_t waves = (int64_t)grid * (TRACE_TPB / 64);
-        const int64_t rows = tape_rows(fp->n_particles, fl->F, fl->N, chunks, prm->max_steps, grid);
-        const int64_t cmax = (fp->n_particles + fl->F * fl->N * fl->N + chunks - 1) / chunks;
-        const int64_t need = waves * rows * 64 * 32 + waves * 8 + cmax * 4;
+        const int64_t cmax = (fp->n_particles + fl->F * NE * NE + chunks - 1) / chunks;
+        const int64_t rows = tape_rows_c(cmax, prm->max_steps, grid);
+        const int64_t need = tape_bytes_c(cmax, prm->max_steps, grid);
         PG_REQUIRE(fp->tape_bytes >= need, "fast mode: tape of %lld bytes, need %lld", (long long)fp->tape_bytes,
                    (long long)need);
         k.tape = reinterpret_cast<float4*>(fp->tape);
@@ -1572,7 +1611,7 @@ int fast_particles(const pg_fast_fields* fl, const pg_trace_params* prm, const p
         // pass 1: the same trajectories, erosion scaled per cell, deficits cancel deposits
         k.pass = 1;
         if (fp->time_trace && timer_record(s, true, k.pass)) return -1;
-        if (k.tape) hipLaunchKernelGGL(replay_kernel, dim3(rgrid), dim3(TRACE_TPB), 0, s, k);
+        if (k.tape) hipLaunchKernelGGL(replay_kernel<false>, dim3(rgrid), dim3(TRACE_TPB), 0, s, k);
         else launch_trace<false, true>(fp->variant, grid, s, k);
         if (fp->time_trace && timer_record(s, false, k.pass)) return -1;
         hipLaunchKernelGGL(resolve_kernel, dim3(grid_for(n)), dim3(TPB), 0, s, n, fl->height, fl->sediment, fl->base,
@@ -1966,10 +2005,27 @@ extern "C" int pg_batch_iterate(const pg_fast_fields* wins, int64_t n_win, const
     k.win_prefix = prefix;
     k.n_win = (int)n_win;
     k.queue_base = qbase;
-    int grid = 0;
+    int grid = 0, rgrid = 0;
     PG_REQUIRE(f0.variant >= 0 && f0.variant <= 2, "pg_batch_iterate: variant must be 0, 1 or 2");
     PG_REQUIRE(!f0.erode_cap, "pg_batch_iterate: the slope limit (erode_cap) is not supported on batched windows");
     if (persistent_grid(trace_fn<true, false>(f0.variant), TRACE_TPB, f0.grid_blocks, &grid)) return -1;
+    // the move tape: prm->max_steps bounds every window's lifetime
+    k.max_steps = (int)prm->max_steps;
+    if (f0.tape) {
+        PG_REQUIRE(prm->max_steps > 0, "pg_batch_iterate: a tape needs the windows' largest max_steps in prm");
+        if (persistent_grid((const void*)replay_kernel<true>, TRACE_TPB, 0, &rgrid)) return -1;
+        const int64_t cmax = batch_chunk_max(wins, fps, n_win, chunks);
+        const int64_t waves = (int64_t)grid * (TRACE_TPB / 64);
+        const int64_t rows = tape_rows_c(cmax, prm->max_steps, grid);
+        const int64_t need = tape_bytes_c(cmax, prm->max_steps, grid);
+        PG_REQUIRE(f0.tape_bytes >= need, "pg_batch_iterate: tape of %lld bytes, need %lld", (long long)f0.tape_bytes,
+                   (long long)need);
+        k.tape = reinterpret_cast<float4*>(f0.tape);
+        k.tape_rows = rows;
+        k.tape_len = reinterpret_cast<ll*>(k.tape + 2 * waves * rows * 64);
+        k.tape_last = reinterpret_cast<int*>(k.tape_len + waves);
+        k.tape_waves = waves;
+    }
     const pg_fast_fields& a = wins[0];  // arena bases
     for (int c = 0; c < chunks; c++) {
         k.chunk = c;
@@ -1985,7 +2041,8 @@ extern "C" int pg_batch_iterate(const pg_fast_fields* wins, int64_t n_win, const
                            at<float>(work, L.rscale), f0.iter_erode, (const double*)nullptr, hscale);
         k.pass = 1;
         if (f0.time_trace && timer_record(s, true, k.pass)) return -1;
-        launch_trace<true, true>(f0.variant, grid, s, k);
+        if (k.tape) hipLaunchKernelGGL(replay_kernel<true>, dim3(rgrid), dim3(TRACE_TPB), 0, s, k);
+        else launch_trace<true, true>(f0.variant, grid, s, k);
         if (f0.time_trace && timer_record(s, false, k.pass)) return -1;
         hipLaunchKernelGGL(resolve_kernel, dim3(grid_for(n)), dim3(TPB), 0, s, n, a.height, a.sediment, a.base,
                            k.chunk_fx, at<ll>(work, L.acc_fx), at<ll>(work, L.pend_fx), at<float4>(work, L.cell4),

@@ -128,6 +128,7 @@ SIGNATURES = [
     ("pg_scale", i32, [i64, vp, dbl, vp, vp]),
     ("pg_divide", i32, [i64, vp, dbl, vp, vp]),
     ("pg_fast_tape_bytes", i64, [i64, i64, i64, i64, i64, i64, i64]),
+    ("pg_batch_tape_bytes", i64, [vp, vp, i64, i64, i64, i64]),
     ("pg_flood_init", i32, [i64, i64, i64, i64, vp, vp, vp, vp, vp, vp, vp, vp]),
     ("pg_flood_sweeps", i32, [i64, i64, vp, vp, vp, dbl, vp, vp, vp, i64, ctypes.c_int32, vp, vp]),
     ("pg_flood_finish", i32, [i64, vp, vp, vp, vp]),

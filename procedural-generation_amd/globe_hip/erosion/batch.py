@@ -9,10 +9,12 @@ arena (every field's windows back to back), and one iteration is
 
   - per window: pack and spawn from the window's own ``params.rng`` stream
     into the window's slice of the arena (``pg_batch_iterate``);
-  - per chunk: the two trace passes as ONE persistent launch over every
+  - per chunk: the census trace as ONE persistent launch over every
     window's share of the chunk (a device work queue across windows; lanes
     pick up the geometry and droplet scalars of the window they draw from),
-    the per-cell erosion scale and resolve over the whole arena;
+    recording each move on the batch's move tape; the per-cell erosion scale;
+    the applying pass replaying the tape (or, without a tape, tracing again
+    in one launch); the resolve over the whole arena;
   - fold / EMA over the arena, then the windows' uplift.
 
 Windows never interact and every accumulation is int64 fixed point, so the
@@ -120,7 +122,8 @@ class WindowBatch:
             raise NotImplementedError("cell-count discharge scales are outside the fast mode")
         for e in eps:
             maps.require_plain_side_passes(e, "the batched-window iteration")
-        self.prm = maps.trace_params(eps[0], st0, 1.0, 1, 1, disc_sat, mom_scale, self.max_steps[0], 0.0)
+        # prm.max_steps: the largest lifetime (each window's own comes from pg_batch_setup), the tape's bound
+        self.prm = maps.trace_params(eps[0], st0, 1.0, 1, 1, disc_sat, mom_scale, max(self.max_steps), 0.0)
         self.ep = eps[0]
         self.fields = (FastFields * self.W)(*[self._fields(st) for st in self.states])
         self.NE_arr = (ctypes.c_int64 * self.W)(*NE)
@@ -136,6 +139,8 @@ class WindowBatch:
         self.variant = 0
         self.time_trace = False
         self.rng_dev = None
+        self.use_tape = True  # the census records every move and the applying pass replays it (same results)
+        self.tape = None
         check(lib().pg_batch_setup(self.fields, self.W, self._fparams((0,)), self.steps_arr,
                                    ctypes.c_void_p(self.work.data_ptr()), nbytes, pk._stream()), "pg_batch_setup")
 
@@ -165,10 +170,33 @@ class WindowBatch:
                                 rng_dev=None if self.rng_dev is None else self.rng_dev.data_ptr() + 32 * w)
         return out
 
+    def tape_for(self, fp):
+        """The batch's move tape for the current grid and variant (allocated
+        on first use, grown when a larger one is needed; allocate it before a
+        graph capture), or None when ``use_tape`` is off or it would exceed
+        fast.TAPE_MAX_BYTES."""
+        if not self.use_tape:
+            return None
+        need = int(lib().pg_batch_tape_bytes(self.fields, fp, self.W, int(self.prm.max_steps), int(self.grid_blocks),
+                                             int(self.variant)))
+        if need < 0:
+            check(-1, "pg_batch_tape_bytes")
+        if need > fast.TAPE_MAX_BYTES:
+            return None
+        if self.tape is None or self.tape.numel() < need:
+            self.tape = None
+            self.tape = torch.empty(need, dtype=torch.uint8, device=self.height.device)
+        return self.tape
+
     def step(self, iteration_key) -> None:
         """One batched iteration of every window (asynchronous)."""
         key = iteration_key if isinstance(iteration_key, (tuple, list)) else (int(iteration_key),)
-        check(lib().pg_batch_iterate(self.fields, self.W, ctypes.byref(self.prm), self._fparams(key),
+        fp = self._fparams(key)
+        tape = self.tape_for(fp)
+        if tape is not None:
+            fp[0].tape = tape.data_ptr()
+            fp[0].tape_bytes = tape.numel()
+        check(lib().pg_batch_iterate(self.fields, self.W, ctypes.byref(self.prm), fp,
                                      ctypes.c_void_p(self.work.data_ptr()), self.work.numel(),
                                      ctypes.c_void_p(self.stats.data_ptr()), ctypes.c_void_p(self.win_stats.data_ptr()),
                                      pk._stream()), "pg_batch_iterate")
@@ -210,6 +238,7 @@ class BatchGraph:
         dev = batch.height.device
         batch.rng_dev = torch.zeros((batch.W, 4), dtype=torch.int64, device=dev)
         batch.time_trace = False
+        batch.tape_for(batch._fparams((0,)))  # allocated before the capture
         torch.cuda.synchronize()
         self._load((0,))
         self.graph = torch.cuda.CUDAGraph()

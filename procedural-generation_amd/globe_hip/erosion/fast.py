@@ -70,7 +70,7 @@ class FastContext:
         ``state.fast_tape`` is False."""
         if not bool(getattr(state, "fast_tape", True)):
             return None
-        need = int(lib().pg_fast_tape_bytes(self.n_particles, state.F, state.N, int(chunks), int(max_steps),
+        need = int(lib().pg_fast_tape_bytes(self.n_particles, state.F, state.NE, int(chunks), int(max_steps),
                                             int(grid_blocks), int(variant)))
         if need < 0:
             check(-1, "pg_fast_tape_bytes")

@@ -39,6 +39,23 @@ def test_batched_equals_per_window(cuda):
     assert float((a.height - _batch(cuda).height).abs().max()) > 0
 
 
+def test_batch_tape_replay_equals_the_traced_applying_pass(cuda):
+    """The batch's census records every move and the applying pass replays
+    it; with the tape off the applying pass traces again: bitwise the same,
+    and no droplet is ever left untraced for want of tape rows."""
+    from globe_hip import _lib
+
+    a, b = _batch(cuda, seed=3), _batch(cuda, seed=3)
+    b.use_tape = False
+    for it in range(3):
+        a.step(it)
+        b.step(it)
+        torch.cuda.synchronize()
+        _same(a, b)
+        assert int(a.stats[_lib.FS["TAPE_SHORT"]]) == 0
+    assert a.tape is not None and b.tape is None
+
+
 def test_batch_graph_replays_match_eager(cuda):
     a, b = _batch(cuda, seed=5), _batch(cuda, seed=5)
     g = bt.BatchGraph(b)
