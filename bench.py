@@ -46,14 +46,15 @@ BYTES_PER_STEP_CENSUS = 188.0
 BYTES_PER_STEP_APPLY = 28.0
 HBM_PEAK_GBS = 8000.0
 #: HBM bytes per chunk (census launch, applying launch) of this workload (N_c=1024, 16 chunks) by
-#: register-budget variant, from rocprofv3 FETCH_SIZE (x 2.000, calibrated on a 1-GiB copy) + WRITE_SIZE
-#: (x 1.000); profiles/trace_kernel_earth_n1024_counters.md holds the runs
-TRAFFIC_PER_CHUNK = {}
+#: (persistent grid, register-budget variant), from rocprofv3 FETCH_SIZE (x 2.000, calibrated on a 1-GiB
+#: copy) + WRITE_SIZE (x 1.000); profiles/trace_kernel_earth_n1024_counters.md holds the runs
+TRAFFIC_PER_CHUNK = {(1024, 0): (5.498e9, 1.727e9), (1280, 1): (10.631e9, 2.009e9), (1536, 2): (24.303e9, 2.105e9),
+                     (2560, 1): (15.298e9, 2.257e9)}
 #: relaxed 64-bit atomicAdd / atomicMin per droplet move of this workload by pass (census: the erosion
 #: request and floor key on eroding steps and the three tracks on land; applying pass: the height change),
 #: from rocprofv3 TCC_ATOMIC_sum (one per lane-atomic: 268.43 M for the probe's 268.44 M adds); every one is
 #: performed past L2 (TCC_EA0_ATOMIC_sum equal); profiles/trace_kernel_earth_n1024_atomics.md
-ATOMICS_PER_MOVE = {"census": 3.00, "applying": 0.32}
+ATOMICS_PER_MOVE = {"census": 2.97, "applying": 0.32}
 
 
 def log(msg):
@@ -424,7 +425,7 @@ def main():
     trace_s = r["trace_ms"] / 1000.0
     achieved = B_STEP * r["moves"] / trace_s / 1e9 if trace_s > 0 else 0.0
     known = (a.N, a.chunks) == (1024, 16)
-    traffic = TRAFFIC_PER_CHUNK.get(r["variant"]) if known else None
+    traffic = TRAFFIC_PER_CHUNK.get((r["grid_blocks"], r["variant"])) if known else None
     out = {
         "metric": METRIC,
         "value": value,

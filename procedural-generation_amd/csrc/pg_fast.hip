@@ -622,7 +622,7 @@ __global__ void __launch_bounds__(TRACE_TPB) __attribute__((amdgpu_waves_per_eu(
                 int nf, nix, niy;
                 float nfx, nfy;
                 const float h0 = S.h;
-                cause = steer_move(k, N, S, g0, g1, vol, vol_rel, sa, sb, f, ix, iy, fx, fy, nf, nix, niy, nfx, nfy);
+                cause = steer_move<true>(k, N, S, g0, g1, vol, vol_rel, sa, sb, f, ix, iy, fx, fy, nf, nix, niy, nfx, nfy);
                 exited = cause == DEATH_EXIT;
                 {
                     if (cause < 0) {

@@ -156,9 +156,12 @@ __device__ __forceinline__ ll wave_sum(ll v) {
 // k_mom, mom_scale, dt, density, slope_gain, slope_sat, H, spherical; N is
 // the droplet's grid's interior size.
 // Returns -1 (moved), DEATH_STOP (no finite direction) or DEATH_EXIT (left a
-// window).  P = the stencil's sample at the current position.
+// window).  P = the stencil's sample at the current position.  F32_TRANSFER:
+// a face crossing in single precision (transfer_velocity_f: the fast mode's
+// trace, ~50 fewer registers) or in double (the differentiable mode, whose
+// float32 / float64 gradients are compared along the same trajectories).
 // ---------------------------------------------------------------------------
-template <class K>
+template <bool F32_TRANSFER, class K>
 __device__ __forceinline__ int steer_move(const K& k, int N, const Samp& P, const float4 g0, const float4 g1,
                                           float vol, float vol_rel, float& sa, float& sb, int f, int ix, int iy,
                                           float fx, float fy, int& nf, int& nix, int& niy, float& nfx, float& nfy) {
@@ -201,7 +204,27 @@ __device__ __forceinline__ int steer_move(const K& k, int N, const Samp& P, cons
     nfx = fminf(nfx - flx, 0.99999994f);
     nfy = fminf(nfy - fly, 0.99999994f);
     nf = f;
-    if (k.spherical) {
+    if (k.spherical && !F32_TRANSFER) {
+        if (nix < 0 || nix >= N || niy < 0 || niy >= N) {
+            // across a face edge in double precision (device libm)
+            const double Nd = (double)N;
+            const Transfer t = transfer_velocity<LibMath>(f, ((double)nix + nfx) / Nd, ((double)niy + nfy) / Nd,
+                                                          (double)sa, (double)sb);
+            nf = t.face;
+            double nx = t.u * Nd, ny = t.v * Nd;
+            if (nx >= Nd) nx = Nd * 0.999999999;
+            if (ny >= Nd) ny = Nd * 0.999999999;
+            if (!(nx >= 0.0)) nx = 0.0;
+            if (!(ny >= 0.0)) ny = 0.0;
+            const double fnx = floor(nx), fny = floor(ny);
+            nix = (int)fnx;
+            niy = (int)fny;
+            nfx = fminf((float)(nx - fnx), 0.99999994f);
+            nfy = fminf((float)(ny - fny), 0.99999994f);
+            sa = (float)t.a;
+            sb = (float)t.b;
+        }
+    } else if (k.spherical) {
         if (nix < 0 || nix >= N || niy < 0 || niy >= N) {
             // across a face edge, in single precision (transfer_velocity_f)
             const float Nf = (float)N;
@@ -226,7 +249,7 @@ __device__ __forceinline__ int steer_move(const K& k, int N, const Samp& P, cons
 }
 
 // the same move from a stencil (h0 = the bilinear surface at the position)
-template <class K>
+template <bool F32_TRANSFER, class K>
 __device__ __forceinline__ int steer_move(const K& k, int N, const Stencil& S, const float4 g0, const float4 g1, float q,
                                           float vol, float vol_rel, float& sa, float& sb, int f, int ix, int iy,
                                           float fx, float fy, int& nf, int& nix, int& niy, float& nfx, float& nfy,
@@ -234,7 +257,7 @@ __device__ __forceinline__ int steer_move(const K& k, int N, const Stencil& S, c
     Samp P = sample_of(S);
     P.q = q;
     h0 = P.h;
-    return steer_move(k, N, P, g0, g1, vol, vol_rel, sa, sb, f, ix, iy, fx, fy, nf, nix, niy, nfx, nfy);
+    return steer_move<F32_TRANSFER>(k, N, P, g0, g1, vol, vol_rel, sa, sb, f, ix, iy, fx, fy, nf, nix, niy, nfx, nfy);
 }
 
 }  // namespace fastmode

@@ -304,7 +304,7 @@ __global__ void __launch_bounds__(TRACE_TPB) grad_forward_kernel(GradCfg<Real> k
                 const float q = fmaxf(lerp2(S.c00.y, S.c10.y, S.c01.y, S.c11.y, S.wi, S.wj), 0.0f);
                 int nf, nix, niy;
                 float nfx, nfy, h0s;
-                cause = steer_move(k, k.N, S, g0, g1, q, vol_s, vol_s * inv_v0, sa, sb, f, ix, iy, fx, fy, nf, nix, niy, nfx,
+                cause = steer_move<false>(k, k.N, S, g0, g1, q, vol_s, vol_s * inv_v0, sa, sb, f, ix, iy, fx, fy, nf, nix, niy, nfx,
                                    nfy, h0s);
                 exited = cause == DEATH_EXIT;
                 if (cause < 0) {
