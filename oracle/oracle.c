@@ -26,6 +26,11 @@
  *                               tilt, pit) and the reference's own invariants
  *                               (mass conservation, offshore split, change-list
  *                               layout, stockpile bound, exit loses mass).
+ *                               Lateral cuts: the reference's own test
+ *                               (test_erosion.py:1757-1791), each cut's amount
+ *                               (rate x |sin turn| x the step's erosion) and
+ *                               side read back from the change lists, and
+ *                               closed-window conservation with cuts on.
  *                               Equality with numba's own codegen is NOT pinned:
  *                               numba is not installable here.
  *   fold / ema / pack / spawn_cells -- closed forms (elementwise / brute force).

@@ -547,8 +547,8 @@ def _check_scope(ep, state):
 
 
 def require_plain_side_passes(ep, what: str) -> None:
-    """Refuse the side passes a driver with its own iteration loop (batched
-    windows, the differentiable mode) does not run."""
+    """Refuse the side passes that a caller with its own iteration loop
+    (batched windows, the differentiable mode) does not run."""
     on = [n for n, v in (("thermal_rate", float(ep.thermal_rate) != 0.0), ("creep_rate", float(ep.creep_rate) > 0.0),
                          ("slope_limit_erode", float(getattr(ep, "slope_limit_erode", 0.0)) > 0.0),
                          ("slope_limit_deposit", float(getattr(ep, "slope_limit_deposit", 0.0)) > 0.0),

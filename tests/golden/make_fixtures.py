@@ -4,7 +4,7 @@ cubesphere_vectors.bin is the reference's golden vector file
 (bake/tests/data/cubesphere_vectors.bin, format in bake/tests/gen_vectors.py:
 magic b"CSV2", int32 K, K records '<idddddidd' = face, u, v, x, y, z, face2,
 u2, v2 from the reference's scalar to_sphere / from_sphere under glibc).  It
-is data, copied byte for byte; the GPU box has no reference checkout.
+is data, copied byte for byte, so the tests run without a reference checkout.
 """
 import os
 import shutil
