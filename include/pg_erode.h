@@ -48,7 +48,7 @@ typedef struct pg_trace_params {
 
 /* Message of the last failed call on this thread ("" if none). */
 const char* pg_last_error(void);
-/* ABI version of this library: 3 (bumped whenever a struct layout or a
+/* ABI version of this library: 4 (bumped whenever a struct layout or a
  * signature changes; 2 added pg_fast_params.erode_cap, 3 its tape). */
 int pg_abi_version(void);
 
@@ -311,6 +311,10 @@ typedef struct pg_fast_params {
                             * results); NULL = trace twice.  pg_batch_iterate takes it from the first
                             * window's params, sized by pg_batch_tape_bytes */
     int64_t tape_bytes;
+    const double* route;       /* optional DEVICE routing surface (F, NE, NE) (ErosionState.route, maps.py:351-358):
+                                * droplets steer on it where it stands above the terrain and cross lakes;
+                                * NULL = steer on the terrain */
+    const uint8_t* lake_flag;  /* optional DEVICE lake flags (F, NE, NE) with the route (maps.py:629-672) */
 } pg_fast_params;
 
 /* Bytes of the move tape for n_particles rain droplets on F faces of NE x NE

@@ -112,13 +112,22 @@ __global__ void setup_kernel(int64_t n, const float* metric, const float* metric
 // ---------------------------------------------------------------------------
 __global__ void pack_kernel(int64_t n, int64_t N, int64_t H, const double* height, const double* sediment,
                             const double* discharge, const double* momentum, const double* base, const float* precip,
-                            const uint8_t* mask, float4* cell4, float2* dyn, int* eligible, Scalars* sc) {
+                            const uint8_t* mask, float4* cell4, float2* dyn, int* eligible, Scalars* sc,
+                            const double* route, const uint8_t* lake_flag, float2* rte) {
     const int64_t NE = N + 2 * H, NE2 = NE * NE;
     int my_emin = 1 << 30;
     GRID_STRIDE(c, n) {
         const double s = sediment[c], surf = height[c] + s;
         cell4[c] = make_float4((float)surf, (float)discharge[c], (float)momentum[2 * c], (float)momentum[2 * c + 1]);
         dyn[c] = make_float2((float)s, (float)base[c]);
+        if (route) {
+            // the routing channels of the packed record (particle.py:174-190): the route as float32 and its
+            // flag -- 2 on an overflowing lake cell (3 with room for its rivers' load), else 1 where the
+            // route differs from the float32 surface
+            const float r = (float)route[c];
+            const int lk = lake_flag ? (int)lake_flag[c] : 0;
+            rte[c] = make_float2(r, lk > 0 ? (lk > 1 ? 3.0f : 2.0f) : (r != (float)surf ? 1.0f : 0.0f));
+        }
         const int64_t r = c % NE2, i = r / NE, j = r - i * NE;
         const bool inter = i >= H && i < H + N && j >= H && j < H + N;
         const float w = precip[c];
@@ -378,6 +387,9 @@ struct TraceCfg {
     // traces again): per wave of the persistent grid, `tape_rows` rows of 64
     // lane records (two float4 each) -- row t holds what every lane of the
     // wave did in its t-th loop iteration -- and the rows used (tape_len)
+    // the routing surface (ROUTE kernels): per extended cell {route, flag}
+    float2* rte;
+    float lake_trap;
     float4* tape;
     ll* tape_len;
     ll tape_rows;
@@ -397,6 +409,7 @@ struct Here {
     int m;
     float h, base, sthick, rs;
     float4 g0, g1;
+    float2 rte;  // {route, flag} of the cell (ROUTE kernels)
 };
 
 // order-preserving key of a float (for atomicMin of surfaces)
@@ -427,8 +440,9 @@ enum : int { RF_MOVE = 1 << 2, RF_START = 1 << 3, RF_END = 1 << 4, RF_EXITED = 1
 // waves trade spills for latency hiding; the autotuner picks).
 // APPLY: false = the census pass (erosion requests and floors; with a tape,
 // every move recorded for replay_kernel), true = the applying pass traced
-// again (batched windows, or no tape).
-template <bool BATCH, int WPE, bool APPLY>
+// again (batched windows, or no tape).  ROUTE: steer on the routing surface
+// and keep the lakes (particle.py:450-460, 552-560, 626-640, 751-769, 951-955).
+template <bool BATCH, int WPE, bool APPLY, bool ROUTE>
 __global__ void __launch_bounds__(TRACE_TPB) __attribute__((amdgpu_waves_per_eu(WPE, WPE))) trace_kernel(TraceCfg k) {
     const int lane = threadIdx.x & 63;
     constexpr bool apply = APPLY;
@@ -463,7 +477,7 @@ __global__ void __launch_bounds__(TRACE_TPB) __attribute__((amdgpu_waves_per_eu(
     bool alive = false, exhausted = false;
     int f = 0, ix = 0, iy = 0;
     float fx = 0, fy = 0, vol = 0, sed = 0, sa = 0, sb = 0, h_prev = 0;
-    bool has_prev = false, in_sea = false;
+    bool has_prev = false, in_sea = false, in_lake = false;
     int step = 0, uphill = 0, sea_steps = 0, pushes = 0, last_cell = -1;
     ll real = 0, deficit = 0;  // load actually held (fixed point) and load cancelled by erosion caps
     ll surplus = 0;            // deposits a ceiling refused, offered to the droplet's next deposits
@@ -489,6 +503,8 @@ __global__ void __launch_bounds__(TRACE_TPB) __attribute__((amdgpu_waves_per_eu(
         h.g0 = k.geo[2 * h.cell];
         h.g1 = k.geo[2 * h.cell + 1];
         h.rs = apply ? k.rscale[h.cell] : 1.0f;
+        if constexpr (ROUTE) h.rte = k.rte[h.cell];
+        else h.rte = make_float2(0.0f, 0.0f);
     };
     // deposit request `d` (fixed point) at `cell`: with the surplus of earlier
     // refused deposits, less the deficit, at most `lim` (particle.py:969-1002)
@@ -583,9 +599,13 @@ __global__ void __launch_bounds__(TRACE_TPB) __attribute__((amdgpu_waves_per_eu(
                         sb = k.d_sb[d];
                         has_prev = false;
                         in_sea = false;
+                        in_lake = false;
                         step = uphill = sea_steps = pushes = 0;
                         last_cell = -1;
-                        S = sample_of(load_stencil(k.cell4 + coff, NE, k.H, f, ix, iy, fx, fy));
+                        if constexpr (ROUTE)
+                            S = sample_route(load_stencil(k.cell4 + coff, NE, k.H, f, ix, iy, fx, fy), k.rte + coff, f, NE);
+                        else
+                            S = sample_of(load_stencil(k.cell4 + coff, NE, k.H, f, ix, iy, fx, fy));
                         load_here(f, ix, iy, cur, nullptr);
                         alive = true;
                     }
@@ -633,7 +653,14 @@ __global__ void __launch_bounds__(TRACE_TPB) __attribute__((amdgpu_waves_per_eu(
                         const int nm = k.mask[ncell];
                         const float2 ndy = k.dyn[ncell];
                         const float4 nrec = corner_of(S1, nei, nej);
-                        const Samp P1 = sample_of(S1);
+                        Samp P1;
+                        float nflag = 0.0f;  // the route flag of the cell stepped to
+                        if constexpr (ROUTE) {
+                            P1 = sample_route(S1, k.rte + coff, nf, NE);
+                            nflag = k.rte[ncell].y;
+                        } else {
+                            P1 = sample_of(S1);
+                        }
                         Here nxt;
                         const bool same = nix >= 0 && nix < N && niy >= 0 && niy < N;
                         load_here(nf, nix, niy, nxt, same ? &nrec : nullptr);
@@ -676,6 +703,12 @@ __global__ void __launch_bounds__(TRACE_TPB) __attribute__((amdgpu_waves_per_eu(
                                     k_e = k.rate * (w_cover + (1.0f - w_cover) * (1.0f - g1.w));
                                 }
                                 float cdiff = k_e * (c_eq - sed);
+                                if constexpr (ROUTE) {
+                                    // crossing a lake: no exchange with the bed; entering one: the load
+                                    // drops at the shore (particle.py:626-640)
+                                    if (in_lake) cdiff = 0.0f;
+                                    else if (sed > 0.0f && nflag >= 1.5f) cdiff = fminf(cdiff, -k.lake_trap * sed);
+                                }
                                 const float kexp = 1.0f + q * conc_scale;
                                 if (cdiff > 0.0f) {
                                     float ecap = fminf(dh / kexp, k.max_erode);
@@ -699,6 +732,8 @@ __global__ void __launch_bounds__(TRACE_TPB) __attribute__((amdgpu_waves_per_eu(
                                     // an entry never takes the cell below its base level nor below the
                                     // cell it steps to (frozen surfaces; particle.py:951-962)
                                     float fl = h_here >= b_here ? b_here : h_here;
+                                    // a lake bed is never eroded below its water level (particle.py:952-955)
+                                    if (ROUTE && cur.rte.y >= 1.5f) fl = fmaxf(fl, cur.rte.x);
                                     fl = fmaxf(fl, nrec.x);
                                     const float room = h_here - fl;
                                     rroom = room;
@@ -754,9 +789,11 @@ __global__ void __launch_bounds__(TRACE_TPB) __attribute__((amdgpu_waves_per_eu(
                             iy = niy;
                             fx = nfx;
                             fy = nfy;
+                            const float dhr = S.hr - P1.hr;  // the drop on the routing surface
                             S = P1;
                             cur = nxt;
                             h_prev = h_here;
+                            if constexpr (ROUTE) in_lake = nflag >= 1.5f;
                             has_prev = true;
                             n_moves += 1;
                             step += 1;
@@ -766,7 +803,7 @@ __global__ void __launch_bounds__(TRACE_TPB) __attribute__((amdgpu_waves_per_eu(
                             }
                             if (cause < 0 && vol < min_volume) cause = in_sea ? DEATH_OCEAN : DEATH_EVAP;
                             if (cause < 0) {
-                                if (h0 - h1 < 0.0f) {
+                                if (dhr < 0.0f && !in_lake) {
                                     uphill += 1;
                                     if (uphill > k.pit_steps) cause = in_sea ? DEATH_OCEAN : DEATH_PIT;
                                 } else {
@@ -853,22 +890,30 @@ constexpr int trace_waves(int variant) {
     return (APPLY ? 3 : 4) + variant;
 }
 
-// the trace kernel of a register-budget variant and pass
+// the trace kernel of a register-budget variant and pass (the persistent
+// grid is sized on the kernel without the routing surface, for both)
 template <bool BATCH, bool APPLY>
 const void* trace_fn(int64_t variant) {
-    if (variant == 1) return (const void*)trace_kernel<BATCH, trace_waves<APPLY>(1), APPLY>;
-    if (variant == 2) return (const void*)trace_kernel<BATCH, trace_waves<APPLY>(2), APPLY>;
-    return (const void*)trace_kernel<BATCH, trace_waves<APPLY>(0), APPLY>;
+    if (variant == 1) return (const void*)trace_kernel<BATCH, trace_waves<APPLY>(1), APPLY, false>;
+    if (variant == 2) return (const void*)trace_kernel<BATCH, trace_waves<APPLY>(2), APPLY, false>;
+    return (const void*)trace_kernel<BATCH, trace_waves<APPLY>(0), APPLY, false>;
 }
 
+template <bool BATCH, bool APPLY, bool ROUTE>
+void launch_trace_v(int64_t variant, int grid, hipStream_t s, const TraceCfg& k) {
+    if (variant == 1)
+        hipLaunchKernelGGL((trace_kernel<BATCH, trace_waves<APPLY>(1), APPLY, ROUTE>), dim3(grid), dim3(TRACE_TPB), 0, s, k);
+    else if (variant == 2)
+        hipLaunchKernelGGL((trace_kernel<BATCH, trace_waves<APPLY>(2), APPLY, ROUTE>), dim3(grid), dim3(TRACE_TPB), 0, s, k);
+    else
+        hipLaunchKernelGGL((trace_kernel<BATCH, trace_waves<APPLY>(0), APPLY, ROUTE>), dim3(grid), dim3(TRACE_TPB), 0, s, k);
+}
+
+// the routing-surface kernels when the launch carries a route (k.rte)
 template <bool BATCH, bool APPLY>
 void launch_trace(int64_t variant, int grid, hipStream_t s, const TraceCfg& k) {
-    if (variant == 1)
-        hipLaunchKernelGGL((trace_kernel<BATCH, trace_waves<APPLY>(1), APPLY>), dim3(grid), dim3(TRACE_TPB), 0, s, k);
-    else if (variant == 2)
-        hipLaunchKernelGGL((trace_kernel<BATCH, trace_waves<APPLY>(2), APPLY>), dim3(grid), dim3(TRACE_TPB), 0, s, k);
-    else
-        hipLaunchKernelGGL((trace_kernel<BATCH, trace_waves<APPLY>(0), APPLY>), dim3(grid), dim3(TRACE_TPB), 0, s, k);
+    if (k.rte) launch_trace_v<BATCH, APPLY, true>(variant, grid, s, k);
+    else launch_trace_v<BATCH, APPLY, false>(variant, grid, s, k);
 }
 
 // ---------------------------------------------------------------------------
@@ -1005,11 +1050,12 @@ __global__ void __launch_bounds__(TRACE_TPB) replay_kernel(TraceCfg k) {
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ float erosion_scale(int64_t c, const double* height, const double* sediment,
                                                const double* base, const ll* acc_fx, ll e, double fl_next,
-                                               double iter_erode, const double* ecap, double inv) {
+                                               double iter_erode, const double* ecap, double inv, const float2* rte) {
     const double acc = (double)acc_fx[c] * inv;
     const double s_c = height[c] + sediment[c] + acc;
     const double b = base[c];
     double fl = s_c >= b ? b : s_c;
+    if (rte && rte[c].y >= 1.5f) fl = fmax(fl, (double)rte[c].x);  // a lake bed stays under its water
     fl = fmax(fl, fl_next);
     double room = iter_erode + acc;
     if (ecap) {  // the slope limit: at most erode_cap + this iteration's net change (particle.py:944-947)
@@ -1025,14 +1071,14 @@ __device__ __forceinline__ float erosion_scale(int64_t c, const double* height, 
 // sweep 1: floor = the lowest surface stepped to (frozen)
 __global__ void erosion_scale_kernel(int64_t n, const double* height, const double* sediment, const double* base,
                                      const ll* ero_fx, const ull* floor_key, const ll* acc_fx, float* r1,
-                                     double iter_erode, const double* ecap, double hscale) {
+                                     double iter_erode, const double* ecap, double hscale, const float2* rte) {
     const double inv = 1.0 / hscale;
     GRID_STRIDE(c, n) {
         const ll e = ero_fx[c];
         const ull fk = floor_key[c];
         r1[c] = e > 0 ? erosion_scale(c, height, sediment, base, acc_fx, e,
                                       fk != ~0ull ? (double)fkey_value((unsigned)(fk >> 32)) : -1.0e300, iter_erode, ecap,
-                                      inv)
+                                      inv, rte)
                       : 1.0f;
     }
 }
@@ -1040,7 +1086,8 @@ __global__ void erosion_scale_kernel(int64_t n, const double* height, const doub
 // sweep 2: that floor lowered by what sweep 1 lets the floor cell lose
 __global__ void erosion_scale2_kernel(int64_t n, const double* height, const double* sediment, const double* base,
                                       const ll* ero_fx, const ull* floor_key, const ll* acc_fx, const float* r1,
-                                      float* rscale, double iter_erode, const double* ecap, double hscale) {
+                                      float* rscale, double iter_erode, const double* ecap, double hscale,
+                                      const float2* rte) {
     const double inv = 1.0 / hscale;
     GRID_STRIDE(c, n) {
         const ll e = ero_fx[c];
@@ -1052,7 +1099,7 @@ __global__ void erosion_scale2_kernel(int64_t n, const double* height, const dou
                 const int64_t nc = (int64_t)(unsigned)(fk & 0xffffffffull);
                 fl_next = (double)fkey_value((unsigned)(fk >> 32)) - (double)ero_fx[nc] * inv * (double)r1[nc];
             }
-            r = erosion_scale(c, height, sediment, base, acc_fx, e, fl_next, iter_erode, ecap, inv);
+            r = erosion_scale(c, height, sediment, base, acc_fx, e, fl_next, iter_erode, ecap, inv, rte);
         }
         rscale[c] = r;
     }
@@ -1063,7 +1110,8 @@ __global__ void erosion_scale2_kernel(int64_t n, const double* height, const dou
 // ---------------------------------------------------------------------------
 __global__ void resolve_kernel(int64_t n, const double* height, const double* sediment, const double* base, ll* chunk_fx,
                                ll* acc_fx, ll* pend_fx, float4* cell4, float2* dyn, double iter_deposit,
-                               double dep_floor, double writeoff, double hscale, ll* stats, ll* ero_fx, ull* floor_key) {
+                               double dep_floor, double writeoff, double hscale, ll* stats, ll* ero_fx, ull* floor_key,
+                               float2* rte) {
     const double inv = 1.0 / hscale;
     ll clamped = 0, to_pend = 0, lost_off = 0;
     GRID_STRIDE(c, n) {
@@ -1101,6 +1149,10 @@ __global__ void resolve_kernel(int64_t n, const double* height, const double* se
         const double sd = sediment[c] + (double)a2 * inv;
         cell4[c].x = (float)live;
         dyn[c].x = (float)(sd > 0.0 ? sd : 0.0);
+        if (rte && rte[c].y < 1.5f) {
+            // the route flag follows the live surface; a lake cell stays one (particle.py:1010-1012)
+            rte[c].y = rte[c].x != (float)live ? 1.0f : 0.0f;
+        }
     }
     clamped = wave_sum(clamped);
     to_pend = wave_sum(to_pend);
@@ -1255,6 +1307,7 @@ int layout_of(int64_t F, int64_t NE, int64_t P, Layout* L) {
     L->cell4 = take(16 * n);
     L->geo = take(32 * n);
     L->dyn = take(8 * n);
+    L->rte = take(8 * n);
     L->chunk_fx = take(8 * n);
     L->acc_fx = take(8 * n);
     L->dis_fx = take(8 * n);
@@ -1467,7 +1520,8 @@ int fast_spawn(const pg_fast_fields* fl, const pg_fast_params* fp, void* work, C
     const int chunks = c.chunks;
     hipLaunchKernelGGL(reset_scalars_kernel, dim3(1), dim3(256), 0, s, sc, st, (int)PG_FAST_NSTATS, chunks);
     hipLaunchKernelGGL(pack_kernel, dim3(grid_for(n)), dim3(TPB), 0, s, n, N, H, fl->height, fl->sediment, fl->discharge,
-                       fl->momentum, fl->base, fl->precip, fl->mask, cell4, dyn, flag, sc);
+                       fl->momentum, fl->base, fl->precip, fl->mask, cell4, dyn, flag, sc, fp->route, fp->lake_flag,
+                       at<float2>(work, L.rte));
     hipLaunchKernelGGL(quantize_kernel, dim3(grid_for(n)), dim3(TPB), 0, s, n, fl->precip, flag, sc, at<ll>(work, L.wq));
     PG_LAUNCHED("pack/quantize");
     PG_HIP(hipcub::DeviceScan::InclusiveSum(temp, tb, at<ll>(work, L.wq), at<ll>(work, L.cdf_fx), (int)n, s));
@@ -1577,6 +1631,8 @@ int fast_particles(const pg_fast_fields* fl, const pg_trace_params* prm, const p
     PG_REQUIRE(fp->variant >= 0 && fp->variant <= 2, "fast mode: variant must be 0, 1 or 2");
     if (persistent_grid(trace_fn<false, false>(fp->variant), TRACE_TPB, fp->grid_blocks, &grid)) return -1;
     if (fp->tape && persistent_grid((const void*)replay_kernel<false>, TRACE_TPB, 0, &rgrid)) return -1;
+    k.rte = fp->route ? at<float2>(work, L.rte) : nullptr;
+    k.lake_trap = (float)prm->lake_trap;
     k.tape = nullptr;
     k.tape_len = nullptr;
     k.tape_rows = 0;
@@ -1604,10 +1660,10 @@ int fast_particles(const pg_fast_fields* fl, const pg_trace_params* prm, const p
         if (fp->time_trace && timer_record(s, false, k.pass)) return -1;
         hipLaunchKernelGGL(erosion_scale_kernel, dim3(grid_for(n)), dim3(TPB), 0, s, n, fl->height, fl->sediment, fl->base,
                            k.ero_fx, k.floor_key, at<ll>(work, L.acc_fx), at<float>(work, L.r1), fp->iter_erode,
-                           fp->erode_cap, hscale);
+                           fp->erode_cap, hscale, k.rte);
         hipLaunchKernelGGL(erosion_scale2_kernel, dim3(grid_for(n)), dim3(TPB), 0, s, n, fl->height, fl->sediment,
                            fl->base, k.ero_fx, k.floor_key, at<ll>(work, L.acc_fx), at<float>(work, L.r1),
-                           at<float>(work, L.rscale), fp->iter_erode, fp->erode_cap, hscale);
+                           at<float>(work, L.rscale), fp->iter_erode, fp->erode_cap, hscale, k.rte);
         // pass 1: the same trajectories, erosion scaled per cell, deficits cancel deposits
         k.pass = 1;
         if (fp->time_trace && timer_record(s, true, k.pass)) return -1;
@@ -1616,7 +1672,7 @@ int fast_particles(const pg_fast_fields* fl, const pg_trace_params* prm, const p
         if (fp->time_trace && timer_record(s, false, k.pass)) return -1;
         hipLaunchKernelGGL(resolve_kernel, dim3(grid_for(n)), dim3(TPB), 0, s, n, fl->height, fl->sediment, fl->base,
                            k.chunk_fx, at<ll>(work, L.acc_fx), pend_fx, cell4, dyn, fp->iter_deposit, prm->dep_floor,
-                           fp->offshore_writeoff, hscale, st, k.ero_fx, k.floor_key);
+                           fp->offshore_writeoff, hscale, st, k.ero_fx, k.floor_key, k.rte);
     }
     PG_LAUNCHED("trace/resolve");
     return finish_iteration(fl, fp, work, c);
@@ -1732,6 +1788,7 @@ int batch_layout(int64_t W, const int64_t* NE, const int64_t* P, BLayout* B) {
     L.cell4 = take(16 * n);
     L.geo = take(32 * n);
     L.dyn = take(8 * n);
+    L.rte = take(8 * n);
     L.chunk_fx = take(8 * n);
     L.acc_fx = take(8 * n);
     L.dis_fx = take(8 * n);
@@ -1775,6 +1832,7 @@ void window_layout(const BLayout& B, int64_t coff, int64_t poff, int64_t doff, i
     Lw->cell4 = L.cell4 + 16 * c;
     Lw->geo = L.geo + 32 * c;
     Lw->dyn = L.dyn + 8 * c;
+    Lw->rte = L.rte + 8 * c;
     Lw->chunk_fx = L.chunk_fx + 8 * c;
     Lw->acc_fx = L.acc_fx + 8 * c;
     Lw->dis_fx = L.dis_fx + 8 * c;
@@ -2009,6 +2067,12 @@ extern "C" int pg_batch_iterate(const pg_fast_fields* wins, int64_t n_win, const
     PG_REQUIRE(f0.variant >= 0 && f0.variant <= 2, "pg_batch_iterate: variant must be 0, 1 or 2");
     PG_REQUIRE(!f0.erode_cap, "pg_batch_iterate: the slope limit (erode_cap) is not supported on batched windows");
     if (persistent_grid(trace_fn<true, false>(f0.variant), TRACE_TPB, f0.grid_blocks, &grid)) return -1;
+    // the routing surface: every window's or none
+    for (int64_t w = 0; w < n_win; w++)
+        PG_REQUIRE((fps[w].route != nullptr) == (f0.route != nullptr),
+                   "pg_batch_iterate: either every window has a routing surface or none");
+    k.rte = f0.route ? at<float2>(work, L.rte) : nullptr;
+    k.lake_trap = (float)prm->lake_trap;
     // the move tape: prm->max_steps bounds every window's lifetime
     k.max_steps = (int)prm->max_steps;
     if (f0.tape) {
@@ -2035,10 +2099,10 @@ extern "C" int pg_batch_iterate(const pg_fast_fields* wins, int64_t n_win, const
         if (f0.time_trace && timer_record(s, false, k.pass)) return -1;
         hipLaunchKernelGGL(erosion_scale_kernel, dim3(grid_for(n)), dim3(TPB), 0, s, n, a.height, a.sediment, a.base,
                            k.ero_fx, k.floor_key, at<ll>(work, L.acc_fx), at<float>(work, L.r1), f0.iter_erode,
-                           (const double*)nullptr, hscale);
+                           (const double*)nullptr, hscale, k.rte);
         hipLaunchKernelGGL(erosion_scale2_kernel, dim3(grid_for(n)), dim3(TPB), 0, s, n, a.height, a.sediment, a.base,
                            k.ero_fx, k.floor_key, at<ll>(work, L.acc_fx), at<float>(work, L.r1),
-                           at<float>(work, L.rscale), f0.iter_erode, (const double*)nullptr, hscale);
+                           at<float>(work, L.rscale), f0.iter_erode, (const double*)nullptr, hscale, k.rte);
         k.pass = 1;
         if (f0.time_trace && timer_record(s, true, k.pass)) return -1;
         if (k.tape) hipLaunchKernelGGL(replay_kernel<true>, dim3(rgrid), dim3(TRACE_TPB), 0, s, k);
@@ -2047,7 +2111,7 @@ extern "C" int pg_batch_iterate(const pg_fast_fields* wins, int64_t n_win, const
         hipLaunchKernelGGL(resolve_kernel, dim3(grid_for(n)), dim3(TPB), 0, s, n, a.height, a.sediment, a.base,
                            k.chunk_fx, at<ll>(work, L.acc_fx), at<ll>(work, L.pend_fx), at<float4>(work, L.cell4),
                            at<float2>(work, L.dyn), f0.iter_deposit, prm->dep_floor, f0.offshore_writeoff, hscale,
-                           (ll*)stats, k.ero_fx, k.floor_key);
+                           (ll*)stats, k.ero_fx, k.floor_key, k.rte);
     }
     PG_LAUNCHED("batch trace/resolve");
     hipLaunchKernelGGL(finish_kernel, dim3(grid_for(n)), dim3(TPB), 0, s, n, a.height, a.sediment, a.discharge,

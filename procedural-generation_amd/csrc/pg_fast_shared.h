@@ -39,7 +39,7 @@ struct Scalars {
 // workspace layout
 // ---------------------------------------------------------------------------
 struct Layout {
-    size_t cell4, geo, dyn, chunk_fx, acc_fx, dis_fx, mom_fx, pend_fx, ero_fx, floor_key, rscale, r1, wq, cdf_fx, flag,
+    size_t cell4, geo, dyn, rte, chunk_fx, acc_fx, dis_fx, mom_fx, pend_fx, ero_fx, floor_key, rscale, r1, wq, cdf_fx, flag,
         rank, comp_idx, comp_cdf,
         draws, u_sorted, d_cell, d_fx, d_fy, d_load, d_sa, d_sb, scalars, temp, total;
     size_t temp_bytes;
@@ -128,6 +128,7 @@ __device__ __forceinline__ float4 corner_of(const Stencil& s, int ei, int ej) {
 // the stencil is read, so a droplet carries 6 registers instead of 16
 struct Samp {
     float h, gi, gj, q, ma, mb;
+    float hr;  // the routing height (= h without a routing surface)
 };
 
 __device__ __forceinline__ Samp sample_of(const Stencil& S) {
@@ -139,6 +140,30 @@ __device__ __forceinline__ Samp sample_of(const Stencil& S) {
     p.q = fmaxf(lerp2(S.c00.y, S.c10.y, S.c01.y, S.c11.y, wi, wj), 0.0f);
     p.ma = lerp2(S.c00.z, S.c10.z, S.c01.z, S.c11.z, wi, wj);
     p.mb = lerp2(S.c00.w, S.c10.w, S.c01.w, S.c11.w, wi, wj);
+    p.hr = p.h;
+    return p;
+}
+
+// the sample with the routing surface (particle.py:450-460, 552-560): where
+// any of the stencil's four cells has a route flag, the bilinear route and
+// its gradient; above the terrain they steer (gi, gj) and set the routing
+// height hr the drop dhr is taken on, while h stays the terrain's.  rte:
+// per extended cell {route, flag} (flag 0 none, 1 route above terrain, 2 a
+// lake cell)
+__device__ __forceinline__ Samp sample_route(const Stencil& S, const float2* rte, int64_t f, int NE) {
+    Samp p = sample_of(S);
+    const int64_t b = ((int64_t)f * NE + S.i0) * NE + S.j0;
+    const float2 r00 = rte[b], r01 = rte[b + 1], r10 = rte[b + NE], r11 = rte[b + NE + 1];
+    p.hr = p.h;
+    if (r00.y + r10.y + r01.y + r11.y != 0.0f) {
+        const float wi = S.wi, wj = S.wj;
+        const float r0 = lerp2(r00.x, r10.x, r01.x, r11.x, wi, wj);
+        if (r0 > p.h) {
+            p.hr = r0;
+            p.gi = (r10.x - r00.x) * (1.0f - wj) + (r11.x - r01.x) * wj;
+            p.gj = (r01.x - r00.x) * (1.0f - wi) + (r11.x - r10.x) * wi;
+        }
+    }
     return p;
 }
 

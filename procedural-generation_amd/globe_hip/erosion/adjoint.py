@@ -84,6 +84,8 @@ class GradRun:
         require_plain_side_passes(ep, "the differentiable mode")
         if float(getattr(ep, "lateral_rate", 0.0)) > 0.0 or state.roots is not None:
             raise NotImplementedError("lateral cuts and roots are outside the differentiable mode")
+        if state.route is not None:
+            raise NotImplementedError("the routing surface is outside the differentiable mode (it steers on the terrain)")
         if float(ep.disc_saturation_cells) > 0.0 or float(ep.momentum_saturation_cells) > 0.0:
             raise NotImplementedError("cell-count discharge scales are outside the differentiable mode")
         if float(state.pending.abs().max().item()) != 0.0:

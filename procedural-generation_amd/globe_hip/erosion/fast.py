@@ -159,7 +159,9 @@ def run_iteration(state, params, iteration_key, n_particles=None, particles_per_
                     variant=int(getattr(state, "fast_variant", 0)),
                     rng_dev=(state._fast_rng_dev.data_ptr() if getattr(state, "_fast_rng_dev", None) is not None else None),
                     erode_cap=None if ecap is None else ecap.data_ptr(),
-                    tape=None if tape is None else tape.data_ptr(), tape_bytes=0 if tape is None else tape.numel())
+                    tape=None if tape is None else tape.data_ptr(), tape_bytes=0 if tape is None else tape.numel(),
+                    route=None if state.route is None else state.route.data_ptr(),
+                    lake_flag=None if state.route is None or state.lake_flag is None else state.lake_flag.data_ptr())
     check(lib().pg_fast_iterate(ctypes.byref(ctx.fields), ctypes.byref(prm), ctypes.byref(fp),
                                 ctypes.c_void_p(ctx.work.data_ptr()), ctx.work.numel(),
                                 ctypes.c_void_p(ctx.stats.data_ptr()), pk._stream()), "pg_fast_iterate")
@@ -206,7 +208,8 @@ class IterationGraph:
     uplift, isostasy, datum hold, halo exchange (bake/globe/erosion/maps.py:
     1516-1617) -- captured once as a HIP graph and replayed per iteration
     (two graphs when isostasy is on: with and without its application).  The
-    sea mask, when due, runs eagerly before the replay.
+    sea mask, the routing flood and a window's lakes, when due, run eagerly
+    before the replay, in place.
 
     The only per-iteration input from the host is the iteration's PCG64
     state (``params.rng(stage, *key)``), copied into a device buffer before
@@ -238,6 +241,9 @@ class IterationGraph:
         if state.spherical:
             state.halo_tables()
         maps.prepare_side_passes(state, params)
+        # the routing surface and lake flags exist (their buffers) before the capture
+        if int(ep.flood_every) > 0:
+            maps.refresh_route_and_lakes(state, params, due=True)
         torch.cuda.synchronize()
         self._load_rng((0,))
         # with isostasy on, a second graph holds the iterations that apply it
@@ -246,14 +252,15 @@ class IterationGraph:
         it0 = state.iteration
         self.graph = torch.cuda.CUDAGraph()
         with torch.cuda.graph(self.graph):
-            self.result = maps.step(state, params, (0,), n_particles=n_particles, isostasy_due=False, sea_due=False)
+            self.result = maps.step(state, params, (0,), n_particles=n_particles, isostasy_due=False, sea_due=False,
+                                    route_due=False)
         self.graph_iso = None
         if self.iso_every:
             state.iteration = it0
             self.graph_iso = torch.cuda.CUDAGraph()
             with torch.cuda.graph(self.graph_iso):
                 self.result_iso = maps.step(state, params, (0,), n_particles=n_particles, isostasy_due=True,
-                                            sea_due=False)
+                                            sea_due=False, route_due=False)
         state.iteration = it0
         self.stats = ctx.stats
 
@@ -267,9 +274,13 @@ class IterationGraph:
         from . import maps
 
         key = iteration_key if isinstance(iteration_key, (tuple, list)) else (int(iteration_key),)
-        if maps.sea_refresh_due(self.state, maps._eparams(self.params)):
+        ep = maps._eparams(self.params)
+        if maps.sea_refresh_due(self.state, ep):
             # host-synchronous, so outside the graph: the replay reads the new base
             self.state.refresh_base(maps.ocean_min_fraction(self.params))
+        if maps.route_refresh_due(self.state, ep) or maps.lakes_refresh_due(self.state, ep):
+            # the flood and the lakes likewise, in place into the buffers the graph reads
+            maps.refresh_route_and_lakes(self.state, self.params)
         self._load_rng(key)
         if self.graph_iso is not None and (self.state.iteration + 1) % self.iso_every == 0:
             self.graph_iso.replay()

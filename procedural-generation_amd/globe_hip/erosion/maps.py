@@ -479,14 +479,16 @@ class ErosionState:
         if float(lake_evap) > 0.0:
             raise NotImplementedError("the window lakes' evaporation balance (erosion.window_lake_evap) is outside "
                                       "the HIP erosion path")
-        flag = torch.zeros(self.height.shape, dtype=torch.uint8, device=self.device)
+        # written in place: a captured fast-mode graph keeps reading the same buffer
+        if self.lake_flag is None:
+            self.lake_flag = torch.zeros(self.height.shape, dtype=torch.uint8, device=self.device)
+        flag = self.lake_flag
+        flag.zero_()
         if self.route is None:
-            self.lake_flag = flag
             return {"lake_cells": 0}
         lake = ((self.route - (self.height + self.sediment)) > float(min_depth)) & (self.mask == pk.MASK_ACTIVE)
         inter = self.interior
         flag[inter] = lake[inter].to(torch.uint8)
-        self.lake_flag = flag
         return {"lake_cells": int(lake[inter].sum().item())}
 
 
@@ -541,9 +543,9 @@ def ocean_min_fraction(params) -> float:
 
 
 def _check_scope(ep, state):
-    """The fast mode's limits: it steers on the terrain (no routing surface)."""
-    if state.route is not None:
-        raise NotImplementedError("the routing surface (erosion.flood_every) runs in the check mode only")
+    """The limits shared by the fast and differentiable modes (none beyond
+    the ones each checks itself)."""
+    return None
 
 
 def require_plain_side_passes(ep, what: str) -> None:
@@ -975,14 +977,42 @@ def sea_refresh_due(state: ErosionState, ep) -> bool:
     return bool(state.spherical and sea_every > 0 and (state.base_at is None or state.iteration % sea_every == 0))
 
 
+def refresh_route_and_lakes(state: ErosionState, params, due=None) -> None:
+    """The routing flood (every ``flood_every``) and then a window's lakes
+    (maps.py:1548-1566); ``due`` overrides the strides (False: neither)."""
+    ep = _eparams(params)
+    if due is False:
+        return
+    lakes = lakes_refresh_due(state, ep) if due is None else (not state.spherical and bool(getattr(ep, "window_lakes", False)))
+    if route_refresh_due(state, ep) if due is None else int(ep.flood_every) > 0:
+        state.refresh_route(cell_units(ep, "route_eps", state.height_unit_m))
+    if lakes:
+        hydro = getattr(params, "hydro", None)
+        min_depth = float(hydro.lake_min_depth) if hydro is not None else 0.5
+        state.refresh_lakes_window(min_depth / float(state.height_unit_m), float(getattr(ep, "window_lake_evap", 0.0)))
+
+
+def route_refresh_due(state: ErosionState, ep) -> bool:
+    """Whether this iteration refreshes the routing surface (maps.py:1548-1549)."""
+    flood = int(ep.flood_every)
+    return bool(flood > 0 and (state.route is None or state.iteration % flood == 0))
+
+
+def lakes_refresh_due(state: ErosionState, ep) -> bool:
+    """Whether this iteration refreshes a window's lakes (maps.py:1557-1566)."""
+    return bool(not state.spherical and bool(getattr(ep, "window_lakes", False)) and int(ep.flood_every) > 0
+                and (state.lake_flag is None or state.iteration % int(ep.flood_every) == 0))
+
+
 def step(state: ErosionState, params, iteration_key, log=None, after_refresh=None, isostasy_due=None, sea_due=None,
-         **kw) -> dict:
+         route_due=None, **kw) -> dict:
     """One iteration in the reference's order (maps.py:1516-1617): the sea
     mask (every ``sea_mask_every``), the routing flood (every
     ``flood_every``) and a window's lakes, particles, mass wasting, uplift,
     isostasy (every ``isostasy_every`` iterations), datum hold (globe), halo
-    exchange.  ``isostasy_due`` and ``sea_due`` override the stride's choice
-    (graph capture: the sea mask is host-synchronous and runs outside the
+    exchange.  ``isostasy_due``, ``sea_due`` and ``route_due`` (the routing
+    flood and a window's lakes) override the stride's choice (graph capture:
+    the sea mask and the flood are host-synchronous and run outside the
     graph)."""
     ep = _eparams(params)
     if state.spherical and bool(getattr(ep, "lake_balance", False)) and int(ep.flood_every) > 0:
@@ -994,14 +1024,7 @@ def step(state: ErosionState, params, iteration_key, log=None, after_refresh=Non
     if sea_refresh_due(state, ep) if sea_due is None else sea_due:
         # before the route: the flood seeds on the sea (maps.py:1541-1547)
         st_base = state.refresh_base(ocean_min_fraction(params))
-    flood = int(ep.flood_every)
-    if flood > 0 and (state.route is None or state.iteration % flood == 0):
-        state.refresh_route(cell_units(ep, "route_eps", state.height_unit_m))
-    if not state.spherical and bool(getattr(ep, "window_lakes", False)) and flood > 0 and \
-            (state.lake_flag is None or state.iteration % flood == 0):
-        hydro = getattr(params, "hydro", None)
-        min_depth = float(hydro.lake_min_depth) if hydro is not None else 0.5
-        state.refresh_lakes_window(min_depth / float(state.height_unit_m), float(getattr(ep, "window_lake_evap", 0.0)))
+    refresh_route_and_lakes(state, params, route_due)
     if after_refresh is not None:
         after_refresh(state)
     tr = time.time() - t0

@@ -165,18 +165,20 @@ def _hypsometry(surf_land, lo, hi, bins=64):
     return np.cumsum(h) / max(h.sum(), 1)
 
 
-@pytest.mark.parametrize("chunks", [48, 16])
-def test_fast_tracks_the_oracle(oracle, cuda, chunks):
-    """20 iterations on the N=128 plain globe (the plain benchmark config).
+@pytest.mark.parametrize("chunks,flood_every", [(48, 0), (16, 0), (48, 5)])
+def test_fast_tracks_the_oracle(oracle, cuda, chunks, flood_every):
+    """20 iterations on the N=128 plain globe (the plain benchmark config),
+    and with the routing flood every 5 iterations (droplets steer on the
+    epsilon-filled surface in depressions, both sides).
 
     Bound asserted: land hypsometry (64-bin CDF) and eroded volume within
     2 %.  The stated target is 0.5 %; the order-independent apply does not
-    reach it (measured ~1 %, against a 0.22 % seed-to-seed noise floor of the
-    same metric): the reference applies each chunk serially, with caps that
-    depend on apply order."""
+    reach it (measured ~1.5 %, against a 0.22 % seed-to-seed noise floor of
+    the same metric): the reference applies each chunk serially, with caps
+    that depend on apply order."""
     N, iters = 128, 20
     grid, f = stubs.plain_globe(N, 2)
-    p = stubs.workload_params(N, 2)
+    p = stubs.workload_params(N, 2, flood_every=flood_every)
     ost = oracle.OracleState.from_grid(grid, f["bedrock_m"], f["hardness"], f["precip"], f["evap"], f["uplift_m"], p.erosion)
     dst = maps.ErosionState.from_grid(grid, f["bedrock_m"], f["hardness"], f["precip"], f["evap"], f["uplift_m"], p.erosion,
                                       device=cuda, mode="fast")

@@ -136,9 +136,17 @@ def test_globe_steps_with_flood_stay_close(oracle, cuda):
     assert np.abs(h_d[inter] - h_o[inter]).max() <= 1e-6 * np.abs(h_o[inter]).max()
 
 
-def test_fast_mode_refuses_the_route(cuda):
+def test_differentiable_mode_refuses_the_route(cuda):
+    """The fast mode steers on the route (tests/test_gpu_fast_route.py); the
+    differentiable mode, which replays frozen walks on the terrain, does not
+    take one."""
+    from globe_hip.erosion import adjoint
+
     p = window_params(0, flood_every=1)
     kw = window_inputs(32, "tilt", p, seed=1)
     st = maps.ErosionState.window(**kw, device=cuda, mode="fast")
+    maps.step(st, p, 0)
+    assert st.route is not None
+    p0 = window_params(0)
     with pytest.raises(NotImplementedError):
-        maps.step(st, p, 0)
+        adjoint.GradRun(st, p0, 2)
