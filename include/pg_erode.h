@@ -48,7 +48,7 @@ typedef struct pg_trace_params {
 
 /* Message of the last failed call on this thread ("" if none). */
 const char* pg_last_error(void);
-/* ABI version of this library: 4 (bumped whenever a struct layout or a
+/* ABI version of this library: 5 (bumped whenever a struct layout or a
  * signature changes; 2 added pg_fast_params.erode_cap, 3 its tape). */
 int pg_abi_version(void);
 
@@ -315,6 +315,10 @@ typedef struct pg_fast_params {
                                 * droplets steer on it where it stands above the terrain and cross lakes;
                                 * NULL = steer on the terrain */
     const uint8_t* lake_flag;  /* optional DEVICE lake flags (F, NE, NE) with the route (maps.py:629-672) */
+    /* the discharge scales are pg_trace_params' disc_saturation and mom_scale, or, with
+     * disc_saturation_cells / momentum_saturation_cells > 0, in cells of upstream rain, computed on the
+     * device from the iteration's spawn (maps.py:911-921; inflow_volume = ErosionState.inflow_volume) */
+    double disc_saturation_cells, momentum_saturation_cells, inflow_volume;
 } pg_fast_params;
 
 /* Bytes of the move tape for n_particles rain droplets on F faces of NE x NE

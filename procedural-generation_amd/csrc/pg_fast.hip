@@ -241,11 +241,14 @@ __device__ double pairwise_sum(const float* w, const int* idx, int64_t n) {
 // spawn totals; if the fixed-point sum is not exact in float64, the serial
 // path reproduces numpy's cumsum (sequential) and sum (pairwise) instead
 __global__ void spawn_totals_kernel(int64_t n, const int* rank, const ll* cdf_fx, const int* comp_idx, double* comp_cdf,
-                                    const float* precip, ll n_particles, Scalars* sc, ll* stats) {
+                                    const float* precip, ll n_particles, Scalars* sc, ll* stats, float inv_sat,
+                                    float mom_scale, double disc_cells, double mom_cells, double inflow) {
     if (threadIdx.x != 0 || blockIdx.x != 0) return;
     const ll M = rank[n - 1];
     sc->eligible = M;
     stats[PG_FS_ELIGIBLE] = M;
+    sc->inv_sat = inv_sat;
+    sc->mom_scale = mom_scale;
     if (M == 0) {
         sc->volume0 = 0.0;
         return;
@@ -269,6 +272,18 @@ __global__ void spawn_totals_kernel(int64_t n, const int* rank, const ll* cdf_fx
     sc->volume0 = total / (double)n_particles;
     stats[PG_FS_VOLUME0_BITS] = __double_as_longlong(sc->volume0);
     stats[PG_FS_RAIN] = n_particles;
+    if (disc_cells > 0.0 || mom_cells > 0.0) {
+        // discharge scales in cells of upstream rain (maps.py:911-921): one cell's rain per iteration is
+        // the spawn's volume less the inflow, over the cells it falls on
+        const double vol = sc->volume0 * (double)n_particles;
+        double rpc = fmax(vol - inflow, 0.0) / (double)M;
+        if (!(rpc > 0.0)) rpc = vol / (double)M;
+        if (disc_cells > 0.0) {
+            const double ds = disc_cells * rpc;
+            sc->inv_sat = ds > 0.0 ? (float)(1.0 / ds) : 0.0f;
+        }
+        if (mom_cells > 0.0) sc->mom_scale = (float)(sc->volume0 / (mom_cells * rpc));
+    }
 }
 
 // rain droplets: the p-th smallest uniform picks the cell, the p-th offset
@@ -358,6 +373,7 @@ struct WinDesc {
     ll coff, dbase, n_total, C;
     int N, NE, max_steps, pad;
     float inv_v0, min_volume, conc_scale, v0f;
+    float inv_sat, mom_scale, pad2, pad3;
 };
 
 struct TraceCfg {
@@ -452,6 +468,7 @@ __global__ void __launch_bounds__(TRACE_TPB) __attribute__((amdgpu_waves_per_eu(
     int N = k.N, NE = k.NE, max_steps = k.max_steps;
     int64_t coff = 0;
     float inv_v0 = 0.0f, min_volume = 0.0f, conc_scale = 0.0f, v0f = 0.0f;
+    float inv_sat = 0.0f, mom_scale = 0.0f;  // the iteration's discharge scales (per window, batched)
     ll d_end, d_begin = 0;
     ll* queue;
     if constexpr (BATCH) {
@@ -470,6 +487,8 @@ __global__ void __launch_bounds__(TRACE_TPB) __attribute__((amdgpu_waves_per_eu(
         // droplets) unless none is given, then this kernel's own chunk
         conc_scale = (float)((k.conc_chunk > 0.0 ? k.conc_chunk : (double)C) / (volume0 * (double)n_total));
         v0f = (float)volume0;
+        inv_sat = k.sc->inv_sat;
+        mom_scale = k.sc->mom_scale;
         queue = &k.sc->queue[2 * k.chunk + k.pass];
     }
 
@@ -572,6 +591,8 @@ __global__ void __launch_bounds__(TRACE_TPB) __attribute__((amdgpu_waves_per_eu(
                             min_volume = wd.min_volume;
                             conc_scale = wd.conc_scale;
                             v0f = wd.v0f;
+                            inv_sat = wd.inv_sat;
+                            mom_scale = wd.mom_scale;
                             d = wd.dbase + (ll)k.chunk * wd.C + (u - pre[lo]);
                         }
                     }
@@ -642,7 +663,8 @@ __global__ void __launch_bounds__(TRACE_TPB) __attribute__((amdgpu_waves_per_eu(
                 int nf, nix, niy;
                 float nfx, nfy;
                 const float h0 = S.h;
-                cause = steer_move<true>(k, N, S, g0, g1, vol, vol_rel, sa, sb, f, ix, iy, fx, fy, nf, nix, niy, nfx, nfy);
+                cause = steer_move<true>(k, N, S, g0, g1, vol, vol_rel, mom_scale, sa, sb, f, ix, iy, fx, fy, nf, nix, niy,
+                                         nfx, nfy);
                 exited = cause == DEATH_EXIT;
                 {
                     if (cause < 0) {
@@ -691,10 +713,10 @@ __global__ void __launch_bounds__(TRACE_TPB) __attribute__((amdgpu_waves_per_eu(
                                 }
                             } else if (cur.m == MASK_ACTIVE) {
                                 float qe;
-                                if (k.inv_sat <= 0.0f) qe = 0.0f;
-                                else if (k.disc_exponent == 0.5f) qe = sqrtf(q * k.inv_sat);
-                                else if (k.disc_exponent > 0.0f) qe = powf(q * k.inv_sat, k.disc_exponent);
-                                else qe = erff(q * k.inv_sat);
+                                if (inv_sat <= 0.0f) qe = 0.0f;
+                                else if (k.disc_exponent == 0.5f) qe = sqrtf(q * inv_sat);
+                                else if (k.disc_exponent > 0.0f) qe = powf(q * inv_sat, k.disc_exponent);
+                                else qe = erff(q * inv_sat);
                                 const float c_eq = fmaxf(k.erodibility * dh * (1.0f + k.k_disc * qe), 0.0f);
                                 float k_e = k.rate;
                                 if (c_eq > sed) {
@@ -1240,6 +1262,9 @@ __global__ void batch_plan_kernel(int W, int chunks, const WinStatic* ws, const 
                                              (volume0 * (double)n_total))
                                    : 0.0f;
         d.v0f = (float)volume0;
+        d.inv_sat = sc->inv_sat;
+        d.mom_scale = sc->mom_scale;
+        d.pad2 = d.pad3 = 0.0f;
         wd[w] = d;
     }
     for (int k = threadIdx.x; k < 2 * chunks; k += blockDim.x) queue[k] = 0;
@@ -1505,7 +1530,7 @@ int prepare(const pg_fast_fields* fl, const pg_trace_params* prm, const pg_fast_
     return 0;
 }
 
-int fast_spawn(const pg_fast_fields* fl, const pg_fast_params* fp, void* work, Ctx& c) {
+int fast_spawn(const pg_fast_fields* fl, const pg_trace_params* prm, const pg_fast_params* fp, void* work, Ctx& c) {
     const int64_t N = c.N, H = c.H, n = c.n;
     const ll P = c.P;
     hipStream_t s = c.s;
@@ -1530,7 +1555,9 @@ int fast_spawn(const pg_fast_fields* fl, const pg_fast_params* fp, void* work, C
     hipLaunchKernelGGL(compact_kernel, dim3(grid_for(n)), dim3(TPB), 0, s, n, flag, rank, at<ll>(work, L.cdf_fx), sc,
                        at<int>(work, L.comp_idx), at<double>(work, L.comp_cdf));
     hipLaunchKernelGGL(spawn_totals_kernel, dim3(1), dim3(64), 0, s, n, rank, at<ll>(work, L.cdf_fx),
-                       at<int>(work, L.comp_idx), at<double>(work, L.comp_cdf), fl->precip, P, sc, st);
+                       at<int>(work, L.comp_idx), at<double>(work, L.comp_cdf), fl->precip, P, sc, st,
+                       (float)(prm->disc_saturation > 0.0 ? 1.0 / prm->disc_saturation : 0.0), (float)prm->mom_scale,
+                       fp->disc_saturation_cells, fp->momentum_saturation_cells, fp->inflow_volume);
     // the rain draws: P uniforms (sorted) then P offset pairs
     double* draws = at<double>(work, L.draws);
     hipLaunchKernelGGL(pcg_fill_kernel, dim3(grid_for((3 * P + 15) / 16, 8192)), dim3(TPB), 0, s, fp->rng_state_lo,
@@ -1693,7 +1720,7 @@ extern "C" int pg_fast_spawn(const pg_fast_fields* fl, const pg_trace_params* pr
     clear_error();
     Ctx c;
     if (prepare(fl, prm, fp, work, work_bytes, stats, stream, &c)) return -1;
-    return fast_spawn(fl, fp, work, c);
+    return fast_spawn(fl, prm, fp, work, c);
 }
 
 extern "C" int pg_fast_particles(const pg_fast_fields* fl, const pg_trace_params* prm, const pg_fast_params* fp,
@@ -1709,7 +1736,7 @@ extern "C" int pg_fast_iterate(const pg_fast_fields* fl, const pg_trace_params* 
     clear_error();
     Ctx c;
     if (prepare(fl, prm, fp, work, work_bytes, stats, stream, &c)) return -1;
-    if (fast_spawn(fl, fp, work, c)) return -1;
+    if (fast_spawn(fl, prm, fp, work, c)) return -1;
     return fast_particles(fl, prm, fp, work, c);
 }
 
@@ -1999,7 +2026,7 @@ extern "C" int pg_batch_iterate(const pg_fast_fields* wins, int64_t n_win, const
         c.NE = NE[w];
         c.n = NE[w] * NE[w];
         c.P = P[w];
-        if (fast_spawn(&wins[w], &fps[w], work, c)) return -1;
+        if (fast_spawn(&wins[w], prm, &fps[w], work, c)) return -1;
         poff += P[w];
         doff += P[w] + NE[w] * NE[w];
     }
@@ -2065,7 +2092,6 @@ extern "C" int pg_batch_iterate(const pg_fast_fields* wins, int64_t n_win, const
     k.queue_base = qbase;
     int grid = 0, rgrid = 0;
     PG_REQUIRE(f0.variant >= 0 && f0.variant <= 2, "pg_batch_iterate: variant must be 0, 1 or 2");
-    PG_REQUIRE(!f0.erode_cap, "pg_batch_iterate: the slope limit (erode_cap) is not supported on batched windows");
     if (persistent_grid(trace_fn<true, false>(f0.variant), TRACE_TPB, f0.grid_blocks, &grid)) return -1;
     // the routing surface: every window's or none
     for (int64_t w = 0; w < n_win; w++)
@@ -2099,10 +2125,10 @@ extern "C" int pg_batch_iterate(const pg_fast_fields* wins, int64_t n_win, const
         if (f0.time_trace && timer_record(s, false, k.pass)) return -1;
         hipLaunchKernelGGL(erosion_scale_kernel, dim3(grid_for(n)), dim3(TPB), 0, s, n, a.height, a.sediment, a.base,
                            k.ero_fx, k.floor_key, at<ll>(work, L.acc_fx), at<float>(work, L.r1), f0.iter_erode,
-                           (const double*)nullptr, hscale, k.rte);
+                           f0.erode_cap, hscale, k.rte);
         hipLaunchKernelGGL(erosion_scale2_kernel, dim3(grid_for(n)), dim3(TPB), 0, s, n, a.height, a.sediment, a.base,
                            k.ero_fx, k.floor_key, at<ll>(work, L.acc_fx), at<float>(work, L.r1),
-                           at<float>(work, L.rscale), f0.iter_erode, (const double*)nullptr, hscale, k.rte);
+                           at<float>(work, L.rscale), f0.iter_erode, f0.erode_cap, hscale, k.rte);
         k.pass = 1;
         if (f0.time_trace && timer_record(s, true, k.pass)) return -1;
         if (k.tape) hipLaunchKernelGGL(replay_kernel<true>, dim3(rgrid), dim3(TRACE_TPB), 0, s, k);

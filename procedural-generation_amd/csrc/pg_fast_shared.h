@@ -32,6 +32,7 @@ struct Scalars {
     ll eligible, pending, n_total, cdf_exact;
     int emin;
     int pad;
+    float inv_sat, mom_scale;  // the iteration's discharge scales (in cells of rain when asked, maps.py:911-921)
     ll queue[1];  // followed by `chunks` queue counters
 };
 
@@ -71,7 +72,7 @@ struct Ctx {
 int prepare(const pg_fast_fields* fl, const pg_trace_params* prm, const pg_fast_params* fp, void* work,
             int64_t work_bytes, int64_t* stats, pg_stream stream, Ctx* c);
 // pack + spawn of `fl` (the droplets land in the workspace's d_* arrays)
-int fast_spawn(const pg_fast_fields* fl, const pg_fast_params* fp, void* work, Ctx& c);
+int fast_spawn(const pg_fast_fields* fl, const pg_trace_params* prm, const pg_fast_params* fp, void* work, Ctx& c);
 // fold of acc_fx into `fl`'s height / sediment, EMA of the tracks, stockpile mirror
 int finish_iteration(const pg_fast_fields* fl, const pg_fast_params* fp, void* work, Ctx& c);
 // persistent grid of a trace kernel: one block set per CU at full occupancy
@@ -181,15 +182,17 @@ __device__ __forceinline__ ll wave_sum(ll v) {
 // k_mom, mom_scale, dt, density, slope_gain, slope_sat, H, spherical; N is
 // the droplet's grid's interior size.
 // Returns -1 (moved), DEATH_STOP (no finite direction) or DEATH_EXIT (left a
-// window).  P = the stencil's sample at the current position.  F32_TRANSFER:
+// window).  P = the stencil's sample at the current position; mom_scale the
+// iteration's momentum scale (k.mom_scale is not read).  F32_TRANSFER:
 // a face crossing in single precision (transfer_velocity_f: the fast mode's
 // trace, ~50 fewer registers) or in double (the differentiable mode, whose
 // float32 / float64 gradients are compared along the same trajectories).
 // ---------------------------------------------------------------------------
 template <bool F32_TRANSFER, class K>
 __device__ __forceinline__ int steer_move(const K& k, int N, const Samp& P, const float4 g0, const float4 g1,
-                                          float vol, float vol_rel, float& sa, float& sb, int f, int ix, int iy,
-                                          float fx, float fy, int& nf, int& nix, int& niy, float& nfx, float& nfy) {
+                                          float vol, float vol_rel, float mom_scale, float& sa, float& sb, int f,
+                                          int ix, int iy, float fx, float fy, int& nf, int& nix, int& niy, float& nfx,
+                                          float& nfy) {
     const float gi = P.gi, gj = P.gj, q = P.q;
     const float di = g0.w * gi + g1.x * gj;
     const float dj = g1.x * gi + g1.y * gj;
@@ -205,7 +208,7 @@ __device__ __forceinline__ int steer_move(const K& k, int N, const Samp& P, cons
         const float slen = sqrtf(fmaxf(g0.x * sa * sa + 2.0f * g0.y * sa * sb + g0.z * sb * sb, 0.0f));
         if (mlen > 1e-12f && slen > 1e-12f) {
             const float cosang = (g0.x * ma * sa + g0.y * (ma * sb + mb * sa) + g0.z * mb * sb) / (mlen * slen);
-            const float kk = k.k_mom * cosang * k.mom_scale / (vol + k.mom_scale * q);
+            const float kk = k.k_mom * cosang * mom_scale / (vol + mom_scale * q);
             sa += kk * ma;
             sb += kk * mb;
         }
@@ -282,7 +285,8 @@ __device__ __forceinline__ int steer_move(const K& k, int N, const Stencil& S, c
     Samp P = sample_of(S);
     P.q = q;
     h0 = P.h;
-    return steer_move<F32_TRANSFER>(k, N, P, g0, g1, vol, vol_rel, sa, sb, f, ix, iy, fx, fy, nf, nix, niy, nfx, nfy);
+    return steer_move<F32_TRANSFER>(k, N, P, g0, g1, vol, vol_rel, k.mom_scale, sa, sb, f, ix, iy, fx, fy, nf, nix, niy,
+                                    nfx, nfy);
 }
 
 }  // namespace fastmode

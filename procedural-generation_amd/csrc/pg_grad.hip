@@ -802,7 +802,7 @@ extern "C" int pg_grad_iterate(const pg_fast_fields* steer, const pg_fast_fields
     PG_REQUIRE(target->height && target->sediment && target->discharge && target->momentum && target->pending &&
                    target->mask,
                "pg_grad_iterate: NULL target field");
-    if (fast_spawn(steer, fp, work, c)) return -1;
+    if (fast_spawn(steer, prm, fp, work, c)) return -1;
     if (ga->real ? launch_forward<double>(steer, prm, fp, ga, work, c) : launch_forward<float>(steer, prm, fp, ga, work, c))
         return -1;
     return finish_iteration(target, fp, work, c);

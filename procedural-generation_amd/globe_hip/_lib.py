@@ -42,7 +42,8 @@ class FastParams(ctypes.Structure):
                 ("iter_erode", dbl), ("conc_chunk", dbl), ("fan_slope", dbl),
                 ("offshore_writeoff", dbl), ("ema", dbl), ("min_volume_frac", dbl), ("height_bits", i64),
                 ("track_bits", i64), ("grid_blocks", i64), ("time_trace", i64), ("rng_dev", vp), ("variant", i64),
-                ("erode_cap", vp), ("tape", vp), ("tape_bytes", i64), ("route", vp), ("lake_flag", vp)]
+                ("erode_cap", vp), ("tape", vp), ("tape_bytes", i64), ("route", vp), ("lake_flag", vp),
+                ("disc_saturation_cells", dbl), ("momentum_saturation_cells", dbl), ("inflow_volume", dbl)]
 
 
 class GradArgs(ctypes.Structure):

@@ -117,11 +117,13 @@ class WindowBatch:
                   for w, st in enumerate(self.states)]
         self.max_steps = [maps.max_steps_of(eps[w], st.N) for w, st in enumerate(self.states)]
         st0 = self.states[0]
-        disc_sat, mom_scale = maps.discharge_scales(eps[0], st0, 1.0, self.P[0], n_rain=1)
-        if float(eps[0].disc_saturation_cells) > 0.0 or float(eps[0].momentum_saturation_cells) > 0.0:
-            raise NotImplementedError("cell-count discharge scales are outside the fast mode")
-        for e in eps:
-            maps.require_plain_side_passes(e, "the batched-window iteration")
+        # discharge scales in cells of upstream rain are computed per window on the device
+        cells = float(eps[0].disc_saturation_cells) > 0.0 or float(eps[0].momentum_saturation_cells) > 0.0
+        disc_sat, mom_scale = (float(eps[0].disc_saturation), 1.0) if cells else \
+            maps.discharge_scales(eps[0], st0, 1.0, self.P[0], n_rain=1)
+        # a window's step (maps.py:1516-1617 with spherical False): the routing flood and the window's lakes,
+        # the particles, mass wasting, uplift -- isostasy, the sea mask, lake balance and glacial carving are
+        # globe passes a window never runs
         # prm.max_steps: the largest lifetime (each window's own comes from pg_batch_setup), the tape's bound
         self.prm = maps.trace_params(eps[0], st0, 1.0, 1, 1, disc_sat, mom_scale, max(self.max_steps), 0.0)
         self.ep = eps[0]
@@ -141,6 +143,13 @@ class WindowBatch:
         self.rng_dev = None
         self.use_tape = True  # the census records every move and the applying pass replays it (same results)
         self.tape = None
+        self.iteration = 0
+        # the slope limit's per-cell cap over the arena: every window's side buffer is its view
+        self.ecap = None
+        if float(getattr(self.ep, "slope_limit_erode", 0.0)) > 0.0:
+            self.ecap = torch.empty_like(self.height)
+            for w, st in enumerate(self.states):
+                st._side["ecap"] = self.ecap[int(off[w]):int(off[w + 1])].view(1, NE[w], NE[w])
         check(lib().pg_batch_setup(self.fields, self.W, self._fparams((0,)), self.steps_arr,
                                    ctypes.c_void_p(self.work.data_ptr()), nbytes, pk._stream()), "pg_batch_setup")
 
@@ -158,17 +167,35 @@ class WindowBatch:
         out = (FastParams * self.W)()
         for w in range(self.W):
             slo, shi, ilo, ihi = fast.pcg64_state(self.params[w].rng(self.rng_stage, *key))
+            st = self.states[w]
             out[w] = FastParams(rng_state_lo=slo, rng_state_hi=shi, rng_inc_lo=ilo, rng_inc_hi=ihi,
                                 n_particles=self.P[w], chunks=self.chunks,
-                                iter_deposit=cell_units(ep, "iter_deposit", self.unit),
+                                iter_deposit=maps.slope_limited_deposit(ep, self.unit),
                                 iter_erode=cell_units(ep, "iter_erode", self.unit),
                                 conc_chunk=float(maps.chunk_size(ep, self.P[w])), fan_slope=float(ep.fan_slope),
                                 offshore_writeoff=float(ep.offshore_writeoff), ema=float(ep.ema),
                                 min_volume_frac=float(ep.min_volume_frac), height_bits=fast.HEIGHT_BITS,
                                 track_bits=fast.TRACK_BITS, grid_blocks=int(self.grid_blocks),
                                 time_trace=int(self.time_trace), variant=int(self.variant),
-                                rng_dev=None if self.rng_dev is None else self.rng_dev.data_ptr() + 32 * w)
+                                rng_dev=None if self.rng_dev is None else self.rng_dev.data_ptr() + 32 * w,
+                                route=None if st.route is None else st.route.data_ptr(),
+                                lake_flag=None if st.route is None or st.lake_flag is None else st.lake_flag.data_ptr(),
+                                disc_saturation_cells=float(ep.disc_saturation_cells),
+                                momentum_saturation_cells=float(ep.momentum_saturation_cells),
+                                inflow_volume=float(st.inflow_volume))
         return out
+
+    def flood_due(self) -> bool:
+        """Whether this iteration refreshes the windows' routing surfaces
+        (every ``flood_every``; maps.py:1548-1549)."""
+        flood = int(self.ep.flood_every)
+        return flood > 0 and (self.iteration % flood == 0 or any(st.route is None for st in self.states))
+
+    def refresh_routes(self) -> None:
+        """Every window's routing flood and, with ``window_lakes``, its lakes
+        (host-synchronous: outside a captured graph)."""
+        for w, st in enumerate(self.states):
+            maps.refresh_route_and_lakes(st, self.params[w], due=True)
 
     def tape_for(self, fp):
         """The batch's move tape for the current grid and variant (allocated
@@ -188,10 +215,18 @@ class WindowBatch:
             self.tape = torch.empty(need, dtype=torch.uint8, device=self.height.device)
         return self.tape
 
-    def step(self, iteration_key) -> None:
-        """One batched iteration of every window (asynchronous)."""
+    def step(self, iteration_key, refresh: bool = True) -> None:
+        """One batched iteration of every window (asynchronous but for the
+        routing flood when it is due; ``refresh=False`` leaves the flood to
+        the caller, as a captured graph does)."""
         key = iteration_key if isinstance(iteration_key, (tuple, list)) else (int(iteration_key),)
+        if refresh and self.flood_due():
+            self.refresh_routes()
         fp = self._fparams(key)
+        if self.ecap is not None:
+            for st in self.states:
+                maps.slope_limit_cap(st, self.ep)
+            fp[0].erode_cap = self.ecap.data_ptr()
         tape = self.tape_for(fp)
         if tape is not None:
             fp[0].tape = tape.data_ptr()
@@ -200,7 +235,10 @@ class WindowBatch:
                                      ctypes.c_void_p(self.work.data_ptr()), self.work.numel(),
                                      ctypes.c_void_p(self.stats.data_ptr()), ctypes.c_void_p(self.win_stats.data_ptr()),
                                      pk._stream()), "pg_batch_iterate")
+        for w, st in enumerate(self.states):
+            maps.thermal_erosion(st, self.params[w])  # nothing at rate 0
         pk.apply_uplift(self.height, self.uplift, self.mask, None, 0.0)
+        self.iteration += 1
 
     def step_per_window(self, iteration_key) -> None:
         """The same iteration as one fast-mode pass per window, one after
@@ -210,13 +248,17 @@ class WindowBatch:
         does not launch a full-chip grid that would idle."""
         key = iteration_key if isinstance(iteration_key, (tuple, list)) else (int(iteration_key),)
         cus = torch.cuda.get_device_properties(0).multi_processor_count
+        if self.flood_due():
+            self.refresh_routes()
         for w, st in enumerate(self.states):
             per_chunk = self.P[w] / max(self.chunks, 1)
             st.fast_grid_blocks = int(min(4 * cus, max(cus // 4, math.ceil(1.5 * per_chunk / 256))))
             st.fast_variant = 0
             st.fast_time_trace = self.time_trace
             fast.run_iteration(st, self.params[w], key, chunks=self.chunks, sync_stats=False)
+            maps.thermal_erosion(st, self.params[w])
             maps.apply_uplift(st)
+        self.iteration += 1
 
     def moves(self, per_window: bool = False) -> int:
         """Droplet moves of the last iteration (synchronises)."""
@@ -238,12 +280,18 @@ class BatchGraph:
         dev = batch.height.device
         batch.rng_dev = torch.zeros((batch.W, 4), dtype=torch.int64, device=dev)
         batch.time_trace = False
+        if int(batch.ep.flood_every) > 0 and batch.flood_due():
+            batch.refresh_routes()  # the routes and lake flags exist before the capture
         batch.tape_for(batch._fparams((0,)))  # allocated before the capture
+        for w, st in enumerate(batch.states):  # side buffers too
+            maps.prepare_side_passes(st, batch.params[w])
         torch.cuda.synchronize()
         self._load((0,))
+        it0 = batch.iteration
         self.graph = torch.cuda.CUDAGraph()
         with torch.cuda.graph(self.graph):
-            batch.step((0,))
+            batch.step((0,), refresh=False)
+        batch.iteration = it0
 
     def _load(self, key):
         b = self.batch
@@ -252,8 +300,12 @@ class BatchGraph:
 
     def step(self, iteration_key) -> None:
         key = iteration_key if isinstance(iteration_key, (tuple, list)) else (int(iteration_key),)
+        b = self.batch
+        if b.flood_due():
+            b.refresh_routes()  # in place into the buffers the graph reads
         self._load(key)
         self.graph.replay()
+        b.iteration += 1
 
 
 class Autotuner:

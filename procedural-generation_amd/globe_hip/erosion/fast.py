@@ -128,8 +128,6 @@ def run_iteration(state, params, iteration_key, n_particles=None, particles_per_
     _check_scope(ep, state)
     if float(getattr(ep, "lateral_rate", 0.0)) > 0.0 or state.roots is not None or lake_park_of(state, ep) > 0.0:
         raise NotImplementedError("lateral cuts, roots and lake parks are outside the fast mode: use mode='check'")
-    if float(ep.disc_saturation_cells) > 0.0 or float(ep.momentum_saturation_cells) > 0.0:
-        raise NotImplementedError("cell-count discharge scales need the spawn volume on the host: use mode='check'")
     key = iteration_key if isinstance(iteration_key, (tuple, list)) else (int(iteration_key),)
     rng = params.rng(rng_stage, *key)
     t0 = time.time()
@@ -140,7 +138,11 @@ def run_iteration(state, params, iteration_key, n_particles=None, particles_per_
     ctx = _fast_context(state, n_particles)
     unit = state.height_unit_m
     max_steps = max_steps_of(ep, state.N)
-    disc_sat, mom_scale = discharge_scales(ep, state, 1.0, n_particles, n_rain=1)
+    # in cells of upstream rain (disc_saturation_cells / momentum_saturation_cells) the scales are computed on
+    # the device from the iteration's spawn; otherwise they are the parameters'
+    cells = float(ep.disc_saturation_cells) > 0.0 or float(ep.momentum_saturation_cells) > 0.0
+    disc_sat, mom_scale = (float(ep.disc_saturation), 1.0) if cells else discharge_scales(ep, state, 1.0, n_particles,
+                                                                                          n_rain=1)
     prm = trace_params(ep, state, 1.0, 1, 1, disc_sat, mom_scale, max_steps, 0.0)
     ecap = slope_limit_cap(state, ep)
     n_chunks = int(chunks if chunks is not None else getattr(state, "fast_chunks", DEFAULT_CHUNKS))
@@ -161,7 +163,10 @@ def run_iteration(state, params, iteration_key, n_particles=None, particles_per_
                     erode_cap=None if ecap is None else ecap.data_ptr(),
                     tape=None if tape is None else tape.data_ptr(), tape_bytes=0 if tape is None else tape.numel(),
                     route=None if state.route is None else state.route.data_ptr(),
-                    lake_flag=None if state.route is None or state.lake_flag is None else state.lake_flag.data_ptr())
+                    lake_flag=None if state.route is None or state.lake_flag is None else state.lake_flag.data_ptr(),
+                    disc_saturation_cells=float(ep.disc_saturation_cells),
+                    momentum_saturation_cells=float(ep.momentum_saturation_cells),
+                    inflow_volume=float(state.inflow_volume))
     check(lib().pg_fast_iterate(ctypes.byref(ctx.fields), ctypes.byref(prm), ctypes.byref(fp),
                                 ctypes.c_void_p(ctx.work.data_ptr()), ctx.work.numel(),
                                 ctypes.c_void_p(ctx.stats.data_ptr()), pk._stream()), "pg_fast_iterate")
@@ -173,6 +178,15 @@ def run_iteration(state, params, iteration_key, n_particles=None, particles_per_
     stats["pending_total"] = float(state.pending[state.interior].sum().item())
     stats["seconds"] = time.time() - t0
     stats["chunk"] = -(-stats["particles"] // fp.chunks) if stats["particles"] else 0
+    if cells and stats["eligible"] > 0:
+        # the device's scales, restated from the spawn it reports (maps.py:911-921)
+        vol = stats["volume"] * float(n_particles)
+        rpc = max(vol - float(state.inflow_volume), 0.0) / stats["eligible"]
+        rpc = rpc if rpc > 0.0 else vol / stats["eligible"]
+        if float(ep.disc_saturation_cells) > 0.0:
+            disc_sat = float(ep.disc_saturation_cells) * rpc
+        if float(ep.momentum_saturation_cells) > 0.0:
+            mom_scale = stats["volume"] / (float(ep.momentum_saturation_cells) * rpc)
     stats["disc_saturation"] = disc_sat
     stats["mom_scale"] = mom_scale
     if log is not None:

@@ -151,14 +151,29 @@ def value_noise_2d(NE: int, rng: np.random.Generator, octaves: int = 6, base: fl
     return out / tot
 
 
+#: the reference's zoom erosion profile at the 305 m level (bake/globe/refine/
+#: zoom.py:31-84 ZOOM_EROSION with zoom_params' level rules, :196-219: the
+#: discharge scales at their cell caps, the wider slope limit above 150 m,
+#: the talus rate above 40 m), max_steps aside -- the parameters a zoom tile
+#: is eroded with
+ZOOM_PROFILE_305M = dict(
+    erodibility=1.0, k_disc=10.0, disc_exponent=0.0, disc_saturation=32.0, min_volume_frac=0.01, friction=0.0,
+    evap_rate=0.001 / 1.2, max_erode=1e9, iter_erode=1e9, iter_deposit=1e9, thermal_max=1e9, thermal_rate=0.8,
+    talus_slope_soft=0.8, talus_slope_hard=0.8, creep_rate=0.0, window_lakes=False, flood_every=5,
+    disc_saturation_cells=1280.0, momentum_saturation_cells=512.0, slope_gain=0.589, slope_limit_erode=0.5,
+    slope_limit_deposit=0.25, k_mom=1.0,
+)
+
+
 def zoom_windows(n: int = 64, seed: int = 0, side_lo: int = 128, side_hi: int = 1024, life_lo: int = 32,
-                 life_hi: int = 256, H: int = 2, cell_m: float = 50.0, relief_m: float = 400.0):
+                 life_hi: int = 256, H: int = 2, cell_m: float = 50.0, relief_m: float = 400.0, **erosion):
     """Synthetic zoom windows: side log-uniform in [side_lo, side_hi] cells,
     each with its own fBm bedrock seed (relief_m of relief on a 100 m
     floor), halo H outside the mask, uniform rain, one droplet per cell and
     a lifetime log-uniform in [life_lo, life_hi].  Returns (window keyword
     arguments for ErosionState.window, WorldParams) per window: the plain
-    profile of workload_params with the window's seed and lifetime."""
+    profile of workload_params with the window's seed and lifetime, then the
+    ``erosion`` overrides (e.g. ZOOM_PROFILE_305M)."""
     rng = np.random.default_rng(seed)
     sides = np.exp(rng.uniform(np.log(side_lo), np.log(side_hi), n)).astype(np.int64)
     lives = np.exp(rng.uniform(np.log(life_lo), np.log(life_hi), n)).astype(np.int64)
@@ -177,7 +192,8 @@ def zoom_windows(n: int = 64, seed: int = 0, side_lo: int = 128, side_hi: int = 
                    hardness=np.zeros((NE, NE), np.float32), precip=np.ones((NE, NE), np.float32),
                    evap=np.ones((NE, NE), np.float32), uplift_m=z, mask=mask, metric=metric, metric_inv=metric,
                    H=H, height_unit_m=cell_m)
-        ep = workload_params(N, H, max_steps=int(lives[w])).erosion
+        ep = workload_params(N, H, **erosion).erosion
+        ep.max_steps = int(lives[w])
         p = WorldParams(world=WorldGroup(seed=seed * 1000 + w, N_c=N, cell_size_m=cell_m, land_fraction=0.3, halo=H),
                         erosion=ep)
         out.append((win, p))

@@ -15,9 +15,38 @@ pytestmark = pytest.mark.gpu
 FIELDS = ("height", "sediment", "discharge", "momentum", "pending")
 
 
-def _batch(cuda, n=6, seed=3, lo=24, hi=96, chunks=4):
-    wins = stubs.zoom_windows(n, seed=seed, side_lo=lo, side_hi=hi, life_lo=16, life_hi=96)
+def _batch(cuda, n=6, seed=3, lo=24, hi=96, chunks=4, **erosion):
+    wins = stubs.zoom_windows(n, seed=seed, side_lo=lo, side_hi=hi, life_lo=16, life_hi=96, **erosion)
     return bt.WindowBatch([w for w, _ in wins], [p for _, p in wins], device=cuda, chunks=chunks)
+
+
+#: the reference's zoom profile (flood every 2 here, to refresh within a short run)
+ZOOM = dict(stubs.ZOOM_PROFILE_305M, flood_every=2)
+
+
+def test_zoom_profile_batched_equals_per_window(cuda):
+    """The reference's zoom-tile erosion profile -- routing flood, talus
+    pass, slope limits, discharge scales in cells of rain, the erf law --
+    batched in one launch per pass and one fast-mode pass per window: the
+    same bits."""
+    a, b = _batch(cuda, seed=4, **ZOOM), _batch(cuda, seed=4, **ZOOM)
+    for it in range(4):
+        a.step(it)
+        b.step_per_window(it)
+        torch.cuda.synchronize()
+        _same(a, b)
+    assert all(st.route is not None for st in a.states)
+    assert float((a.height - _batch(cuda, seed=4, **ZOOM).height).abs().max()) > 0
+
+
+def test_zoom_profile_graph_replays_match_eager(cuda):
+    a, b = _batch(cuda, seed=6, **ZOOM), _batch(cuda, seed=6, **ZOOM)
+    g = bt.BatchGraph(b)
+    for it in range(4):
+        a.step(it)
+        g.step(it)
+        torch.cuda.synchronize()
+        _same(a, b)
 
 
 def _same(a, b):

@@ -160,6 +160,23 @@ def test_tape_replay_equals_the_traced_applying_pass(cuda, grid_blocks, variant)
         np.testing.assert_array_equal(a, b)
 
 
+def test_cell_count_discharge_scales(cuda, monkeypatch):
+    """disc_saturation_cells / momentum_saturation_cells (maps.py:911-921):
+    the scales are computed on the device from the iteration's spawn; the
+    same iteration with those scales handed in as parameters gives the same
+    bits (and the stats report them)."""
+    kw = dict(k_mom=1.0, k_disc=10.0, disc_exponent=0.0)
+    _, _, pa, a = _fast_state(48, cuda, disc_saturation_cells=1280.0, momentum_saturation_cells=512.0, **kw)
+    sa = fast.run_iteration(a, pa, 0)
+    D, M = sa["disc_saturation"], sa["mom_scale"]
+    assert D > 0 and M > 0 and D != pa.erosion.disc_saturation
+    _, _, pb, b = _fast_state(48, cuda, **kw)
+    monkeypatch.setattr(maps, "discharge_scales", lambda *args, **kws: (D, M))
+    fast.run_iteration(b, pb, 0)
+    for n in ("height", "sediment", "discharge", "momentum", "pending"):
+        assert torch.equal(getattr(a, n), getattr(b, n)), n
+
+
 def _hypsometry(surf_land, lo, hi, bins=64):
     h, _ = np.histogram(surf_land, bins=bins, range=(lo, hi))
     return np.cumsum(h) / max(h.sum(), 1)
