@@ -24,12 +24,14 @@
 // apply order: here all droplets of a chunk act on the same frozen terrain
 // and the caps are order independent -- a census pass sums each cell's
 // erosion requests and the lowest surface stepped to from it, a per-cell
-// scale limits the cell to its iteration budget and that floor, and the
-// applying pass clamps each entry at its own next cell's frozen surface
-// (deficit carried to later deposits) and each deposit at the previous path
-// cell + dep_floor and the waterline (surplus carried on); the per-cell
-// resolve applies the deposit budget and stockpiles the excess.  Plain
-// profile only.  Batched windows (pg_batch_*) run the same kernels over an
+// scale limits the cell to its iteration budget and to that floor lowered by
+// what the floor cell itself loses in the chunk (the reference's next-cell
+// floor is live), the applying pass scales each entry (deficit carried to
+// later deposits), clamps each deposit at the previous path cell + dep_floor
+// -- with the droplet's own change there -- and the waterline (surplus
+// carried on), and spreads a land death's load over its last SPREAD cells
+// under the same ceilings; the per-cell resolve applies the deposit budget
+// and stockpiles the excess.  Plain profile only.  Batched windows (pg_batch_*) run the same kernels over an
 // arena of single-face windows, one persistent launch per chunk and pass.
 #include <hipcub/hipcub.hpp>
 
@@ -399,6 +401,11 @@ struct TraceCfg {
     ll* mom_fx;    // momentum track a; track b in its own plane (a move's atomics on separate lines)
     ll* mom_fx_b;
     ll* stats;
+    // the applying pass's final deposits: the iteration's net change so far,
+    // the stockpiles, the iteration's deposit budget (fixed point)
+    const ll* acc_fx;
+    ll* pend_fx;
+    ll dep_budget;
     // the census's move tape (single grid only; NULL = the applying pass
     // traces again): per wave of the persistent grid, `tape_rows` rows of 64
     // lane records (two float4 each) -- row t holds what every lane of the
@@ -445,11 +452,73 @@ __device__ __forceinline__ ll ceiling_fx(float lim, double hscale) {
     return lim > 0.0f ? llrint((double)lim * hscale) : 0;
 }
 
+// a land step deposit's ceiling is the previous path cell's live surface
+// (particle.py:986-991): the frozen ceiling plus what this droplet itself
+// changed there one move earlier (its own erosion lowers it, its own deposit
+// raises it); other droplets' changes are not seen (order independent)
+__device__ __forceinline__ ll own_ceiling(ll lim, int64_t cell, ll own_prev, int own_cell) {
+    if (lim >= (ll)9.0e18 || own_cell < 0 || (int64_t)own_cell == cell) return lim;
+    return lim + own_prev;
+}
+
+// the last SPREAD active cells a droplet stood in, newest first
+// (particle.py:443-447 and the stop cell, :786-793)
+struct Ring {
+    int c[SPREAD];
+    int n;
+    __device__ __forceinline__ void push(int cell) {
+#pragma unroll
+        for (int r = SPREAD - 1; r > 0; r--) c[r] = c[r - 1];
+        c[0] = cell;
+        n = n < SPREAD ? n + 1 : SPREAD;
+    }
+};
+
+// the final deposits of a droplet that stopped on land (particle.py:794-815,
+// and apply_changes' final entries, :964-1002): the newest ring cell is
+// offered the whole load, each older one what the newer ones refused; a slot
+// takes at most the room left of the iteration's deposit budget and never
+// rises above the next-older ring cell + dep_floor (the oldest slot has no
+// such ceiling), nor above the waterline less dep_floor when submerged.
+// Surfaces and budgets are the chunk's frozen ones (a cell listed twice is
+// offered once); the per-cell resolve then applies the live budget.
+// Returns the load no slot took (parked on the stop cell's stockpile).
+__device__ __forceinline__ ll final_cascade(const TraceCfg& k, const Ring& ring, ll load) {
+    ll left = load;
+#pragma unroll
+    for (int r = 0; r < SPREAD; r++) {
+        if (r < ring.n && left > 0) {
+            const int c = ring.c[r];
+            bool seen = false;
+#pragma unroll
+            for (int q = 0; q < r; q++) seen |= ring.c[q] == c;
+            if (!seen) {
+                const float s_c = k.cell4[c].x;
+                const float b_c = k.dyn[c].y;
+                float lim = r + 1 < ring.n ? k.cell4[ring.c[r + 1]].x + k.dep_floor - s_c : 3.0e38f;
+                if (s_c < b_c) lim = fminf(lim, b_c - k.dep_floor - s_c);
+                ll dep = left;
+                const ll room = k.dep_budget - k.acc_fx[c];
+                if (dep > room) dep = room;
+                const ll cap = ceiling_fx(lim, k.hscale);
+                if (dep > cap) dep = cap;
+                if (dep > 0) {
+                    atomicAdd((ull*)&k.chunk_fx[c], (ull)dep);
+                    left -= dep;
+                }
+            }
+        }
+    }
+    return left;
+}
+
 // tape record: flags word = transfer kind (bits 0-1: none / sea deposit /
 // land step), MOVE (bit 2), START of a droplet (bit 3), END (bit 4), the end's
-// cause (bits 8-15), exited (16), final deposit (17), final entries (20-23)
+// cause (bits 8-15), exited (16), final deposit (17), the end's step stood
+// on an active cell without recording a move (18), final entries (20-23)
 enum : int { REC_NONE = 0, REC_SEA = 1, REC_LAND = 2 };
-enum : int { RF_MOVE = 1 << 2, RF_START = 1 << 3, RF_END = 1 << 4, RF_EXITED = 1 << 16, RF_DEPOSIT = 1 << 17 };
+enum : int { RF_MOVE = 1 << 2, RF_START = 1 << 3, RF_END = 1 << 4, RF_EXITED = 1 << 16, RF_DEPOSIT = 1 << 17,
+             RF_STOOD = 1 << 18 };
 
 // WPE: the register budget as waves per SIMD (trace_waves: the census's
 // natural allocation fits 4, the traced applying pass's 3; one or two more
@@ -500,6 +569,10 @@ __global__ void __launch_bounds__(TRACE_TPB) __attribute__((amdgpu_waves_per_eu(
     int step = 0, uphill = 0, sea_steps = 0, pushes = 0, last_cell = -1;
     ll real = 0, deficit = 0;  // load actually held (fixed point) and load cancelled by erosion caps
     ll surplus = 0;            // deposits a ceiling refused, offered to the droplet's next deposits
+    ll own_prev = 0;           // what the droplet changed at its previous path cell (applying pass)
+    int own_cell = -1;         // that cell
+    Ring ring;                 // applying pass: the cells of the final deposits
+    ring.n = 0;
     int dl = 0;                // the droplet's slot in the chunk (a chunk holds < 2^31 droplets)
     // the wave's tape rows (census with a tape): wave-uniform row counter
     const ll gw = (ll)blockIdx.x * (TRACE_TPB / 64) + (threadIdx.x >> 6);
@@ -508,7 +581,7 @@ __global__ void __launch_bounds__(TRACE_TPB) __attribute__((amdgpu_waves_per_eu(
     Samp S;  // the sample of the stencil at the droplet's position
     Here cur;
     // lane totals
-    ll n_entries = 0, n_moves = 0, exit_fx = 0, vanished_fx = 0, lost_fx = 0;
+    ll n_entries = 0, n_moves = 0, exit_fx = 0, vanished_fx = 0, lost_fx = 0, to_pend_fx = 0;
     ll deaths[N_DEATHS] = {0, 0, 0, 0, 0, 0};
 
     auto load_here = [&](int ff, int cx, int cy, Here& h, const float4* rec) {
@@ -527,7 +600,7 @@ __global__ void __launch_bounds__(TRACE_TPB) __attribute__((amdgpu_waves_per_eu(
     };
     // deposit request `d` (fixed point) at `cell`: with the surplus of earlier
     // refused deposits, less the deficit, at most `lim` (particle.py:969-1002)
-    auto deposit = [&](int64_t cell, ll d, ll lim) {
+    auto deposit = [&](int64_t cell, ll d, ll lim) -> ll {
         ll want = d + surplus;
         surplus = 0;
         ll a = want < deficit ? want : deficit;
@@ -539,6 +612,7 @@ __global__ void __launch_bounds__(TRACE_TPB) __attribute__((amdgpu_waves_per_eu(
         surplus = want - dep;
         if (dep) atomicAdd((ull*)&k.chunk_fx[cell], (ull)dep);
         real -= dep;
+        return dep;
     };
     // ceiling of a step deposit on the frozen terrain: the previous path cell
     // (+ dep_floor on land, - fan_slope under water) and the waterline
@@ -619,6 +693,9 @@ __global__ void __launch_bounds__(TRACE_TPB) __attribute__((amdgpu_waves_per_eu(
                         sa = k.d_sa[d];
                         sb = k.d_sb[d];
                         has_prev = false;
+                        own_prev = 0;
+                        own_cell = -1;
+                        ring.n = 0;
                         in_sea = false;
                         in_lake = false;
                         step = uphill = sea_steps = pushes = 0;
@@ -639,7 +716,7 @@ __global__ void __launch_bounds__(TRACE_TPB) __attribute__((amdgpu_waves_per_eu(
 
         // ---- one step of this lane's droplet (particle.py:421-769)
         int cause = -1;
-        bool exited = false;
+        bool exited = false, stood = false;
         if (step >= max_steps) {
             cause = DEATH_AGE;
         } else if (cur.m == MASK_OUTSIDE) {
@@ -656,6 +733,8 @@ __global__ void __launch_bounds__(TRACE_TPB) __attribute__((amdgpu_waves_per_eu(
                 if (cur.m == MASK_ACTIVE) {
                     last_cell = (int)cur.cell;
                     pushes += 1;
+                    stood = true;
+                    if (apply) ring.push((int)cur.cell);
                 }
                 const float4 g0 = cur.g0, g1 = cur.g1;
                 const float q = S.q;
@@ -695,6 +774,7 @@ __global__ void __launch_bounds__(TRACE_TPB) __attribute__((amdgpu_waves_per_eu(
                             const float dh = h0 - fmaxf(h1, b1);
                             int rk = REC_NONE;
                             float rx = 0.0f, rroom = 0.0f;
+                            ll chg = 0;  // applying pass: the change made at this cell
                             // mass transfer at the cell being left
                             if (cur.m == MASK_ACTIVE && in_sea) {
                                 float cdiff = -k.ocean_rate * sed;
@@ -705,7 +785,8 @@ __global__ void __launch_bounds__(TRACE_TPB) __attribute__((amdgpu_waves_per_eu(
                                 if (-cdiff * vol_rel > room) cdiff = -room / vol_rel;
                                 sed += cdiff;
                                 if (apply) {
-                                    deposit(cur.cell, llrint((double)(-cdiff * vol_rel) * k.hscale), step_ceiling(h_here, b_here));
+                                    chg = deposit(cur.cell, llrint((double)(-cdiff * vol_rel) * k.hscale),
+                                                  step_ceiling(h_here, b_here));
                                     n_entries += 1;
                                 } else if (taping) {
                                     rk = REC_SEA;
@@ -751,12 +832,12 @@ __global__ void __launch_bounds__(TRACE_TPB) __attribute__((amdgpu_waves_per_eu(
                                     rx = amount;
                                 }
                                 if (dq > 0) {
-                                    // an entry never takes the cell below its base level nor below the
-                                    // cell it steps to (frozen surfaces; particle.py:951-962)
+                                    // an entry never takes the cell below its base level (particle.py:951);
+                                    // the floor of the cell it steps to, which the reference takes live
+                                    // (:956-960), is the per-cell scale's (census floor key, two sweeps)
                                     float fl = h_here >= b_here ? b_here : h_here;
                                     // a lake bed is never eroded below its water level (particle.py:952-955)
                                     if (ROUTE && cur.rte.y >= 1.5f) fl = fmaxf(fl, cur.rte.x);
-                                    fl = fmaxf(fl, nrec.x);
                                     const float room = h_here - fl;
                                     rroom = room;
                                     const ll lim = room > 0.0f ? llrint((double)room * k.hscale) : 0;
@@ -783,15 +864,21 @@ __global__ void __launch_bounds__(TRACE_TPB) __attribute__((amdgpu_waves_per_eu(
                                         const ll taken = (ll)floor((double)dq * (double)cur.rs);
                                         deficit += dq - taken;
                                         real += taken;
+                                        chg = -taken;
                                         if (taken) atomicAdd((ull*)&k.chunk_fx[cur.cell], (ull)(-taken));
                                     } else if (dq < 0) {
-                                        deposit(cur.cell, -dq, step_ceiling(h_here, b_here));
+                                        chg = deposit(cur.cell, -dq,
+                                                      own_ceiling(step_ceiling(h_here, b_here), cur.cell, own_prev, own_cell));
                                     }
                                     atomicAdd((ull*)&k.dis_fx[cur.cell], (ull)llrint((double)vol * k.tscale));
                                     atomicAdd((ull*)&k.mom_fx[cur.cell], (ull)llrint((double)(vol * sa) * k.tscale));
                                     atomicAdd((ull*)&k.mom_fx_b[cur.cell], (ull)llrint((double)(vol * sb) * k.tscale));
                                     n_entries += 1;
                                 }
+                            }
+                            if (apply) {
+                                own_prev = chg;
+                                own_cell = (int)cur.cell;
                             }
                             if (taping) {
                                 // the move as the applying pass needs it: cell, kind, amount, and the
@@ -851,7 +938,8 @@ __global__ void __launch_bounds__(TRACE_TPB) __attribute__((amdgpu_waves_per_eu(
             }
             const bool dodep = pu > 0 && (!exited || k.deposit_on_exit);
             const int nfin = dodep && sed > 0.0f ? (cz == DEATH_OCEAN ? 1 : (pu < SPREAD ? pu : SPREAD)) : 0;
-            r_flags |= RF_END | (cz << 8) | (exited ? RF_EXITED : 0) | (dodep ? RF_DEPOSIT : 0) | (nfin << 20);
+            r_flags |= RF_END | (cz << 8) | (exited ? RF_EXITED : 0) | (dodep ? RF_DEPOSIT : 0) | (nfin << 20) |
+                       (stood && !(r_flags & RF_MOVE) ? RF_STOOD : 0);
             k.tape_last[dl] = lc;
         }
         if (apply) {
@@ -863,12 +951,21 @@ __global__ void __launch_bounds__(TRACE_TPB) __attribute__((amdgpu_waves_per_eu(
                     if (pushes == 0 || last_cell != (int)cur.cell) {
                         last_cell = (int)cur.cell;
                         pushes += 1;
+                        ring.push((int)cur.cell);
                     }
                 }
             }
             if (pushes > 0 && (!exited || k.deposit_on_exit)) {
                 if (sed > 0.0f) n_entries += cause == DEATH_OCEAN ? 1 : (pushes < SPREAD ? pushes : SPREAD);
-                if (real) atomicAdd((ull*)&k.chunk_fx[last_cell], (ull)real);
+                if (cause != DEATH_OCEAN && real > 0) {
+                    const ll left = final_cascade(k, ring, real);
+                    if (left) {
+                        atomicAdd((ull*)&k.pend_fx[last_cell], (ull)left);
+                        to_pend_fx += left;
+                    }
+                } else if (real) {
+                    atomicAdd((ull*)&k.chunk_fx[last_cell], (ull)real);
+                }
             } else if (exited) {
                 exit_fx += real;
             } else {
@@ -892,15 +989,16 @@ __global__ void __launch_bounds__(TRACE_TPB) __attribute__((amdgpu_waves_per_eu(
     if (taping && lane == 0) k.tape_len[gw] = t_row;
     if (!apply) return;
     // lane totals -> device statistics (one atomic per wave and counter)
-    ll v[5 + N_DEATHS] = {n_entries, n_moves, exit_fx, vanished_fx, lost_fx};
-    for (int c = 0; c < N_DEATHS; c++) v[5 + c] = deaths[c];
-    for (int c = 0; c < 5 + N_DEATHS; c++) v[c] = wave_sum(v[c]);
+    ll v[6 + N_DEATHS] = {n_entries, n_moves, exit_fx, vanished_fx, lost_fx, to_pend_fx};
+    for (int c = 0; c < N_DEATHS; c++) v[6 + c] = deaths[c];
+    for (int c = 0; c < 6 + N_DEATHS; c++) v[c] = wave_sum(v[c]);
     if (lane == 0) {
-        const int slot[5] = {PG_FS_ENTRIES, PG_FS_MOVES, PG_FS_EXIT_FX, PG_FS_VANISHED_FX, PG_FS_DEFICIT_FX};
-        for (int c = 0; c < 5; c++)
+        const int slot[6] = {PG_FS_ENTRIES, PG_FS_MOVES, PG_FS_EXIT_FX, PG_FS_VANISHED_FX, PG_FS_DEFICIT_FX,
+                             PG_FS_TO_PENDING_FX};
+        for (int c = 0; c < 6; c++)
             if (v[c]) atomicAdd((ull*)&k.stats[slot[c]], (ull)v[c]);
         for (int c = 0; c < N_DEATHS; c++)
-            if (v[5 + c]) atomicAdd((ull*)&k.stats[PG_FS_DEATHS + c], (ull)v[5 + c]);
+            if (v[6 + c]) atomicAdd((ull*)&k.stats[PG_FS_DEATHS + c], (ull)v[6 + c]);
     }
 }
 
@@ -947,7 +1045,7 @@ void launch_trace(int64_t variant, int grid, hipStream_t s, const TraceCfg& k) {
 // momentum tracks, which the scales do not touch, were added by the census.
 // ---------------------------------------------------------------------------
 template <bool BATCH>
-__global__ void __launch_bounds__(TRACE_TPB) replay_kernel(TraceCfg k) {
+__global__ void __launch_bounds__(TRACE_TPB) __attribute__((amdgpu_waves_per_eu(7))) replay_kernel(TraceCfg k) {
     const int lane = threadIdx.x & 63;
     const ll rw = (ll)blockIdx.x * (TRACE_TPB / 64) + (threadIdx.x >> 6);
     const ll rwaves = (ll)gridDim.x * (TRACE_TPB / 64);
@@ -965,9 +1063,13 @@ __global__ void __launch_bounds__(TRACE_TPB) replay_kernel(TraceCfg k) {
         if (k.sc->volume0 <= 0.0) return;  // the census traced nothing (and wrote no rows)
     }
     ll real = 0, deficit = 0, surplus = 0;
-    int dl = 0;
-    ll n_entries = 0, n_moves = 0, exit_fx = 0, vanished_fx = 0, lost_fx = 0;
-    ll deaths[N_DEATHS] = {0, 0, 0, 0, 0, 0};
+    ll own_prev = 0;  // what the droplet changed at its previous path cell
+    int own_cell = -1, dl = 0;
+    Ring ring;        // the cells of its final deposits
+    ring.n = 0;
+    int n_entries = 0, n_moves = 0;  // per lane: below 2^31
+    ll exit_fx = 0, vanished_fx = 0, lost_fx = 0, to_pend_fx = 0;
+    int deaths[N_DEATHS] = {0, 0, 0, 0, 0, 0};
     // a deposit request: the amount laid down (the caller adds it to the cell)
     auto deposit = [&](ll d, ll lim) -> ll {
         ll want = d + surplus;
@@ -997,6 +1099,9 @@ __global__ void __launch_bounds__(TRACE_TPB) replay_kernel(TraceCfg k) {
                 dl = __float_as_int(B.w);
                 real = k.d_load[__float_as_int(B.z)];
                 deficit = surplus = 0;
+                own_prev = 0;
+                own_cell = -1;
+                ring.n = 0;
             }
             if (fl & RF_MOVE) {
                 // one height-change atomic per row for erosion and deposit alike (full lanes per instruction)
@@ -1023,10 +1128,13 @@ __global__ void __launch_bounds__(TRACE_TPB) replay_kernel(TraceCfg k) {
                         real += taken;
                         change = -taken;
                     } else if (dq < 0) {
-                        change = deposit(-dq, ceiling_fx(A.w, k.hscale));
+                        change = deposit(-dq, own_ceiling(ceiling_fx(A.w, k.hscale), cell, own_prev, own_cell));
                     }
                     n_entries += 1;
                 }
+                own_prev = change;
+                own_cell = (int)cell;
+                if (kind != REC_NONE) ring.push((int)cell);  // a move from an active cell
                 if (change) atomicAdd((ull*)&k.chunk_fx[cell], (ull)change);
                 n_moves += 1;
             }
@@ -1035,7 +1143,19 @@ __global__ void __launch_bounds__(TRACE_TPB) replay_kernel(TraceCfg k) {
                 deaths[(fl >> 8) & 0xff] += 1;
                 n_entries += (fl >> 20) & 0xf;
                 if (fl & RF_DEPOSIT) {
-                    if (real) atomicAdd((ull*)&k.chunk_fx[k.tape_last[dl]], (ull)real);
+                    // the ring as the droplet left it: the end's own step, then the stop cell
+                    const int last = k.tape_last[dl];
+                    if (fl & RF_STOOD) ring.push(last);
+                    if (!(fl & RF_EXITED) && (ring.n == 0 || ring.c[0] != last)) ring.push(last);
+                    if (((fl >> 8) & 0xff) != DEATH_OCEAN && real > 0) {
+                        const ll left = final_cascade(k, ring, real);
+                        if (left) {
+                            atomicAdd((ull*)&k.pend_fx[last], (ull)left);
+                            to_pend_fx += left;
+                        }
+                    } else if (real) {
+                        atomicAdd((ull*)&k.chunk_fx[last], (ull)real);
+                    }
                 } else if (fl & RF_EXITED) {
                     exit_fx += real;
                 } else {
@@ -1046,15 +1166,16 @@ __global__ void __launch_bounds__(TRACE_TPB) replay_kernel(TraceCfg k) {
             }
         }
     }
-    ll v[5 + N_DEATHS] = {n_entries, n_moves, exit_fx, vanished_fx, lost_fx};
-    for (int c = 0; c < N_DEATHS; c++) v[5 + c] = deaths[c];
-    for (int c = 0; c < 5 + N_DEATHS; c++) v[c] = wave_sum(v[c]);
+    ll v[6 + N_DEATHS] = {n_entries, n_moves, exit_fx, vanished_fx, lost_fx, to_pend_fx};
+    for (int c = 0; c < N_DEATHS; c++) v[6 + c] = deaths[c];
+    for (int c = 0; c < 6 + N_DEATHS; c++) v[c] = wave_sum(v[c]);
     if (lane == 0) {
-        const int slot[5] = {PG_FS_ENTRIES, PG_FS_MOVES, PG_FS_EXIT_FX, PG_FS_VANISHED_FX, PG_FS_DEFICIT_FX};
-        for (int c = 0; c < 5; c++)
+        const int slot[6] = {PG_FS_ENTRIES, PG_FS_MOVES, PG_FS_EXIT_FX, PG_FS_VANISHED_FX, PG_FS_DEFICIT_FX,
+                             PG_FS_TO_PENDING_FX};
+        for (int c = 0; c < 6; c++)
             if (v[c]) atomicAdd((ull*)&k.stats[slot[c]], (ull)v[c]);
         for (int c = 0; c < N_DEATHS; c++)
-            if (v[5 + c]) atomicAdd((ull*)&k.stats[PG_FS_DEATHS + c], (ull)v[5 + c]);
+            if (v[6 + c]) atomicAdd((ull*)&k.stats[PG_FS_DEATHS + c], (ull)v[6 + c]);
     }
     // droplets the census left untraced because every wave's rows were full: must be none
     if (rw == 0 && lane == 0) {
@@ -1654,6 +1775,9 @@ int fast_particles(const pg_fast_fields* fl, const pg_trace_params* prm, const p
     k.mom_fx = at<ll>(work, L.mom_fx);
     k.mom_fx_b = k.mom_fx + n;
     k.stats = st;
+    k.acc_fx = at<ll>(work, L.acc_fx);
+    k.pend_fx = at<ll>(work, L.pend_fx);
+    k.dep_budget = (ll)fmin(floor(fp->iter_deposit * hscale), 4.0e18);
     int grid = 0, rgrid = 0;
     PG_REQUIRE(fp->variant >= 0 && fp->variant <= 2, "fast mode: variant must be 0, 1 or 2");
     if (persistent_grid(trace_fn<false, false>(fp->variant), TRACE_TPB, fp->grid_blocks, &grid)) return -1;
@@ -2086,6 +2210,9 @@ extern "C" int pg_batch_iterate(const pg_fast_fields* wins, int64_t n_win, const
     k.mom_fx = at<ll>(work, L.mom_fx);
     k.mom_fx_b = k.mom_fx + n;
     k.stats = (ll*)stats;
+    k.acc_fx = at<ll>(work, L.acc_fx);
+    k.pend_fx = at<ll>(work, L.pend_fx);
+    k.dep_budget = (ll)fmin(floor(f0.iter_deposit * hscale), 4.0e18);
     k.wins = wd;
     k.win_prefix = prefix;
     k.n_win = (int)n_win;

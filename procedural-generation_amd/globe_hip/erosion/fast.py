@@ -27,9 +27,9 @@ from . import particle as pk
 DEFAULT_CHUNKS = 16
 HEIGHT_BITS = 48
 TRACK_BITS = 32
-#: largest move tape allocated by default (a planet at N_c=8192 would need more: it traces twice instead,
-#: or uses more chunks)
-TAPE_MAX_BYTES = 24 << 30
+#: largest move tape allocated by default: 64 GiB, under a quarter of the part's 288 GB of HBM (a planet
+#: at N_c=8192 would need more: it traces twice instead, or uses more chunks)
+TAPE_MAX_BYTES = 64 << 30
 
 
 class FastContext:

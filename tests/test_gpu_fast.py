@@ -9,7 +9,7 @@
 * the applying pass replayed from the census's tape equals the traced one;
 * against the oracle (serial reference semantics) after 20 iterations on the
   N=128 plain globe: land hypsometry and eroded volume within 2 % (the
-  order-independent caps' systematic deficit; measured 1.4-2.0 %).
+  order-independent caps' systematic deficit; measured 1.1-1.8 %).
 """
 from __future__ import annotations
 
@@ -190,7 +190,7 @@ def test_fast_tracks_the_oracle(oracle, cuda, chunks, flood_every):
 
     Bound asserted: land hypsometry (64-bin CDF) and eroded volume within
     2 %.  The stated target is 0.5 %; the order-independent apply does not
-    reach it (measured ~1.5 %, against a 0.22 % seed-to-seed noise floor of
+    reach it (measured 1.1-1.8 %, against a 0.22 % seed-to-seed noise floor of
     the same metric): the reference applies each chunk serially, with caps
     that depend on apply order."""
     N, iters = 128, 20
