@@ -396,6 +396,7 @@ struct TraceCfg {
     double hscale, tscale, conc_chunk;
     ll* chunk_fx;   // pass 1: net applied change per cell
     ll* ero_fx;     // pass 0: erosion requested per cell
+    ll* fin_fx;     // pass 0: final loads of the land deaths that stop on a cell
     ull* floor_key;       // pass 0: lowest (surface key << 32 | cell) stepped to from a cell
     ll* dis_fx;
     ll* mom_fx;    // momentum track a; track b in its own plane (a move's atomics on separate lines)
@@ -498,7 +499,10 @@ __device__ __forceinline__ ll final_cascade(const TraceCfg& k, const Ring& ring,
                 float lim = r + 1 < ring.n ? k.cell4[ring.c[r + 1]].x + k.dep_floor - s_c : 3.0e38f;
                 if (s_c < b_c) lim = fminf(lim, b_c - k.dep_floor - s_c);
                 ll dep = left;
-                const ll room = k.dep_budget - k.acc_fx[c];
+                ll room = k.dep_budget - k.acc_fx[c];
+                // the loads that stop on this cell share its room in proportion
+                const ll demand = k.fin_fx[c];
+                if (room > 0 && demand > room) room = (ll)((double)room * fmin((double)load / (double)demand, 1.0));
                 if (dep > room) dep = room;
                 const ll cap = ceiling_fx(lim, k.hscale);
                 if (dep > cap) dep = cap;
@@ -942,6 +946,21 @@ __global__ void __launch_bounds__(TRACE_TPB) __attribute__((amdgpu_waves_per_eu(
                        (stood && !(r_flags & RF_MOVE) ? RF_STOOD : 0);
             k.tape_last[dl] = lc;
         }
+        if (!apply) {
+            // a land death's final load, on the cell it stops in: the applying pass shares each
+            // cell's deposit room among the loads that stop there
+            int cz = cause;
+            if (in_sea && (cz == DEATH_AGE || cz == DEATH_STOP)) cz = DEATH_OCEAN;
+            int lc = last_cell, pu = pushes;
+            if (!exited && cur.m == MASK_ACTIVE && !(in_sea && cur.h >= cur.base) && (pu == 0 || lc != (int)cur.cell)) {
+                lc = (int)cur.cell;
+                pu += 1;
+            }
+            if (pu > 0 && (!exited || k.deposit_on_exit) && cz != DEATH_OCEAN && sed > 0.0f) {
+                const ll load = llrint((double)(sed * vol * inv_v0) * k.hscale);
+                if (load > 0) atomicAdd((ull*)&k.fin_fx[lc], (ull)load);
+            }
+        }
         if (apply) {
             if (in_sea && (cause == DEATH_AGE || cause == DEATH_STOP)) cause = DEATH_OCEAN;
             deaths[cause] += 1;
@@ -1254,11 +1273,12 @@ __global__ void erosion_scale2_kernel(int64_t n, const double* height, const dou
 __global__ void resolve_kernel(int64_t n, const double* height, const double* sediment, const double* base, ll* chunk_fx,
                                ll* acc_fx, ll* pend_fx, float4* cell4, float2* dyn, double iter_deposit,
                                double dep_floor, double writeoff, double hscale, ll* stats, ll* ero_fx, ull* floor_key,
-                               float2* rte) {
+                               float2* rte, ll* fin_fx) {
     const double inv = 1.0 / hscale;
     ll clamped = 0, to_pend = 0, lost_off = 0;
     GRID_STRIDE(c, n) {
         ero_fx[c] = 0;
+        fin_fx[c] = 0;
         floor_key[c] = ~0ull;
         const ll net = chunk_fx[c];
         if (net == 0) continue;
@@ -1460,6 +1480,7 @@ int layout_of(int64_t F, int64_t NE, int64_t P, Layout* L) {
     L->mom_fx = take(16 * n);
     L->pend_fx = take(8 * n);
     L->ero_fx = take(8 * n);
+    L->fin_fx = take(8 * n);
     L->floor_key = take(8 * n);
     L->rscale = take(4 * n);
     L->r1 = take(4 * n);
@@ -1767,6 +1788,7 @@ int fast_particles(const pg_fast_fields* fl, const pg_trace_params* prm, const p
     k.tscale = tscale;
     k.chunk_fx = at<ll>(work, L.chunk_fx);
     k.ero_fx = at<ll>(work, L.ero_fx);
+    k.fin_fx = at<ll>(work, L.fin_fx);
     k.floor_key = at<ull>(work, L.floor_key);
     k.fan_slope = (float)fp->fan_slope;
     k.rscale = at<float>(work, L.rscale);
@@ -1823,7 +1845,7 @@ int fast_particles(const pg_fast_fields* fl, const pg_trace_params* prm, const p
         if (fp->time_trace && timer_record(s, false, k.pass)) return -1;
         hipLaunchKernelGGL(resolve_kernel, dim3(grid_for(n)), dim3(TPB), 0, s, n, fl->height, fl->sediment, fl->base,
                            k.chunk_fx, at<ll>(work, L.acc_fx), pend_fx, cell4, dyn, fp->iter_deposit, prm->dep_floor,
-                           fp->offshore_writeoff, hscale, st, k.ero_fx, k.floor_key, k.rte);
+                           fp->offshore_writeoff, hscale, st, k.ero_fx, k.floor_key, k.rte, k.fin_fx);
     }
     PG_LAUNCHED("trace/resolve");
     return finish_iteration(fl, fp, work, c);
@@ -1946,6 +1968,7 @@ int batch_layout(int64_t W, const int64_t* NE, const int64_t* P, BLayout* B) {
     L.mom_fx = take(16 * n);
     L.pend_fx = take(8 * n);
     L.ero_fx = take(8 * n);
+    L.fin_fx = take(8 * n);
     L.floor_key = take(8 * n);
     L.rscale = take(4 * n);
     L.r1 = take(4 * n);
@@ -1990,6 +2013,7 @@ void window_layout(const BLayout& B, int64_t coff, int64_t poff, int64_t doff, i
     Lw->mom_fx = L.mom_fx + 16 * c;
     Lw->pend_fx = L.pend_fx + 8 * c;
     Lw->ero_fx = L.ero_fx + 8 * c;
+    Lw->fin_fx = L.fin_fx + 8 * c;
     Lw->floor_key = L.floor_key + 8 * c;
     Lw->rscale = L.rscale + 4 * c;
     Lw->r1 = L.r1 + 4 * c;
@@ -2204,6 +2228,7 @@ extern "C" int pg_batch_iterate(const pg_fast_fields* wins, int64_t n_win, const
     k.tscale = tscale;
     k.chunk_fx = at<ll>(work, L.chunk_fx);
     k.ero_fx = at<ll>(work, L.ero_fx);
+    k.fin_fx = at<ll>(work, L.fin_fx);
     k.floor_key = at<ull>(work, L.floor_key);
     k.fan_slope = (float)f0.fan_slope;
     k.dis_fx = at<ll>(work, L.dis_fx);
@@ -2264,7 +2289,7 @@ extern "C" int pg_batch_iterate(const pg_fast_fields* wins, int64_t n_win, const
         hipLaunchKernelGGL(resolve_kernel, dim3(grid_for(n)), dim3(TPB), 0, s, n, a.height, a.sediment, a.base,
                            k.chunk_fx, at<ll>(work, L.acc_fx), at<ll>(work, L.pend_fx), at<float4>(work, L.cell4),
                            at<float2>(work, L.dyn), f0.iter_deposit, prm->dep_floor, f0.offshore_writeoff, hscale,
-                           (ll*)stats, k.ero_fx, k.floor_key, k.rte);
+                           (ll*)stats, k.ero_fx, k.floor_key, k.rte, k.fin_fx);
     }
     PG_LAUNCHED("batch trace/resolve");
     hipLaunchKernelGGL(finish_kernel, dim3(grid_for(n)), dim3(TPB), 0, s, n, a.height, a.sediment, a.discharge,

@@ -40,7 +40,7 @@ struct Scalars {
 // workspace layout
 // ---------------------------------------------------------------------------
 struct Layout {
-    size_t cell4, geo, dyn, rte, chunk_fx, acc_fx, dis_fx, mom_fx, pend_fx, ero_fx, floor_key, rscale, r1, wq, cdf_fx, flag,
+    size_t cell4, geo, dyn, rte, chunk_fx, acc_fx, dis_fx, mom_fx, pend_fx, ero_fx, fin_fx, floor_key, rscale, r1, wq, cdf_fx, flag,
         rank, comp_idx, comp_cdf,
         draws, u_sorted, d_cell, d_fx, d_fy, d_load, d_sa, d_sb, scalars, temp, total;
     size_t temp_bytes;
