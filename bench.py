@@ -48,14 +48,14 @@ HBM_PEAK_GBS = 8000.0
 #: HBM bytes per chunk (census launch, applying launch) of this workload (N_c=1024, 16 chunks) by
 #: (persistent grid, register-budget variant), from rocprofv3 FETCH_SIZE (x 2.000, calibrated on a 1-GiB
 #: copy) + WRITE_SIZE (x 1.000); profiles/trace_kernel_earth_n1024_counters.md holds the runs
-TRAFFIC_PER_CHUNK = {(1024, 0): (5.437e9, 1.822e9), (1280, 1): (10.098e9, 2.139e9), (1536, 2): (22.017e9, 2.225e9),
-                     (2560, 1): (14.577e9, 2.418e9)}
+TRAFFIC_PER_CHUNK = {(1024, 0): (5.454e9, 1.851e9), (1280, 1): (10.181e9, 2.187e9), (1536, 2): (21.021e9, 2.282e9),
+                     (2560, 1): (14.789e9, 2.486e9)}
 #: relaxed 64-bit atomicAdd / atomicMin per droplet move of this workload by pass (census: the erosion
 #: request and floor key on eroding steps and the three tracks on land; applying pass: the height change
 #: and a land death's final deposits),
 #: from rocprofv3 TCC_ATOMIC_sum (one per lane-atomic: 268.43 M for the probe's 268.44 M adds); every one is
 #: performed past L2 (TCC_EA0_ATOMIC_sum equal); profiles/trace_kernel_earth_n1024_atomics.md
-ATOMICS_PER_MOVE = {"census": 3.01, "applying": 0.36}
+ATOMICS_PER_MOVE = {"census": 3.02, "applying": 0.37}
 
 
 def log(msg):
