@@ -116,6 +116,29 @@ int pg_apply_changes(const int32_t* cl_cell, const double* cl_delta, const float
                      double dep_floor, double offshore_writeoff, const double* erode_cap, int use_erode_cap,
                      double* lake_load, const int32_t* lake_id, double* lake_room, double* out4, pg_stream stream);
 
+/* ---- check mode: the same apply in parallel, bit for bit ------------- */
+/* Replaces apply_changes (particle.py:821-1042) like pg_apply_changes, with
+ * the same results to the bit, computed in parallel: the serial order's
+ * values are the unique solution of "each entry's amount from the live
+ * values it reads; each read value from the earlier amounts of its cell",
+ * solved by alternating droplet walks and per-cell scans in serial order
+ * until a sweep changes nothing (at most max_sweeps; 64-128 suffice on
+ * the earth globe).  n_cells = F * NE * NE.  status: DEVICE int32[4],
+ * zeroed by the caller once per iteration; the call ORs bit 1 into
+ * status[0] when the list holds lake parks (volume -3, not supported: the
+ * chunk is then not applied), bit 2 when max_sweeps did not converge (not
+ * applied); status[1] = most sweeps of a chunk, status[2] += this chunk's
+ * sweeps.  work: caller-allocated DEVICE scratch of
+ * pg_ordered_workspace_bytes(P, cap, max_sweeps) bytes. */
+int64_t pg_ordered_workspace_bytes(int64_t P, int64_t cap, int max_sweeps);
+int pg_apply_changes_ordered(const int32_t* cl_cell, const double* cl_delta, const float* cl_vol, const float* cl_mom,
+                             const int64_t* cl_count, int64_t P, int64_t cap, const double* height,
+                             const double* sediment, double* acc, double* pending, float* samp, double* disch_track,
+                             double* mom_track, const uint8_t* mask, int64_t n_cells, double iter_erode,
+                             double iter_deposit, double fan_slope, int use_route, double dep_floor,
+                             double offshore_writeoff, const double* erode_cap, int use_erode_cap, double* out4,
+                             int32_t* status, int max_sweeps, void* work, int64_t work_bytes, pg_stream stream);
+
 /* Adds a chunk's entry count and death histogram to the DEVICE array
  * counts7 (int64[7]: entries, then the six DEATH_* causes), the statistics
  * run_iteration gathers per chunk (bake/globe/erosion/maps.py:1028-1029). */

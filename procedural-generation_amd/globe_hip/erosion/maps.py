@@ -122,6 +122,10 @@ class ErosionState:
     height_unit_m: float
     grid: object = None
     mode: str = "check"
+    #: the check mode's apply: "ordered" (the serial apply's results computed in
+    #: parallel, bit for bit) or "serial" (one lane walks the lists)
+    check_apply: str = "ordered"
+    check_max_sweeps: int = 1024
     route: torch.Tensor | None = None
     lake_flag: torch.Tensor | None = None
     disch_track: torch.Tensor = field(default=None, repr=False)
@@ -147,6 +151,7 @@ class ErosionState:
     _mask_host: np.ndarray | None = field(default=None, repr=False)
     _select_work: torch.Tensor | None = field(default=None, repr=False)
     _fast: object = field(default=None, repr=False)
+    _ordered: dict = field(default_factory=dict, repr=False)
 
     def __post_init__(self):
         F, NE = self.height.shape[0], self.height.shape[1]
@@ -176,6 +181,8 @@ class ErosionState:
             raise ValueError("array width must be N + 2H")
         if self.mode not in ("check", "fast"):
             raise ValueError(f"mode must be 'check' or 'fast', got {self.mode!r}")
+        if self.check_apply not in ("ordered", "serial"):
+            raise ValueError(f"check_apply must be 'ordered' or 'serial', got {self.check_apply!r}")
         for a, shape, n in ((self.sediment, (F, NE, NE), "sediment"), (self.discharge, (F, NE, NE), "discharge"),
                             (self.momentum, (F, NE, NE, 2), "momentum"), (self.hardness, (F, NE, NE), "hardness"),
                             (self.precip, (F, NE, NE), "precip"), (self.evap, (F, NE, NE), "evap"),
@@ -697,6 +704,16 @@ def run_iteration(state: ErosionState, params, iteration_key, n_particles=None, 
     dep_floor = cell_units(ep, "dep_floor_m", unit)
     use_roots = state.roots is not None
     t_trace = t_apply = 0.0
+    lake_park = lake_park_of(state, ep)
+    ordered = state.check_apply == "ordered" and lake_park == 0.0
+    if ordered:
+        key = (chunk, cap, int(state.check_max_sweeps))
+        if state._ordered.get("key") != key:
+            nbytes = pk.ordered_workspace_bytes(chunk, cap, int(state.check_max_sweeps))
+            state._ordered = {"key": key, "work": torch.empty(nbytes, dtype=torch.uint8, device=dev),
+                              "status": torch.zeros(4, dtype=torch.int32, device=dev)}
+        ord_work, ord_status = state._ordered["work"], state._ordered["status"]
+        ord_status.zero_()
     for ci, c0 in enumerate(range(0, P, chunk)):
         c1 = min(P, c0 + chunk)
         m = c1 - c0
@@ -707,10 +724,18 @@ def run_iteration(state: ErosionState, params, iteration_key, n_particles=None, 
         t2 = time.time()
         if diag is not None:
             diag(state, cl_cell, cl_vol, cl_count[:m], cap, sp_death[:m])
-        pk.apply_changes(cl_cell, cl_delta, cl_vol, cl_mom, cl_count[:m], cap, state.height, state.sediment, state.acc,
-                         state.pending, state.samp, state.disch_track, state.mom_track, state.mask, iter_erode,
-                         iter_deposit, float(ep.fan_slope), state.route is not None, dep_floor, float(ep.offshore_writeoff),
-                         ecap, ecap is not None, state.lake_load, state.lake_id, state.lake_room, outs[ci])
+        if ordered:
+            pk.apply_changes_ordered(cl_cell, cl_delta, cl_vol, cl_mom, cl_count[:m], cap, state.height, state.sediment,
+                                     state.acc, state.pending, state.samp, state.disch_track, state.mom_track, state.mask,
+                                     iter_erode, iter_deposit, float(ep.fan_slope), state.route is not None, dep_floor,
+                                     float(ep.offshore_writeoff), ecap, ecap is not None, outs[ci], ord_status, ord_work,
+                                     int(state.check_max_sweeps))
+        else:
+            pk.apply_changes(cl_cell, cl_delta, cl_vol, cl_mom, cl_count[:m], cap, state.height, state.sediment, state.acc,
+                             state.pending, state.samp, state.disch_track, state.mom_track, state.mask, iter_erode,
+                             iter_deposit, float(ep.fan_slope), state.route is not None, dep_floor,
+                             float(ep.offshore_writeoff), ecap, ecap is not None, state.lake_load, state.lake_id,
+                             state.lake_room, outs[ci])
         pk.chunk_stats(cl_count[:m], sp_death[:m], m, counts)
         t_trace += t2 - t1
         t_apply += time.time() - t2
@@ -718,6 +743,13 @@ def run_iteration(state: ErosionState, params, iteration_key, n_particles=None, 
     pk.ema_update(state.discharge, state.momentum, state.disch_track, state.mom_track, state.mask, float(ep.ema))
     o = outs.cpu().numpy()
     cnt = counts.cpu().numpy()
+    if ordered:
+        ost = ord_status.cpu().numpy()
+        err = pk.ordered_status_error(ost)
+        if err is not None:
+            raise RuntimeError(f"run_iteration: {err}")
+        stats["sweeps_max"] = int(ost[1])
+        stats["sweeps_total"] = int(ost[2])
     # per-chunk partial results summed in chunk order, as the reference's loop does
     n_clamp = 0
     to_pending = lost = lost_offshore = 0.0

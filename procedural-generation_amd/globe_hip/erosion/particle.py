@@ -139,6 +139,63 @@ def apply_changes(cl_cell, cl_delta, cl_vol, cl_mom, cl_count, cap, height, sedi
                                  _ptr(lake_id), _ptr(lake_room), _ptr(out4), _stream()), "apply_changes")
 
 
+def ordered_workspace_bytes(P: int, cap: int, max_sweeps: int) -> int:
+    """Device scratch bytes of ``apply_changes_ordered`` for chunks of up to P droplets."""
+    n = int(lib().pg_ordered_workspace_bytes(int(P), int(cap), int(max_sweeps)))
+    if n < 0:
+        raise RuntimeError(lib().pg_last_error().decode(errors="replace"))
+    return n
+
+
+def apply_changes_ordered(cl_cell, cl_delta, cl_vol, cl_mom, cl_count, cap, height, sediment, acc, pending, samp,
+                          disch_track, mom_track, mask, iter_erode, iter_deposit, fan_slope, use_route, dep_floor,
+                          offshore_writeoff, erode_cap, use_erode_cap, out4, status, work, max_sweeps):
+    """particle.py:821-1042 computed in parallel with the serial apply's
+    results to the bit (fixed-point sweeps over the serial order).  ``status``
+    (int32[4], zeroed per iteration by the caller) collects the error bits
+    (1: lake parks, unsupported; 2: no convergence within ``max_sweeps``)
+    and the sweep counts; ``work`` is a uint8 scratch tensor of
+    ``ordered_workspace_bytes`` bytes."""
+    P = cl_count.shape[0]
+    e = tuple(height.shape)
+    for t, n in ((height, "height"), (sediment, "sediment"), (acc, "acc"), (pending, "pending"), (disch_track, "disch_track")):
+        _need(t, torch.float64, e, n)
+    _need(mom_track, torch.float64, e + (2,), "mom_track")
+    _need(samp, torch.float32, e + (NS,), "samp")
+    _need(mask, torch.uint8, e, "mask")
+    _need(out4, torch.float64, (4,), "out4")
+    _need(status, torch.int32, (4,), "status")
+    _need(cl_count, torch.int64, (P,), "cl_count")
+    _need(work, torch.uint8, None, "work")
+    for t, dt, n in ((cl_cell, torch.int32, "cl_cell"), (cl_delta, torch.float64, "cl_delta"), (cl_vol, torch.float32, "cl_vol")):
+        _need(t, dt, None, n)
+        if t.numel() < P * cap:
+            raise ValueError(f"{n}: {t.numel()} entries, need P * cap = {P * cap}")
+    _need(cl_mom, torch.float32, None, "cl_mom")
+    if cl_mom.numel() < 2 * P * cap:
+        raise ValueError("cl_mom too small")
+    if use_erode_cap:
+        _need(erode_cap, torch.float64, e, "erode_cap")
+    n_cells = int(height.numel())
+    check(lib().pg_apply_changes_ordered(_ptr(cl_cell), _ptr(cl_delta), _ptr(cl_vol), _ptr(cl_mom), _ptr(cl_count), P,
+                                         int(cap), _ptr(height), _ptr(sediment), _ptr(acc), _ptr(pending), _ptr(samp),
+                                         _ptr(disch_track), _ptr(mom_track), _ptr(mask), n_cells, float(iter_erode),
+                                         float(iter_deposit), float(fan_slope), int(bool(use_route)), float(dep_floor),
+                                         float(offshore_writeoff), _ptr(erode_cap if use_erode_cap else None),
+                                         int(bool(use_erode_cap)), _ptr(out4), _ptr(status), int(max_sweeps), _ptr(work),
+                                         int(work.numel()), _stream()), "apply_changes_ordered")
+
+
+def ordered_status_error(status) -> str | None:
+    """The error an ordered apply's status words report, or None."""
+    bits = int(status[0])
+    if bits & 1:
+        return "the change list holds lake parks (volume -3), which the parallel apply does not handle"
+    if bits & 2:
+        return f"the parallel apply did not converge within its sweep limit (status {list(map(int, status))})"
+    return None
+
+
 def chunk_stats(cl_count, sp_death, P, counts7):
     """Entries and death causes of a chunk, added to ``counts7`` (int64[7])."""
     _need(counts7, torch.int64, (7,), "counts7")
