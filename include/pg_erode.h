@@ -121,15 +121,18 @@ int pg_apply_changes(const int32_t* cl_cell, const double* cl_delta, const float
  * the same results to the bit, computed in parallel: the serial order's
  * values are the unique solution of "each entry's amount from the live
  * values it reads; each read value from the earlier amounts of its cell",
- * solved by alternating droplet walks and per-cell scans in serial order
- * until a sweep changes nothing (at most max_sweeps; 64-128 suffice on
- * the earth globe).  n_cells = F * NE * NE.  status: DEVICE int32[4],
- * zeroed by the caller once per iteration; the call ORs bit 1 into
- * status[0] when the list holds lake parks (volume -3, not supported: the
- * chunk is then not applied), bit 2 when max_sweeps did not converge (not
- * applied); status[1] = most sweeps of a chunk, status[2] += this chunk's
- * sweeps.  work: caller-allocated DEVICE scratch of
- * pg_ordered_workspace_bytes(P, cap, max_sweeps) bytes. */
+ * solved by alternating droplet walks and per-cell walks in serial order
+ * until a sweep changes nothing (at most max_sweeps; 10-70 on the earth
+ * globe).  The first launched_sweeps sweeps are enqueued as plain kernel
+ * launches that return at once after convergence; a chunk needing more
+ * finishes in one cooperative launch.  n_cells = F * NE * NE.  status:
+ * DEVICE int32[4], zeroed by the caller once per iteration; the call ORs
+ * bit 1 into status[0] when the list holds lake parks (volume -3, not
+ * supported: the chunk is then not applied), bit 2 when max_sweeps did not
+ * converge (not applied); status[1] = most sweeps of a chunk, status[2] +=
+ * this chunk's sweeps.  work: caller-allocated DEVICE scratch of
+ * pg_ordered_workspace_bytes(P, cap, max_sweeps) bytes.  No host
+ * synchronisation. */
 int64_t pg_ordered_workspace_bytes(int64_t P, int64_t cap, int max_sweeps);
 int pg_apply_changes_ordered(const int32_t* cl_cell, const double* cl_delta, const float* cl_vol, const float* cl_mom,
                              const int64_t* cl_count, int64_t P, int64_t cap, const double* height,
@@ -137,7 +140,27 @@ int pg_apply_changes_ordered(const int32_t* cl_cell, const double* cl_delta, con
                              double* mom_track, const uint8_t* mask, int64_t n_cells, double iter_erode,
                              double iter_deposit, double fan_slope, int use_route, double dep_floor,
                              double offshore_writeoff, const double* erode_cap, int use_erode_cap, double* out4,
-                             int32_t* status, int max_sweeps, void* work, int64_t work_bytes, pg_stream stream);
+                             int32_t* status, int max_sweeps, int launched_sweeps, void* work, int64_t work_bytes,
+                             pg_stream stream);
+
+/* The check mode's chunk loop (bake/globe/erosion/maps.py:953-1029): for each
+ * chunk of `chunk` droplets of the P spawned, pg_trace_particles on the frozen
+ * record, pg_apply_changes_ordered, pg_chunk_stats -- the same results as
+ * calling them chunk by chunk, without a host round trip per chunk.  The
+ * change-list buffers hold chunk * cap entries; outs (DEVICE float64
+ * [n_chunks * 4]) receives each chunk's 4-tuple in chunk order, summed in
+ * droplet order from drop_out (DEVICE float64 [P * 4] scratch: each
+ * droplet's clamped entries, stockpile, deficit and offshore loss). */
+int pg_check_chunks(int64_t P, int64_t chunk, const int64_t* sp_face, const double* sp_x, const double* sp_y,
+                    const double* sp_sed, const double* sp_dir, double volume0, float* samp, int use_route,
+                    const uint8_t* mask, int64_t N, int64_t H, int spherical, const pg_trace_params* prm,
+                    const float* roots, int use_roots, int32_t* cl_cell, double* cl_delta, float* cl_vol,
+                    float* cl_mom, int64_t* cl_count, int64_t* cl_lo, int64_t* cl_hi, int8_t* sp_death,
+                    const double* height, const double* sediment, double* acc, double* pending,
+                    double* disch_track, double* mom_track, double iter_erode, double iter_deposit,
+                    double fan_slope, double dep_floor, double offshore_writeoff, const double* erode_cap,
+                    int use_erode_cap, double* outs, int64_t* counts7, int32_t* status, int max_sweeps,
+                    int launched_sweeps, void* work, int64_t work_bytes, double* drop_out, pg_stream stream);
 
 /* Adds a chunk's entry count and death histogram to the DEVICE array
  * counts7 (int64[7]: entries, then the six DEATH_* causes), the statistics

@@ -83,6 +83,59 @@ __device__ __forceinline__ bool route_flagged(const Record& R, int64_t f, const 
     return s != 0.0f;
 }
 
+// A cell's whole 16-channel record: 64 bytes, four float4 loads.
+struct Rec16 {
+    float4 q[4];
+};
+__device__ __forceinline__ Rec16 load_rec(const float* s, int64_t cell) {
+    const float4* r = (const float4*)(s + cell * NS);
+    return {{r[0], r[1], r[2], r[3]}};
+}
+__device__ __forceinline__ float chan(const Rec16& r, int c) {
+    const float4& v = r.q[c >> 2];
+    return (c & 3) == 0 ? v.x : (c & 3) == 1 ? v.y : (c & 3) == 2 ? v.z : v.w;
+}
+
+// The corner channels a step interpolates -- surface, route, discharge,
+// momentum (both components) and the routing flag -- of the four corners of
+// a bilinear stencil, in quad()'s order.  A step reads its stencil from these
+// registers, and the stencil it steps to is loaded once, with the next
+// cell's record, all in flight together, and becomes the next step's.  The
+// record is frozen during the trace, so holding it changes no value.
+struct Corner6 {
+    float4 a;  // S_SURF, S_ROUTE, S_DISCH, S_MOMA
+    float momb, rflag;
+};
+struct Sten {
+    Corner6 c[4];
+};
+__device__ __forceinline__ Corner6 load_corner(const float* s, int64_t cell, bool route) {
+    const float4* r = (const float4*)(s + cell * NS);
+    return {r[0], s[cell * NS + S_MOMB], route ? s[cell * NS + S_RFLAG] : 0.0f};
+}
+__device__ __forceinline__ Sten load_sten(const Record& R, int64_t f, const Corner& k, bool route) {
+    const int64_t NE = R.NE, b = (f * NE + k.i0) * NE + k.j0;
+    return {{load_corner(R.s, b, route), load_corner(R.s, b + NE, route), load_corner(R.s, b + 1, route),
+             load_corner(R.s, b + NE + 1, route)}};
+}
+__device__ __forceinline__ float corner_chan(const Corner6& c, int ch) {
+    return ch == S_SURF ? c.a.x : ch == S_ROUTE ? c.a.y : ch == S_DISCH ? c.a.z : ch == S_MOMA ? c.a.w : ch == S_MOMB ? c.momb : c.rflag;
+}
+__device__ __forceinline__ Quad squad(const Sten& S, int ch) {
+    return {corner_chan(S.c[0], ch), corner_chan(S.c[1], ch), corner_chan(S.c[2], ch), corner_chan(S.c[3], ch)};
+}
+// Any routing flag in the stencil (particle.py:227-233), float32 sum.
+__device__ __forceinline__ bool sflagged(const Sten& S) {
+    const Quad q = squad(S, S_RFLAG);
+    float sum = q.v00 + q.v10;
+    sum = sum + q.v01;
+    sum = sum + q.v11;
+    return sum != 0.0f;
+}
+__device__ __forceinline__ Rec16 cell_rec(const Record& R, int64_t f, int64_t i, int64_t j) {
+    return load_rec(R.s, (f * R.NE + i) * R.NE + j);
+}
+
 // Metric norm and dot product at a cell (particle.py:236-252).
 struct Metric {
     double g0, g1, g2;
@@ -94,6 +147,13 @@ struct Metric {
         return g0 * a * c + g1 * (a * d + b * c) + g2 * b * d;
     }
 };
+
+// The face transfer out of line: a wave runs its double-double tan / atan
+// only for the lanes that cross a face edge, and the step loop keeps its
+// registers.
+__device__ __noinline__ Transfer transfer_out_of_line(int face, double u, double v, double a, double b) {
+    return transfer_velocity(face, u, v, a, b);
+}
 
 struct TraceArgs {
     int64_t P;
@@ -163,13 +223,24 @@ __global__ void __launch_bounds__(64) trace_check_kernel(TraceArgs a) {
     int64_t uphill = 0, parks = 0, sea_steps = 0;
     int cause = DEATH_AGE;
 
+    // the first step's stencil, cell record and mask; later steps get theirs
+    // from the step before
+    Sten S;
+    Rec16 cr;
+    int m;
+    {
+        const int64_t ei = cell_of(x, N) + H, ej = cell_of(y, N) + H;
+        const Corner kc = corner_of(x - 0.5 + (double)H, y - 0.5 + (double)H, NE);
+        S = load_sten(R, f, kc, a.use_route);
+        cr = cell_rec(R, f, ei, ej);
+        m = maskat(f, ei, ej);
+    }
     for (int64_t step = 0; step < P.max_steps; step++) {
         const int64_t ei = cell_of(x, N) + H, ej = cell_of(y, N) + H;
-        const int m = maskat(f, ei, ej);
         if (m == MASK_OUTSIDE) { exited = true; cause = DEATH_EXIT; break; }
         const int64_t cell = (f * NE + ei) * NE + ej;
-        const float h_here = R.at(f, ei, ej, S_SURF);
-        const float b_here = R.at(f, ei, ej, S_BASE);
+        const float h_here = chan(cr, S_SURF);
+        const float b_here = chan(cr, S_BASE);
         if (step == 0 && h_here < b_here) in_sea = true;
         if (in_sea) {
             if (h_here >= b_here || sed * vol / a.volume0 < 1e-4 || sea_steps >= P.ocean_steps) { cause = DEATH_OCEAN; break; }
@@ -183,20 +254,20 @@ __global__ void __launch_bounds__(64) trace_check_kernel(TraceArgs a) {
         const double fi = x - 0.5 + (double)H, fj = y - 0.5 + (double)H;
         const Corner k0 = corner_of(fi, fj, NE);
         // -- forces (particle.py:450-511)
-        const Quad hs = quad(R, f, k0, S_SURF);
+        const Quad hs = squad(S, S_SURF);
         const double h0 = bilinear(hs, k0);
         double gi, gj;
         gradient(hs, k0, &gi, &gj);
         double h0r = h0;
-        if (a.use_route && route_flagged(R, f, k0)) {
-            const Quad rs = quad(R, f, k0, S_ROUTE);
+        if (a.use_route && sflagged(S)) {
+            const Quad rs = squad(S, S_ROUTE);
             const double r0 = bilinear(rs, k0);
             if (r0 > h0) {
                 h0r = r0;
                 gradient(rs, k0, &gi, &gj);
             }
         }
-        const double q0 = R.at(f, ei, ej, S_GINV), q1 = R.at(f, ei, ej, S_GINV + 1), q2 = R.at(f, ei, ej, S_GINV + 2);
+        const double q0 = chan(cr, S_GINV), q1 = chan(cr, S_GINV + 1), q2 = chan(cr, S_GINV + 2);
         const double di = q0 * gi + q1 * gj;
         const double dj = q1 * gi + q2 * gj;
         double s2 = gi * di + gj * dj;
@@ -204,9 +275,9 @@ __global__ void __launch_bounds__(64) trace_check_kernel(TraceArgs a) {
         const double nrm = P.slope_sat > 0.0 ? P.slope_gain / sqrt(s2 + P.slope_sat * P.slope_sat)
                                              : P.slope_gain / sqrt(1.0 + s2);
         const double na = -di * nrm, nb = -dj * nrm;
-        const double ma = bilinear(quad(R, f, k0, S_MOMA), k0);
-        const double mb = bilinear(quad(R, f, k0, S_MOMB), k0);
-        double q = bilinear(quad(R, f, k0, S_DISCH), k0);
+        const double ma = bilinear(squad(S, S_MOMA), k0);
+        const double mb = bilinear(squad(S, S_MOMB), k0);
+        double q = bilinear(squad(S, S_DISCH), k0);
         if (q < 0.0) q = 0.0;
         const double vol_rel = vol / a.volume0;
         const double psa = sa, psb = sb;
@@ -214,7 +285,7 @@ __global__ void __launch_bounds__(64) trace_check_kernel(TraceArgs a) {
         if (fr < 0.0) fr = 0.0;
         sa *= fr;
         sb *= fr;
-        const Metric g{(double)R.at(f, ei, ej, S_G), (double)R.at(f, ei, ej, S_G + 1), (double)R.at(f, ei, ej, S_G + 2)};
+        const Metric g{(double)chan(cr, S_G), (double)chan(cr, S_G + 1), (double)chan(cr, S_G + 2)};
         const double mlen = g.norm(ma, mb);
         double slen = g.norm(sa, sb);
         if (mlen > 1e-12 && slen > 1e-12) {
@@ -235,7 +306,7 @@ __global__ void __launch_bounds__(64) trace_check_kernel(TraceArgs a) {
         int64_t nf = f;
         if (a.spherical) {
             if (nx < 0.0 || nx >= Nf || ny < 0.0 || ny >= Nf) {
-                const Transfer t = transfer_velocity((int)f, nx / Nf, ny / Nf, sa, sb);
+                const Transfer t = transfer_out_of_line((int)f, nx / Nf, ny / Nf, sa, sb);
                 nf = t.face;
                 nx = t.u * Nf;
                 ny = t.v * Nf;
@@ -251,19 +322,26 @@ __global__ void __launch_bounds__(64) trace_check_kernel(TraceArgs a) {
         }
         const int64_t nei = (int64_t)floor(nx) + H, nej = (int64_t)floor(ny) + H;
         if (nei < 0 || nei >= NE || nej < 0 || nej >= NE) { exited = true; cause = DEATH_EXIT; break; }
-        if (maskat(nf, nei, nej) == MASK_OUTSIDE) { exited = true; cause = DEATH_EXIT; break; }
+        // everything the rest of the step and the next step read, in flight at once
         const double nfi = nx - 0.5 + (double)H, nfj = ny - 0.5 + (double)H;
         const Corner k1 = corner_of(nfi, nfj, NE);
-        const double h1 = bilinear(quad(R, nf, k1, S_SURF), k1);
-        const float b1 = R.at(nf, nei, nej, S_BASE);
+        const Sten Sn = load_sten(R, nf, k1, a.use_route);
+        const int nmask = maskat(nf, nei, nej);
+        const int64_t ei2 = cell_of(nx, N) + H, ej2 = cell_of(ny, N) + H;
+        const Rec16 crn = cell_rec(R, nf, ei2, ej2);
+        const int m2 = maskat(nf, ei2, ej2);  // the next step's cell (nei, nej on the globe)
+        const Rec16 nrec = cell_rec(R, nf, nei, nej);
+        if (nmask == MASK_OUTSIDE) { exited = true; cause = DEATH_EXIT; break; }
+        const double h1 = bilinear(squad(Sn, S_SURF), k1);
+        const float b1 = chan(nrec, S_BASE);
         const double dh = h0 - (h1 > (double)b1 ? h1 : (double)b1);
         double h1r = h1;
-        if (a.use_route && route_flagged(R, nf, k1)) {
-            const double r1 = bilinear(quad(R, nf, k1, S_ROUTE), k1);
+        if (a.use_route && sflagged(Sn)) {
+            const double r1 = bilinear(squad(Sn, S_ROUTE), k1);
             if (r1 > h1) h1r = r1;
         }
         const double dhr = h0r - h1r;
-        const float nflag = R.at(nf, nei, nej, S_RFLAG);
+        const float nflag = chan(nrec, S_RFLAG);
         // -- mass transfer at the cell being left (particle.py:561-740)
         if (m == MASK_ACTIVE && in_sea) {
             double cdiff = -P.ocean_rate * sed;
@@ -285,7 +363,7 @@ __global__ void __launch_bounds__(64) trace_check_kernel(TraceArgs a) {
             if (c_eq < 0.0) c_eq = 0.0;
             double k_e;
             if (c_eq > sed) {
-                const float sthick = R.at(f, ei, ej, S_SED);
+                const float sthick = chan(cr, S_SED);
                 double w_cover;
                 if (P.cover_depth > 0.0) {
                     w_cover = (double)sthick / P.cover_depth;
@@ -293,7 +371,7 @@ __global__ void __launch_bounds__(64) trace_check_kernel(TraceArgs a) {
                 } else {
                     w_cover = sthick > 0.0f ? 1.0 : 0.0;
                 }
-                k_e = P.deposition_rate * (w_cover + (1.0 - w_cover) * (1.0 - (double)R.at(f, ei, ej, S_HARD)));
+                k_e = P.deposition_rate * (w_cover + (1.0 - w_cover) * (1.0 - (double)chan(cr, S_HARD)));
             } else {
                 k_e = P.deposition_rate;
             }
@@ -353,16 +431,19 @@ __global__ void __launch_bounds__(64) trace_check_kernel(TraceArgs a) {
             }
         }
         // -- evaporation and stop tests (particle.py:741-769)
-        double ev = 1.0 - P.dt * P.evap_rate * (double)R.at(f, ei, ej, S_EVAP);
+        double ev = 1.0 - P.dt * P.evap_rate * (double)chan(cr, S_EVAP);
         if (ev < 0.01) ev = 0.01;
         vol *= ev;
         sed /= ev;
         x = nx;
         y = ny;
         f = nf;
+        S = Sn;
+        cr = crn;
+        m = m2;
         h_prev = (double)h_here;
         in_lake = (double)nflag >= 1.5;
-        if (R.at(nf, nei, nej, S_SURF) < R.at(nf, nei, nej, S_BASE) && !in_sea) {
+        if (chan(nrec, S_SURF) < chan(nrec, S_BASE) && !in_sea) {
             in_sea = true;
             if (sed <= 0.0) { cause = DEATH_OCEAN; break; }
         }

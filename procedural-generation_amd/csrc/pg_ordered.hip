@@ -10,31 +10,38 @@
 // values an entry reads R and the amount it adds to its cell W.  Given all W,
 // every R is a prefix: acc0 + the earlier W of that cell, added in serial
 // order.  Given all R, every W (and every carry) follows by walking each
-// droplet's list.  The serial result is the one solution of that system: the
-// system is triangular in serial order, so it has exactly one, and an
-// iteration that alternates the two halves reaches it in finitely many
-// sweeps (at most the depth of the dependency chain) -- and once a sweep
-// changes nothing, the values are that solution, bit for bit.
+// droplet's list.  The serial result is the one solution of that system: it
+// is triangular in serial order, so it has exactly one, and alternating the
+// two halves reaches it in finitely many sweeps (at most the depth of the
+// dependency chain; 10-70 on the earth globe) -- once a sweep changes no W,
+// the values are that solution, bit for bit.
 //
-// One chunk:
-//  1. build: one lane per droplet lays out its entries (per-entry constants
-//     in an entry-major layout so lanes of a wave read adjacent words), the
-//     first R (the chunk's starting acc) and its accesses -- (cell, kind)
-//     pairs in serial order: self read, neighbour read, write per entry, the
-//     final read of the death cell;
-//  2. a stable radix sort of the accesses by cell: each cell's accesses in
-//     serial order, one segment per touched cell;
-//  3. sweeps (one cooperative launch, grid barriers between the halves):
-//     the droplet pass walks each droplet's list from its R and yields W,
-//     the wants of its deposits and its carries; the cell scan walks each
-//     segment in serial order with the live value, recomputing each write
-//     from it (same-cell chains resolve within one scan) and refreshing
-//     the R of every read.  A droplet pass that changes no W and no want
-//     ends the loop;
-//  4. finish: per segment the final acc, the discharge / momentum tracks
-//     and the stockpile adds in serial order (float64 sums in the
-//     reference's order), the packed record (S_SURF, S_SED, S_RFLAG) of
-//     every written cell;
+// One chunk (all launches on the caller's stream, no host synchronisation):
+//  1. build (one wave per droplet): the per-entry constants in the change
+//     list's own droplet-major layout, the first R (the chunk's starting
+//     acc), and each entry's accesses -- (cell, kind) pairs in serial order:
+//     self read, neighbour read, write; then the read of the death cell;
+//  2. a stable radix sort of the accesses by cell: each touched cell's
+//     accesses form one segment in serial order;
+//  3. sweeps, each two launches (a launch boundary is the cheapest grid-wide
+//     barrier on this part: ~2 us against ~17 us for a cooperative grid
+//     sync of 128 blocks):
+//     - the droplet half, one wave per droplet: every lane evaluates its
+//       entry's limits from R in parallel, then the wave walks the entries
+//       that touch the carries (deposits, clamped erosions) in order with
+//       the carries held uniform -- the only serial arithmetic of a droplet;
+//       changed amounts are written both per entry and at their sorted slot;
+//     - the cell half, one wave per range of 256 sorted slots: the running
+//       value of each segment over its writes in serial order (the same
+//       float64 additions as the reference, one lane after the other), each
+//       read's value then taken in parallel from the last write before it;
+//     a droplet half that changes nothing marks the chunk done and every
+//     later launch of it returns at once.  The first `budget` sweeps are
+//     enqueued as plain launches; a chunk that needs more finishes in one
+//     cooperative launch with grid barriers;
+//  4. finish (one wave per range): per segment the final acc, the discharge
+//     and momentum tracks and the stockpiles in serial order (float64 sums
+//     in the reference's order), the packed record of every written cell;
 //  5. the chunk's 4-tuple, summed in droplet order.
 //
 // Lake parks (volume -3 entries, emitted only with lake_park > 0) are not
@@ -54,16 +61,24 @@ namespace cg = cooperative_groups;
 namespace pg {
 namespace {
 
+using ull = unsigned long long;
+
+constexpr int WAVE = 64;
+constexpr int BLOCK = 256;                  // 4 waves
+constexpr int WAVES_PER_BLOCK = BLOCK / WAVE;
+constexpr int RANGE = 256;                  // sorted slots per cell-half wave
 constexpr uint8_t EF_ACTIVE = 1, EF_ERODE = 2, EF_STEP = 4, EF_NB = 8, EF_RFLOOR = 16, EF_TRACK = 32;
 constexpr uint32_t K_SELF = 0, K_NB = 1, K_WRITE = 2, K_END = 3;
 constexpr uint32_t KIND_SHIFT = 30, IDX_MASK = (1u << KIND_SHIFT) - 1u;
-constexpr int SWEEP_THREADS = 256;
+constexpr uint32_t NONE = 0xffffffffu;
+constexpr int CTL_DONE = 0, CTL_SWEEPS = 1, CTL_CHANGED = 2;  // control words; CTL_CHANGED + s per sweep
 
 struct Layout {
-    size_t flags, d, hs, hsn, ec, b, r, rs, rn, rn2, w, want;   // per entry (P * cap)
-    size_t death, hsd, bd, re, drop, pend, pendf;          // per droplet
-    size_t keys_in, vals_in, keys_out, vals_out;            // per access slot
-    size_t seg, nseg, changes, status, temp;
+    size_t flags, d, hs, hsn, ec, b, r, rs, rn, w, posw, cd, cs, cn;   // per entry (P * cap)
+    size_t death, hsd, bd, re, drop, pend, pendf, fdirty, fc;        // per droplet
+    size_t keys_in, vals_in, keys_out, vals_out, sw, owner;          // per access slot
+    size_t rdirty;                                                   // per range
+    size_t ctl, temp;
     size_t temp_bytes, total;
 };
 
@@ -86,9 +101,11 @@ int layout_of(int64_t P, int64_t cap, int max_sweeps, Layout* L) {
     L->r = take(ne * 4);
     L->rs = take(ne * 8);
     L->rn = take(ne * 8);
-    L->rn2 = take(ne * 8);
     L->w = take(ne * 8);
-    L->want = take(ne * 8);
+    L->posw = take(ne * 4);
+    L->cd = take(ne * 8);
+    L->cs = take(ne * 8);
+    L->cn = take(ne * 4);
     L->death = take((size_t)P * 4);
     L->hsd = take((size_t)P * 8);
     L->bd = take((size_t)P * 4);
@@ -96,14 +113,16 @@ int layout_of(int64_t P, int64_t cap, int max_sweeps, Layout* L) {
     L->drop = take((size_t)P * 4 * 8);
     L->pend = take((size_t)P * 8);
     L->pendf = take((size_t)P);
+    L->fdirty = take((size_t)P * 4);
+    L->fc = take((size_t)P * 24);
     L->keys_in = take(na * 4);
     L->vals_in = take(na * 4);
     L->keys_out = take(na * 4);
     L->vals_out = take(na * 4);
-    L->seg = take(na * 4);
-    L->nseg = take(8);
-    L->changes = take((size_t)(max_sweeps + 1) * 4);
-    L->status = take(8);
+    L->sw = take(na * 8);
+    L->owner = take(na * 4);
+    L->rdirty = take((na + RANGE - 1) / RANGE + 1);
+    L->ctl = take((size_t)(CTL_CHANGED + max_sweeps + 1) * 4);
     size_t tb = 0;
     if (hipcub::DeviceRadixSort::SortPairs(nullptr, tb, (const uint32_t*)nullptr, (uint32_t*)nullptr,
                                            (const uint32_t*)nullptr, (uint32_t*)nullptr, (int)(na > 0 ? na : 1)) != hipSuccess)
@@ -140,326 +159,545 @@ struct OrdArgs {
     uint8_t* flags;
     double *d, *hs, *hsn, *ec;
     float *b, *r;
-    double *rs, *w, *want;
-    double* rn[2];  // neighbour reads: the scan reads one buffer and fills the other
+    double *rs, *rn, *w;
+    uint32_t* posw;
+    double *cd, *cs;   // carries (deficit, surplus) entering each entry
+    int32_t* cn;       // clamped entries before each entry
     int32_t* death;
     double* hsd;
     float* bd;
     double *re, *drop, *pend;
     uint8_t* pendf;
-    uint32_t *keys_in, *vals_in, *keys_out, *vals_out, *seg, *nseg, *changes;
+    uint32_t* fdirty;  // per droplet: first entry whose read value changed (NONE: clean)
+    double* fc;        // per droplet: final carries (deficit, surplus, clamped count)
+    uint32_t *keys_in, *vals_in, *keys_out, *vals_out;
+    double* sw;
+    uint32_t* owner;   // per slot: the range whose wave walks its segment
+    uint8_t* rdirty;   // per range: a write of its segments changed
+    uint32_t* ctl;
     int32_t* status;  // [0] status bits (1 lake park, 2 no convergence), [1] most sweeps, [2] sweeps summed
     uint32_t hole;
-    int64_t n_slots;
+    int64_t n_slots, n_ranges;
     int max_sweeps;
     double* out4;
 };
 
-// One entry's amount given its cell's live acc `ac`, the neighbour's live acc
-// `an` and (deposits) the droplet's want; the reference's operand order
-// (particle.py:934-1000).  *clamped: the entry was limited.
-__device__ __forceinline__ double entry_amount(const OrdArgs& a, uint8_t fl, double d, double hs, double hsn, float b_c,
-                                               float r_c, double ec, double ac, double an, double want, bool* clamped) {
-    const double s_c = hs + ac;
-    if (fl & EF_ERODE) {
-        const double e = -d;
-        double e_act = e;
-        double room = a.iter_erode + ac;
-        if (a.use_ecap) {
-            const double r2 = ec + ac;
-            if (r2 < room) room = r2;
-        }
-        if (e_act > room) e_act = room;
-        double floor_ = s_c >= (double)b_c ? (double)b_c : s_c;
-        if ((fl & EF_RFLOOR) && (double)r_c > floor_) floor_ = (double)r_c;
-        if (fl & EF_NB) {
-            const double ns = hsn + an;
-            if (ns > floor_) floor_ = ns;
-        }
-        if (e_act > s_c - floor_) e_act = s_c - floor_;
-        if (e_act < 0.0) e_act = 0.0;
-        *clamped = e_act < e;
-        return -e_act;
-    }
-    double d_act = want;
-    const double room = a.iter_deposit - ac;
-    if (d_act > room) d_act = room;
-    if (fl & EF_NB) {
-        double ceil_ = hsn + an + a.dep_floor;
-        if (s_c < (double)b_c && (fl & EF_STEP)) ceil_ -= a.dep_floor + a.fan_slope;
-        const double lim = ceil_ - s_c;
-        if (d_act > lim) d_act = lim;
-    }
-    if (s_c < (double)b_c) {
-        const double lim = (double)b_c - a.dep_floor - s_c;
-        if (d_act > lim) d_act = lim;
-    }
-    if (d_act < 0.0) d_act = 0.0;
-    *clamped = d_act < want;
-    return d_act;
+__device__ __forceinline__ int lane_id() { return (int)(threadIdx.x & (WAVE - 1)); }
+
+// lane j's value of x (j uniform across the wave)
+__device__ __forceinline__ double rl(double x, int j) {
+    const long long bits = __double_as_longlong(x);
+    const int lo = __builtin_amdgcn_readlane((int)bits, j);
+    const int hi = __builtin_amdgcn_readlane((int)(bits >> 32), j);
+    return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
 }
+__device__ __forceinline__ int rli(int x, int j) { return __builtin_amdgcn_readlane(x, j); }
 
 __device__ __forceinline__ bool same_bits(double x, double y) {
     return __double_as_longlong(x) == __double_as_longlong(y);
 }
 
-// 1. per-droplet layout of the entries and their accesses
-__global__ void __launch_bounds__(256) ord_build_kernel(OrdArgs a) {
-    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (p == 0) {
-        *a.nseg = 0;
-        for (int i = 0; i <= a.max_sweeps; i++) a.changes[i] = 0;
-    }
+// highest set bit of m below bit `lane` (-1 if none)
+__device__ __forceinline__ int last_below(ull m, int lane) {
+    const ull x = lane == 0 ? 0ull : (m & ((1ull << lane) - 1ull));
+    return x ? 63 - __clzll(x) : -1;
+}
+// highest set bit of m at or below bit `lane` (-1 if none)
+__device__ __forceinline__ int last_upto(ull m, int lane) {
+    const ull x = lane == 63 ? m : (m & ((2ull << lane) - 1ull));
+    return x ? 63 - __clzll(x) : -1;
+}
+
+// ---------------------------------------------------------------------------
+// 1. build: one wave per droplet
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(BLOCK) ord_build_kernel(OrdArgs a) {
+    if (blockIdx.x == 0)
+        for (int i = threadIdx.x; i < CTL_CHANGED + a.max_sweeps + 1; i += BLOCK) a.ctl[i] = 0;
+    for (int64_t r = (int64_t)blockIdx.x * BLOCK + threadIdx.x; r < a.n_ranges; r += (int64_t)gridDim.x * BLOCK) a.rdirty[r] = 0;
+    const int lane = lane_id();
+    const int64_t p = (int64_t)blockIdx.x * WAVES_PER_BLOCK + (threadIdx.x / WAVE);
     if (p >= a.P) return;
-    const int64_t P = a.P, cap = a.cap, base = p * cap, cnt = a.cl_count[p];
-    const int64_t sbase = p * (3 * cap + 1);
-    int64_t prev = -1, dth = -1;
-    int32_t bad = 0;
-    for (int64_t k = 0; k < cap; k++) {
-        const int64_t t = k * P + p;
-        uint32_t* ks = a.keys_in + sbase + 3 * k;
-        uint32_t* vs = a.vals_in + sbase + 3 * k;
-        ks[0] = ks[1] = ks[2] = a.hole;
-        vs[0] = vs[1] = vs[2] = 0;
-        a.flags[t] = 0;
-        a.w[t] = 0.0;
-        a.want[t] = 0.0;
-        if (k >= cnt) continue;
-        const int64_t c = a.cl_cell[base + k];
-        const float v = a.cl_vol[base + k];
-        if (v == -3.0f) {
-            bad = 1;
-            continue;
-        }
-        if (a.mask[c] != MASK_ACTIVE) continue;
-        const double d = a.cl_delta[base + k];
-        const bool is_step = v >= 0.0f;
-        int64_t nb = -1;
-        uint8_t fl = EF_ACTIVE;
-        if (d < 0.0) {
-            fl |= EF_ERODE;
-            if (k + 1 < cnt) nb = a.cl_cell[base + k + 1];
-        } else {
-            if (is_step) nb = prev;
-            else if (k + 1 < cnt) nb = a.cl_cell[base + k + 1];
-            if (!is_step && dth < 0) dth = c;
-        }
-        if (is_step) fl |= EF_STEP;
-        if (is_step && v > 0.0f) fl |= EF_TRACK;
-        const float* rec = a.samp + c * NS;
-        if (a.use_route && (double)rec[S_RFLAG] >= 1.5) fl |= EF_RFLOOR;
-        a.d[t] = d;
-        a.hs[t] = a.h[c] + a.s[c];
-        a.b[t] = rec[S_BASE];
-        a.r[t] = rec[S_ROUTE];
-        a.ec[t] = a.use_ecap ? a.ecap[c] : 0.0;
-        a.rs[t] = a.acc[c];
-        ks[0] = (uint32_t)c;
-        vs[0] = (K_SELF << KIND_SHIFT) | (uint32_t)t;
-        if (nb >= 0) {
-            fl |= EF_NB;
-            a.hsn[t] = a.h[nb] + a.s[nb];
-            a.rn[0][t] = a.acc[nb];
-            ks[1] = (uint32_t)nb;
-            vs[1] = (K_NB << KIND_SHIFT) | (uint32_t)t;
-        } else {
-            a.hsn[t] = 0.0;
-            a.rn[0][t] = 0.0;
-            a.rn[1][t] = 0.0;
-        }
-        ks[2] = (uint32_t)c;
-        vs[2] = (K_WRITE << KIND_SHIFT) | (uint32_t)t;
-        a.flags[t] = fl;
-        if (is_step) prev = c;
-    }
-    a.death[p] = (int32_t)dth;
-    a.keys_in[sbase + 3 * cap] = a.hole;
-    a.vals_in[sbase + 3 * cap] = 0;
-    if (dth >= 0) {
-        a.hsd[p] = a.h[dth] + a.s[dth];
-        a.bd[p] = a.samp[dth * NS + S_BASE];
-        a.re[p] = a.acc[dth];
-        a.keys_in[sbase + 3 * cap] = (uint32_t)dth;
-        a.vals_in[sbase + 3 * cap] = (K_END << KIND_SHIFT) | (uint32_t)p;
-    }
-    if (bad) atomicOr(a.status, 1);
-}
-
-// 2b. segment starts of the sorted accesses (one per touched cell; list order free)
-__global__ void __launch_bounds__(256) ord_segments_kernel(OrdArgs a) {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= a.n_slots) return;
-    const uint32_t k = a.keys_out[i];
-    if (k == a.hole) return;
-    if (i == 0 || a.keys_out[i - 1] != k) a.seg[atomicAdd(a.nseg, 1u)] = (uint32_t)i;
-}
-
-// The droplet half: walk droplet p's list from the current R.  Returns true
-// when a W or a want changed.  On the converged pass the per-droplet results
-// (clamped entries, stockpile, deficit, offshore loss) are final.
-__device__ bool droplet_pass(const OrdArgs& a, const double* rn, int64_t p) {
-    const int64_t P = a.P, cnt = a.cl_count[p];
-    double deficit = 0.0, surplus = 0.0;
-    int64_t n_clamp = 0;
-    bool changed = false;
-    for (int64_t k = 0; k < cnt; k++) {
-        const int64_t t = k * P + p;
-        const uint8_t fl = a.flags[t];
-        if (!(fl & EF_ACTIVE)) continue;
-        const double d = a.d[t];
-        double wn = 0.0;
-        if (!(fl & EF_ERODE)) {
-            wn = d + surplus;
-            surplus = 0.0;
-            if (deficit > 0.0) {
-                const double x = wn < deficit ? wn : deficit;
-                wn -= x;
-                deficit -= x;
+    const int64_t cap = a.cap, base = p * cap, cnt = a.cl_count[p], sbase = p * (3 * cap + 1);
+    int prev = -1, dth = -1;  // uniform
+    bool bad = false;
+    for (int64_t k0 = 0; k0 < cap; k0 += WAVE) {
+        const int64_t k = k0 + lane;
+        const bool in = k < cap, live = k < cnt;
+        const int c = live ? a.cl_cell[base + k] : 0;
+        const float v = live ? a.cl_vol[base + k] : 0.0f;
+        const double d = live ? a.cl_delta[base + k] : 0.0;
+        const int cnext = (live && k + 1 < cnt) ? a.cl_cell[base + k + 1] : -1;
+        const bool lake = live && v == -3.0f;
+        bad |= lake;
+        const bool act = live && !lake && a.mask[c] == MASK_ACTIVE;
+        const bool ero = act && d < 0.0;
+        const bool step = act && v >= 0.0f;
+        // the previous step cell: the last active step entry before this one
+        const ull sm = __ballot(step);
+        const int src = last_below(sm, lane);
+        const int csrc = __shfl(c, src < 0 ? lane : src);
+        const int prev_here = src < 0 ? prev : csrc;
+        int nb = -1;
+        if (act) nb = ero ? cnext : (step ? prev_here : cnext);
+        // the death cell: the first active final deposit
+        const ull fm = __ballot(act && !ero && !step);
+        if (dth < 0 && fm) dth = __shfl(c, __ffsll((long long)fm) - 1);
+        if (sm) prev = __shfl(c, 63 - __clzll(sm));
+        if (in) {
+            const int64_t e = base + k;
+            uint32_t* ks = a.keys_in + sbase + 3 * k;
+            uint32_t* vs = a.vals_in + sbase + 3 * k;
+            uint8_t fl = 0;
+            if (act) {
+                const float* rec = a.samp + (int64_t)c * NS;
+                fl = EF_ACTIVE | (ero ? EF_ERODE : 0) | (step ? EF_STEP : 0) | ((step && v > 0.0f) ? EF_TRACK : 0) |
+                     (nb >= 0 ? EF_NB : 0);
+                if (a.use_route && (double)rec[S_RFLAG] >= 1.5) fl |= EF_RFLOOR;
+                a.d[e] = d;
+                a.hs[e] = a.h[c] + a.s[c];
+                a.b[e] = rec[S_BASE];
+                a.r[e] = rec[S_ROUTE];
+                a.ec[e] = a.use_ecap ? a.ecap[c] : 0.0;
+                a.rs[e] = a.acc[c];
+                if (nb >= 0) {
+                    a.hsn[e] = a.h[nb] + a.s[nb];
+                    a.rn[e] = a.acc[nb];
+                }
+                ks[0] = (uint32_t)c;
+                vs[0] = (K_SELF << KIND_SHIFT) | (uint32_t)e;
+                ks[1] = nb >= 0 ? (uint32_t)nb : a.hole;
+                vs[1] = (K_NB << KIND_SHIFT) | (uint32_t)e;
+                ks[2] = (uint32_t)c;
+                vs[2] = (K_WRITE << KIND_SHIFT) | (uint32_t)e;
+            } else {
+                ks[0] = ks[1] = ks[2] = a.hole;
+                vs[0] = vs[1] = vs[2] = 0;
             }
-            if (!same_bits(wn, a.want[t])) {
-                a.want[t] = wn;
+            a.flags[e] = fl;
+            a.w[e] = 0.0;
+        }
+    }
+    if (lane == 0) {
+        a.fdirty[p] = 0;
+        a.death[p] = dth;
+        uint32_t* ks = a.keys_in + sbase + 3 * cap;
+        uint32_t* vs = a.vals_in + sbase + 3 * cap;
+        if (dth >= 0) {
+            a.hsd[p] = a.h[dth] + a.s[dth];
+            a.bd[p] = a.samp[(int64_t)dth * NS + S_BASE];
+            a.re[p] = a.acc[dth];
+            *ks = (uint32_t)dth;
+            *vs = (K_END << KIND_SHIFT) | (uint32_t)p;
+        } else {
+            *ks = a.hole;
+            *vs = 0;
+        }
+    }
+    if (__ballot(bad) && lane == 0) atomicOr(a.status, 1);
+}
+
+// 2b. sorted slot of every entry's write (the droplet half writes its amounts there)
+__global__ void __launch_bounds__(BLOCK) ord_posw_kernel(OrdArgs a) {
+    const int64_t i = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
+    if (i >= a.n_slots || a.keys_out[i] == a.hole) return;
+    const uint32_t val = a.vals_out[i];
+    if ((val >> KIND_SHIFT) == K_WRITE) {
+        a.posw[val & IDX_MASK] = (uint32_t)i;
+        a.sw[i] = 0.0;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// 3a. the droplet half: one wave per droplet
+// ---------------------------------------------------------------------------
+__device__ void droplet_wave(const OrdArgs& a, int64_t p, int sweep) {
+    const int lane = lane_id();
+    const int64_t cap = a.cap, base = p * cap, cnt = a.cl_count[p];
+    const uint32_t fd = a.fdirty[p];  // uniform
+    if (fd == NONE) return;           // no read value changed since the last walk
+    if (lane == 0) a.fdirty[p] = NONE;
+    // the carries entering entry fd: stored by the last walk (fd == 0: none yet)
+    double D = 0.0, S = 0.0;  // deficit, surplus: uniform
+    int n_clamp = 0;          // uniform
+    const int64_t k_first = fd < (uint32_t)cnt ? (int64_t)fd : cnt;
+    if (k_first > 0) {
+        if (k_first < cnt) {
+            D = a.cd[base + k_first];
+            S = a.cs[base + k_first];
+            n_clamp = a.cn[base + k_first];
+        } else {
+            D = a.fc[3 * p];
+            S = a.fc[3 * p + 1];
+            n_clamp = (int)a.fc[3 * p + 2];
+        }
+    }
+    bool changed = false;
+    for (int64_t k0 = (k_first / WAVE) * WAVE; k0 < cnt; k0 += WAVE) {
+        const int64_t k = k0 + lane, e = base + k;
+        const bool mine = k >= k_first && k < cnt;
+        const uint8_t fl = mine ? a.flags[e] : 0;
+        const bool act = fl & EF_ACTIVE, ero = fl & EF_ERODE;
+        double d = 0.0, w = 0.0, x = 0.0, U = 0.0;
+        bool clamped = false;
+        if (act) {
+            d = a.d[e];
+            const double hs = a.hs[e], ac = a.rs[e], an = a.rn[e];
+            const float b_c = a.b[e];
+            const double s_c = hs + ac;
+            const bool sub = s_c < (double)b_c;
+            if (ero) {
+                const double ev = -d;
+                double e_act = ev;
+                double rm = a.iter_erode + ac;
+                if (a.use_ecap) {
+                    const double r2 = a.ec[e] + ac;
+                    if (r2 < rm) rm = r2;
+                }
+                if (e_act > rm) e_act = rm;
+                double floor_ = s_c >= (double)b_c ? (double)b_c : s_c;
+                if (fl & EF_RFLOOR) {
+                    const float r_c = a.r[e];
+                    if ((double)r_c > floor_) floor_ = (double)r_c;
+                }
+                if (fl & EF_NB) {
+                    const double ns = a.hsn[e] + an;
+                    if (ns > floor_) floor_ = ns;
+                }
+                if (e_act > s_c - floor_) e_act = s_c - floor_;
+                if (e_act < 0.0) e_act = 0.0;
+                clamped = e_act < ev;
+                x = ev - e_act;
+                w = -e_act;
+            } else {
+                // the deposit's limits, one bound: the reference clamps by the
+                // budget, then the ceiling, then the waterline; with no NaN the
+                // successive "if (x > lim) x = lim" give the least of them
+                U = a.iter_deposit - ac;
+                if (fl & EF_NB) {
+                    double ceil_ = a.hsn[e] + an + a.dep_floor;
+                    if (sub && (fl & EF_STEP)) ceil_ -= a.dep_floor + a.fan_slope;
+                    const double lim = ceil_ - s_c;
+                    if (lim < U) U = lim;
+                }
+                if (sub) {
+                    const double lim = (double)b_c - a.dep_floor - s_c;
+                    if (lim < U) U = lim;
+                }
+            }
+        }
+        // the carries, entry after entry (deposits and clamped erosions only)
+        const ull rel = __ballot(act && (!ero || clamped));
+        ull m = rel;
+        double Da = 0.0, Sa = 0.0;  // carries after this lane's entry
+        int Na = 0;
+        const double D_in = D, S_in = S;
+        const int N_in = n_clamp;
+        while (m) {
+            const int j = __ffsll((long long)m) - 1;
+            m &= m - 1;
+            if (__builtin_amdgcn_readlane((int)ero, j)) {
+                D = D + rl(x, j);
+                n_clamp += 1;
+            } else {
+                double want = rl(d, j) + S;
+                S = 0.0;
+                if (D > 0.0) {
+                    const double t = want < D ? want : D;
+                    want -= t;
+                    D -= t;
+                }
+                const double u = rl(U, j);
+                double d_act = want > u ? u : want;
+                if (d_act < 0.0) d_act = 0.0;
+                if (d_act < want) n_clamp += 1;
+                S = want - d_act;
+                if (lane == j) w = d_act;
+            }
+            if (lane == j) {
+                Da = D;
+                Sa = S;
+                Na = n_clamp;
+            }
+        }
+        // the carries entering each entry: after the last carry-changing entry before it
+        const int lr = last_below(rel, lane);
+        const double Db = __shfl(Da, lr < 0 ? lane : lr), Sb = __shfl(Sa, lr < 0 ? lane : lr);
+        const int Nb = __shfl(Na, lr < 0 ? lane : lr);
+        if (act) {
+            a.cd[e] = lr < 0 ? D_in : Db;
+            a.cs[e] = lr < 0 ? S_in : Sb;
+            a.cn[e] = lr < 0 ? N_in : Nb;
+            if (!same_bits(w, a.w[e])) {
+                a.w[e] = w;
+                const uint32_t slot = a.posw[e];
+                a.sw[slot] = w;
+                if (sweep > 0) a.rdirty[a.owner[slot]] = 1;
                 changed = true;
             }
         }
-        bool clamped;
-        const double w = entry_amount(a, fl, d, a.hs[t], a.hsn[t], a.b[t], a.r[t], a.ec[t], a.rs[t], rn[t], wn, &clamped);
-        if (clamped) n_clamp += 1;
-        if (fl & EF_ERODE) {
-            if (clamped) deficit += -d - (-w);
-        } else {
-            surplus = wn - w;
-        }
-        if (!same_bits(w, a.w[t])) {
-            a.w[t] = w;
-            changed = true;
-        }
     }
     // the stockpile of what the droplet could not place (particle.py:1019-1041)
-    double tp = 0.0, lo = 0.0, pend = 0.0;
-    uint8_t pf = 0;
-    const int32_t dc = a.death[p];
-    if (surplus > 0.0 && dc >= 0) {
-        pf = 1;
-        if (a.hsd[p] + a.re[p] < (double)a.bd[p]) {
-            double keep = surplus * (1.0 - a.writeoff);
-            if (keep < 1e-4) keep = 0.0;
-            pend = keep;
-            tp = keep;
-            lo = surplus - keep;
-        } else {
-            pend = surplus;
-            tp = surplus;
-        }
-    }
-    double* o = a.drop + 4 * p;
-    o[0] = (double)n_clamp;
-    o[1] = tp;
-    o[2] = deficit;
-    o[3] = lo;
-    a.pend[p] = pend;
-    a.pendf[p] = pf;
-    return changed;
-}
-
-// The cell half: walk one segment in serial order with the live value,
-// recomputing each write from it and handing every read its value.
-__device__ void cell_scan(const OrdArgs& a, const double* rn_in, double* rn_out, uint32_t g) {
-    uint32_t i = a.seg[g];
-    const uint32_t cell = a.keys_out[i];
-    double v = a.acc[cell];
-    for (; i < (uint32_t)a.n_slots && a.keys_out[i] == cell; i++) {
-        const uint32_t val = a.vals_out[i];
-        const uint32_t kind = val >> KIND_SHIFT, idx = val & IDX_MASK;
-        if (kind == K_SELF) {
-            a.rs[idx] = v;
-        } else if (kind == K_NB) {
-            rn_out[idx] = v;
-        } else if (kind == K_END) {
-            a.re[idx] = v;
-        } else {
-            bool clamped;
-            const double w = entry_amount(a, a.flags[idx], a.d[idx], a.hs[idx], a.hsn[idx], a.b[idx], a.r[idx], a.ec[idx], v,
-                                          rn_in[idx], a.want[idx], &clamped);
-            a.w[idx] = w;
-            v = v + w;
-        }
-    }
-}
-
-// 3. the sweeps, in one cooperative launch.  Every block runs the same number
-// of grid barriers (the stop test reads a counter all blocks see after the
-// same barrier), so the grid leaves the loop together.
-__global__ void __launch_bounds__(SWEEP_THREADS) ord_sweeps_kernel(OrdArgs a) {
-    cg::grid_group grid = cg::this_grid();
-    const int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const int64_t nth = (int64_t)gridDim.x * blockDim.x;
-    __shared__ int any;
-    int sweep = 0;
-    for (; sweep < a.max_sweeps; sweep++) {
-        if (threadIdx.x == 0) any = 0;
-        __syncthreads();
-        bool ch = false;
-        const double* rn = a.rn[sweep & 1];
-        for (int64_t p = tid; p < a.P; p += nth) ch |= droplet_pass(a, rn, p);
-        if (ch) any = 1;
-        __syncthreads();
-        if (threadIdx.x == 0 && any) atomicAdd(&a.changes[sweep], 1u);
-        grid.sync();
-        if (__hip_atomic_load(&a.changes[sweep], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0) break;
-        const uint32_t nseg = *a.nseg;
-        for (int64_t g = tid; g < (int64_t)nseg; g += nth) cell_scan(a, rn, a.rn[(sweep & 1) ^ 1], (uint32_t)g);
-        grid.sync();
-    }
-    if (tid == 0) {
-        atomicMax(&a.status[1], sweep + 1);
-        atomicAdd(&a.status[2], sweep + 1);
-        if (sweep >= a.max_sweeps) atomicOr(a.status, 2);
-    }
-}
-
-// 4. per segment: final acc, tracks and stockpiles in serial order, packed record
-__device__ void finish_segment(const OrdArgs& a, int64_t g) {
-    uint32_t i = a.seg[g];
-    const int64_t c = a.keys_out[i];
-    double v = a.acc[c];
-    bool wrote = false;
-    for (; i < (uint32_t)a.n_slots && a.keys_out[i] == (uint32_t)c; i++) {
-        const uint32_t val = a.vals_out[i];
-        const uint32_t kind = val >> KIND_SHIFT, idx = val & IDX_MASK;
-        if (kind == K_WRITE) {
-            v = v + a.w[idx];
-            wrote = true;
-            if (a.flags[idx] & EF_TRACK) {
-                const int64_t p = idx % a.P, k = idx / a.P, e = p * a.cap + k;
-                a.disch[c] += (double)a.cl_vol[e];
-                a.mom[2 * c] += (double)a.cl_mom[2 * e];
-                a.mom[2 * c + 1] += (double)a.cl_mom[2 * e + 1];
+    if (lane == 0) {
+        a.fc[3 * p] = D;
+        a.fc[3 * p + 1] = S;
+        a.fc[3 * p + 2] = (double)n_clamp;
+        double tp = 0.0, lo = 0.0, pend = 0.0;
+        uint8_t pf = 0;
+        const int32_t dc = a.death[p];
+        if (S > 0.0 && dc >= 0) {
+            pf = 1;
+            if (a.hsd[p] + a.re[p] < (double)a.bd[p]) {
+                double keep = S * (1.0 - a.writeoff);
+                if (keep < 1e-4) keep = 0.0;
+                pend = keep;
+                tp = keep;
+                lo = S - keep;
+            } else {
+                pend = S;
+                tp = S;
             }
-        } else if (kind == K_END) {
-            if (a.pendf[idx]) a.pending[c] += a.pend[idx];
         }
+        double* o = a.drop + 4 * p;
+        o[0] = (double)n_clamp;
+        o[1] = tp;
+        o[2] = D;
+        o[3] = lo;
+        a.pend[p] = pend;
+        a.pendf[p] = pf;
     }
-    if (!wrote) return;
-    a.acc[c] = v;
-    float* rec = a.samp + c * NS;
-    rec[S_SURF] = (float)(a.h[c] + a.s[c] + v);
-    const double sd = a.s[c] + v;
-    rec[S_SED] = (float)(sd > 0.0 ? sd : 0.0);
-    if (a.use_route && (double)rec[S_RFLAG] < 1.5) rec[S_RFLAG] = rec[S_ROUTE] != rec[S_SURF] ? 1.0f : 0.0f;
+    if (__ballot(changed) && lane == 0) atomicAdd(&a.ctl[CTL_CHANGED + sweep], 1u);
 }
 
-__global__ void __launch_bounds__(256) ord_finish_kernel(OrdArgs a) {
+// ---------------------------------------------------------------------------
+// 3b. the cell half: one wave per range of sorted slots
+// ---------------------------------------------------------------------------
+// The wave takes every segment that starts in [r * RANGE, (r + 1) * RANGE)
+// and walks it to its end (possibly past the range).  Per batch of 64 slots:
+// coalesced loads, then the segment's running value over its writes in order
+// (one lane after the other), then each read's value from the last write
+// before it (or the segment's start value, or the running value carried in).
+struct Batch {
+    uint32_t key, val;
+    bool valid, start, write;
+};
+
+__device__ __forceinline__ bool first_start(const OrdArgs& a, int64_t r, int64_t* out) {
+    const int lane = lane_id();
+    const int64_t lo = r * RANGE, hi = (r + 1) * RANGE < a.n_slots ? (r + 1) * RANGE : a.n_slots;
+    for (int64_t i0 = lo; i0 < hi; i0 += WAVE) {
+        const int64_t i = i0 + lane;
+        bool st = false;
+        if (i < hi) {
+            const uint32_t k = a.keys_out[i];
+            st = k != a.hole && (i == 0 || a.keys_out[i - 1] != k);
+        }
+        const ull m = __ballot(st);
+        if (m) {
+            *out = i0 + __ffsll((long long)m) - 1;
+            return true;
+        }
+    }
+    return false;
+}
+
+template <bool FINISH>
+__device__ void range_wave(const OrdArgs& a, int64_t r, int sweep) {
+    const int lane = lane_id();
+    if (!FINISH && sweep > 0) {
+        if (!a.rdirty[r]) return;  // no write of its segments changed: every read keeps its value
+        if (lane == 0) a.rdirty[r] = 0;
+    }
+    int64_t i0;
+    if (!first_start(a, r, &i0)) return;
+    const int64_t hi = (r + 1) * RANGE;  // segments starting at or past hi belong to the next range
+    double v = 0.0;                      // running value of the current segment (uniform)
+    double dv = 0.0, m0 = 0.0, m1 = 0.0, pv = 0.0;  // finish: tracks and stockpile running values
+    bool wrote = false, tracked = false, parked = false;
+    uint32_t cur = 0xffffffffu;
+    auto write_back = [&]() {
+        if (!FINISH || cur == 0xffffffffu || lane != 0) return;
+        if (wrote) {
+            a.acc[cur] = v;
+            float* rec = a.samp + (int64_t)cur * NS;
+            rec[S_SURF] = (float)(a.h[cur] + a.s[cur] + v);
+            const double sd = a.s[cur] + v;
+            rec[S_SED] = (float)(sd > 0.0 ? sd : 0.0);
+            if (a.use_route && (double)rec[S_RFLAG] < 1.5) rec[S_RFLAG] = rec[S_ROUTE] != rec[S_SURF] ? 1.0f : 0.0f;
+        }
+        if (tracked) {
+            a.disch[cur] = dv;
+            a.mom[2 * (int64_t)cur] = m0;
+            a.mom[2 * (int64_t)cur + 1] = m1;
+        }
+        if (parked) a.pending[cur] = pv;
+    };
+    for (int64_t base = i0;; base += WAVE) {
+        const int64_t i = base + lane;
+        uint32_t key = a.hole, val = 0, prevkey = 0xffffffffu;
+        if (i < a.n_slots) {
+            key = a.keys_out[i];
+            val = a.vals_out[i];
+            if (i > 0) prevkey = a.keys_out[i - 1];
+        }
+        const bool start = key != a.hole && (i == 0 || prevkey != key);
+        // stop at the first start at or past hi, or at a hole (the tail of the sort)
+        const ull stop_m = __ballot(key == a.hole || (start && i >= hi));
+        const int stop = stop_m ? __ffsll((long long)stop_m) - 1 : WAVE;
+        const bool valid = lane < stop;
+        const uint32_t kind = val >> KIND_SHIFT, idx = val & IDX_MASK;
+        const bool write = valid && kind == K_WRITE;
+        double wv = 0.0, a0 = 0.0;
+        if (write) wv = a.sw[i];
+        if (valid && start) a0 = a.acc[key];
+        double tv = 0.0, t0 = 0.0, t1 = 0.0, pk = 0.0, d0 = 0.0, mm0 = 0.0, mm1 = 0.0, p0 = 0.0;
+        bool trk = false, pnd = false;
+        if (FINISH) {
+            if (write && (a.flags[idx] & EF_TRACK)) {
+                trk = true;
+                tv = (double)a.cl_vol[idx];
+                t0 = (double)a.cl_mom[2 * (int64_t)idx];
+                t1 = (double)a.cl_mom[2 * (int64_t)idx + 1];
+            }
+            if (valid && kind == K_END && a.pendf[idx]) {
+                pnd = true;
+                pk = a.pend[idx];
+            }
+            if (valid && start) {
+                d0 = a.disch[key];
+                mm0 = a.mom[2 * (int64_t)key];
+                mm1 = a.mom[2 * (int64_t)key + 1];
+                p0 = a.pending[key];
+            }
+        }
+        const ull sm = __ballot(valid && start), wm = __ballot(write);
+        const ull tm = FINISH ? __ballot(trk) : 0ull, pm = FINISH ? __ballot(pnd) : 0ull;
+        const double v_in = v;
+        // the serial part: segment starts and writes (and the finish's sums), in slot order
+        double vafter = 0.0;
+        ull m = sm | wm | tm | pm;
+        while (m) {
+            const int j = __ffsll((long long)m) - 1;
+            m &= m - 1;
+            const ull bit = 1ull << j;
+            if (sm & bit) {
+                write_back();
+                cur = (uint32_t)rli((int)key, j);
+                v = rl(a0, j);
+                wrote = tracked = parked = false;
+                if (FINISH) {
+                    dv = rl(d0, j);
+                    m0 = rl(mm0, j);
+                    m1 = rl(mm1, j);
+                    pv = rl(p0, j);
+                }
+            }
+            if (wm & bit) {
+                v = v + rl(wv, j);
+                wrote = true;
+                if (lane == j) vafter = v;
+            }
+            if (FINISH && (tm & bit)) {
+                dv = dv + rl(tv, j);
+                m0 = m0 + rl(t0, j);
+                m1 = m1 + rl(t1, j);
+                tracked = true;
+            }
+            if (FINISH && (pm & bit)) {
+                pv = pv + rl(pk, j);
+                parked = true;
+            }
+        }
+        if (!FINISH) {
+            // each read's value: the running value just before it -- after the last
+            // write before it in its segment, else its segment's start value, else
+            // the value carried into the batch
+            const int lw = last_below(wm, lane), ls = last_upto(sm, lane);
+            const double from_w = __shfl(vafter, lw < 0 ? lane : lw);
+            const double from_s = __shfl(a0, ls < 0 ? lane : ls);
+            if (valid) {
+                if (sweep == 0) a.owner[i] = (uint32_t)r;
+                if (kind != K_WRITE) {
+                    const double rv = (lw >= 0 && lw > ls) ? from_w : (ls >= 0 ? from_s : v_in);
+                    double* dst = kind == K_SELF ? a.rs + idx : (kind == K_NB ? a.rn + idx : a.re + idx);
+                    if (!same_bits(rv, *dst)) {
+                        *dst = rv;
+                        // the droplet reading it walks again from this entry
+                        const int64_t dp = kind == K_END ? (int64_t)idx : (int64_t)idx / a.cap;
+                        const uint32_t dk = kind == K_END ? (uint32_t)a.cap : (uint32_t)((int64_t)idx - dp * a.cap);
+                        atomicMin(a.fdirty + dp, dk);
+                    }
+                }
+            }
+        }
+        if (stop < WAVE || base + WAVE >= a.n_slots) break;
+    }
+    write_back();
+}
+
+// 3. the sweep launches: a droplet half that changed nothing ends the chunk
+__global__ void __launch_bounds__(BLOCK) ord_droplets_kernel(OrdArgs a, int sweep) {
+    if (a.ctl[CTL_DONE]) return;
+    const int64_t p = (int64_t)blockIdx.x * WAVES_PER_BLOCK + (threadIdx.x / WAVE);
+    if (p < a.P) droplet_wave(a, p, sweep);
+}
+
+__global__ void __launch_bounds__(BLOCK) ord_cells_kernel(OrdArgs a, int sweep) {
+    if (a.ctl[CTL_DONE]) return;
+    if (a.ctl[CTL_CHANGED + sweep] == 0) {
+        if (blockIdx.x == 0 && threadIdx.x == 0) {
+            a.ctl[CTL_DONE] = 1;
+            a.ctl[CTL_SWEEPS] = (uint32_t)sweep + 1;
+        }
+        return;
+    }
+    const int64_t r = (int64_t)blockIdx.x * WAVES_PER_BLOCK + (threadIdx.x / WAVE);
+    if (r < a.n_ranges) range_wave<false>(a, r, sweep);
+}
+
+// The sweeps past the launched ones, in one cooperative launch.  Every block
+// passes the same grid barriers: the stop test reads a word all blocks see
+// after the same barrier, so the grid leaves the loop together.
+__global__ void __launch_bounds__(BLOCK) ord_tail_kernel(OrdArgs a, int first) {
+    cg::grid_group grid = cg::this_grid();
+    if (a.ctl[CTL_DONE]) return;  // read by every block before any block can set it below
+    const int64_t wv = (int64_t)blockIdx.x * WAVES_PER_BLOCK + (threadIdx.x / WAVE);
+    const int64_t nw = (int64_t)gridDim.x * WAVES_PER_BLOCK;
+    int sweep = first;
+    for (; sweep < a.max_sweeps; sweep++) {
+        for (int64_t p = wv; p < a.P; p += nw) droplet_wave(a, p, sweep);
+        grid.sync();
+        if (__hip_atomic_load(&a.ctl[CTL_CHANGED + sweep], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0) break;
+        for (int64_t r = wv; r < a.n_ranges; r += nw) range_wave<false>(a, r, sweep);
+        grid.sync();
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        if (sweep < a.max_sweeps) {
+            a.ctl[CTL_DONE] = 1;
+            a.ctl[CTL_SWEEPS] = (uint32_t)sweep + 1;
+        } else {
+            atomicOr(a.status, 2);
+        }
+    }
+}
+
+// 4. finish: final acc, tracks, stockpiles and packed record per segment
+__global__ void __launch_bounds__(BLOCK) ord_finish_kernel(OrdArgs a) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        const int sw = a.ctl[CTL_DONE] ? (int)a.ctl[CTL_SWEEPS] : a.max_sweeps;
+        atomicMax(&a.status[1], sw);
+        atomicAdd(&a.status[2], sw);
+    }
     if (a.status[0] != 0) return;
-    const int64_t n = (int64_t)*a.nseg;
-    for (int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; g < n; g += (int64_t)gridDim.x * blockDim.x)
-        finish_segment(a, g);
+    const int64_t r = (int64_t)blockIdx.x * WAVES_PER_BLOCK + (threadIdx.x / WAVE);
+    if (r < a.n_ranges) range_wave<true>(a, r, 0);
 }
 
 // 5. the chunk's (clamped, to pending, deficit lost, lost offshore), summed in
 // droplet order: the block stages up to STAT_TILE droplets' values in LDS
 // (coalesced loads), then one lane adds them in order.
 constexpr int STAT_TILE = 1024;
-__global__ void __launch_bounds__(256) ord_stats_kernel(OrdArgs a) {
+__global__ void __launch_bounds__(BLOCK) ord_stats_kernel(OrdArgs a) {
     __shared__ double tile[3 * STAT_TILE];
     __shared__ unsigned long long n_clamp;
     if (threadIdx.x == 0) n_clamp = 0;
@@ -493,16 +731,20 @@ __global__ void __launch_bounds__(256) ord_stats_kernel(OrdArgs a) {
     }
 }
 
-int sweep_grid(int* blocks) {
+__global__ void ord_tail_kernel_flag(OrdArgs a) {
+    if (!a.ctl[CTL_DONE]) atomicOr(a.status, 2);
+}
+
+int tail_grid(int* blocks) {
     static int cached = 0;
     if (cached == 0) {
         int dev = 0, per_cu = 0;
         hipDeviceProp_t prop;
         if (hipGetDevice(&dev) != hipSuccess || hipGetDeviceProperties(&prop, dev) != hipSuccess) return -1;
-        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)ord_sweeps_kernel, SWEEP_THREADS, 0) != hipSuccess)
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)ord_tail_kernel, BLOCK, 0) != hipSuccess)
             return -1;
-        int n = per_cu * prop.multiProcessorCount;
-        cached = n < 128 ? n : 128;
+        const int n = per_cu * prop.multiProcessorCount;
+        cached = n < 64 ? n : 64;
         if (cached < 1) cached = 1;
     }
     *blocks = cached;
@@ -528,21 +770,23 @@ extern "C" int64_t pg_ordered_workspace_bytes(int64_t P, int64_t cap, int max_sw
     return (int64_t)L.total;
 }
 
-extern "C" int pg_apply_changes_ordered(const int32_t* cl_cell, const double* cl_delta, const float* cl_vol,
-                                        const float* cl_mom, const int64_t* cl_count, int64_t P, int64_t cap,
-                                        const double* height, const double* sediment, double* acc, double* pending,
-                                        float* samp, double* disch_track, double* mom_track, const uint8_t* mask,
-                                        int64_t n_cells, double iter_erode, double iter_deposit, double fan_slope,
-                                        int use_route, double dep_floor, double offshore_writeoff,
-                                        const double* erode_cap, int use_erode_cap, double* out4, int32_t* status,
-                                        int max_sweeps, void* work, int64_t work_bytes, pg_stream stream) {
-    clear_error();
+namespace pg {
+namespace {
+// drop_out: NULL -> the 4-tuple is summed into out4 here; else each droplet's
+// contributions land in drop_out[4 * p ...] (the caller sums them in order)
+int ordered_enqueue(const int32_t* cl_cell, const double* cl_delta, const float* cl_vol, const float* cl_mom,
+                    const int64_t* cl_count, int64_t P, int64_t cap, const double* height, const double* sediment,
+                    double* acc, double* pending, float* samp, double* disch_track, double* mom_track,
+                    const uint8_t* mask, int64_t n_cells, double iter_erode, double iter_deposit, double fan_slope,
+                    int use_route, double dep_floor, double offshore_writeoff, const double* erode_cap,
+                    int use_erode_cap, double* out4, int32_t* status, int max_sweeps, int launched_sweeps, void* work,
+                    int64_t work_bytes, hipStream_t stream, double* drop_out) {
     PG_REQUIRE(P >= 0 && cap > 0 && n_cells > 0, "pg_apply_changes_ordered: bad sizes P=%lld cap=%lld n_cells=%lld",
                (long long)P, (long long)cap, (long long)n_cells);
-    PG_REQUIRE(max_sweeps >= 1, "pg_apply_changes_ordered: max_sweeps must be >= 1");
-    PG_REQUIRE(out4 && status, "pg_apply_changes_ordered: out4 / status is NULL");
+    PG_REQUIRE(max_sweeps >= 1 && launched_sweeps >= 0, "pg_apply_changes_ordered: max_sweeps must be >= 1, launched_sweeps >= 0");
+    PG_REQUIRE((out4 || drop_out) && status, "pg_apply_changes_ordered: out4 / status is NULL");
     PG_REQUIRE(!use_erode_cap || erode_cap, "pg_apply_changes_ordered: use_erode_cap without erode_cap");
-    PG_REQUIRE((uint64_t)n_cells < 0xffffffffull, "pg_apply_changes_ordered: %lld cells exceed 32-bit cell ids",
+    PG_REQUIRE((uint64_t)n_cells < 0x7fffffffull, "pg_apply_changes_ordered: %lld cells exceed 31-bit cell ids",
                (long long)n_cells);
     PG_REQUIRE((uint64_t)P * (uint64_t)cap <= IDX_MASK, "pg_apply_changes_ordered: P * cap = %lld exceeds %u entries",
                (long long)(P * cap), IDX_MASK);
@@ -551,16 +795,16 @@ extern "C" int pg_apply_changes_ordered(const int32_t* cl_cell, const double* cl
     PG_REQUIRE(layout_of(P, cap, max_sweeps, &L) == 0, "pg_apply_changes_ordered: hipcub temp-size query failed");
     PG_REQUIRE(work && work_bytes >= (int64_t)L.total, "pg_apply_changes_ordered: workspace of %lld bytes, %lld needed",
                (long long)work_bytes, (long long)L.total);
-    hipStream_t s = (hipStream_t)stream;
+    hipStream_t s = stream;
     if (P == 0) {
-        PG_HIP(hipMemsetAsync(out4, 0, 4 * sizeof(double), s));
+        if (out4) PG_HIP(hipMemsetAsync(out4, 0, 4 * sizeof(double), s));
         return 0;
     }
     PG_REQUIRE(cl_cell && cl_delta && cl_vol && cl_mom && cl_count && height && sediment && acc && pending && samp &&
                    disch_track && mom_track && mask,
                "pg_apply_changes_ordered: NULL array");
     int nbits = 1;
-    while (nbits < 32 && ((uint64_t)1 << nbits) <= (uint64_t)n_cells) nbits++;
+    while (nbits < 31 && ((uint64_t)1 << nbits) <= (uint64_t)n_cells) nbits++;
     OrdArgs a{};
     a.cl_cell = cl_cell;
     a.cl_delta = cl_delta;
@@ -593,46 +837,164 @@ extern "C" int pg_apply_changes_ordered(const int32_t* cl_cell, const double* cl
     a.b = at<float>(work, L.b);
     a.r = at<float>(work, L.r);
     a.rs = at<double>(work, L.rs);
-    a.rn[0] = at<double>(work, L.rn);
-    a.rn[1] = at<double>(work, L.rn2);
+    a.rn = at<double>(work, L.rn);
     a.w = at<double>(work, L.w);
-    a.want = at<double>(work, L.want);
+    a.posw = at<uint32_t>(work, L.posw);
+    a.cd = at<double>(work, L.cd);
+    a.cs = at<double>(work, L.cs);
+    a.cn = at<int32_t>(work, L.cn);
+    a.fdirty = at<uint32_t>(work, L.fdirty);
+    a.fc = at<double>(work, L.fc);
+    a.owner = at<uint32_t>(work, L.owner);
+    a.rdirty = at<uint8_t>(work, L.rdirty);
     a.death = at<int32_t>(work, L.death);
     a.hsd = at<double>(work, L.hsd);
     a.bd = at<float>(work, L.bd);
     a.re = at<double>(work, L.re);
-    a.drop = at<double>(work, L.drop);
+    a.drop = drop_out ? drop_out : at<double>(work, L.drop);
     a.pend = at<double>(work, L.pend);
     a.pendf = at<uint8_t>(work, L.pendf);
     a.keys_in = at<uint32_t>(work, L.keys_in);
     a.vals_in = at<uint32_t>(work, L.vals_in);
     a.keys_out = at<uint32_t>(work, L.keys_out);
     a.vals_out = at<uint32_t>(work, L.vals_out);
-    a.seg = at<uint32_t>(work, L.seg);
-    a.nseg = at<uint32_t>(work, L.nseg);
-    a.changes = at<uint32_t>(work, L.changes);
+    a.sw = at<double>(work, L.sw);
+    a.ctl = at<uint32_t>(work, L.ctl);
     a.status = status;
     a.hole = (uint32_t)(((uint64_t)1 << nbits) - 1);
     a.n_slots = P * (3 * cap + 1);
+    a.n_ranges = (a.n_slots + RANGE - 1) / RANGE;
     a.max_sweeps = max_sweeps;
     a.out4 = out4;
-    hipLaunchKernelGGL(ord_build_kernel, dim3(blocks_for(P, 256)), dim3(256), 0, s, a);
+    const int dblocks = (int)((P + WAVES_PER_BLOCK - 1) / WAVES_PER_BLOCK);
+    const int rblocks = (int)((a.n_ranges + WAVES_PER_BLOCK - 1) / WAVES_PER_BLOCK);
+    hipLaunchKernelGGL(ord_build_kernel, dim3(dblocks), dim3(BLOCK), 0, s, a);
     PG_LAUNCHED("ord_build_kernel");
     size_t tb = L.temp_bytes;
     PG_HIP(hipcub::DeviceRadixSort::SortPairs(at<void>(work, L.temp), tb, a.keys_in, a.keys_out, a.vals_in, a.vals_out,
                                               (int)a.n_slots, 0, nbits, s));
-    hipLaunchKernelGGL(ord_segments_kernel, dim3(blocks_for(a.n_slots, 256)), dim3(256), 0, s, a);
-    PG_LAUNCHED("ord_segments_kernel");
-    int blocks = 0;
-    PG_REQUIRE(sweep_grid(&blocks) == 0, "pg_apply_changes_ordered: occupancy query failed");
-    void* kargs[] = {&a};
-    PG_HIP(hipLaunchCooperativeKernel((const void*)ord_sweeps_kernel, dim3(blocks), dim3(SWEEP_THREADS), kargs, 0, s));
-    // the finish pass may touch every segment; its grid covers the largest possible count
-    int fblocks = blocks_for(a.n_slots, 256);
-    if (fblocks > 512) fblocks = 512;
-    hipLaunchKernelGGL(ord_finish_kernel, dim3(fblocks), dim3(256), 0, s, a);
+    hipLaunchKernelGGL(ord_posw_kernel, dim3(blocks_for(a.n_slots, BLOCK)), dim3(BLOCK), 0, s, a);
+    PG_LAUNCHED("ord_posw_kernel");
+    const int launched = launched_sweeps < max_sweeps ? launched_sweeps : max_sweeps;
+    for (int sw = 0; sw < launched; sw++) {
+        hipLaunchKernelGGL(ord_droplets_kernel, dim3(dblocks), dim3(BLOCK), 0, s, a, sw);
+        hipLaunchKernelGGL(ord_cells_kernel, dim3(rblocks), dim3(BLOCK), 0, s, a, sw);
+    }
+    PG_LAUNCHED("ord_droplets_kernel / ord_cells_kernel");
+    if (launched < max_sweeps) {
+        int blocks = 0;
+        PG_REQUIRE(tail_grid(&blocks) == 0, "pg_apply_changes_ordered: occupancy query failed");
+        int first = launched;
+        void* kargs[] = {&a, &first};
+        PG_HIP(hipLaunchCooperativeKernel((const void*)ord_tail_kernel, dim3(blocks), dim3(BLOCK), kargs, 0, s));
+    } else {
+        // every sweep was launched: a chunk still running after them did not converge
+        hipLaunchKernelGGL(ord_tail_kernel_flag, dim3(1), dim3(1), 0, s, a);
+        PG_LAUNCHED("ord_tail_kernel_flag");
+    }
+    hipLaunchKernelGGL(ord_finish_kernel, dim3(rblocks), dim3(BLOCK), 0, s, a);
     PG_LAUNCHED("ord_finish_kernel");
-    hipLaunchKernelGGL(ord_stats_kernel, dim3(1), dim3(256), 0, s, a);
-    PG_LAUNCHED("ord_stats_kernel");
+    if (!drop_out) {
+        hipLaunchKernelGGL(ord_stats_kernel, dim3(1), dim3(BLOCK), 0, s, a);
+        PG_LAUNCHED("ord_stats_kernel");
+    }
+    return 0;
+}
+
+// Each chunk's 4-tuple from its droplets' contributions, in droplet order:
+// one block per chunk (stage in LDS, one lane adds in order), chunks in parallel.
+__global__ void __launch_bounds__(BLOCK) ord_chunk_sums_kernel(const double* drop, int64_t P, int64_t chunk, double* outs) {
+    __shared__ double tile[3 * STAT_TILE];
+    __shared__ unsigned long long n_clamp;
+    const int64_t c0 = (int64_t)blockIdx.x * chunk;
+    const int64_t m_all = P - c0 < chunk ? P - c0 : chunk;
+    if (threadIdx.x == 0) n_clamp = 0;
+    double tp = 0.0, lost = 0.0, lo = 0.0;
+    for (int64_t p0 = 0; p0 < m_all; p0 += STAT_TILE) {
+        const int64_t m = m_all - p0 < STAT_TILE ? m_all - p0 : STAT_TILE;
+        __syncthreads();
+        unsigned long long nc = 0;
+        for (int64_t j = threadIdx.x; j < m; j += blockDim.x) {
+            const double* o = drop + 4 * (c0 + p0 + j);
+            nc += (unsigned long long)o[0];
+            tile[3 * j] = o[1];
+            tile[3 * j + 1] = o[2];
+            tile[3 * j + 2] = o[3];
+        }
+        if (nc) atomicAdd(&n_clamp, nc);
+        __syncthreads();
+        if (threadIdx.x == 0)
+            for (int64_t j = 0; j < m; j++) {
+                tp += tile[3 * j];
+                lost += tile[3 * j + 1];
+                lo += tile[3 * j + 2];
+            }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double* o = outs + 4 * blockIdx.x;
+        o[0] = (double)n_clamp;
+        o[1] = tp;
+        o[2] = lost;
+        o[3] = lo;
+    }
+}
+}  // namespace
+}  // namespace pg
+
+extern "C" int pg_apply_changes_ordered(const int32_t* cl_cell, const double* cl_delta, const float* cl_vol,
+                                        const float* cl_mom, const int64_t* cl_count, int64_t P, int64_t cap,
+                                        const double* height, const double* sediment, double* acc, double* pending,
+                                        float* samp, double* disch_track, double* mom_track, const uint8_t* mask,
+                                        int64_t n_cells, double iter_erode, double iter_deposit, double fan_slope,
+                                        int use_route, double dep_floor, double offshore_writeoff,
+                                        const double* erode_cap, int use_erode_cap, double* out4, int32_t* status,
+                                        int max_sweeps, int launched_sweeps, void* work, int64_t work_bytes,
+                                        pg_stream stream) {
+    clear_error();
+    PG_REQUIRE(out4 != nullptr, "pg_apply_changes_ordered: out4 is NULL");
+    return ordered_enqueue(cl_cell, cl_delta, cl_vol, cl_mom, cl_count, P, cap, height, sediment, acc, pending, samp,
+                           disch_track, mom_track, mask, n_cells, iter_erode, iter_deposit, fan_slope, use_route,
+                           dep_floor, offshore_writeoff, erode_cap, use_erode_cap, out4, status, max_sweeps,
+                           launched_sweeps, work, work_bytes, (hipStream_t)stream, nullptr);
+}
+
+extern "C" int pg_check_chunks(int64_t P, int64_t chunk, const int64_t* sp_face, const double* sp_x, const double* sp_y,
+                               const double* sp_sed, const double* sp_dir, double volume0, float* samp, int use_route,
+                               const uint8_t* mask, int64_t N, int64_t H, int spherical, const pg_trace_params* prm,
+                               const float* roots, int use_roots, int32_t* cl_cell, double* cl_delta, float* cl_vol,
+                               float* cl_mom, int64_t* cl_count, int64_t* cl_lo, int64_t* cl_hi, int8_t* sp_death,
+                               const double* height, const double* sediment, double* acc, double* pending,
+                               double* disch_track, double* mom_track, double iter_erode, double iter_deposit,
+                               double fan_slope, double dep_floor, double offshore_writeoff, const double* erode_cap,
+                               int use_erode_cap, double* outs, int64_t* counts7, int32_t* status, int max_sweeps,
+                               int launched_sweeps, void* work, int64_t work_bytes, double* drop_out,
+                               pg_stream stream) {
+    clear_error();
+    PG_REQUIRE(P >= 0 && chunk >= 1 && N >= 2 && H >= 1, "pg_check_chunks: bad sizes P=%lld chunk=%lld", (long long)P,
+               (long long)chunk);
+    PG_REQUIRE(prm && outs && counts7 && status && drop_out, "pg_check_chunks: NULL argument");
+    const int64_t NE = N + 2 * H, F = spherical ? 6 : 1;
+    const int64_t cap = pg_change_list_cap(prm->max_steps, prm->lateral_rate > 0.0);
+    for (int64_t c0 = 0, ci = 0; c0 < P; c0 += chunk, ci++) {
+        const int64_t m = P - c0 < chunk ? P - c0 : chunk;
+        if (pg_trace_particles(m, sp_face + c0, sp_x + c0, sp_y + c0, sp_sed + c0, sp_dir + c0, volume0, samp, use_route,
+                               mask, N, H, spherical, prm, roots, use_roots, cl_cell, cl_delta, cl_vol, cl_mom, cl_count,
+                               cl_lo, cl_hi, sp_death, stream) != 0)
+            return -1;
+        if (ordered_enqueue(cl_cell, cl_delta, cl_vol, cl_mom, cl_count, m, cap, height, sediment, acc, pending, samp,
+                            disch_track, mom_track, mask, F * NE * NE, iter_erode, iter_deposit, fan_slope, use_route,
+                            dep_floor, offshore_writeoff, erode_cap, use_erode_cap, nullptr, status, max_sweeps,
+                            launched_sweeps, work, work_bytes, (hipStream_t)stream, drop_out + 4 * c0) != 0)
+            return -1;
+        if (pg_chunk_stats(cl_count, sp_death, m, counts7, stream) != 0) return -1;
+        (void)ci;
+    }
+    if (P > 0) {
+        const int64_t n_chunks = (P + chunk - 1) / chunk;
+        hipLaunchKernelGGL(ord_chunk_sums_kernel, dim3((unsigned)n_chunks), dim3(BLOCK), 0, (hipStream_t)stream, drop_out, P,
+                           chunk, outs);
+        PG_LAUNCHED("ord_chunk_sums_kernel");
+    }
     return 0;
 }

@@ -82,7 +82,10 @@ SIGNATURES = [
                                vp, i32, vp, vp, vp, vp, vp]),
     ("pg_ordered_workspace_bytes", i64, [i64, i64, i32]),
     ("pg_apply_changes_ordered", i32, [vp, vp, vp, vp, vp, i64, i64, vp, vp, vp, vp, vp, vp, vp, vp, i64, dbl, dbl, dbl,
-                                       i32, dbl, dbl, vp, i32, vp, vp, i32, vp, i64, vp]),
+                                       i32, dbl, dbl, vp, i32, vp, vp, i32, i32, vp, i64, vp]),
+    ("pg_check_chunks", i32, [i64, i64, vp, vp, vp, vp, vp, dbl, vp, i32, vp, i64, i64, i32, ctypes.POINTER(TraceParams),
+                              vp, i32, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, dbl, dbl, dbl, dbl, dbl,
+                              vp, i32, vp, vp, vp, i32, i32, vp, i64, vp, vp]),
     ("pg_chunk_stats", i32, [vp, vp, i64, vp, vp]),
     ("pg_fold_changes", i32, [i64, vp, vp, vp, vp, vp]),
     ("pg_ema_update", i32, [i64, vp, vp, vp, vp, vp, dbl, vp]),

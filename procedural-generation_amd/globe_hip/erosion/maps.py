@@ -631,6 +631,15 @@ def lake_park_of(state, ep) -> float:
 # ---------------------------------------------------------------------------
 # one iteration (maps.py:839-1047)
 # ---------------------------------------------------------------------------
+def _scratch(state: ErosionState, name: str, n: int, dtype) -> torch.Tensor:
+    """A per-state device scratch tensor of at least n elements, reused across iterations."""
+    t = state._ordered.get(name)
+    if t is None or t.numel() < n or t.dtype != dtype:
+        t = torch.empty(max(n, 1), dtype=dtype, device=state.device)
+        state._ordered[name] = t
+    return t
+
+
 def run_iteration(state: ErosionState, params, iteration_key, n_particles=None, particles_per_cell=None,
                   rng_stage="erosion", log=None, diag=None, spawn=None) -> dict:
     """The particle part of an iteration: spawn (rain + re-injected
@@ -710,11 +719,20 @@ def run_iteration(state: ErosionState, params, iteration_key, n_particles=None, 
         key = (chunk, cap, int(state.check_max_sweeps))
         if state._ordered.get("key") != key:
             nbytes = pk.ordered_workspace_bytes(chunk, cap, int(state.check_max_sweeps))
-            state._ordered = {"key": key, "work": torch.empty(nbytes, dtype=torch.uint8, device=dev),
-                              "status": torch.zeros(4, dtype=torch.int32, device=dev)}
+            state._ordered.update({"key": key, "work": torch.empty(nbytes, dtype=torch.uint8, device=dev),
+                                   "status": torch.zeros(4, dtype=torch.int32, device=dev)})
         ord_work, ord_status = state._ordered["work"], state._ordered["status"]
         ord_status.zero_()
-    for ci, c0 in enumerate(range(0, P, chunk)):
+    t1 = time.time()
+    if ordered and diag is None:
+        # the whole chunk loop in one library call (no host round trip per chunk)
+        pk.check_chunks(sp_face, sp_x, sp_y, sp_sed, sp_dir, volume0, chunk, state.samp, state.route is not None,
+                        state.mask, state.N, state.H, state.spherical, prm, state.roots, use_roots, cl_cell, cl_delta,
+                        cl_vol, cl_mom, cl_count, cl_lo, cl_hi, sp_death, state.height, state.sediment, state.acc,
+                        state.pending, state.disch_track, state.mom_track, iter_erode, iter_deposit, float(ep.fan_slope),
+                        dep_floor, float(ep.offshore_writeoff), ecap, ecap is not None, outs, counts, ord_status,
+                        ord_work, _scratch(state, "ordered_drops", 4 * P, torch.float64), int(state.check_max_sweeps))
+    for ci, c0 in enumerate(range(0, P, chunk) if not (ordered and diag is None) else ()):
         c1 = min(P, c0 + chunk)
         m = c1 - c0
         t1 = time.time()

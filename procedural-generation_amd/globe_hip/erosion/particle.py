@@ -149,13 +149,15 @@ def ordered_workspace_bytes(P: int, cap: int, max_sweeps: int) -> int:
 
 def apply_changes_ordered(cl_cell, cl_delta, cl_vol, cl_mom, cl_count, cap, height, sediment, acc, pending, samp,
                           disch_track, mom_track, mask, iter_erode, iter_deposit, fan_slope, use_route, dep_floor,
-                          offshore_writeoff, erode_cap, use_erode_cap, out4, status, work, max_sweeps):
+                          offshore_writeoff, erode_cap, use_erode_cap, out4, status, work, max_sweeps,
+                          launched_sweeps=40):
     """particle.py:821-1042 computed in parallel with the serial apply's
     results to the bit (fixed-point sweeps over the serial order).  ``status``
     (int32[4], zeroed per iteration by the caller) collects the error bits
     (1: lake parks, unsupported; 2: no convergence within ``max_sweeps``)
     and the sweep counts; ``work`` is a uint8 scratch tensor of
-    ``ordered_workspace_bytes`` bytes."""
+    ``ordered_workspace_bytes`` bytes.  The first ``launched_sweeps`` sweeps
+    are plain launches, the rest one cooperative launch."""
     P = cl_count.shape[0]
     e = tuple(height.shape)
     for t, n in ((height, "height"), (sediment, "sediment"), (acc, "acc"), (pending, "pending"), (disch_track, "disch_track")):
@@ -182,8 +184,71 @@ def apply_changes_ordered(cl_cell, cl_delta, cl_vol, cl_mom, cl_count, cap, heig
                                          _ptr(disch_track), _ptr(mom_track), _ptr(mask), n_cells, float(iter_erode),
                                          float(iter_deposit), float(fan_slope), int(bool(use_route)), float(dep_floor),
                                          float(offshore_writeoff), _ptr(erode_cap if use_erode_cap else None),
-                                         int(bool(use_erode_cap)), _ptr(out4), _ptr(status), int(max_sweeps), _ptr(work),
+                                         int(bool(use_erode_cap)), _ptr(out4), _ptr(status), int(max_sweeps),
+                                         int(launched_sweeps), _ptr(work),
                                          int(work.numel()), _stream()), "apply_changes_ordered")
+
+
+def check_chunks(sp_face, sp_x, sp_y, sp_sed, sp_dir, volume0, chunk, samp, use_route, mask, N, H, spherical,
+                 params: TraceParams, roots, use_roots, cl_cell, cl_delta, cl_vol, cl_mom, cl_count, cl_lo, cl_hi,
+                 sp_death, height, sediment, acc, pending, disch_track, mom_track, iter_erode, iter_deposit, fan_slope,
+                 dep_floor, offshore_writeoff, erode_cap, use_erode_cap, outs, counts7, status, work, drop_out,
+                 max_sweeps, launched_sweeps=40):
+    """The check mode's chunk loop (maps.py:953-1029) in one call: per chunk
+    of ``chunk`` droplets trace_particles, the parallel apply and the chunk
+    statistics, enqueued without a host round trip.  ``outs`` (float64
+    [n_chunks, 4]) receives each chunk's 4-tuple, summed from ``drop_out``
+    (float64 [P, 4] scratch)."""
+    P = sp_face.shape[0]
+    F, NE = samp.shape[0], samp.shape[1]
+    e = (F, NE, NE)
+    if NE != N + 2 * H:
+        raise ValueError(f"samp width {NE} != N + 2H = {N + 2 * H}")
+    _need(sp_face, torch.int64, (P,), "sp_face")
+    for t, n in ((sp_x, "sp_x"), (sp_y, "sp_y"), (sp_sed, "sp_sed"), (sp_dir, "sp_dir")):
+        _need(t, torch.float64, (P,), n)
+    _need(samp, torch.float32, e + (NS,), "samp")
+    _need(mask, torch.uint8, e, "mask")
+    for t, n in ((height, "height"), (sediment, "sediment"), (acc, "acc"), (pending, "pending"), (disch_track, "disch_track")):
+        _need(t, torch.float64, e, n)
+    _need(mom_track, torch.float64, e + (2,), "mom_track")
+    cap = change_list_cap(params.max_steps, params.lateral_rate > 0.0)
+    for t, dt, n in ((cl_cell, torch.int32, "cl_cell"), (cl_delta, torch.float64, "cl_delta"), (cl_vol, torch.float32, "cl_vol")):
+        _need(t, dt, None, n)
+        if t.numel() < chunk * cap:
+            raise ValueError(f"{n}: {t.numel()} entries, need chunk * cap = {chunk * cap}")
+    _need(cl_mom, torch.float32, None, "cl_mom")
+    if cl_mom.numel() < 2 * chunk * cap:
+        raise ValueError("cl_mom too small")
+    for t, n in ((cl_count, "cl_count"), (cl_lo, "cl_lo"), (cl_hi, "cl_hi")):
+        _need(t, torch.int64, None, n)
+        if t.numel() < chunk:
+            raise ValueError(f"{n} too small")
+    _need(sp_death, torch.int8, None, "sp_death")
+    if sp_death.numel() < chunk:
+        raise ValueError("sp_death too small")
+    n_chunks = -(-P // chunk) if P else 0
+    _need(outs, torch.float64, (n_chunks, 4), "outs")
+    _need(counts7, torch.int64, (7,), "counts7")
+    _need(status, torch.int32, (4,), "status")
+    _need(work, torch.uint8, None, "work")
+    _need(drop_out, torch.float64, None, "drop_out")
+    if drop_out.numel() < 4 * P:
+        raise ValueError("drop_out too small")
+    if use_roots:
+        _need(roots, torch.float32, e, "roots")
+    if use_erode_cap:
+        _need(erode_cap, torch.float64, e, "erode_cap")
+    check(lib().pg_check_chunks(P, int(chunk), _ptr(sp_face), _ptr(sp_x), _ptr(sp_y), _ptr(sp_sed), _ptr(sp_dir),
+                                float(volume0), _ptr(samp), int(bool(use_route)), _ptr(mask), int(N), int(H),
+                                int(bool(spherical)), ctypes.byref(params), _ptr(roots if use_roots else None),
+                                int(bool(use_roots)), _ptr(cl_cell), _ptr(cl_delta), _ptr(cl_vol), _ptr(cl_mom),
+                                _ptr(cl_count), _ptr(cl_lo), _ptr(cl_hi), _ptr(sp_death), _ptr(height), _ptr(sediment),
+                                _ptr(acc), _ptr(pending), _ptr(disch_track), _ptr(mom_track), float(iter_erode),
+                                float(iter_deposit), float(fan_slope), float(dep_floor), float(offshore_writeoff),
+                                _ptr(erode_cap if use_erode_cap else None), int(bool(use_erode_cap)), _ptr(outs),
+                                _ptr(counts7), _ptr(status), int(max_sweeps), int(launched_sweeps), _ptr(work),
+                                int(work.numel()), _ptr(drop_out), _stream()), "check_chunks")
 
 
 def ordered_status_error(status) -> str | None:
