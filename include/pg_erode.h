@@ -14,12 +14,16 @@
  * exception crosses the ABI.  Work is enqueued on `stream` (a hipStream_t;
  * NULL = the default stream) and is asynchronous unless stated otherwise.
  *
- * Modes.  The check mode (pg_trace_particles + pg_apply_changes) reproduces
- * the reference's algorithm bit for bit: float64 droplet state, the packed
- * float32 record, chunked trace against frozen maps, then a serial apply in
- * droplet order.  The fast mode (pg_fast_*) traces every droplet of a chunk
- * in float32 and accumulates its effects as int64 fixed point with atomics:
- * deterministic, exactly mass-conserving, with order-independent caps.
+ * Modes.  The exact (check) mode reproduces the reference's algorithm bit
+ * for bit: float64 droplet state, the packed float32 record, chunked trace
+ * against frozen maps (pg_trace_particles), then the chunk's change lists
+ * applied with the serial apply's results -- computed in parallel by
+ * pg_apply_changes_ordered (or serially by pg_apply_changes), the whole
+ * chunk loop in one call by pg_check_chunks.  The fast mode (pg_fast_*)
+ * traces every droplet of a chunk in float32 and accumulates its effects as
+ * int64 fixed point with atomics: deterministic, exactly mass-conserving,
+ * with order-independent caps -- a different algorithm, far from the
+ * reference's results at the earth globe's size.
  */
 #ifndef PG_ERODE_H
 #define PG_ERODE_H
@@ -150,7 +154,9 @@ int pg_apply_changes_ordered(const int32_t* cl_cell, const double* cl_delta, con
  * change-list buffers hold chunk * cap entries; outs (DEVICE float64
  * [n_chunks * 4]) receives each chunk's 4-tuple in chunk order, summed in
  * droplet order from drop_out (DEVICE float64 [P * 4] scratch: each
- * droplet's clamped entries, stockpile, deficit and offshore loss). */
+ * droplet's clamped entries, stockpile, deficit and offshore loss).  moves
+ * (DEVICE int64, may be NULL) += the steps traced (entries of volume >= 0);
+ * timing != 0 records HIP events around every trace launch. */
 int pg_check_chunks(int64_t P, int64_t chunk, const int64_t* sp_face, const double* sp_x, const double* sp_y,
                     const double* sp_sed, const double* sp_dir, double volume0, float* samp, int use_route,
                     const uint8_t* mask, int64_t N, int64_t H, int spherical, const pg_trace_params* prm,
@@ -160,7 +166,12 @@ int pg_check_chunks(int64_t P, int64_t chunk, const int64_t* sp_face, const doub
                     double* disch_track, double* mom_track, double iter_erode, double iter_deposit,
                     double fan_slope, double dep_floor, double offshore_writeoff, const double* erode_cap,
                     int use_erode_cap, double* outs, int64_t* counts7, int32_t* status, int max_sweeps,
-                    int launched_sweeps, void* work, int64_t work_bytes, double* drop_out, pg_stream stream);
+                    int launched_sweeps, void* work, int64_t work_bytes, double* drop_out, int64_t* moves,
+                    int timing, pg_stream stream);
+/* Summed duration (ms) and count of the trace launches pg_check_chunks timed
+ * with timing != 0 since the last call (HIP events on the launch stream);
+ * synchronises on the last one. */
+int pg_check_trace_time(double* ms_total, int64_t* launches);
 
 /* Adds a chunk's entry count and death histogram to the DEVICE array
  * counts7 (int64[7]: entries, then the six DEATH_* causes), the statistics
@@ -307,17 +318,7 @@ int pg_refresh_base(const double* height, const double* sediment, const int64_t*
                     double threshold_fx, int64_t N, int64_t H, double* base, void* work, int64_t work_bytes,
                     double* info5, pg_stream stream);
 
-/* ---- calibration (no counterpart in the reference) ---------------------- */
-/* The device's relaxed 64-bit atomicAdd rate: `lanes` lanes each take
- * `steps` steps of a droplet-like walk (pattern 0: to one of the 8 neighbours
- * on an NE-wide grid of n cells; 1: to a uniformly random cell; h >= 2: to one
- * of the first h cells, same-address contention) and add 1 to
- * the cell in each of `planes` int64 planes of n (targets: DEVICE
- * int64[planes * n], n > 4 NE).  For the trace kernel's roofline. */
-int pg_atomic_probe(int64_t* targets, int64_t n, int64_t NE, int32_t planes, int64_t lanes, int32_t steps,
-                    int32_t pattern, uint64_t seed, pg_stream stream);
-
-/* ---- fast mode (the production path of one iteration's particle pass) -- */
+/* ---- fast mode (float32 droplets, order-independent fixed-point apply) -- */
 /* Device pointers to the erosion state (bake/globe/erosion/maps.py:56-115
  * dtypes): the stage's fields in cell units. */
 typedef struct pg_fast_fields {

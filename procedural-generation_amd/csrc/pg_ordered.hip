@@ -52,6 +52,8 @@
 #include <hip/hip_cooperative_groups.h>
 #include <hipcub/hipcub.hpp>
 
+#include <vector>
+
 #include "pg_common.h"
 #include "pg_erode.h"
 #include "pg_internal.h"
@@ -76,8 +78,9 @@ constexpr int CTL_DONE = 0, CTL_SWEEPS = 1, CTL_CHANGED = 2;  // control words; 
 struct Layout {
     size_t flags, d, hs, hsn, ec, b, r, rs, rn, w, posw, cd, cs, cn;   // per entry (P * cap)
     size_t death, hsd, bd, re, drop, pend, pendf, fdirty, fc;        // per droplet
-    size_t keys_in, vals_in, keys_out, vals_out, sw, owner;          // per access slot
-    size_t rdirty;                                                   // per range
+    size_t keys_in, vals_in, keys_out, vals_out, sw;                 // per access slot
+    size_t ownerw;                                                   // per entry
+    size_t rmin;                                                     // per range
     size_t ctl, temp;
     size_t temp_bytes, total;
 };
@@ -120,8 +123,8 @@ int layout_of(int64_t P, int64_t cap, int max_sweeps, Layout* L) {
     L->keys_out = take(na * 4);
     L->vals_out = take(na * 4);
     L->sw = take(na * 8);
-    L->owner = take(na * 4);
-    L->rdirty = take((na + RANGE - 1) / RANGE + 1);
+    L->ownerw = take(ne * 4);
+    L->rmin = take(((na + RANGE - 1) / RANGE + 1) * 4);
     L->ctl = take((size_t)(CTL_CHANGED + max_sweeps + 1) * 4);
     size_t tb = 0;
     if (hipcub::DeviceRadixSort::SortPairs(nullptr, tb, (const uint32_t*)nullptr, (uint32_t*)nullptr,
@@ -172,14 +175,15 @@ struct OrdArgs {
     double* fc;        // per droplet: final carries (deficit, surplus, clamped count)
     uint32_t *keys_in, *vals_in, *keys_out, *vals_out;
     double* sw;
-    uint32_t* owner;   // per slot: the range whose wave walks its segment
-    uint8_t* rdirty;   // per range: a write of its segments changed
+    uint32_t* ownerw;  // per entry: the range whose wave walks the segment of its write
+    uint32_t* rmin;    // per range: the first slot of a write that changed (NONE: clean)
     uint32_t* ctl;
     int32_t* status;  // [0] status bits (1 lake park, 2 no convergence), [1] most sweeps, [2] sweeps summed
     uint32_t hole;
     int64_t n_slots, n_ranges;
     int max_sweeps;
     double* out4;
+    unsigned long long* moves;  // steps traced (entries of volume >= 0) added here, if not NULL
 };
 
 __device__ __forceinline__ int lane_id() { return (int)(threadIdx.x & (WAVE - 1)); }
@@ -214,13 +218,14 @@ __device__ __forceinline__ int last_upto(ull m, int lane) {
 __global__ void __launch_bounds__(BLOCK) ord_build_kernel(OrdArgs a) {
     if (blockIdx.x == 0)
         for (int i = threadIdx.x; i < CTL_CHANGED + a.max_sweeps + 1; i += BLOCK) a.ctl[i] = 0;
-    for (int64_t r = (int64_t)blockIdx.x * BLOCK + threadIdx.x; r < a.n_ranges; r += (int64_t)gridDim.x * BLOCK) a.rdirty[r] = 0;
+    for (int64_t r = (int64_t)blockIdx.x * BLOCK + threadIdx.x; r < a.n_ranges; r += (int64_t)gridDim.x * BLOCK) a.rmin[r] = NONE;
     const int lane = lane_id();
     const int64_t p = (int64_t)blockIdx.x * WAVES_PER_BLOCK + (threadIdx.x / WAVE);
     if (p >= a.P) return;
     const int64_t cap = a.cap, base = p * cap, cnt = a.cl_count[p], sbase = p * (3 * cap + 1);
     int prev = -1, dth = -1;  // uniform
     bool bad = false;
+    int n_moves = 0;  // uniform
     for (int64_t k0 = 0; k0 < cap; k0 += WAVE) {
         const int64_t k = k0 + lane;
         const bool in = k < cap, live = k < cnt;
@@ -229,6 +234,7 @@ __global__ void __launch_bounds__(BLOCK) ord_build_kernel(OrdArgs a) {
         const double d = live ? a.cl_delta[base + k] : 0.0;
         const int cnext = (live && k + 1 < cnt) ? a.cl_cell[base + k + 1] : -1;
         const bool lake = live && v == -3.0f;
+        n_moves += __popcll(__ballot(live && v >= 0.0f));
         bad |= lake;
         const bool act = live && !lake && a.mask[c] == MASK_ACTIVE;
         const bool ero = act && d < 0.0;
@@ -295,6 +301,7 @@ __global__ void __launch_bounds__(BLOCK) ord_build_kernel(OrdArgs a) {
         }
     }
     if (__ballot(bad) && lane == 0) atomicOr(a.status, 1);
+    if (a.moves && lane == 0 && n_moves) atomicAdd(a.moves, (unsigned long long)n_moves);
 }
 
 // 2b. sorted slot of every entry's write (the droplet half writes its amounts there)
@@ -317,6 +324,15 @@ __device__ void droplet_wave(const OrdArgs& a, int64_t p, int sweep) {
     const uint32_t fd = a.fdirty[p];  // uniform
     if (fd == NONE) return;           // no read value changed since the last walk
     if (lane == 0) a.fdirty[p] = NONE;
+    // the stockpile's inputs, in flight with the entries' (used by lane 0 at the end)
+    const int32_t dc = a.death[p];
+    double hsd = 0.0, red = 0.0;
+    float bdd = 0.0f;
+    if (lane == 0 && dc >= 0) {
+        hsd = a.hsd[p];
+        red = a.re[p];
+        bdd = a.bd[p];
+    }
     // the carries entering entry fd: stored by the last walk (fd == 0: none yet)
     double D = 0.0, S = 0.0;  // deficit, surplus: uniform
     int n_clamp = 0;          // uniform
@@ -338,9 +354,13 @@ __device__ void droplet_wave(const OrdArgs& a, int64_t p, int sweep) {
         const bool mine = k >= k_first && k < cnt;
         const uint8_t fl = mine ? a.flags[e] : 0;
         const bool act = fl & EF_ACTIVE, ero = fl & EF_ERODE;
-        double d = 0.0, w = 0.0, x = 0.0, U = 0.0;
+        double d = 0.0, w = 0.0, x = 0.0, U = 0.0, wold = 0.0;
+        uint32_t slot = 0, owner = 0;
         bool clamped = false;
         if (act) {
+            wold = a.w[e];
+            slot = a.posw[e];
+            if (sweep > 0) owner = a.ownerw[e];
             d = a.d[e];
             const double hs = a.hs[e], ac = a.rs[e], an = a.rn[e];
             const float b_c = a.b[e];
@@ -428,11 +448,10 @@ __device__ void droplet_wave(const OrdArgs& a, int64_t p, int sweep) {
             a.cd[e] = lr < 0 ? D_in : Db;
             a.cs[e] = lr < 0 ? S_in : Sb;
             a.cn[e] = lr < 0 ? N_in : Nb;
-            if (!same_bits(w, a.w[e])) {
+            if (!same_bits(w, wold)) {
                 a.w[e] = w;
-                const uint32_t slot = a.posw[e];
                 a.sw[slot] = w;
-                if (sweep > 0) a.rdirty[a.owner[slot]] = 1;
+                if (sweep > 0) atomicMin(a.rmin + owner, slot);
                 changed = true;
             }
         }
@@ -444,10 +463,9 @@ __device__ void droplet_wave(const OrdArgs& a, int64_t p, int sweep) {
         a.fc[3 * p + 2] = (double)n_clamp;
         double tp = 0.0, lo = 0.0, pend = 0.0;
         uint8_t pf = 0;
-        const int32_t dc = a.death[p];
         if (S > 0.0 && dc >= 0) {
             pf = 1;
-            if (a.hsd[p] + a.re[p] < (double)a.bd[p]) {
+            if (hsd + red < (double)bdd) {
                 double keep = S * (1.0 - a.writeoff);
                 if (keep < 1e-4) keep = 0.0;
                 pend = keep;
@@ -504,17 +522,25 @@ __device__ __forceinline__ bool first_start(const OrdArgs& a, int64_t r, int64_t
 template <bool FINISH>
 __device__ void range_wave(const OrdArgs& a, int64_t r, int sweep) {
     const int lane = lane_id();
-    if (!FINISH && sweep > 0) {
-        if (!a.rdirty[r]) return;  // no write of its segments changed: every read keeps its value
-        if (lane == 0) a.rdirty[r] = 0;
-    }
     int64_t i0;
-    if (!first_start(a, r, &i0)) return;
+    double v = 0.0;  // running value of the current segment (uniform)
+    uint32_t cur = 0xffffffffu;
+    if (!FINISH && sweep > 0) {
+        // only from the first write of this range's segments that changed: every
+        // access before it keeps its value, and the value entering that write is
+        // the read of its own entry just before it (same cell, no write between)
+        const uint32_t first = a.rmin[r];
+        if (first == NONE) return;
+        if (lane == 0) a.rmin[r] = NONE;
+        i0 = first;
+        cur = a.keys_out[first];
+        v = a.rs[a.vals_out[first] & IDX_MASK];
+    } else if (!first_start(a, r, &i0)) {
+        return;
+    }
     const int64_t hi = (r + 1) * RANGE;  // segments starting at or past hi belong to the next range
-    double v = 0.0;                      // running value of the current segment (uniform)
     double dv = 0.0, m0 = 0.0, m1 = 0.0, pv = 0.0;  // finish: tracks and stockpile running values
     bool wrote = false, tracked = false, parked = false;
-    uint32_t cur = 0xffffffffu;
     auto write_back = [&]() {
         if (!FINISH || cur == 0xffffffffu || lane != 0) return;
         if (wrote) {
@@ -540,7 +566,8 @@ __device__ void range_wave(const OrdArgs& a, int64_t r, int sweep) {
             val = a.vals_out[i];
             if (i > 0) prevkey = a.keys_out[i - 1];
         }
-        const bool start = key != a.hole && (i == 0 || prevkey != key);
+        // (resuming: the first slot continues its segment)
+        const bool start = key != a.hole && (i == 0 || prevkey != key) && !(base == i0 && lane == 0 && cur != 0xffffffffu);
         // stop at the first start at or past hi, or at a hole (the tail of the sort)
         const ull stop_m = __ballot(key == a.hole || (start && i >= hi));
         const int stop = stop_m ? __ffsll((long long)stop_m) - 1 : WAVE;
@@ -616,7 +643,7 @@ __device__ void range_wave(const OrdArgs& a, int64_t r, int sweep) {
             const double from_w = __shfl(vafter, lw < 0 ? lane : lw);
             const double from_s = __shfl(a0, ls < 0 ? lane : ls);
             if (valid) {
-                if (sweep == 0) a.owner[i] = (uint32_t)r;
+                if (sweep == 0 && kind == K_WRITE) a.ownerw[idx] = (uint32_t)r;
                 if (kind != K_WRITE) {
                     const double rv = (lw >= 0 && lw > ls) ? from_w : (ls >= 0 ? from_s : v_in);
                     double* dst = kind == K_SELF ? a.rs + idx : (kind == K_NB ? a.rn + idx : a.re + idx);
@@ -780,7 +807,7 @@ int ordered_enqueue(const int32_t* cl_cell, const double* cl_delta, const float*
                     const uint8_t* mask, int64_t n_cells, double iter_erode, double iter_deposit, double fan_slope,
                     int use_route, double dep_floor, double offshore_writeoff, const double* erode_cap,
                     int use_erode_cap, double* out4, int32_t* status, int max_sweeps, int launched_sweeps, void* work,
-                    int64_t work_bytes, hipStream_t stream, double* drop_out) {
+                    int64_t work_bytes, hipStream_t stream, double* drop_out, int64_t* moves) {
     PG_REQUIRE(P >= 0 && cap > 0 && n_cells > 0, "pg_apply_changes_ordered: bad sizes P=%lld cap=%lld n_cells=%lld",
                (long long)P, (long long)cap, (long long)n_cells);
     PG_REQUIRE(max_sweeps >= 1 && launched_sweeps >= 0, "pg_apply_changes_ordered: max_sweeps must be >= 1, launched_sweeps >= 0");
@@ -845,8 +872,8 @@ int ordered_enqueue(const int32_t* cl_cell, const double* cl_delta, const float*
     a.cn = at<int32_t>(work, L.cn);
     a.fdirty = at<uint32_t>(work, L.fdirty);
     a.fc = at<double>(work, L.fc);
-    a.owner = at<uint32_t>(work, L.owner);
-    a.rdirty = at<uint8_t>(work, L.rdirty);
+    a.ownerw = at<uint32_t>(work, L.ownerw);
+    a.rmin = at<uint32_t>(work, L.rmin);
     a.death = at<int32_t>(work, L.death);
     a.hsd = at<double>(work, L.hsd);
     a.bd = at<float>(work, L.bd);
@@ -866,6 +893,7 @@ int ordered_enqueue(const int32_t* cl_cell, const double* cl_delta, const float*
     a.n_ranges = (a.n_slots + RANGE - 1) / RANGE;
     a.max_sweeps = max_sweeps;
     a.out4 = out4;
+    a.moves = (unsigned long long*)moves;
     const int dblocks = (int)((P + WAVES_PER_BLOCK - 1) / WAVES_PER_BLOCK);
     const int rblocks = (int)((a.n_ranges + WAVES_PER_BLOCK - 1) / WAVES_PER_BLOCK);
     hipLaunchKernelGGL(ord_build_kernel, dim3(dblocks), dim3(BLOCK), 0, s, a);
@@ -939,6 +967,32 @@ __global__ void __launch_bounds__(BLOCK) ord_chunk_sums_kernel(const double* dro
         o[3] = lo;
     }
 }
+// Trace-launch timing of pg_check_chunks: event pairs reused across calls,
+// summed and released by pg_check_trace_time.
+struct CheckTimer {
+    std::vector<hipEvent_t> start, stop;
+    size_t recorded = 0;
+};
+CheckTimer g_check_timer;
+
+int timer_mark(hipStream_t s, bool is_start) {
+    CheckTimer& t = g_check_timer;
+    const size_t i = t.recorded;
+    if (is_start && i >= t.start.size()) {
+        hipEvent_t a, b;
+        PG_HIP(hipEventCreate(&a));
+        PG_HIP(hipEventCreate(&b));
+        t.start.push_back(a);
+        t.stop.push_back(b);
+    }
+    if (is_start) {
+        PG_HIP(hipEventRecord(t.start[i], s));
+    } else {
+        PG_HIP(hipEventRecord(t.stop[i], s));
+        t.recorded = i + 1;
+    }
+    return 0;
+}
 }  // namespace
 }  // namespace pg
 
@@ -956,7 +1010,7 @@ extern "C" int pg_apply_changes_ordered(const int32_t* cl_cell, const double* cl
     return ordered_enqueue(cl_cell, cl_delta, cl_vol, cl_mom, cl_count, P, cap, height, sediment, acc, pending, samp,
                            disch_track, mom_track, mask, n_cells, iter_erode, iter_deposit, fan_slope, use_route,
                            dep_floor, offshore_writeoff, erode_cap, use_erode_cap, out4, status, max_sweeps,
-                           launched_sweeps, work, work_bytes, (hipStream_t)stream, nullptr);
+                           launched_sweeps, work, work_bytes, (hipStream_t)stream, nullptr, nullptr);
 }
 
 extern "C" int pg_check_chunks(int64_t P, int64_t chunk, const int64_t* sp_face, const double* sp_x, const double* sp_y,
@@ -969,7 +1023,7 @@ extern "C" int pg_check_chunks(int64_t P, int64_t chunk, const int64_t* sp_face,
                                double fan_slope, double dep_floor, double offshore_writeoff, const double* erode_cap,
                                int use_erode_cap, double* outs, int64_t* counts7, int32_t* status, int max_sweeps,
                                int launched_sweeps, void* work, int64_t work_bytes, double* drop_out,
-                               pg_stream stream) {
+                               int64_t* moves, int timing, pg_stream stream) {
     clear_error();
     PG_REQUIRE(P >= 0 && chunk >= 1 && N >= 2 && H >= 1, "pg_check_chunks: bad sizes P=%lld chunk=%lld", (long long)P,
                (long long)chunk);
@@ -978,14 +1032,16 @@ extern "C" int pg_check_chunks(int64_t P, int64_t chunk, const int64_t* sp_face,
     const int64_t cap = pg_change_list_cap(prm->max_steps, prm->lateral_rate > 0.0);
     for (int64_t c0 = 0, ci = 0; c0 < P; c0 += chunk, ci++) {
         const int64_t m = P - c0 < chunk ? P - c0 : chunk;
+        if (timing && timer_mark((hipStream_t)stream, true) != 0) return -1;
         if (pg_trace_particles(m, sp_face + c0, sp_x + c0, sp_y + c0, sp_sed + c0, sp_dir + c0, volume0, samp, use_route,
                                mask, N, H, spherical, prm, roots, use_roots, cl_cell, cl_delta, cl_vol, cl_mom, cl_count,
                                cl_lo, cl_hi, sp_death, stream) != 0)
             return -1;
+        if (timing && timer_mark((hipStream_t)stream, false) != 0) return -1;
         if (ordered_enqueue(cl_cell, cl_delta, cl_vol, cl_mom, cl_count, m, cap, height, sediment, acc, pending, samp,
                             disch_track, mom_track, mask, F * NE * NE, iter_erode, iter_deposit, fan_slope, use_route,
                             dep_floor, offshore_writeoff, erode_cap, use_erode_cap, nullptr, status, max_sweeps,
-                            launched_sweeps, work, work_bytes, (hipStream_t)stream, drop_out + 4 * c0) != 0)
+                            launched_sweeps, work, work_bytes, (hipStream_t)stream, drop_out + 4 * c0, moves) != 0)
             return -1;
         if (pg_chunk_stats(cl_count, sp_death, m, counts7, stream) != 0) return -1;
         (void)ci;
@@ -996,5 +1052,22 @@ extern "C" int pg_check_chunks(int64_t P, int64_t chunk, const int64_t* sp_face,
                            chunk, outs);
         PG_LAUNCHED("ord_chunk_sums_kernel");
     }
+    return 0;
+}
+
+extern "C" int pg_check_trace_time(double* ms_total, int64_t* launches) {
+    clear_error();
+    PG_REQUIRE(ms_total && launches, "pg_check_trace_time: NULL argument");
+    CheckTimer& t = g_check_timer;
+    double ms = 0.0;
+    if (t.recorded) PG_HIP(hipEventSynchronize(t.stop[t.recorded - 1]));
+    for (size_t i = 0; i < t.recorded; i++) {
+        float x = 0.0f;
+        PG_HIP(hipEventElapsedTime(&x, t.start[i], t.stop[i]));
+        ms += x;
+    }
+    *ms_total = ms;
+    *launches = (int64_t)t.recorded;
+    t.recorded = 0;
     return 0;
 }

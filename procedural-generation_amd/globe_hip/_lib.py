@@ -85,7 +85,8 @@ SIGNATURES = [
                                        i32, dbl, dbl, vp, i32, vp, vp, i32, i32, vp, i64, vp]),
     ("pg_check_chunks", i32, [i64, i64, vp, vp, vp, vp, vp, dbl, vp, i32, vp, i64, i64, i32, ctypes.POINTER(TraceParams),
                               vp, i32, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, dbl, dbl, dbl, dbl, dbl,
-                              vp, i32, vp, vp, vp, i32, i32, vp, i64, vp, vp]),
+                              vp, i32, vp, vp, vp, i32, i32, vp, i64, vp, vp, i32, vp]),
+    ("pg_check_trace_time", i32, [pd, ctypes.POINTER(ctypes.c_int64)]),
     ("pg_chunk_stats", i32, [vp, vp, i64, vp, vp]),
     ("pg_fold_changes", i32, [i64, vp, vp, vp, vp, vp]),
     ("pg_ema_update", i32, [i64, vp, vp, vp, vp, vp, dbl, vp]),
@@ -139,7 +140,6 @@ SIGNATURES = [
     ("pg_flood_init", i32, [i64, i64, i64, i64, vp, vp, vp, vp, vp, vp, vp, vp]),
     ("pg_flood_sweeps", i32, [i64, i64, vp, vp, vp, dbl, vp, vp, vp, i64, ctypes.c_int32, vp, vp]),
     ("pg_flood_finish", i32, [i64, vp, vp, vp, vp]),
-    ("pg_atomic_probe", i32, [vp, i64, i64, ctypes.c_int32, i64, ctypes.c_int32, ctypes.c_int32, ctypes.c_uint64, vp]),
 ]
 
 _lib = None

@@ -126,6 +126,10 @@ class ErosionState:
     #: parallel, bit for bit) or "serial" (one lane walks the lists)
     check_apply: str = "ordered"
     check_max_sweeps: int = 1024
+    #: sweeps enqueued as plain launches per chunk (the rest run in one cooperative launch)
+    check_launched_sweeps: int = 40
+    #: record HIP events around each check-mode trace launch (particle.check_trace_time reads them)
+    check_time_trace: bool = False
     route: torch.Tensor | None = None
     lake_flag: torch.Tensor | None = None
     disch_track: torch.Tensor = field(default=None, repr=False)
@@ -724,6 +728,7 @@ def run_iteration(state: ErosionState, params, iteration_key, n_particles=None, 
         ord_work, ord_status = state._ordered["work"], state._ordered["status"]
         ord_status.zero_()
     t1 = time.time()
+    moves_dev = torch.zeros(1, dtype=torch.int64, device=dev) if ordered and diag is None else None
     if ordered and diag is None:
         # the whole chunk loop in one library call (no host round trip per chunk)
         pk.check_chunks(sp_face, sp_x, sp_y, sp_sed, sp_dir, volume0, chunk, state.samp, state.route is not None,
@@ -731,7 +736,8 @@ def run_iteration(state: ErosionState, params, iteration_key, n_particles=None, 
                         cl_vol, cl_mom, cl_count, cl_lo, cl_hi, sp_death, state.height, state.sediment, state.acc,
                         state.pending, state.disch_track, state.mom_track, iter_erode, iter_deposit, float(ep.fan_slope),
                         dep_floor, float(ep.offshore_writeoff), ecap, ecap is not None, outs, counts, ord_status,
-                        ord_work, _scratch(state, "ordered_drops", 4 * P, torch.float64), int(state.check_max_sweeps))
+                        ord_work, _scratch(state, "ordered_drops", 4 * P, torch.float64), int(state.check_max_sweeps),
+                        int(state.check_launched_sweeps), moves=moves_dev, timing=state.check_time_trace)
     for ci, c0 in enumerate(range(0, P, chunk) if not (ordered and diag is None) else ()):
         c1 = min(P, c0 + chunk)
         m = c1 - c0
@@ -747,7 +753,7 @@ def run_iteration(state: ErosionState, params, iteration_key, n_particles=None, 
                                      state.acc, state.pending, state.samp, state.disch_track, state.mom_track, state.mask,
                                      iter_erode, iter_deposit, float(ep.fan_slope), state.route is not None, dep_floor,
                                      float(ep.offshore_writeoff), ecap, ecap is not None, outs[ci], ord_status, ord_work,
-                                     int(state.check_max_sweeps))
+                                     int(state.check_max_sweeps), int(state.check_launched_sweeps))
         else:
             pk.apply_changes(cl_cell, cl_delta, cl_vol, cl_mom, cl_count[:m], cap, state.height, state.sediment, state.acc,
                              state.pending, state.samp, state.disch_track, state.mom_track, state.mask, iter_erode,
@@ -768,6 +774,8 @@ def run_iteration(state: ErosionState, params, iteration_key, n_particles=None, 
             raise RuntimeError(f"run_iteration: {err}")
         stats["sweeps_max"] = int(ost[1])
         stats["sweeps_total"] = int(ost[2])
+        if moves_dev is not None:
+            stats["moves"] = int(moves_dev.item())
     # per-chunk partial results summed in chunk order, as the reference's loop does
     n_clamp = 0
     to_pending = lost = lost_offshore = 0.0

@@ -193,12 +193,14 @@ def check_chunks(sp_face, sp_x, sp_y, sp_sed, sp_dir, volume0, chunk, samp, use_
                  params: TraceParams, roots, use_roots, cl_cell, cl_delta, cl_vol, cl_mom, cl_count, cl_lo, cl_hi,
                  sp_death, height, sediment, acc, pending, disch_track, mom_track, iter_erode, iter_deposit, fan_slope,
                  dep_floor, offshore_writeoff, erode_cap, use_erode_cap, outs, counts7, status, work, drop_out,
-                 max_sweeps, launched_sweeps=40):
+                 max_sweeps, launched_sweeps=40, moves=None, timing=False):
     """The check mode's chunk loop (maps.py:953-1029) in one call: per chunk
     of ``chunk`` droplets trace_particles, the parallel apply and the chunk
     statistics, enqueued without a host round trip.  ``outs`` (float64
     [n_chunks, 4]) receives each chunk's 4-tuple, summed from ``drop_out``
-    (float64 [P, 4] scratch)."""
+    (float64 [P, 4] scratch).  ``moves`` (int64 [1], optional) accumulates the
+    steps traced; ``timing`` records HIP events around each trace launch
+    (read them with ``check_trace_time``)."""
     P = sp_face.shape[0]
     F, NE = samp.shape[0], samp.shape[1]
     e = (F, NE, NE)
@@ -233,6 +235,8 @@ def check_chunks(sp_face, sp_x, sp_y, sp_sed, sp_dir, volume0, chunk, samp, use_
     _need(status, torch.int32, (4,), "status")
     _need(work, torch.uint8, None, "work")
     _need(drop_out, torch.float64, None, "drop_out")
+    if moves is not None:
+        _need(moves, torch.int64, (1,), "moves")
     if drop_out.numel() < 4 * P:
         raise ValueError("drop_out too small")
     if use_roots:
@@ -248,7 +252,16 @@ def check_chunks(sp_face, sp_x, sp_y, sp_sed, sp_dir, volume0, chunk, samp, use_
                                 float(iter_deposit), float(fan_slope), float(dep_floor), float(offshore_writeoff),
                                 _ptr(erode_cap if use_erode_cap else None), int(bool(use_erode_cap)), _ptr(outs),
                                 _ptr(counts7), _ptr(status), int(max_sweeps), int(launched_sweeps), _ptr(work),
-                                int(work.numel()), _ptr(drop_out), _stream()), "check_chunks")
+                                int(work.numel()), _ptr(drop_out), _ptr(moves), int(bool(timing)), _stream()),
+          "check_chunks")
+
+
+def check_trace_time():
+    """(ms, launches) of the trace launches timed by ``check_chunks`` since the last call."""
+    ms = ctypes.c_double(0.0)
+    n = ctypes.c_int64(0)
+    check(lib().pg_check_trace_time(ctypes.byref(ms), ctypes.byref(n)), "check_trace_time")
+    return ms.value, n.value
 
 
 def ordered_status_error(status) -> str | None:
