@@ -147,6 +147,27 @@ int pg_apply_changes_ordered(const int32_t* cl_cell, const double* cl_delta, con
                              int32_t* status, int max_sweeps, int launched_sweeps, void* work, int64_t work_bytes,
                              pg_stream stream);
 
+/* No counterpart in the reference: pg_apply_changes_ordered for callers that
+ * cannot allocate device memory (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc).  Same arguments and results, no
+ * workspace: the scratch is a file-scope device array of
+ * pg_ordered_scratch_bytes() bytes (96 MiB: chunks of up to 2048 droplets
+ * with lifetimes up to 128 and max_sweeps up to 256 fit), looked up once per
+ * device and held by every GPU the library runs on until the process ends.
+ * Sizes beyond it return an error (nothing is launched).  Calls that overlap
+ * on one GPU share the array and race.  No cooperative launch: every sweep is
+ * a plain launch, max_sweeps (<= 4096) of them at most.  Calls no hipMalloc
+ * or allocating routine. */
+int64_t pg_ordered_scratch_bytes(void);
+int pg_apply_changes_ordered_noalloc(const int32_t* cl_cell, const double* cl_delta, const float* cl_vol,
+                                     const float* cl_mom, const int64_t* cl_count, int64_t P, int64_t cap,
+                                     const double* height, const double* sediment, double* acc, double* pending,
+                                     float* samp, double* disch_track, double* mom_track, const uint8_t* mask,
+                                     int64_t n_cells, double iter_erode, double iter_deposit, double fan_slope,
+                                     int use_route, double dep_floor, double offshore_writeoff,
+                                     const double* erode_cap, int use_erode_cap, double* out4, int32_t* status,
+                                     int max_sweeps, pg_stream stream);
+
 /* The check mode's chunk loop (bake/globe/erosion/maps.py:953-1029): for each
  * chunk of `chunk` droplets of the P spawned, pg_trace_particles on the frozen
  * record, pg_apply_changes_ordered, pg_chunk_stats -- the same results as

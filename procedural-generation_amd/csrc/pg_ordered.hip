@@ -52,6 +52,7 @@
 #include <hip/hip_cooperative_groups.h>
 #include <hipcub/hipcub.hpp>
 
+#include <mutex>
 #include <vector>
 
 #include "pg_common.h"
@@ -1011,6 +1012,73 @@ extern "C" int pg_apply_changes_ordered(const int32_t* cl_cell, const double* cl
                            disch_track, mom_track, mask, n_cells, iter_erode, iter_deposit, fan_slope, use_route,
                            dep_floor, offshore_writeoff, erode_cap, use_erode_cap, out4, status, max_sweeps,
                            launched_sweeps, work, work_bytes, (hipStream_t)stream, nullptr, nullptr);
+}
+
+// ---- the same apply with no allocation and no workspace argument ----------
+// For callers that cannot allocate device memory (code under graph capture,
+// tight serving loops): the workspace is a file-scope device array, held by
+// every GPU the library runs on until the process ends.  Calls that overlap
+// on one GPU share it and race.  Nothing in it needs clearing between calls:
+// the build kernel initialises everything the apply reads.  Every sweep is a
+// plain launch (no cooperative launch), max_sweeps of them at most.
+namespace pg {
+namespace {
+constexpr size_t ORDERED_SCRATCH_BYTES = size_t(96) << 20;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[64];
+}  // namespace
+}  // namespace pg
+
+static __device__ __align__(256) unsigned char scratch[pg::ORDERED_SCRATCH_BYTES];
+
+namespace pg {
+namespace {
+// The array's device address on the current device, looked up once per device.
+int scratch_address(void** out) {
+    int dev = 0;
+    PG_HIP(hipGetDevice(&dev));
+    PG_REQUIRE(dev >= 0 && dev < 64, "scratch: device %d out of range", dev);
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* ptr = nullptr;
+        PG_HIP(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+        PG_REQUIRE(ptr != nullptr, "scratch: hipGetSymbolAddress returned NULL");
+        g_scratch_ptr[dev] = ptr;
+    }
+    *out = g_scratch_ptr[dev];
+    return 0;
+}
+}  // namespace
+}  // namespace pg
+
+extern "C" int64_t pg_ordered_scratch_bytes(void) { return (int64_t)ORDERED_SCRATCH_BYTES; }
+
+extern "C" int pg_apply_changes_ordered_noalloc(const int32_t* cl_cell, const double* cl_delta, const float* cl_vol,
+                                                const float* cl_mom, const int64_t* cl_count, int64_t P, int64_t cap,
+                                                const double* height, const double* sediment, double* acc,
+                                                double* pending, float* samp, double* disch_track, double* mom_track,
+                                                const uint8_t* mask, int64_t n_cells, double iter_erode,
+                                                double iter_deposit, double fan_slope, int use_route, double dep_floor,
+                                                double offshore_writeoff, const double* erode_cap, int use_erode_cap,
+                                                double* out4, int32_t* status, int max_sweeps, pg_stream stream) {
+    clear_error();
+    PG_REQUIRE(out4 != nullptr, "pg_apply_changes_ordered_noalloc: out4 is NULL");
+    PG_REQUIRE(P >= 0 && cap > 0 && max_sweeps >= 1 && max_sweeps <= 4096,
+               "pg_apply_changes_ordered_noalloc: bad sizes P=%lld cap=%lld max_sweeps=%d", (long long)P, (long long)cap,
+               max_sweeps);
+    Layout L;
+    PG_REQUIRE(layout_of(P, cap, max_sweeps, &L) == 0, "pg_apply_changes_ordered_noalloc: hipcub temp-size query failed");
+    PG_REQUIRE(L.total <= ORDERED_SCRATCH_BYTES,
+               "pg_apply_changes_ordered_noalloc: P=%lld cap=%lld max_sweeps=%d need %llu bytes of scratch, the maximum "
+               "is %llu (chunks of up to 2048 droplets with lifetimes up to 128 fit)",
+               (long long)P, (long long)cap, max_sweeps, (unsigned long long)L.total,
+               (unsigned long long)ORDERED_SCRATCH_BYTES);
+    void* ptr = nullptr;
+    if (scratch_address(&ptr) != 0) return -1;
+    return ordered_enqueue(cl_cell, cl_delta, cl_vol, cl_mom, cl_count, P, cap, height, sediment, acc, pending, samp,
+                           disch_track, mom_track, mask, n_cells, iter_erode, iter_deposit, fan_slope, use_route,
+                           dep_floor, offshore_writeoff, erode_cap, use_erode_cap, out4, status, max_sweeps, max_sweeps,
+                           ptr, (int64_t)ORDERED_SCRATCH_BYTES, (hipStream_t)stream, nullptr, nullptr);
 }
 
 extern "C" int pg_check_chunks(int64_t P, int64_t chunk, const int64_t* sp_face, const double* sp_x, const double* sp_y,
