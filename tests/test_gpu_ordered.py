@@ -7,9 +7,13 @@ record and the chunk's 4-tuple.  Checked two ways on seeded inputs:
 * against the one-lane serial kernel on the device, over full steps
   (windows of each synthetic kind, plain and earth globes): every state array
   and every statistic identical;
-* against the oracle (the C restatement of the serial apply) on the N_c=256
-  earth globe's first iteration, with the oracle on the correctly rounded
+* against the oracle (the C restatement of the serial apply) on the earth
+  globe's first iteration at N_c=256 and at the benchmark's N_c=1024, with the oracle on the correctly rounded
   tan / atan the device uses: change lists, statistics and state identical.
+
+* on hand-built change lists whose outcome was worked out by hand from the
+  reference's rules (tests/apply_cases.py): both device applies give exactly
+  those numbers.
 
 Error path: a sweep limit too small to converge is reported as an error and
 no height changes (the kernels still exit normally).
@@ -20,8 +24,10 @@ import numpy as np
 import pytest
 import torch
 
+import apply_cases
 from globe_hip import stubs
 from globe_hip.erosion import maps
+from globe_hip.erosion import particle as pk
 from windows import window_inputs, window_params
 
 pytestmark = pytest.mark.gpu
@@ -79,8 +85,10 @@ def test_globe_steps_ordered_equals_serial(cuda, N, kind, iters):
               f"sweeps max {sb['sweeps_max']} total {sb['sweeps_total']}", flush=True)
 
 
-def test_earth_256_first_iteration_matches_oracle(oracle, hiplib, cuda):
-    N = 256
+@pytest.mark.parametrize("N", [256, 1024])
+def test_earth_first_iteration_matches_oracle(oracle, hiplib, cuda, N):
+    """The earth workload's first iteration (N_c=1024: the benchmark's own
+    size and settings, 6.3 M droplets in 3072 chunks, ~5e8 entries)."""
     grid, f = stubs.earth_globe(N, 2)
     p = stubs.workload_params(N, 2, max_steps=128)
     args = (grid, f["bedrock_m"], f["hardness"], f["precip"], f["evap"], f["uplift_m"], p.erosion)
@@ -116,3 +124,27 @@ def test_sweep_limit_is_an_error_and_applies_nothing(cuda):
     # from the first chunk that needs a second sweep on, no chunk is applied;
     # every earlier one converged on its first sweep, i.e. changed no height
     assert torch.equal(st.height, h0)
+
+
+@pytest.mark.parametrize("apply", ["serial", "ordered"])
+@pytest.mark.parametrize("case", apply_cases.CASES, ids=lambda c: c.name)
+def test_hand_built_lists_on_device(cuda, case, apply):
+    print(f"hand-built list {case.name}, {apply} apply", flush=True)
+    (cl_cell, cl_delta, cl_vol, cl_mom, cl_count), f, cap = apply_cases.inputs(case)
+    d = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(cuda)
+    t = {k: d(v) for k, v in f.items()}
+    lists = [d(cl_cell), d(cl_delta), d(cl_vol), d(cl_mom), d(cl_count)]
+    prm = case.params()
+    out4 = torch.zeros(4, dtype=torch.float64, device=cuda)
+    common = (*lists, cap, t["height"], t["sediment"], t["acc"], t["pending"], t["samp"], t["disch_track"],
+              t["mom_track"], t["mask"], prm["iter_erode"], prm["iter_deposit"], prm["fan_slope"], prm["use_route"],
+              prm["dep_floor"], prm["offshore_writeoff"], t["erode_cap"], prm["use_erode_cap"])
+    if apply == "serial":
+        pk.apply_changes(*common, None, None, None, out4)
+    else:
+        status = torch.zeros(4, dtype=torch.int32, device=cuda)
+        work = torch.empty(pk.ordered_workspace_bytes(len(case.droplets), cap, 64), dtype=torch.uint8, device=cuda)
+        pk.apply_changes_ordered(*common, out4, status, work, 64)
+        assert status[0].item() == 0, status
+    torch.cuda.synchronize()
+    apply_cases.check(case, {k: v.cpu().numpy() for k, v in t.items()}, out4.cpu().numpy())
