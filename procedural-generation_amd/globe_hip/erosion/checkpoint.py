@@ -1,7 +1,8 @@
 """Erosion checkpoints in the reference's file layout
 (bake/globe/erosion/run.py:99-225): ``erosion_iterNNNN.npz`` with the state
 arrays (height, sediment, discharge, momentum, pending, lake_load; cell
-units, float64) beside ``erosion_iterNNNN.json`` with ``iteration``,
+units, float64; plus route, base, iso_acc and the lakes' lake_flag, lake_id
+and lake_room when the state has them) beside ``erosion_iterNNNN.json`` with ``iteration``,
 ``params_hash``, ``height_unit_m`` and ``time``; the two newest checkpoints of
 a parameter family are kept.  A device state is written from and read back
 into its tensors, so a run of this path and a run of the reference resume
@@ -48,6 +49,10 @@ def save_checkpoint(directory, state, params_hash: str, keep: int = 2) -> Path:
         arrays["base"] = _host(state.base)
     if getattr(state, "iso_acc", None) is not None:  # run.py:112-113: a rebound not yet applied is state
         arrays["iso_acc"] = _host(state.iso_acc)
+    if getattr(state, "lake_flag", None) is not None:  # run.py:125-128: the lakes and the room each has left
+        arrays["lake_flag"] = _host(state.lake_flag)
+        arrays["lake_id"] = _host(state.lake_id)
+        arrays["lake_room"] = _host(state.lake_room)
     with open(tmp, "wb") as fh:
         np.savez(fh, **arrays)
     tmp.replace(p)
@@ -104,6 +109,30 @@ def find_checkpoint(directory, params_hash: str, max_iteration: int | None = Non
     return best
 
 
+def _load_lakes(state, z) -> None:
+    """run.py:221-224: lake_flag, lake_id, lake_room as written, or no lakes.
+    The flag is copied into the state's own buffer when it has one (a
+    captured iteration graph keeps reading that pointer)."""
+    if "lake_flag" not in z.files:
+        state.lake_flag = None
+        return
+    flag = torch.from_numpy(np.ascontiguousarray(z["lake_flag"], np.uint8))
+    if tuple(flag.shape) != tuple(state.height.shape):
+        raise ValueError(f"lake_flag: checkpoint shape {tuple(flag.shape)}, state {tuple(state.height.shape)}")
+    if getattr(state, "lake_flag", None) is None:
+        state.lake_flag = torch.zeros(tuple(state.height.shape), dtype=torch.uint8, device=state.height.device)
+    state.lake_flag.copy_(flag)
+    if "lake_id" in z.files:
+        lid = torch.from_numpy(np.ascontiguousarray(z["lake_id"], np.int32))
+        if getattr(state, "lake_id", None) is None or tuple(state.lake_id.shape) != tuple(lid.shape):
+            state.lake_id = torch.empty(tuple(lid.shape), dtype=torch.int32, device=state.height.device)
+        state.lake_id.copy_(lid)
+        room = torch.from_numpy(np.ascontiguousarray(z["lake_room"], np.float64))
+        if getattr(state, "lake_room", None) is None or tuple(state.lake_room.shape) != tuple(room.shape):
+            state.lake_room = torch.empty(tuple(room.shape), dtype=torch.float64, device=state.height.device)
+        state.lake_room.copy_(room)
+
+
 def load_checkpoint(state, path, meta: dict) -> None:
     """Copy a checkpoint's arrays into state's tensors (absent pending /
     lake_load read as zero) and set state.iteration."""
@@ -118,8 +147,19 @@ def load_checkpoint(state, path, meta: dict) -> None:
             if tuple(src.shape) != tuple(t.shape):
                 raise ValueError(f"{n}: checkpoint shape {src.shape}, state {tuple(t.shape)}")
             t.copy_(torch.from_numpy(np.ascontiguousarray(src)).to(t.dtype))
-        if "route" in z.files and getattr(state, "route", None) is not None:
-            state.route.copy_(torch.from_numpy(np.ascontiguousarray(z["route"])))
+        # run.py:203: the routing surface is the file's, or none (a fresh
+        # state has none: the next step floods again, as the continuous run's
+        # schedule says)
+        if "route" in z.files:
+            r = torch.from_numpy(np.ascontiguousarray(z["route"], np.float64))
+            if tuple(r.shape) != tuple(state.height.shape):
+                raise ValueError(f"route: checkpoint shape {tuple(r.shape)}, state {tuple(state.height.shape)}")
+            if getattr(state, "route", None) is None:
+                state.route = torch.empty_like(state.height)
+            state.route.copy_(r)
+        else:
+            state.route = None
+        _load_lakes(state, z)
         if "iso_acc" in z.files:  # run.py:209
             acc = np.ascontiguousarray(z["iso_acc"], np.float64)
             if tuple(acc.shape) != tuple(state.height.shape):

@@ -21,6 +21,7 @@ import torch
 
 from globe_hip import _lib, stubs
 from globe_hip.erosion import fast, maps
+from windows import window_inputs, window_params
 
 pytestmark = pytest.mark.gpu
 
@@ -268,6 +269,43 @@ def test_checkpoint_resume_is_bitwise(cuda, tmp_path):
         maps.step(a, p, it)
         maps.step(b, p, it)
     for n in ("height", "sediment", "discharge", "momentum", "pending"):
+        assert torch.equal(getattr(a, n), getattr(b, n)), n
+
+
+@pytest.mark.parametrize("mode", ["fast", "check"])
+def test_checkpoint_resume_with_route_and_lakes_is_bitwise(cuda, tmp_path, mode):
+    """With the routing flood (every 5 iterations) and window lakes on, a
+    run checkpointed at iteration 3 -- between two floods -- and resumed into
+    a fresh state (no route, no lakes yet) continues bit for bit: the route,
+    the lake flags and each lake's remaining room come from the file
+    (run.py:125-128, 203, 221-224), not from a flood at the resume."""
+    from globe_hip.erosion import checkpoint as ck
+
+    p = window_params(0, window_lakes=True, flood_every=5)
+    kw = window_inputs(40, "tilt", p, seed=5, deposit_on_exit=True)
+
+    def mk():
+        st = maps.ErosionState.window(**kw, device=cuda, mode=mode)
+        c = st.H + 20
+        st.height[0, c - 3:c + 3, c - 3:c + 3] -= 3.0
+        if mode == "fast":
+            st.fast_chunks = 4
+        return st
+
+    a, b = mk(), mk()
+    for it in range(3):
+        maps.step(a, p, it)
+    assert a.route is not None and a.lake_flag is not None and int(a.lake_flag.sum()) > 0
+    ck.save_checkpoint(tmp_path, a, "lakes")
+    assert b.route is None and b.lake_flag is None
+    ck.load_checkpoint(b, *ck.find_checkpoint(tmp_path, "lakes"))
+    assert b.iteration == 3
+    assert torch.equal(a.route, b.route) and torch.equal(a.lake_flag, b.lake_flag)
+    for it in range(3, 7):
+        print(f"resume {mode}: iteration {it}", flush=True)
+        maps.step(a, p, it)
+        maps.step(b, p, it)
+    for n in ("height", "sediment", "discharge", "momentum", "pending", "route", "lake_flag", "lake_room"):
         assert torch.equal(getattr(a, n), getattr(b, n)), n
 
 

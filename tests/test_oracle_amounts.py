@@ -78,7 +78,7 @@ TRACE = dict(dt=1.2, density=1.0, friction=0.25, deposition_rate=0.1, evap_rate=
              lake_park=0.0, lateral_rate=0.0)
 
 
-def _trace(oracle, h, x, y, hardness=0.0, sed_thick=0.0, **over):
+def _trace(oracle, h, x, y, hardness=0.0, sed_thick=0.0, sed0=0.0, dir0=0.0, **over):
     """One droplet of volume 1 from (x, y) over h (NE x NE cell units,
     H = 2, flat metric, base 0, uniform hardness and sediment cover).
     Returns the change list (cells, deltas, vols) and the death cause."""
@@ -98,7 +98,7 @@ def _trace(oracle, h, x, y, hardness=0.0, sed_thick=0.0, **over):
     cl_vol, cl_mom = np.zeros(cap, np.float32), np.zeros((cap, 2), np.float32)
     cnt, lo, hi, death = np.zeros(1, np.int64), np.zeros(1, np.int64), np.zeros(1, np.int64), np.zeros(1, np.int8)
     a = lambda v, dt: np.array([v], dt)
-    sp = [a(0, np.int64), a(x, np.float64), a(y, np.float64), a(0.0, np.float64), a(0.0, np.float64)]
+    sp = [a(0, np.int64), a(x, np.float64), a(y, np.float64), a(sed0, np.float64), a(dir0, np.float64)]
     P = oracle._p
     oracle.lib().or_trace_particles(1, *[P(v) for v in sp], 1.0, P(st.samp), 0, P(st.mask), N, H, 0, ctypes.byref(prm),
                                     P(np.zeros((1, 1, 1), np.float32)), 0, P(cl_cell), P(cl_delta), P(cl_vol),
@@ -190,6 +190,33 @@ def test_flat_run_out_deposits_dep_floor_per_step(oracle):
             np.testing.assert_allclose(deltas[i], 0.5 * load, rtol=1e-10)
         load -= deltas[i]
     assert n_capped >= 4
+
+
+@pytest.mark.parametrize("floor,slope,sed0,rooms", [
+    (-10.0, 0.5, 5.0, (1.0, 1.0)),      # drops of 0.5 < fan_room: every step may take fan_room
+    (-10.0, 2.0, 20.0, (1.0, 2.0)),     # first step fan_room (no previous cell), then the 2.0 drop
+    (-0.5, 0.0, 5.0, (0.48, 0.48)),     # shallow flat shelf: the waterline b - dep_floor - h binds
+])
+def test_seafloor_walk_fills_to_its_room(oracle, floor, slope, sed0, rooms):
+    """A droplet born under water carrying a load (a re-injected stockpile)
+    walks the seafloor in deposit-only mode: each step lays ocean_rate * load,
+    at most room = min(max(h_prev - h_here, fan_room), b - dep_floor - h_here)
+    (fan_room on the first step).  Evaporation leaves the load sed * vol
+    unchanged, so the deposits obey d_k = min(0.3 u_k, room_k), u_{k+1} =
+    u_k - d_k from u_0 = sed0; every entry is a volume-0 seafloor step."""
+    NE = 24
+    j = np.arange(NE, dtype=np.float64)
+    h = np.broadcast_to(floor - slope * j[None, :], (NE, NE)).copy()
+    cells, deltas, vols, death = _trace(oracle, h, 9.5, 2.5, sed0=sed0, dir0=np.pi / 2)
+    assert death == 2 and cells.size == NE - 2 * 2 - 3
+    assert np.all(vols == 0.0) and np.all(cells % NE == np.arange(cells.size) + 4)
+    u = sed0
+    for k, d in enumerate(deltas):
+        room = rooms[0] if k == 0 else rooms[1]
+        want = min(0.3 * u, room)
+        assert abs(d - want) <= 1e-13 * max(abs(want), 1.0), (k, d, want)
+        u -= d
+    assert deltas[0] == pytest.approx(rooms[0], rel=1e-15)   # the cap binds on the first step of each case
 
 
 # ---------------------------------------------------------------------------
