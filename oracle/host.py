@@ -311,15 +311,11 @@ class OracleState:
         the vector momentum rotated into the destination face first)."""
         if not self.spherical:
             return
-        hm = self.grid.halo
         for arr in (self.height, self.sediment, self.discharge):
-            flat = arr.reshape(-1)
-            flat[hm.dst] = np.einsum("ks,ks...->k...", hm.w.astype(flat.dtype), flat[hm.src])
-        flat = self.momentum.reshape(-1, 2)
-        flat[hm.dst] = np.einsum("ks,ksab,ksb->ka", hm.w, hm.rot, flat[hm.src])
+            exchange(self.grid, arr)
+        exchange_vector(self.grid, self.momentum)
         if self.route is not None:
-            flat = self.route.reshape(-1)
-            flat[hm.dst] = np.einsum("ks,ks...->k...", hm.w_lin, flat[hm.src])
+            exchange(self.grid, self.route, linear=True)
 
 
 # ---------------------------------------------------------------------------
@@ -676,6 +672,15 @@ def exchange(grid, data, linear=False):
     flat = data.reshape(6 * grid.NE * grid.NE)
     w = (hm.w_lin if linear else hm.w).astype(flat.dtype)
     flat[hm.dst] = np.einsum("ks,ks...->k...", w, flat[hm.src])
+    return data
+
+
+def exchange_vector(grid, data):
+    """field.py:104-124 for a (..., 2) tangent vector field: each source
+    rotated into the destination face's frame, then weighted, in place."""
+    hm = grid.halo
+    flat = data.reshape(6 * grid.NE * grid.NE, 2)
+    flat[hm.dst] = np.einsum("ks,ksab,ksb->ka", hm.w, hm.rot, flat[hm.src])
     return data
 
 

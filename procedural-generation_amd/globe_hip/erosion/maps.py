@@ -1023,7 +1023,8 @@ def start_replay(state: ErosionState, params):
     inter = state.interior
     surf = (state.height + state.sediment)[inter]
     info = {"replay_max": float(total[inter].max().item()), "replay_min": float(total[inter].min().item()),
-            "land_fraction": float((surf >= 0).double().mean().item()), "datum_shift": 0.0}
+            # an exact count over M, correctly rounded once (numpy's mean of the boolean mask)
+            "land_fraction": int((surf >= 0).sum().item()) / surf.numel(), "datum_shift": 0.0}
     if hasattr(params, "world"):
         info["datum_shift"] = hold_datum(state, held_land_fraction(state, params))
     return info
