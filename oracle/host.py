@@ -80,6 +80,7 @@ def lib():
         L.or_ema_update.argtypes = [i64, vp, vp, vp, vp, vp, dbl]
         L.or_spawn_cells.argtypes = [vp, i64, vp, i64, vp]
         L.or_set_libm.argtypes = [vp, vp]
+        L.or_set_sincos.argtypes = [vp]
         L.or_change_list_cap.argtypes = [i64, i32]
         L.or_change_list_cap.restype = i64
         d = ctypes.POINTER(ctypes.c_double)
@@ -131,11 +132,19 @@ def transfer_velocity(face, u, v, a, b):
     return (f.value,) + tuple(x.value for x in o)
 
 
-def set_libm(tan_fn=None, atan_fn=None):
-    """Use other tan/atan (C function pointers, e.g. ctypes function
-    objects) in the geometry; None restores glibc's."""
+def set_libm(tan_fn=None, atan_fn=None, sincos_fn=None):
+    """Use other tan / atan (the geometry) and sincos(x, &sin, &cos) (a
+    loaded droplet's initial direction) -- C function pointers, e.g. ctypes
+    function objects; None restores glibc's."""
     as_ptr = lambda fn: ctypes.cast(fn, ctypes.c_void_p) if fn is not None else None
     lib().or_set_libm(as_ptr(tan_fn), as_ptr(atan_fn))
+    lib().or_set_sincos(as_ptr(sincos_fn))
+
+
+def device_libm(hiplib):
+    """The correctly rounded tan / atan / sincos of the library under test
+    (its host-side pg_geom_* functions) as set_libm's arguments."""
+    return hiplib.pg_geom_tan, hiplib.pg_geom_atan, hiplib.pg_geom_sincos
 
 
 def change_list_cap(max_steps: int, lateral: bool) -> int:
@@ -667,20 +676,41 @@ def slope_erode_cap(state, coef, exit_slope):
 # isostasy (maps.py:1325-1410) and the smoother (field.py:104-124, 179-188)
 # ---------------------------------------------------------------------------
 def exchange(grid, data, linear=False):
-    """field.py:104-124 for a float64 scalar field, in place."""
+    """field.py:104-124 for a float64 scalar field, in place: each halo cell
+    the weighted sum of its 16 interior sources.  The reference writes it as
+    np.einsum, which leaves the summation order to numpy's SIMD kernels (so
+    the reference's own result depends on the machine); the oracle fixes the
+    order -- source after source, one rounding per product and per sum -- and
+    the einsum agrees with it to the forward error bound of a 16-term dot
+    product (tests/test_gpu_sidefuncs.py)."""
     hm = grid.halo
     flat = data.reshape(6 * grid.NE * grid.NE)
     w = (hm.w_lin if linear else hm.w).astype(flat.dtype)
-    flat[hm.dst] = np.einsum("ks,ks...->k...", w, flat[hm.src])
+    src = flat[hm.src]
+    acc = np.zeros(hm.dst.size, flat.dtype)
+    for s in range(w.shape[1]):
+        acc = acc + w[:, s] * src[:, s]
+    flat[hm.dst] = acc
     return data
 
 
 def exchange_vector(grid, data):
-    """field.py:104-124 for a (..., 2) tangent vector field: each source
-    rotated into the destination face's frame, then weighted, in place."""
+    """field.py:104-124 for a (..., 2) tangent vector field, in place: each
+    source rotated into the destination face's frame (R x, the two products
+    summed), weighted and summed source after source (the einsum's order
+    fixed as in `exchange`)."""
     hm = grid.halo
     flat = data.reshape(6 * grid.NE * grid.NE, 2)
-    flat[hm.dst] = np.einsum("ks,ksab,ksb->ka", hm.w, hm.rot, flat[hm.src])
+    x = flat[hm.src]
+    R = hm.rot
+    a0 = np.zeros(hm.dst.size)
+    a1 = np.zeros(hm.dst.size)
+    for s in range(hm.w.shape[1]):
+        ws = hm.w[:, s]
+        a0 = a0 + ws * (R[:, s, 0, 0] * x[:, s, 0] + R[:, s, 0, 1] * x[:, s, 1])
+        a1 = a1 + ws * (R[:, s, 1, 0] * x[:, s, 0] + R[:, s, 1, 1] * x[:, s, 1])
+    flat[hm.dst, 0] = a0
+    flat[hm.dst, 1] = a1
     return data
 
 

@@ -77,12 +77,18 @@ static const double BASES[6][3][3] = {
  * run the oracle with correctly rounded tan/atan, which isolates the one
  * place where glibc's last-bit rounding differs from the device's. */
 typedef double (*or_fn1)(double);
+typedef void (*or_sincos_fn)(double, double *, double *);
 static or_fn1 or_tan = tan;
 static or_fn1 or_atan = atan;
+/* the initial direction of a droplet that carries a load (particle.py:400-401:
+ * math.cos / math.sin of its angle): glibc's by default, or a swapped-in
+ * sincos(x, &sin, &cos) */
+static or_sincos_fn or_sincos = 0;
 void or_set_libm(or_fn1 t, or_fn1 a) {
     or_tan = t ? t : tan;
     or_atan = a ? a : atan;
 }
+void or_set_sincos(or_sincos_fn f) { or_sincos = f; }
 static const double HALF_PI = OR_PI / 2.0;     /* cubesphere.py:50 */
 static const double TWO_OVER_PI = 2.0 / OR_PI; /* cubesphere.py:51 */
 
@@ -336,8 +342,12 @@ static void trace_one(int64_t p, const int64_t *sp_face, const double *sp_x, con
     double sed = sp_sed[p];
     double sa = 0.0, sb = 0.0;
     if (sed > 0.0) {
-        sa = cos(sp_dir[p]);
-        sb = sin(sp_dir[p]);
+        if (or_sincos) {
+            or_sincos(sp_dir[p], &sb, &sa);
+        } else {
+            sa = cos(sp_dir[p]);
+            sb = sin(sp_dir[p]);
+        }
     }
     int64_t base = p * cap;
     int64_t n = 0;

@@ -114,13 +114,13 @@ def test_globe_first_iteration_matches_oracle(oracle, hiplib, cuda, N, libm):
     print(f"globe N={N} iteration 0, oracle libm {libm}", flush=True)
     cr = libm == "correctly-rounded"
     if cr:
-        oracle.set_libm(hiplib.pg_geom_tan, hiplib.pg_geom_atan)
+        oracle.set_libm(*oracle.device_libm(hiplib))
     # glibc: measured 52 of 98,304 droplets at N=128 (0.05 %); bound 0.1 %
     allow = 0 if cr else -(-int(round(p.erosion.particles_per_cell * 6 * N * N)) // 1000)
     try:
         so, sd = _run_pair(oracle, ost, dst, p, 0, allow=allow)
     finally:
-        oracle.set_libm(None, None)
+        oracle.set_libm()
     if cr:
         fr = _cmp_state(ost, dst, exact_frac=1.0, rtol=0.0)
     else:
@@ -135,26 +135,32 @@ def test_globe_first_iteration_matches_oracle(oracle, hiplib, cuda, N, libm):
     print("bit-identical fractions", fr, flush=True)
 
 
-def test_globe_steps_stay_close(oracle, cuda):
-    """Three full steps (particles, uplift, datum hold, halo exchange) on the
-    N=64 globe: identical cell sequences need not survive the halo exchange
-    (the reference's einsum order is unpinned), so the bar is closeness."""
-    N = 64
+@pytest.mark.parametrize("N,iters", [(64, 3), (256, 8)])
+def test_globe_steps_match_oracle(oracle, hiplib, cuda, N, iters):
+    """Whole steps (particles, uplift, datum hold, halo exchange) on the
+    earth globe, with the oracle on the device's correctly rounded tan /
+    atan and the halo sums in source order: the state and every step's
+    statistics bit for bit, iteration after iteration."""
     grid, f = stubs.earth_globe(N, 2, n_blobs=16)
-    p = stubs.workload_params(N, 2)
-    ost = oracle.OracleState.from_grid(grid, f["bedrock_m"], f["hardness"], f["precip"], f["evap"], f["uplift_m"], p.erosion)
-    dst = maps.ErosionState.from_grid(grid, f["bedrock_m"], f["hardness"], f["precip"], f["evap"], f["uplift_m"], p.erosion, device=cuda)
-    for it in range(3):
-        print(f"globe step {it}", flush=True)
-        so = oracle.step(ost, p, it)
-        sd = maps.step(dst, p, it)
-        assert sd["particles"] == so["particles"]
-        assert sd["datum_shift"] == pytest.approx(so["datum_shift"], rel=1e-6, abs=1e-12)
-    h_o = ost.height + ost.sediment
-    h_d = (dst.height + dst.sediment).cpu().numpy()
-    inter = ost.interior
-    scale = np.abs(h_o[inter]).max()
-    assert np.abs(h_d[inter] - h_o[inter]).max() <= 1e-6 * scale
+    p = stubs.workload_params(N, 2, max_steps=128, uplift_mode="replay", iterations=iters)
+    args = (grid, f["bedrock_m"], f["hardness"], f["precip"], f["evap"], f["uplift_m"], p.erosion)
+    ost = oracle.OracleState.from_grid(*args)
+    dst = maps.ErosionState.from_grid(*args, device=cuda)
+    oracle.set_libm(*oracle.device_libm(hiplib))
+    try:
+        io, idv = oracle.start_replay(ost, p), maps.start_replay(dst, p)
+        assert float(idv["datum_shift"]) == float(io["datum_shift"])
+        for it in range(iters):
+            print(f"globe N={N} step {it}", flush=True)
+            so = oracle.step(ost, p, it)
+            sd = maps.step(dst, p, it)
+            for k in ("particles", "entries", "deaths", "clamped", "to_pending", "deficit_out", "lost_offshore"):
+                assert sd[k] == so[k], (it, k, sd[k], so[k])
+            assert float(sd["datum_shift"]) == float(so["datum_shift"]), it
+            for n in ("height", "sediment", "pending", "discharge", "momentum"):
+                assert np.array_equal(getattr(dst, n).cpu().numpy(), getattr(ost, n)), (it, n)
+    finally:
+        oracle.set_libm()
 
 
 @pytest.mark.parametrize("inputs,rate", [("plain", 50.0), ("tilt", 5.0)])

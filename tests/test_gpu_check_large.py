@@ -37,12 +37,12 @@ def test_globe_512_first_iteration_bit_exact(oracle, hiplib, cuda):
     ost = oracle.OracleState.from_grid(*args)
     dst = maps.ErosionState.from_grid(*args, device=cuda)
     o, d = [], []
-    oracle.set_libm(hiplib.pg_geom_tan, hiplib.pg_geom_atan)
+    oracle.set_libm(*oracle.device_libm(hiplib))
     try:
         print(f"globe N={N}: oracle iteration (correctly rounded tan / atan)", flush=True)
         so = oracle.run_iteration(ost, p, 0, diag=_digest(o, lambda x: x))
     finally:
-        oracle.set_libm(None, None)
+        oracle.set_libm()
     print(f"globe N={N}: check-mode iteration on the device", flush=True)
     sd = maps.run_iteration(dst, p, 0, diag=_digest(d, lambda t: t.cpu().numpy()))
     assert len(o) == len(d)

@@ -94,12 +94,12 @@ def test_earth_first_iteration_matches_oracle(oracle, hiplib, cuda, N):
     args = (grid, f["bedrock_m"], f["hardness"], f["precip"], f["evap"], f["uplift_m"], p.erosion)
     ost = oracle.OracleState.from_grid(*args)
     dst = maps.ErosionState.from_grid(*args, device=cuda)
-    oracle.set_libm(hiplib.pg_geom_tan, hiplib.pg_geom_atan)
+    oracle.set_libm(*oracle.device_libm(hiplib))
     try:
         print(f"earth N={N}: oracle iteration", flush=True)
         so = oracle.run_iteration(ost, p, 0)
     finally:
-        oracle.set_libm(None, None)
+        oracle.set_libm()
     print(f"earth N={N}: device iteration (parallel apply)", flush=True)
     sd = maps.run_iteration(dst, p, 0)
     print(f"  {sd['particles']} droplets, {sd['entries']} entries, clamped {sd['clamped']}, "

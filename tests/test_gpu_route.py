@@ -2,8 +2,8 @@
 
 Bar: the filled surface bit for bit on every cell the flood assigns (the
 device relaxes the flood's fixed point; its values are the flood's own
-chains m + eps), the halos refreshed by the linear exchange to 1e-12
-relative (the einsum order is not pinned); a window's step with the flood and
+chains m + eps), the halos refreshed by the linear exchange bit for bit
+(both sum the sources in source order); a window's step with the flood and
 its lakes bit for bit (no halo exchange there); the reference's window-lakes
 test on the device.
 """
@@ -32,8 +32,7 @@ def _cmp_route(ost, dst):
     b = ost.route
     inter = ost.interior
     assert np.array_equal(a[inter], b[inter])
-    scale = np.abs(b).max()
-    assert np.abs(a - b).max() <= 1e-12 * scale
+    assert np.array_equal(a, b)
 
 
 def test_globe_route_matches_the_heap_flood(oracle, cuda):
