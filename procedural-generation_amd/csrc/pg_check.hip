@@ -21,22 +21,26 @@
 namespace pg {
 namespace {
 
+static_assert((SPREAD & (SPREAD - 1)) == 0, "the ring index wraps with a mask");
+
+// Cell ids are 32-bit (F * NE * NE < 2^31, checked on the host); only the
+// record's address is formed in 64 bits.
 struct Record {
     const float* s;
-    int64_t NE;
-    __device__ __forceinline__ float at(int64_t f, int64_t i, int64_t j, int c) const {
-        return s[((f * NE + i) * NE + j) * NS + c];
+    int NE;
+    __device__ __forceinline__ float at(int f, int i, int j, int c) const {
+        return s[(size_t)((f * NE + i) * NE + j) * NS + c];
     }
 };
 
 // Bilinear corner and weights at a fractional extended index (particle.py:108-130).
 struct Corner {
-    int64_t i0, j0;
+    int i0, j0;
     double wi, wj;
 };
 
-__device__ __forceinline__ Corner corner_of(double fi, double fj, int64_t NE) {
-    int64_t i0 = (int64_t)floor(fi), j0 = (int64_t)floor(fj);
+__device__ __forceinline__ Corner corner_of(double fi, double fj, int NE) {
+    int i0 = (int)floor(fi), j0 = (int)floor(fj);
     i0 = i0 < 0 ? 0 : (i0 > NE - 2 ? NE - 2 : i0);
     j0 = j0 < 0 ? 0 : (j0 > NE - 2 ? NE - 2 : j0);
     double wi = fi - (double)i0, wj = fj - (double)j0;
@@ -46,8 +50,8 @@ __device__ __forceinline__ Corner corner_of(double fi, double fj, int64_t NE) {
 }
 
 // particle.py:133-140
-__device__ __forceinline__ int64_t cell_of(double x, int64_t N) {
-    int64_t i = (int64_t)floor(x);
+__device__ __forceinline__ int cell_of(double x, int N) {
+    int i = (int)floor(x);
     return i < 0 ? 0 : (i > N - 1 ? N - 1 : i);
 }
 
@@ -56,7 +60,7 @@ struct Quad {
     float v00, v10, v01, v11;
 };
 
-__device__ __forceinline__ Quad quad(const Record& R, int64_t f, const Corner& k, int c) {
+__device__ __forceinline__ Quad quad(const Record& R, int f, const Corner& k, int c) {
     return {R.at(f, k.i0, k.j0, c), R.at(f, k.i0 + 1, k.j0, c), R.at(f, k.i0, k.j0 + 1, c), R.at(f, k.i0 + 1, k.j0 + 1, c)};
 }
 
@@ -75,7 +79,7 @@ __device__ __forceinline__ void gradient(const Quad& q, const Corner& k, double*
 }
 
 // Any routing flag in the stencil (particle.py:227-233), float32 sum.
-__device__ __forceinline__ bool route_flagged(const Record& R, int64_t f, const Corner& k) {
+__device__ __forceinline__ bool route_flagged(const Record& R, int f, const Corner& k) {
     Quad q = quad(R, f, k, S_RFLAG);
     float s = q.v00 + q.v10;
     s = s + q.v01;
@@ -87,8 +91,8 @@ __device__ __forceinline__ bool route_flagged(const Record& R, int64_t f, const 
 struct Rec16 {
     float4 q[4];
 };
-__device__ __forceinline__ Rec16 load_rec(const float* s, int64_t cell) {
-    const float4* r = (const float4*)(s + cell * NS);
+__device__ __forceinline__ Rec16 load_rec(const float* s, int cell) {
+    const float4* r = (const float4*)(s + (size_t)cell * NS);
     return {{r[0], r[1], r[2], r[3]}};
 }
 __device__ __forceinline__ float chan(const Rec16& r, int c) {
@@ -109,12 +113,12 @@ struct Corner6 {
 struct Sten {
     Corner6 c[4];
 };
-__device__ __forceinline__ Corner6 load_corner(const float* s, int64_t cell, bool route) {
-    const float4* r = (const float4*)(s + cell * NS);
-    return {r[0], s[cell * NS + S_MOMB], route ? s[cell * NS + S_RFLAG] : 0.0f};
+__device__ __forceinline__ Corner6 load_corner(const float* s, int cell, bool route) {
+    const float* c = s + (size_t)cell * NS;
+    return {*(const float4*)c, c[S_MOMB], route ? c[S_RFLAG] : 0.0f};
 }
-__device__ __forceinline__ Sten load_sten(const Record& R, int64_t f, const Corner& k, bool route) {
-    const int64_t NE = R.NE, b = (f * NE + k.i0) * NE + k.j0;
+__device__ __forceinline__ Sten load_sten(const Record& R, int f, const Corner& k, bool route) {
+    const int NE = R.NE, b = (f * NE + k.i0) * NE + k.j0;
     return {{load_corner(R.s, b, route), load_corner(R.s, b + NE, route), load_corner(R.s, b + 1, route),
              load_corner(R.s, b + NE + 1, route)}};
 }
@@ -132,7 +136,7 @@ __device__ __forceinline__ bool sflagged(const Sten& S) {
     sum = sum + q.v11;
     return sum != 0.0f;
 }
-__device__ __forceinline__ Rec16 cell_rec(const Record& R, int64_t f, int64_t i, int64_t j) {
+__device__ __forceinline__ Rec16 cell_rec(const Record& R, int f, int i, int j) {
     return load_rec(R.s, (f * R.NE + i) * R.NE + j);
 }
 
@@ -182,9 +186,10 @@ struct Slice {
     double* delta;
     float* vol;
     float* mom;
-    int64_t base, n, lo, hi;
-    __device__ __forceinline__ void put(int64_t c, double d, float v, float m0, float m1) {
-        int64_t e = base + n;
+    int64_t base;
+    int n, lo, hi;
+    __device__ __forceinline__ void put(int c, double d, float v, float m0, float m1) {
+        const int64_t e = base + n;
         cell[e] = (int32_t)c;
         delta[e] = d;
         vol[e] = v;
@@ -201,13 +206,13 @@ __global__ void __launch_bounds__(64) trace_check_kernel(TraceArgs a) {
     if (p >= a.P) return;
     const pg_trace_params& P = a.prm;
     const Record& R = a.R;
-    const int64_t N = a.N, H = a.H, NE = N + 2 * H;
+    const int N = (int)a.N, H = (int)a.H, NE = N + 2 * H;
     const int64_t cap = change_list_cap(P.max_steps, P.lateral_rate > 0.0);
     const double Nf = (double)N;
     const double inv_sat = P.disc_saturation > 0.0 ? 1.0 / P.disc_saturation : 0.0;
-    auto maskat = [&](int64_t f, int64_t i, int64_t j) -> int { return a.mask[(f * NE + i) * NE + j]; };
+    auto maskat = [&](int f, int i, int j) -> int { return a.mask[(f * NE + i) * NE + j]; };
 
-    int64_t f = a.sp_face[p];
+    int f = (int)a.sp_face[p];
     double x = a.sp_x[p], y = a.sp_y[p];
     double vol = a.volume0;
     double sed = a.sp_sed[p];
@@ -217,28 +222,25 @@ __global__ void __launch_bounds__(64) trace_check_kernel(TraceArgs a) {
     Slice L{a.cl_cell, a.cl_delta, a.cl_vol, a.cl_mom, p * cap, 0, 2147483647, -1};
     // ring of the last SPREAD active cells, in the slice's tail (particle.py:407-412)
     const int64_t rbase = L.base + cap - SPREAD;
-    int64_t nring = 0, rpos = 0;
+    int nring = 0, rpos = 0;  // rpos in [0, SPREAD)
     double h_prev = -1e300;
     bool exited = false, in_sea = false, in_lake = false;
-    int64_t uphill = 0, parks = 0, sea_steps = 0;
+    int64_t uphill = 0, sea_steps = 0;
+    int parks = 0;
     int cause = DEATH_AGE;
 
     // the first step's stencil, cell record and mask; later steps get theirs
     // from the step before
-    Sten S;
-    Rec16 cr;
-    int m;
-    {
-        const int64_t ei = cell_of(x, N) + H, ej = cell_of(y, N) + H;
-        const Corner kc = corner_of(x - 0.5 + (double)H, y - 0.5 + (double)H, NE);
-        S = load_sten(R, f, kc, a.use_route);
-        cr = cell_rec(R, f, ei, ej);
-        m = maskat(f, ei, ej);
-    }
+    // (and the step's cell and bilinear corner, which are the previous step's
+    // next cell and next corner: the same expressions of the same position)
+    int ei = cell_of(x, N) + H, ej = cell_of(y, N) + H;
+    Corner k0 = corner_of(x - 0.5 + (double)H, y - 0.5 + (double)H, NE);
+    Sten S = load_sten(R, f, k0, a.use_route);
+    Rec16 cr = cell_rec(R, f, ei, ej);
+    int m = maskat(f, ei, ej);
     for (int64_t step = 0; step < P.max_steps; step++) {
-        const int64_t ei = cell_of(x, N) + H, ej = cell_of(y, N) + H;
         if (m == MASK_OUTSIDE) { exited = true; cause = DEATH_EXIT; break; }
-        const int64_t cell = (f * NE + ei) * NE + ej;
+        const int cell = (f * NE + ei) * NE + ej;
         const float h_here = chan(cr, S_SURF);
         const float b_here = chan(cr, S_BASE);
         if (step == 0 && h_here < b_here) in_sea = true;
@@ -248,11 +250,9 @@ __global__ void __launch_bounds__(64) trace_check_kernel(TraceArgs a) {
         }
         if (m == MASK_ACTIVE) {
             a.cl_cell[rbase + rpos] = (int32_t)cell;
-            rpos = (rpos + 1) % SPREAD;
+            rpos = (rpos + 1) & (SPREAD - 1);
             if (nring < SPREAD) nring += 1;
         }
-        const double fi = x - 0.5 + (double)H, fj = y - 0.5 + (double)H;
-        const Corner k0 = corner_of(fi, fj, NE);
         // -- forces (particle.py:450-511)
         const Quad hs = squad(S, S_SURF);
         const double h0 = bilinear(hs, k0);
@@ -303,7 +303,7 @@ __global__ void __launch_bounds__(64) trace_check_kernel(TraceArgs a) {
         sb /= slen;
         // -- one cell step, across a face edge if need be (particle.py:513-549)
         double nx = x + sa, ny = y + sb;
-        int64_t nf = f;
+        int nf = f;
         if (a.spherical) {
             if (nx < 0.0 || nx >= Nf || ny < 0.0 || ny >= Nf) {
                 const Transfer t = transfer_out_of_line((int)f, nx / Nf, ny / Nf, sa, sb);
@@ -320,17 +320,19 @@ __global__ void __launch_bounds__(64) trace_check_kernel(TraceArgs a) {
         } else if (nx < (double)(-H) || nx >= Nf + (double)H || ny < (double)(-H) || ny >= Nf + (double)H) {
             exited = true; cause = DEATH_EXIT; break;
         }
-        const int64_t nei = (int64_t)floor(nx) + H, nej = (int64_t)floor(ny) + H;
+        const int nei = (int)floor(nx) + H, nej = (int)floor(ny) + H;
         if (nei < 0 || nei >= NE || nej < 0 || nej >= NE) { exited = true; cause = DEATH_EXIT; break; }
         // everything the rest of the step and the next step read, in flight at once
         const double nfi = nx - 0.5 + (double)H, nfj = ny - 0.5 + (double)H;
         const Corner k1 = corner_of(nfi, nfj, NE);
         const Sten Sn = load_sten(R, nf, k1, a.use_route);
-        const int nmask = maskat(nf, nei, nej);
-        const int64_t ei2 = cell_of(nx, N) + H, ej2 = cell_of(ny, N) + H;
+        const int ei2 = cell_of(nx, N) + H, ej2 = cell_of(ny, N) + H;
         const Rec16 crn = cell_rec(R, nf, ei2, ej2);
-        const int m2 = maskat(nf, ei2, ej2);  // the next step's cell (nei, nej on the globe)
-        const Rec16 nrec = cell_rec(R, nf, nei, nej);
+        const int m2 = maskat(nf, ei2, ej2);
+        // the cell stepped to: on the globe the position is inside its face, so
+        // it is the next step's cell; a window's can lie in the halo
+        const int nmask = a.spherical ? m2 : maskat(nf, nei, nej);
+        const Rec16 nrec = a.spherical ? crn : cell_rec(R, nf, nei, nej);
         if (nmask == MASK_OUTSIDE) { exited = true; cause = DEATH_EXIT; break; }
         const double h1 = bilinear(squad(Sn, S_SURF), k1);
         const float b1 = chan(nrec, S_BASE);
@@ -413,7 +415,7 @@ __global__ void __launch_bounds__(64) trace_check_kernel(TraceArgs a) {
                 if (turn > 1e-6) {
                     const double sgn = cross > 0.0 ? 1.0 : -1.0;
                     const double bx = x + sgn * sb, by = y - sgn * sa;
-                    const int64_t bei = (int64_t)floor(bx) + H, bej = (int64_t)floor(by) + H;
+                    const int bei = (int)floor(bx) + H, bej = (int)floor(by) + H;
                     if (0 <= bei && bei < NE && 0 <= bej && bej < NE && maskat(f, bei, bej) == MASK_ACTIVE) {
                         const float bank = R.at(f, bei, bej, S_SURF) - h_here;
                         if (bank > 0.0f) {
@@ -438,6 +440,9 @@ __global__ void __launch_bounds__(64) trace_check_kernel(TraceArgs a) {
         x = nx;
         y = ny;
         f = nf;
+        ei = ei2;
+        ej = ej2;
+        k0 = k1;
         S = Sn;
         cr = crn;
         m = m2;
@@ -459,22 +464,22 @@ __global__ void __launch_bounds__(64) trace_check_kernel(TraceArgs a) {
     if (in_sea && (cause == DEATH_AGE || cause == DEATH_STOP)) cause = DEATH_OCEAN;
     a.sp_death[p] = (int8_t)cause;
     if (!exited) {
-        const int64_t ei = cell_of(x, N) + H, ej = cell_of(y, N) + H;
-        if (maskat(f, ei, ej) == MASK_ACTIVE && !(in_sea && R.at(f, ei, ej, S_SURF) >= R.at(f, ei, ej, S_BASE))) {
-            const int64_t cell = (f * NE + ei) * NE + ej;
-            const int64_t newest = (rpos + SPREAD - 1) % SPREAD;
+        const int fei = cell_of(x, N) + H, fej = cell_of(y, N) + H;
+        if (maskat(f, fei, fej) == MASK_ACTIVE && !(in_sea && R.at(f, fei, fej, S_SURF) >= R.at(f, fei, fej, S_BASE))) {
+            const int cell = (f * NE + fei) * NE + fej;
+            const int newest = (rpos + SPREAD - 1) & (SPREAD - 1);
             if (nring == 0 || a.cl_cell[rbase + newest] != cell) {
                 a.cl_cell[rbase + rpos] = (int32_t)cell;
-                rpos = (rpos + 1) % SPREAD;
+                rpos = (rpos + 1) & (SPREAD - 1);
                 if (nring < SPREAD) nring += 1;
             }
         }
     }
     if (sed > 0.0 && nring > 0 && (!exited || P.deposit_on_exit)) {
         const double load = sed * vol / a.volume0;
-        const int64_t nout = cause == DEATH_OCEAN ? 1 : nring;
-        for (int64_t r = 0; r < nout; r++) {
-            const int64_t c = a.cl_cell[rbase + (rpos + SPREAD - 1 - r) % SPREAD];
+        const int nout = cause == DEATH_OCEAN ? 1 : nring;
+        for (int r = 0; r < nout; r++) {
+            const int c = a.cl_cell[rbase + ((rpos + SPREAD - 1 - r) & (SPREAD - 1))];
             L.put(c, r == 0 ? load : 0.0, -1.0f, 0.0f, 0.0f);
         }
     }
@@ -675,7 +680,10 @@ extern "C" int pg_trace_particles(int64_t P, const int64_t* sp_face, const doubl
     PG_REQUIRE(sp_face && sp_x && sp_y && sp_sed && sp_dir && samp && mask && cl_cell && cl_delta && cl_vol && cl_mom &&
                    cl_count && cl_lo && cl_hi && sp_death,
                "pg_trace_particles: NULL array");
-    TraceArgs a{P, sp_face, sp_x, sp_y, sp_sed, sp_dir, volume0, Record{samp, N + 2 * H}, use_route, mask, N, H,
+    // 32-bit cell ids: six faces of NE x NE cells (a window has one) below 2^31
+    PG_REQUIRE(6 * (N + 2 * H) * (N + 2 * H) < ((int64_t)1 << 31), "pg_trace_particles: N=%lld too large for 32-bit cell ids",
+               (long long)N);
+    TraceArgs a{P, sp_face, sp_x, sp_y, sp_sed, sp_dir, volume0, Record{samp, (int)(N + 2 * H)}, use_route, mask, N, H,
                 spherical, *prm, roots, use_roots, cl_cell, cl_delta, cl_vol, cl_mom, cl_count, cl_lo, cl_hi, sp_death};
     hipLaunchKernelGGL(trace_check_kernel, dim3(blocks_for(P, 64)), dim3(64), 0, (hipStream_t)stream, a);
     PG_LAUNCHED("trace_check_kernel");

@@ -79,7 +79,7 @@ constexpr int CTL_DONE = 0, CTL_SWEEPS = 1, CTL_CHANGED = 2;  // control words; 
 struct Layout {
     size_t flags, d, hs, hsn, ec, b, r, rs, rn, w, posw, cd, cs, cn;   // per entry (P * cap)
     size_t death, hsd, bd, re, drop, pend, pendf, fdirty, fc;        // per droplet
-    size_t keys_in, vals_in, keys_out, vals_out, sw;                 // per access slot
+    size_t keys_in, vals_in, keys_out, vals_out, sv;                 // per access slot
     size_t ownerw;                                                   // per entry
     size_t rmin;                                                     // per range
     size_t ctl, temp;
@@ -123,7 +123,7 @@ int layout_of(int64_t P, int64_t cap, int max_sweeps, Layout* L) {
     L->vals_in = take(na * 4);
     L->keys_out = take(na * 4);
     L->vals_out = take(na * 4);
-    L->sw = take(na * 8);
+    L->sv = take(na * 8);
     L->ownerw = take(ne * 4);
     L->rmin = take(((na + RANGE - 1) / RANGE + 1) * 4);
     L->ctl = take((size_t)(CTL_CHANGED + max_sweeps + 1) * 4);
@@ -175,7 +175,7 @@ struct OrdArgs {
     uint32_t* fdirty;  // per droplet: first entry whose read value changed (NONE: clean)
     double* fc;        // per droplet: final carries (deficit, surplus, clamped count)
     uint32_t *keys_in, *vals_in, *keys_out, *vals_out;
-    double* sw;
+    double* sv;  // per sorted slot: a write's amount W, a read's value R (coalesced copies)
     uint32_t* ownerw;  // per entry: the range whose wave walks the segment of its write
     uint32_t* rmin;    // per range: the first slot of a write that changed (NONE: clean)
     uint32_t* ctl;
@@ -305,20 +305,60 @@ __global__ void __launch_bounds__(BLOCK) ord_build_kernel(OrdArgs a) {
     if (a.moves && lane == 0 && n_moves) atomicAdd(a.moves, (unsigned long long)n_moves);
 }
 
-// 2b. sorted slot of every entry's write (the droplet half writes its amounts there)
+// 2b. sorted slot of every entry's write (the droplet half writes its amounts
+// there), and the slot values: a write's amount starts at 0, every read's
+// value at the chunk's starting acc of its cell (the value build gave it)
 __global__ void __launch_bounds__(BLOCK) ord_posw_kernel(OrdArgs a) {
     const int64_t i = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
-    if (i >= a.n_slots || a.keys_out[i] == a.hole) return;
+    if (i >= a.n_slots) return;
+    const uint32_t key = a.keys_out[i];
+    if (key == a.hole) return;
     const uint32_t val = a.vals_out[i];
     if ((val >> KIND_SHIFT) == K_WRITE) {
         a.posw[val & IDX_MASK] = (uint32_t)i;
-        a.sw[i] = 0.0;
+        a.sv[i] = 0.0;
+    } else {
+        a.sv[i] = a.acc[key];
     }
 }
 
 // ---------------------------------------------------------------------------
 // 3a. the droplet half: one wave per droplet
 // ---------------------------------------------------------------------------
+// One batch of a droplet's entries, every field loaded without waiting on the
+// flags (the workspace holds all P * cap entries), so the next batch's loads
+// can be in flight while this batch's carries are walked.
+struct Ent {
+    uint8_t fl;
+    double d, hs, ac, an, hsn, ec, wold;
+    float b, r;
+    uint32_t slot, owner;
+};
+
+__device__ __forceinline__ Ent load_ent(const OrdArgs& a, int64_t base, int64_t k0, int64_t k_first, int64_t cnt,
+                                        int sweep) {
+    // unconditional loads at an index clamped into the droplet's slice (see load_slots)
+    const int64_t k = k0 + lane_id();
+    const int64_t e = base + (k < a.cap ? k : a.cap - 1);
+    Ent t;
+    t.fl = a.flags[e];
+    t.d = a.d[e];
+    t.hs = a.hs[e];
+    t.ac = a.rs[e];
+    t.an = a.rn[e];
+    t.hsn = a.hsn[e];
+    t.ec = a.ec[e];
+    t.wold = a.w[e];
+    t.b = a.b[e];
+    t.r = a.r[e];
+    t.slot = a.posw[e];
+    t.owner = a.ownerw[e];
+    if (!(k >= k_first && k < cnt)) t.fl = 0;
+    if (!a.use_ecap) t.ec = 0.0;
+    if (sweep == 0) t.owner = 0u;
+    return t;
+}
+
 __device__ void droplet_wave(const OrdArgs& a, int64_t p, int sweep) {
     const int lane = lane_id();
     const int64_t cap = a.cap, base = p * cap, cnt = a.cl_count[p];
@@ -350,21 +390,21 @@ __device__ void droplet_wave(const OrdArgs& a, int64_t p, int sweep) {
         }
     }
     bool changed = false;
-    for (int64_t k0 = (k_first / WAVE) * WAVE; k0 < cnt; k0 += WAVE) {
+    const int64_t kb0 = (k_first / WAVE) * WAVE;
+    Ent en = load_ent(a, base, kb0, k_first, cnt, sweep);
+    for (int64_t k0 = kb0; k0 < cnt; k0 += WAVE) {
         const int64_t k = k0 + lane, e = base + k;
-        const bool mine = k >= k_first && k < cnt;
-        const uint8_t fl = mine ? a.flags[e] : 0;
+        const Ent nx = load_ent(a, base, k0 + WAVE, k_first, cnt, sweep);  // in flight during this batch's walk
+        const uint8_t fl = en.fl;
         const bool act = fl & EF_ACTIVE, ero = fl & EF_ERODE;
-        double d = 0.0, w = 0.0, x = 0.0, U = 0.0, wold = 0.0;
-        uint32_t slot = 0, owner = 0;
+        double d = 0.0, w = 0.0, x = 0.0, U = 0.0;
+        const double wold = en.wold;
+        const uint32_t slot = en.slot, owner = en.owner;
         bool clamped = false;
         if (act) {
-            wold = a.w[e];
-            slot = a.posw[e];
-            if (sweep > 0) owner = a.ownerw[e];
-            d = a.d[e];
-            const double hs = a.hs[e], ac = a.rs[e], an = a.rn[e];
-            const float b_c = a.b[e];
+            d = en.d;
+            const double hs = en.hs, ac = en.ac, an = en.an;
+            const float b_c = en.b;
             const double s_c = hs + ac;
             const bool sub = s_c < (double)b_c;
             if (ero) {
@@ -372,17 +412,17 @@ __device__ void droplet_wave(const OrdArgs& a, int64_t p, int sweep) {
                 double e_act = ev;
                 double rm = a.iter_erode + ac;
                 if (a.use_ecap) {
-                    const double r2 = a.ec[e] + ac;
+                    const double r2 = en.ec + ac;
                     if (r2 < rm) rm = r2;
                 }
                 if (e_act > rm) e_act = rm;
                 double floor_ = s_c >= (double)b_c ? (double)b_c : s_c;
                 if (fl & EF_RFLOOR) {
-                    const float r_c = a.r[e];
+                    const float r_c = en.r;
                     if ((double)r_c > floor_) floor_ = (double)r_c;
                 }
                 if (fl & EF_NB) {
-                    const double ns = a.hsn[e] + an;
+                    const double ns = en.hsn + an;
                     if (ns > floor_) floor_ = ns;
                 }
                 if (e_act > s_c - floor_) e_act = s_c - floor_;
@@ -396,7 +436,7 @@ __device__ void droplet_wave(const OrdArgs& a, int64_t p, int sweep) {
                 // successive "if (x > lim) x = lim" give the least of them
                 U = a.iter_deposit - ac;
                 if (fl & EF_NB) {
-                    double ceil_ = a.hsn[e] + an + a.dep_floor;
+                    double ceil_ = en.hsn + an + a.dep_floor;
                     if (sub && (fl & EF_STEP)) ceil_ -= a.dep_floor + a.fan_slope;
                     const double lim = ceil_ - s_c;
                     if (lim < U) U = lim;
@@ -451,11 +491,12 @@ __device__ void droplet_wave(const OrdArgs& a, int64_t p, int sweep) {
             a.cn[e] = lr < 0 ? N_in : Nb;
             if (!same_bits(w, wold)) {
                 a.w[e] = w;
-                a.sw[slot] = w;
+                a.sv[slot] = w;
                 if (sweep > 0) atomicMin(a.rmin + owner, slot);
                 changed = true;
             }
         }
+        en = nx;
     }
     // the stockpile of what the droplet could not place (particle.py:1019-1041)
     if (lane == 0) {
@@ -493,13 +534,35 @@ __device__ void droplet_wave(const OrdArgs& a, int64_t p, int sweep) {
 // ---------------------------------------------------------------------------
 // The wave takes every segment that starts in [r * RANGE, (r + 1) * RANGE)
 // and walks it to its end (possibly past the range).  Per batch of 64 slots:
-// coalesced loads, then the segment's running value over its writes in order
-// (one lane after the other), then each read's value from the last write
-// before it (or the segment's start value, or the running value carried in).
-struct Batch {
+// the slots' keys, kinds and values (coalesced; the next batch's loads are
+// issued before this batch's serial part), then the segment's running value
+// over its writes in order (one lane after the other), then each read's value
+// from the last write before it (or the segment's start value, or the
+// running value carried in).  A segment's first access is always a read (an
+// entry's self read precedes its write), and no write precedes it: its value
+// is the segment's start value, the chunk's starting acc of the cell.
+struct Slots {
     uint32_t key, val;
-    bool valid, start, write;
+    double v;  // the slot's value: a write's amount, a read's value
 };
+
+// Unconditional loads (the index clamped into the slots, the values past the
+// end masked afterwards): with no branch around them the compiler can keep
+// the next batch's loads in flight while it waits for this batch's.
+__device__ __forceinline__ Slots load_slots(const OrdArgs& a, int64_t base) {
+    const int64_t i = base + lane_id();
+    const int64_t ic = i < a.n_slots ? i : a.n_slots - 1;
+    Slots b;
+    b.key = a.keys_out[ic];
+    b.val = a.vals_out[ic];
+    b.v = a.sv[ic];
+    if (i >= a.n_slots) {
+        b.key = a.hole;
+        b.val = 0;
+        b.v = 0.0;
+    }
+    return b;
+}
 
 __device__ __forceinline__ bool first_start(const OrdArgs& a, int64_t r, int64_t* out) {
     const int lane = lane_id();
@@ -529,17 +592,18 @@ __device__ void range_wave(const OrdArgs& a, int64_t r, int sweep) {
     if (!FINISH && sweep > 0) {
         // only from the first write of this range's segments that changed: every
         // access before it keeps its value, and the value entering that write is
-        // the read of its own entry just before it (same cell, no write between)
+        // the read just before it (its own entry's: same cell, no write between)
         const uint32_t first = a.rmin[r];
         if (first == NONE) return;
         if (lane == 0) a.rmin[r] = NONE;
         i0 = first;
         cur = a.keys_out[first];
-        v = a.rs[a.vals_out[first] & IDX_MASK];
+        v = a.sv[first - 1];
     } else if (!first_start(a, r, &i0)) {
         return;
     }
     const int64_t hi = (r + 1) * RANGE;  // segments starting at or past hi belong to the next range
+    uint32_t carry_key = i0 > 0 ? a.keys_out[i0 - 1] : 0xffffffffu;  // the key of the slot before the batch
     double dv = 0.0, m0 = 0.0, m1 = 0.0, pv = 0.0;  // finish: tracks and stockpile running values
     bool wrote = false, tracked = false, parked = false;
     auto write_back = [&]() {
@@ -559,14 +623,15 @@ __device__ void range_wave(const OrdArgs& a, int64_t r, int sweep) {
         }
         if (parked) a.pending[cur] = pv;
     };
+    Slots b = load_slots(a, i0);
     for (int64_t base = i0;; base += WAVE) {
         const int64_t i = base + lane;
-        uint32_t key = a.hole, val = 0, prevkey = 0xffffffffu;
-        if (i < a.n_slots) {
-            key = a.keys_out[i];
-            val = a.vals_out[i];
-            if (i > 0) prevkey = a.keys_out[i - 1];
-        }
+        const bool more = base + WAVE < a.n_slots;
+        const Slots nb = load_slots(a, base + WAVE);  // in flight during this batch's serial part
+        const uint32_t key = b.key, val = b.val;
+        uint32_t prevkey = __shfl_up(key, 1);
+        if (lane == 0) prevkey = carry_key;
+        carry_key = (uint32_t)__builtin_amdgcn_readlane((int)key, WAVE - 1);
         // (resuming: the first slot continues its segment)
         const bool start = key != a.hole && (i == 0 || prevkey != key) && !(base == i0 && lane == 0 && cur != 0xffffffffu);
         // stop at the first start at or past hi, or at a hole (the tail of the sort)
@@ -575,9 +640,7 @@ __device__ void range_wave(const OrdArgs& a, int64_t r, int sweep) {
         const bool valid = lane < stop;
         const uint32_t kind = val >> KIND_SHIFT, idx = val & IDX_MASK;
         const bool write = valid && kind == K_WRITE;
-        double wv = 0.0, a0 = 0.0;
-        if (write) wv = a.sw[i];
-        if (valid && start) a0 = a.acc[key];
+        const double wv = write ? b.v : 0.0, a0 = (valid && start) ? b.v : 0.0;
         double tv = 0.0, t0 = 0.0, t1 = 0.0, pk = 0.0, d0 = 0.0, mm0 = 0.0, mm1 = 0.0, p0 = 0.0;
         bool trk = false, pnd = false;
         if (FINISH) {
@@ -647,8 +710,9 @@ __device__ void range_wave(const OrdArgs& a, int64_t r, int sweep) {
                 if (sweep == 0 && kind == K_WRITE) a.ownerw[idx] = (uint32_t)r;
                 if (kind != K_WRITE) {
                     const double rv = (lw >= 0 && lw > ls) ? from_w : (ls >= 0 ? from_s : v_in);
-                    double* dst = kind == K_SELF ? a.rs + idx : (kind == K_NB ? a.rn + idx : a.re + idx);
-                    if (!same_bits(rv, *dst)) {
+                    if (!same_bits(rv, b.v)) {
+                        a.sv[i] = rv;
+                        double* dst = kind == K_SELF ? a.rs + idx : (kind == K_NB ? a.rn + idx : a.re + idx);
                         *dst = rv;
                         // the droplet reading it walks again from this entry
                         const int64_t dp = kind == K_END ? (int64_t)idx : (int64_t)idx / a.cap;
@@ -658,7 +722,8 @@ __device__ void range_wave(const OrdArgs& a, int64_t r, int sweep) {
                 }
             }
         }
-        if (stop < WAVE || base + WAVE >= a.n_slots) break;
+        if (stop < WAVE || !more) break;
+        b = nb;
     }
     write_back();
 }
@@ -886,7 +951,7 @@ int ordered_enqueue(const int32_t* cl_cell, const double* cl_delta, const float*
     a.vals_in = at<uint32_t>(work, L.vals_in);
     a.keys_out = at<uint32_t>(work, L.keys_out);
     a.vals_out = at<uint32_t>(work, L.vals_out);
-    a.sw = at<double>(work, L.sw);
+    a.sv = at<double>(work, L.sv);
     a.ctl = at<uint32_t>(work, L.ctl);
     a.status = status;
     a.hole = (uint32_t)(((uint64_t)1 << nbits) - 1);
