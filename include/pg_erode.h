@@ -126,8 +126,8 @@ int pg_apply_changes(const int32_t* cl_cell, const double* cl_delta, const float
  * values are the unique solution of "each entry's amount from the live
  * values it reads; each read value from the earlier amounts of its cell",
  * solved by alternating droplet walks and per-cell walks in serial order
- * until a sweep changes nothing (at most max_sweeps; 10-70 on the earth
- * globe).  The first launched_sweeps sweeps are enqueued as plain kernel
+ * until a sweep changes nothing (at most max_sweeps; 10-170 on the earth
+ * globe, thousands on clamp-dominated windows).  The first launched_sweeps sweeps are enqueued as plain kernel
  * launches that return at once after convergence; a chunk needing more
  * finishes in one cooperative launch.  n_cells = F * NE * NE.  status:
  * DEVICE int32[4], zeroed by the caller once per iteration; the call ORs
@@ -177,7 +177,13 @@ int pg_apply_changes_ordered_noalloc(const int32_t* cl_cell, const double* cl_de
  * droplet order from drop_out (DEVICE float64 [P * 4] scratch: each
  * droplet's clamped entries, stockpile, deficit and offshore loss).  moves
  * (DEVICE int64, may be NULL) += the steps traced (entries of volume >= 0);
- * timing != 0 records HIP events around every trace launch. */
+ * timing != 0 records HIP events around every trace launch.  A chunk whose
+ * lists hold lake parks (volume -3), or that does not converge within
+ * max_sweeps, is applied by the serial apply instead (pg_apply_changes'
+ * kernel, enqueued behind a device-side test: no host round trip; the same
+ * results); status[3] counts those chunks, and status[0] stays 0.
+ * lake_load / lake_id / lake_room as in pg_apply_changes (may be NULL when no
+ * list holds lake entries). */
 int pg_check_chunks(int64_t P, int64_t chunk, const int64_t* sp_face, const double* sp_x, const double* sp_y,
                     const double* sp_sed, const double* sp_dir, double volume0, float* samp, int use_route,
                     const uint8_t* mask, int64_t N, int64_t H, int spherical, const pg_trace_params* prm,
@@ -186,7 +192,8 @@ int pg_check_chunks(int64_t P, int64_t chunk, const int64_t* sp_face, const doub
                     const double* height, const double* sediment, double* acc, double* pending,
                     double* disch_track, double* mom_track, double iter_erode, double iter_deposit,
                     double fan_slope, double dep_floor, double offshore_writeoff, const double* erode_cap,
-                    int use_erode_cap, double* outs, int64_t* counts7, int32_t* status, int max_sweeps,
+                    int use_erode_cap, double* lake_load, const int32_t* lake_id, double* lake_room,
+                    double* outs, int64_t* counts7, int32_t* status, int max_sweeps,
                     int launched_sweeps, void* work, int64_t work_bytes, double* drop_out, int64_t* moves,
                     int timing, pg_stream stream);
 /* Summed duration (ms) and count of the trace launches pg_check_chunks timed

@@ -508,12 +508,17 @@ struct ApplyArgs {
     double* lake_load;
     const int32_t* lake_id;
     double* lake_room;
-    double* out4;
+    double* out4;          // the chunk's 4-tuple (NULL: not written)
+    const uint32_t* gate;  // run only if *gate != 0 (NULL: always)
+    double* drop;          // per droplet: its share of the 4-tuple (NULL: not written)
+    int32_t* count;        // + 1 when the kernel runs (NULL: not counted)
 };
 
 // One lane walks the chunk in droplet order (particle.py:902-1041).
 __global__ void __launch_bounds__(64) apply_serial_kernel(ApplyArgs a) {
     if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    if (a.gate && *a.gate == 0) return;
+    if (a.count) atomicAdd(a.count, 1);
     int64_t n_clamp = 0;
     double to_pending = 0.0, lost = 0.0, lost_offshore = 0.0;
     auto live = [&](int64_t c) { return a.h[c] + a.s[c] + a.acc[c]; };
@@ -522,6 +527,8 @@ __global__ void __launch_bounds__(64) apply_serial_kernel(ApplyArgs a) {
         const int64_t base = p * a.cap, cnt = a.cl_count[p];
         double deficit = 0.0, surplus = 0.0, refused = 0.0;
         int64_t prev = -1, death = -1;
+        const int64_t clamp0 = n_clamp;
+        double tp_p = 0.0, lo_p = 0.0;
         for (int64_t k = 0; k < cnt; k++) {
             const int64_t c = a.cl_cell[base + k];
             const float v = a.cl_vol[base + k];
@@ -625,19 +632,30 @@ __global__ void __launch_bounds__(64) apply_serial_kernel(ApplyArgs a) {
                 double keep = surplus * (1.0 - a.writeoff);
                 if (keep < 1e-4) keep = 0.0;
                 a.pending[death] += keep;
-                to_pending += keep;
-                lost_offshore += surplus - keep;
+                tp_p = keep;
+                lo_p = surplus - keep;
             } else {
                 a.pending[death] += surplus;
-                to_pending += surplus;
+                tp_p = surplus;
             }
         }
+        to_pending += tp_p;
+        lost_offshore += lo_p;
         lost += deficit;
+        if (a.drop) {
+            double* o = a.drop + 4 * p;
+            o[0] = (double)(n_clamp - clamp0);
+            o[1] = tp_p;
+            o[2] = deficit;
+            o[3] = lo_p;
+        }
     }
-    a.out4[0] = (double)n_clamp;
-    a.out4[1] = to_pending;
-    a.out4[2] = lost;
-    a.out4[3] = lost_offshore;
+    if (a.out4) {
+        a.out4[0] = (double)n_clamp;
+        a.out4[1] = to_pending;
+        a.out4[2] = lost;
+        a.out4[3] = lost_offshore;
+    }
 }
 
 // Entries and death causes of a chunk, added to counts7 (integers: exact).
@@ -708,11 +726,29 @@ extern "C" int pg_apply_changes(const int32_t* cl_cell, const double* cl_delta, 
                "pg_apply_changes: NULL array");
     ApplyArgs a{cl_cell, cl_delta, cl_vol, cl_mom, cl_count, P, cap, height, sediment, acc, pending, samp,
                 disch_track, mom_track, mask, iter_erode, iter_deposit, fan_slope, use_route, dep_floor,
-                offshore_writeoff, erode_cap, use_erode_cap, lake_load, lake_id, lake_room, out4};
+                offshore_writeoff, erode_cap, use_erode_cap, lake_load, lake_id, lake_room, out4, nullptr, nullptr,
+                nullptr};
     hipLaunchKernelGGL(apply_serial_kernel, dim3(1), dim3(64), 0, (hipStream_t)stream, a);
     PG_LAUNCHED("apply_serial_kernel");
     return 0;
 }
+
+namespace pg {
+int enqueue_serial_apply(const int32_t* cl_cell, const double* cl_delta, const float* cl_vol, const float* cl_mom,
+                         const int64_t* cl_count, int64_t P, int64_t cap, const double* height, const double* sediment,
+                         double* acc, double* pending, float* samp, double* disch_track, double* mom_track,
+                         const uint8_t* mask, double iter_erode, double iter_deposit, double fan_slope, int use_route,
+                         double dep_floor, double offshore_writeoff, const double* erode_cap, int use_erode_cap,
+                         double* lake_load, const int32_t* lake_id, double* lake_room, const uint32_t* gate,
+                         double* drop, int32_t* count, hipStream_t stream) {
+    ApplyArgs a{cl_cell, cl_delta, cl_vol, cl_mom, cl_count, P, cap, height, sediment, acc, pending, samp,
+                disch_track, mom_track, mask, iter_erode, iter_deposit, fan_slope, use_route, dep_floor,
+                offshore_writeoff, erode_cap, use_erode_cap, lake_load, lake_id, lake_room, nullptr, gate, drop, count};
+    hipLaunchKernelGGL(apply_serial_kernel, dim3(1), dim3(64), 0, stream, a);
+    PG_LAUNCHED("apply_serial_kernel (gated)");
+    return 0;
+}
+}  // namespace pg
 
 extern "C" int pg_chunk_stats(const int64_t* cl_count, const int8_t* sp_death, int64_t P, int64_t* counts7, pg_stream stream) {
     clear_error();

@@ -42,4 +42,17 @@ void clear_error();
 
 inline int blocks_for(int64_t n, int threads) { return (int)((n + threads - 1) / threads); }
 
+// The serial change-list apply (one lane, droplet order) enqueued behind a
+// device gate: it runs only if *gate != 0 when it starts (NULL: always),
+// writes each droplet's (clamped, to pending, deficit lost, lost offshore) to
+// drop[4 p ...] and, if `count` is given, adds 1 to it when it runs.
+// Defined in pg_check.hip.
+int enqueue_serial_apply(const int32_t* cl_cell, const double* cl_delta, const float* cl_vol, const float* cl_mom,
+                         const int64_t* cl_count, int64_t P, int64_t cap, const double* height, const double* sediment,
+                         double* acc, double* pending, float* samp, double* disch_track, double* mom_track,
+                         const uint8_t* mask, double iter_erode, double iter_deposit, double fan_slope, int use_route,
+                         double dep_floor, double offshore_writeoff, const double* erode_cap, int use_erode_cap,
+                         double* lake_load, const int32_t* lake_id, double* lake_room, const uint32_t* gate,
+                         double* drop, int32_t* count, hipStream_t stream);
+
 }  // namespace pg

@@ -45,7 +45,11 @@
 //  5. the chunk's 4-tuple, summed in droplet order.
 //
 // Lake parks (volume -3 entries, emitted only with lake_park > 0) are not
-// handled here: the status word reports them and nothing is applied.
+// handled here, and a chunk may need more than max_sweeps sweeps (a cell that
+// one chunk's droplets enter tens of thousands of times, as in a pit of a
+// clamp-dominated window, chains that many amounts).  Standalone, the status
+// word then reports it and nothing is applied; in the chunk loop the chunk
+// goes to the serial apply (one lane, the reference's walk) instead.
 //
 // Built with -ffp-contract=off: every expression rounds where the
 // reference's does (its operand order is kept below).
@@ -74,7 +78,8 @@ constexpr uint8_t EF_ACTIVE = 1, EF_ERODE = 2, EF_STEP = 4, EF_NB = 8, EF_RFLOOR
 constexpr uint32_t K_SELF = 0, K_NB = 1, K_WRITE = 2, K_END = 3;
 constexpr uint32_t KIND_SHIFT = 30, IDX_MASK = (1u << KIND_SHIFT) - 1u;
 constexpr uint32_t NONE = 0xffffffffu;
-constexpr int CTL_DONE = 0, CTL_SWEEPS = 1, CTL_CHANGED = 2;  // control words; CTL_CHANGED + s per sweep
+// control words: done, sweeps taken, the serial fallback's gate, then one change count per sweep
+constexpr int CTL_DONE = 0, CTL_SWEEPS = 1, CTL_FALLBACK = 2, CTL_CHANGED = 3;
 
 struct Layout {
     size_t flags, d, hs, hsn, ec, b, r, rs, rn, w, posw, cd, cs, cn;   // per entry (P * cap)
@@ -179,7 +184,9 @@ struct OrdArgs {
     uint32_t* ownerw;  // per entry: the range whose wave walks the segment of its write
     uint32_t* rmin;    // per range: the first slot of a write that changed (NONE: clean)
     uint32_t* ctl;
-    int32_t* status;  // [0] status bits (1 lake park, 2 no convergence), [1] most sweeps, [2] sweeps summed
+    int32_t* status;  // [0] status bits (1 lake park, 2 no convergence), [1] most sweeps, [2] sweeps summed,
+                      // [3] chunks applied by the serial fallback
+    int fallback;     // lake parks and non-convergence hand the chunk to the serial apply instead of failing
     uint32_t hole;
     int64_t n_slots, n_ranges;
     int max_sweeps;
@@ -216,9 +223,12 @@ __device__ __forceinline__ int last_upto(ull m, int lane) {
 // ---------------------------------------------------------------------------
 // 1. build: one wave per droplet
 // ---------------------------------------------------------------------------
+// 0. the chunk's control words start at zero (their own launch: build may set them)
+__global__ void __launch_bounds__(BLOCK) ord_reset_kernel(OrdArgs a) {
+    for (int i = threadIdx.x; i < CTL_CHANGED + a.max_sweeps + 1; i += BLOCK) a.ctl[i] = 0;
+}
+
 __global__ void __launch_bounds__(BLOCK) ord_build_kernel(OrdArgs a) {
-    if (blockIdx.x == 0)
-        for (int i = threadIdx.x; i < CTL_CHANGED + a.max_sweeps + 1; i += BLOCK) a.ctl[i] = 0;
     for (int64_t r = (int64_t)blockIdx.x * BLOCK + threadIdx.x; r < a.n_ranges; r += (int64_t)gridDim.x * BLOCK) a.rmin[r] = NONE;
     const int lane = lane_id();
     const int64_t p = (int64_t)blockIdx.x * WAVES_PER_BLOCK + (threadIdx.x / WAVE);
@@ -301,7 +311,15 @@ __global__ void __launch_bounds__(BLOCK) ord_build_kernel(OrdArgs a) {
             *vs = 0;
         }
     }
-    if (__ballot(bad) && lane == 0) atomicOr(a.status, 1);
+    if (__ballot(bad) && lane == 0) {
+        if (a.fallback) {
+            // lake parks: the serial apply takes the chunk, the sweeps return at once
+            a.ctl[CTL_FALLBACK] = 1;
+            a.ctl[CTL_DONE] = 1;
+        } else {
+            atomicOr(a.status, 1);
+        }
+    }
     if (a.moves && lane == 0 && n_moves) atomicAdd(a.moves, (unsigned long long)n_moves);
 }
 
@@ -768,6 +786,8 @@ __global__ void __launch_bounds__(BLOCK) ord_tail_kernel(OrdArgs a, int first) {
         if (sweep < a.max_sweeps) {
             a.ctl[CTL_DONE] = 1;
             a.ctl[CTL_SWEEPS] = (uint32_t)sweep + 1;
+        } else if (a.fallback) {
+            a.ctl[CTL_FALLBACK] = 1;
         } else {
             atomicOr(a.status, 2);
         }
@@ -781,7 +801,7 @@ __global__ void __launch_bounds__(BLOCK) ord_finish_kernel(OrdArgs a) {
         atomicMax(&a.status[1], sw);
         atomicAdd(&a.status[2], sw);
     }
-    if (a.status[0] != 0) return;
+    if (a.status[0] != 0 || a.ctl[CTL_FALLBACK]) return;
     const int64_t r = (int64_t)blockIdx.x * WAVES_PER_BLOCK + (threadIdx.x / WAVE);
     if (r < a.n_ranges) range_wave<true>(a, r, 0);
 }
@@ -825,7 +845,9 @@ __global__ void __launch_bounds__(BLOCK) ord_stats_kernel(OrdArgs a) {
 }
 
 __global__ void ord_tail_kernel_flag(OrdArgs a) {
-    if (!a.ctl[CTL_DONE]) atomicOr(a.status, 2);
+    if (a.ctl[CTL_DONE]) return;
+    if (a.fallback) a.ctl[CTL_FALLBACK] = 1;
+    else atomicOr(a.status, 2);
 }
 
 int tail_grid(int* blocks) {
@@ -873,7 +895,8 @@ int ordered_enqueue(const int32_t* cl_cell, const double* cl_delta, const float*
                     const uint8_t* mask, int64_t n_cells, double iter_erode, double iter_deposit, double fan_slope,
                     int use_route, double dep_floor, double offshore_writeoff, const double* erode_cap,
                     int use_erode_cap, double* out4, int32_t* status, int max_sweeps, int launched_sweeps, void* work,
-                    int64_t work_bytes, hipStream_t stream, double* drop_out, int64_t* moves) {
+                    int64_t work_bytes, hipStream_t stream, double* drop_out, int64_t* moves, int fallback = 0,
+                    double* lake_load = nullptr, const int32_t* lake_id = nullptr, double* lake_room = nullptr) {
     PG_REQUIRE(P >= 0 && cap > 0 && n_cells > 0, "pg_apply_changes_ordered: bad sizes P=%lld cap=%lld n_cells=%lld",
                (long long)P, (long long)cap, (long long)n_cells);
     PG_REQUIRE(max_sweeps >= 1 && launched_sweeps >= 0, "pg_apply_changes_ordered: max_sweeps must be >= 1, launched_sweeps >= 0");
@@ -958,10 +981,14 @@ int ordered_enqueue(const int32_t* cl_cell, const double* cl_delta, const float*
     a.n_slots = P * (3 * cap + 1);
     a.n_ranges = (a.n_slots + RANGE - 1) / RANGE;
     a.max_sweeps = max_sweeps;
+    a.fallback = fallback;
     a.out4 = out4;
     a.moves = (unsigned long long*)moves;
     const int dblocks = (int)((P + WAVES_PER_BLOCK - 1) / WAVES_PER_BLOCK);
     const int rblocks = (int)((a.n_ranges + WAVES_PER_BLOCK - 1) / WAVES_PER_BLOCK);
+    PG_REQUIRE(!fallback || drop_out, "pg_apply_changes_ordered: the serial fallback needs per-droplet outputs");
+    hipLaunchKernelGGL(ord_reset_kernel, dim3(1), dim3(BLOCK), 0, s, a);
+    PG_LAUNCHED("ord_reset_kernel");
     hipLaunchKernelGGL(ord_build_kernel, dim3(dblocks), dim3(BLOCK), 0, s, a);
     PG_LAUNCHED("ord_build_kernel");
     size_t tb = L.temp_bytes;
@@ -988,6 +1015,15 @@ int ordered_enqueue(const int32_t* cl_cell, const double* cl_delta, const float*
     }
     hipLaunchKernelGGL(ord_finish_kernel, dim3(rblocks), dim3(BLOCK), 0, s, a);
     PG_LAUNCHED("ord_finish_kernel");
+    if (fallback) {
+        // a chunk with lake parks, or one that did not converge within max_sweeps (nothing of it
+        // applied above), goes through the serial apply: the reference's own walk, one lane
+        if (enqueue_serial_apply(cl_cell, cl_delta, cl_vol, cl_mom, cl_count, P, cap, height, sediment, acc, pending,
+                                 samp, disch_track, mom_track, mask, iter_erode, iter_deposit, fan_slope, use_route,
+                                 dep_floor, offshore_writeoff, erode_cap, use_erode_cap, lake_load, lake_id, lake_room,
+                                 a.ctl + CTL_FALLBACK, drop_out, status + 3, s) != 0)
+            return -1;
+    }
     if (!drop_out) {
         hipLaunchKernelGGL(ord_stats_kernel, dim3(1), dim3(BLOCK), 0, s, a);
         PG_LAUNCHED("ord_stats_kernel");
@@ -1154,7 +1190,8 @@ extern "C" int pg_check_chunks(int64_t P, int64_t chunk, const int64_t* sp_face,
                                const double* height, const double* sediment, double* acc, double* pending,
                                double* disch_track, double* mom_track, double iter_erode, double iter_deposit,
                                double fan_slope, double dep_floor, double offshore_writeoff, const double* erode_cap,
-                               int use_erode_cap, double* outs, int64_t* counts7, int32_t* status, int max_sweeps,
+                               int use_erode_cap, double* lake_load, const int32_t* lake_id, double* lake_room,
+                               double* outs, int64_t* counts7, int32_t* status, int max_sweeps,
                                int launched_sweeps, void* work, int64_t work_bytes, double* drop_out,
                                int64_t* moves, int timing, pg_stream stream) {
     clear_error();
@@ -1174,7 +1211,8 @@ extern "C" int pg_check_chunks(int64_t P, int64_t chunk, const int64_t* sp_face,
         if (ordered_enqueue(cl_cell, cl_delta, cl_vol, cl_mom, cl_count, m, cap, height, sediment, acc, pending, samp,
                             disch_track, mom_track, mask, F * NE * NE, iter_erode, iter_deposit, fan_slope, use_route,
                             dep_floor, offshore_writeoff, erode_cap, use_erode_cap, nullptr, status, max_sweeps,
-                            launched_sweeps, work, work_bytes, (hipStream_t)stream, drop_out + 4 * c0, moves) != 0)
+                            launched_sweeps, work, work_bytes, (hipStream_t)stream, drop_out + 4 * c0, moves, 1,
+                            lake_load, lake_id, lake_room) != 0)
             return -1;
         if (pg_chunk_stats(cl_count, sp_death, m, counts7, stream) != 0) return -1;
         (void)ci;

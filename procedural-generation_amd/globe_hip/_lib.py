@@ -85,7 +85,7 @@ SIGNATURES = [
                                        i32, dbl, dbl, vp, i32, vp, vp, i32, i32, vp, i64, vp]),
     ("pg_check_chunks", i32, [i64, i64, vp, vp, vp, vp, vp, dbl, vp, i32, vp, i64, i64, i32, ctypes.POINTER(TraceParams),
                               vp, i32, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, dbl, dbl, dbl, dbl, dbl,
-                              vp, i32, vp, vp, vp, i32, i32, vp, i64, vp, vp, i32, vp]),
+                              vp, i32, vp, vp, vp, vp, vp, vp, i32, i32, vp, i64, vp, vp, i32, vp]),
     ("pg_check_trace_time", i32, [pd, ctypes.POINTER(ctypes.c_int64)]),
     ("pg_ordered_scratch_bytes", i64, []),
     ("pg_apply_changes_ordered_noalloc", i32, [vp, vp, vp, vp, vp, i64, i64, vp, vp, vp, vp, vp, vp, vp, vp, i64, dbl, dbl,

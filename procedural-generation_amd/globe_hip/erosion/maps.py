@@ -718,7 +718,10 @@ def run_iteration(state: ErosionState, params, iteration_key, n_particles=None, 
     use_roots = state.roots is not None
     t_trace = t_apply = 0.0
     lake_park = lake_park_of(state, ep)
-    ordered = state.check_apply == "ordered" and lake_park == 0.0
+    # the one-call chunk loop hands a chunk with lake parks (or one that does not converge) to the
+    # serial apply on the device; the chunk-by-chunk loop (diag hook) takes the serial apply for lake parks
+    one_call = state.check_apply == "ordered" and diag is None
+    ordered = state.check_apply == "ordered" and (lake_park == 0.0 or one_call)
     if ordered:
         key = (chunk, cap, int(state.check_max_sweeps))
         if state._ordered.get("key") != key:
@@ -728,8 +731,8 @@ def run_iteration(state: ErosionState, params, iteration_key, n_particles=None, 
         ord_work, ord_status = state._ordered["work"], state._ordered["status"]
         ord_status.zero_()
     t1 = time.time()
-    moves_dev = torch.zeros(1, dtype=torch.int64, device=dev) if ordered and diag is None else None
-    if ordered and diag is None:
+    moves_dev = torch.zeros(1, dtype=torch.int64, device=dev) if one_call else None
+    if one_call:
         # the whole chunk loop in one library call (no host round trip per chunk)
         pk.check_chunks(sp_face, sp_x, sp_y, sp_sed, sp_dir, volume0, chunk, state.samp, state.route is not None,
                         state.mask, state.N, state.H, state.spherical, prm, state.roots, use_roots, cl_cell, cl_delta,
@@ -737,8 +740,9 @@ def run_iteration(state: ErosionState, params, iteration_key, n_particles=None, 
                         state.pending, state.disch_track, state.mom_track, iter_erode, iter_deposit, float(ep.fan_slope),
                         dep_floor, float(ep.offshore_writeoff), ecap, ecap is not None, outs, counts, ord_status,
                         ord_work, _scratch(state, "ordered_drops", 4 * P, torch.float64), int(state.check_max_sweeps),
-                        int(state.check_launched_sweeps), moves=moves_dev, timing=state.check_time_trace)
-    for ci, c0 in enumerate(range(0, P, chunk) if not (ordered and diag is None) else ()):
+                        int(state.check_launched_sweeps), moves=moves_dev, timing=state.check_time_trace,
+                        lake_load=state.lake_load, lake_id=state.lake_id, lake_room=state.lake_room)
+    for ci, c0 in enumerate(range(0, P, chunk) if not one_call else ()):
         c1 = min(P, c0 + chunk)
         m = c1 - c0
         t1 = time.time()
@@ -774,6 +778,7 @@ def run_iteration(state: ErosionState, params, iteration_key, n_particles=None, 
             raise RuntimeError(f"run_iteration: {err}")
         stats["sweeps_max"] = int(ost[1])
         stats["sweeps_total"] = int(ost[2])
+        stats["serial_chunks"] = int(ost[3])
         if moves_dev is not None:
             stats["moves"] = int(moves_dev.item())
     # per-chunk partial results summed in chunk order, as the reference's loop does

@@ -193,14 +193,18 @@ def check_chunks(sp_face, sp_x, sp_y, sp_sed, sp_dir, volume0, chunk, samp, use_
                  params: TraceParams, roots, use_roots, cl_cell, cl_delta, cl_vol, cl_mom, cl_count, cl_lo, cl_hi,
                  sp_death, height, sediment, acc, pending, disch_track, mom_track, iter_erode, iter_deposit, fan_slope,
                  dep_floor, offshore_writeoff, erode_cap, use_erode_cap, outs, counts7, status, work, drop_out,
-                 max_sweeps, launched_sweeps=40, moves=None, timing=False):
+                 max_sweeps, launched_sweeps=40, moves=None, timing=False, lake_load=None, lake_id=None,
+                 lake_room=None):
     """The check mode's chunk loop (maps.py:953-1029) in one call: per chunk
     of ``chunk`` droplets trace_particles, the parallel apply and the chunk
     statistics, enqueued without a host round trip.  ``outs`` (float64
     [n_chunks, 4]) receives each chunk's 4-tuple, summed from ``drop_out``
     (float64 [P, 4] scratch).  ``moves`` (int64 [1], optional) accumulates the
     steps traced; ``timing`` records HIP events around each trace launch
-    (read them with ``check_trace_time``)."""
+    (read them with ``check_trace_time``).  A chunk with lake parks, or one
+    that does not converge within ``max_sweeps``, is applied by the serial
+    apply (same results); ``status[3]`` counts them.  ``lake_load`` /
+    ``lake_id`` / ``lake_room`` as in ``apply_changes``."""
     P = sp_face.shape[0]
     F, NE = samp.shape[0], samp.shape[1]
     e = (F, NE, NE)
@@ -243,6 +247,10 @@ def check_chunks(sp_face, sp_x, sp_y, sp_sed, sp_dir, volume0, chunk, samp, use_
         _need(roots, torch.float32, e, "roots")
     if use_erode_cap:
         _need(erode_cap, torch.float64, e, "erode_cap")
+    if lake_id is not None:
+        _need(lake_id, torch.int32, e, "lake_id")
+        _need(lake_load, torch.float64, e, "lake_load")
+        _need(lake_room, torch.float64, None, "lake_room")
     check(lib().pg_check_chunks(P, int(chunk), _ptr(sp_face), _ptr(sp_x), _ptr(sp_y), _ptr(sp_sed), _ptr(sp_dir),
                                 float(volume0), _ptr(samp), int(bool(use_route)), _ptr(mask), int(N), int(H),
                                 int(bool(spherical)), ctypes.byref(params), _ptr(roots if use_roots else None),
@@ -250,7 +258,9 @@ def check_chunks(sp_face, sp_x, sp_y, sp_sed, sp_dir, volume0, chunk, samp, use_
                                 _ptr(cl_count), _ptr(cl_lo), _ptr(cl_hi), _ptr(sp_death), _ptr(height), _ptr(sediment),
                                 _ptr(acc), _ptr(pending), _ptr(disch_track), _ptr(mom_track), float(iter_erode),
                                 float(iter_deposit), float(fan_slope), float(dep_floor), float(offshore_writeoff),
-                                _ptr(erode_cap if use_erode_cap else None), int(bool(use_erode_cap)), _ptr(outs),
+                                _ptr(erode_cap if use_erode_cap else None), int(bool(use_erode_cap)),
+                                _ptr(lake_load if lake_id is not None else None), _ptr(lake_id),
+                                _ptr(lake_room if lake_id is not None else None), _ptr(outs),
                                 _ptr(counts7), _ptr(status), int(max_sweeps), int(launched_sweeps), _ptr(work),
                                 int(work.numel()), _ptr(drop_out), _ptr(moves), int(bool(timing)), _stream()),
           "check_chunks")

@@ -7,7 +7,9 @@ Frozen-terrain gradient (feedback off, stable at any horizon) over the
 full 50 iterations: the float32 adjoint (the product path) against the
 float64 adjoint of the same trajectories (the float64 reference gradient),
 and that against central differences of the float64 frozen map on 32
-random land cells and all six scalars (relative steps 1e-3 and 1e-6).
+random land cells and all six scalars, asserted at both relative steps:
+1e-6 to the adjoint's own accuracy, 1e-3 (the configuration's step, which
+crosses the caps' kinks) on the median cell, the worst cell and the scalars.
 
 Exact frozen-trajectory gradient (feedback on): the same checks at a
 horizon where that map is still well conditioned (5 iterations; float32
@@ -71,7 +73,14 @@ def _fd(run, erod, theta, land, cells, h_rel):
     return np.array(fd_e), np.array(fd_t)
 
 
-def _check(st, run32, run64, erod, theta, T, label, n_cells=32, prec=1e-2, theta_tol=1e-3):
+#: finite-difference bars per relative step.  1e-6: the adjoint itself (cells to 1e-3 of the gradient plus 1e-4 of
+#: the largest sampled one; scalars to theta_tol).  1e-3 (the configuration's nominal step): the step crosses the
+#: kinks of the trace caps (min / max in the erosion and deposit limits), so the central difference averages
+#: across them -- the bar is on the median cell error and the scalars, with a looser bound on single cells
+FD_BARS = {1e-6: None, 1e-3: dict(cell_median=2e-2, cell_max=0.25, theta=1e-2)}
+
+
+def _check(st, run32, run64, erod, theta, T, label, n_cells=32, prec=1e-2, theta_tol=1e-3, bars=None):
     t0 = time.time()
     run32.forward(erod, theta, record=True, tapes="none")
     loss32, ge32, gt32, land = _grad(run32, erod, theta)
@@ -88,6 +97,8 @@ def _check(st, run32, run64, erod, theta, T, label, n_cells=32, prec=1e-2, theta
     cand = np.flatnonzero(inter.reshape(-1) & (np.abs(ge) > 0))
     cells = np.random.default_rng(0).choice(cand, n_cells, replace=False)
     gt = gt64.numpy()
+    scale = np.abs(ge[cells]).max()
+    bars = {**FD_BARS, **(bars or {})}
     for h_rel in (1e-3, 1e-6):
         t0 = time.time()
         fd_e, fd_t = _fd(run64, erod, theta, land, cells, h_rel)
@@ -95,9 +106,14 @@ def _check(st, run32, run64, erod, theta, T, label, n_cells=32, prec=1e-2, theta
         err_t = np.abs(fd_t - gt) / np.maximum(np.abs(gt), 1e-300)
         print(f"  FD h_rel {h_rel:g} ({time.time() - t0:.1f}s): field rel err median {np.median(err_e):.2e} "
               f"max {err_e.max():.2e}; theta rel err {np.array2string(err_t, precision=2)}", flush=True)
-    scale = np.abs(ge[cells]).max()
-    assert np.all(np.abs(fd_e - ge[cells]) <= 1e-3 * np.abs(ge[cells]) + 1e-4 * scale)
-    assert np.all(np.abs(fd_t - gt) <= theta_tol * np.abs(gt) + 1e-9)
+        bar = bars[h_rel]
+        if bar is None:
+            assert np.all(np.abs(fd_e - ge[cells]) <= 1e-3 * np.abs(ge[cells]) + 1e-4 * scale), h_rel
+            assert np.all(np.abs(fd_t - gt) <= theta_tol * np.abs(gt) + 1e-9), h_rel
+        else:
+            assert np.median(err_e) <= bar["cell_median"] and err_e.max() <= bar["cell_max"], (h_rel, err_e)
+            if bar.get("theta") is not None:
+                assert np.all(err_t <= bar["theta"]), (h_rel, err_t)
 
 
 def test_cfg4_frozen_terrain_gradient(cuda):
@@ -114,13 +130,16 @@ def test_cfg4_feedback_gradient(cuda):
     # 5 iterations, float32 within 5 % of float64, float64 against FD; the
     # six scalars touch all ~1e8 exchanges, some of which sit within even the
     # smallest step of a cap's kink, whose jump the surface feedback carries
-    # on: their bar is 1e-2 (measured 1.6e-5 .. 3.7e-3)
+    # on: their bar is 1e-2 (measured 1.6e-5 .. 3.7e-3).  At the configuration's
+    # step (1e-3) the feedback carries every crossed kink's jump on: the scalars'
+    # central differences are then no derivative estimate (measured 1e-3 .. 0.29
+    # from the adjoint) and only the cells are asserted (median 3.1e-3, worst 0.39)
     T = 5
     st, p, erod = _setup(cuda, T)
     run32 = adj.GradRun(st, p, T, real="f32", checkpoint_every=2, feedback=True)
     run64 = adj.GradRun(st, p, T, real="f64", checkpoint_every=2, feedback=True, replay_of=run32)
     _check(st, run32, run64, erod, adj.theta_nominal(run32), T, "cfg4 feedback", n_cells=16, prec=5e-2,
-           theta_tol=1e-2)
+           theta_tol=1e-2, bars={1e-3: dict(cell_median=2e-2, cell_max=0.5, theta=None)})
     # the full horizon: the surface adjoint's growth (reported)
     run = adj.GradRun(st, p, 50, real="f32", checkpoint_every=CE, feedback=True)
     out = run.forward(erod, adj.theta_nominal(run))

@@ -106,3 +106,44 @@ def test_autotuner_keeps_results(cuda):
         _same(a, b)
     print("autotuner times (ms)", tuner.times, "best", tuner.best, flush=True)
     assert tuner.best in tuner.candidates
+
+
+def _budget_closes(raw):
+    """The fixed-point mass ledger: deposited - eroded + new stockpile + lost
+    offshore + exits + vanished = released.  The totals are int64 sums of
+    2^-48-cell amounts, accumulated with wrapping int64 atomics; over a
+    15.6 M-cell batch they exceed 2^63, so the ledger is checked modulo
+    2^64, where it is exact (each per-cell amount is far inside the range)."""
+    from globe_hip import _lib
+
+    FS = _lib.FS
+    r = [int(x) for x in raw]
+    lhs = (r[FS["DEPOSITED_FX"]] - r[FS["ERODED_FX"]] + r[FS["TO_PENDING_FX"]] + r[FS["LOST_OFFSHORE_FX"]]
+           + r[FS["EXIT_FX"]] + r[FS["VANISHED_FX"]])
+    return (lhs - r[FS["RELEASED_FX"]]) % (1 << 64) == 0, lhs, r[FS["RELEASED_FX"]]
+
+
+def test_full_size_batch_equals_per_window(cuda):
+    """The batched-windows configuration at its own size: 64 windows of side
+    log-uniform in [128, 1024] (15.6 M interior cells), each its own fBm
+    seed, lifetime log-uniform in [32, 256].  Two iterations batched (one
+    launch per chunk and pass over all windows) and per window: every state
+    array bit for bit, and the batch's fixed-point mass ledger closes exactly."""
+    wins = stubs.zoom_windows(64, seed=0)
+    a = bt.WindowBatch([w for w, _ in wins], [p for _, p in wins], device=cuda)
+    b = bt.WindowBatch([w for w, _ in wins], [p for _, p in wins], device=cuda)
+    sides = sorted(int(x) - 4 for x in a.NE)
+    lives = sorted(int(x) for x in a.max_steps)
+    print(f"64 windows: sides {sides[0]}..{sides[-1]}, lifetimes {lives[0]}..{lives[-1]}", flush=True)
+    assert len(a.NE) == 64 and sides[0] >= 128 and sides[-1] <= 1024 and lives[0] >= 32 and lives[-1] <= 256
+    for it in range(2):
+        a.step(it)
+        b.step_per_window(it)
+        torch.cuda.synchronize()
+        print(f"iteration {it}: moves batched {a.moves()} per window {b.moves(per_window=True)}", flush=True)
+        assert a.moves() == b.moves(per_window=True)
+        # the batch's passes count into its own statistics, each window's spawn (the stockpiles it
+        # releases) into the window's: the ledger is over both
+        ok, lhs, rel = _budget_closes(a.stats.cpu().numpy() + a.win_stats.sum(0).cpu().numpy())
+        assert ok, (it, lhs, rel)
+        _same(a, b)

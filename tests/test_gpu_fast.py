@@ -364,3 +364,50 @@ def test_land_deaths_spread_their_load_like_the_oracle(oracle, cuda):
     assert land > 200 and so["deaths"]["ocean"] == 0
     assert abs(pend_d - pend_o) <= 0.25 * pend_o
     assert abs(dep_d - dep_o) <= 0.35 * dep_o
+
+
+def _hyps_and_eroded(s0, s, bins=64):
+    lo, hi = min(s0.min(), s.min()), max(s0.max(), s.max())
+    return np.histogram(s, bins, (lo, hi))[0], float(np.maximum(s0 - s, 0.0).sum())
+
+
+@pytest.mark.parametrize("w", [0, 1, 2])
+def test_windows_against_the_oracle(oracle, cuda, w):
+    """Zoom windows (side 128-256, own fBm seed, lifetime 32-128, 50 m
+    cells, plain profile) against the oracle's window run of the reference
+    algorithm over 10 iterations:
+
+    * the exact (check) mode: the surface bit for bit after every iteration;
+    * the fast mode (as many chunks as the reference's 2048-droplet chunks):
+      land hypsometry (64-bin CDF, max gap) and eroded volume, measured and
+      bounded.  These windows are clamp-dominated -- the serial apply clamps
+      ~45 % of their entries, mostly at the next-cell floor and the budget --
+      and the fast mode's per-cell rules erode 51-60 % more than the
+      reference (hypsometry 5.7-9.6 %): far outside the 0.5 % tolerance.  The
+      bounds pin that measured deviation against regressions; they are not a
+      tolerance claim (the exact mode is the parity path for windows)."""
+    win, p = stubs.zoom_windows(3, seed=11, side_lo=128, side_hi=256, life_lo=32, life_hi=128)[w]
+    ost = oracle.OracleState.window(**win)
+    chk = maps.ErosionState.window(**win, device=cuda, mode="check")
+    dst = maps.ErosionState.window(**win, device=cuda, mode="fast")
+    P = int(round(p.erosion.particles_per_cell * dst.N * dst.N))
+    dst.fast_chunks = max(1, -(-P // 2048))
+    inter = ost.interior
+    s0 = (ost.height + ost.sediment)[inter].ravel().copy()
+    for it in range(10):
+        oracle.step(ost, p, it)
+        maps.step(chk, p, it)
+        maps.step(dst, p, it)
+        for n in ("height", "sediment", "pending"):
+            assert np.array_equal(getattr(chk, n).cpu().numpy(), getattr(ost, n)), (it, n)
+    so = (ost.height + ost.sediment)[inter].ravel()
+    sd = (dst.height + dst.sediment)[dst.interior].cpu().numpy().ravel()
+    ho, eo = _hyps_and_eroded(s0, so)
+    hd, ed = _hyps_and_eroded(s0, sd)
+    gap = float(np.abs(np.cumsum(ho) / ho.sum() - np.cumsum(hd) / hd.sum()).max())
+    ero = (ed - eo) / eo
+    dh = (sd - so) * float(ost.height_unit_m)
+    print(f"window {w}: N={dst.N}, {P} droplets in {dst.fast_chunks} chunks, lifetime {p.erosion.max_steps}: "
+          f"fast mode hypsometry gap {100 * gap:.3f} %, eroded volume {100 * ero:+.3f} %, |dh| rms "
+          f"{np.sqrt(np.mean(dh ** 2)):.3f} m max {np.abs(dh).max():.2f} m", flush=True)
+    assert gap <= 0.12 and 0.3 <= ero <= 0.75

@@ -15,8 +15,10 @@ record and the chunk's 4-tuple.  Checked two ways on seeded inputs:
   reference's rules (tests/apply_cases.py): both device applies give exactly
   those numbers.
 
-Error path: a sweep limit too small to converge is reported as an error and
-no height changes (the kernels still exit normally).
+Error path: standalone, a sweep limit too small to converge is reported as
+an error and nothing is applied (the kernels still exit normally); in the
+chunk loop such a chunk, and any chunk with lake parks, goes to the serial
+apply on the device, with the same results.
 """
 from __future__ import annotations
 
@@ -111,19 +113,82 @@ def test_earth_first_iteration_matches_oracle(oracle, hiplib, cuda, N):
     assert np.array_equal(dst.samp.cpu().numpy(), ost.samp)
 
 
-def test_sweep_limit_is_an_error_and_applies_nothing(cuda):
-    N = 64
+def _earth(cuda, N=64, **erosion):
     grid, f = stubs.earth_globe(N, 2, n_blobs=16)
-    p = stubs.workload_params(N, 2, max_steps=128)
+    p = stubs.workload_params(N, 2, max_steps=128, **erosion)
     args = (grid, f["bedrock_m"], f["hardness"], f["precip"], f["evap"], f["uplift_m"], p.erosion)
-    st = maps.ErosionState.from_grid(*args, device=cuda)
-    st.check_max_sweeps = 1
-    h0 = st.height.clone()
-    with pytest.raises(RuntimeError, match="did not converge"):
-        maps.run_iteration(st, p, 0)
-    # from the first chunk that needs a second sweep on, no chunk is applied;
-    # every earlier one converged on its first sweep, i.e. changed no height
-    assert torch.equal(st.height, h0)
+    return p, (lambda: maps.ErosionState.from_grid(*args, device=cuda))
+
+
+def test_sweep_limit_alone_is_an_error_and_applies_nothing(cuda):
+    """The standalone parallel apply with a sweep limit too small: status bit
+    2, and no state array changes (the kernels still exit normally)."""
+    p, mk = _earth(cuda)
+    st = mk()
+    from test_zz_no_alloc import _chunk_lists
+
+    cl, cap = _chunk_lists(st, p, 0, 2048)
+    ep = p.erosion
+    out4 = torch.zeros(4, dtype=torch.float64, device=cuda)
+    status = torch.zeros(4, dtype=torch.int32, device=cuda)
+    work = torch.empty(pk.ordered_workspace_bytes(2048, cap, 1), dtype=torch.uint8, device=cuda)
+    before = [t.clone() for t in (st.acc, st.pending, st.samp, st.disch_track, st.mom_track)]
+    unit = st.height_unit_m
+    pk.apply_changes_ordered(cl["cell"], cl["delta"], cl["vol"], cl["mom"], cl["count"], cap, st.height, st.sediment,
+                             st.acc, st.pending, st.samp, st.disch_track, st.mom_track, st.mask,
+                             maps.cell_units(ep, "iter_erode", unit), maps.slope_limited_deposit(ep, unit),
+                             float(ep.fan_slope), False, maps.cell_units(ep, "dep_floor_m", unit),
+                             float(ep.offshore_writeoff), None, False, out4, status, work, 1, 1)
+    torch.cuda.synchronize()
+    assert int(status[0]) & 2
+    assert "did not converge" in pk.ordered_status_error(status.cpu().numpy())
+    for a, b in zip(before, (st.acc, st.pending, st.samp, st.disch_track, st.mom_track)):
+        assert torch.equal(a, b)
+
+
+def test_sweep_limit_in_the_chunk_loop_falls_back_to_the_serial_apply(cuda):
+    """In the chunk loop a chunk that does not converge within its sweeps is
+    applied by the serial apply: with a limit of one sweep every chunk but
+    the trivial ones goes that way, and the iteration is the unlimited one
+    bit for bit."""
+    p, mk = _earth(cuda)
+    a, b = mk(), mk()
+    a.check_max_sweeps = 1
+    for it in range(2):
+        sa = maps.step(a, p, it)
+        sb = maps.step(b, p, it)
+        print(f"step {it}: serial chunks {sa['serial_chunks']} / {sb['serial_chunks']}", flush=True)
+        assert sa["serial_chunks"] > 0 and sb["serial_chunks"] == 0
+        _same_stats(sa, sb)
+        _same_state(a, b)
+
+
+def test_lake_parks_fall_back_to_the_serial_apply(cuda):
+    """Lake parks (volume -3 entries: a load a lake keeps, particle.py:681-700,
+    912-933) are outside the parallel apply; the chunk loop hands every chunk
+    holding one to the serial apply on the device.  Lakes laid on the
+    epsilon-filled route of an earth globe with the lake balance's parking on:
+    the one-call loop and the serial apply agree bit for bit, lakes included."""
+    p, mk = _earth(cuda, lake_fill=True, lake_balance=True)
+    a, b = mk(), mk()
+    b.check_apply = "serial"
+    for st in (a, b):
+        st.refresh_route(0.05 / st.height_unit_m)
+        lake = ((st.route - (st.height + st.sediment)) > 1e-4) & (st.mask == pk.MASK_ACTIVE)
+        st.lake_flag = torch.zeros(st.height.shape, dtype=torch.uint8, device=cuda)
+        st.lake_flag[lake] = 2
+        st.lake_id = torch.full(st.height.shape, -1, dtype=torch.int32, device=cuda)
+        st.lake_id[lake] = 0
+        st.lake_room = torch.full((1,), 1e3, dtype=torch.float64, device=cuda)
+    assert int(a.lake_flag.sum()) > 0
+    for it in range(2):
+        sa = maps.run_iteration(a, p, it)
+        sb = maps.run_iteration(b, p, it)
+        print(f"iteration {it}: serial chunks {sa['serial_chunks']}, lake load {float(a.lake_load.sum()):.4g}", flush=True)
+        _same_stats(sa, sb)
+        _same_state(a, b)
+        assert torch.equal(a.lake_load, b.lake_load) and torch.equal(a.lake_room, b.lake_room)
+    assert sa["serial_chunks"] > 0 and float(a.lake_load.sum()) > 0.0
 
 
 @pytest.mark.parametrize("apply", ["serial", "ordered"])
