@@ -210,24 +210,42 @@ def test_isostasy_stable_on_white_noise(oracle, cuda):
     assert float(np.abs(dh).max()) <= 0.8 * float(np.abs(acc[ost.interior]).max()) * 1.05
 
 
-def test_steps_with_side_passes_stay_close(oracle, cuda):
+def test_steps_with_side_passes_stay_close(oracle, hiplib, cuda):
     """Three check-mode steps with thermal, creep, slope limits and isostasy
-    every 2 iterations against the oracle's steps."""
+    every 2 iterations against the oracle's steps (the oracle on the device's
+    correctly rounded tan / atan / sincos).  Before the first rebound the
+    steps are bit for bit; the rebound's norms and mean are device
+    reductions in another order than numpy's, so after it the surfaces
+    differ by rounding, and a droplet at a bifurcation of its path may then
+    walk another way: the third step is bounded by the share of cells that
+    moved beyond rounding."""
     g, f = _relief_globe(N=32, seed=7)
     p = _params(thermal_rate=0.5, creep_rate=0.05, thermal_max=60.0, isostasy=0.8, flexure_km=0.2, isostasy_every=2,
                 slope_limit_erode=0.05, slope_limit_deposit=0.5)
     ost, dst = _pair(oracle, cuda, g, f, p.erosion)
-    for it in range(3):
-        print(f"side-pass globe step {it}", flush=True)
-        so = oracle.step(ost, p, it)
-        sd = maps.step(dst, p, it)
-        assert sd["particles"] == so["particles"]
-        assert ("isostasy" in sd) == ("isostasy" in so) == (it == 1)
-    h_o = ost.height + ost.sediment
-    h_d = (dst.height + dst.sediment).cpu().numpy()
     inter = ost.interior
-    scale = np.abs(h_o[inter]).max()
-    assert np.abs(h_d[inter] - h_o[inter]).max() <= 1e-6 * scale
+    gaps = []
+    oracle.set_libm(*oracle.device_libm(hiplib))
+    try:
+        for it in range(3):
+            so = oracle.step(ost, p, it)
+            sd = maps.step(dst, p, it)
+            assert sd["particles"] == so["particles"]
+            assert ("isostasy" in sd) == ("isostasy" in so) == (it == 1)
+            h_o = (ost.height + ost.sediment)[inter]
+            h_d = (dst.height + dst.sediment).cpu().numpy()[inter]
+            scale = np.abs(h_o).max()
+            d = np.abs(h_d - h_o)
+            gaps.append((float(d.max() / scale), float((d > 1e-6 * scale).mean()),
+                         abs(float(h_d.mean() - h_o.mean())) / scale))
+            print(f"side-pass globe step {it}: max gap {gaps[-1][0]:.3g} x peak, cells beyond 1e-6 "
+                  f"{100 * gaps[-1][1]:.3f} %, mean gap {gaps[-1][2]:.3g}", flush=True)
+            if it == 0:
+                assert np.array_equal(h_d, h_o)
+    finally:
+        oracle.set_libm()
+    assert gaps[1][0] <= 1e-9
+    assert gaps[2][1] <= 0.01 and gaps[2][2] <= 1e-6
 
 
 def test_slope_limit_check_mode_bit_exact(oracle, cuda):
