@@ -52,8 +52,9 @@ typedef struct pg_trace_params {
 
 /* Message of the last failed call on this thread ("" if none). */
 const char* pg_last_error(void);
-/* ABI version of this library: 5 (bumped whenever a struct layout or a
- * signature changes; 2 added pg_fast_params.erode_cap, 3 its tape). */
+/* ABI version of this library: 6 (bumped whenever a struct layout or a
+ * signature changes; 2 added pg_fast_params.erode_cap, 3 its tape, 6 the
+ * brush). */
 int pg_abi_version(void);
 
 /* ---- geometry, host side (bake/globe/cubesphere.py:70-220) ------------- */
@@ -394,6 +395,16 @@ typedef struct pg_fast_params {
      * disc_saturation_cells / momentum_saturation_cells > 0, in cells of upstream rain, computed on the
      * device from the iteration's spawn (maps.py:911-921; inflow_volume = ErosionState.inflow_volume) */
     double disc_saturation_cells, momentum_saturation_cells, inflow_volume;
+    /* The erosion brush (no counterpart in the reference, whose droplets change one cell per step,
+     * particle.py:660-667): radius r >= 1 spreads each chunk's per-cell net height change over the
+     * offsets |o| < r, weighted max(0, r - |o|), before the per-cell resolve (pg_brush_spread's rule);
+     * radius 1 is the identity, 0 = off (the default).  brush_owner: DEVICE (F, NE, NE) int64 owner table
+     * of the globe (bake/globe/cubesphere.py:665-686) through which halo shares reach their interior
+     * cells; NULL on a window.  brush_tile: the stencil's LDS tile, 0 = 64 x 4, 1 = 32 x 8, 2 = 16 x 16
+     * (same results).  pg_batch_iterate: every window's radius and tile are the first's, no owner. */
+    int64_t brush_radius;
+    const int64_t* brush_owner;
+    int64_t brush_tile;
 } pg_fast_params;
 
 /* Bytes of the move tape for n_particles rain droplets on F faces of NE x NE
@@ -468,6 +479,31 @@ int pg_fast_trace_time(double* ms_total, int64_t* launches);
  * launches (trace pass 0), ms[1] / launches[1] the applying pass (the
  * replay, or the traced applying pass without a tape).  Same reset. */
 int pg_fast_trace_time_passes(double* ms, int64_t* launches);
+
+/* The brush launches timed the same way (pg_fast_params.time_trace): total
+ * ms and count since the last call; every timed launch read back so far is
+ * kept per kind, so this and pg_fast_trace_time_passes may be called in
+ * either order. */
+int pg_fast_brush_time(double* ms, int64_t* launches);
+
+/* ---- the erosion brush (no counterpart in the reference: its droplets
+ * change one cell per step, bake/globe/erosion/particle.py:660-667) ------ */
+/* The integer weights of a brush radius (0..4): (2A+1)^2 shares in units of
+ * 2^-16, row-major over the offsets (dy, dx) in [-A, A]^2 with A = radius - 1
+ * (A = 0 for radius 0 and 1), the centre's 0.  Returns A, or -1 when n is too
+ * small or the radius out of range. */
+int pg_brush_weights(int64_t radius, int32_t* w, int64_t n);
+/* dst = src (int64 fixed-point amounts per extended cell, DEVICE, (F, NE,
+ * NE)) spread over the brush footprint: every non-zero src[s] gives
+ * sign(d) floor(|d| W_o / 2^16) to each valid target s + o (o != 0) in its
+ * face's extended array and keeps the rest; a target is valid when it is an
+ * active cell (mask 1) or a halo cell whose owner (owner table, DEVICE int64
+ * (F, NE, NE); NULL = identity, a window) is active, and a halo cell's
+ * shares are then added to its owner.  sum(dst) = sum(src) exactly.
+ * valid: DEVICE scratch of F * NE * NE bytes (the target flags).  tile: 0,
+ * 1, 2 as pg_fast_params.brush_tile. */
+int pg_brush_spread(const int64_t* src, int64_t* dst, const uint8_t* mask, const int64_t* owner, int64_t F,
+                    int64_t NE, int64_t radius, int64_t tile, uint8_t* valid, pg_stream stream);
 
 /* ---- batched windows (no counterpart in the reference, which erodes its
  * zoom tiles one process each: bake/globe/zoom/bake.py:777-923) ----------

@@ -21,6 +21,14 @@ window's lakes without the evaporation balance (maps.py:629-672).  The
 remaining side passes (sea mask, the planet's lake balance, glacial, ice,
 strata) raise NotImplementedError here.
 
+One function has no counterpart in the reference: ``brush_spread``, the
+fast mode's erosion brush (the reference changes one cell per step,
+particle.py:660-667), restated here as a plain scatter through the owner
+table and pinned by closed forms in tests/test_oracle_brush.py (a lone
+source's footprint is the weight table, Lague's max(0, r - |o|) normalised;
+the sum over the grid is preserved; shares that have no target stay with
+their source; halo shares land on the owner).
+
 Parity pins: the reference's own invariants (mass conservation on a closed
 window, the offshore split, the change-list layout, the stockpile bound,
 uplift mean-free, datum held) are ported in tests/test_oracle_*.py.  The
@@ -911,3 +919,70 @@ def step(state, params, iteration_key, **kw):
     st["seconds_particles"] = t1 - t0
     st["seconds_total"] = time.time() - t0
     return st
+
+
+# ---------------------------------------------------------------------------
+# the erosion brush (no counterpart in the reference)
+# ---------------------------------------------------------------------------
+def brush_weights(radius: int) -> np.ndarray:
+    """Integer shares (units of 2^-16) of the offsets |o| < radius, weighted
+    max(0, radius - |o|) and normalised to 2^16, on the (2A+1)^2 block with
+    A = radius - 1 (A = 0 for radius 0 and 1); the centre's share is 0: it
+    keeps the remainder.  The weight total is summed row-major, one term at
+    a time."""
+    A = int(radius) - 1 if int(radius) > 1 else 0
+    K = 2 * A + 1
+    wt = np.zeros((K, K))
+    total = 0.0
+    for dy in range(-A, A + 1):
+        for dx in range(-A, A + 1):
+            w = float(radius) - math.sqrt(float(dx * dx + dy * dy))
+            wt[dy + A, dx + A] = w if w > 0.0 else 0.0
+            total += w if w > 0.0 else 0.0
+    out = np.zeros((K, K), np.int64)
+    for dy in range(K):
+        for dx in range(K):
+            if (dy, dx) != (A, A) and total > 0.0:
+                out[dy, dx] = int(math.floor(65536.0 * wt[dy, dx] / total + 0.5))
+    return out
+
+
+def brush_share(d: int, w: int) -> int:
+    """sign(d) floor(|d| w / 2^16) in exact integer arithmetic."""
+    m = -d if d < 0 else d
+    s = (m * w) >> 16
+    return -s if d < 0 else s
+
+
+def brush_spread(src: np.ndarray, mask: np.ndarray, owner: np.ndarray | None, radius: int) -> np.ndarray:
+    """Each non-zero src[s] (int64, (F, NE, NE)) gives brush_share to every
+    valid target s + o, o != 0, inside its face's extended array and keeps
+    the rest; a target is valid when it is active (mask 1) or its owner is;
+    a target that is not its own owner passes its share to the owner (so
+    does a source's remainder).  A scatter over the sources, one at a
+    time."""
+    W = brush_weights(radius)
+    A = (W.shape[0] - 1) // 2
+    F, NE = src.shape[0], src.shape[1]
+    own = np.arange(src.size).reshape(src.shape) if owner is None else np.asarray(owner)
+    flat_mask = mask.reshape(-1)
+    out = np.zeros(src.size, dtype=object)
+    for f, i, j in zip(*np.nonzero(src)):
+        d = int(src[f, i, j])
+        given = 0
+        for dy in range(-A, A + 1):
+            for dx in range(-A, A + 1):
+                if dy == 0 and dx == 0:
+                    continue
+                ti, tj = i + dy, j + dx
+                if not (0 <= ti < NE and 0 <= tj < NE):
+                    continue
+                o = int(own[f, ti, tj])
+                if flat_mask[o] != MASK_ACTIVE:
+                    continue
+                sh = brush_share(d, int(W[dy + A, dx + A]))
+                out[o] += sh
+                given += sh
+        keep = int(own[f, i, j])
+        out[keep if flat_mask[keep] == MASK_ACTIVE else (f * NE + i) * NE + j] += d - given
+    return np.array([int(v) for v in out], dtype=np.int64).reshape(src.shape)

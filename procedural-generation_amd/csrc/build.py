@@ -25,7 +25,7 @@ INCLUDE = os.path.join(ROOT, "include")
 OUT_DIR = os.path.join(os.path.dirname(HERE), "globe_hip")
 LIB = os.path.join(OUT_DIR, "liberode_hip.so")
 BUILD_DIR = os.path.join(HERE, "_build")
-SOURCES = ["pg_api.hip", "pg_check.hip", "pg_ordered.hip", "pg_fields.hip", "pg_fast.hip", "pg_grad.hip", "pg_wasting.hip", "pg_route.hip", "pg_sea.hip"]
+SOURCES = ["pg_api.hip", "pg_check.hip", "pg_ordered.hip", "pg_fields.hip", "pg_fast.hip", "pg_grad.hip", "pg_wasting.hip", "pg_route.hip", "pg_sea.hip", "pg_brush.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-ffp-contract=off",
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",

@@ -1273,13 +1273,14 @@ __global__ void erosion_scale2_kernel(int64_t n, const double* height, const dou
 __global__ void resolve_kernel(int64_t n, const double* height, const double* sediment, const double* base, ll* chunk_fx,
                                ll* acc_fx, ll* pend_fx, float4* cell4, float2* dyn, double iter_deposit,
                                double dep_floor, double writeoff, double hscale, ll* stats, ll* ero_fx, ull* floor_key,
-                               float2* rte, ll* fin_fx) {
+                               float2* rte, ll* fin_fx, ll* zero_also) {
     const double inv = 1.0 / hscale;
     ll clamped = 0, to_pend = 0, lost_off = 0;
     GRID_STRIDE(c, n) {
         ero_fx[c] = 0;
         fin_fx[c] = 0;
         floor_key[c] = ~0ull;
+        if (zero_also) zero_also[c] = 0;  // the brushed chunk's unbrushed change
         const ll net = chunk_fx[c];
         if (net == 0) continue;
         chunk_fx[c] = 0;
@@ -1490,6 +1491,8 @@ int layout_of(int64_t F, int64_t NE, int64_t P, Layout* L) {
     L->rank = take(4 * n);
     L->comp_idx = take(4 * n);
     L->comp_cdf = take(8 * n);
+    L->bbuf = take(8 * n);
+    L->bval = take(n);
     L->draws = take(8 * 3 * p);
     L->u_sorted = take(8 * p);
     L->d_cell = take(4 * nd);
@@ -1824,6 +1827,17 @@ int fast_particles(const pg_fast_fields* fl, const pg_trace_params* prm, const p
         k.tape_last = reinterpret_cast<int*>(k.tape_len + waves);
         k.tape_waves = waves;
     }
+    // the brush (radius >= 1; radius 1 is the identity): its weights and the per-cell target flags
+    const bool brushed = fp->brush_radius >= 1;
+    brush::Weights BW;
+    ll* bbuf = at<ll>(work, L.bbuf);
+    uint8_t* bval = at<uint8_t>(work, L.bval);
+    if (brushed) {
+        if (brush::weights_of(fp->brush_radius, &BW)) return -1;
+        PG_REQUIRE((fp->brush_owner != nullptr) == (fl->spherical != 0),
+                   "fast mode: a brush needs the owner table on the globe, and none on a window");
+        if (brush::enqueue_valid(n, fl->mask, fp->brush_owner, bval, s)) return -1;
+    }
     for (int c = 0; c < chunks; c++) {
         k.chunk = c;
         // pass 0: erosion census on the frozen terrain (and, with a tape, every move recorded)
@@ -1843,9 +1857,19 @@ int fast_particles(const pg_fast_fields* fl, const pg_trace_params* prm, const p
         if (k.tape) hipLaunchKernelGGL(replay_kernel<false>, dim3(rgrid), dim3(TRACE_TPB), 0, s, k);
         else launch_trace<false, true>(fp->variant, grid, s, k);
         if (fp->time_trace && timer_record(s, false, k.pass)) return -1;
+        // the brush: the chunk's net change spread over each cell's footprint before the resolve
+        ll* net = k.chunk_fx;
+        if (brushed) {
+            if (fp->time_trace && timer_record(s, true, 2)) return -1;
+            if (brush::enqueue_spread(fl->F, NE, BW, fp->brush_tile, k.chunk_fx, bval, fp->brush_owner, bbuf, s))
+                return -1;
+            if (fp->time_trace && timer_record(s, false, 2)) return -1;
+            net = bbuf;
+        }
         hipLaunchKernelGGL(resolve_kernel, dim3(grid_for(n)), dim3(TPB), 0, s, n, fl->height, fl->sediment, fl->base,
-                           k.chunk_fx, at<ll>(work, L.acc_fx), pend_fx, cell4, dyn, fp->iter_deposit, prm->dep_floor,
-                           fp->offshore_writeoff, hscale, st, k.ero_fx, k.floor_key, k.rte, k.fin_fx);
+                           net, at<ll>(work, L.acc_fx), pend_fx, cell4, dyn, fp->iter_deposit, prm->dep_floor,
+                           fp->offshore_writeoff, hscale, st, k.ero_fx, k.floor_key, k.rte, k.fin_fx,
+                           brushed ? k.chunk_fx : nullptr);
     }
     PG_LAUNCHED("trace/resolve");
     return finish_iteration(fl, fp, work, c);
@@ -1886,20 +1910,46 @@ extern "C" int pg_fast_iterate(const pg_fast_fields* fl, const pg_trace_params* 
     return fast_particles(fl, prm, fp, work, c);
 }
 
-extern "C" int pg_fast_trace_time_passes(double* ms, int64_t* launches) {
-    clear_error();
-    PG_REQUIRE(ms && launches, "pg_fast_trace_time_passes: NULL argument");
-    ms[0] = ms[1] = 0.0;
-    launches[0] = launches[1] = 0;
+namespace {
+// timed launches read back so far, per kind: census, applying pass, brush
+double g_bucket_ms[3];
+int64_t g_bucket_n[3];
+
+int drain_timer() {
     for (int i = 0; i < g_timer.recorded; i++) {
         PG_HIP(hipEventSynchronize(g_timer.stop[i]));
         float t = 0.0f;
         PG_HIP(hipEventElapsedTime(&t, g_timer.start[i], g_timer.stop[i]));
-        const int p = g_timer.pass[i] ? 1 : 0;
-        ms[p] += t;
-        launches[p] += 1;
+        const int p = g_timer.pass[i] >= 2 ? 2 : g_timer.pass[i] ? 1 : 0;
+        g_bucket_ms[p] += t;
+        g_bucket_n[p] += 1;
     }
     g_timer.recorded = 0;
+    return 0;
+}
+}  // namespace
+
+extern "C" int pg_fast_trace_time_passes(double* ms, int64_t* launches) {
+    clear_error();
+    PG_REQUIRE(ms && launches, "pg_fast_trace_time_passes: NULL argument");
+    if (drain_timer()) return -1;
+    for (int p = 0; p < 2; p++) {
+        ms[p] = g_bucket_ms[p];
+        launches[p] = g_bucket_n[p];
+        g_bucket_ms[p] = 0.0;
+        g_bucket_n[p] = 0;
+    }
+    return 0;
+}
+
+extern "C" int pg_fast_brush_time(double* ms, int64_t* launches) {
+    clear_error();
+    PG_REQUIRE(ms && launches, "pg_fast_brush_time: NULL argument");
+    if (drain_timer()) return -1;
+    *ms = g_bucket_ms[2];
+    *launches = g_bucket_n[2];
+    g_bucket_ms[2] = 0.0;
+    g_bucket_n[2] = 0;
     return 0;
 }
 
@@ -1978,6 +2028,8 @@ int batch_layout(int64_t W, const int64_t* NE, const int64_t* P, BLayout* B) {
     L.rank = take(4 * n);
     L.comp_idx = take(4 * n);
     L.comp_cdf = take(8 * n);
+    L.bbuf = take(8 * n);
+    L.bval = take(n);
     L.draws = take(8 * 3 * p);
     L.u_sorted = take(8 * p);
     L.d_cell = take(4 * nd);
@@ -2023,6 +2075,8 @@ void window_layout(const BLayout& B, int64_t coff, int64_t poff, int64_t doff, i
     Lw->rank = L.rank + 4 * c;
     Lw->comp_idx = L.comp_idx + 4 * c;
     Lw->comp_cdf = L.comp_cdf + 8 * c;
+    Lw->bbuf = L.bbuf + 8 * c;
+    Lw->bval = L.bval + c;
     Lw->draws = L.draws + 8 * 3 * (size_t)poff;
     Lw->u_sorted = L.u_sorted + 8 * (size_t)poff;
     Lw->d_cell = L.d_cell + 4 * (size_t)doff;
@@ -2269,6 +2323,18 @@ extern "C" int pg_batch_iterate(const pg_fast_fields* wins, int64_t n_win, const
         k.tape_waves = waves;
     }
     const pg_fast_fields& a = wins[0];  // arena bases
+    // the brush: every window's radius and tile are the first's
+    const bool brushed = f0.brush_radius >= 1;
+    brush::Weights BW;
+    ll* bbuf = at<ll>(work, L.bbuf);
+    uint8_t* bval = at<uint8_t>(work, L.bval);
+    for (int64_t w = 0; w < n_win; w++)
+        PG_REQUIRE(fps[w].brush_radius == f0.brush_radius && fps[w].brush_owner == nullptr,
+                   "pg_batch_iterate: windows differ in brush radius, or have an owner table");
+    if (brushed) {
+        if (brush::weights_of(f0.brush_radius, &BW)) return -1;
+        if (brush::enqueue_valid(n, a.mask, nullptr, bval, s)) return -1;
+    }
     for (int c = 0; c < chunks; c++) {
         k.chunk = c;
         k.pass = 0;
@@ -2286,10 +2352,21 @@ extern "C" int pg_batch_iterate(const pg_fast_fields* wins, int64_t n_win, const
         if (k.tape) hipLaunchKernelGGL(replay_kernel<true>, dim3(rgrid), dim3(TRACE_TPB), 0, s, k);
         else launch_trace<true, true>(f0.variant, grid, s, k);
         if (f0.time_trace && timer_record(s, false, k.pass)) return -1;
+        ll* net = k.chunk_fx;
+        if (brushed) {
+            // each window's own footprints: a window is a single face with no owner table
+            if (f0.time_trace && timer_record(s, true, 2)) return -1;
+            for (int64_t w = 0; w < n_win; w++)
+                if (brush::enqueue_spread(1, NE[w], BW, f0.brush_tile, k.chunk_fx + coff[w], bval + coff[w], nullptr,
+                                          bbuf + coff[w], s))
+                    return -1;
+            if (f0.time_trace && timer_record(s, false, 2)) return -1;
+            net = bbuf;
+        }
         hipLaunchKernelGGL(resolve_kernel, dim3(grid_for(n)), dim3(TPB), 0, s, n, a.height, a.sediment, a.base,
-                           k.chunk_fx, at<ll>(work, L.acc_fx), at<ll>(work, L.pend_fx), at<float4>(work, L.cell4),
+                           net, at<ll>(work, L.acc_fx), at<ll>(work, L.pend_fx), at<float4>(work, L.cell4),
                            at<float2>(work, L.dyn), f0.iter_deposit, prm->dep_floor, f0.offshore_writeoff, hscale,
-                           (ll*)stats, k.ero_fx, k.floor_key, k.rte, k.fin_fx);
+                           (ll*)stats, k.ero_fx, k.floor_key, k.rte, k.fin_fx, brushed ? k.chunk_fx : nullptr);
     }
     PG_LAUNCHED("batch trace/resolve");
     hipLaunchKernelGGL(finish_kernel, dim3(grid_for(n)), dim3(TPB), 0, s, n, a.height, a.sediment, a.discharge,

@@ -41,7 +41,7 @@ struct Scalars {
 // ---------------------------------------------------------------------------
 struct Layout {
     size_t cell4, geo, dyn, rte, chunk_fx, acc_fx, dis_fx, mom_fx, pend_fx, ero_fx, fin_fx, floor_key, rscale, r1, wq, cdf_fx, flag,
-        rank, comp_idx, comp_cdf,
+        rank, comp_idx, comp_cdf, bbuf, bval,
         draws, u_sorted, d_cell, d_fx, d_fy, d_load, d_sa, d_sb, scalars, temp, total;
     size_t temp_bytes;
 };
@@ -290,4 +290,37 @@ __device__ __forceinline__ int steer_move(const K& k, int N, const Stencil& S, c
 }
 
 }  // namespace fastmode
+}  // namespace pg
+
+// ---------------------------------------------------------------------------
+// the erosion brush (pg_brush.hip): a chunk's per-cell net change spread over
+// a footprint, halo shares folded onto their owners
+// ---------------------------------------------------------------------------
+namespace pg {
+namespace brush {
+
+using fastmode::ll;
+using fastmode::ull;
+
+constexpr int MAX_APRON = 3;  // radius - 1: radius at most 4
+constexpr int MAX_TAPS = (2 * MAX_APRON + 1) * (2 * MAX_APRON + 1);
+constexpr int N_TILES = 3;    // LDS tiles 64 x 4, 32 x 8, 16 x 16
+
+// integer shares in units of 2^-16 of the footprint's offsets, row-major over
+// (2 apron + 1)^2, the centre's 0 (it keeps the remainder)
+struct Weights {
+    int w[MAX_TAPS];
+    int apron;
+};
+
+// the weights of a radius (host)
+int weights_of(int64_t radius, Weights* W);
+// the per-cell target flags from the mask and the owner table (NULL: identity)
+int enqueue_valid(int64_t n, const uint8_t* mask, const int64_t* owner, uint8_t* bval, hipStream_t s);
+// dst = the spread of src (every cell of dst written), halo shares folded onto
+// their owners when an owner table is given
+int enqueue_spread(int64_t F, int64_t NE, const Weights& W, int64_t tile, const ll* src, const uint8_t* bval,
+                   const int64_t* owner, ll* dst, hipStream_t s);
+
+}  // namespace brush
 }  // namespace pg

@@ -43,7 +43,8 @@ class FastParams(ctypes.Structure):
                 ("offshore_writeoff", dbl), ("ema", dbl), ("min_volume_frac", dbl), ("height_bits", i64),
                 ("track_bits", i64), ("grid_blocks", i64), ("time_trace", i64), ("rng_dev", vp), ("variant", i64),
                 ("erode_cap", vp), ("tape", vp), ("tape_bytes", i64), ("route", vp), ("lake_flag", vp),
-                ("disc_saturation_cells", dbl), ("momentum_saturation_cells", dbl), ("inflow_volume", dbl)]
+                ("disc_saturation_cells", dbl), ("momentum_saturation_cells", dbl), ("inflow_volume", dbl),
+                ("brush_radius", i64), ("brush_owner", vp), ("brush_tile", i64)]
 
 
 class GradArgs(ctypes.Structure):
@@ -111,6 +112,9 @@ SIGNATURES = [
     ("pg_fast_layout", i32, [i64, i64, i64, ctypes.POINTER(ctypes.c_int64), i32]),
     ("pg_fast_trace_time", i32, [pd, ctypes.POINTER(ctypes.c_int64)]),
     ("pg_fast_trace_time_passes", i32, [pd, ctypes.POINTER(ctypes.c_int64)]),
+    ("pg_fast_brush_time", i32, [pd, ctypes.POINTER(ctypes.c_int64)]),
+    ("pg_brush_weights", i32, [i64, ctypes.POINTER(ctypes.c_int32), i64]),
+    ("pg_brush_spread", i32, [vp, vp, vp, vp, i64, i64, i64, i64, vp, vp]),
     ("pg_sea_work_bytes", i64, [i64]),
     ("pg_refresh_base", i32, [vp, vp, vp, vp, dbl, i64, i64, vp, vp, i64, pd, vp]),
     ("pg_batch_workspace_bytes", i64, [i64, ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int64)]),

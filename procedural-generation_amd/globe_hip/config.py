@@ -157,6 +157,10 @@ class ErosionParams:
     pit_steps: int = 16
     chunk: int = 2048
     backend: str = "cpu"
+    #: the fast mode's erosion brush (no field of the reference, whose droplets change one cell per step):
+    #: each chunk's height changes spread over the offsets |o| < brush_radius cells, weighted
+    #: max(0, brush_radius - |o|); 0 = off, 1 = the cell alone
+    brush_radius: int = 0
 
 
 #: parameters declared in metres (bake/globe/config.py:603-605)

@@ -99,6 +99,8 @@ class GradRun:
         self.rng_stage = rng_stage
         self.replay_of = replay_of
         self.feedback = bool(feedback)
+        if int(getattr(ep, "brush_radius", 0)) > 1:
+            raise NotImplementedError("the differentiable mode has no brush")
         dev = state.device
         n_cells = int(np.count_nonzero(state._mask_host[state.interior] == pk.MASK_ACTIVE))
         if n_particles is None:

@@ -142,6 +142,7 @@ class WindowBatch:
         self.time_trace = False
         self.rng_dev = None
         self.use_tape = True  # the census records every move and the applying pass replays it (same results)
+        self.brush_tile = 0  # the brush stencil's LDS tile (same results)
         self.tape = None
         self.iteration = 0
         # the slope limit's per-cell cap over the arena: every window's side buffer is its view
@@ -182,7 +183,8 @@ class WindowBatch:
                                 lake_flag=None if st.route is None or st.lake_flag is None else st.lake_flag.data_ptr(),
                                 disc_saturation_cells=float(ep.disc_saturation_cells),
                                 momentum_saturation_cells=float(ep.momentum_saturation_cells),
-                                inflow_volume=float(st.inflow_volume))
+                                inflow_volume=float(st.inflow_volume),
+                                brush_radius=int(getattr(ep, "brush_radius", 0)), brush_tile=int(self.brush_tile))
         return out
 
     def flood_due(self) -> bool:

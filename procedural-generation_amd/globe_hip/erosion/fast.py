@@ -148,6 +148,8 @@ def run_iteration(state, params, iteration_key, n_particles=None, particles_per_
     n_chunks = int(chunks if chunks is not None else getattr(state, "fast_chunks", DEFAULT_CHUNKS))
     tape = ctx.tape_for(state, n_chunks, max_steps, int(getattr(state, "fast_grid_blocks", 0)),
                         int(getattr(state, "fast_variant", 0)))
+    brush = int(getattr(ep, "brush_radius", 0))
+    owner = state.owner_device() if brush >= 1 else None
     slo, shi, ilo, ihi = pcg64_state(rng)
     fp = FastParams(rng_state_lo=slo, rng_state_hi=shi, rng_inc_lo=ilo, rng_inc_hi=ihi, n_particles=int(n_particles),
                     chunks=n_chunks,
@@ -166,7 +168,9 @@ def run_iteration(state, params, iteration_key, n_particles=None, particles_per_
                     lake_flag=None if state.route is None or state.lake_flag is None else state.lake_flag.data_ptr(),
                     disc_saturation_cells=float(ep.disc_saturation_cells),
                     momentum_saturation_cells=float(ep.momentum_saturation_cells),
-                    inflow_volume=float(state.inflow_volume))
+                    inflow_volume=float(state.inflow_volume), brush_radius=brush,
+                    brush_owner=(None if owner is None else owner.data_ptr()),
+                    brush_tile=int(getattr(state, "fast_brush_tile", 0)))
     check(lib().pg_fast_iterate(ctypes.byref(ctx.fields), ctypes.byref(prm), ctypes.byref(fp),
                                 ctypes.c_void_p(ctx.work.data_ptr()), ctx.work.numel(),
                                 ctypes.c_void_p(ctx.stats.data_ptr()), pk._stream()), "pg_fast_iterate")
@@ -211,6 +215,38 @@ def trace_time_passes():
     n = (ctypes.c_int64 * 2)()
     check(lib().pg_fast_trace_time_passes(ms, n), "pg_fast_trace_time_passes")
     return (ms[0], n[0]), (ms[1], n[1])
+
+
+def brush_time():
+    """(ms, launches) of the brush launches recorded since the last call
+    (states with ``fast_time_trace = True``); waits for them."""
+    ms = ctypes.c_double()
+    n = ctypes.c_int64()
+    check(lib().pg_fast_brush_time(ctypes.byref(ms), ctypes.byref(n)), "pg_fast_brush_time")
+    return ms.value, n.value
+
+
+def brush_weights(radius: int) -> np.ndarray:
+    """The library's integer brush weights of a radius, (2A+1, 2A+1) in
+    units of 2^-16 (the centre 0: it keeps the remainder)."""
+    w = (ctypes.c_int32 * 81)()
+    A = lib().pg_brush_weights(int(radius), w, 81)
+    if A < 0:
+        check(-1, "pg_brush_weights")
+    K = 2 * A + 1
+    return np.array(w[:K * K], np.int64).reshape(K, K)
+
+
+def brush_spread(src: torch.Tensor, mask: torch.Tensor, owner: torch.Tensor | None, radius: int, tile: int = 0):
+    """pg_brush_spread on device tensors: src int64 (F, NE, NE) fixed-point
+    amounts, mask uint8, owner int64 or None; returns the spread amounts."""
+    F, NE = int(src.shape[0]), int(src.shape[1])
+    dst = torch.empty_like(src)
+    valid = torch.empty(src.shape, dtype=torch.uint8, device=src.device)
+    check(lib().pg_brush_spread(src.data_ptr(), dst.data_ptr(), mask.data_ptr(),
+                                None if owner is None else owner.data_ptr(), F, NE, int(radius), int(tile),
+                                valid.data_ptr(), pk._stream()), "pg_brush_spread")
+    return dst
 
 
 def _signed(w: int) -> int:

@@ -406,6 +406,17 @@ class ErosionState:
             self._owner = np.ascontiguousarray(own, dtype=np.int64)
         return self._owner
 
+    def owner_device(self) -> torch.Tensor | None:
+        """The owner table on the device (int64, (F, NE, NE)) on the globe;
+        None on a window, whose halo has no owners."""
+        if not self.spherical:
+            return None
+        t = getattr(self, "_owner_dev", None)
+        if t is None:
+            t = _dev(self.owner_table(), self.device)
+            self._owner_dev = t
+        return t
+
     def _flood_tables(self) -> dict:
         """The flood's static cell classes and the extra in-edges of the cells
         a cross-face halo cell maps onto (see pg_flood_*), built once on the
@@ -657,6 +668,9 @@ def run_iteration(state: ErosionState, params, iteration_key, n_particles=None, 
         return fast.run_iteration(state, params, iteration_key, n_particles=n_particles,
                                   particles_per_cell=particles_per_cell, rng_stage=rng_stage, log=log)
     ep = _eparams(params)
+    if int(getattr(ep, "brush_radius", 0)) > 1:
+        raise NotImplementedError("the brush is an extension of the fast mode; the check mode reproduces the "
+                                  "reference, whose droplets change one cell per step")
     key = iteration_key if isinstance(iteration_key, (tuple, list)) else (int(iteration_key),)
     rng = params.rng(rng_stage, *key)
     t0 = time.time()
