@@ -51,6 +51,9 @@ B_MOVE_EXACT = 156.0
 #: the same for the fast mode's particle pass (float32 record, four int64 fixed-point writes instead of the entry)
 B_STEP_FAST = 164.0
 HBM_PEAK_GBS = 8000.0
+#: algorithmic HBM bytes per extended cell of one brush launch pair: the chunk's int64 change in (8), the
+#: target flag in (1), the spread change out (8), the fold's flag read (1); the apron is served from L2
+B_CELL_BRUSH = 18.0
 PROFILE_COUNTERS = os.path.join(ROOT, "profiles", "trace_check_kernel_earth_n1024_counters.md")
 
 
@@ -145,14 +148,22 @@ def exact_run(grid, f, p, steps, warmup):
                 iterations=warmup + steps)
 
 
-def fast_run(grid, f, p, iterations, steps, chunks=16):
+def fast_run(grid, f, p, iterations, steps, chunks=16, brush=0):
     """The fast mode on the same workload for the same iterations; the last
-    `steps` timed."""
+    `steps` timed.  With brush = r > 1 the erosion brush of radius r is on;
+    its LDS tile is picked by the autotuner over the first warm-up
+    iterations."""
+    import copy
+
     import torch
 
+    from globe_hip.erosion import batch as bt
     from globe_hip.erosion import fast, maps
 
     dev = torch.device("cuda:0")
+    if brush:
+        p = copy.deepcopy(p)
+        p.erosion.brush_radius = int(brush)
     args = (grid, f["bedrock_m"], f["hardness"], f["precip"], f["evap"], f["uplift_m"], p.erosion)
     st = maps.ErosionState.from_grid(*args, device=dev, mode="fast")
     st.fast_chunks = chunks
@@ -160,10 +171,26 @@ def fast_run(grid, f, p, iterations, steps, chunks=16):
     maps.start_replay(st, p)
     shift = 0.0
     warm = iterations - steps
+    tuner = None
+    if brush:
+        warm = max(warm, 4)  # the autotuner's grid candidate and three tiles
+        iterations = warm + steps
+        cus = torch.cuda.get_device_properties(0).multi_processor_count
+        tuner = bt.Autotuner(st, candidates=[(4 * cus, 0)], tiles=(0, 1, 2))
     for it in range(warm):
-        shift += float(maps.step(st, p, it)["datum_shift"].item())
+        if tuner is not None and tuner.best is None:
+            box = {}
+            tuner.step(lambda k: box.setdefault("s", maps.step(st, p, k)), it)
+            shift += float(box["s"]["datum_shift"].item())
+        else:
+            shift += float(maps.step(st, p, it)["datum_shift"].item())
+    if tuner is not None:
+        tuner.finish()
+        log(f"fast mode brush {brush}: LDS tile {tuner.best_tile} (ms per brush launch per tile {tuner.tile_times})")
+    st.fast_grid_blocks, st.fast_variant = 0, 0
     torch.cuda.synchronize()
     fast.trace_time()
+    fast.brush_time()
     st.fast_time_trace = True
     raw = []
     t0 = time.perf_counter()
@@ -174,11 +201,24 @@ def fast_run(grid, f, p, iterations, steps, chunks=16):
     torch.cuda.synchronize()
     dt = time.perf_counter() - t0
     trace_ms, launches = fast.trace_time()
+    brush_ms, brush_launches = fast.brush_time()
     dec = [fast.decode_stats(r.cpu().numpy(), 2.0 ** fast.HEIGHT_BITS) for r in raw]
     moves = sum(d["moves"] for d in dec)
     sf, _ = surface_stats(st)
-    log(f"fast mode ({chunks} chunks): {steps} iterations in {dt:.3f}s, {moves} moves")
-    return dict(seconds=dt, moves=moves, trace_ms=trace_ms, launches=launches, s=sf, shift=shift)
+    log(f"fast mode ({chunks} chunks, brush {brush}): {steps} iterations in {dt:.3f}s, {moves} moves")
+    return dict(seconds=dt, moves=moves, trace_ms=trace_ms, launches=launches, s=sf, shift=shift, brush_ms=brush_ms,
+                brush_launches=brush_launches, tile=None if tuner is None else tuner.best_tile,
+                tile_ms=None if tuner is None else tuner.tile_times, cells=int(st.height.numel()),
+                budget_exact=all(_budget_exact(d) for d in dec))
+
+
+def _budget_exact(d):
+    from globe_hip._lib import FS
+
+    r = d["raw_fx"]
+    lhs = (int(r[FS["DEPOSITED_FX"]]) - int(r[FS["ERODED_FX"]]) + int(r[FS["TO_PENDING_FX"]])
+           + int(r[FS["LOST_OFFSHORE_FX"]]) + int(r[FS["EXIT_FX"]]) + int(r[FS["VANISHED_FX"]]))
+    return lhs == int(r[FS["RELEASED_FX"]])
 
 
 def cpu_baseline(grid, f, p, fraction=8):
@@ -422,6 +462,7 @@ def main():
     ap.add_argument("--N", type=int, default=1024)
     ap.add_argument("--chunks", type=int, default=16, help="the fast mode's chunks per iteration")
     ap.add_argument("--max-steps", type=int, default=128)
+    ap.add_argument("--brush", type=int, default=3, help="the fast mode's brush leg: radius (0 = no leg)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fast", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
@@ -486,6 +527,25 @@ def main():
                          "bytes_per_move": B_STEP_FAST}}
         log(f"fast mode vs exact after {r['iterations']} iterations: hypsometry gap {100 * gap:.2f} %, eroded volume "
             f"{100 * ero:+.2f} %")
+    if not a.no_fast and a.brush > 1:
+        import torch
+
+        torch.cuda.empty_cache()
+        br = fast_run(grid, f, p, r["iterations"], a.steps, a.chunks, brush=a.brush)
+        per_launch = br["brush_ms"] / max(br["brush_launches"], 1)
+        bach = B_CELL_BRUSH * br["cells"] / (per_launch * 1e-3) / 1e9 if per_launch > 0 else 0.0
+        out["fast_mode_brush"] = {
+            "value": br["moves"] / br["seconds"], "unit": "droplet-steps/s", "brush_radius": a.brush,
+            "ms_per_step": 1000.0 * br["seconds"] / a.steps, "dtype": "f32", "chunks_per_iteration": a.chunks,
+            "mass_budget_exact": br["budget_exact"], "lds_tile": ["64x4", "32x8", "16x16"][br["tile"]],
+            "autotune_ms_per_brush_launch": {["64x4", "32x8", "16x16"][k]: v for k, v in br["tile_ms"].items()},
+            "brush_share_of_wall": br["brush_ms"] / 1000.0 / br["seconds"],
+            "roofline": {"kernel": "brush_spread_kernel + brush_fold_kernel (one launch pair per chunk)",
+                         "achieved": bach, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": bach / HBM_PEAK_GBS,
+                         "bytes_per_cell": B_CELL_BRUSH, "launch_ms_mean": per_launch,
+                         "launches": br["brush_launches"]}}
+        log(f"brush {a.brush}: {per_launch * 1000:.1f} us per launch pair ({bach:.0f} GB/s), "
+            f"{br['moves'] / br['seconds']:.3g} moves/s")
     if not a.no_adjoint:
         import torch
 

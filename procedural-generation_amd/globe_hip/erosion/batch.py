@@ -313,15 +313,17 @@ class BatchGraph:
 class Autotuner:
     """Picks the persistent trace grid and the trace kernel's register-budget
     variant (0: the kernels' natural budget, 1 / 2: one / two more waves per
-    SIMD) of a batch or of a fast-mode state
-    from real iterations: the first len(candidates) iterations each run one
-    (grid, variant) candidate with the trace launches timed (HIP events on
-    the launch stream) and normalised by the iteration's droplet moves (the
-    iterations differ), then the fastest per move is kept.  Results are
-    independent of both (fixed-point accumulation), so no iteration is
-    wasted."""
+    SIMD) of a batch or of a fast-mode state, and with a brush its stencil's
+    LDS tile (64 x 4, 32 x 8, 16 x 16), from real iterations: the first
+    len(candidates) iterations each run one (grid, variant) candidate with
+    the trace launches timed (HIP events on the launch stream) and
+    normalised by the iteration's droplet moves (the iterations differ);
+    with a brush the next len(tiles) iterations each run one tile at the
+    fastest (grid, variant), timed by the brush launches.  Results are
+    independent of all three (fixed-point accumulation), so no iteration is
+    wasted.  ``best`` is set once every candidate has run."""
 
-    def __init__(self, target, candidates=None):
+    def __init__(self, target, candidates=None, tiles=None):
         self.target = target
         if candidates is None:
             cus = torch.cuda.get_device_properties(0).multi_processor_count
@@ -329,8 +331,15 @@ class Autotuner:
         self.candidates = [(int(c[0]), int(c[1])) if isinstance(c, (tuple, list)) else (int(c), 0)
                            for c in candidates]
         self.candidates = [c for c in self.candidates if c[0] > 0 and c[1] in (0, 1, 2)]
+        if tiles is None:
+            brushed = isinstance(target, WindowBatch) and int(getattr(target.ep, "brush_radius", 0)) > 1
+            tiles = (0, 1, 2) if brushed else ()
+        self.tiles = [int(t) for t in tiles if int(t) in (0, 1, 2)]
         self.times: dict = {}
+        self.tile_times: dict = {}
         self.best = None
+        self.best_tile = None
+        self._gv = None
 
     def _set(self, cand, timed: bool):
         grid, variant = cand
@@ -340,27 +349,56 @@ class Autotuner:
         else:
             t.fast_grid_blocks, t.fast_variant, t.fast_time_trace = grid, variant, timed
 
+    def _set_tile(self, tile):
+        t = self.target
+        if isinstance(t, WindowBatch):
+            t.brush_tile = tile
+        else:
+            t.fast_brush_tile = tile
+
     def step(self, run, iteration_key) -> None:
         """run(iteration_key) performs one iteration of the target."""
         if self.best is not None:
             run(iteration_key)
             return
-        cand = self.candidates[len(self.times)]
-        fast.trace_time()
-        self._set(cand, True)
+        if self._gv is None:
+            cand = self.candidates[len(self.times)]
+            fast.trace_time()
+            self._set(cand, True)
+            run(iteration_key)
+            ms, _ = fast.trace_time()
+            fast.brush_time()
+            t = self.target
+            moves = t.moves() if isinstance(t, WindowBatch) else int(t._fast.stats[FS["MOVES"]])
+            self.times[cand] = ms / max(moves, 1) * 1e6  # ns per move
+            if len(self.times) < len(self.candidates):
+                self._set((0, 0), False)
+                return
+            self._gv = min(self.times, key=self.times.get)
+            if not self.tiles:
+                self.finish()
+            else:
+                self._set(self._gv, False)
+            return
+        tile = self.tiles[len(self.tile_times)]
+        fast.brush_time()
+        self._set(self._gv, True)
+        self._set_tile(tile)
         run(iteration_key)
-        ms, _ = fast.trace_time()
-        t = self.target
-        moves = t.moves() if isinstance(t, WindowBatch) else int(t._fast.stats[FS["MOVES"]])
-        self.times[cand] = ms / max(moves, 1) * 1e6  # ns per move
-        if len(self.times) == len(self.candidates):
+        ms, n = fast.brush_time()
+        fast.trace_time()
+        self.tile_times[tile] = ms / max(n, 1)  # ms per brush launch
+        if len(self.tile_times) == len(self.tiles):
             self.finish()
         else:
-            self._set((0, 0), False)
+            self._set(self._gv, False)
 
     def finish(self):
-        """Keep the fastest candidate measured so far."""
+        """Keep the fastest candidates measured so far."""
         if self.times:
-            self.best = min(self.times, key=self.times.get)
+            self.best = self._gv if self._gv is not None else min(self.times, key=self.times.get)
             self._set(self.best, False)
+        if self.tile_times:
+            self.best_tile = min(self.tile_times, key=self.tile_times.get)
+            self._set_tile(self.best_tile)
         return self.best

@@ -145,3 +145,26 @@ def test_batched_windows_with_a_brush_equal_per_window(cuda):
         for n in ("height", "sediment", "discharge", "momentum", "pending"):
             x, y = getattr(a, n), getattr(b, n)
             assert torch.equal(x, y), (it, n, float((x - y).abs().max()))
+
+
+def test_autotuner_picks_a_brush_tile_and_keeps_results(cuda):
+    def make():
+        wins = stubs.zoom_windows(4, seed=9, side_lo=32, side_hi=64, life_lo=16, life_hi=48, brush_radius=2)
+        return bt.WindowBatch([w for w, _ in wins], [p for _, p in wins], device=cuda, chunks=2)
+
+    a, b = make(), make()
+    cus = torch.cuda.get_device_properties(0).multi_processor_count
+    tuner = bt.Autotuner(a, candidates=[(cus, 0), (2 * cus, 1)])
+    assert tuner.tiles == [0, 1, 2]
+    it = 0
+    while tuner.best is None:
+        tuner.step(a.step, it)
+        b.step(it)
+        it += 1
+    assert it == 5 and tuner.best_tile in (0, 1, 2) and a.brush_tile == tuner.best_tile
+    assert all(v > 0 for v in tuner.tile_times.values())
+    a.step(it)
+    b.step(it)
+    torch.cuda.synchronize()
+    for n in ("height", "sediment", "discharge", "momentum", "pending"):
+        assert torch.equal(getattr(a, n), getattr(b, n)), n
