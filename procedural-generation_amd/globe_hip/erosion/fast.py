@@ -27,8 +27,8 @@ from . import particle as pk
 DEFAULT_CHUNKS = 16
 HEIGHT_BITS = 48
 TRACK_BITS = 32
-#: largest move tape allocated by default: 64 GiB, under a quarter of the part's 288 GB of HBM (a planet
-#: at N_c=8192 would need more: it traces twice instead, or uses more chunks)
+#: largest move tape allocated by default: 64 GiB, under a quarter of the part's 288 GB of HBM, and never
+#: more than half of the device's free memory (a planet at N_c=8192 would need more: it traces twice instead)
 TAPE_MAX_BYTES = 64 << 30
 
 
@@ -78,7 +78,14 @@ class FastContext:
             return None  # too large for this chunking: the applying pass traces again (same results)
         if getattr(self, "tape", None) is None or self.tape.numel() < need:
             self.tape = None
-            self.tape = torch.empty(need, dtype=torch.uint8, device=state.device)
+            # at most half of what the device has free; without a tape the applying pass traces again
+            if need > torch.cuda.mem_get_info(state.device)[0] // 2:
+                return None
+            try:
+                self.tape = torch.empty(need, dtype=torch.uint8, device=state.device)
+            except torch.cuda.OutOfMemoryError:
+                self.tape = None
+                return None
         return self.tape
 
 

@@ -80,6 +80,21 @@ def _dev(a: np.ndarray, device) -> torch.Tensor:
     return torch.from_numpy(np.ascontiguousarray(a)).to(device)
 
 
+_TORCH_DTYPE = {np.float64: torch.float64, np.float32: torch.float32, np.uint8: torch.uint8}
+
+
+def _to_dev(x, dtype, device, divide_by=None) -> torch.Tensor:
+    """A field (numpy array, FaceField or tensor) on the device as ``dtype``,
+    optionally divided by a scalar; a tensor already on the device stays
+    there, a host array is converted on the host first (the same values)."""
+    d = x.data if isinstance(x, FaceField) else x
+    if isinstance(d, torch.Tensor) and d.device.type == "cuda":
+        t = d.detach().to(device=device, dtype=_TORCH_DTYPE[dtype])
+        return (t / divide_by if divide_by is not None else t).contiguous()
+    a = _host(d, dtype)
+    return _dev(a / divide_by if divide_by is not None else a, device)
+
+
 class HaloTables:
     """A grid's halo gather tables resident on the device."""
 
@@ -163,14 +178,8 @@ class ErosionState:
         if dev.type != "cuda":
             raise ValueError("ErosionState lives on the GPU: pass device tensors (or use from_grid / window)")
         z = lambda: torch.zeros((F, NE, NE), dtype=torch.float64, device=dev)
-        if self.disch_track is None:
-            self.disch_track = z()
-        if self.mom_track is None:
-            self.mom_track = torch.zeros((F, NE, NE, 2), dtype=torch.float64, device=dev)
-        if self.samp is None:
-            self.samp = torch.zeros((F, NE, NE, pk.NS), dtype=torch.float32, device=dev)
-        if self.acc is None:
-            self.acc = z()
+        if self.mode == "check":
+            self._check_buffers()
         if self.pending is None:
             self.pending = z()
         if self.lake_load is None:
@@ -199,20 +208,23 @@ class ErosionState:
     # -- constructors --------------------------------------------------------
     @classmethod
     def from_grid(cls, grid, bedrock_m, hardness, precip, evap, uplift_m, eparams=None, mask=None, device="cuda",
-                  mode: str = "check") -> "ErosionState":
+                  mode: str = "check", metric=None, metric_inv=None) -> "ErosionState":
         """Global 6-face state from extended arrays / FaceFields in metres
-        (bake/globe/erosion/maps.py:152-191)."""
+        (bake/globe/erosion/maps.py:152-191).  Inputs may also be tensors
+        already on the device (taken without a host round trip), and the
+        metric tensors may be given (default: the grid's)."""
         unit = height_unit(grid, eparams)
         F, NE = 6, grid.NE
+        dev = torch.device(device)
         m = np.ones((F, NE, NE), np.uint8) if mask is None else _host(mask, np.uint8)
-        z = np.zeros((F, NE, NE))
-        return cls(height=_dev(_host(bedrock_m, np.float64) / unit, device), sediment=_dev(z, device),
-                   discharge=_dev(z, device), momentum=_dev(np.zeros((F, NE, NE, 2)), device),
-                   hardness=_dev(_host(hardness, np.float32), device), precip=_dev(_host(precip, np.float32), device),
-                   evap=_dev(_host(evap, np.float32), device), uplift=_dev(_host(uplift_m, np.float64) / unit, device),
-                   mask=_dev(m, device), metric=_dev(np.asarray(grid.metric, np.float32), device),
-                   metric_inv=_dev(np.asarray(grid.metric_inv, np.float32), device), N=grid.N, H=grid.H,
-                   spherical=True, height_unit_m=unit, grid=grid, mode=mode)
+        zeros = lambda *extra: torch.zeros((F, NE, NE) + extra, dtype=torch.float64, device=dev)
+        return cls(height=_to_dev(bedrock_m, np.float64, dev, unit), sediment=zeros(), discharge=zeros(),
+                   momentum=zeros(2), hardness=_to_dev(hardness, np.float32, dev),
+                   precip=_to_dev(precip, np.float32, dev), evap=_to_dev(evap, np.float32, dev),
+                   uplift=_to_dev(uplift_m, np.float64, dev, unit), mask=_dev(m, dev),
+                   metric=_to_dev(grid.metric if metric is None else metric, np.float32, dev),
+                   metric_inv=_to_dev(grid.metric_inv if metric_inv is None else metric_inv, np.float32, dev),
+                   N=grid.N, H=grid.H, spherical=True, height_unit_m=unit, grid=grid, mode=mode)
 
     @classmethod
     def window(cls, height_m, sediment_m, discharge, momentum, hardness, precip, evap, uplift_m, mask, metric,
@@ -279,7 +291,22 @@ class ErosionState:
                 out[n] = FaceField(self.grid, self.momentum.cpu().numpy().astype(np.float32), is_vector=True, name="momentum")
         return out
 
+    def _check_buffers(self) -> None:
+        """The check mode's working arrays -- the packed record, the live acc,
+        the discharge / momentum tracks -- allocated on first use (the fast
+        mode keeps its own in its workspace: at N_c=8192 these are 50 GB)."""
+        F, NE, dev = self.height.shape[0], self.height.shape[1], self.height.device
+        if self.disch_track is None:
+            self.disch_track = torch.zeros((F, NE, NE), dtype=torch.float64, device=dev)
+        if self.mom_track is None:
+            self.mom_track = torch.zeros((F, NE, NE, 2), dtype=torch.float64, device=dev)
+        if self.samp is None:
+            self.samp = torch.zeros((F, NE, NE, pk.NS), dtype=torch.float32, device=dev)
+        if self.acc is None:
+            self.acc = torch.zeros((F, NE, NE), dtype=torch.float64, device=dev)
+
     def pack(self) -> None:
+        self._check_buffers()
         pk.pack_samples(self.samp, self.height, self.sediment, self.route, self.route is not None, self.discharge,
                         self.momentum, self.evap, self.hardness, self.metric, self.metric_inv, self.base, self.lake_flag)
 

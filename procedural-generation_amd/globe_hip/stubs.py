@@ -198,3 +198,128 @@ def zoom_windows(n: int = 64, seed: int = 0, side_lo: int = 128, side_hi: int = 
                         erosion=ep)
         out.append((win, p))
     return out
+
+
+# ---------------------------------------------------------------------------
+# the planet-wide workload, built on the device (N_c = 8192: 403 M cells,
+# whose host construction would take tens of minutes and ~250 GB)
+# ---------------------------------------------------------------------------
+def _face_points_device(grid: Grid, f: int, device):
+    """(s, t, p) of face f's extended cell centres on the device: the
+    gnomonic coordinates tan((u - 1/2) pi / 2), tan((v - 1/2) pi / 2) and the
+    unit vectors (the equi-angular mapping of cubesphere.to_sphere_v)."""
+    import torch
+
+    from .cubesphere import BASES, HALF_PI
+
+    e = (torch.arange(grid.NE, dtype=torch.float64, device=device) - grid.H + 0.5) / grid.N
+    U, V = torch.meshgrid(e, e, indexing="ij")
+    s = torch.tan((U - 0.5) * HALF_PI)
+    t = torch.tan((V - 0.5) * HALF_PI)
+    b = torch.from_numpy(BASES[f]).to(device)
+    q = b[2] + s[..., None] * b[0] + t[..., None] * b[1]
+    return s, t, q / torch.linalg.vector_norm(q, dim=-1, keepdim=True)
+
+
+def fbm_lattices(rng: np.random.Generator, octaves: int = 5, base_freq: float = 2.0):
+    """fbm_at's random lattices, drawn from ``rng`` in fbm_at's order:
+    (frequency, n, m, lattice values) per octave."""
+    out = []
+    for o in range(octaves):
+        freq = base_freq * 2 ** o
+        n = int(np.ceil(freq)) + 2
+        m = n + 1
+        out.append((freq, n, m, rng.random((m, m, m)).reshape(-1)))
+    return out
+
+
+def fbm_device(p, lattices):
+    """fbm_at on the device at unit vectors ``p`` (..., 3) (a float64
+    tensor) from fbm_lattices' draws: the same smoothstep trilinear value
+    noise, float64 torch ops."""
+    import torch
+
+    q3 = p.reshape(-1, 3)
+    out = torch.zeros(q3.shape[0], dtype=torch.float64, device=p.device)
+    amp, total = 1.0, 0.0
+    for freq, n, m, lat in lattices:
+        lattice = torch.from_numpy(lat).to(p.device)
+        q = (q3 + 1.0) * (0.5 * freq)
+        i0 = torch.floor(q)
+        t = q - i0
+        t = t * t * (3.0 - 2.0 * t)
+        i0 = i0.to(torch.int64).clamp_(0, n - 1)
+        base = (i0[:, 0] * m + i0[:, 1]) * m + i0[:, 2]
+        del q, i0
+        acc = torch.zeros_like(out)
+        for dx in (0, 1):
+            wx = t[:, 0] if dx else 1.0 - t[:, 0]
+            for dy in (0, 1):
+                wxy = wx * (t[:, 1] if dy else 1.0 - t[:, 1])
+                off = (dx * m + dy) * m
+                acc += wxy * ((1.0 - t[:, 2]) * lattice[base + off] + t[:, 2] * lattice[base + off + 1])
+        out += amp * (acc * 2.0 - 1.0)
+        total += amp
+        amp *= 0.5
+    return (out / total).reshape(p.shape[:-1])
+
+
+def metric_device(grid: Grid, device):
+    """The cell metric and its inverse, (6, NE, NE, 3) float32 on the device.
+    The equi-angular metric depends only on the gnomonic coordinates (s, t),
+    the same on every face: with r^2 = 1 + s^2 + t^2,
+    g_uu = (pi/2)^2 (1 + s^2)^2 (1 + t^2) / r^4, g_uv = -(pi/2)^2 (1 + s^2)(1 + t^2) s t / r^4,
+    g_vv = (pi/2)^2 (1 + t^2)^2 (1 + s^2) / r^4 (Grid.metric_uv's Jacobian products in closed form),
+    scaled to cell units as Grid.metric is."""
+    import torch
+
+    from .cubesphere import HALF_PI
+
+    s, t, _ = _face_points_device(grid, 0, device)
+    s2, t2 = s * s, t * t
+    r4 = (1.0 + s2 + t2) ** 2
+    k = HALF_PI * HALF_PI
+    scale = (grid.R_planet / (grid.N * grid.cell_size_m)) ** 2
+    g = torch.stack([k * (1.0 + s2) ** 2 * (1.0 + t2) / r4, -k * (1.0 + s2) * (1.0 + t2) * s * t / r4,
+                     k * (1.0 + t2) ** 2 * (1.0 + s2) / r4], dim=-1) * scale
+    g32 = g.to(torch.float32)
+    g64 = g32.to(torch.float64)
+    det = g64[..., 0] * g64[..., 2] - g64[..., 1] ** 2
+    gi = torch.stack([g64[..., 2] / det, -g64[..., 1] / det, g64[..., 0] / det], dim=-1).to(torch.float32)
+    return g32.expand(6, *g32.shape).contiguous(), gi.expand(6, *gi.shape).contiguous()
+
+
+def planet_globe(N: int = 8192, H: int = 2, seed: int = 0, octaves: int = 10, device="cuda"):
+    """The planet-wide workload's inputs built on the device: (grid,
+    fields, metric, metric_inv), fields as device tensors -- bedrock =
+    ``octaves``-octave fBm x 5000 m shifted so 30 % of the interior is land,
+    rain 2 cos(lat)^2 + 0.3 + 0.2 fBm (4 octaves), no hardness, evaporation
+    1, no uplift.  The same recipe as earth_globe (whose host arithmetic it
+    follows up to the device's tan / floating-point order)."""
+    import torch
+
+    grid = get_grid(N, H, earth_cell_size(N))
+    p = WorldParams(world=WorldGroup(seed=seed, N_c=N, cell_size_m=grid.cell_size_m, halo=H))
+    F, NE = 6, grid.NE
+    bed_lat = fbm_lattices(p.rng("stub", 1), octaves)
+    rain_lat = fbm_lattices(p.rng("stub", 2), 4)
+    bed = torch.empty((F, NE, NE), dtype=torch.float64, device=device)
+    rain = torch.empty((F, NE, NE), dtype=torch.float32, device=device)
+    for f in range(F):
+        _, _, pts = _face_points_device(grid, f, device)
+        bed[f] = fbm_device(pts, bed_lat) * 5000.0
+        lat = torch.asin(pts[..., 2].clamp(-1.0, 1.0))
+        rain[f] = (2.0 * torch.cos(lat) ** 2 + 0.3 + 0.2 * fbm_device(pts, rain_lat)).to(torch.float32)
+        del pts, lat
+    inter = bed[:, H:H + N, H:H + N].reshape(-1)
+    srt = torch.sort(inter).values
+    pos = 0.7 * (srt.numel() - 1)
+    lo = int(math.floor(pos))
+    q = float(srt[lo]) + (pos - lo) * (float(srt[min(lo + 1, srt.numel() - 1)]) - float(srt[lo]))
+    del srt, inter
+    bed -= q
+    metric, metric_inv = metric_device(grid, device)
+    fields = {"bedrock_m": bed, "hardness": torch.zeros((F, NE, NE), dtype=torch.float32, device=device),
+              "precip": rain, "evap": torch.ones((F, NE, NE), dtype=torch.float32, device=device),
+              "uplift_m": torch.zeros((F, NE, NE), dtype=torch.float64, device=device)}
+    return grid, fields, metric, metric_inv
