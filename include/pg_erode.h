@@ -52,9 +52,9 @@ typedef struct pg_trace_params {
 
 /* Message of the last failed call on this thread ("" if none). */
 const char* pg_last_error(void);
-/* ABI version of this library: 6 (bumped whenever a struct layout or a
+/* ABI version of this library: 7 (bumped whenever a struct layout or a
  * signature changes; 2 added pg_fast_params.erode_cap, 3 its tape, 6 the
- * brush). */
+ * brush, 7 the deterministic adjoint's workspace and halo table). */
 int pg_abi_version(void);
 
 /* ---- geometry, host side (bake/globe/cubesphere.py:70-220) ------------- */
@@ -235,8 +235,13 @@ int pg_exchange_halos(double* data, int64_t ncomp, int64_t K, int64_t S, const i
 /* Adjoint of the scalar exchange (no counterpart in the reference, which has
  * no adjoint): adj[src[k,s]] += w[k,s] adj[dst[k]], then adj[dst[k]] = 0 --
  * the halo entries of a gradient folded onto the interior cells they are
- * interpolated from (order of the double additions unspecified). */
-int pg_halo_scalar_adjoint(double* adj, int64_t K, int64_t S, const int64_t* dst, const int64_t* src, const double* w,
+ * interpolated from.  Deterministic: the inverse table lists, per source
+ * cell src_cell[i], the entries k * S + s that read it in increasing order
+ * (src_entry[src_start[i] .. src_start[i + 1])); each cell's contributions
+ * are summed in that order and added once.  Sources must not be
+ * destinations.  All arrays DEVICE. */
+int pg_halo_scalar_adjoint(double* adj, int64_t K, int64_t S, const int64_t* dst, const double* w, int64_t n_src,
+                           const int64_t* src_cell, const int64_t* src_start, const int64_t* src_entry,
                            pg_stream stream);
 
 /* ---- datum hold (bake/globe/erosion/maps.py:1446-1508, hold_datum) ----- */
@@ -540,7 +545,15 @@ typedef struct pg_grad_args {
     int64_t tape_bytes;
     const int32_t* fixed_moves; /* NULL, or per droplet (moves | cause << 24) of the run being replayed */
     int32_t* moves_out;         /* NULL, or per droplet (moves | cause << 24) of this run */
+    void* rev_work;             /* reverse: DEVICE scratch of pg_grad_reverse_work_bytes() bytes (the
+                                 * fixed-point accumulators that make the sums order-free) */
+    int64_t rev_work_bytes;
 } pg_grad_args;
+
+/* Bytes of the reverse sweep's scratch for F faces of N x N cells, halo H:
+ * a signed 128-bit fixed-point accumulator (unit 2^-64) per extended cell for
+ * the surface and the erodibility adjoints, and eight for the scalars. */
+int64_t pg_grad_reverse_work_bytes(int64_t F, int64_t N, int64_t H);
 
 /* Bytes of a tape holding `capacity` droplets of up to max_steps moves. */
 int pg_grad_tape_bytes(int64_t capacity, int64_t max_steps, int32_t real, int64_t* bytes);
@@ -558,8 +571,11 @@ int pg_grad_iterate(const pg_fast_fields* steer, const pg_fast_fields* target, c
 /* Reverse sweep of a taped iteration: with g = dL/d(surface after the
  * particle pass) (F*NE*NE double), adds the pass's share of dL/d(surface
  * before it) into lam, dL/d(erodibility factor) into grad_erod (F*NE*NE
- * double) and dL/d(theta) into grad_theta (double[6]); double atomics, so
- * sums are reproducible to rounding only. */
+ * double) and dL/d(theta) into grad_theta (double[6]).  Every contribution
+ * is accumulated in 128-bit fixed point (args->rev_work; each truncated
+ * toward zero at 2^-64, |contribution| < 2^62) and added to the outputs
+ * once, so the result does not depend on the order of the lanes: bit for
+ * bit reproducible. */
 int pg_grad_reverse(int64_t F, int64_t N, int64_t H, const pg_trace_params* prm, const pg_grad_args* args,
                     const double* g, double* lam, double* grad_erod, double* grad_theta, pg_stream stream);
 

@@ -121,14 +121,23 @@ __global__ void halo_scalar_kernel(double* data, int64_t K, int64_t S, const int
     }
 }
 
-__global__ void halo_scalar_adjoint_kernel(double* adj, int64_t K, int64_t S, const int64_t* dst, const int64_t* src,
-                                           const double* w) {
-    GRID_STRIDE(k, K) {
-        const double a = adj[dst[k]];
-        adj[dst[k]] = 0.0;
-        if (a != 0.0)
-            for (int64_t s = 0; s < S; s++) unsafeAtomicAdd(&adj[src[k * S + s]], w[k * S + s] * a);
+// one thread per source cell: the contributions of every halo entry that
+// reads it, summed in entry order (deterministic), then added to the cell
+__global__ void halo_scalar_adjoint_gather_kernel(double* adj, int64_t S, const int64_t* dst, const double* w,
+                                                  int64_t n_src, const int64_t* src_cell, const int64_t* src_start,
+                                                  const int64_t* src_entry) {
+    GRID_STRIDE(i, n_src) {
+        double v = 0.0;
+        for (int64_t j = src_start[i]; j < src_start[i + 1]; j++) {
+            const int64_t e = src_entry[j];
+            v += w[e] * adj[dst[e / S]];
+        }
+        if (v != 0.0) adj[src_cell[i]] += v;
     }
+}
+
+__global__ void halo_clear_kernel(double* adj, int64_t K, const int64_t* dst) {
+    GRID_STRIDE(k, K) { adj[dst[k]] = 0.0; }
 }
 
 __global__ void halo_vector_kernel(double* data, int64_t K, int64_t S, const int64_t* dst, const int64_t* src,
@@ -296,14 +305,21 @@ extern "C" int pg_exchange_halos(double* data, int64_t ncomp, int64_t K, int64_t
     return 0;
 }
 
-extern "C" int pg_halo_scalar_adjoint(double* adj, int64_t K, int64_t S, const int64_t* dst, const int64_t* src,
-                                      const double* w, pg_stream stream) {
+extern "C" int pg_halo_scalar_adjoint(double* adj, int64_t K, int64_t S, const int64_t* dst, const double* w,
+                                      int64_t n_src, const int64_t* src_cell, const int64_t* src_start,
+                                      const int64_t* src_entry, pg_stream stream) {
     clear_error();
-    PG_REQUIRE(K >= 0 && S > 0 && adj && dst && src && w, "pg_halo_scalar_adjoint: bad arguments");
+    PG_REQUIRE(K >= 0 && S > 0 && n_src >= 0 && adj && dst && w && src_cell && src_start && src_entry,
+               "pg_halo_scalar_adjoint: bad arguments");
     if (K == 0) return 0;
-    hipLaunchKernelGGL(halo_scalar_adjoint_kernel, dim3(grid_for(K)), dim3(TPB), 0, (hipStream_t)stream, adj, K, S, dst,
-                       src, w);
-    PG_LAUNCHED("halo_scalar_adjoint_kernel");
+    hipStream_t s = (hipStream_t)stream;
+    if (n_src > 0) {
+        hipLaunchKernelGGL(halo_scalar_adjoint_gather_kernel, dim3(grid_for(n_src)), dim3(TPB), 0, s, adj, S, dst, w,
+                           n_src, src_cell, src_start, src_entry);
+        PG_LAUNCHED("halo_scalar_adjoint_gather_kernel");
+    }
+    hipLaunchKernelGGL(halo_clear_kernel, dim3(grid_for(K)), dim3(TPB), 0, s, adj, K, dst);
+    PG_LAUNCHED("halo_clear_kernel");
     return 0;
 }
 

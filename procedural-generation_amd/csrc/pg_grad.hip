@@ -472,21 +472,67 @@ __global__ void __launch_bounds__(TRACE_TPB) grad_forward_kernel(GradCfg<Real> k
 // ---------------------------------------------------------------------------
 // reverse sweep
 // ---------------------------------------------------------------------------
+// Deterministic accumulation: every contribution is added as a signed 128-bit
+// fixed-point number of unit 2^-64 (two 64-bit integer atomics with the
+// carry), so the sum does not depend on the order the lanes arrive in; each
+// contribution is truncated toward zero at 2^-64, |contribution| < 2^62.
+struct Fx128 {
+    ull lo;
+    ll hi;
+};
+
+__device__ __forceinline__ void fx_add(Fx128* a, double v) {
+    if (v == 0.0) return;
+    const double m = fabs(v);
+    const double fh = floor(m);
+    ull H = (ull)fh;
+    ull L = (ull)((m - fh) * 18446744073709551616.0);  // m - fh is exact; the scaling by 2^64 too
+    if (v < 0.0) {  // two's complement of (H, L)
+        L = ~L + 1ull;
+        H = ~H + (L == 0ull ? 1ull : 0ull);
+    }
+    const ull old = atomicAdd(&a->lo, L);
+    const ull carry = old + L < old ? 1ull : 0ull;
+    if (H + carry) atomicAdd((ull*)&a->hi, H + carry);
+}
+
+// the accumulated sum, in units of 1: hi + lo 2^-64
+__device__ __forceinline__ double fx_value(const Fx128& a) {
+    return (double)a.hi + (double)a.lo * 5.421010862427522e-20;
+}
+
+// the reverse sweep's accumulators: surface and erodibility adjoints per
+// extended cell, the six scalars (and two spare)
+int64_t reverse_work_bytes(int64_t n) { return (2 * n + 8) * (int64_t)sizeof(Fx128); }
+
 template <typename Real>
 struct RevCfg {
     const char* tape;
     ll capacity;
     const double* g;   // dL/d(surface after the particle pass), extended cells
-    double* lam;       // += the particle pass's share of dL/d(surface before it)
-    double* grad_erod; // += dL/d(erodibility factor), extended cells
-    double* grad_theta;
+    Fx128* lam;        // the particle pass's share of dL/d(surface before it)
+    Fx128* grad_erod;  // dL/d(erodibility factor), extended cells
+    Fx128* grad_theta;
     int N, H, NE, feedback;
     float dt, inv_sat, disc_exponent, max_erode, fan_room;
     Real erodibility, rate, evap_rate, k_disc, ocean_rate, dep_floor;
 };
 
-__device__ __forceinline__ void add_d(double* p, double v) {
-    if (v != 0.0) unsafeAtomicAdd(p, v);
+__device__ __forceinline__ void add_d(Fx128* p, double v) { fx_add(p, v); }
+
+__global__ void fx_zero_kernel(int64_t n, Fx128* acc) {
+    for (int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; c < n; c += (int64_t)gridDim.x * blockDim.x) {
+        acc[c].lo = 0ull;
+        acc[c].hi = 0;
+    }
+}
+
+// out[c] += the accumulated sum
+__global__ void fx_flush_kernel(int64_t n, const Fx128* acc, double* out) {
+    for (int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; c < n; c += (int64_t)gridDim.x * blockDim.x) {
+        const Fx128 a = acc[c];
+        if (a.lo != 0ull || a.hi != 0) out[c] += fx_value(a);
+    }
 }
 
 // scatter `st` over the bilinear stencil of a taped position and `ca` onto its cell
@@ -743,12 +789,17 @@ int launch_reverse(int64_t F, int64_t N, int64_t H, const pg_trace_params* prm, 
     ll cap = (ll)(((size_t)ga->tape_bytes - up(sizeof(TapeHead))) / per);
     while (cap > 0 && tape_bytes<Real>(cap, stride) > (size_t)ga->tape_bytes) cap--;
     PG_REQUIRE(cap > 0, "pg_grad_reverse: tape too small");
+    const int64_t n = F * (N + 2 * H) * (N + 2 * H);
+    PG_REQUIRE(ga->rev_work && ga->rev_work_bytes >= reverse_work_bytes(n),
+               "pg_grad_reverse: reverse workspace of %lld bytes, need %lld", (long long)ga->rev_work_bytes,
+               (long long)reverse_work_bytes(n));
+    Fx128* acc = reinterpret_cast<Fx128*>(ga->rev_work);
     r.tape = reinterpret_cast<const char*>(ga->tape);
     r.capacity = cap;
     r.g = g;
-    r.lam = lam;
-    r.grad_erod = grad_erod;
-    r.grad_theta = grad_theta;
+    r.lam = acc;
+    r.grad_erod = acc + n;
+    r.grad_theta = acc + 2 * n;
     r.N = (int)N;
     r.H = (int)H;
     r.NE = (int)(N + 2 * H);
@@ -764,10 +815,16 @@ int launch_reverse(int64_t F, int64_t N, int64_t H, const pg_trace_params* prm, 
     r.k_disc = (Real)ga->theta[3];
     r.ocean_rate = (Real)ga->theta[4];
     r.dep_floor = (Real)ga->theta[5];
-    (void)F;
+    const int zgrid = grid_for(2 * n + N_THETA, 65535);
+    hipLaunchKernelGGL(fx_zero_kernel, dim3(zgrid), dim3(TPB), 0, s, 2 * n + N_THETA, acc);
     const int grid = grid_for(cap, 65535);
     hipLaunchKernelGGL(grad_reverse_kernel<Real>, dim3(grid), dim3(TPB), 0, s, r);
     PG_LAUNCHED("grad_reverse_kernel");
+    const int fgrid = grid_for(n, 65535);
+    hipLaunchKernelGGL(fx_flush_kernel, dim3(fgrid), dim3(TPB), 0, s, n, acc, lam);
+    hipLaunchKernelGGL(fx_flush_kernel, dim3(fgrid), dim3(TPB), 0, s, n, acc + n, grad_erod);
+    hipLaunchKernelGGL(fx_flush_kernel, dim3(1), dim3(64), 0, s, (int64_t)N_THETA, acc + 2 * n, grad_theta);
+    PG_LAUNCHED("fx_flush_kernel");
     return 0;
 }
 
@@ -778,6 +835,12 @@ int launch_reverse(int64_t F, int64_t N, int64_t H, const pg_trace_params* prm, 
 using namespace pg;
 using namespace pg::fastmode;
 using namespace pg::gradmode;
+
+extern "C" int64_t pg_grad_reverse_work_bytes(int64_t F, int64_t N, int64_t H) {
+    clear_error();
+    PG_REQUIRE(F > 0 && N >= 2 && H >= 1, "pg_grad_reverse_work_bytes: bad grid");
+    return reverse_work_bytes(F * (N + 2 * H) * (N + 2 * H));
+}
 
 extern "C" int pg_grad_tape_bytes(int64_t capacity, int64_t max_steps, int32_t real, int64_t* bytes) {
     clear_error();

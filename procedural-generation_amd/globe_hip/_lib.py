@@ -51,7 +51,8 @@ class GradArgs(ctypes.Structure):
     """``pg_grad_args``: the differentiable mode's inputs of one iteration."""
 
     _fields_ = [("real", ctypes.c_int32), ("feedback", ctypes.c_int32), ("surf", vp), ("erod", vp), ("theta", dbl * 6),
-                ("tape", vp), ("tape_bytes", i64), ("fixed_moves", vp), ("moves_out", vp)]
+                ("tape", vp), ("tape_bytes", i64), ("fixed_moves", vp), ("moves_out", vp), ("rev_work", vp),
+                ("rev_work_bytes", i64)]
 
 
 #: statistics slots of a fast iteration (include/pg_erode.h, PG_FS_*)
@@ -97,7 +98,8 @@ SIGNATURES = [
     ("pg_apply_uplift", i32, [i64, vp, vp, vp, dbl, i32, dbl, vp]),
     ("pg_spawn_cells", i32, [vp, i64, vp, i64, vp, vp]),
     ("pg_exchange_halos", i32, [vp, i64, i64, i64, vp, vp, vp, vp, vp]),
-    ("pg_halo_scalar_adjoint", i32, [vp, i64, i64, vp, vp, vp, vp]),
+    ("pg_halo_scalar_adjoint", i32, [vp, i64, i64, vp, vp, i64, vp, vp, vp, vp]),
+    ("pg_grad_reverse_work_bytes", i64, [i64, i64, i64]),
     ("pg_select_work_bytes", i64, []),
     ("pg_select_kth_surface", i32, [vp, vp, i64, i64, i64, i64, vp, vp, vp]),
     ("pg_datum_shift", i32, [i64, vp, vp, vp, vp, vp]),

@@ -297,14 +297,20 @@ class GradRun:
         st = self.state
         if st.spherical:
             ht = self.halo
-            check(lib().pg_halo_scalar_adjoint(pk._ptr(lam), ht.K, ht.S, pk._ptr(ht.dst), pk._ptr(ht.src), pk._ptr(ht.w),
-                                               pk._stream()), "halo_scalar_adjoint")
+            ht.adjoint(lam)
         if self.datum:
             total = lam.sum().reshape(1)
             lam.view(-1).index_add_(0, idx, -total)
         out = lam.clone()
         surf = st.height  # unused by the reverse sweep; any device pointer of the right size
         ga = self._args(surf, erod, theta, tape=tape)
+        if getattr(self, "_rev_work", None) is None:
+            nbytes = int(lib().pg_grad_reverse_work_bytes(st.F, st.N, st.H))
+            if nbytes < 0:
+                check(-1, "pg_grad_reverse_work_bytes")
+            self._rev_work = torch.empty(nbytes, dtype=torch.uint8, device=st.device)
+        ga.rev_work = self._rev_work.data_ptr()
+        ga.rev_work_bytes = self._rev_work.numel()
         check(lib().pg_grad_reverse(st.F, st.N, st.H, ctypes.byref(self.prm), ctypes.byref(ga), pk._ptr(lam),
                                     pk._ptr(out), pk._ptr(g_erod), pk._ptr(g_theta), pk._stream()), "pg_grad_reverse")
         return out

@@ -106,6 +106,24 @@ class HaloTables:
         self.w = _dev(hm.w.astype(np.float64), device)
         self.w_lin = _dev(hm.w_lin.astype(np.float64), device)
         self.rot = _dev(hm.rot.astype(np.float64), device)
+        # the transpose's gather table: per source cell the entries k * S + s that read it, in order
+        src = hm.src.astype(np.int64).reshape(-1)
+        if np.isin(hm.dst, src).any():
+            raise ValueError("halo table: a destination is also a source")
+        order = np.argsort(src, kind="stable")
+        cells, start = np.unique(src[order], return_index=True)
+        self.n_src = int(cells.size)
+        self.src_cell = _dev(cells.astype(np.int64), device)
+        self.src_start = _dev(np.append(start, src.size).astype(np.int64), device)
+        self.src_entry = _dev(order.astype(np.int64), device)
+
+    def adjoint(self, adj: torch.Tensor) -> None:
+        """The scalar exchange's transpose in place (pg_halo_scalar_adjoint):
+        halo entries folded onto their sources, halo cells cleared."""
+        pk._need(adj, torch.float64, None, "halo adjoint")
+        check(lib().pg_halo_scalar_adjoint(pk._ptr(adj), self.K, self.S, pk._ptr(self.dst), pk._ptr(self.w), self.n_src,
+                                           pk._ptr(self.src_cell), pk._ptr(self.src_start), pk._ptr(self.src_entry),
+                                           pk._stream()), "halo_scalar_adjoint")
 
     def exchange(self, data: torch.Tensor, vector: bool = False, linear: bool = False) -> None:
         pk._need(data, torch.float64, None, "halo field")

@@ -111,7 +111,9 @@ def _check(st, run32, run64, erod, theta, T, label, n_cells=32, prec=1e-2, theta
             assert np.all(np.abs(fd_e - ge[cells]) <= 1e-3 * np.abs(ge[cells]) + 1e-4 * scale), h_rel
             assert np.all(np.abs(fd_t - gt) <= theta_tol * np.abs(gt) + 1e-9), h_rel
         else:
-            assert np.median(err_e) <= bar["cell_median"] and err_e.max() <= bar["cell_max"], (h_rel, err_e)
+            assert np.median(err_e) <= bar["cell_median"], (h_rel, err_e)
+            if bar.get("cell_max") is not None:
+                assert err_e.max() <= bar["cell_max"], (h_rel, err_e)
             if bar.get("theta") is not None:
                 assert np.all(err_t <= bar["theta"]), (h_rel, err_t)
 
@@ -133,13 +135,16 @@ def test_cfg4_feedback_gradient(cuda):
     # on: their bar is 1e-2 (measured 1.6e-5 .. 3.7e-3).  At the configuration's
     # step (1e-3) the feedback carries every crossed kink's jump on: the scalars'
     # central differences are then no derivative estimate (measured 1e-3 .. 0.29
-    # from the adjoint) and only the cells are asserted (median 3.1e-3, worst 0.39)
+    # from the adjoint), nor are a single cell's whose step crosses a kink near
+    # it (the worst of a 16-cell sample measured 0.39 on one sample, 2.67 on
+    # another): at that step the median cell is asserted (measured 3.1e-3,
+    # 9.5e-3); at 1e-6 every sampled cell and scalar is
     T = 5
     st, p, erod = _setup(cuda, T)
     run32 = adj.GradRun(st, p, T, real="f32", checkpoint_every=2, feedback=True)
     run64 = adj.GradRun(st, p, T, real="f64", checkpoint_every=2, feedback=True, replay_of=run32)
     _check(st, run32, run64, erod, adj.theta_nominal(run32), T, "cfg4 feedback", n_cells=16, prec=5e-2,
-           theta_tol=1e-2, bars={1e-3: dict(cell_median=2e-2, cell_max=0.5, theta=None)})
+           theta_tol=1e-2, bars={1e-3: dict(cell_median=2e-2, cell_max=None, theta=None)})
     # the full horizon: the surface adjoint's growth (reported)
     run = adj.GradRun(st, p, 50, real="f32", checkpoint_every=CE, feedback=True)
     out = run.forward(erod, adj.theta_nominal(run))
