@@ -201,10 +201,53 @@ struct Slice {
     }
 };
 
+// A loop-invariant double held in vector registers.  The trace loop reads
+// ~25 parameters; as scalars they exceed the SGPR file and the compiler
+// spills them to VGPR lanes, paying a v_readlane (and its wait states) at
+// every use.  A lane has room for them (one wave per SIMD here), and a VGPR
+// operand costs nothing extra.
+__device__ __forceinline__ double vreg(double x) {
+    asm volatile("" : "+v"(x));
+    return x;
+}
+template <class T>
+__device__ __forceinline__ T* vptr(T* x) {
+    asm volatile("" : "+v"(x));
+    return x;
+}
+
 __global__ void __launch_bounds__(64) trace_check_kernel(TraceArgs a) {
     const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= a.P) return;
-    const pg_trace_params& P = a.prm;
+    pg_trace_params P = a.prm;
+    P.dt = vreg(P.dt);
+    P.density = vreg(P.density);
+    P.friction = vreg(P.friction);
+    P.deposition_rate = vreg(P.deposition_rate);
+    P.evap_rate = vreg(P.evap_rate);
+    P.k_mom = vreg(P.k_mom);
+    P.mom_scale = vreg(P.mom_scale);
+    P.k_disc = vreg(P.k_disc);
+    P.disc_exponent = vreg(P.disc_exponent);
+    P.slope_gain = vreg(P.slope_gain);
+    P.slope_sat = vreg(P.slope_sat);
+    P.erodibility = vreg(P.erodibility);
+    P.cover_depth = vreg(P.cover_depth);
+    P.min_volume = vreg(P.min_volume);
+    P.max_erode = vreg(P.max_erode);
+    P.conc_scale = vreg(P.conc_scale);
+    P.ocean_rate = vreg(P.ocean_rate);
+    P.fan_room = vreg(P.fan_room);
+    P.dep_floor = vreg(P.dep_floor);
+    P.lake_trap = vreg(P.lake_trap);
+    P.lateral_rate = vreg(P.lateral_rate);
+    a.cl_cell = vptr(a.cl_cell);
+    a.cl_delta = vptr(a.cl_delta);
+    a.cl_vol = vptr(a.cl_vol);
+    a.cl_mom = vptr(a.cl_mom);
+    a.mask = vptr(a.mask);
+    a.roots = vptr(a.roots);
+    a.R.s = vptr(a.R.s);
     const Record& R = a.R;
     const int N = (int)a.N, H = (int)a.H, NE = N + 2 * H;
     const int64_t cap = change_list_cap(P.max_steps, P.lateral_rate > 0.0);
