@@ -409,10 +409,12 @@ __device__ void droplet_wave(const OrdArgs& a, int64_t p, int sweep) {
     }
     bool changed = false;
     const int64_t kb0 = (k_first / WAVE) * WAVE;
+    // two batches ahead in flight: a walk is a chain of batches whose loads come from HBM or the MALL
     Ent en = load_ent(a, base, kb0, k_first, cnt, sweep);
+    Ent nx = load_ent(a, base, kb0 + WAVE, k_first, cnt, sweep);
     for (int64_t k0 = kb0; k0 < cnt; k0 += WAVE) {
         const int64_t k = k0 + lane, e = base + k;
-        const Ent nx = load_ent(a, base, k0 + WAVE, k_first, cnt, sweep);  // in flight during this batch's walk
+        const Ent nx2 = load_ent(a, base, k0 + 2 * WAVE, k_first, cnt, sweep);  // in flight during this batch's walk
         const uint8_t fl = en.fl;
         const bool act = fl & EF_ACTIVE, ero = fl & EF_ERODE;
         double d = 0.0, w = 0.0, x = 0.0, U = 0.0;
@@ -515,6 +517,7 @@ __device__ void droplet_wave(const OrdArgs& a, int64_t p, int sweep) {
             }
         }
         en = nx;
+        nx = nx2;
     }
     // the stockpile of what the droplet could not place (particle.py:1019-1041)
     if (lane == 0) {
@@ -641,11 +644,13 @@ __device__ void range_wave(const OrdArgs& a, int64_t r, int sweep) {
         }
         if (parked) a.pending[cur] = pv;
     };
+    // two batches ahead in flight (a long segment is a chain of batches)
     Slots b = load_slots(a, i0);
+    Slots nb = load_slots(a, i0 + WAVE);
     for (int64_t base = i0;; base += WAVE) {
         const int64_t i = base + lane;
         const bool more = base + WAVE < a.n_slots;
-        const Slots nb = load_slots(a, base + WAVE);  // in flight during this batch's serial part
+        const Slots nb2 = load_slots(a, base + 2 * WAVE);  // in flight during this batch's serial part
         const uint32_t key = b.key, val = b.val;
         uint32_t prevkey = __shfl_up(key, 1);
         if (lane == 0) prevkey = carry_key;
@@ -742,6 +747,7 @@ __device__ void range_wave(const OrdArgs& a, int64_t r, int sweep) {
         }
         if (stop < WAVE || !more) break;
         b = nb;
+        nb = nb2;
     }
     write_back();
 }
