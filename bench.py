@@ -103,7 +103,7 @@ def deviation(s0, sa, shift_a, sb, shift_b, unit):
     return gap, (eb - ea) / ea
 
 
-def exact_run(grid, f, p, steps, warmup):
+def exact_run(grid, f, p, steps, warmup, lanes=64):
     """The headline: the exact mode (check mode with the parallel apply)."""
     import torch
 
@@ -117,8 +117,22 @@ def exact_run(grid, f, p, steps, warmup):
     maps.start_replay(st, p)
     shift = 0.0
     log(f"exact mode: warmup {warmup}")
+    # the warm-up iterations pick the trace's droplets per wave (the same results at every setting)
+    tuner = None
+    if lanes:
+        st.check_lanes = lanes
+    else:
+        # the measured best first: a short warm-up tries the likeliest settings
+        tuner = maps.CheckAutotuner(st, candidates=(8, 16, 4, 64)[:max(1, min(warmup, 4))])
     for it in range(warmup):
-        shift += float(maps.step(st, p, it)["datum_shift"])
+        if tuner is not None:
+            shift += float(tuner.step(lambda k: maps.step(st, p, k), it)["datum_shift"])
+        else:
+            shift += float(maps.step(st, p, it)["datum_shift"])
+    if tuner is not None:
+        if tuner.best is None:
+            tuner.finish()
+        log(f"exact mode: droplets per wave {st.check_lanes} (trace ns per move {tuner.times})")
     torch.cuda.synchronize()
     st.check_time_trace = True
     pk.check_trace_time()  # drop anything recorded before the timed region
@@ -145,7 +159,8 @@ def exact_run(grid, f, p, steps, warmup):
     return dict(seconds=dt, moves=moves, particles=particles, entries=entries, chunks=chunks, trace_ms=trace_ms,
                 launches=launches, sweeps_per_chunk=sweeps / max(chunks, 1),
                 sweeps_max=max(s["sweeps_max"] for s in stats), s0=s0, s=sf, shift=shift, unit=unit,
-                iterations=warmup + steps)
+                iterations=warmup + steps, lanes=int(st.check_lanes),
+                lane_times=None if tuner is None else {str(k): v for k, v in tuner.times.items()})
 
 
 def fast_run(grid, f, p, iterations, steps, chunks=16, brush=0):
@@ -463,6 +478,8 @@ def main():
     ap.add_argument("--chunks", type=int, default=16, help="the fast mode's chunks per iteration")
     ap.add_argument("--max-steps", type=int, default=128)
     ap.add_argument("--brush", type=int, default=3, help="the fast mode's brush leg: radius (0 = no leg)")
+    ap.add_argument("--check-lanes", type=int, default=0,
+                    help="droplets per wave of the exact mode's trace (0: autotuned over the warm-up iterations)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fast", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
@@ -479,7 +496,7 @@ def main():
               f"iteration (designed, not built); --gpus {a.gpus} is not supported", file=sys.stderr)
         sys.exit(2)
     grid, f, p = build_inputs(a.N, 2, a.max_steps, 200)
-    r = exact_run(grid, f, p, a.steps, a.warmup)
+    r = exact_run(grid, f, p, a.steps, a.warmup, a.check_lanes)
     value = r["moves"] / r["seconds"]
     trace_s = r["trace_ms"] / 1000.0
     achieved = B_MOVE_EXACT * r["moves"] / trace_s / 1e9 if trace_s > 0 else 0.0
@@ -503,7 +520,8 @@ def main():
                                f"computed in parallel",
                    "droplets_per_iteration": r["particles"] // a.steps, "moves": r["moves"],
                    "steps_mean": r["entries"] / max(r["particles"], 1), "chunks_per_iteration": r["chunks"] // a.steps,
-                   "apply_sweeps_per_chunk": r["sweeps_per_chunk"], "apply_sweeps_max": r["sweeps_max"]},
+                   "apply_sweeps_per_chunk": r["sweeps_per_chunk"], "apply_sweeps_max": r["sweeps_max"],
+                   "trace_droplets_per_wave": r["lanes"], "autotune_trace_ns_per_move": r["lane_times"]},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "kernel": "trace_check_kernel (one lane per droplet, float64, one launch per 2048-droplet chunk)",

@@ -103,6 +103,12 @@ int pg_trace_particles(int64_t P, const int64_t* sp_face, const double* sp_x, co
                        float* cl_mom, int64_t* cl_count, int64_t* cl_lo, int64_t* cl_hi, int8_t* sp_death,
                        pg_stream stream);
 
+/* Droplets per wave of pg_trace_particles (and of the chunk loop): a power
+ * of two in [1, 64] (default 64); the same results at every setting (each
+ * lane traces its own droplet).  Library-wide state, not thread-safe.  No
+ * counterpart in the reference (a launch shape). */
+int pg_check_set_lanes(int lanes);
+
 /* particle.py:258-272, change_list_cap */
 int64_t pg_change_list_cap(int64_t max_steps, int lateral);
 

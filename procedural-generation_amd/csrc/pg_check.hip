@@ -178,7 +178,14 @@ struct TraceArgs {
     float* cl_mom;
     int64_t *cl_count, *cl_lo, *cl_hi;
     int8_t* sp_death;
+    int lanes;  // droplets per wave (the rest of the wave idles)
 };
+
+// Droplets per wave of the trace (pg_check_set_lanes): a wave steps through
+// the union of the branches its droplets take, so fewer droplets per wave
+// can mean fewer instructions per step; a chunk's 2048 droplets leave most
+// SIMDs idle either way.  Library-wide, like the launch timers.
+int g_trace_lanes = 64;
 
 // A droplet's slice of the change list.
 struct Slice {
@@ -217,7 +224,8 @@ __device__ __forceinline__ T* vptr(T* x) {
 }
 
 __global__ void __launch_bounds__(64) trace_check_kernel(TraceArgs a) {
-    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if ((int)threadIdx.x >= a.lanes) return;
+    const int64_t p = (int64_t)blockIdx.x * a.lanes + threadIdx.x;
     if (p >= a.P) return;
     pg_trace_params P = a.prm;
     P.dt = vreg(P.dt);
@@ -745,9 +753,18 @@ extern "C" int pg_trace_particles(int64_t P, const int64_t* sp_face, const doubl
     PG_REQUIRE(6 * (N + 2 * H) * (N + 2 * H) < ((int64_t)1 << 31), "pg_trace_particles: N=%lld too large for 32-bit cell ids",
                (long long)N);
     TraceArgs a{P, sp_face, sp_x, sp_y, sp_sed, sp_dir, volume0, Record{samp, (int)(N + 2 * H)}, use_route, mask, N, H,
-                spherical, *prm, roots, use_roots, cl_cell, cl_delta, cl_vol, cl_mom, cl_count, cl_lo, cl_hi, sp_death};
-    hipLaunchKernelGGL(trace_check_kernel, dim3(blocks_for(P, 64)), dim3(64), 0, (hipStream_t)stream, a);
+                spherical, *prm, roots, use_roots, cl_cell, cl_delta, cl_vol, cl_mom, cl_count, cl_lo, cl_hi, sp_death,
+                g_trace_lanes};
+    hipLaunchKernelGGL(trace_check_kernel, dim3(blocks_for(P, g_trace_lanes)), dim3(64), 0, (hipStream_t)stream, a);
     PG_LAUNCHED("trace_check_kernel");
+    return 0;
+}
+
+extern "C" int pg_check_set_lanes(int lanes) {
+    clear_error();
+    PG_REQUIRE(lanes >= 1 && lanes <= 64 && (lanes & (lanes - 1)) == 0, "pg_check_set_lanes: %d is not a power of two "
+               "in [1, 64]", lanes);
+    g_trace_lanes = lanes;
     return 0;
 }
 

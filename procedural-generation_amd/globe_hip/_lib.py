@@ -115,6 +115,7 @@ SIGNATURES = [
     ("pg_fast_trace_time", i32, [pd, ctypes.POINTER(ctypes.c_int64)]),
     ("pg_fast_trace_time_passes", i32, [pd, ctypes.POINTER(ctypes.c_int64)]),
     ("pg_fast_brush_time", i32, [pd, ctypes.POINTER(ctypes.c_int64)]),
+    ("pg_check_set_lanes", i32, [i32]),
     ("pg_brush_weights", i32, [i64, ctypes.POINTER(ctypes.c_int32), i64]),
     ("pg_brush_spread", i32, [vp, vp, vp, vp, i64, i64, i64, i64, vp, vp]),
     ("pg_sea_work_bytes", i64, [i64]),

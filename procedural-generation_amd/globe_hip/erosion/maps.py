@@ -163,6 +163,9 @@ class ErosionState:
     check_launched_sweeps: int = 40
     #: record HIP events around each check-mode trace launch (particle.check_trace_time reads them)
     check_time_trace: bool = False
+    #: droplets per wave of the check-mode trace (a power of two in [1, 64]: the same results); 8 measured
+    #: fastest on MI355X (fewer divergent branches per wave), CheckAutotuner re-measures it
+    check_lanes: int = 8
     route: torch.Tensor | None = None
     lake_flag: torch.Tensor | None = None
     disch_track: torch.Tensor = field(default=None, repr=False)
@@ -790,6 +793,7 @@ def run_iteration(state: ErosionState, params, iteration_key, n_particles=None, 
         ord_work, ord_status = state._ordered["work"], state._ordered["status"]
         ord_status.zero_()
     t1 = time.time()
+    pk.set_trace_lanes(int(state.check_lanes))
     moves_dev = torch.zeros(1, dtype=torch.int64, device=dev) if one_call else None
     if one_call:
         # the whole chunk loop in one library call (no host round trip per chunk)
@@ -1183,3 +1187,43 @@ def step(state: ErosionState, params, iteration_key, log=None, after_refresh=Non
 
 __all__ = ["ErosionState", "run_iteration", "thermal_erosion", "apply_uplift", "uplift_replay", "start_replay",
            "apply_isostasy", "hold_datum", "step", "spawn_particles", "height_unit", "max_steps_of"]
+
+
+class CheckAutotuner:
+    """Picks the exact mode's droplets per wave from real iterations: the
+    first len(candidates) iterations each run one setting with the trace
+    launches timed (HIP events on the launch stream), normalised by the
+    iteration's droplet moves; the fastest per move is kept.  Results do not
+    depend on the setting, so no iteration is wasted.  ``step(run, key)``
+    runs ``run(key)`` (one iteration of the state, returning its stats) and
+    returns what it returns."""
+
+    def __init__(self, state: ErosionState, candidates=(64, 32, 16, 8, 4)):
+        self.state = state
+        self.candidates = [int(c) for c in candidates]
+        self.times: dict = {}
+        self.best = None
+
+    def step(self, run, iteration_key):
+        if self.best is not None:
+            return run(iteration_key)
+        lanes = self.candidates[len(self.times)]
+        timed = self.state.check_time_trace
+        self.state.check_lanes = lanes
+        self.state.check_time_trace = True
+        pk.check_trace_time()
+        out = run(iteration_key)
+        ms, _ = pk.check_trace_time()
+        self.state.check_time_trace = timed
+        self.times[lanes] = ms / max(int(out["moves"]), 1) * 1e6  # ns per move
+        if len(self.times) == len(self.candidates):
+            self.finish()
+        return out
+
+    def finish(self):
+        """Keep the fastest setting measured so far."""
+        if self.times:
+            self.best = min(self.times, key=self.times.get)
+            self.state.check_lanes = self.best
+        return self.best
+

@@ -266,6 +266,12 @@ def check_chunks(sp_face, sp_x, sp_y, sp_sed, sp_dir, volume0, chunk, samp, use_
           "check_chunks")
 
 
+def set_trace_lanes(lanes: int) -> None:
+    """Droplets per wave of the check-mode trace (8, 16, 32, 64; the same
+    results at every setting)."""
+    check(lib().pg_check_set_lanes(int(lanes)), "pg_check_set_lanes")
+
+
 def check_trace_time():
     """(ms, launches) of the trace launches timed by ``check_chunks`` since the last call."""
     ms = ctypes.c_double(0.0)

@@ -185,3 +185,32 @@ def test_window_lateral_cuts_match_oracle(oracle, cuda, inputs, rate):
     n_cut = sum(int((v[q * cap: q * cap + int(c[q])] == -2.0).sum())
                 for c, _, v, _, cap, n in cuts for q in range(n))
     assert n_cut > 100, n_cut
+
+
+def test_droplets_per_wave_keep_results(cuda):
+    """The trace's droplets per wave change only the launch shape: three
+    steps at 64, 8 and 1 droplets per wave, and under the autotuner, give the
+    same bits."""
+    N = 64
+    grid, f = stubs.earth_globe(N, 2, n_blobs=16)
+    p = stubs.workload_params(N, 2, max_steps=128, uplift_mode="replay", iterations=3)
+    args = (grid, f["bedrock_m"], f["hardness"], f["precip"], f["evap"], f["uplift_m"], p.erosion)
+    outs = []
+    for lanes in (64, 8, 1, None):
+        st = maps.ErosionState.from_grid(*args, device=cuda)
+        tuner = None
+        if lanes is None:
+            tuner = maps.CheckAutotuner(st, candidates=(64, 8, 1))
+        else:
+            st.check_lanes = lanes
+        for it in range(3):
+            if tuner is not None:
+                tuner.step(lambda k: maps.step(st, p, k), it)
+            else:
+                maps.step(st, p, it)
+        if tuner is not None:
+            assert tuner.best in (64, 8, 1) and st.check_lanes == tuner.best and len(tuner.times) == 3
+        outs.append([t.cpu().numpy() for t in (st.height, st.sediment, st.discharge, st.momentum, st.pending)])
+    for other in outs[1:]:
+        for a, b in zip(outs[0], other):
+            assert np.array_equal(a, b)
