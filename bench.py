@@ -236,9 +236,10 @@ def _budget_exact(d):
     return lhs == int(r[FS["RELEASED_FX"]])
 
 
-def cpu_baseline(grid, f, p, fraction=8):
-    """The oracle on one iteration of the same workload with 1/fraction of
-    the droplets (reference chunking, serial apply), on this host's cores."""
+def cpu_baseline(grid, f, p, fraction=1):
+    """The oracle on one iteration of the same workload (1/fraction of the
+    droplets; the whole spawn by default: ~10-15 s of CPU work), reference
+    chunking and serial apply, on this host's cores."""
     from oracle import host as oracle
 
     # every core this process may use (the host's share: OMP_NUM_THREADS where the host sets one)
@@ -263,8 +264,9 @@ def cpu_baseline(grid, f, p, fraction=8):
     log(f"cpu baseline: {P} droplets, {moves[0]} moves in {dt:.2f}s on {threads} threads "
         f"(trace {s['seconds_trace']:.2f}s, apply {s['seconds_apply']:.2f}s)")
     return {"value": moves[0] / dt, "unit": "droplet-steps/s", "cores": threads, "kind": "port",
-            "sample": f"one iteration of the N_c=1024 workload with {P} droplets (1/{fraction} of the spawn), "
-                      f"reference chunking ({s['chunk']}) and serial apply; {moves[0]} moves in {dt:.2f} s"}
+            "sample": f"one iteration of the N_c=1024 workload with {P} droplets"
+                      f"{'' if fraction == 1 else f' (1/{fraction} of the spawn)'}, reference chunking ({s['chunk']}) and "
+                      f"serial apply; {moves[0]} moves in {dt:.2f} s"}
 
 
 def adjoint_run(N=256, iterations=50, checkpoint_every=10, max_steps=128, seed=0, feedback=False):
@@ -524,7 +526,8 @@ def main():
                    "trace_droplets_per_wave": r["lanes"], "autotune_trace_ns_per_move": r["lane_times"]},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": "trace_check_kernel (one lane per droplet, float64, one launch per 2048-droplet chunk)",
+                     "kernel": f"trace_check_kernel (one lane per droplet, {r['lanes']} droplets per wave, float64, one "
+                               f"launch per 2048-droplet chunk)",
                      "bytes_per_move": B_MOVE_EXACT, "launch_ms_mean": r["trace_ms"] / max(r["launches"], 1),
                      "launches": r["launches"],
                      "trace_share_of_wall": trace_s / r["seconds"]},
