@@ -186,12 +186,14 @@ def fast_run(grid, f, p, iterations, steps, chunks=16, brush=0):
     maps.start_replay(st, p)
     shift = 0.0
     warm = iterations - steps
-    tuner = None
+    cus = torch.cuda.get_device_properties(0).multi_processor_count
     if brush:
         warm = max(warm, 4)  # the autotuner's grid candidate and three tiles
         iterations = warm + steps
-        cus = torch.cuda.get_device_properties(0).multi_processor_count
         tuner = bt.Autotuner(st, candidates=[(4 * cus, 0)], tiles=(0, 1, 2))
+    else:
+        # the persistent grid and register budget over the warm-up iterations, the measured best first
+        tuner = bt.Autotuner(st, candidates=[(10 * cus, 1), (4 * cus, 0), (5 * cus, 1), (12 * cus, 2)][:max(warm, 1)])
     for it in range(warm):
         if tuner is not None and tuner.best is None:
             box = {}
@@ -199,10 +201,12 @@ def fast_run(grid, f, p, iterations, steps, chunks=16, brush=0):
             shift += float(box["s"]["datum_shift"].item())
         else:
             shift += float(maps.step(st, p, it)["datum_shift"].item())
-    if tuner is not None:
-        tuner.finish()
+    tuner.finish()
+    if brush:
         log(f"fast mode brush {brush}: LDS tile {tuner.best_tile} (ms per brush launch per tile {tuner.tile_times})")
-    st.fast_grid_blocks, st.fast_variant = 0, 0
+        st.fast_grid_blocks, st.fast_variant = 0, 0
+    else:
+        log(f"fast mode: (grid, variant) {tuner.best} (trace ns per move {tuner.times})")
     torch.cuda.synchronize()
     fast.trace_time()
     fast.brush_time()
@@ -222,9 +226,9 @@ def fast_run(grid, f, p, iterations, steps, chunks=16, brush=0):
     sf, _ = surface_stats(st)
     log(f"fast mode ({chunks} chunks, brush {brush}): {steps} iterations in {dt:.3f}s, {moves} moves")
     return dict(seconds=dt, moves=moves, trace_ms=trace_ms, launches=launches, s=sf, shift=shift, brush_ms=brush_ms,
-                brush_launches=brush_launches, tile=None if tuner is None else tuner.best_tile,
-                tile_ms=None if tuner is None else tuner.tile_times, cells=int(st.height.numel()),
-                budget_exact=all(_budget_exact(d) for d in dec))
+                brush_launches=brush_launches, tile=tuner.best_tile, tile_ms=tuner.tile_times,
+                cells=int(st.height.numel()), budget_exact=all(_budget_exact(d) for d in dec),
+                grid=tuner.best, grid_ns=tuner.times)
 
 
 def _budget_exact(d):
@@ -543,6 +547,8 @@ def main():
             "value": fr["moves"] / fr["seconds"], "unit": "droplet-steps/s",
             "ms_per_step": 1000.0 * fr["seconds"] / a.steps, "dtype": "f32", "chunks_per_iteration": a.chunks,
             "vs_exact_after_iterations": r["iterations"], "hypsometry_cdf_gap": gap, "eroded_volume_rel": ero,
+            "grid_blocks_variant": list(fr["grid"]) if fr["grid"] else None,
+            "autotune_trace_ns_per_move": {f"{g}/{v}": t for (g, v), t in fr["grid_ns"].items()},
             "meets_tolerance": bool(gap <= 0.005 and abs(ero) <= 0.005),
             "roofline": {"achieved": fach, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": fach / HBM_PEAK_GBS,
                          "bytes_per_move": B_STEP_FAST}}
