@@ -500,9 +500,11 @@ __device__ __forceinline__ ll final_cascade(const TraceCfg& k, const Ring& ring,
                 if (s_c < b_c) lim = fminf(lim, b_c - k.dep_floor - s_c);
                 ll dep = left;
                 ll room = k.dep_budget - k.acc_fx[c];
-                // the loads that stop on this cell share its room in proportion
+                // the loads that stop on this cell share its room in proportion (the stop cell only:
+                // fin_fx counts the loads that stop there, not the overflow reaching older ring cells)
                 const ll demand = k.fin_fx[c];
-                if (room > 0 && demand > room) room = (ll)((double)room * fmin((double)load / (double)demand, 1.0));
+                if (r == 0 && room > 0 && demand > room)
+                    room = (ll)((double)room * fmin((double)load / (double)demand, 1.0));
                 if (dep > room) dep = room;
                 const ll cap = ceiling_fx(lim, k.hscale);
                 if (dep > cap) dep = cap;
