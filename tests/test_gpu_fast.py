@@ -331,8 +331,8 @@ def test_land_deaths_spread_their_load_like_the_oracle(oracle, cuda):
     offers the overflow to the last 8 cells of the path, newest first, each
     under the next-older cell + dep_floor, before parking the rest
     (particle.py:794-815, 964-1040).  The fast mode's stockpile and land
-    deposition after one iteration stay near the oracle's (measured +16 %
-    and -28 %: the serial apply fills a pit cell's room droplet by droplet
+    deposition after one iteration stay near the oracle's (measured +15 %
+    and -25 %: the serial apply fills a pit cell's room droplet by droplet
     and overflows the later ones up their paths, where every fast-mode
     droplet is offered the chunk's frozen room and the per-cell resolve
     parks the excess), and the budget stays exact."""
@@ -362,8 +362,8 @@ def test_land_deaths_spread_their_load_like_the_oracle(oracle, cuda):
           f"deposited oracle {dep_o:.5g} fast {dep_d:.5g}", flush=True)
     land = sum(so["deaths"][c] for c in ("age", "pit", "evap", "stop"))
     assert land > 200 and so["deaths"]["ocean"] == 0
-    assert abs(pend_d - pend_o) <= 0.25 * pend_o
-    assert abs(dep_d - dep_o) <= 0.35 * dep_o
+    assert abs(pend_d - pend_o) <= 0.20 * pend_o  # measured +15 %
+    assert abs(dep_d - dep_o) <= 0.30 * dep_o  # measured -25 %
 
 
 def _hyps_and_eroded(s0, s, bins=64):
