@@ -54,7 +54,7 @@
 // Built with -ffp-contract=off: every expression rounds where the
 // reference's does (its operand order is kept below).
 #include <hip/hip_cooperative_groups.h>
-#include <hipcub/hipcub.hpp>
+#include <rocprim/device/device_radix_sort.hpp>
 
 #include <mutex>
 #include <vector>
@@ -69,6 +69,13 @@ namespace pg {
 namespace {
 
 using ull = unsigned long long;
+
+// The accesses' sort: an LSD radix sort (stable, as the segments' serial order
+// needs) by onesweep at every size.  rocprim's default takes a block sort plus
+// merge passes below 2^20 items -- 21 launches for a chunk's ~1 M slots --
+// where onesweep needs a histogram, its scan and one pass per 8 key bits.
+using AccessSortConfig = rocprim::radix_sort_config<rocprim::default_config, rocprim::default_config,
+                                                    rocprim::default_config, 0>;
 
 constexpr int WAVE = 64;
 constexpr int BLOCK = 256;                  // 4 waves
@@ -133,8 +140,9 @@ int layout_of(int64_t P, int64_t cap, int max_sweeps, Layout* L) {
     L->rmin = take(((na + RANGE - 1) / RANGE + 1) * 4);
     L->ctl = take((size_t)(CTL_CHANGED + max_sweeps + 1) * 4);
     size_t tb = 0;
-    if (hipcub::DeviceRadixSort::SortPairs(nullptr, tb, (const uint32_t*)nullptr, (uint32_t*)nullptr,
-                                           (const uint32_t*)nullptr, (uint32_t*)nullptr, (int)(na > 0 ? na : 1)) != hipSuccess)
+    if (rocprim::radix_sort_pairs<AccessSortConfig>(nullptr, tb, (const uint32_t*)nullptr, (uint32_t*)nullptr,
+                                                    (const uint32_t*)nullptr, (uint32_t*)nullptr,
+                                                    (unsigned int)(na > 0 ? na : 1), 0u, 32u) != hipSuccess)
         return -1;
     L->temp_bytes = tb;
     L->temp = take(tb);
@@ -885,7 +893,7 @@ extern "C" int64_t pg_ordered_workspace_bytes(int64_t P, int64_t cap, int max_sw
     }
     Layout L;
     if (layout_of(P, cap, max_sweeps, &L) != 0) {
-        set_error("pg_ordered_workspace_bytes: hipcub temp-size query failed");
+        set_error("pg_ordered_workspace_bytes: sort temp-size query failed");
         return -1;
     }
     return (int64_t)L.total;
@@ -914,7 +922,7 @@ int ordered_enqueue(const int32_t* cl_cell, const double* cl_delta, const float*
                (long long)(P * cap), IDX_MASK);
     PG_REQUIRE((uint64_t)P * (uint64_t)(3 * cap + 1) < 0x7fffffffull, "pg_apply_changes_ordered: too many accesses");
     Layout L;
-    PG_REQUIRE(layout_of(P, cap, max_sweeps, &L) == 0, "pg_apply_changes_ordered: hipcub temp-size query failed");
+    PG_REQUIRE(layout_of(P, cap, max_sweeps, &L) == 0, "pg_apply_changes_ordered: sort temp-size query failed");
     PG_REQUIRE(work && work_bytes >= (int64_t)L.total, "pg_apply_changes_ordered: workspace of %lld bytes, %lld needed",
                (long long)work_bytes, (long long)L.total);
     hipStream_t s = stream;
@@ -998,8 +1006,9 @@ int ordered_enqueue(const int32_t* cl_cell, const double* cl_delta, const float*
     hipLaunchKernelGGL(ord_build_kernel, dim3(dblocks), dim3(BLOCK), 0, s, a);
     PG_LAUNCHED("ord_build_kernel");
     size_t tb = L.temp_bytes;
-    PG_HIP(hipcub::DeviceRadixSort::SortPairs(at<void>(work, L.temp), tb, a.keys_in, a.keys_out, a.vals_in, a.vals_out,
-                                              (int)a.n_slots, 0, nbits, s));
+    PG_HIP(rocprim::radix_sort_pairs<AccessSortConfig>(at<void>(work, L.temp), tb, a.keys_in, a.keys_out, a.vals_in,
+                                                       a.vals_out, (unsigned int)a.n_slots, 0u, (unsigned int)nbits,
+                                                       s));
     hipLaunchKernelGGL(ord_posw_kernel, dim3(blocks_for(a.n_slots, BLOCK)), dim3(BLOCK), 0, s, a);
     PG_LAUNCHED("ord_posw_kernel");
     const int launched = launched_sweeps < max_sweeps ? launched_sweeps : max_sweeps;
@@ -1174,7 +1183,7 @@ extern "C" int pg_apply_changes_ordered_noalloc(const int32_t* cl_cell, const do
                "pg_apply_changes_ordered_noalloc: bad sizes P=%lld cap=%lld max_sweeps=%d", (long long)P, (long long)cap,
                max_sweeps);
     Layout L;
-    PG_REQUIRE(layout_of(P, cap, max_sweeps, &L) == 0, "pg_apply_changes_ordered_noalloc: hipcub temp-size query failed");
+    PG_REQUIRE(layout_of(P, cap, max_sweeps, &L) == 0, "pg_apply_changes_ordered_noalloc: sort temp-size query failed");
     PG_REQUIRE(L.total <= ORDERED_SCRATCH_BYTES,
                "pg_apply_changes_ordered_noalloc: P=%lld cap=%lld max_sweeps=%d need %llu bytes of scratch, the maximum "
                "is %llu (chunks of up to 2048 droplets with lifetimes up to 128 fit)",
