@@ -52,9 +52,10 @@ typedef struct pg_trace_params {
 
 /* Message of the last failed call on this thread ("" if none). */
 const char* pg_last_error(void);
-/* ABI version of this library: 7 (bumped whenever a struct layout or a
+/* ABI version of this library: 8 (bumped whenever a struct layout or a
  * signature changes; 2 added pg_fast_params.erode_cap, 3 its tape, 6 the
- * brush, 7 the deterministic adjoint's workspace and halo table). */
+ * brush, 7 the deterministic adjoint's workspace and halo table, 8 the
+ * adjoint's tape in pieces). */
 int pg_abi_version(void);
 
 /* ---- geometry, host side (bake/globe/cubesphere.py:70-220) ------------- */
@@ -554,6 +555,15 @@ typedef struct pg_grad_args {
     void* rev_work;             /* reverse: DEVICE scratch of pg_grad_reverse_work_bytes() bytes (the
                                  * fixed-point accumulators that make the sums order-free) */
     int64_t rev_work_bytes;
+    int64_t d_lo, d_hi;         /* forward: trace only the spawn's droplets [d_lo, d_hi), droplet d_lo + i on
+                                 * tape slot i (d_hi <= d_lo: all).  The droplets of an iteration do not
+                                 * interact, so an iteration's tape may be recorded and reversed in pieces */
+    int32_t rev_flags;          /* reverse: bit 0 keeps the accumulators of the previous call, bit 1 leaves
+                                 * them unflushed; pieces reversed with 2, 3, ..., 3, 1 give the whole
+                                 * tape's gradients bit for bit (0: one call per tape) */
+    int32_t reuse_spawn;        /* forward: 1 = the workspace still holds this iteration's packed record
+                                 * and spawn (the previous call was the same iteration from the same
+                                 * steering state): skip them */
 } pg_grad_args;
 
 /* Bytes of the reverse sweep's scratch for F faces of N x N cells, halo H:

@@ -118,6 +118,8 @@ struct GradCfg {
     int* moves_out;
     char* tape;
     ll tape_capacity, tape_stride;
+    ll d_lo, d_hi;  // the droplets traced: [d_lo, d_hi) of the spawn (d_hi <= d_lo: all); droplet d_lo + i is
+                    // tape slot i
 };
 
 template <typename Real>
@@ -180,12 +182,15 @@ __device__ __forceinline__ Real bilerp(const Real* surf, int64_t b, int NE, floa
 template <typename Real>
 __global__ void __launch_bounds__(TRACE_TPB) grad_forward_kernel(GradCfg<Real> k) {
     const int lane = threadIdx.x & 63;
-    const ll n_total = k.sc->n_total;
+    // the spawn's droplet count sets the trace caps' concentration; a range traces only its droplets
+    const ll n_spawn = k.sc->n_total;
+    const ll d_lo = k.d_hi > k.d_lo ? (k.d_lo < n_spawn ? k.d_lo : n_spawn) : 0;
+    const ll n_total = k.d_hi > k.d_lo ? (k.d_hi < n_spawn ? k.d_hi : n_spawn) - d_lo : n_spawn;
     const double volume0 = k.sc->volume0;
     if (volume0 <= 0.0) return;
     const float inv_v0 = (float)(1.0 / volume0);
     const float min_volume = (float)(k.min_volume_frac * volume0);
-    const Real conc_scale = (Real)((k.conc_chunk > 0.0 ? k.conc_chunk : (double)n_total) / (volume0 * (double)n_total));
+    const Real conc_scale = (Real)((k.conc_chunk > 0.0 ? k.conc_chunk : (double)n_spawn) / (volume0 * (double)n_spawn));
     const Real v0 = (Real)volume0;
     const float v0f = (float)volume0;
     ll* queue = &k.sc->queue[0];
@@ -245,26 +250,27 @@ __global__ void __launch_bounds__(TRACE_TPB) grad_forward_kernel(GradCfg<Real> k
                 if (!alive) {
                     d = base + __popcll(dead & ((1ull << lane) - 1));
                     if (d < n_total) {
-                        const int c0 = k.d_cell[d];
+                        const ll dd = d_lo + d;  // the droplet's index in the spawn (d: its tape slot)
+                        const int c0 = k.d_cell[dd];
                         f = c0 / NE2;
                         const int r = c0 - f * NE2;
                         ix = r / k.NE - k.H;
                         iy = r % k.NE - k.H;
-                        fx = k.d_fx[d];
-                        fy = k.d_fy[d];
-                        real = k.d_load[d];
+                        fx = k.d_fx[dd];
+                        fy = k.d_fy[dd];
+                        real = k.d_load[dd];
                         vol_s = v0f;
                         vol = v0;
                         sed = (Real)((double)real / k.hscale);
-                        sa = k.d_sa[d];
-                        sb = k.d_sb[d];
+                        sa = k.d_sa[dd];
+                        sb = k.d_sb[dd];
                         has_prev = false;
                         in_sea = false;
                         step = uphill = sea_steps = pushes = 0;
                         last_cell = -1;
                         if (replay) {
-                            fixed = k.fixed_moves[d] & 0xffffff;
-                            fixed_cause = (k.fixed_moves[d] >> 24) & 0xff;
+                            fixed = k.fixed_moves[dd] & 0xffffff;
+                            fixed_cause = (k.fixed_moves[dd] >> 24) & 0xff;
                         }
                         if (taping && d < k.tape_capacity) tp.drops[d].sed0 = (double)sed;
                         S = load_stencil(k.cell4, k.NE, k.H, f, ix, iy, fx, fy);
@@ -434,7 +440,7 @@ __global__ void __launch_bounds__(TRACE_TPB) grad_forward_kernel(GradCfg<Real> k
         } else {
             vanished_fx += real;
         }
-        if (k.moves_out) k.moves_out[d] = step | (cause << 24);
+        if (k.moves_out) k.moves_out[d_lo + d] = step | (cause << 24);
         if (taping && d < k.tape_capacity) {
             TapeDroplet td;
             td.moves = step;
@@ -481,19 +487,36 @@ struct Fx128 {
     ll hi;
 };
 
-__device__ __forceinline__ void fx_add(Fx128* a, double v) {
-    if (v == 0.0) return;
+// v as a two's-complement 128-bit integer (H, L) of unit 2^-64
+__device__ __forceinline__ void fx_split(double v, ull& H, ull& L) {
     const double m = fabs(v);
     const double fh = floor(m);
-    ull H = (ull)fh;
-    ull L = (ull)((m - fh) * 18446744073709551616.0);  // m - fh is exact; the scaling by 2^64 too
+    H = (ull)fh;
+    L = (ull)((m - fh) * 18446744073709551616.0);  // m - fh is exact; the scaling by 2^64 too
     if (v < 0.0) {  // two's complement of (H, L)
         L = ~L + 1ull;
         H = ~H + (L == 0ull ? 1ull : 0ull);
     }
-    const ull old = atomicAdd(&a->lo, L);
+}
+
+// (H, L) += (h, l) modulo 2^128: integer sums, so any grouping gives the same bits
+__device__ __forceinline__ void fx_acc(ull& H, ull& L, ull h, ull l) {
+    const ull s = L + l;
+    H += h + (s < L ? 1ull : 0ull);
+    L = s;
+}
+
+__device__ __forceinline__ void fx_add_raw(Fx128* a, ull H, ull L) {
+    const ull old = L ? atomicAdd(&a->lo, L) : 0ull;
     const ull carry = old + L < old ? 1ull : 0ull;
     if (H + carry) atomicAdd((ull*)&a->hi, H + carry);
+}
+
+__device__ __forceinline__ void fx_add(Fx128* a, double v) {
+    if (v == 0.0) return;
+    ull H, L;
+    fx_split(v, H, L);
+    fx_add_raw(a, H, L);
 }
 
 // the accumulated sum, in units of 1: hi + lo 2^-64
@@ -519,6 +542,10 @@ struct RevCfg {
 };
 
 __device__ __forceinline__ void add_d(Fx128* p, double v) { fx_add(p, v); }
+
+__global__ void queue_reset_kernel(Scalars* sc) {
+    if (threadIdx.x < 2) sc->queue[threadIdx.x] = 0;
+}
 
 __global__ void fx_zero_kernel(int64_t n, Fx128* acc) {
     for (int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; c < n; c += (int64_t)gridDim.x * blockDim.x) {
@@ -565,11 +592,14 @@ __global__ void __launch_bounds__(TPB) grad_reverse_kernel(RevCfg<Real> r) {
     const ll n_total = tp.head->n_total < r.capacity ? tp.head->n_total : r.capacity;
     const ll stride = tp.head->stride;
     const Real v0 = (Real)tp.head->volume0, conc_scale = (Real)tp.head->conc_scale;
-    Real gt[N_THETA] = {0, 0, 0, 0, 0, 0};
+    // the scalars' adjoints: summed per droplet in Real, then as 128-bit integers (fixed point), so the
+    // totals do not depend on which droplets share a lane or a wave (a tape in pieces gives the same bits)
+    ull tH[N_THETA] = {0, 0, 0, 0, 0, 0}, tL[N_THETA] = {0, 0, 0, 0, 0, 0};
     for (ll d = (ll)blockIdx.x * blockDim.x + threadIdx.x; d < n_total; d += (ll)gridDim.x * blockDim.x) {
         const TapeDroplet td = tp.drops[d];
         const int K = td.moves;
         if (K <= 0) continue;
+        Real gt[N_THETA] = {0, 0, 0, 0, 0, 0};
         const TapeRec<Real>* rec = tp.recs + d * stride;
         const Real mu_final = td.deposited ? (Real)r.g[td.last] : (Real)0;
         Real a_sed = 0, a_vol = 0;
@@ -697,11 +727,16 @@ __global__ void __launch_bounds__(TPB) grad_reverse_kernel(RevCfg<Real> r) {
             rc = rp;
         }
         if (r.feedback) scatter(r, rn, st_adj, cell_adj);
+        for (int c = 0; c < N_THETA; c++) {
+            ull h, l;
+            fx_split((double)gt[c], h, l);
+            fx_acc(tH[c], tL[c], h, l);
+        }
     }
     for (int c = 0; c < N_THETA; c++) {
-        double v = (double)gt[c];
-        for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
-        if ((threadIdx.x & 63) == 0) add_d(&r.grad_theta[c], v);
+        ull h = tH[c], l = tL[c];
+        for (int o = 32; o > 0; o >>= 1) fx_acc(h, l, __shfl_xor(h, o), __shfl_xor(l, o));
+        if ((threadIdx.x & 63) == 0 && (h | l)) fx_add_raw(&r.grad_theta[c], h, l);
     }
 }
 
@@ -769,10 +804,14 @@ int launch_forward(const pg_fast_fields* steer, const pg_trace_params* prm, cons
         const size_t per = sizeof(TapeDroplet) + (size_t)k.tape_stride * sizeof(TapeRec<Real>);
         ll cap = (ll)(((size_t)ga->tape_bytes - up(sizeof(TapeHead))) / per);
         while (cap > 0 && tape_bytes<Real>(cap, k.tape_stride) > (size_t)ga->tape_bytes) cap--;
-        PG_REQUIRE(cap >= c.P, "grad mode: a tape of %lld bytes holds %lld droplets, the spawn has %lld",
-                   (long long)ga->tape_bytes, (long long)cap, (long long)c.P);
+        const ll want = ga->d_hi > ga->d_lo ? (ll)(ga->d_hi - ga->d_lo) : (ll)c.P;
+        PG_REQUIRE(cap >= want, "grad mode: a tape of %lld bytes holds %lld droplets, %lld are traced",
+                   (long long)ga->tape_bytes, (long long)cap, (long long)want);
         k.tape_capacity = cap;
     }
+    PG_REQUIRE(ga->d_lo >= 0, "grad mode: d_lo < 0");
+    k.d_lo = ga->d_lo;
+    k.d_hi = ga->d_hi;
     int grid = 0;
     if (persistent_grid((const void*)grad_forward_kernel<Real>, TRACE_TPB, fp->grid_blocks, &grid)) return -1;
     hipLaunchKernelGGL(grad_forward_kernel<Real>, dim3(grid), dim3(TRACE_TPB), 0, c.s, k);
@@ -815,16 +854,22 @@ int launch_reverse(int64_t F, int64_t N, int64_t H, const pg_trace_params* prm, 
     r.k_disc = (Real)ga->theta[3];
     r.ocean_rate = (Real)ga->theta[4];
     r.dep_floor = (Real)ga->theta[5];
-    const int zgrid = grid_for(2 * n + N_THETA, 65535);
-    hipLaunchKernelGGL(fx_zero_kernel, dim3(zgrid), dim3(TPB), 0, s, 2 * n + N_THETA, acc);
+    // rev_flags bit 0: keep the accumulators of the previous call (a tape in pieces); bit 1: leave them
+    // unflushed for the next call -- the pieces' sum is then exactly the whole tape's
+    if (!(ga->rev_flags & 1)) {
+        const int zgrid = grid_for(2 * n + N_THETA, 65535);
+        hipLaunchKernelGGL(fx_zero_kernel, dim3(zgrid), dim3(TPB), 0, s, 2 * n + N_THETA, acc);
+    }
     const int grid = grid_for(cap, 65535);
     hipLaunchKernelGGL(grad_reverse_kernel<Real>, dim3(grid), dim3(TPB), 0, s, r);
     PG_LAUNCHED("grad_reverse_kernel");
-    const int fgrid = grid_for(n, 65535);
-    hipLaunchKernelGGL(fx_flush_kernel, dim3(fgrid), dim3(TPB), 0, s, n, acc, lam);
-    hipLaunchKernelGGL(fx_flush_kernel, dim3(fgrid), dim3(TPB), 0, s, n, acc + n, grad_erod);
-    hipLaunchKernelGGL(fx_flush_kernel, dim3(1), dim3(64), 0, s, (int64_t)N_THETA, acc + 2 * n, grad_theta);
-    PG_LAUNCHED("fx_flush_kernel");
+    if (!(ga->rev_flags & 2)) {
+        const int fgrid = grid_for(n, 65535);
+        hipLaunchKernelGGL(fx_flush_kernel, dim3(fgrid), dim3(TPB), 0, s, n, acc, lam);
+        hipLaunchKernelGGL(fx_flush_kernel, dim3(fgrid), dim3(TPB), 0, s, n, acc + n, grad_erod);
+        hipLaunchKernelGGL(fx_flush_kernel, dim3(1), dim3(64), 0, s, (int64_t)N_THETA, acc + 2 * n, grad_theta);
+        PG_LAUNCHED("fx_flush_kernel");
+    }
     return 0;
 }
 
@@ -865,7 +910,13 @@ extern "C" int pg_grad_iterate(const pg_fast_fields* steer, const pg_fast_fields
     PG_REQUIRE(target->height && target->sediment && target->discharge && target->momentum && target->pending &&
                    target->mask,
                "pg_grad_iterate: NULL target field");
-    if (fast_spawn(steer, prm, fp, work, c)) return -1;
+    if (ga->reuse_spawn) {
+        // the workspace holds this iteration's record and spawn from the previous call: only the queue restarts
+        hipLaunchKernelGGL(queue_reset_kernel, dim3(1), dim3(64), 0, c.s, c.sc);
+        PG_LAUNCHED("queue_reset_kernel");
+    } else if (fast_spawn(steer, prm, fp, work, c)) {
+        return -1;
+    }
     if (ga->real ? launch_forward<double>(steer, prm, fp, ga, work, c) : launch_forward<float>(steer, prm, fp, ga, work, c))
         return -1;
     return finish_iteration(target, fp, work, c);

@@ -52,7 +52,8 @@ class GradArgs(ctypes.Structure):
 
     _fields_ = [("real", ctypes.c_int32), ("feedback", ctypes.c_int32), ("surf", vp), ("erod", vp), ("theta", dbl * 6),
                 ("tape", vp), ("tape_bytes", i64), ("fixed_moves", vp), ("moves_out", vp), ("rev_work", vp),
-                ("rev_work_bytes", i64)]
+                ("rev_work_bytes", i64), ("d_lo", i64), ("d_hi", i64), ("rev_flags", ctypes.c_int32),
+                ("reuse_spawn", ctypes.c_int32)]
 
 
 #: statistics slots of a fast iteration (include/pg_erode.h, PG_FS_*)

@@ -67,11 +67,20 @@ class GradRun:
     answers the same surface change).  False holds that surface at the
     nominal run's (a replay then reads the nominal surface too), so the
     surface adjoint passes each particle pass unchanged: the derivative of
-    the frozen-terrain map, stable at any horizon."""
+    the frozen-terrain map, stable at any horizon.
+
+    tape_batch: record each iteration's tape in pieces of this many droplets
+    (None: one tape of the whole spawn).  The droplets of an iteration do
+    not interact, so the backward pass then re-traces the iteration one
+    piece at a time from the iteration's starting state and reverses each
+    piece into the same fixed-point accumulators: the gradients are the
+    whole tape's bit for bit, at one piece's memory (what the N_c=8192
+    planet needs).  The forward pass records no tapes; the backward pass
+    keeps the starting states of one checkpoint segment."""
 
     def __init__(self, state, params, iterations: int, real: str = "f32", checkpoint_every: int = 10,
                  first_key: int = 0, rng_stage: str = "erosion", n_particles=None, replay_of: "GradRun | None" = None,
-                 feedback: bool = True):
+                 feedback: bool = True, tape_batch: "int | None" = None):
         from .maps import (_check_scope, _eparams, discharge_scales, max_steps_of, require_plain_side_passes,
                            trace_params, uplift_cap)
 
@@ -135,9 +144,12 @@ class GradRun:
         fp = FastParams(n_particles=self.P, chunks=1, height_bits=self.height_bits, track_bits=TRACK_BITS)
         check(lib().pg_fast_setup(ctypes.byref(self._fields(state)), ctypes.byref(fp), ctypes.c_void_p(self.work.data_ptr()),
                                   nbytes, pk._stream()), "pg_fast_setup")
+        if tape_batch is not None and int(tape_batch) < 1:
+            raise ValueError("tape_batch must be >= 1 droplet")
+        self.tape_batch = None if tape_batch is None or int(tape_batch) >= self.P else int(tape_batch)
         tb = ctypes.c_int64()
-        check(lib().pg_grad_tape_bytes(self.P, self.max_steps, 1 if real == "f64" else 0, ctypes.byref(tb)),
-              "pg_grad_tape_bytes")
+        check(lib().pg_grad_tape_bytes(self.tape_batch or self.P, self.max_steps, 1 if real == "f64" else 0,
+                                       ctypes.byref(tb)), "pg_grad_tape_bytes")
         self.tape_bytes = int(tb.value)
         self._tapes: dict[int, torch.Tensor] = {}
         self._taped: dict[int, int] = {}  # slot -> iteration whose tape it holds
@@ -238,6 +250,8 @@ class GradRun:
         if record:
             self.nominal, self.moves = [], []
         last = ((T - 1) // ce) * ce
+        if self.tape_batch is not None:
+            tapes = "none"  # the backward pass re-traces piece by piece
         for t in range(T):
             if t % ce == 0:
                 self.checkpoints[t] = _snapshot(target)
@@ -280,6 +294,8 @@ class GradRun:
         g_erod = torch.zeros(st.height.shape, dtype=torch.float64, device=st.device)
         g_theta = torch.zeros(6, dtype=torch.float64, device=st.device)
         segs = [(a, min(a + ce, T)) for a in range(0, T, ce)]
+        if self.tape_batch is not None:
+            return self._backward_pieces(lam, segs, erod, theta, g_erod, g_theta)
         for a, b in reversed(segs):
             slot_of = (lambda t: t) if self._tapes_all else (lambda t: t - a)
             if not all(self._taped.get(slot_of(t)) == t for t in range(a, b)):
@@ -293,7 +309,48 @@ class GradRun:
                 lam = self._reverse(lam, self._tape(slot_of(t)), self.idx[t], erod, theta, g_erod, g_theta)
         return g_erod, g_theta, lam
 
-    def _reverse(self, lam, tape, idx, erod, theta, g_erod, g_theta):
+    def _backward_pieces(self, lam, segs, erod, theta, g_erod, g_theta):
+        """backward() with the tape in pieces of tape_batch droplets."""
+        pieces = [(lo, min(lo + self.tape_batch, self.P)) for lo in range(0, self.P, self.tape_batch)]
+        tape = self._tape(0)
+        scratch = None
+        for a, b in reversed(segs):
+            target = _snapshot(self.checkpoints[a])
+            starts = []
+            for t in range(a, b):
+                starts.append(_snapshot(target))
+                if t < b - 1:
+                    steer, fixed, _ = self._steer(t, target)
+                    self._iterate(target, t, erod, theta, steer=steer, fixed=fixed, idx=self.idx[t])
+            del target
+            for t in range(b - 1, a - 1, -1):
+                start = starts[t - a]
+                steer, fixed, _ = self._steer(t, start)
+                src = start if self.feedback else steer
+                surf = (src["height"] + src["sediment"]).to(self.dtype)
+                if scratch is None:
+                    scratch = _snapshot(start)
+
+                def record(k, t=t, steer=steer, fixed=fixed, surf=surf):
+                    # piece k of iteration t's tape; the target is scratch (only the tape is kept)
+                    ga = self._args(surf, erod, theta, tape, fixed)
+                    ga.d_lo, ga.d_hi = pieces[k]
+                    ga.reuse_spawn = 1 if k else 0
+                    check(lib().pg_grad_iterate(ctypes.byref(self._fields(steer)), ctypes.byref(self._fields(scratch)),
+                                                ctypes.byref(self.prm), ctypes.byref(self._fparams(t)),
+                                                ctypes.byref(ga), ctypes.c_void_p(self.work.data_ptr()),
+                                                self.work.numel(), ctypes.c_void_p(self.stats.data_ptr()),
+                                                pk._stream()), "pg_grad_iterate")
+
+                lam = self._reverse(lam, tape, self.idx[t], erod, theta, g_erod, g_theta, record=record,
+                                    n_pieces=len(pieces))
+                del starts[t - a:]
+        return g_erod, g_theta, lam
+
+    def _reverse(self, lam, tape, idx, erod, theta, g_erod, g_theta, record=None, n_pieces: int = 1):
+        """One iteration's reverse sweep.  With record: the tape in n_pieces
+        pieces, record(k) writing piece k; the pieces accumulate in the
+        fixed-point sums and are flushed once, after the last."""
         st = self.state
         if st.spherical:
             ht = self.halo
@@ -311,8 +368,14 @@ class GradRun:
             self._rev_work = torch.empty(nbytes, dtype=torch.uint8, device=st.device)
         ga.rev_work = self._rev_work.data_ptr()
         ga.rev_work_bytes = self._rev_work.numel()
-        check(lib().pg_grad_reverse(st.F, st.N, st.H, ctypes.byref(self.prm), ctypes.byref(ga), pk._ptr(lam),
-                                    pk._ptr(out), pk._ptr(g_erod), pk._ptr(g_theta), pk._stream()), "pg_grad_reverse")
+        for k in range(n_pieces):
+            if record is not None:
+                record(k)
+            # bit 0: keep the earlier pieces' sums; bit 1: more pieces follow, flush after the last
+            ga.rev_flags = (1 if k else 0) | (2 if k < n_pieces - 1 else 0)
+            check(lib().pg_grad_reverse(st.F, st.N, st.H, ctypes.byref(self.prm), ctypes.byref(ga), pk._ptr(lam),
+                                        pk._ptr(out), pk._ptr(g_erod), pk._ptr(g_theta), pk._stream()),
+                  "pg_grad_reverse")
         return out
 
 
