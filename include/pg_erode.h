@@ -556,7 +556,8 @@ typedef struct pg_grad_args {
                                  * fixed-point accumulators that make the sums order-free) */
     int64_t rev_work_bytes;
     int64_t d_lo, d_hi;         /* forward: trace only the spawn's droplets [d_lo, d_hi), droplet d_lo + i on
-                                 * tape slot i (d_hi <= d_lo: all).  The droplets of an iteration do not
+                                 * tape slot i, and record the tape only: the target is not written
+                                 * (d_hi <= d_lo: all droplets, applied).  The droplets of an iteration do not
                                  * interact, so an iteration's tape may be recorded and reversed in pieces */
     int32_t rev_flags;          /* reverse: bit 0 keeps the accumulators of the previous call, bit 1 leaves
                                  * them unflushed; pieces reversed with 2, 3, ..., 3, 1 give the whole

@@ -120,6 +120,7 @@ struct GradCfg {
     ll tape_capacity, tape_stride;
     ll d_lo, d_hi;  // the droplets traced: [d_lo, d_hi) of the spawn (d_hi <= d_lo: all); droplet d_lo + i is
                     // tape slot i
+    int apply;      // 0: record the tape only (a range), no cell accumulators
 };
 
 template <typename Real>
@@ -348,7 +349,7 @@ __global__ void __launch_bounds__(TRACE_TPB) grad_forward_kernel(GradCfg<Real> k
                                                         k.erodibility, k.rate, k.k_disc, k.dep_floor, k.inv_sat,
                                                         k.disc_exponent, k.max_erode);
                             flags |= F_LAND;
-                            if (!replay) {
+                            if (!replay && k.apply) {
                                 atomicAdd((ull*)&k.dis_fx[cell], (ull)llrint((double)vol_s * k.tscale));
                                 atomicAdd((ull*)&k.mom_fx[cell], (ull)llrint((double)(vol_s * sa) * k.tscale));
                                 atomicAdd((ull*)&k.mom_fx[k.n_cells + cell], (ull)llrint((double)(vol_s * sb) * k.tscale));
@@ -374,7 +375,7 @@ __global__ void __launch_bounds__(TRACE_TPB) grad_forward_kernel(GradCfg<Real> k
                         }
                         if (flags & (F_LAND | F_SEA)) {
                             const ll dq = llrint((double)(cdiff * vr) * k.hscale);  // > 0: taken from the cell
-                            if (dq) atomicAdd((ull*)&k.acc_fx[cell], (ull)(-dq));
+                            if (dq && k.apply) atomicAdd((ull*)&k.acc_fx[cell], (ull)(-dq));
                             real += dq;
                             sed += cdiff;
                             n_entries += 1;
@@ -433,7 +434,7 @@ __global__ void __launch_bounds__(TRACE_TPB) grad_forward_kernel(GradCfg<Real> k
         }
         int deposited = 0;
         if (pushes > 0 && (!exited || k.deposit_on_exit)) {
-            if (real) atomicAdd((ull*)&k.acc_fx[last_cell], (ull)real);
+            if (real && k.apply) atomicAdd((ull*)&k.acc_fx[last_cell], (ull)real);
             deposited = 1;
         } else if (exited) {
             exit_fx += real;
@@ -812,6 +813,7 @@ int launch_forward(const pg_fast_fields* steer, const pg_trace_params* prm, cons
     PG_REQUIRE(ga->d_lo >= 0, "grad mode: d_lo < 0");
     k.d_lo = ga->d_lo;
     k.d_hi = ga->d_hi;
+    k.apply = ga->d_hi > ga->d_lo ? 0 : 1;
     int grid = 0;
     if (persistent_grid((const void*)grad_forward_kernel<Real>, TRACE_TPB, fp->grid_blocks, &grid)) return -1;
     hipLaunchKernelGGL(grad_forward_kernel<Real>, dim3(grid), dim3(TRACE_TPB), 0, c.s, k);
@@ -910,6 +912,7 @@ extern "C" int pg_grad_iterate(const pg_fast_fields* steer, const pg_fast_fields
     PG_REQUIRE(target->height && target->sediment && target->discharge && target->momentum && target->pending &&
                    target->mask,
                "pg_grad_iterate: NULL target field");
+    PG_REQUIRE(ga->d_hi <= ga->d_lo || ga->tape, "pg_grad_iterate: a droplet range records a tape and needs one");
     if (ga->reuse_spawn) {
         // the workspace holds this iteration's record and spawn from the previous call: only the queue restarts
         hipLaunchKernelGGL(queue_reset_kernel, dim3(1), dim3(64), 0, c.s, c.sc);
@@ -919,6 +922,7 @@ extern "C" int pg_grad_iterate(const pg_fast_fields* steer, const pg_fast_fields
     }
     if (ga->real ? launch_forward<double>(steer, prm, fp, ga, work, c) : launch_forward<float>(steer, prm, fp, ga, work, c))
         return -1;
+    if (ga->d_hi > ga->d_lo) return 0;  // a range records its tape only: the target is not written
     return finish_iteration(target, fp, work, c);
 }
 

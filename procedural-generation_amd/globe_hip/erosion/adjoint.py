@@ -70,7 +70,8 @@ class GradRun:
     the frozen-terrain map, stable at any horizon.
 
     tape_batch: record each iteration's tape in pieces of this many droplets
-    (None: one tape of the whole spawn).  The droplets of an iteration do
+    (None: one tape of the whole spawn; "auto": pieces sized at the backward
+    pass to the device memory then free).  The droplets of an iteration do
     not interact, so the backward pass then re-traces the iteration one
     piece at a time from the iteration's starting state and reverses each
     piece into the same fixed-point accumulators: the gradients are the
@@ -144,20 +145,47 @@ class GradRun:
         fp = FastParams(n_particles=self.P, chunks=1, height_bits=self.height_bits, track_bits=TRACK_BITS)
         check(lib().pg_fast_setup(ctypes.byref(self._fields(state)), ctypes.byref(fp), ctypes.c_void_p(self.work.data_ptr()),
                                   nbytes, pk._stream()), "pg_fast_setup")
-        if tape_batch is not None and int(tape_batch) < 1:
-            raise ValueError("tape_batch must be >= 1 droplet")
-        self.tape_batch = None if tape_batch is None or int(tape_batch) >= self.P else int(tape_batch)
-        tb = ctypes.c_int64()
-        check(lib().pg_grad_tape_bytes(self.tape_batch or self.P, self.max_steps, 1 if real == "f64" else 0,
-                                       ctypes.byref(tb)), "pg_grad_tape_bytes")
-        self.tape_bytes = int(tb.value)
         self._tapes: dict[int, torch.Tensor] = {}
+        self._auto_batch = tape_batch == "auto"
+        self.set_tape_batch(None if self._auto_batch else tape_batch)
         self._taped: dict[int, int] = {}  # slot -> iteration whose tape it holds
         self.s0 = _snapshot(state)
         self.nominal = None
         self.moves = None
         self.idx: list = [None] * self.T
         self.checkpoints: dict[int, dict] = {}
+
+    def _tape_bytes_for(self, droplets: int) -> int:
+        tb = ctypes.c_int64()
+        check(lib().pg_grad_tape_bytes(int(droplets), self.max_steps, 1 if self.real == "f64" else 0, ctypes.byref(tb)),
+              "pg_grad_tape_bytes")
+        return int(tb.value)
+
+    def set_tape_batch(self, tape_batch) -> None:
+        """Droplets per tape piece (None: the whole spawn); drops the tapes held."""
+        if tape_batch is not None and int(tape_batch) < 1:
+            raise ValueError("tape_batch must be >= 1 droplet")
+        self.tape_batch = None if tape_batch is None or int(tape_batch) >= self.P else int(tape_batch)
+        self.tape_bytes = self._tape_bytes_for(self.tape_batch or self.P)
+        self._tapes = {}
+        self._taped = {}
+
+    def _size_auto_batch(self) -> None:
+        """tape_batch="auto": one piece as large as the device's free memory
+        allows once the backward pass's other buffers are set aside."""
+        st = self.state
+        # free on the device plus what torch's cache holds unused
+        free = int(torch.cuda.mem_get_info(st.device)[0]) + int(torch.cuda.memory_reserved(st.device)
+                                                                 - torch.cuda.memory_allocated(st.device))
+        n = st.height.numel()
+        snap = sum(t.numel() * t.element_size() for t in self.s0.values())
+        reserve = (int(lib().pg_grad_reverse_work_bytes(st.F, st.N, st.H)) + 6 * 8 * n + (self.ce - 1) * snap
+                   + (2 << 30))
+        per = self._tape_bytes_for(2) - self._tape_bytes_for(1)
+        fit = (free - reserve) // max(per, 1)
+        if fit < 1:
+            raise MemoryError(f"no room for a tape piece: {free / 1e9:.1f} GB free, {reserve / 1e9:.1f} GB set aside")
+        self.set_tape_batch(min(fit, self.P))
 
     # -- plumbing ------------------------------------------------------------
     def _fields(self, s) -> FastFields:
@@ -250,14 +278,16 @@ class GradRun:
         if record:
             self.nominal, self.moves = [], []
         last = ((T - 1) // ce) * ce
-        if self.tape_batch is not None:
+        if self.tape_batch is not None or self._auto_batch:
             tapes = "none"  # the backward pass re-traces piece by piece
         for t in range(T):
             if t % ce == 0:
-                self.checkpoints[t] = _snapshot(target)
+                # the first checkpoint is the starting state itself (nothing writes either)
+                self.checkpoints[t] = self.s0 if t == 0 else _snapshot(target)
             mv = None
             if record:
-                self.nominal.append(_snapshot(target))
+                # a nominal state is read only: the first is the starting state itself
+                self.nominal.append(self.s0 if t == 0 else _snapshot(target))
                 mv = torch.zeros(self.P, dtype=torch.int32, device=self.state.device)
                 self.moves.append(mv)
             tape = None
@@ -294,7 +324,10 @@ class GradRun:
         g_erod = torch.zeros(st.height.shape, dtype=torch.float64, device=st.device)
         g_theta = torch.zeros(6, dtype=torch.float64, device=st.device)
         segs = [(a, min(a + ce, T)) for a in range(0, T, ce)]
-        if self.tape_batch is not None:
+        if self._auto_batch:
+            self._tapes = {}
+            self._size_auto_batch()
+        if self.tape_batch is not None or self._auto_batch:
             return self._backward_pieces(lam, segs, erod, theta, g_erod, g_theta)
         for a, b in reversed(segs):
             slot_of = (lambda t: t) if self._tapes_all else (lambda t: t - a)
@@ -311,32 +344,31 @@ class GradRun:
 
     def _backward_pieces(self, lam, segs, erod, theta, g_erod, g_theta):
         """backward() with the tape in pieces of tape_batch droplets."""
-        pieces = [(lo, min(lo + self.tape_batch, self.P)) for lo in range(0, self.P, self.tape_batch)]
+        step = self.tape_batch or self.P
+        pieces = [(lo, min(lo + step, self.P)) for lo in range(0, self.P, step)]
         tape = self._tape(0)
-        scratch = None
         for a, b in reversed(segs):
-            target = _snapshot(self.checkpoints[a])
-            starts = []
-            for t in range(a, b):
-                starts.append(_snapshot(target))
-                if t < b - 1:
+            # the segment's starting states: the checkpoint itself (read only), then copies
+            starts = [self.checkpoints[a]]
+            if b - a > 1:
+                target = _snapshot(self.checkpoints[a])
+                for t in range(a, b - 1):
                     steer, fixed, _ = self._steer(t, target)
                     self._iterate(target, t, erod, theta, steer=steer, fixed=fixed, idx=self.idx[t])
-            del target
+                    starts.append(_snapshot(target))
+                del target
             for t in range(b - 1, a - 1, -1):
                 start = starts[t - a]
                 steer, fixed, _ = self._steer(t, start)
                 src = start if self.feedback else steer
                 surf = (src["height"] + src["sediment"]).to(self.dtype)
-                if scratch is None:
-                    scratch = _snapshot(start)
 
                 def record(k, t=t, steer=steer, fixed=fixed, surf=surf):
-                    # piece k of iteration t's tape; the target is scratch (only the tape is kept)
+                    # piece k of iteration t's tape: a range records the tape only and writes no target
                     ga = self._args(surf, erod, theta, tape, fixed)
                     ga.d_lo, ga.d_hi = pieces[k]
                     ga.reuse_spawn = 1 if k else 0
-                    check(lib().pg_grad_iterate(ctypes.byref(self._fields(steer)), ctypes.byref(self._fields(scratch)),
+                    check(lib().pg_grad_iterate(ctypes.byref(self._fields(steer)), ctypes.byref(self._fields(steer)),
                                                 ctypes.byref(self.prm), ctypes.byref(self._fparams(t)),
                                                 ctypes.byref(ga), ctypes.c_void_p(self.work.data_ptr()),
                                                 self.work.numel(), ctypes.c_void_p(self.stats.data_ptr()),
@@ -417,6 +449,33 @@ def beta25_cdf(x: torch.Tensor) -> torch.Tensor:
     return 1.0 - (1.0 - x) ** 6 - 6.0 * x * (1.0 - x) ** 5
 
 
+class _SmoothCDF(torch.autograd.Function):
+    """cdf_k = mean_i sigmoid((z_k - h_i) / tau), evaluated over the cells in
+    slices (a levels x cells matrix of the planet's 121 M land cells would
+    take 62 GB); the backward pass recomputes the slices."""
+
+    SLICE = 1 << 22
+
+    @staticmethod
+    def forward(ctx, h, z, tau):
+        cdf = torch.zeros_like(z)
+        for a in range(0, h.numel(), _SmoothCDF.SLICE):
+            cdf += torch.sigmoid((z[:, None] - h[None, a:a + _SmoothCDF.SLICE]) / tau).sum(dim=1)
+        ctx.save_for_backward(h, z)
+        ctx.tau = tau
+        return cdf / max(h.numel(), 1)
+
+    @staticmethod
+    def backward(ctx, g):
+        h, z = ctx.saved_tensors
+        tau, n = ctx.tau, max(h.numel(), 1)
+        gh = torch.empty_like(h)
+        for a in range(0, h.numel(), _SmoothCDF.SLICE):
+            sg = torch.sigmoid((z[:, None] - h[None, a:a + _SmoothCDF.SLICE]) / tau)
+            gh[a:a + _SmoothCDF.SLICE] = -(g[:, None] * sg * (1.0 - sg)).sum(dim=0) / (tau * n)
+        return gh, None, None
+
+
 def land_hypsometry_loss(h_m: torch.Tensor, land: torch.Tensor, bins: int = 64, zmax_m: float = 4000.0,
                          mean_m: float = 800.0, weight: float = 0.1, tau_m: float | None = None):
     """L2 misfit of the land hypsometric CDF (``bins`` levels z_k = k zmax /
@@ -427,7 +486,7 @@ def land_hypsometry_loss(h_m: torch.Tensor, land: torch.Tensor, bins: int = 64, 
     hl = h_m[land]
     z = torch.arange(1, bins + 1, dtype=h_m.dtype, device=h_m.device) * (zmax_m / bins)
     tau = tau_m if tau_m is not None else 0.5 * zmax_m / bins
-    cdf = torch.sigmoid((z[:, None] - hl[None, :]) / tau).mean(dim=1)
+    cdf = _SmoothCDF.apply(hl, z, float(tau))
     target = beta25_cdf(z / zmax_m)
     return ((cdf - target) ** 2).sum() + weight * (hl.mean() - mean_m) ** 2
 

@@ -212,3 +212,6 @@ def test_tape_in_pieces_gives_the_whole_tapes_gradients(cuda, real):
     assert torch.equal(out[0][1], out[1][1])
     assert torch.equal(out[0][2], out[1][2])
     assert float(out[0][1].abs().max()) > 0 and float(out[0][2].abs().max()) > 0
+    # a piece records its tape only: the starting state it steers on (the run's own) is not written
+    for name in whole.s0:
+        assert torch.equal(whole.s0[name], pieces.s0[name]), name
