@@ -77,11 +77,18 @@ class GradRun:
     piece into the same fixed-point accumulators: the gradients are the
     whole tape's bit for bit, at one piece's memory (what the N_c=8192
     planet needs).  The forward pass records no tapes; the backward pass
-    keeps the starting states of one checkpoint segment."""
+    keeps the starting states of one checkpoint segment.
+
+    store: where the stored states live -- the starting state, the
+    checkpoints, a segment's starting states at the backward pass and a
+    recorded run's nominal states: "device", or "host" (pinned host memory,
+    copied to the device when read; the gradients are the same bits).  At
+    N_c=8192 a state is 22.6 GB: "host" lets a run keep checkpoints every
+    few iterations beyond what device memory holds."""
 
     def __init__(self, state, params, iterations: int, real: str = "f32", checkpoint_every: int = 10,
                  first_key: int = 0, rng_stage: str = "erosion", n_particles=None, replay_of: "GradRun | None" = None,
-                 feedback: bool = True, tape_batch: "int | None" = None):
+                 feedback: bool = True, tape_batch: "int | None" = None, store: str = "device"):
         from .maps import (_check_scope, _eparams, discharge_scales, max_steps_of, require_plain_side_passes,
                            trace_params, uplift_cap)
 
@@ -109,6 +116,9 @@ class GradRun:
         self.rng_stage = rng_stage
         self.replay_of = replay_of
         self.feedback = bool(feedback)
+        if store not in ("device", "host"):
+            raise ValueError("store must be 'device' or 'host'")
+        self.store = store
         if int(getattr(ep, "brush_radius", 0)) > 1:
             raise NotImplementedError("the differentiable mode has no brush")
         dev = state.device
@@ -149,11 +159,36 @@ class GradRun:
         self._auto_batch = tape_batch == "auto"
         self.set_tape_batch(None if self._auto_batch else tape_batch)
         self._taped: dict[int, int] = {}  # slot -> iteration whose tape it holds
-        self.s0 = _snapshot(state)
+        self.s0 = self._keep(state)
         self.nominal = None
         self.moves = None
         self.idx: list = [None] * self.T
         self.checkpoints: dict[int, dict] = {}
+
+    # -- stored states ---------------------------------------------------------
+    def _keep(self, src) -> dict:
+        """A stored copy of a state (device clone, or pinned host copy)."""
+        get = (lambda n: src[n]) if isinstance(src, dict) else (lambda n: getattr(src, n))
+        if self.store == "device":
+            return {n: get(n).clone() for n in _SNAP}
+        out = {}
+        for n in _SNAP:
+            v = get(n)
+            out[n] = torch.empty(v.shape, dtype=v.dtype, pin_memory=True)
+            out[n].copy_(v)
+        return out
+
+    def _fetch(self, kept: dict) -> dict:
+        """A stored state for reading on the device (the stored tensors
+        themselves when they are on the device)."""
+        if self.store == "device":
+            return kept
+        dev = self.state.device
+        return {n: v.to(dev, non_blocking=True) for n, v in kept.items()}
+
+    def _restore(self, kept: dict) -> dict:
+        """A new device copy of a stored state, to run on."""
+        return _snapshot(kept) if self.store == "device" else self._fetch(kept)
 
     def _tape_bytes_for(self, droplets: int) -> int:
         tb = ctypes.c_int64()
@@ -179,8 +214,8 @@ class GradRun:
                                                                  - torch.cuda.memory_allocated(st.device))
         n = st.height.numel()
         snap = sum(t.numel() * t.element_size() for t in self.s0.values())
-        reserve = (int(lib().pg_grad_reverse_work_bytes(st.F, st.N, st.H)) + 6 * 8 * n + (self.ce - 1) * snap
-                   + (2 << 30))
+        held = (self.ce - 1) * snap if self.store == "device" else snap  # the segment's states on the device
+        reserve = int(lib().pg_grad_reverse_work_bytes(st.F, st.N, st.H)) + 6 * 8 * n + held + (2 << 30)
         per = self._tape_bytes_for(2) - self._tape_bytes_for(1)
         fit = (free - reserve) // max(per, 1)
         if fit < 1:
@@ -233,7 +268,7 @@ class GradRun:
         if self.replay_of is None:
             return target, None, self.idx[t]
         r = self.replay_of
-        return r.nominal[t], r.moves[t], r.idx[t]
+        return r._fetch(r.nominal[t]), r.moves[t], r.idx[t]
 
     def _iterate(self, target: dict, t: int, erod, theta, tape=None, moves_out=None, steer=None, fixed=None,
                  idx=None):
@@ -272,7 +307,7 @@ class GradRun:
         tapes: "last" (the last segment's, for the backward pass), "all" or
         "none"."""
         T, ce = self.T, self.ce
-        target = _snapshot(self.s0)
+        target = self._restore(self.s0)
         self.checkpoints = {}
         self._taped = {}
         if record:
@@ -283,11 +318,11 @@ class GradRun:
         for t in range(T):
             if t % ce == 0:
                 # the first checkpoint is the starting state itself (nothing writes either)
-                self.checkpoints[t] = self.s0 if t == 0 else _snapshot(target)
+                self.checkpoints[t] = self.s0 if t == 0 else self._keep(target)
             mv = None
             if record:
                 # a nominal state is read only: the first is the starting state itself
-                self.nominal.append(self.s0 if t == 0 else _snapshot(target))
+                self.nominal.append(self.s0 if t == 0 else self._keep(target))
                 mv = torch.zeros(self.P, dtype=torch.int32, device=self.state.device)
                 self.moves.append(mv)
             tape = None
@@ -309,9 +344,9 @@ class GradRun:
         src = self.replay_of if self.replay_of is not None else self
         if src.nominal is None:
             raise ValueError("replay needs a forward pass with record=True")
-        target = _snapshot(self.s0)
+        target = self._restore(self.s0)
         for t in range(self.T):
-            self._iterate(target, t, erod, theta, steer=src.nominal[t], fixed=src.moves[t], idx=src.idx[t])
+            self._iterate(target, t, erod, theta, steer=src._fetch(src.nominal[t]), fixed=src.moves[t], idx=src.idx[t])
         return target
 
     def backward(self, g_surface: torch.Tensor, erod, theta):
@@ -332,7 +367,7 @@ class GradRun:
         for a, b in reversed(segs):
             slot_of = (lambda t: t) if self._tapes_all else (lambda t: t - a)
             if not all(self._taped.get(slot_of(t)) == t for t in range(a, b)):
-                target = _snapshot(self.checkpoints[a])
+                target = self._restore(self.checkpoints[a])
                 for t in range(a, b):
                     steer, fixed, _ = self._steer(t, target)
                     self._iterate(target, t, erod, theta, tape=self._tape(slot_of(t)), steer=steer, fixed=fixed,
@@ -351,14 +386,14 @@ class GradRun:
             # the segment's starting states: the checkpoint itself (read only), then copies
             starts = [self.checkpoints[a]]
             if b - a > 1:
-                target = _snapshot(self.checkpoints[a])
+                target = self._restore(self.checkpoints[a])
                 for t in range(a, b - 1):
                     steer, fixed, _ = self._steer(t, target)
                     self._iterate(target, t, erod, theta, steer=steer, fixed=fixed, idx=self.idx[t])
-                    starts.append(_snapshot(target))
+                    starts.append(self._keep(target))
                 del target
             for t in range(b - 1, a - 1, -1):
-                start = starts[t - a]
+                start = self._fetch(starts[t - a])
                 steer, fixed, _ = self._steer(t, start)
                 src = start if self.feedback else steer
                 surf = (src["height"] + src["sediment"]).to(self.dtype)

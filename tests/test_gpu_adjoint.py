@@ -215,3 +215,29 @@ def test_tape_in_pieces_gives_the_whole_tapes_gradients(cuda, real):
     # a piece records its tape only: the starting state it steers on (the run's own) is not written
     for name in whole.s0:
         assert torch.equal(whole.s0[name], pieces.s0[name]), name
+
+
+def test_states_in_host_memory_give_the_same_gradients(cuda):
+    """store="host": the starting state, checkpoints, a segment's starting
+    states and a recorded run's nominal states in pinned host memory --
+    the forward result, the gradients and a replay are the device store's
+    bits (with the tape whole and in pieces)."""
+    T = 5
+    st, p = _state(48, cuda)
+    erod = _erod(st)
+    out = []
+    for store, batch in (("device", None), ("host", None), ("host", 2000)):
+        run = adj.GradRun(st, p, iterations=T, real="f32", checkpoint_every=2, tape_batch=batch, store=store)
+        theta = adj.theta_nominal(run)
+        print(f"state store {store}, tape_batch {batch}, N=48 T={T}", flush=True)
+        e, t = erod.clone().requires_grad_(True), theta.clone().requires_grad_(True)
+        s = adj.erosion(e, t, run)
+        adj.hypsometry_loss(s, run).backward()
+        run.forward(erod, theta, record=True, tapes="none")
+        r = run.replay(erod, theta)
+        out.append((s.detach().clone(), e.grad.clone(), t.grad.clone(), r["height"].clone()))
+        if store == "host":
+            assert not run.checkpoints[2]["height"].is_cuda and run.checkpoints[2]["height"].is_pinned()
+    for o in out[1:]:
+        for a, b in zip(out[0], o):
+            assert torch.equal(a, b)
