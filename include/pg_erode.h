@@ -52,10 +52,10 @@ typedef struct pg_trace_params {
 
 /* Message of the last failed call on this thread ("" if none). */
 const char* pg_last_error(void);
-/* ABI version of this library: 8 (bumped whenever a struct layout or a
+/* ABI version of this library: 9 (bumped whenever a struct layout or a
  * signature changes; 2 added pg_fast_params.erode_cap, 3 its tape, 6 the
  * brush, 7 the deterministic adjoint's workspace and halo table, 8 the
- * adjoint's tape in pieces). */
+ * adjoint's tape in pieces, 9 the fast mode's bank cuts and their stat). */
 int pg_abi_version(void);
 
 /* ---- geometry, host side (bake/globe/cubesphere.py:70-220) ------------- */
@@ -441,6 +441,7 @@ enum {
     PG_FS_CLAMPED = 10, PG_FS_TO_PENDING_FX, PG_FS_LOST_OFFSHORE_FX, PG_FS_EXIT_FX, PG_FS_VANISHED_FX,
     PG_FS_RELEASED_FX, PG_FS_VOLUME0_BITS, PG_FS_ELIGIBLE, PG_FS_CDF_SERIAL, PG_FS_ERODED_FX, PG_FS_DEPOSITED_FX,
     PG_FS_DEFICIT_FX,
+    PG_FS_LATERAL_FX = 22, /* the bank cuts' erosion taken (fixed point, erosion.lateral_rate) */
     PG_FS_TAPE_SHORT = 23, /* droplets a chunk's census left untraced, its tape rows full (always 0) */
     PG_FAST_NSTATS = 24
 };

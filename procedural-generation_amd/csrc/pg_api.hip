@@ -27,7 +27,7 @@ using namespace pg;
 
 extern "C" const char* pg_last_error(void) { return g_error; }
 
-extern "C" int pg_abi_version(void) { return 8; }
+extern "C" int pg_abi_version(void) { return 9; }
 
 extern "C" int pg_geom_to_sphere(int face, double u, double v, double* xyz3) {
     clear_error();

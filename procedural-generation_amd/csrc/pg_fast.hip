@@ -414,6 +414,7 @@ struct TraceCfg {
     // the routing surface (ROUTE kernels): per extended cell {route, flag}
     float2* rte;
     float lake_trap;
+    float lateral_rate;  // bank cuts on the outside of a bend (0: off)
     float4* tape;
     ll* tape_len;
     ll tape_rows;
@@ -521,10 +522,12 @@ __device__ __forceinline__ ll final_cascade(const TraceCfg& k, const Ring& ring,
 // tape record: flags word = transfer kind (bits 0-1: none / sea deposit /
 // land step), MOVE (bit 2), START of a droplet (bit 3), END (bit 4), the end's
 // cause (bits 8-15), exited (16), final deposit (17), the end's step stood
-// on an active cell without recording a move (18), final entries (20-23)
+// on an active cell without recording a move (18), a bank cut (19: its cell
+// and amount in the second float4, beside a start's droplet ids), final
+// entries (20-23)
 enum : int { REC_NONE = 0, REC_SEA = 1, REC_LAND = 2 };
 enum : int { RF_MOVE = 1 << 2, RF_START = 1 << 3, RF_END = 1 << 4, RF_EXITED = 1 << 16, RF_DEPOSIT = 1 << 17,
-             RF_STOOD = 1 << 18 };
+             RF_STOOD = 1 << 18, RF_LATERAL = 1 << 19 };
 
 // WPE: the register budget as waves per SIMD (trace_waves: the census's
 // natural allocation fits 4, the traced applying pass's 3; one or two more
@@ -533,6 +536,26 @@ enum : int { RF_MOVE = 1 << 2, RF_START = 1 << 3, RF_END = 1 << 4, RF_EXITED = 1
 // every move recorded for replay_kernel), true = the applying pass traced
 // again (batched windows, or no tape).  ROUTE: steer on the routing surface
 // and keep the lakes (particle.py:450-460, 552-560, 626-640, 751-769, 951-955).
+// a bank cut's erosion request at `bcell` (fixed point), at most the room
+// above the bank's own floor -- its base level, or its surface when
+// submerged, or a lake's water level -- on the frozen terrain; `cut` gets
+// the part refused (particle.py:941-966, the floor of the cell stepped to
+// being the per-cell scale's)
+__device__ __forceinline__ ll bank_request(const TraceCfg& k, int bcell, float amount, ll& cut) {
+    const float hb = k.cell4[bcell].x, bb = k.dyn[bcell].y;
+    float fl = hb >= bb ? bb : hb;
+    if (k.rte && k.rte[bcell].y >= 1.5f) fl = fmaxf(fl, k.rte[bcell].x);
+    const float room = hb - fl;
+    const ll lim = room > 0.0f ? llrint((double)room * k.hscale) : 0;
+    ll dq = llrint((double)amount * k.hscale);
+    cut = 0;
+    if (dq > lim) {
+        cut = dq - lim;
+        dq = lim;
+    }
+    return dq;
+}
+
 template <bool BATCH, int WPE, bool APPLY, bool ROUTE>
 __global__ void __launch_bounds__(TRACE_TPB) __attribute__((amdgpu_waves_per_eu(WPE, WPE))) trace_kernel(TraceCfg k) {
     const int lane = threadIdx.x & 63;
@@ -587,7 +610,7 @@ __global__ void __launch_bounds__(TRACE_TPB) __attribute__((amdgpu_waves_per_eu(
     Samp S;  // the sample of the stencil at the droplet's position
     Here cur;
     // lane totals
-    ll n_entries = 0, n_moves = 0, exit_fx = 0, vanished_fx = 0, lost_fx = 0, to_pend_fx = 0;
+    ll n_entries = 0, n_moves = 0, exit_fx = 0, vanished_fx = 0, lost_fx = 0, to_pend_fx = 0, lat_fx = 0;
     ll deaths[N_DEATHS] = {0, 0, 0, 0, 0, 0};
 
     auto load_here = [&](int ff, int cx, int cy, Here& h, const float4* rec) {
@@ -636,8 +659,8 @@ __global__ void __launch_bounds__(TRACE_TPB) __attribute__((amdgpu_waves_per_eu(
 
     while (true) {
         // this iteration's tape record
-        int r_flags = 0, r_cell = 0, r_dl = 0, r_d = 0;
-        float r_x = 0.0f, r_aux = 0.0f;
+        int r_flags = 0, r_cell = 0, r_dl = 0, r_d = 0, r_bank = 0;
+        float r_x = 0.0f, r_aux = 0.0f, r_lat = 0.0f;
         // a wave whose tape rows could not hold one more droplet takes none
         if (taping && !exhausted && t_row + k.max_steps + 2 > k.tape_rows) exhausted = true;
         // refill the lanes whose droplet died from the chunk's queue
@@ -748,6 +771,7 @@ __global__ void __launch_bounds__(TRACE_TPB) __attribute__((amdgpu_waves_per_eu(
                 int nf, nix, niy;
                 float nfx, nfy;
                 const float h0 = S.h;
+                const float psa = sa, psb = sb;  // the direction before this step (a bank cut's turn)
                 cause = steer_move<true>(k, N, S, g0, g1, vol, vol_rel, mom_scale, sa, sb, f, ix, iy, fx, fy, nf, nix, niy,
                                          nfx, nfy);
                 exited = cause == DEATH_EXIT;
@@ -881,6 +905,56 @@ __global__ void __launch_bounds__(TRACE_TPB) __attribute__((amdgpu_waves_per_eu(
                                     atomicAdd((ull*)&k.mom_fx_b[cur.cell], (ull)llrint((double)(vol * sb) * k.tscale));
                                     n_entries += 1;
                                 }
+                                if (k.lateral_rate > 0.0f && cdiff > 0.0f) {
+                                    // the bank on the outside of the bend is cut as well, by the turn's
+                                    // sine times the bed's erosion, at most the bank's height over the
+                                    // bed (particle.py:707-740); it erodes like a step's request, its
+                                    // floor the cell stepped to
+                                    const float cross = psa * sb - psb * sa;
+                                    const float turn = fabsf(cross);
+                                    if (turn > 1e-6f) {
+                                        const float sgn = cross > 0.0f ? 1.0f : -1.0f;
+                                        const int bei = ix + k.H + (int)floorf(fx + sgn * sb);
+                                        const int bej = iy + k.H + (int)floorf(fy - sgn * sa);
+                                        if (bei >= 0 && bei < NE && bej >= 0 && bej < NE) {
+                                            const int bcell = (int)(coff + ((int64_t)f * NE + bei) * NE + bej);
+                                            const float bank = k.cell4[bcell].x - h_here;
+                                            if (k.mask[bcell] == MASK_ACTIVE && bank > 0.0f) {
+                                                float lat = k.lateral_rate * turn * cdiff;
+                                                const float lcap = fminf(bank / kexp, k.max_erode);
+                                                if (lat * vol_rel > lcap) lat = lcap / vol_rel;
+                                                if (lat > 0.0f) {
+                                                    sed += lat;
+                                                    const float lamount = lat * vol_rel;
+                                                    if (taping) {
+                                                        r_flags |= RF_LATERAL;
+                                                        r_bank = bcell;
+                                                        r_lat = lamount;
+                                                    }
+                                                    ll cutb = 0;
+                                                    const ll dqb = bank_request(k, bcell, lamount, cutb);
+                                                    if (!apply) {
+                                                        if (dqb > 0) {
+                                                            atomicAdd((ull*)&k.ero_fx[bcell], (ull)dqb);
+                                                            atomicMin(&k.floor_key[bcell],
+                                                                      ((ull)fkey(nrec.x) << 32) | (ull)(unsigned)ncell);
+                                                        }
+                                                    } else {
+                                                        deficit += cutb;
+                                                        if (dqb > 0) {
+                                                            const ll taken = (ll)floor((double)dqb * (double)k.rscale[bcell]);
+                                                            deficit += dqb - taken;
+                                                            real += taken;
+                                                            lat_fx += taken;
+                                                            if (taken) atomicAdd((ull*)&k.chunk_fx[bcell], (ull)(-taken));
+                                                        }
+                                                        n_entries += 1;
+                                                    }
+                                                }
+                                            }
+                                        }
+                                    }
+                                }
                             }
                             if (apply) {
                                 own_prev = chg;
@@ -1003,23 +1077,24 @@ __global__ void __launch_bounds__(TRACE_TPB) __attribute__((amdgpu_waves_per_eu(
             // one row per loop iteration: the wave's 64 records side by side
             float4* row = wtape + 2 * (t_row * 64 + lane);
             row[0] = make_float4(__int_as_float(r_flags), __int_as_float(r_cell), r_x, r_aux);
-            if (r_flags & RF_START) row[1] = make_float4(0.0f, 0.0f, __int_as_float(r_d), __int_as_float(r_dl));
+            if (r_flags & (RF_START | RF_LATERAL))
+                row[1] = make_float4(__int_as_float(r_bank), r_lat, __int_as_float(r_d), __int_as_float(r_dl));
         }
         t_row += 1;
     }
     if (taping && lane == 0) k.tape_len[gw] = t_row;
     if (!apply) return;
     // lane totals -> device statistics (one atomic per wave and counter)
-    ll v[6 + N_DEATHS] = {n_entries, n_moves, exit_fx, vanished_fx, lost_fx, to_pend_fx};
-    for (int c = 0; c < N_DEATHS; c++) v[6 + c] = deaths[c];
-    for (int c = 0; c < 6 + N_DEATHS; c++) v[c] = wave_sum(v[c]);
+    ll v[7 + N_DEATHS] = {n_entries, n_moves, exit_fx, vanished_fx, lost_fx, to_pend_fx, lat_fx};
+    for (int c = 0; c < N_DEATHS; c++) v[7 + c] = deaths[c];
+    for (int c = 0; c < 7 + N_DEATHS; c++) v[c] = wave_sum(v[c]);
     if (lane == 0) {
-        const int slot[6] = {PG_FS_ENTRIES, PG_FS_MOVES, PG_FS_EXIT_FX, PG_FS_VANISHED_FX, PG_FS_DEFICIT_FX,
-                             PG_FS_TO_PENDING_FX};
-        for (int c = 0; c < 6; c++)
+        const int slot[7] = {PG_FS_ENTRIES, PG_FS_MOVES, PG_FS_EXIT_FX, PG_FS_VANISHED_FX, PG_FS_DEFICIT_FX,
+                             PG_FS_TO_PENDING_FX, PG_FS_LATERAL_FX};
+        for (int c = 0; c < 7; c++)
             if (v[c]) atomicAdd((ull*)&k.stats[slot[c]], (ull)v[c]);
         for (int c = 0; c < N_DEATHS; c++)
-            if (v[6 + c]) atomicAdd((ull*)&k.stats[PG_FS_DEATHS + c], (ull)v[6 + c]);
+            if (v[7 + c]) atomicAdd((ull*)&k.stats[PG_FS_DEATHS + c], (ull)v[7 + c]);
     }
 }
 
@@ -1089,7 +1164,7 @@ __global__ void __launch_bounds__(TRACE_TPB) __attribute__((amdgpu_waves_per_eu(
     Ring ring;        // the cells of its final deposits
     ring.n = 0;
     int n_entries = 0, n_moves = 0;  // per lane: below 2^31
-    ll exit_fx = 0, vanished_fx = 0, lost_fx = 0, to_pend_fx = 0;
+    ll exit_fx = 0, vanished_fx = 0, lost_fx = 0, to_pend_fx = 0, lat_fx = 0;
     int deaths[N_DEATHS] = {0, 0, 0, 0, 0, 0};
     // a deposit request: the amount laid down (the caller adds it to the cell)
     auto deposit = [&](ll d, ll lim) -> ll {
@@ -1159,6 +1234,22 @@ __global__ void __launch_bounds__(TRACE_TPB) __attribute__((amdgpu_waves_per_eu(
                 if (change) atomicAdd((ull*)&k.chunk_fx[cell], (ull)change);
                 n_moves += 1;
             }
+            if (fl & RF_LATERAL) {
+                // the move's bank cut, after the move's own change (as the traced pass)
+                const float4 B = row[1];
+                const int bcell = __float_as_int(B.x);
+                ll cutb = 0;
+                const ll dqb = bank_request(k, bcell, B.y, cutb);
+                deficit += cutb;
+                if (dqb > 0) {
+                    const ll taken = (ll)floor((double)dqb * (double)k.rscale[bcell]);
+                    deficit += dqb - taken;
+                    real += taken;
+                    lat_fx += taken;
+                    if (taken) atomicAdd((ull*)&k.chunk_fx[bcell], (ull)(-taken));
+                }
+                n_entries += 1;
+            }
             if (fl & RF_END) {
                 // the droplet's end (particle.py:770-815): cause, stop cell and final entries as the census saw them
                 deaths[(fl >> 8) & 0xff] += 1;
@@ -1187,16 +1278,16 @@ __global__ void __launch_bounds__(TRACE_TPB) __attribute__((amdgpu_waves_per_eu(
             }
         }
     }
-    ll v[6 + N_DEATHS] = {n_entries, n_moves, exit_fx, vanished_fx, lost_fx, to_pend_fx};
-    for (int c = 0; c < N_DEATHS; c++) v[6 + c] = deaths[c];
-    for (int c = 0; c < 6 + N_DEATHS; c++) v[c] = wave_sum(v[c]);
+    ll v[7 + N_DEATHS] = {n_entries, n_moves, exit_fx, vanished_fx, lost_fx, to_pend_fx, lat_fx};
+    for (int c = 0; c < N_DEATHS; c++) v[7 + c] = deaths[c];
+    for (int c = 0; c < 7 + N_DEATHS; c++) v[c] = wave_sum(v[c]);
     if (lane == 0) {
-        const int slot[6] = {PG_FS_ENTRIES, PG_FS_MOVES, PG_FS_EXIT_FX, PG_FS_VANISHED_FX, PG_FS_DEFICIT_FX,
-                             PG_FS_TO_PENDING_FX};
-        for (int c = 0; c < 6; c++)
+        const int slot[7] = {PG_FS_ENTRIES, PG_FS_MOVES, PG_FS_EXIT_FX, PG_FS_VANISHED_FX, PG_FS_DEFICIT_FX,
+                             PG_FS_TO_PENDING_FX, PG_FS_LATERAL_FX};
+        for (int c = 0; c < 7; c++)
             if (v[c]) atomicAdd((ull*)&k.stats[slot[c]], (ull)v[c]);
         for (int c = 0; c < N_DEATHS; c++)
-            if (v[6 + c]) atomicAdd((ull*)&k.stats[PG_FS_DEATHS + c], (ull)v[6 + c]);
+            if (v[7 + c]) atomicAdd((ull*)&k.stats[PG_FS_DEATHS + c], (ull)v[7 + c]);
     }
     // droplets the census left untraced because every wave's rows were full: must be none
     if (rw == 0 && lane == 0) {
@@ -1650,8 +1741,7 @@ int prepare(const pg_fast_fields* fl, const pg_trace_params* prm, const pg_fast_
     PG_REQUIRE(fp->chunks >= 1 && fp->chunks <= 4096, "fast mode: chunks must be in [1, 4096]");
     PG_REQUIRE(fp->conc_chunk >= 0.0 && fp->iter_erode >= 0.0, "fast mode: conc_chunk and iter_erode must be >= 0");
     PG_REQUIRE(prm->max_steps >= 0 && prm->max_steps < (1ll << 30), "fast mode: bad max_steps");
-    PG_REQUIRE(prm->lateral_rate <= 0.0 && prm->lake_park <= 0.0,
-               "fast mode: lateral cuts and lake parks are outside the fast mode (plain profile)");
+    PG_REQUIRE(prm->lake_park <= 0.0, "fast mode: lake parks are outside the fast mode");
     PG_REQUIRE(fp->height_bits > 0 && fp->height_bits < 62 && fp->track_bits > 0 && fp->track_bits < 62,
                "fast mode: fixed-point bits out of range");
     c->N = fl->N;
@@ -1811,6 +1901,7 @@ int fast_particles(const pg_fast_fields* fl, const pg_trace_params* prm, const p
     if (fp->tape && persistent_grid((const void*)replay_kernel<false>, TRACE_TPB, 0, &rgrid)) return -1;
     k.rte = fp->route ? at<float2>(work, L.rte) : nullptr;
     k.lake_trap = (float)prm->lake_trap;
+    k.lateral_rate = (float)(prm->lateral_rate > 0.0 ? prm->lateral_rate : 0.0);
     k.tape = nullptr;
     k.tape_len = nullptr;
     k.tape_rows = 0;
@@ -2190,7 +2281,7 @@ extern "C" int pg_batch_iterate(const pg_fast_fields* wins, int64_t n_win, const
     PG_REQUIRE(prm && fps && work && stats && win_stats, "pg_batch_iterate: NULL argument");
     const pg_fast_params& f0 = fps[0];
     PG_REQUIRE(f0.chunks >= 1 && f0.chunks <= MAX_CHUNKS, "pg_batch_iterate: chunks must be in [1, %d]", MAX_CHUNKS);
-    PG_REQUIRE(prm->lateral_rate <= 0.0 && prm->lake_park <= 0.0, "pg_batch_iterate: plain profile only");
+    PG_REQUIRE(prm->lake_park <= 0.0, "pg_batch_iterate: lake parks are outside the fast mode");
     std::vector<int64_t> NE(n_win), P(n_win);
     for (int64_t w = 0; w < n_win; w++) {
         NE[w] = wins[w].N + 2 * wins[w].H;
@@ -2307,6 +2398,7 @@ extern "C" int pg_batch_iterate(const pg_fast_fields* wins, int64_t n_win, const
                    "pg_batch_iterate: either every window has a routing surface or none");
     k.rte = f0.route ? at<float2>(work, L.rte) : nullptr;
     k.lake_trap = (float)prm->lake_trap;
+    k.lateral_rate = (float)(prm->lateral_rate > 0.0 ? prm->lateral_rate : 0.0);
     // the move tape: prm->max_steps bounds every window's lifetime
     k.max_steps = (int)prm->max_steps;
     if (f0.tape) {

@@ -904,6 +904,7 @@ extern "C" int pg_grad_iterate(const pg_fast_fields* steer, const pg_fast_fields
     PG_REQUIRE(target && ga && ga->surf, "pg_grad_iterate: NULL argument");
     PG_REQUIRE(ga->real == 0 || ga->real == 1, "pg_grad_iterate: real must be 0 (float) or 1 (double)");
     PG_REQUIRE(fp && fp->chunks == 1, "pg_grad_iterate: the differentiable mode runs one chunk per iteration");
+    PG_REQUIRE(prm && prm->lateral_rate <= 0.0, "pg_grad_iterate: lateral cuts are outside the differentiable mode");
     PG_REQUIRE(!fp->erode_cap, "pg_grad_iterate: the slope limit (erode_cap) is not differentiated");
     PG_REQUIRE(steer && target->F == steer->F && target->N == steer->N && target->H == steer->H,
                "pg_grad_iterate: steering and target states differ in shape");

@@ -59,7 +59,7 @@ class GradArgs(ctypes.Structure):
 #: statistics slots of a fast iteration (include/pg_erode.h, PG_FS_*)
 FS = dict(PARTICLES=0, RAIN=1, ENTRIES=2, MOVES=3, DEATHS=4, CLAMPED=10, TO_PENDING_FX=11, LOST_OFFSHORE_FX=12,
           EXIT_FX=13, VANISHED_FX=14, RELEASED_FX=15, VOLUME0_BITS=16, ELIGIBLE=17, CDF_SERIAL=18, ERODED_FX=19,
-          DEPOSITED_FX=20, DEFICIT_FX=21, TAPE_SHORT=23)
+          DEPOSITED_FX=20, DEFICIT_FX=21, LATERAL_FX=22, TAPE_SHORT=23)
 FAST_NSTATS = 24
 
 

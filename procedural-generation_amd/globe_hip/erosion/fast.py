@@ -117,7 +117,8 @@ def decode_stats(raw: np.ndarray, hscale: float) -> dict:
             "moves": int(raw[FS["MOVES"]]), "volume": vol0, "deaths": deaths, "clamped": int(raw[FS["CLAMPED"]]),
             "to_pending": fx("TO_PENDING_FX"), "released": fx("RELEASED_FX"), "deficit_out": fx("EXIT_FX"),
             "lost_offshore": fx("LOST_OFFSHORE_FX"), "vanished": fx("VANISHED_FX"), "eroded": fx("ERODED_FX"),
-            "deposited": fx("DEPOSITED_FX"), "deficit_cancelled": fx("DEFICIT_FX"), "eligible": int(raw[FS["ELIGIBLE"]]),
+            "deposited": fx("DEPOSITED_FX"), "deficit_cancelled": fx("DEFICIT_FX"), "lateral": fx("LATERAL_FX"),
+            "eligible": int(raw[FS["ELIGIBLE"]]),
             "cdf_serial": bool(raw[FS["CDF_SERIAL"]]), "tape_short": int(raw[FS["TAPE_SHORT"]]), "raw_fx": raw.copy()}
 
 
@@ -133,8 +134,8 @@ def run_iteration(state, params, iteration_key, n_particles=None, particles_per_
 
     ep = _eparams(params)
     _check_scope(ep, state)
-    if float(getattr(ep, "lateral_rate", 0.0)) > 0.0 or state.roots is not None or lake_park_of(state, ep) > 0.0:
-        raise NotImplementedError("lateral cuts, roots and lake parks are outside the fast mode: use mode='check'")
+    if state.roots is not None or lake_park_of(state, ep) > 0.0:
+        raise NotImplementedError("roots and lake parks are outside the fast mode: use mode='check'")
     key = iteration_key if isinstance(iteration_key, (tuple, list)) else (int(iteration_key),)
     rng = params.rng(rng_stage, *key)
     t0 = time.time()

@@ -147,3 +147,18 @@ def test_full_size_batch_equals_per_window(cuda):
         ok, lhs, rel = _budget_closes(a.stats.cpu().numpy() + a.win_stats.sum(0).cpu().numpy())
         assert ok, (it, lhs, rel)
         _same(a, b)
+
+
+def test_lateral_cuts_batched_equals_per_window(cuda):
+    """Bank cuts (erosion.lateral_rate) in the batched launch and one
+    fast-mode pass per window: the same bits, with the tape and without."""
+    a, b, c = (_batch(cuda, seed=8, lateral_rate=20.0) for _ in range(3))
+    c.use_tape = False
+    for it in range(3):
+        a.step(it)
+        b.step_per_window(it)
+        c.step(it)
+        torch.cuda.synchronize()
+        _same(a, b)
+        _same(a, c)
+    assert float((a.height - _batch(cuda, seed=8).height).abs().max()) > 0

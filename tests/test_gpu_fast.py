@@ -161,6 +161,85 @@ def test_tape_replay_equals_the_traced_applying_pass(cuda, grid_blocks, variant)
         np.testing.assert_array_equal(a, b)
 
 
+def test_lateral_cuts_budget_and_tape(cuda):
+    """Bank cuts (erosion.lateral_rate, particle.py:707-740) in the fast
+    mode: the fixed-point budget stays exact, the tape replay is the traced
+    applying pass bit for bit, and the cuts change the result."""
+    outs = []
+    for tape in (False, True):
+        _, _, p, st = _fast_state(64, cuda, lateral_rate=20.0)
+        st.fast_tape = tape
+        for it in range(3):
+            s = fast.run_iteration(st, p, it)
+            r = s["raw_fx"]
+            FS = _lib.FS
+            lhs = (int(r[FS["DEPOSITED_FX"]]) - int(r[FS["ERODED_FX"]]) + int(r[FS["TO_PENDING_FX"]])
+                   + int(r[FS["LOST_OFFSHORE_FX"]]) + int(r[FS["EXIT_FX"]]) + int(r[FS["VANISHED_FX"]]))
+            assert lhs == int(r[FS["RELEASED_FX"]]), (tape, it)
+            assert s["tape_short"] == 0
+        outs.append([t.cpu().numpy().copy() for t in (st.height, st.sediment, st.discharge, st.momentum, st.pending)]
+                    + [s["raw_fx"]])
+    for a, b in zip(*outs):
+        np.testing.assert_array_equal(a, b)
+    _, _, p0, st0 = _fast_state(64, cuda)
+    for it in range(3):
+        fast.run_iteration(st0, p0, it)
+    assert float(np.abs(outs[0][0] - st0.height.cpu().numpy()).max()) > 0
+
+
+def test_lateral_cuts_track_the_oracle(oracle, cuda):
+    """Bank cuts against the oracle (which applies the reference's volume -2
+    entries serially): on the reference test's own window (an incline with
+    1 m of loose sediment, bake/tests/test_erosion.py:1757-1791) at rate 50,
+    the first iteration's bank erosion -- the oracle's requested -2 entries
+    against what the fast mode took -- within 5 %; and 20 iterations of the
+    N=128 plain globe at rate 20 within the fast mode's 2 % bound."""
+    from test_oracle_lateral import _plain_window_inputs
+    from windows import window_params
+
+    p = window_params(0, lateral_rate=50.0, max_steps=60)
+    kw = _plain_window_inputs(p)
+    ost = oracle.OracleState.window(**kw)
+    dst = maps.ErosionState.window(**kw, device=cuda, mode="fast")
+    got = []
+
+    def hook(cl_cell, cl_delta, cl_vol, cl_mom, cl_count, cap):
+        for q in range(cl_count.size):
+            sl = slice(q * cap, q * cap + int(cl_count[q]))
+            got.append(float(-cl_delta[sl][cl_vol[sl] == -2.0].sum()))
+
+    oracle.run_iteration(ost, p, 0, particles_per_cell=0.3, rng_stage="refine", diag_lists=hook)
+    s = fast.run_iteration(dst, p, 0, particles_per_cell=0.3, rng_stage="refine")
+    lat_o, lat_d = sum(got), s["lateral"]
+    print(f"window, rate 50, first iteration: bank cuts oracle {lat_o:.6g} (requested), fast {lat_d:.6g} (taken)",
+          flush=True)
+    assert lat_o > 0 and abs(lat_d - lat_o) <= 0.05 * lat_o  # measured 0.03 %
+
+    N, iters = 128, 20
+    grid, f = stubs.plain_globe(N, 2)
+    p = stubs.workload_params(N, 2, lateral_rate=20.0)
+    ost = oracle.OracleState.from_grid(grid, f["bedrock_m"], f["hardness"], f["precip"], f["evap"], f["uplift_m"], p.erosion)
+    dst = maps.ErosionState.from_grid(grid, f["bedrock_m"], f["hardness"], f["precip"], f["evap"], f["uplift_m"], p.erosion,
+                                      device=cuda, mode="fast")
+    dst.fast_chunks = 48
+    s0 = (ost.height + ost.sediment)[ost.interior].copy()
+    shift_o = shift_d = 0.0
+    for it in range(iters):
+        shift_o += oracle.step(ost, p, it)["datum_shift"]
+        shift_d += float(maps.step(dst, p, it)["datum_shift"].item())
+    so = (ost.height + ost.sediment)[ost.interior]
+    sd = (dst.height + dst.sediment)[dst.interior].cpu().numpy()
+    unit = ost.height_unit_m
+    land_o, land_d = so[so >= 0] * unit, sd[sd >= 0] * unit
+    hi = max(land_o.max(), land_d.max())
+    dev_h = float(np.abs(_hypsometry(land_o, 0.0, hi) - _hypsometry(land_d, 0.0, hi)).max())
+    ero_o = float(np.maximum(s0 - (so + shift_o), 0.0).sum())
+    ero_d = float(np.maximum(s0 - (sd + shift_d), 0.0).sum())
+    print(f"globe N=128, rate 20: hypsometry gap {dev_h:.5f}; eroded volume oracle {ero_o:.6g} fast {ero_d:.6g} "
+          f"(rel {(ero_d - ero_o) / ero_o:+.4f})", flush=True)
+    assert dev_h <= 0.02 and abs(ero_d - ero_o) / ero_o <= 0.02
+
+
 def test_cell_count_discharge_scales(cuda, monkeypatch):
     """disc_saturation_cells / momentum_saturation_cells (maps.py:911-921):
     the scales are computed on the device from the iteration's spawn; the

@@ -36,7 +36,7 @@ def test_binding_covers_the_header():
 def test_host_entry_points_and_errors(hiplib):
     from globe_hip import _lib
 
-    assert hiplib.pg_abi_version() == 8
+    assert hiplib.pg_abi_version() == 9
     assert hiplib.pg_change_list_cap(64, 0) == 64 + 16 + 1 + 8 + 8
     assert hiplib.pg_change_list_cap(64, 1) == 128 + 16 + 1 + 8 + 8
     assert hiplib.pg_select_work_bytes() > 0
