@@ -90,7 +90,8 @@ constexpr int CTL_DONE = 0, CTL_SWEEPS = 1, CTL_FALLBACK = 2, CTL_CHANGED = 3;
 
 struct Layout {
     size_t flags, d, hs, hsn, ec, b, r, rs, rn, w, posw, cd, cs, cn;   // per entry (P * cap)
-    size_t death, hsd, bd, re, drop, pend, pendf, fdirty, fc;        // per droplet
+    size_t death, hsd, bd, re, drop, pend, pendf, dend, fc;          // per droplet
+    size_t dflag;                                                    // per entry, padded rows
     size_t keys_in, vals_in, keys_out, vals_out, sv;                 // per access slot
     size_t ownerw;                                                   // per entry
     size_t rmin;                                                     // per range
@@ -129,7 +130,8 @@ int layout_of(int64_t P, int64_t cap, int max_sweeps, Layout* L) {
     L->drop = take((size_t)P * 4 * 8);
     L->pend = take((size_t)P * 8);
     L->pendf = take((size_t)P);
-    L->fdirty = take((size_t)P * 4);
+    L->dend = take((size_t)P);
+    L->dflag = take((size_t)P * (size_t)((cap + 3) & ~(int64_t)3));
     L->fc = take((size_t)P * 24);
     L->keys_in = take(na * 4);
     L->vals_in = take(na * 4);
@@ -185,7 +187,14 @@ struct OrdArgs {
     float* bd;
     double *re, *drop, *pend;
     uint8_t* pendf;
-    uint32_t* fdirty;  // per droplet: first entry whose read value changed (NONE: clean)
+    // per droplet p, a row of capw = cap rounded up to 4 bytes: byte k is set
+    // when a read of entry k changed value (the droplet walks again from its
+    // first set byte); dend[p] when its end read did.  Plain byte stores, no
+    // atomics: the cell half that sets them and the droplet half that reads
+    // and clears them are separate launches.
+    uint8_t* dflag;
+    uint8_t* dend;
+    int64_t capw;
     double* fc;        // per droplet: final carries (deficit, surplus, clamped count)
     uint32_t *keys_in, *vals_in, *keys_out, *vals_out;
     double* sv;  // per sorted slot: a write's amount W, a read's value R (coalesced copies)
@@ -303,8 +312,10 @@ __global__ void __launch_bounds__(BLOCK) ord_build_kernel(OrdArgs a) {
             a.w[e] = 0.0;
         }
     }
+    // no dirty marks: sweep 0 walks every droplet from its first entry
+    for (int64_t wi = lane; wi < a.capw / 4; wi += WAVE) ((uint32_t*)(a.dflag + p * a.capw))[wi] = 0u;
     if (lane == 0) {
-        a.fdirty[p] = 0;
+        a.dend[p] = 0;
         a.death[p] = dth;
         uint32_t* ks = a.keys_in + sbase + 3 * cap;
         uint32_t* vs = a.vals_in + sbase + 3 * cap;
@@ -388,9 +399,31 @@ __device__ __forceinline__ Ent load_ent(const OrdArgs& a, int64_t base, int64_t 
 __device__ void droplet_wave(const OrdArgs& a, int64_t p, int sweep) {
     const int lane = lane_id();
     const int64_t cap = a.cap, base = p * cap, cnt = a.cl_count[p];
-    const uint32_t fd = a.fdirty[p];  // uniform
-    if (fd == NONE) return;           // no read value changed since the last walk
-    if (lane == 0) a.fdirty[p] = NONE;
+    // the first entry whose read value changed since the last walk (uniform):
+    // the marks of all entries (4 to a word) and of the end read, in one round
+    // trip; the walk consumes every mark, so they are cleared here
+    int64_t fd = 0;
+    if (sweep > 0) {
+        uint32_t* fw = (uint32_t*)(a.dflag + p * a.capw);
+        const int64_t nw = a.capw / 4;
+        const uint8_t de = a.dend[p];
+        fd = -1;
+        for (int64_t w0 = 0; w0 < nw; w0 += WAVE) {
+            const int64_t wi = w0 + lane;
+            const uint32_t wd = wi < nw ? fw[wi] : 0u;
+            const ull m = __ballot(wd != 0u);
+            if (m && fd < 0) {
+                const int j = __ffsll((long long)m) - 1;
+                fd = 4 * (w0 + j) + (__builtin_ctz((uint32_t)rli((int)wd, j)) >> 3);
+            }
+            if (wd) fw[wi] = 0u;
+        }
+        if (fd < 0) {
+            if (!de) return;  // no read value changed since the last walk
+            fd = cnt;
+        }
+        if (de && lane == 0) a.dend[p] = 0;
+    }
     // the stockpile's inputs, in flight with the entries' (used by lane 0 at the end)
     const int32_t dc = a.death[p];
     double hsd = 0.0, red = 0.0;
@@ -403,7 +436,7 @@ __device__ void droplet_wave(const OrdArgs& a, int64_t p, int sweep) {
     // the carries entering entry fd: stored by the last walk (fd == 0: none yet)
     double D = 0.0, S = 0.0;  // deficit, surplus: uniform
     int n_clamp = 0;          // uniform
-    const int64_t k_first = fd < (uint32_t)cnt ? (int64_t)fd : cnt;
+    const int64_t k_first = fd < cnt ? fd : cnt;
     if (k_first > 0) {
         if (k_first < cnt) {
             D = a.cd[base + k_first];
@@ -746,9 +779,12 @@ __device__ void range_wave(const OrdArgs& a, int64_t r, int sweep) {
                         double* dst = kind == K_SELF ? a.rs + idx : (kind == K_NB ? a.rn + idx : a.re + idx);
                         *dst = rv;
                         // the droplet reading it walks again from this entry
-                        const int64_t dp = kind == K_END ? (int64_t)idx : (int64_t)idx / a.cap;
-                        const uint32_t dk = kind == K_END ? (uint32_t)a.cap : (uint32_t)((int64_t)idx - dp * a.cap);
-                        atomicMin(a.fdirty + dp, dk);
+                        if (kind == K_END) {
+                            a.dend[idx] = 1;
+                        } else {
+                            const int64_t dp = (int64_t)idx / a.cap;
+                            a.dflag[dp * a.capw + ((int64_t)idx - dp * a.cap)] = 1;
+                        }
                     }
                 }
             }
@@ -973,7 +1009,9 @@ int ordered_enqueue(const int32_t* cl_cell, const double* cl_delta, const float*
     a.cd = at<double>(work, L.cd);
     a.cs = at<double>(work, L.cs);
     a.cn = at<int32_t>(work, L.cn);
-    a.fdirty = at<uint32_t>(work, L.fdirty);
+    a.dflag = at<uint8_t>(work, L.dflag);
+    a.dend = at<uint8_t>(work, L.dend);
+    a.capw = (cap + 3) & ~(int64_t)3;
     a.fc = at<double>(work, L.fc);
     a.ownerw = at<uint32_t>(work, L.ownerw);
     a.rmin = at<uint32_t>(work, L.rmin);
