@@ -609,30 +609,6 @@ __global__ void __launch_bounds__(TRACE_TPB) __attribute__((amdgpu_waves_per_eu(
     ll t_row = 0;
     Samp S;  // the sample of the stencil at the droplet's position
     Here cur;
-    // The census's per-step atomics (erosion request, floor key, tracks) are
-    // issued one step late, after the next step's loads: the load counter
-    // drains in issue order, and an atomic (performed past L2, behind every
-    // other lane's) issued before a load would hold that load up.  Integer
-    // adds and a min: the order changes nothing.
-    int pq_cell = -1;            // the cell of the pending step (-1: none)
-    ll pq_dq = 0;                // its erosion request (fixed point, > 0 to add)
-    float pq_nh = 0.0f;          // the surface stepped to (the floor key) ...
-    int pq_ncell = 0;            // ... and that cell
-    float pq_vol = 0.0f, pq_sa = 0.0f, pq_sb = 0.0f;  // the tracks' operands (with a tape)
-    auto flush_step = [&]() {
-        if (pq_cell >= 0) {
-            if (pq_dq > 0) {
-                atomicAdd((ull*)&k.ero_fx[pq_cell], (ull)pq_dq);
-                atomicMin(&k.floor_key[pq_cell], ((ull)fkey(pq_nh) << 32) | (ull)(unsigned)pq_ncell);
-            }
-            if (taping) {
-                atomicAdd((ull*)&k.dis_fx[pq_cell], (ull)llrint((double)pq_vol * k.tscale));
-                atomicAdd((ull*)&k.mom_fx[pq_cell], (ull)llrint((double)(pq_vol * pq_sa) * k.tscale));
-                atomicAdd((ull*)&k.mom_fx_b[pq_cell], (ull)llrint((double)(pq_vol * pq_sb) * k.tscale));
-            }
-            pq_cell = -1;
-        }
-    };
     // lane totals
     ll n_entries = 0, n_moves = 0, exit_fx = 0, vanished_fx = 0, lost_fx = 0, to_pend_fx = 0, lat_fx = 0;
     ll deaths[N_DEATHS] = {0, 0, 0, 0, 0, 0};
@@ -819,7 +795,6 @@ __global__ void __launch_bounds__(TRACE_TPB) __attribute__((amdgpu_waves_per_eu(
                         Here nxt;
                         const bool same = nix >= 0 && nix < N && niy >= 0 && niy < N;
                         load_here(nf, nix, niy, nxt, same ? &nrec : nullptr);
-                        if (!apply) flush_step();  // the previous step's atomics, behind this step's loads
                         if (nm == MASK_OUTSIDE) {
                             cause = DEATH_EXIT;
                             exited = true;
@@ -902,16 +877,17 @@ __global__ void __launch_bounds__(TRACE_TPB) __attribute__((amdgpu_waves_per_eu(
                                     }
                                 }
                                 if (!apply) {
-                                    // pending until the next step's loads are issued (flush_step); the
-                                    // tracks do not depend on the erosion scale: added here, where the
-                                    // trace leaves the atomic units idle, not in the replay
-                                    pq_cell = (int)cur.cell;
-                                    pq_dq = dq;
-                                    pq_nh = nrec.x;
-                                    pq_ncell = (int)ncell;
-                                    pq_vol = vol;
-                                    pq_sa = sa;
-                                    pq_sb = sb;
+                                    if (dq > 0) {
+                                        atomicAdd((ull*)&k.ero_fx[cur.cell], (ull)dq);
+                                        atomicMin(&k.floor_key[cur.cell], ((ull)fkey(nrec.x) << 32) | (ull)(unsigned)ncell);
+                                    }
+                                    if (taping) {
+                                        // the tracks do not depend on the erosion scale: added here, where the
+                                        // trace leaves the atomic units idle, not in the replay
+                                        atomicAdd((ull*)&k.dis_fx[cur.cell], (ull)llrint((double)vol * k.tscale));
+                                        atomicAdd((ull*)&k.mom_fx[cur.cell], (ull)llrint((double)(vol * sa) * k.tscale));
+                                        atomicAdd((ull*)&k.mom_fx_b[cur.cell], (ull)llrint((double)(vol * sb) * k.tscale));
+                                    }
                                 } else {
                                     deficit += cut;
                                     if (dq > 0) {
@@ -1047,7 +1023,6 @@ __global__ void __launch_bounds__(TRACE_TPB) __attribute__((amdgpu_waves_per_eu(
             k.tape_last[dl] = lc;
         }
         if (!apply) {
-            flush_step();
             // a land death's final load, on the cell it stops in: the applying pass shares each
             // cell's deposit room among the loads that stop there
             int cz = cause;
